@@ -1,0 +1,271 @@
+"""Test infrastructure only: ctypes access to the serial oracle and the
+reference checker.
+
+* ``liboracle.so`` -- the quirk-exact serial restatement of the reference's
+  octree-construction path (pgvd_oracle.c).
+* ``_ref/libpgvdref{2,3}.so`` -- the reference's own C sources (C/*.c of
+  edwardsjohnmartin/pgvd) linked with a small harness that restates
+  opencl/Kernels.cpp:388-411.  Built only where the reference tree exists.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package; the product library (pgvd_amd) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = {}
+
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+
+
+def build(ref: bool = True) -> None:
+    """Compile liboracle.so (and _ref when the reference is present)."""
+    target = "all" if ref else "oracle"
+    subprocess.run(["make", "-s", "-C", HERE, target], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build(ref=False)
+        L = C.CDLL(path)
+        L.oracle_next_pow2.argtypes = [C.c_int]
+        L.oracle_next_pow2.restype = C.c_int
+        L.oracle_bbox.argtypes = [f32p, C.c_int64, C.c_int, f32p, f32p]
+        L.oracle_bbox.restype = C.c_float
+        L.oracle_quantize.argtypes = [f32p, C.c_int64, C.c_int, C.c_int64, C.c_int,
+                                      C.c_void_p, C.c_void_p, C.c_int, i64p]
+        L.oracle_quantize.restype = C.c_int64
+        L.oracle_points_to_keys.argtypes = [i64p, C.c_int64, C.c_int, C.c_int, u64p, u64p]
+        L.oracle_compute_lcp_length.argtypes = [C.c_uint64] * 4 + [C.c_int]
+        L.oracle_compute_lcp_length.restype = C.c_int
+        L.oracle_stream_scan.argtypes = [u32p, u32p, C.c_int]
+        L.oracle_build.argtypes = [i64p, C.c_int64, C.c_int, C.c_int, C.c_void_p]
+        L.oracle_build.restype = C.c_void_p
+        L.oracle_build_from_keys.argtypes = [u64p, u64p, C.c_int64, C.c_int, C.c_int]
+        L.oracle_build_from_keys.restype = C.c_void_p
+        for f in ("oracle_tree_m", "oracle_tree_octree_size"):
+            getattr(L, f).argtypes = [C.c_void_p]
+            getattr(L, f).restype = C.c_int64
+        L.oracle_tree_keys.argtypes = [C.c_void_p, u64p, u64p]
+        L.oracle_tree_brt.argtypes = [C.c_void_p, i32p, u8p, u8p, u64p, u64p, i32p, i32p]
+        L.oracle_tree_splits.argtypes = [C.c_void_p, u32p, u32p]
+        L.oracle_tree_octree.argtypes = [C.c_void_p, i32p, i32p]
+        L.oracle_tree_conflicts.argtypes = [C.c_void_p, u8p, u32p]
+        L.oracle_tree_conflicts.restype = C.c_int
+        L.oracle_tree_free.argtypes = [C.c_void_p]
+        L.oracle_run_pipeline.argtypes = [f32p, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_int64)]
+        L.oracle_run_pipeline.restype = C.c_int64
+        _LIB = L
+    return _LIB
+
+
+def ref_available(dim: int = 2) -> bool:
+    return os.path.exists(os.path.join(HERE, "_ref", f"libpgvdref{dim}.so"))
+
+
+def ref(dim: int = 2):
+    """The reference's own C sources (DIM 2 or 3) behind ref_harness.cpp."""
+    if dim not in _REF:
+        path = os.path.join(HERE, "_ref", f"libpgvdref{dim}.so")
+        L = C.CDLL(path)
+        L.ref_compute_lcp_length.argtypes = [C.c_uint64] * 4 + [C.c_int]
+        L.ref_compute_lcp_length.restype = C.c_int
+        L.ref_compute_lcp.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int,
+                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.ref_stream_scan.argtypes = [u32p, u32p, C.c_int]
+        L.ref_next_pow2.argtypes = [C.c_int]
+        L.ref_next_pow2.restype = C.c_int
+        if dim == 2:
+            L.ref_xyz2z.argtypes = [C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+            L.ref_build_octree.argtypes = [i32p, C.c_int, C.c_int, C.c_int]
+            L.ref_build_octree.restype = C.c_void_p
+        L.ref_build_from_keys.argtypes = [u64p, u64p, C.c_int, C.c_int]
+        L.ref_build_from_keys.restype = C.c_void_p
+        for f in ("ref_tree_status", "ref_tree_m", "ref_tree_octree_size"):
+            getattr(L, f).argtypes = [C.c_void_p]
+            getattr(L, f).restype = C.c_int
+        L.ref_tree_keys.argtypes = [C.c_void_p, u64p, u64p]
+        L.ref_tree_brt.argtypes = [C.c_void_p, i32p, u8p, u8p, u64p, u64p, i32p, i32p]
+        L.ref_tree_splits.argtypes = [C.c_void_p, u32p, u32p]
+        L.ref_tree_octree.argtypes = [C.c_void_p, i32p, i32p]
+        L.ref_tree_free.argtypes = [C.c_void_p]
+        _REF[dim] = L
+    return _REF[dim]
+
+
+# ---------------------------------------------------------------------------
+# numpy-level helpers
+# ---------------------------------------------------------------------------
+
+def next_pow2(n: int) -> int:
+    return lib().oracle_next_pow2(int(n))
+
+
+def bbox(pts: np.ndarray):
+    pts = np.ascontiguousarray(pts, dtype=np.float32)
+    dim = pts.shape[1]
+    mn = np.zeros(dim, np.float32)
+    mx = np.zeros(dim, np.float32)
+    dw = lib().oracle_bbox(pts.reshape(-1), pts.shape[0], dim, mn, mx)
+    return mn, mx, np.float32(dw)
+
+
+def quantize(pts: np.ndarray, width: int, bb=None, clamped: bool = False) -> np.ndarray:
+    """Karras.cpp:69-99.  Raises ValueError where the reference throws."""
+    pts = np.ascontiguousarray(pts, dtype=np.float32)
+    n, dim = pts.shape
+    out = np.zeros(max(n, 1) * dim, np.int64)
+    if bb is not None:
+        mn = np.ascontiguousarray(bb[0], np.float32)
+        mx = np.ascontiguousarray(bb[1], np.float32)
+        r = lib().oracle_quantize(pts.reshape(-1), n, dim, int(width), 1,
+                                  mn.ctypes.data, mx.ctypes.data, int(clamped), out)
+    else:
+        r = lib().oracle_quantize(pts.reshape(-1), n, dim, int(width), 0, None, None,
+                                  int(clamped), out)
+    if r < 0:
+        raise ValueError("bad coordinate")
+    return out[: r * dim].reshape(r, dim)
+
+
+def morton(q: np.ndarray, bits: int):
+    """xyz2z for every row of q (n x dim int64), no padding -> (lo, hi)."""
+    q = np.ascontiguousarray(q, dtype=np.int64)
+    n, dim = q.shape
+    lo = np.zeros(n, np.uint64)
+    hi = np.zeros(n, np.uint64)
+    lib().oracle_points_to_keys(q.reshape(-1), n, dim, bits, lo, hi)
+    return lo, hi
+
+
+def lcp_length(a: int, b: int, mbits: int) -> int:
+    M = (1 << 64) - 1
+    return lib().oracle_compute_lcp_length(a & M, a >> 64, b & M, b >> 64, mbits)
+
+
+def stream_scan(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    out = np.zeros_like(x)
+    lib().oracle_stream_scan(x, out, x.size)
+    return out
+
+
+def _collect(L, h, prefix: str, dim: int, labels: bool = False) -> dict:
+    g = lambda name: getattr(L, prefix + name)
+    m = int(g("tree_m")(h))
+    O = int(g("tree_octree_size")(h))
+    ni = max(m - 1, 0)
+    lo = np.zeros(m, np.uint64)
+    hi = np.zeros(m, np.uint64)
+    g("tree_keys")(h, lo, hi)
+    left = np.zeros(max(ni, 1), np.int32)
+    ll = np.zeros(max(ni, 1), np.uint8)
+    rl = np.zeros(max(ni, 1), np.uint8)
+    lcl = np.zeros(max(ni, 1), np.uint64)
+    lch = np.zeros(max(ni, 1), np.uint64)
+    lcn = np.zeros(max(ni, 1), np.int32)
+    par = np.zeros(max(ni, 1), np.int32)
+    g("tree_brt")(h, left, ll, rl, lcl, lch, lcn, par)
+    ls = np.zeros(m, np.uint32)
+    ps = np.zeros(m, np.uint32)
+    g("tree_splits")(h, ls, ps)
+    nc = 1 << dim
+    ch = np.zeros(max(O, 1) * nc, np.int32)
+    lf = np.zeros(max(O, 1), np.int32)
+    g("tree_octree")(h, ch, lf)
+    out = dict(m=m, octree_size=O, keys_lo=lo, keys_hi=hi,
+               left=left[:ni], left_leaf=ll[:ni], right_leaf=rl[:ni],
+               lcp_lo=lcl[:ni], lcp_hi=lch[:ni], lcp_length=lcn[:ni], parent=par[:ni],
+               local_splits=ls, prefix_sums=ps,
+               children=ch[: O * nc].reshape(O, nc), leaf=lf[:O])
+    if labels:
+        kf = np.zeros(m, np.uint8)
+        nm = np.zeros(max(O, 1), np.uint32)
+        L.oracle_tree_conflicts(h, kf, nm)
+        out["key_conflict"] = kf
+        out["node_conflict"] = nm[:O]
+    return out
+
+
+def build(q: np.ndarray, bits: int, labels=None) -> dict:
+    """BuildOctree_s (opencl/Kernels.cpp:388-411) from quantized points."""
+    q = np.ascontiguousarray(q, dtype=np.int64)
+    n, dim = q.shape
+    L = lib()
+    lab = None
+    if labels is not None:
+        lab = np.ascontiguousarray(labels, dtype=np.uint32)
+    h = L.oracle_build(q.reshape(-1), n, dim, bits, None if lab is None else lab.ctypes.data)
+    if not h:
+        raise ValueError("oracle_build rejected its arguments")
+    try:
+        return _collect(L, h, "oracle_", dim, labels=lab is not None)
+    finally:
+        L.oracle_tree_free(h)
+
+
+def build_from_keys(lo: np.ndarray, hi, dim: int, mbits: int) -> dict:
+    lo = np.ascontiguousarray(lo, dtype=np.uint64)
+    hi = np.zeros_like(lo) if hi is None else np.ascontiguousarray(hi, dtype=np.uint64)
+    L = lib()
+    h = L.oracle_build_from_keys(lo, hi, lo.size, dim, mbits)
+    if not h:
+        raise ValueError("oracle_build_from_keys rejected its arguments")
+    try:
+        return _collect(L, h, "oracle_", dim)
+    finally:
+        L.oracle_tree_free(h)
+
+
+def ref_build_octree_2d(q: np.ndarray, bits: int) -> dict:
+    """The reference's own BuildOctree_s path (2D, int32 points)."""
+    q = np.ascontiguousarray(q, dtype=np.int32)
+    L = ref(2)
+    h = L.ref_build_octree(q.reshape(-1), q.shape[0], bits, 2 * bits)
+    try:
+        if L.ref_tree_status(h) != 0:
+            raise RuntimeError("reference harness could not map its key arena")
+        return _collect(L, h, "ref_", 2)
+    finally:
+        L.ref_tree_free(h)
+
+
+def ref_build_from_keys(lo: np.ndarray, hi, dim: int, mbits: int) -> dict:
+    lo = np.ascontiguousarray(lo, dtype=np.uint64)
+    hi = np.zeros_like(lo) if hi is None else np.ascontiguousarray(hi, dtype=np.uint64)
+    L = ref(dim)
+    h = L.ref_build_from_keys(lo, hi, lo.size, mbits)
+    try:
+        if L.ref_tree_status(h) != 0:
+            raise RuntimeError("reference harness could not map its key arena")
+        return _collect(L, h, "ref_", dim)
+    finally:
+        L.ref_tree_free(h)
+
+
+def ref_xyz2z_2d(x: int, y: int, bits: int) -> int:
+    lo = C.c_uint64()
+    hi = C.c_uint64()
+    ref(2).ref_xyz2z(int(x), int(y), int(bits), C.byref(lo), C.byref(hi))
+    return int(lo.value) | (int(hi.value) << 64)
+
+
+def ref_lcp_length(a: int, b: int, mbits: int, dim: int = 2) -> int:
+    M = (1 << 64) - 1
+    return ref(dim).ref_compute_lcp_length(a & M, a >> 64, b & M, b >> 64, mbits)

@@ -1,0 +1,494 @@
+/*
+ * pgvd_oracle.c -- TEST INFRASTRUCTURE ONLY (see pgvd_oracle.h).
+ *
+ * Serial restatement of the reference octree-construction path.  Citations
+ * are file:line in edwardsjohnmartin/pgvd.  Compile with
+ * -ffp-contract=off: the quantize expression is bit-relevant.
+ */
+#include "pgvd_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+
+static inline u128 mk(uint64_t lo, uint64_t hi) { return ((u128)hi << 64) | lo; }
+
+/* opencl/Kernels.cpp:20 */
+int oracle_next_pow2(int num) {
+  int p = (int)pow(2, ceil(log(num) / log(2)));
+  return p > 8 ? p : 8;
+}
+
+/* BoundingBox.h:166-175: first point initialises, then component-wise
+ * (a<b?a:b) / (a>b?a:b) (opencl/vec_cpp.h:255-277).  max_size:
+ * BoundingBox.h:80-87. */
+float oracle_bbox(const float *pts, int64_t n, int dim, float *bbmin, float *bbmax) {
+  for (int k = 0; k < dim; ++k) { bbmin[k] = pts[k]; bbmax[k] = pts[k]; }
+  for (int64_t i = 1; i < n; ++i) {
+    for (int k = 0; k < dim; ++k) {
+      const float v = pts[i * dim + k];
+      bbmin[k] = (bbmin[k] < v) ? bbmin[k] : v;
+      bbmax[k] = (bbmax[k] > v) ? bbmax[k] : v;
+    }
+  }
+  float m = bbmax[0] - bbmin[0];
+  for (int k = 1; k < dim; ++k) {
+    const float s = bbmax[k] - bbmin[k];
+    m = (s > m) ? s : m;
+  }
+  return m;
+}
+
+/* Karras.cpp:42-67 for one point.  The expression is evaluated exactly as
+ * the reference writes it: (p-min)/dwidth in float, times the int width
+ * converted to float, widened to double, +0.5, truncated.  int64 replaces
+ * int so that widths above 2^31 are representable (the reference's int
+ * overflows there). */
+static int quantize_point(const float *p, const float *bbmin, float dwidth,
+                          int64_t eff_width, int dim, int64_t *q) {
+  for (int k = 0; k < dim; ++k) {
+    const float t = (p[k] - bbmin[k]) / dwidth;
+    const float f = (float)eff_width * t;
+    const double d = (double)f;
+    const int64_t v = (int64_t)(d + 0.5);
+    if (v < 0) return -1;
+    q[k] = v;
+  }
+  return 0;
+}
+
+/* Karras.cpp:69-99 */
+int64_t oracle_quantize(const float *pts, int64_t n, int dim, int64_t width,
+                        int custom_bb, const float *bbmin_in,
+                        const float *bbmax_in, int clamped, int64_t *out) {
+  if (n <= 0) return 0;
+  float bbmin[3] = {0, 0, 0}, bbmax[3] = {0, 0, 0}, dwidth;
+  if (custom_bb) {
+    for (int k = 0; k < dim; ++k) { bbmin[k] = bbmin_in[k]; bbmax[k] = bbmax_in[k]; }
+    dwidth = bbmax[0] - bbmin[0];
+    for (int k = 1; k < dim; ++k) {
+      const float s = bbmax[k] - bbmin[k];
+      dwidth = (s > dwidth) ? s : dwidth;
+    }
+  } else {
+    dwidth = oracle_bbox(pts, n, dim, bbmin, bbmax);
+  }
+  if (dwidth == 0) {              /* Karras.cpp:84-88: a single {0,0} */
+    for (int k = 0; k < dim; ++k) out[k] = 0;
+    return 1;
+  }
+  const int64_t eff = clamped ? width : width - 1;   /* Karras.cpp:46-49 */
+  for (int64_t i = 0; i < n; ++i)
+    if (quantize_point(pts + i * dim, bbmin, dwidth, eff, dim, out + i * dim)) return -1;
+  return n;
+}
+
+/* C/z_order.h:14-47: bit by bit, key bit i*DIM+k <- bit i of coordinate k.
+ * The reference tests `p.x & (1 << i)` (int shift, so it is limited to
+ * i <= 30); this restatement tests bit i of a 64-bit coordinate. */
+static u128 xyz2z(const int64_t *p, int dim, int bits) {
+  u128 r = 0;
+  for (int i = 0; i < bits; ++i)
+    for (int k = 0; k < dim; ++k)
+      if ((uint64_t)p[k] & ((uint64_t)1 << i)) r |= (u128)1 << (i * dim + k);
+  return r;
+}
+
+void oracle_xyz2z(const int64_t *p, int dim, int bits, uint64_t *lo, uint64_t *hi) {
+  const u128 z = xyz2z(p, dim, bits);
+  *lo = (uint64_t)z;
+  *hi = (uint64_t)(z >> 64);
+}
+
+void oracle_points_to_keys(const int64_t *p, int64_t n, int dim, int bits,
+                           uint64_t *lo, uint64_t *hi) {
+  for (int64_t i = 0; i < n; ++i) oracle_xyz2z(p + i * dim, dim, bits, &lo[i], &hi[i]);
+}
+
+/* BigUnsigned right shift (C/BigUnsigned.c:417-471) on a native 128-bit
+ * value: every bit shifted past the end is dropped, shifts >= 128 give 0. */
+static inline u128 shr(u128 a, int s) { return s >= 128 ? 0 : (a >> s); }
+
+/* C/BuildBRT.c:54-88: binary search over right-shift amounts for the
+ * smallest shift that makes a and b equal. */
+static int compute_lcp_length(u128 a, u128 b, int mbits) {
+  unsigned int v = mbits;
+  v--; v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16; v++;
+  int offset = v >> 1;
+  for (int i = v >> 2; i > 0; i >>= 1) {
+    if (shr(a, offset) == shr(b, offset)) offset -= i;
+    else offset += i;
+  }
+  if (shr(a, offset) == shr(b, offset)) {
+    if (shr(a, offset - 1) == shr(b, offset - 1)) return mbits - (offset - 1);
+    return mbits - offset;
+  }
+  return mbits - (offset + 1);
+}
+
+int oracle_compute_lcp_length(uint64_t alo, uint64_t ahi, uint64_t blo,
+                              uint64_t bhi, int mbits) {
+  return compute_lcp_length(mk(alo, ahi), mk(blo, bhi), mbits);
+}
+
+/* C/BuildBRT.c:34-52: mask the top `length` bits of an mbits-wide value
+ * and shift them down. */
+static u128 compute_lcp(u128 value, int length, int mbits) {
+  u128 mask = 0;
+  for (int i = 0; i < length; ++i) mask |= (u128)1 << (mbits - 1 - i);
+  return shr(value & mask, mbits - length);
+}
+
+/* opencl/Kernels.cpp:125-159 with C/ParallelAlgorithms.c:33-51:
+ * zero-padded inclusive Hillis-Steele scan, ping-ponging two buffers. */
+void oracle_stream_scan(const uint32_t *in, uint32_t *out, int size) {
+  if (size <= 0) return;
+  const int p2 = (int)pow(2, ceil(log(size) / log(2)));
+  uint32_t *a = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)p2);
+  uint32_t *b = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)p2);
+  for (int i = 0; i < size; ++i) a[i] = b[i] = in[i];
+  for (int i = size; i < p2; ++i) a[i] = b[i] = 0;
+  for (unsigned int s = 1; s < (unsigned int)p2; s <<= 1) {
+    for (int j = 0; j < p2; ++j)
+      b[j] = (j > (int)(s - 1)) ? a[j] + a[j - s] : a[j];
+    uint32_t *t = a; a = b; b = t;
+  }
+  for (int i = 0; i < size; ++i) out[i] = a[i];
+  free(a);
+  free(b);
+}
+
+/* ------------------------------------------------------------------ */
+
+struct oracle_tree {
+  int dim, mbits;
+  int64_t m, O;
+  u128 *keys;                 /* m sorted unique keys                  */
+  /* BrtNode, C/BrtNode.h:10-21 */
+  int32_t *left, *lcp_length, *parent;
+  uint8_t *left_leaf, *right_leaf;
+  u128 *lcp;
+  uint32_t *ls, *ps;          /* m each                                 */
+  int32_t *children, *leaf;   /* O * (1<<dim), O                        */
+  uint8_t *key_conflict;      /* m, or NULL without labels             */
+};
+
+static int cmp_u128(const void *a, const void *b) {
+  const u128 x = *(const u128 *)a, y = *(const u128 *)b;
+  return (x > y) - (x < y);
+}
+
+typedef struct { u128 key; uint32_t label; uint32_t real; } keyed_label;
+static int cmp_keyed(const void *a, const void *b) {
+  const keyed_label *x = (const keyed_label *)a, *y = (const keyed_label *)b;
+  if (x->key != y->key) return (x->key > y->key) - (x->key < y->key);
+  return 0;
+}
+
+/* C/BuildBRT.c:90-181 for one gid, including the quirks the reference
+ * has:  the l_max loop tests a key loaded one doubling earlier (:118-126),
+ * and the split step is the FP32 expression of :151.  The out-of-range
+ * read at :119 (gid=1, d=-1 wraps the unsigned index) is not performed;
+ * its value is never used because the range test at :120-121 fails
+ * first. */
+static void build_brt_node(oracle_tree *t, int size, int gid) {
+  const u128 *mp = t->keys;
+  const int mbits = t->mbits;
+  if (!(gid < size - 1)) return;
+  const u128 current = mp[gid];
+  int d;
+  if (gid == 0) d = 1;
+  else {
+    const int l_pos = compute_lcp_length(current, mp[gid + 1], mbits);
+    const int l_neg = compute_lcp_length(current, mp[gid - 1], mbits);
+    d = (l_pos - l_neg > 0) - (l_pos - l_neg < 0);
+  }
+  int l;
+  if (gid == 0) {
+    l = size - 1;
+  } else {
+    const int lcp_min = (d == -1) ? compute_lcp_length(current, mp[gid + 1], mbits)
+                                  : compute_lcp_length(current, mp[gid - 1], mbits);
+    int l_max = 2;
+    long long idx = (long long)gid + (long long)l_max * d;
+    u128 temp = (idx >= 0 && idx <= size - 1) ? mp[idx] : 0;
+    for (;;) {
+      idx = (long long)gid + (long long)l_max * d;
+      if (!(idx >= 0 && idx <= size - 1)) break;
+      if (!(compute_lcp_length(current, temp, mbits) > lcp_min)) break;
+      temp = mp[idx];
+      l_max = l_max << 1;
+    }
+    const int l_cutoff = (d == -1) ? gid : size - gid - 1;
+    l = 0;
+    for (int tt = l_max >> 1; tt >= 1; tt >>= 1) {
+      if (l + tt <= l_cutoff) {
+        if (compute_lcp_length(current, mp[gid + (l + tt) * d], mbits) > lcp_min) l = l + tt;
+      }
+    }
+  }
+  const int j = gid + l * d;
+  const int lcp_node = compute_lcp_length(current, mp[j], mbits);
+  const int s_cutoff = (d == -1) ? gid - 1 : size - gid - 2;
+  int s = 0;
+  for (int den = 2; den < 2 * l; den *= 2) {
+    const int tt = (int)((l + (float)den - 1) / den);
+    if (s + tt <= s_cutoff) {
+      if (compute_lcp_length(current, mp[gid + (s + tt) * d], mbits) > lcp_node) s = s + tt;
+    }
+  }
+  const int split = gid + s * d + (d < 0 ? d : 0);
+  const int mn = gid < j ? gid : j, mx = gid > j ? gid : j;
+  t->left[gid] = split;
+  t->left_leaf[gid] = (mn == split);
+  t->right_leaf[gid] = (mx == split + 1);
+  t->lcp[gid] = compute_lcp(current, lcp_node, mbits);
+  t->lcp_length[gid] = lcp_node;
+  if (gid == 0) t->parent[gid] = -1;
+  if (!t->left_leaf[gid]) t->parent[split] = gid;
+  if (!t->right_leaf[gid]) t->parent[split + 1] = gid;
+}
+
+/* C/BuildOctree.c:50-64 */
+static int quadrant_in_lcp(const oracle_tree *t, int node, int i) {
+  const int dim = t->dim;
+  const int mask = (dim == 2) ? 3 : 7;
+  const int rshift = i * dim + t->lcp_length[node] % dim;
+  return (int)(shr(t->lcp[node], rshift) & (u128)mask);
+}
+
+static void *zalloc(size_t n) { return calloc(n ? n : 1, 1); }
+
+/* BRT, local splits, scan and octree from t->keys[0..m).  Follows
+ * opencl/Kernels.cpp:404-409 and :370-386. */
+static void build_tree(oracle_tree *t) {
+  const int64_t m = t->m;
+  const int dim = t->dim;
+  const int nc = 1 << dim;
+  const int64_t ni = m > 1 ? m - 1 : 0;
+  /* vector<BrtNode> I(numPoints - 1): value-initialised (zero). */
+  t->left = zalloc(sizeof(int32_t) * ni);
+  t->lcp_length = zalloc(sizeof(int32_t) * ni);
+  t->parent = zalloc(sizeof(int32_t) * ni);
+  t->left_leaf = zalloc(ni);
+  t->right_leaf = zalloc(ni);
+  t->lcp = zalloc(sizeof(u128) * ni);
+  /* BuildBinaryRadixTree_s, opencl/Kernels.cpp:266-273 */
+  for (int i = 0; i < (int)m - 1; ++i) build_brt_node(t, (int)m, i);
+
+  t->ls = zalloc(sizeof(uint32_t) * m);
+  t->ps = zalloc(sizeof(uint32_t) * m);
+  if (m == 1) {
+    /* The reference reads I[0] of an empty vector here
+     * (opencl/Kernels.cpp:305-306); defined as lcp_length 0, so one node. */
+    t->ls[0] = 1;
+    t->ps[0] = 1;
+  } else {
+    /* ComputeLocalSplits_s, opencl/Kernels.cpp:303-313 with
+     * C/BuildOctree.c:17-27 */
+    t->ls[0] = 1 + t->lcp_length[0] / dim;
+    for (int i = 0; i < (int)m - 1; ++i) {
+      const int local = t->lcp_length[i] / dim;
+      const int left = t->left[i], right = left + 1;
+      if (!t->left_leaf[i]) t->ls[left] = (uint32_t)(t->lcp_length[left] / dim - local);
+      if (!t->right_leaf[i]) t->ls[right] = (uint32_t)(t->lcp_length[right] / dim - local);
+    }
+    oracle_stream_scan(t->ls, t->ps, (int)m);
+  }
+  /* BinaryRadixToOctree_s, opencl/Kernels.cpp:370-386 */
+  const int64_t O = t->ps[m - 1];
+  t->O = O;
+  t->children = zalloc(sizeof(int32_t) * (size_t)O * nc);
+  t->leaf = zalloc(sizeof(int32_t) * (size_t)O);
+  /* brt2octree_init, C/BuildOctree.c:134-139.  The reference sets
+   * leaf = 15 and uses leaf_masks = {1,2,4,8} (C/OctNode.h:7), which is
+   * defined for DIM == 2 only; the 3D definition here is all 2^DIM bits
+   * set and mask 1<<octant. */
+  const int32_t full = (1 << nc) - 1;
+  for (int64_t i = 0; i < O; ++i) {
+    t->leaf[i] = full;
+    for (int c = 0; c < nc; ++c) t->children[i * nc + c] = -1;
+  }
+  /* brt2octree, C/BuildOctree.c:73-132, for brt_i = 1 .. m-2 in order */
+  for (int brt_i = 1; brt_i < (int)m - 1; ++brt_i) {
+    if (t->ls[brt_i] == 0) continue;
+    const int num_splits = (int)t->ls[brt_i];
+    int current = (int)t->ps[brt_i - 1];
+    for (int i = 0; i < num_splits - 1; ++i) {
+      const int oct_parent = current + 1;
+      const int onode = quadrant_in_lcp(t, brt_i, i);
+      t->children[(int64_t)oct_parent * nc + onode] = current;
+      t->leaf[oct_parent] &= ~(1 << onode);
+      current = oct_parent;
+    }
+    int brt_parent = t->parent[brt_i];
+    while (t->ls[brt_parent] == 0) brt_parent = t->parent[brt_parent];
+    const int oct_parent = (brt_parent == 0) ? 0 : (int)t->ps[brt_parent - 1];
+    const int q = quadrant_in_lcp(t, brt_i, num_splits - 1);
+    t->children[(int64_t)oct_parent * nc + q] = current;
+    t->leaf[oct_parent] &= ~(1 << q);
+  }
+}
+
+static oracle_tree *new_tree(int dim, int mbits) {
+  oracle_tree *t = (oracle_tree *)calloc(1, sizeof(oracle_tree));
+  t->dim = dim;
+  t->mbits = mbits;
+  return t;
+}
+
+/* opencl/Kernels.cpp:388-411 (BuildOctree_s).  With labels, the key sort
+ * carries (label, real) so the point-label conflict flags can be
+ * computed per unique key; padding keys carry no label. */
+oracle_tree *oracle_build(const int64_t *qpts, int64_t n, int dim, int bits,
+                          const uint32_t *labels) {
+  if (n <= 0 || (dim != 2 && dim != 3) || bits <= 0 || bits * dim > 128) return NULL;
+  oracle_tree *t = new_tree(dim, bits * dim);
+  const int64_t rn = oracle_next_pow2((int)n);
+  keyed_label *kl = (keyed_label *)malloc(sizeof(keyed_label) * (size_t)rn);
+  /* PointsToMorton_s, opencl/Kernels.cpp:62-75 */
+  for (int64_t g = 0; g < rn; ++g) {
+    if (g < n) {
+      kl[g].key = xyz2z(qpts + g * dim, dim, bits);
+      kl[g].label = labels ? labels[g] : 0;
+      kl[g].real = 1;
+    } else {
+      kl[g].key = 0;
+      kl[g].label = 0;
+      kl[g].real = 0;
+    }
+  }
+  /* sort (ascending) and unique, opencl/Kernels.cpp:401-402 */
+  qsort(kl, (size_t)rn, sizeof(keyed_label), cmp_keyed);
+  t->keys = (u128 *)malloc(sizeof(u128) * (size_t)rn);
+  if (labels) t->key_conflict = (uint8_t *)zalloc((size_t)rn);
+  int64_t m = 0;
+  int64_t run_start = 0;
+  for (int64_t i = 0; i <= rn; ++i) {
+    if (i == rn || (i > 0 && kl[i].key != kl[i - 1].key)) {
+      /* close the run [run_start, i) */
+      if (labels) {
+        int have = 0;
+        uint32_t lo = 0, hi = 0;
+        for (int64_t k = run_start; k < i; ++k) {
+          if (!kl[k].real) continue;
+          if (!have) { lo = hi = kl[k].label; have = 1; }
+          else { if (kl[k].label < lo) lo = kl[k].label; if (kl[k].label > hi) hi = kl[k].label; }
+        }
+        t->key_conflict[m] = (uint8_t)(have && lo != hi);
+      }
+      t->keys[m++] = kl[run_start].key;
+      run_start = i;
+    }
+  }
+  free(kl);
+  t->m = m;
+  build_tree(t);
+  return t;
+}
+
+oracle_tree *oracle_build_from_keys(const uint64_t *lo, const uint64_t *hi,
+                                    int64_t m, int dim, int mbits) {
+  if (m <= 0 || (dim != 2 && dim != 3) || mbits <= 0 || mbits > 128) return NULL;
+  oracle_tree *t = new_tree(dim, mbits);
+  t->m = m;
+  t->keys = (u128 *)malloc(sizeof(u128) * (size_t)m);
+  for (int64_t i = 0; i < m; ++i) t->keys[i] = mk(lo[i], hi ? hi[i] : 0);
+  build_tree(t);
+  return t;
+}
+
+int64_t oracle_tree_m(const oracle_tree *t) { return t->m; }
+int64_t oracle_tree_octree_size(const oracle_tree *t) { return t->O; }
+
+void oracle_tree_keys(const oracle_tree *t, uint64_t *lo, uint64_t *hi) {
+  for (int64_t i = 0; i < t->m; ++i) {
+    lo[i] = (uint64_t)t->keys[i];
+    if (hi) hi[i] = (uint64_t)(t->keys[i] >> 64);
+  }
+}
+
+void oracle_tree_brt(const oracle_tree *t, int32_t *left, uint8_t *left_leaf,
+                     uint8_t *right_leaf, uint64_t *lcp_lo, uint64_t *lcp_hi,
+                     int32_t *lcp_length, int32_t *parent) {
+  const int64_t ni = t->m > 1 ? t->m - 1 : 0;
+  for (int64_t i = 0; i < ni; ++i) {
+    left[i] = t->left[i];
+    left_leaf[i] = t->left_leaf[i];
+    right_leaf[i] = t->right_leaf[i];
+    lcp_lo[i] = (uint64_t)t->lcp[i];
+    if (lcp_hi) lcp_hi[i] = (uint64_t)(t->lcp[i] >> 64);
+    lcp_length[i] = t->lcp_length[i];
+    parent[i] = t->parent[i];
+  }
+}
+
+void oracle_tree_splits(const oracle_tree *t, uint32_t *ls, uint32_t *ps) {
+  memcpy(ls, t->ls, sizeof(uint32_t) * (size_t)t->m);
+  memcpy(ps, t->ps, sizeof(uint32_t) * (size_t)t->m);
+}
+
+void oracle_tree_octree(const oracle_tree *t, int32_t *children, int32_t *leaf) {
+  const int nc = 1 << t->dim;
+  memcpy(children, t->children, sizeof(int32_t) * (size_t)t->O * nc);
+  memcpy(leaf, t->leaf, sizeof(int32_t) * (size_t)t->O);
+}
+
+/* Point-label conflict cells.  The reference's version
+ * (CellIntersections.h:9-55, viewer/Octree2.cpp:366-506) walks polyline
+ * segments through the octree and keeps up to two labels per
+ * (node, octant) leaf slot, `is_multi` meaning a second label is present.
+ * This restatement is the point-label form: a unique key (one octree leaf
+ * cell) is in conflict when the labels of the points that map to it are
+ * not all equal, and the flag is also reported on the (node, octant) slot
+ * that holds the key: the octree node of the key's BRT parent (walking up
+ * through nodes with no local split, as brt2octree's attach does,
+ * C/BuildOctree.c:102-112), octant = the key's digit at that node's
+ * level. */
+int oracle_tree_conflicts(const oracle_tree *t, uint8_t *key_flags, uint32_t *node_masks) {
+  if (!t->key_conflict) return -1;
+  memcpy(key_flags, t->key_conflict, (size_t)t->m);
+  memset(node_masks, 0, sizeof(uint32_t) * (size_t)t->O);
+  const int dim = t->dim, mbits = t->mbits;
+  const int mask = (1 << dim) - 1;
+  for (int64_t x = 0; x + 1 < t->m; ++x) {
+    for (int side = 0; side < 2; ++side) {
+      const int is_leaf = side ? t->right_leaf[x] : t->left_leaf[x];
+      if (!is_leaf) continue;
+      const int64_t c = t->left[x] + side;
+      if (!t->key_conflict[c]) continue;
+      int64_t q = x;
+      while (q != 0 && t->ls[q] == 0) q = t->parent[q];
+      const int64_t T = (q == 0) ? 0 : t->ps[q - 1];
+      const int lvl = t->lcp_length[q] / dim;
+      const int sh = mbits - (lvl + 1) * dim;
+      const int o = sh >= 0 ? (int)(shr(t->keys[c], sh) & (u128)mask) : 0;
+      node_masks[T] |= 1u << o;
+    }
+  }
+  return 0;
+}
+
+void oracle_tree_free(oracle_tree *t) {
+  if (!t) return;
+  free(t->keys); free(t->left); free(t->lcp_length); free(t->parent);
+  free(t->left_leaf); free(t->right_leaf); free(t->lcp);
+  free(t->ls); free(t->ps); free(t->children); free(t->leaf);
+  free(t->key_conflict);
+  free(t);
+}
+
+int64_t oracle_run_pipeline(const float *pts, int64_t n, int dim, int bits, int64_t *m_out) {
+  int64_t *q = (int64_t *)malloc(sizeof(int64_t) * (size_t)n * dim);
+  const int64_t nq = oracle_quantize(pts, n, dim, (int64_t)1 << bits, 0, NULL, NULL, 0, q);
+  if (nq < 0) { free(q); return -1; }
+  oracle_tree *t = oracle_build(q, nq, dim, bits, NULL);
+  free(q);
+  if (!t) return -1;
+  const int64_t O = t->O;
+  if (m_out) *m_out = t->m;
+  oracle_tree_free(t);
+  return O;
+}
