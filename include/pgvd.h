@@ -1,0 +1,207 @@
+/*
+ * pgvd.h -- C ABI of libpgvd.so, the MI355X (gfx950) octree-construction
+ * path of edwardsjohnmartin/pgvd.
+ *
+ * Every compute entry point replaces one function of the reference's
+ * OpenCL driver layer (namespace Kernels, opencl/Kernels.h:26-50) or of its
+ * facade (Karras.h:23-36); the comment above each names it.  The reference
+ * passes cl::Buffer handles and returns an OR-accumulated cl_int; this ABI
+ * takes plain device pointers and sizes and returns a pgvd_status code
+ * (0 = success) with a message in pgvd_last_error().  Stream arguments are
+ * hipStream_t values passed as void*; NULL means the context's own stream.
+ *
+ * Keys are native words chosen by mbits (= bits * dim): uint32_t for
+ * mbits <= 32, uint64_t for <= 64, and 16-byte (lo, hi) pairs for <= 128.
+ * They carry the same numeric value as the reference's BigUnsigned
+ * (C/BigUnsigned.h:8-16); key_bytes arguments are 4, 8 or 16.
+ *
+ * Sizes: n < 2^30 points.  dim is 2 or 3; bits * dim <= 128 (3D: bits <= 42).
+ */
+#ifndef PGVD_H
+#define PGVD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pgvd_ctx pgvd_ctx;
+typedef void *pgvd_stream;
+
+typedef enum pgvd_status {
+  PGVD_OK = 0,
+  PGVD_ERR_INVALID = 1,     /* bad argument: null pointer, dim, bits, key_bytes */
+  PGVD_ERR_EMPTY = 2,       /* zero points (opencl/Kernels.cpp:389-391 throws) */
+  PGVD_ERR_BAD_COORD = 3,   /* coordinate quantized below zero (Karras.cpp:56-63 throws) */
+  PGVD_ERR_NO_DEVICE = 4,   /* no HIP device / device index out of range */
+  PGVD_ERR_ALLOC = 5,       /* device or pinned allocation failed */
+  PGVD_ERR_HIP = 6,         /* a HIP runtime call or kernel launch failed */
+  PGVD_ERR_CAPACITY = 7,    /* caller's output buffer too small; see *required */
+  PGVD_ERR_TOO_LARGE = 8,   /* n >= 2^30 */
+  PGVD_ERR_DUP_KEY = 9,     /* BRT input has equal neighbouring keys */
+  PGVD_ERR_INTERNAL = 10
+} pgvd_status;
+
+/* Message for the last failing call on this thread. */
+const char *pgvd_last_error(void);
+const char *pgvd_version(void);
+
+/* ---- context, memory, streams (the role of CLFW, CLFW/src/clfw.cpp:82-265) ----
+ * A context owns one HIP stream (default priority) and a keyed pool of
+ * device buffers reused across calls (CLFW::get's keyed reuse,
+ * CLFW/src/clfw.cpp:212-226). */
+int pgvd_ctx_create(int device, pgvd_ctx **out);
+int pgvd_ctx_destroy(pgvd_ctx *ctx);
+int pgvd_ctx_stream(pgvd_ctx *ctx, pgvd_stream *out);
+int pgvd_synchronize(pgvd_ctx *ctx, pgvd_stream s);
+int pgvd_alloc(pgvd_ctx *ctx, size_t bytes, void **dptr);
+int pgvd_free(pgvd_ctx *ctx, void *dptr);
+int pgvd_memcpy_htod(pgvd_ctx *ctx, void *dst, const void *src, size_t bytes, pgvd_stream s);
+int pgvd_memcpy_dtoh(pgvd_ctx *ctx, void *dst, const void *src, size_t bytes, pgvd_stream s);
+/* Device-to-device copy through a plain copy kernel (also the calibration
+ * kernel for HBM byte counters). */
+int pgvd_copy_kernel(pgvd_ctx *ctx, void *dst, const void *src, size_t bytes, pgvd_stream s);
+
+/* Per-stage kernel timing with HIP events on the launch stream.
+ * pgvd_timing_read returns the summed milliseconds and launch count of one
+ * stage ("bbox", "encode", "sort_pass", "unique", "brt", "splits_scan",
+ * "octree") since the last reset; it synchronizes. */
+int pgvd_timing_enable(pgvd_ctx *ctx, int on);
+int pgvd_timing_reset(pgvd_ctx *ctx);
+int pgvd_timing_read(pgvd_ctx *ctx, const char *stage, double *total_ms, int64_t *count);
+
+/* ---- stage entry points (device pointers) ---- */
+
+/* Kernels::nextPow2 (opencl/Kernels.cpp:20). */
+int pgvd_next_pow2(int n);
+
+/* Bounding box of n points (BoundingBox.h:166-175 over the points, as in
+ * Karras.cpp:75-83).  bb_out (host, 2*dim floats): min then max.  Returns
+ * after the result is on the host. */
+int pgvd_bbox(pgvd_ctx *ctx, const float *pts, int64_t n, int dim, float *bb_out, pgvd_stream s);
+
+/* Karras::Quantize (Karras.cpp:69-99): FP32 quantize to a 2^bits grid.
+ * custom_bb (host, 2*dim floats or NULL) plays customBB; clamped as the
+ * reference.  qpts_out: device, n*dim int64.  *n_out (host) receives n, or
+ * 1 when the box has zero size (Karras.cpp:84-88).  Synchronizes.
+ * PGVD_ERR_BAD_COORD where the reference throws. */
+int pgvd_quantize(pgvd_ctx *ctx, const float *pts, int64_t n, int dim, int bits,
+                  const float *custom_bb, int clamped, int64_t *qpts_out, int64_t *n_out,
+                  pgvd_stream s);
+
+/* Kernels::PointsToMorton_p (opencl/Kernels.cpp:47-60, kernel
+ * kernels.cl:2-22): keys_out (device) receives next_pow2(n) keys, the
+ * Morton keys of the n points followed by zero keys.  pts: device, n*dim
+ * integers of coord_bytes (4 or 8). */
+int pgvd_points_to_morton(pgvd_ctx *ctx, const void *pts, int coord_bytes, int64_t n, int dim,
+                          int bits, void *keys_out, int key_bytes, pgvd_stream s);
+
+/* Kernels::RadixSortBigUnsigned (opencl/Kernels.cpp:219-247): ascending
+ * stable sort of n keys in place, over the low mbits bits; vals (nullable,
+ * u32 per key) is permuted with the keys. */
+int pgvd_radix_sort(pgvd_ctx *ctx, void *keys, uint32_t *vals, int64_t n, int key_bytes,
+                    int mbits, pgvd_stream s);
+
+/* Kernels::UniqueSorted (opencl/Kernels.cpp:198-217): drop adjacent equal
+ * keys of a sorted array.  keys_out (device) has room for n keys; *m_out
+ * (host) receives the count.  Synchronizes (the reference does a blocking
+ * read of the count, :214). */
+int pgvd_unique_sorted(pgvd_ctx *ctx, const void *keys_in, void *keys_out, int64_t n,
+                       int key_bytes, int64_t *m_out, pgvd_stream s);
+
+/* Kernels::StreamScan_p / StreamScan_s (opencl/Kernels.cpp:102-159):
+ * inclusive scan of n u32 values, modulo 2^32. */
+int pgvd_stream_scan(pgvd_ctx *ctx, const uint32_t *in, uint32_t *out, int64_t n, pgvd_stream s);
+
+/* BRT node records (C/BrtNode.h:10-21 without `parent`, which is a separate
+ * int32 array with the reference's values: -1 for the root, 0 for a node no
+ * other node claims). */
+typedef struct pgvd_brt_node32 {
+  uint32_t lcp;
+  int32_t left;
+  uint16_t lcp_length;
+  uint8_t left_leaf, right_leaf;
+} pgvd_brt_node32;
+typedef struct pgvd_brt_node64 {
+  uint64_t lcp;
+  int32_t left;
+  uint16_t lcp_length;
+  uint8_t left_leaf, right_leaf;
+} pgvd_brt_node64;
+typedef struct pgvd_brt_node128 {
+  uint64_t lcp_lo, lcp_hi;
+  int32_t left;
+  uint16_t lcp_length;
+  uint8_t left_leaf, right_leaf;
+} pgvd_brt_node128;
+
+/* Kernels::BuildBinaryRadixTree_p (opencl/Kernels.cpp:249-264, kernel
+ * C/BuildBRT.c:90-181): m sorted unique keys -> m-1 nodes (brt_out, record
+ * type by key_bytes) and parent_out (m-1 int32). */
+int pgvd_build_brt(pgvd_ctx *ctx, const void *keys, int64_t m, int key_bytes, int mbits,
+                   void *brt_out, int32_t *parent_out, pgvd_stream s);
+
+/* Kernels::ComputeLocalSplits_p (opencl/Kernels.cpp:275-301, kernel
+ * C/BuildOctree.c:17-27): ls_out (device, m u32). */
+int pgvd_local_splits(pgvd_ctx *ctx, const void *brt, int64_t m, int key_bytes, int dim,
+                      uint32_t *ls_out, pgvd_stream s);
+
+/* Octree node (C/OctNode.h:16-26): children[1<<dim] int32 then leaf int32;
+ * 20 bytes in 2D, 36 in 3D.  leaf bit o is set when children[o] == -1 (the
+ * reference's 2D definition, extended to 8 octants in 3D). */
+
+/* Kernels::BinaryRadixToOctree_p (opencl/Kernels.cpp:333-368): local
+ * splits, inclusive scan and octree fill (C/BuildOctree.c:73-139) for a BRT
+ * of m keys.  octree_out (device) holds `capacity` nodes; *octree_size
+ * (host) receives the node count; PGVD_ERR_CAPACITY if it exceeds
+ * capacity.  Synchronizes (the reference reads the size back, :347). */
+int pgvd_brt_to_octree(pgvd_ctx *ctx, const void *brt, const int32_t *parent, int64_t m,
+                       int key_bytes, int dim, void *octree_out, int64_t capacity,
+                       int64_t *octree_size, pgvd_stream s);
+
+/* ---- whole path ---- */
+
+typedef struct pgvd_build_opts {
+  int use_custom_bb;        /* Karras::Quantize customBB (Karras.cpp:76-77)      */
+  float bb_min[3], bb_max[3];
+  int clamped;              /* Karras::Quantize clamped (Karras.cpp:46-49)       */
+  int quantized_input;      /* 1: pts are int32 grid coordinates (n*dim),
+                               as Kernels::BuildOctree_p takes vector<intn>     */
+} pgvd_build_opts;
+
+typedef struct pgvd_octree_result {
+  int64_t n;                /* input points                                      */
+  int64_t m;                /* unique keys (including the padding zero key)      */
+  int64_t octree_size;      /* octree nodes                                      */
+  int dim, bits, mbits, key_bytes;
+  int malformed;            /* 1 if some BRT split took the FP32-quirk value     */
+  /* device pointers owned by the context, valid until its next build:       */
+  const void *keys;         /* m sorted unique keys                              */
+  const void *brt;          /* m-1 pgvd_brt_node{32,64,128}                      */
+  const int32_t *parent;    /* m-1 parent links                                  */
+  const uint32_t *prefix_sums;   /* m inclusive prefix sums of local splits      */
+  const int32_t *octree;    /* octree_size nodes                                 */
+  const uint32_t *key_conflict;  /* bit r: run r has >1 label (labels given)     */
+  const uint32_t *node_conflict; /* per node: octant mask of conflicting leaves  */
+} pgvd_octree_result;
+
+/* Kernels::BuildOctree_p (opencl/Kernels.cpp:413-429) preceded by the
+ * bounding box and quantization of Karras::Quantize (Karras.cpp:69-99):
+ * float points (device, n*dim) -> octree, all on the device.  labels
+ * (device, n u32, nullable) are object ids carried through the sort for the
+ * conflict-cell pass.  Asynchronous: enqueues the whole path on the stream
+ * and returns; pgvd_build_octree_finish waits and reports the sizes. */
+int pgvd_build_octree(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
+                      const uint32_t *labels, const pgvd_build_opts *opts, pgvd_stream s);
+
+/* Waits for the last pgvd_build_octree on ctx, grows the octree buffer and
+ * refills it if it was too small, and fills *out.  Errors the device
+ * detected (PGVD_ERR_BAD_COORD, PGVD_ERR_DUP_KEY) are returned here. */
+int pgvd_build_octree_finish(pgvd_ctx *ctx, pgvd_octree_result *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PGVD_H */

@@ -1,0 +1,449 @@
+"""pgvd_amd -- Python binding of libpgvd.so, the MI355X (gfx950) octree
+construction path of edwardsjohnmartin/pgvd.
+
+The binding is ctypes over the C ABI declared in include/pgvd.h.  Method
+names follow the reference's stage drivers (namespace Kernels,
+opencl/Kernels.h:26-50) and facade (Karras.h:23-36):
+
+    Device.quantize           Karras::Quantize             (Karras.cpp:69-99)
+    Device.points_to_morton   Kernels::PointsToMorton_p    (opencl/Kernels.cpp:47-60)
+    Device.radix_sort         Kernels::RadixSortBigUnsigned(opencl/Kernels.cpp:219-247)
+    Device.unique_sorted      Kernels::UniqueSorted        (opencl/Kernels.cpp:198-217)
+    Device.stream_scan        Kernels::StreamScan_p        (opencl/Kernels.cpp:102-123)
+    Device.build_brt          Kernels::BuildBinaryRadixTree_p (opencl/Kernels.cpp:249-264)
+    Device.local_splits       Kernels::ComputeLocalSplits_p   (opencl/Kernels.cpp:275-301)
+    Device.brt_to_octree      Kernels::BinaryRadixToOctree_p  (opencl/Kernels.cpp:333-368)
+    Device.build_octree       Kernels::BuildOctree_p       (opencl/Kernels.cpp:413-429)
+
+Every call runs HIP kernels; there is no CPU fallback.  Importing works
+without a GPU (tests check the exported symbols); creating a Device needs
+one.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpgvd.so")
+ROOT = os.path.dirname(HERE)
+HEADER = os.path.join(ROOT, "include", "pgvd.h")
+
+STATUS = {
+    0: "PGVD_OK", 1: "PGVD_ERR_INVALID", 2: "PGVD_ERR_EMPTY", 3: "PGVD_ERR_BAD_COORD",
+    4: "PGVD_ERR_NO_DEVICE", 5: "PGVD_ERR_ALLOC", 6: "PGVD_ERR_HIP", 7: "PGVD_ERR_CAPACITY",
+    8: "PGVD_ERR_TOO_LARGE", 9: "PGVD_ERR_DUP_KEY", 10: "PGVD_ERR_INTERNAL",
+}
+
+
+class PgvdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+class OctreeResult(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64), ("m", C.c_int64), ("octree_size", C.c_int64),
+        ("dim", C.c_int), ("bits", C.c_int), ("mbits", C.c_int), ("key_bytes", C.c_int),
+        ("malformed", C.c_int),
+        ("keys", C.c_void_p), ("brt", C.c_void_p), ("parent", C.c_void_p),
+        ("prefix_sums", C.c_void_p), ("octree", C.c_void_p),
+        ("key_conflict", C.c_void_p), ("node_conflict", C.c_void_p),
+    ]
+
+
+class BuildOpts(C.Structure):
+    _fields_ = [
+        ("use_custom_bb", C.c_int), ("bb_min", C.c_float * 3), ("bb_max", C.c_float * 3),
+        ("clamped", C.c_int), ("quantized_input", C.c_int),
+    ]
+
+
+def lib():
+    """Load libpgvd.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m pgvd_amd.build` "
+                              "(or __graft_entry__.build()) first")
+        L = C.CDLL(LIB_PATH)
+        vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int
+        sig = {
+            "pgvd_last_error": ([], C.c_char_p),
+            "pgvd_version": ([], C.c_char_p),
+            "pgvd_next_pow2": ([i32], i32),
+            "pgvd_ctx_create": ([i32, C.POINTER(vp)], i32),
+            "pgvd_ctx_destroy": ([vp], i32),
+            "pgvd_ctx_stream": ([vp, C.POINTER(vp)], i32),
+            "pgvd_synchronize": ([vp, vp], i32),
+            "pgvd_alloc": ([vp, C.c_size_t, C.POINTER(vp)], i32),
+            "pgvd_free": ([vp, vp], i32),
+            "pgvd_memcpy_htod": ([vp, vp, vp, C.c_size_t, vp], i32),
+            "pgvd_memcpy_dtoh": ([vp, vp, vp, C.c_size_t, vp], i32),
+            "pgvd_copy_kernel": ([vp, vp, vp, C.c_size_t, vp], i32),
+            "pgvd_timing_enable": ([vp, i32], i32),
+            "pgvd_timing_reset": ([vp], i32),
+            "pgvd_timing_read": ([vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(i64)], i32),
+            "pgvd_bbox": ([vp, vp, i64, i32, vp, vp], i32),
+            "pgvd_quantize": ([vp, vp, i64, i32, i32, vp, i32, vp, C.POINTER(i64), vp], i32),
+            "pgvd_points_to_morton": ([vp, vp, i32, i64, i32, i32, vp, i32, vp], i32),
+            "pgvd_radix_sort": ([vp, vp, vp, i64, i32, i32, vp], i32),
+            "pgvd_unique_sorted": ([vp, vp, vp, i64, i32, C.POINTER(i64), vp], i32),
+            "pgvd_stream_scan": ([vp, vp, vp, i64, vp], i32),
+            "pgvd_build_brt": ([vp, vp, i64, i32, i32, vp, vp, vp], i32),
+            "pgvd_local_splits": ([vp, vp, i64, i32, i32, vp, vp], i32),
+            "pgvd_brt_to_octree": ([vp, vp, vp, i64, i32, i32, vp, i64, C.POINTER(i64), vp], i32),
+            "pgvd_build_octree": ([vp, vp, i64, i32, i32, vp, C.POINTER(BuildOpts), vp], i32),
+            "pgvd_build_octree_finish": ([vp, C.POINTER(OctreeResult)], i32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols() -> list[str]:
+    """Function names declared in include/pgvd.h."""
+    import re
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(pgvd_\w+)\s*\(", text, re.M)))
+
+
+def key_bytes_for(mbits: int) -> int:
+    return 4 if mbits <= 32 else (8 if mbits <= 64 else 16)
+
+
+def key_dtype(key_bytes: int):
+    return {4: np.uint32, 8: np.uint64, 16: np.uint64}[key_bytes]
+
+
+BRT_DTYPES = {
+    4: np.dtype([("lcp", "<u4"), ("left", "<i4"), ("lcp_length", "<u2"),
+                 ("left_leaf", "u1"), ("right_leaf", "u1")]),
+    8: np.dtype([("lcp", "<u8"), ("left", "<i4"), ("lcp_length", "<u2"),
+                 ("left_leaf", "u1"), ("right_leaf", "u1")]),
+    16: np.dtype([("lcp_lo", "<u8"), ("lcp_hi", "<u8"), ("left", "<i4"), ("lcp_length", "<u2"),
+                  ("left_leaf", "u1"), ("right_leaf", "u1")]),
+}
+assert BRT_DTYPES[4].itemsize == 12 and BRT_DTYPES[8].itemsize == 16
+# pgvd_brt_node128 is 8-byte aligned: 16 + 4 + 2 + 1 + 1 = 24 bytes
+assert BRT_DTYPES[16].itemsize == 24
+
+
+def keys_to_words(lo: np.ndarray, hi, key_bytes: int) -> np.ndarray:
+    """(lo, hi) u64 key halves -> the device key array layout."""
+    lo = np.asarray(lo, np.uint64)
+    if key_bytes == 4:
+        return lo.astype(np.uint32)
+    if key_bytes == 8:
+        return lo.copy()
+    hi = np.zeros_like(lo) if hi is None else np.asarray(hi, np.uint64)
+    out = np.empty(lo.size * 2, np.uint64)
+    out[0::2] = lo
+    out[1::2] = hi
+    return out
+
+
+def words_to_keys(words: np.ndarray, key_bytes: int):
+    """device key array -> (lo, hi) u64 halves."""
+    if key_bytes == 16:
+        return words[0::2].copy(), words[1::2].copy()
+    lo = words.astype(np.uint64)
+    return lo, np.zeros_like(lo)
+
+
+@dataclass
+class DeviceBuffer:
+    dev: "Device"
+    ptr: int
+    nbytes: int
+
+    def free(self):
+        if self.ptr:
+            self.dev._check(lib().pgvd_free(self.dev.ctx, self.ptr))
+            self.ptr = 0
+
+
+class Device:
+    """One pgvd context (one HIP stream and a keyed buffer pool) on a GPU."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        ctx = C.c_void_p()
+        rc = L.pgvd_ctx_create(device, C.byref(ctx))
+        if rc != 0:
+            raise PgvdError(rc, L.pgvd_last_error().decode())
+        self.ctx = ctx
+        self._bufs: list[DeviceBuffer] = []
+
+    # -- plumbing ---------------------------------------------------------
+    def _check(self, rc: int):
+        if rc != 0:
+            raise PgvdError(rc, lib().pgvd_last_error().decode())
+
+    def close(self):
+        if self.ctx:
+            for b in self._bufs:
+                if b.ptr:
+                    lib().pgvd_free(self.ctx, b.ptr)
+                    b.ptr = 0
+            lib().pgvd_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        p = C.c_void_p()
+        self._check(lib().pgvd_alloc(self.ctx, max(int(nbytes), 1), C.byref(p)))
+        b = DeviceBuffer(self, p.value, int(nbytes))
+        self._bufs.append(b)
+        return b
+
+    def upload(self, arr: np.ndarray) -> DeviceBuffer:
+        arr = np.ascontiguousarray(arr)
+        b = self.alloc(arr.nbytes)
+        if arr.nbytes:
+            self._check(lib().pgvd_memcpy_htod(self.ctx, b.ptr, arr.ctypes.data, arr.nbytes, None))
+        return b
+
+    def download(self, ptr, dtype, count: int) -> np.ndarray:
+        out = np.empty(int(count), dtype)
+        p = ptr.ptr if isinstance(ptr, DeviceBuffer) else ptr
+        if out.nbytes:
+            self._check(lib().pgvd_memcpy_dtoh(self.ctx, out.ctypes.data, p, out.nbytes, None))
+        return out
+
+    def synchronize(self):
+        self._check(lib().pgvd_synchronize(self.ctx, None))
+
+    def copy_kernel(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int):
+        self._check(lib().pgvd_copy_kernel(self.ctx, dst.ptr, src.ptr, nbytes, None))
+
+    def timing(self, on: bool = True):
+        self._check(lib().pgvd_timing_enable(self.ctx, int(on)))
+
+    def timing_reset(self):
+        self._check(lib().pgvd_timing_reset(self.ctx))
+
+    def timing_read(self, stage: str):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        self._check(lib().pgvd_timing_read(self.ctx, stage.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+    # -- stages -----------------------------------------------------------
+    def bbox(self, pts: np.ndarray):
+        pts = np.ascontiguousarray(pts, np.float32)
+        n, dim = pts.shape
+        d = self.upload(pts)
+        out = np.zeros(2 * dim, np.float32)
+        self._check(lib().pgvd_bbox(self.ctx, d.ptr, n, dim, out.ctypes.data, None))
+        d.free()
+        return out[:dim], out[dim:]
+
+    def quantize(self, pts: np.ndarray, bits: int, bb=None, clamped: bool = False) -> np.ndarray:
+        pts = np.ascontiguousarray(pts, np.float32)
+        n, dim = pts.shape
+        d = self.upload(pts)
+        q = self.alloc(n * dim * 8)
+        nout = C.c_int64()
+        cbb = None
+        if bb is not None:
+            cbb_arr = np.concatenate([np.asarray(bb[0], np.float32), np.asarray(bb[1], np.float32)])
+            cbb = cbb_arr.ctypes.data
+        try:
+            self._check(lib().pgvd_quantize(self.ctx, d.ptr, n, dim, bits, cbb, int(clamped),
+                                            q.ptr, C.byref(nout), None))
+            out = self.download(q, np.int64, n * dim).reshape(n, dim)
+        finally:
+            d.free()
+            q.free()
+        return out[: nout.value]
+
+    def points_to_morton(self, q: np.ndarray, bits: int, key_bytes: int | None = None):
+        """Keys of integer points, padded with zero keys to next_pow2(n)."""
+        q = np.ascontiguousarray(q)
+        n, dim = q.shape
+        cb = q.dtype.itemsize
+        kb = key_bytes or key_bytes_for(dim * bits)
+        padded = lib().pgvd_next_pow2(n)
+        d = self.upload(q)
+        k = self.alloc(padded * kb)
+        try:
+            self._check(lib().pgvd_points_to_morton(self.ctx, d.ptr, cb, n, dim, bits, k.ptr, kb, None))
+            words = self.download(k, key_dtype(kb), padded * (2 if kb == 16 else 1))
+        finally:
+            d.free()
+            k.free()
+        return words_to_keys(words, kb)
+
+    def radix_sort(self, lo, hi=None, mbits: int = 64, vals=None, key_bytes: int | None = None):
+        kb = key_bytes or key_bytes_for(mbits)
+        words = keys_to_words(lo, hi, kb)
+        n = np.asarray(lo).size
+        dk = self.upload(words)
+        dv = self.upload(np.ascontiguousarray(vals, np.uint32)) if vals is not None else None
+        try:
+            self._check(lib().pgvd_radix_sort(self.ctx, dk.ptr, dv.ptr if dv else None, n, kb,
+                                              mbits, None))
+            out = words_to_keys(self.download(dk, key_dtype(kb), words.size), kb)
+            outv = self.download(dv, np.uint32, n) if dv else None
+        finally:
+            dk.free()
+            if dv:
+                dv.free()
+        return (out, outv) if vals is not None else out
+
+    def unique_sorted(self, lo, hi=None, key_bytes: int = 8):
+        words = keys_to_words(lo, hi, key_bytes)
+        n = np.asarray(lo).size
+        di = self.upload(words)
+        do = self.alloc(max(words.nbytes, 1))
+        m = C.c_int64()
+        try:
+            self._check(lib().pgvd_unique_sorted(self.ctx, di.ptr, do.ptr, n, key_bytes,
+                                                 C.byref(m), None))
+            out = self.download(do, key_dtype(key_bytes), m.value * (2 if key_bytes == 16 else 1))
+        finally:
+            di.free()
+            do.free()
+        return words_to_keys(out, key_bytes)
+
+    def stream_scan(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.uint32)
+        di = self.upload(x)
+        do = self.alloc(max(x.nbytes, 4))
+        try:
+            self._check(lib().pgvd_stream_scan(self.ctx, di.ptr, do.ptr, x.size, None))
+            return self.download(do, np.uint32, x.size)
+        finally:
+            di.free()
+            do.free()
+
+    def build_brt(self, lo, hi, mbits: int, key_bytes: int | None = None):
+        kb = key_bytes or key_bytes_for(mbits)
+        words = keys_to_words(lo, hi, kb)
+        m = np.asarray(lo).size
+        dk = self.upload(words)
+        nn = max(m - 1, 1)
+        db = self.alloc(nn * BRT_DTYPES[kb].itemsize)
+        dp = self.alloc(nn * 4)
+        try:
+            self._check(lib().pgvd_build_brt(self.ctx, dk.ptr, m, kb, mbits, db.ptr, dp.ptr, None))
+            raw = self.download(db, np.uint8, nn * BRT_DTYPES[kb].itemsize)
+            brt = raw.view(BRT_DTYPES[kb])[: m - 1]
+            parent = self.download(dp, np.int32, nn)[: m - 1]
+        finally:
+            dk.free()
+            db.free()
+            dp.free()
+        return brt, parent
+
+    def local_splits(self, brt: np.ndarray, m: int, dim: int, key_bytes: int) -> np.ndarray:
+        db = self.upload(np.ascontiguousarray(brt).view(np.uint8))
+        dl = self.alloc(m * 4)
+        try:
+            self._check(lib().pgvd_local_splits(self.ctx, db.ptr, m, key_bytes, dim, dl.ptr, None))
+            return self.download(dl, np.uint32, m)
+        finally:
+            db.free()
+            dl.free()
+
+    def brt_to_octree(self, brt: np.ndarray, parent: np.ndarray, m: int, dim: int,
+                      key_bytes: int, capacity: int | None = None):
+        nc = 1 << dim
+        cap = capacity if capacity is not None else max(2 * m, 16)
+        db = self.upload(np.ascontiguousarray(brt).view(np.uint8) if m > 1 else np.zeros(64, np.uint8))
+        dp = self.upload(np.ascontiguousarray(parent, np.int32) if m > 1 else np.zeros(1, np.int32))
+        do = self.alloc(cap * (nc + 1) * 4)
+        osz = C.c_int64()
+        try:
+            rc = lib().pgvd_brt_to_octree(self.ctx, db.ptr, dp.ptr, m, key_bytes, dim, do.ptr,
+                                          cap, C.byref(osz), None)
+            self._check(rc)
+            raw = self.download(do, np.int32, osz.value * (nc + 1)).reshape(osz.value, nc + 1)
+        finally:
+            db.free()
+            dp.free()
+            do.free()
+        return raw[:, :nc].copy(), raw[:, nc].copy()
+
+    def build_octree_async(self, pts: DeviceBuffer, n: int, dim: int, bits: int,
+                           labels: DeviceBuffer | None = None, custom_bb=None,
+                           clamped: bool = False, quantized_input: bool = False):
+        opts = BuildOpts()
+        if custom_bb is not None:
+            opts.use_custom_bb = 1
+            for k in range(dim):
+                opts.bb_min[k] = float(custom_bb[0][k])
+                opts.bb_max[k] = float(custom_bb[1][k])
+        opts.clamped = int(clamped)
+        opts.quantized_input = int(quantized_input)
+        self._check(lib().pgvd_build_octree(self.ctx, pts.ptr, n, dim, bits,
+                                            labels.ptr if labels else None, C.byref(opts), None))
+
+    def build_octree_finish(self) -> OctreeResult:
+        r = OctreeResult()
+        self._check(lib().pgvd_build_octree_finish(self.ctx, C.byref(r)))
+        return r
+
+    def fetch(self, r: OctreeResult) -> dict:
+        """Download every output of a finished build into numpy arrays."""
+        kb, m, O, dim = r.key_bytes, r.m, r.octree_size, r.dim
+        nc = 1 << dim
+        words = self.download(r.keys, key_dtype(kb), m * (2 if kb == 16 else 1))
+        lo, hi = words_to_keys(words, kb)
+        ni = max(m - 1, 0)
+        brt = self.download(r.brt, np.uint8, ni * BRT_DTYPES[kb].itemsize).view(BRT_DTYPES[kb])
+        parent = self.download(r.parent, np.int32, ni)
+        ps = self.download(r.prefix_sums, np.uint32, m)
+        raw = self.download(r.octree, np.int32, O * (nc + 1)).reshape(O, nc + 1)
+        out = dict(m=m, octree_size=O, malformed=r.malformed, keys_lo=lo, keys_hi=hi,
+                   brt=brt, parent=parent, prefix_sums=ps,
+                   children=raw[:, :nc].copy(), leaf=raw[:, nc].copy())
+        if kb == 16:
+            out["lcp_lo"], out["lcp_hi"] = brt["lcp_lo"].copy(), brt["lcp_hi"].copy()
+        else:
+            out["lcp_lo"], out["lcp_hi"] = brt["lcp"].astype(np.uint64), np.zeros(ni, np.uint64)
+        out["left"] = brt["left"].copy()
+        out["left_leaf"] = brt["left_leaf"].copy()
+        out["right_leaf"] = brt["right_leaf"].copy()
+        out["lcp_length"] = brt["lcp_length"].astype(np.int32)
+        if r.key_conflict:
+            bits = self.download(r.key_conflict, np.uint32, (m + 31) // 32)
+            flags = np.unpackbits(bits.view(np.uint8), bitorder="little")[:m]
+            out["key_conflict"] = flags.astype(np.uint8)
+            out["node_conflict"] = self.download(r.node_conflict, np.uint32, O)
+        return out
+
+    def build_octree(self, pts: np.ndarray, bits: int, labels=None, custom_bb=None,
+                     clamped: bool = False) -> dict:
+        """Whole path from host points (float n x dim, or int32 grid points)."""
+        pts = np.ascontiguousarray(pts)
+        n, dim = pts.shape
+        quant = pts.dtype == np.int32
+        if not quant:
+            pts = pts.astype(np.float32, copy=False)
+        dp = self.upload(pts)
+        dl = self.upload(np.ascontiguousarray(labels, np.uint32)) if labels is not None else None
+        try:
+            self.build_octree_async(dp, n, dim, bits, dl, custom_bb, clamped, quant)
+            r = self.build_octree_finish()
+            return self.fetch(r)
+        finally:
+            dp.free()
+            if dl:
+                dl.free()

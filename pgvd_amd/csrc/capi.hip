@@ -1,0 +1,874 @@
+// capi.hip -- the C ABI of libpgvd.so (include/pgvd.h): context, keyed
+// device-buffer pool, stage entry points and the whole-path pipeline.
+//
+// The pipeline mirrors Kernels::BuildOctree_p (opencl/Kernels.cpp:413-429)
+// with the bounding box and quantization of Karras::Quantize
+// (Karras.cpp:69-99) in front, but keeps every intermediate on the device:
+// the reference's two blocking reads (unique count, :214; octree size,
+// :347) are replaced by device-side counts that later kernels read, so the
+// whole path is one stream of launches and one small readback at the end.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/pgvd.h"
+#include "kernels.hpp"
+
+using namespace pgvd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(PGVD_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+#define CHK(x)                    \
+  do {                            \
+    int r_ = (x);                 \
+    if (r_ != PGVD_OK) return r_; \
+  } while (0)
+
+inline float fdec_host(uint32_t e) {
+  const uint32_t b = (e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e;
+  float f;
+  std::memcpy(&f, &b, 4);
+  return f;
+}
+
+inline int key_bytes_for(int mbits) { return mbits <= 32 ? 4 : (mbits <= 64 ? 8 : 16); }
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// opencl/Kernels.cpp:20
+int ref_next_pow2(int num) {
+  const int p = (int)pow(2, ceil(log(num) / log(2)));
+  return p > 8 ? p : 8;
+}
+
+struct PoolBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct TimingRec {
+  std::string stage;
+  hipEvent_t a, b;
+};
+
+struct LastBuild {
+  bool valid = false;
+  int64_t n = 0;
+  int dim = 0, bits = 0, mbits = 0, kb = 0;
+  bool labels = false;
+  void *keys_u = nullptr;
+  void *brt = nullptr;
+  int32_t *parent = nullptr;
+  uint32_t *ps = nullptr;
+  uint32_t *conflict_bits = nullptr;
+  hipStream_t stream = nullptr;
+};
+
+}  // namespace
+
+struct pgvd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::unordered_map<std::string, PoolBuf> pool;
+  RunState *st = nullptr;       // device
+  RunState *st_host = nullptr;  // pinned mirror
+  hipEvent_t done = nullptr;
+  bool timing = false;
+  std::vector<TimingRec> recs;
+  std::vector<hipEvent_t> spare;
+  LastBuild last;
+  int64_t octree_cap = 0;       // nodes the pooled octree buffer holds
+  int octree_stride = 0;        // ints per node of that buffer
+};
+
+namespace {
+
+int use_device(pgvd_ctx *ctx) {
+  HIPCHK(hipSetDevice(ctx->device));
+  return PGVD_OK;
+}
+
+hipStream_t pick(pgvd_ctx *ctx, pgvd_stream s) {
+  return s ? static_cast<hipStream_t>(s) : ctx->stream;
+}
+
+// Keyed pool (CLFW::get, CLFW/src/clfw.cpp:212-226): reuse when large
+// enough, else reallocate with headroom.
+int pool_get(pgvd_ctx *ctx, const char *key, size_t bytes, void **out) {
+  if (bytes == 0) bytes = 256;
+  PoolBuf &b = ctx->pool[key];
+  if (b.bytes < bytes) {
+    if (b.p) {
+      // the old buffer may be in use on any stream the caller passed
+      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(hipFree(b.p));
+      b.p = nullptr;
+      b.bytes = 0;
+    }
+    const size_t want = align256(bytes + bytes / 8);
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+      b.p = nullptr;
+      return fail(PGVD_ERR_ALLOC, std::string("hipMalloc failed for pool buffer ") + key);
+    }
+    b.bytes = want;
+  }
+  *out = b.p;
+  return PGVD_OK;
+}
+
+hipEvent_t take_event(pgvd_ctx *ctx) {
+  if (!ctx->spare.empty()) {
+    hipEvent_t e = ctx->spare.back();
+    ctx->spare.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Records HIP events around a stage's launches when timing is on.
+struct Stage {
+  pgvd_ctx *ctx;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  const char *name;
+  Stage(pgvd_ctx *c, const char *nm, hipStream_t st) : ctx(c), s(st), name(nm) {
+    if (ctx->timing) {
+      a = take_event(ctx);
+      if (a) (void)hipEventRecord(a, s);
+    }
+  }
+  ~Stage() {
+    if (a) {
+      hipEvent_t b = take_event(ctx);
+      if (b) {
+        (void)hipEventRecord(b, s);
+        ctx->recs.push_back({name, a, b});
+      }
+    }
+  }
+};
+
+// Events around each onesweep pass, recorded as separate "sort_pass" stage
+// entries so that a per-launch average can be formed.
+struct PassTimer {
+  pgvd_ctx *ctx;
+  std::vector<hipEvent_t> ev;
+  PassTimer(pgvd_ctx *c, int passes) : ctx(c) {
+    if (!ctx->timing) return;
+    for (int i = 0; i < 2 * passes; ++i) {
+      hipEvent_t e = take_event(ctx);
+      if (!e) { ev.clear(); return; }
+      ev.push_back(e);
+    }
+  }
+  hipEvent_t *events() { return ev.empty() ? nullptr : ev.data(); }
+  ~PassTimer() {
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) ctx->recs.push_back({"sort_pass", ev[i], ev[i + 1]});
+  }
+};
+
+int launch_status(const char *what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(PGVD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return PGVD_OK;
+}
+
+// Workspace that must be zero at the start of a run, as one block.
+struct ZeroLayout {
+  size_t hist = 0, sort_lb = 0, lb_unique = 0, lb_splits = 0, lb_scan = 0, parent = 0,
+         cbits = 0, total = 0;
+};
+
+ZeroLayout zero_layout(int64_t n, int kb, int passes, bool labels, bool with_tree) {
+  ZeroLayout z;
+  size_t off = 0;
+  z.hist = off;
+  off += align256((size_t)kMaxPasses * 256 * 4);
+  z.sort_lb = off;
+  const int64_t tiles = onesweep_tiles(kb, labels, n > 0 ? n : 1);
+  off += align256((size_t)passes * (size_t)tiles * 256 * 4);
+  z.lb_unique = off;
+  off += align256((size_t)scan_tiles(n + 1) * 8);
+  z.lb_splits = off;
+  off += align256((size_t)scan_tiles(n + 1) * 8);
+  z.lb_scan = off;
+  off += align256((size_t)scan_tiles(n + 1) * 8);
+  z.parent = off;
+  if (with_tree) off += align256((size_t)(n + 1) * 4);
+  z.cbits = off;
+  if (labels) off += align256((size_t)((n + 1 + 31) / 32) * 4);
+  z.total = off;
+  return z;
+}
+
+int copy_state_and_wait(pgvd_ctx *ctx, hipStream_t s) {
+  HIPCHK(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(RunState), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return PGVD_OK;
+}
+
+int status_error(uint32_t st) {
+  if (st & ST_NEG_COORD) return fail(PGVD_ERR_BAD_COORD, "bad coordinate: a point quantized below zero");
+  if (st & ST_DUP_KEY) return fail(PGVD_ERR_DUP_KEY, "BRT input has equal neighbouring keys");
+  if (st & (ST_STACK | ST_WALK)) return fail(PGVD_ERR_INTERNAL, "octree walk exceeded its bound");
+  if (st & ST_TOO_LARGE) return fail(PGVD_ERR_TOO_LARGE, "count exceeds 2^30");
+  return PGVD_OK;
+}
+
+int check_dims(int dim, int bits) {
+  if (dim != 2 && dim != 3) return fail(PGVD_ERR_INVALID, "dim must be 2 or 3");
+  if (bits < 1 || bits * dim > 128) return fail(PGVD_ERR_INVALID, "bits must satisfy 1 <= bits*dim <= 128");
+  return PGVD_OK;
+}
+
+int check_n(int64_t n) {
+  if (n <= 0) return fail(PGVD_ERR_EMPTY, "Zero points not supported");
+  if (n >= (int64_t(1) << 30) - 1) return fail(PGVD_ERR_TOO_LARGE, "n must be below 2^30 - 1");
+  return PGVD_OK;
+}
+
+template <typename K>
+size_t brt_node_bytes() { return sizeof(BrtNodeT<K>); }
+
+// ---------------------------------------------------------------------------
+// whole path
+// ---------------------------------------------------------------------------
+template <typename K>
+int ensure_octree(pgvd_ctx *ctx, int dim, int64_t nodes, bool labels, int32_t **oct,
+                  uint32_t **nconf) {
+  const int stride = (1 << dim) + 1;
+  void *p = nullptr;
+  CHK(pool_get(ctx, "octree", (size_t)nodes * stride * 4, &p));
+  *oct = static_cast<int32_t *>(p);
+  ctx->octree_cap = (int64_t)(ctx->pool["octree"].bytes / ((size_t)stride * 4));
+  ctx->octree_stride = stride;
+  *nconf = nullptr;
+  if (labels) {
+    CHK(pool_get(ctx, "node_conflict", (size_t)ctx->octree_cap * 4, &p));
+    *nconf = static_cast<uint32_t *>(p);
+  }
+  return PGVD_OK;
+}
+
+template <typename K>
+int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
+               const uint32_t *labels, const pgvd_build_opts *opts, hipStream_t s) {
+  const int mbits = dim * bits;
+  const int kb = (int)sizeof(K);
+  const int passes = (mbits + 7) / 8;
+  const int prepend = ref_next_pow2((int)n) != n ? 1 : 0;   // zero-key padding (Q1)
+  const int64_t nk = n + 1;
+  const bool lab = labels != nullptr;
+  void *p;
+  CHK(pool_get(ctx, "keys_a", (size_t)nk * kb, &p));
+  K *kA = static_cast<K *>(p);
+  CHK(pool_get(ctx, "keys_b", (size_t)nk * kb, &p));
+  K *kB = static_cast<K *>(p);
+  uint32_t *vA = nullptr, *vB = nullptr;
+  if (lab) {
+    CHK(pool_get(ctx, "vals_a", (size_t)n * 4, &p));
+    vA = static_cast<uint32_t *>(p);
+    CHK(pool_get(ctx, "vals_b", (size_t)n * 4, &p));
+    vB = static_cast<uint32_t *>(p);
+  }
+  CHK(pool_get(ctx, "brt", (size_t)nk * brt_node_bytes<K>(), &p));
+  BrtNodeT<K> *brt = static_cast<BrtNodeT<K> *>(p);
+  CHK(pool_get(ctx, "ps", (size_t)nk * 4, &p));
+  uint32_t *ps = static_cast<uint32_t *>(p);
+  const ZeroLayout z = zero_layout(n, kb, passes, lab, true);
+  CHK(pool_get(ctx, "zero", z.total, &p));
+  char *zb = static_cast<char *>(p);
+  uint32_t *hist = reinterpret_cast<uint32_t *>(zb + z.hist);
+  uint32_t *sort_lb = reinterpret_cast<uint32_t *>(zb + z.sort_lb);
+  uint64_t *lb_u = reinterpret_cast<uint64_t *>(zb + z.lb_unique);
+  uint64_t *lb_s = reinterpret_cast<uint64_t *>(zb + z.lb_splits);
+  int32_t *parent = reinterpret_cast<int32_t *>(zb + z.parent);
+  uint32_t *cbits = lab ? reinterpret_cast<uint32_t *>(zb + z.cbits) : nullptr;
+  int32_t *octree;
+  uint32_t *nconf;
+  int64_t want = 2 * nk;
+  if (want < 1024) want = 1024;
+  if (ctx->octree_cap < want || ctx->octree_stride != (1 << dim) + 1 ||
+      (lab && ctx->pool["node_conflict"].bytes < (size_t)ctx->octree_cap * 4))
+    CHK(ensure_octree<K>(ctx, dim, want, lab, &octree, &nconf));
+  octree = static_cast<int32_t *>(ctx->pool["octree"].p);
+  nconf = lab ? static_cast<uint32_t *>(ctx->pool["node_conflict"].p) : nullptr;
+
+  const bool quantized = opts && opts->quantized_input;
+  const bool custom = opts && opts->use_custom_bb;
+  const bool clamped = opts && opts->clamped;
+  {
+    Stage t(ctx, "prep", s);
+    launch_prep(ctx->st, reinterpret_cast<uint32_t *>(zb), z.total / 4, s);
+  }
+  CHK(launch_status("prep"));
+  if (quantized) {
+    Stage t(ctx, "encode", s);
+    launch_encode_int<K>(static_cast<const int32_t *>(pts), n, dim, bits, kA, hist, passes, s);
+  } else {
+    if (!custom) {
+      Stage t(ctx, "bbox", s);
+      launch_bbox(static_cast<const float *>(pts), n, dim, ctx->st, s);
+    }
+    const int64_t width = (int64_t)1 << bits;
+    const float effw = (float)(clamped ? width : width - 1);
+    Stage t(ctx, "encode", s);
+    launch_encode<K>(static_cast<const float *>(pts), n, dim, bits, effw, custom ? 1 : 0,
+                     custom ? opts->bb_min : nullptr, custom ? opts->bb_max : nullptr, kA,
+                     nullptr, hist, passes, ctx->st, s);
+  }
+  CHK(launch_status("encode"));
+  int parity;
+  {
+    PassTimer pt(ctx, n > 1 ? passes : 0);
+    parity = launch_onesweep<K>(kA, kB, labels, vA, vB, n, passes, hist, sort_lb, ctx->st, s,
+                                pt.events());
+  }
+  CHK(launch_status("onesweep"));
+  K *sorted = parity ? kB : kA;
+  K *uniq = parity ? kA : kB;
+  const uint32_t *sortedv = lab ? (n <= 1 ? labels : (parity ? vB : vA)) : nullptr;
+  {
+    Stage t(ctx, "unique", s);
+    launch_unique<K>(sorted, sortedv, n, prepend, uniq, cbits, lb_u, ctx->st, s);
+  }
+  CHK(launch_status("unique"));
+  {
+    Stage t(ctx, "brt", s);
+    launch_brt<K>(uniq, nk, -1, mbits, brt, parent, ctx->st, s);
+  }
+  CHK(launch_status("brt"));
+  {
+    Stage t(ctx, "splits_scan", s);
+    launch_splits_scan<K>(brt, parent, nullptr, ps, nk, dim, lb_s, ctx->st, s);
+  }
+  CHK(launch_status("splits_scan"));
+  {
+    Stage t(ctx, "octree", s);
+    launch_octree<K>(brt, parent, ps, uniq, cbits, nk, dim, mbits, octree, nconf,
+                     ctx->octree_cap, 0, ctx->st, s);
+  }
+  CHK(launch_status("octree"));
+  HIPCHK(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(RunState), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(ctx->done, s));
+  LastBuild &L = ctx->last;
+  L.valid = true;
+  L.n = n;
+  L.dim = dim;
+  L.bits = bits;
+  L.mbits = mbits;
+  L.kb = kb;
+  L.labels = lab;
+  L.keys_u = uniq;
+  L.brt = brt;
+  L.parent = parent;
+  L.ps = ps;
+  L.conflict_bits = cbits;
+  L.stream = s;
+  return PGVD_OK;
+}
+
+template <typename K>
+int refill_octree(pgvd_ctx *ctx, int64_t nodes) {
+  LastBuild &L = ctx->last;
+  int32_t *octree;
+  uint32_t *nconf;
+  CHK(ensure_octree<K>(ctx, L.dim, nodes, L.labels, &octree, &nconf));
+  // clear the capacity bit and refill from the kept BRT / prefix sums
+  ctx->st_host->status &= ~(uint32_t)ST_CAPACITY;
+  HIPCHK(hipMemcpyAsync(&ctx->st->status, &ctx->st_host->status, 4, hipMemcpyHostToDevice,
+                        L.stream));
+  launch_octree<K>(static_cast<const BrtNodeT<K> *>(L.brt), L.parent, L.ps,
+                   static_cast<const K *>(L.keys_u), L.conflict_bits, L.n + 1, L.dim, L.mbits,
+                   octree, nconf, ctx->octree_cap, 0, ctx->st, L.stream);
+  CHK(launch_status("octree refill"));
+  return copy_state_and_wait(ctx, L.stream);
+}
+
+// ---------------------------------------------------------------------------
+// stage helpers
+// ---------------------------------------------------------------------------
+template <typename K>
+int radix_sort_impl(pgvd_ctx *ctx, K *keys, uint32_t *vals, int64_t n, int mbits, hipStream_t s) {
+  const int passes = (mbits + 7) / 8;
+  void *p;
+  CHK(pool_get(ctx, "sort_alt_k", (size_t)n * sizeof(K), &p));
+  K *alt = static_cast<K *>(p);
+  uint32_t *altv = nullptr;
+  if (vals) {
+    CHK(pool_get(ctx, "sort_alt_v", (size_t)n * 4, &p));
+    altv = static_cast<uint32_t *>(p);
+  }
+  const ZeroLayout z = zero_layout(n, (int)sizeof(K), passes, vals != nullptr, false);
+  CHK(pool_get(ctx, "zero_stage", z.total, &p));
+  char *zb = static_cast<char *>(p);
+  uint32_t *hist = reinterpret_cast<uint32_t *>(zb + z.hist);
+  launch_prep(ctx->st, reinterpret_cast<uint32_t *>(zb), z.total / 4, s);
+  launch_histogram<K>(keys, n, hist, passes, s);
+  int parity;
+  {
+    PassTimer pt(ctx, passes);
+    parity = launch_onesweep<K>(keys, alt, vals, vals, altv, n, passes, hist,
+                                reinterpret_cast<uint32_t *>(zb + z.sort_lb), ctx->st, s,
+                                pt.events());
+  }
+  CHK(launch_status("onesweep"));
+  if (parity) {
+    HIPCHK(hipMemcpyAsync(keys, alt, (size_t)n * sizeof(K), hipMemcpyDeviceToDevice, s));
+    if (vals) HIPCHK(hipMemcpyAsync(vals, altv, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+  }
+  return PGVD_OK;
+}
+
+template <typename K>
+int unique_impl(pgvd_ctx *ctx, const K *in, K *out, int64_t n, int64_t *m_out, hipStream_t s) {
+  const ZeroLayout z = zero_layout(n, (int)sizeof(K), 0, false, false);
+  void *p;
+  CHK(pool_get(ctx, "zero_stage", z.total, &p));
+  char *zb = static_cast<char *>(p);
+  launch_prep(ctx->st, reinterpret_cast<uint32_t *>(zb), z.total / 4, s);
+  {
+    Stage t(ctx, "unique", s);
+    launch_unique<K>(in, nullptr, n, 0, out, nullptr, reinterpret_cast<uint64_t *>(zb + z.lb_unique),
+                     ctx->st, s);
+  }
+  CHK(launch_status("unique"));
+  CHK(copy_state_and_wait(ctx, s));
+  *m_out = ctx->st_host->m;
+  return PGVD_OK;
+}
+
+template <typename K>
+int brt_impl(pgvd_ctx *ctx, const K *keys, int64_t m, int mbits, BrtNodeT<K> *brt,
+             int32_t *parent, hipStream_t s) {
+  launch_prep(ctx->st, nullptr, 0, s);
+  if (m > 1) {
+    launch_fill_u32(reinterpret_cast<uint32_t *>(parent), (size_t)(m - 1), 0u, s);
+    Stage t(ctx, "brt", s);
+    launch_brt<K>(keys, m, m, mbits, brt, parent, ctx->st, s);
+    launch_finalize_parents(parent, m, m, ctx->st, s);
+  }
+  CHK(launch_status("brt"));
+  CHK(copy_state_and_wait(ctx, s));
+  return status_error(ctx->st_host->status);
+}
+
+template <typename K>
+int local_splits_impl(pgvd_ctx *ctx, const BrtNodeT<K> *brt, int64_t m, int dim, uint32_t *ls,
+                      hipStream_t s) {
+  void *p;
+  CHK(pool_get(ctx, "ls_tmp", (size_t)m * 8, &p));
+  launch_fill_u32(static_cast<uint32_t *>(p), (size_t)m * 2, 0u, s);
+  launch_local_splits<K>(brt, m, dim, ls, static_cast<uint64_t *>(p), s);
+  return launch_status("local splits");
+}
+
+template <typename K>
+int brt_to_octree_impl(pgvd_ctx *ctx, const BrtNodeT<K> *brt, const int32_t *parent, int64_t m,
+                       int dim, int32_t *octree, int64_t capacity, int64_t *octree_size,
+                       hipStream_t s) {
+  void *p;
+  CHK(pool_get(ctx, "stage_ls", (size_t)m * 4, &p));
+  uint32_t *ls = static_cast<uint32_t *>(p);
+  CHK(pool_get(ctx, "stage_ps", (size_t)m * 4, &p));
+  uint32_t *ps = static_cast<uint32_t *>(p);
+  const ZeroLayout z = zero_layout(m, (int)sizeof(K), 0, false, false);
+  CHK(pool_get(ctx, "zero_stage", z.total, &p));
+  char *zb = static_cast<char *>(p);
+  launch_prep(ctx->st, reinterpret_cast<uint32_t *>(zb), z.total / 4, s);
+  CHK(local_splits_impl<K>(ctx, brt, m, dim, ls, s));
+  launch_scan_u32(ls, ps, m, reinterpret_cast<uint64_t *>(zb + z.lb_scan), ctx->st, s);
+  launch_set_sizes(ctx->st, ps, m, s);
+  CHK(launch_status("splits scan"));
+  CHK(copy_state_and_wait(ctx, s));
+  const int64_t O = ctx->st_host->octree_size;
+  *octree_size = O;
+  if (O > capacity) return fail(PGVD_ERR_CAPACITY, "octree buffer too small");
+  {
+    Stage t(ctx, "octree", s);
+    // mbits is not needed without conflict flags; keys are not read.
+    launch_octree<K>(brt, parent, ps, nullptr, nullptr, m, dim, 0, octree, nullptr, capacity, 1,
+                     ctx->st, s);
+  }
+  CHK(launch_status("octree"));
+  CHK(copy_state_and_wait(ctx, s));
+  return status_error(ctx->st_host->status);
+}
+
+}  // namespace
+
+// ===========================================================================
+// exported entry points
+// ===========================================================================
+extern "C" {
+
+const char *pgvd_last_error(void) { return g_err.c_str(); }
+const char *pgvd_version(void) { return "pgvd-mi355x 0.1 (gfx950)"; }
+
+int pgvd_next_pow2(int n) { return ref_next_pow2(n); }
+
+int pgvd_ctx_create(int device, pgvd_ctx **out) {
+  if (!out) return fail(PGVD_ERR_INVALID, "null out pointer");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return fail(PGVD_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= count) return fail(PGVD_ERR_NO_DEVICE, "device index out of range");
+  pgvd_ctx *ctx = new pgvd_ctx();
+  ctx->device = device;
+  int rc = PGVD_OK;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
+      hipMalloc(&ctx->st, sizeof(RunState)) != hipSuccess ||
+      hipHostMalloc(&ctx->st_host, sizeof(RunState), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming) != hipSuccess) {
+    rc = fail(PGVD_ERR_HIP, "context setup failed");
+  }
+  if (rc != PGVD_OK) {
+    pgvd_ctx_destroy(ctx);
+    return rc;
+  }
+  std::memset(ctx->st_host, 0, sizeof(RunState));
+  *out = ctx;
+  return PGVD_OK;
+}
+
+int pgvd_ctx_destroy(pgvd_ctx *ctx) {
+  if (!ctx) return PGVD_OK;
+  // Teardown is best effort: errors here cannot be acted on.
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->pool)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  for (auto &r : ctx->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : ctx->spare) (void)hipEventDestroy(e);
+  if (ctx->done) (void)hipEventDestroy(ctx->done);
+  if (ctx->st) (void)hipFree(ctx->st);
+  if (ctx->st_host) (void)hipHostFree(ctx->st_host);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PGVD_OK;
+}
+
+int pgvd_ctx_stream(pgvd_ctx *ctx, pgvd_stream *out) {
+  if (!ctx || !out) return fail(PGVD_ERR_INVALID, "null argument");
+  *out = ctx->stream;
+  return PGVD_OK;
+}
+
+int pgvd_synchronize(pgvd_ctx *ctx, pgvd_stream s) {
+  if (!ctx) return fail(PGVD_ERR_INVALID, "null context");
+  CHK(use_device(ctx));
+  HIPCHK(hipStreamSynchronize(pick(ctx, s)));
+  return PGVD_OK;
+}
+
+int pgvd_alloc(pgvd_ctx *ctx, size_t bytes, void **dptr) {
+  if (!ctx || !dptr) return fail(PGVD_ERR_INVALID, "null argument");
+  CHK(use_device(ctx));
+  if (hipMalloc(dptr, bytes ? bytes : 1) != hipSuccess) return fail(PGVD_ERR_ALLOC, "hipMalloc failed");
+  return PGVD_OK;
+}
+
+int pgvd_free(pgvd_ctx *ctx, void *dptr) {
+  if (!ctx) return fail(PGVD_ERR_INVALID, "null context");
+  CHK(use_device(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipFree(dptr));
+  return PGVD_OK;
+}
+
+int pgvd_memcpy_htod(pgvd_ctx *ctx, void *dst, const void *src, size_t bytes, pgvd_stream s) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(PGVD_ERR_INVALID, "null argument");
+  CHK(use_device(ctx));
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, pick(ctx, s)));
+  HIPCHK(hipStreamSynchronize(pick(ctx, s)));
+  return PGVD_OK;
+}
+
+int pgvd_memcpy_dtoh(pgvd_ctx *ctx, void *dst, const void *src, size_t bytes, pgvd_stream s) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(PGVD_ERR_INVALID, "null argument");
+  CHK(use_device(ctx));
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, pick(ctx, s)));
+  HIPCHK(hipStreamSynchronize(pick(ctx, s)));
+  return PGVD_OK;
+}
+
+int pgvd_copy_kernel(pgvd_ctx *ctx, void *dst, const void *src, size_t bytes, pgvd_stream s) {
+  if (!ctx || !dst || !src) return fail(PGVD_ERR_INVALID, "null argument");
+  if (((uintptr_t)dst | (uintptr_t)src) & 15) return fail(PGVD_ERR_INVALID, "copy needs 16-byte aligned pointers");
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  {
+    Stage t(ctx, "copy", st);
+    launch_copy(dst, src, bytes, st);
+  }
+  return launch_status("copy");
+}
+
+int pgvd_timing_enable(pgvd_ctx *ctx, int on) {
+  if (!ctx) return fail(PGVD_ERR_INVALID, "null context");
+  ctx->timing = on != 0;
+  return PGVD_OK;
+}
+
+int pgvd_timing_reset(pgvd_ctx *ctx) {
+  if (!ctx) return fail(PGVD_ERR_INVALID, "null context");
+  CHK(use_device(ctx));
+  HIPCHK(hipDeviceSynchronize());
+  for (auto &r : ctx->recs) {
+    ctx->spare.push_back(r.a);
+    ctx->spare.push_back(r.b);
+  }
+  ctx->recs.clear();
+  return PGVD_OK;
+}
+
+int pgvd_timing_read(pgvd_ctx *ctx, const char *stage, double *total_ms, int64_t *count) {
+  if (!ctx || !stage || !total_ms || !count) return fail(PGVD_ERR_INVALID, "null argument");
+  CHK(use_device(ctx));
+  HIPCHK(hipDeviceSynchronize());
+  double tot = 0;
+  int64_t c = 0;
+  for (auto &r : ctx->recs) {
+    if (r.stage != stage) continue;
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+    tot += ms;
+    ++c;
+  }
+  *total_ms = tot;
+  *count = c;
+  return PGVD_OK;
+}
+
+int pgvd_bbox(pgvd_ctx *ctx, const float *pts, int64_t n, int dim, float *bb_out, pgvd_stream s) {
+  if (!ctx || !pts || !bb_out) return fail(PGVD_ERR_INVALID, "null argument");
+  if (dim != 2 && dim != 3) return fail(PGVD_ERR_INVALID, "dim must be 2 or 3");
+  CHK(check_n(n));
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  launch_prep(ctx->st, nullptr, 0, st);
+  {
+    Stage t(ctx, "bbox", st);
+    launch_bbox(pts, n, dim, ctx->st, st);
+  }
+  CHK(launch_status("bbox"));
+  CHK(copy_state_and_wait(ctx, st));
+  for (int k = 0; k < dim; ++k) {
+    bb_out[k] = fdec_host(ctx->st_host->bb_min[k]);
+    bb_out[dim + k] = fdec_host(ctx->st_host->bb_max[k]);
+  }
+  return PGVD_OK;
+}
+
+int pgvd_quantize(pgvd_ctx *ctx, const float *pts, int64_t n, int dim, int bits,
+                  const float *custom_bb, int clamped, int64_t *qpts_out, int64_t *n_out,
+                  pgvd_stream s) {
+  if (!ctx || !pts || !qpts_out || !n_out) return fail(PGVD_ERR_INVALID, "null argument");
+  CHK(check_dims(dim, bits));
+  if (bits > 62) return fail(PGVD_ERR_INVALID, "bits must be <= 62 for int64 coordinates");
+  CHK(check_n(n));
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  launch_prep(ctx->st, nullptr, 0, st);
+  if (!custom_bb) launch_bbox(pts, n, dim, ctx->st, st);
+  const int64_t width = (int64_t)1 << bits;
+  const float effw = (float)(clamped ? width : width - 1);
+  {
+    Stage t(ctx, "encode", st);
+    launch_encode<uint64_t>(pts, n, dim, bits, effw, custom_bb ? 1 : 0, custom_bb,
+                            custom_bb ? custom_bb + dim : nullptr, nullptr, qpts_out, nullptr, 0,
+                            ctx->st, st);
+  }
+  CHK(launch_status("quantize"));
+  CHK(copy_state_and_wait(ctx, st));
+  CHK(status_error(ctx->st_host->status));
+  float mn[3], mx[3];
+  for (int k = 0; k < dim; ++k) {
+    mn[k] = custom_bb ? custom_bb[k] : fdec_host(ctx->st_host->bb_min[k]);
+    mx[k] = custom_bb ? custom_bb[dim + k] : fdec_host(ctx->st_host->bb_max[k]);
+  }
+  float dw = mx[0] - mn[0];
+  for (int k = 1; k < dim; ++k) {
+    const float sz = mx[k] - mn[k];
+    dw = (sz > dw) ? sz : dw;
+  }
+  *n_out = (dw == 0.0f) ? 1 : n;
+  return PGVD_OK;
+}
+
+int pgvd_points_to_morton(pgvd_ctx *ctx, const void *pts, int coord_bytes, int64_t n, int dim,
+                          int bits, void *keys_out, int key_bytes, pgvd_stream s) {
+  if (!ctx || !pts || !keys_out) return fail(PGVD_ERR_INVALID, "null argument");
+  if (coord_bytes != 4 && coord_bytes != 8) return fail(PGVD_ERR_INVALID, "coord_bytes must be 4 or 8");
+  CHK(check_dims(dim, bits));
+  CHK(check_n(n));
+  if (key_bytes < key_bytes_for(dim * bits)) return fail(PGVD_ERR_INVALID, "key_bytes too small for bits*dim");
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  const int64_t padded = ref_next_pow2((int)n);
+  Stage t(ctx, "encode", st);
+  if (key_bytes == 4) launch_points_to_morton<uint32_t>(pts, coord_bytes, n, padded, dim, bits, static_cast<uint32_t *>(keys_out), st);
+  else if (key_bytes == 8) launch_points_to_morton<uint64_t>(pts, coord_bytes, n, padded, dim, bits, static_cast<uint64_t *>(keys_out), st);
+  else if (key_bytes == 16) launch_points_to_morton<Key128>(pts, coord_bytes, n, padded, dim, bits, static_cast<Key128 *>(keys_out), st);
+  else return fail(PGVD_ERR_INVALID, "key_bytes must be 4, 8 or 16");
+  return launch_status("points_to_morton");
+}
+
+int pgvd_radix_sort(pgvd_ctx *ctx, void *keys, uint32_t *vals, int64_t n, int key_bytes, int mbits,
+                    pgvd_stream s) {
+  if (!ctx || !keys) return fail(PGVD_ERR_INVALID, "null argument");
+  if (n < 0 || n >= (int64_t(1) << 30) - 1) return fail(PGVD_ERR_TOO_LARGE, "n must be in [0, 2^30-1)");
+  if (mbits < 1 || mbits > key_bytes * 8) return fail(PGVD_ERR_INVALID, "mbits out of range for key_bytes");
+  if (n <= 1) return PGVD_OK;
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  if (key_bytes == 4) return radix_sort_impl<uint32_t>(ctx, static_cast<uint32_t *>(keys), vals, n, mbits, st);
+  if (key_bytes == 8) return radix_sort_impl<uint64_t>(ctx, static_cast<uint64_t *>(keys), vals, n, mbits, st);
+  if (key_bytes == 16) return radix_sort_impl<Key128>(ctx, static_cast<Key128 *>(keys), vals, n, mbits, st);
+  return fail(PGVD_ERR_INVALID, "key_bytes must be 4, 8 or 16");
+}
+
+int pgvd_unique_sorted(pgvd_ctx *ctx, const void *keys_in, void *keys_out, int64_t n,
+                       int key_bytes, int64_t *m_out, pgvd_stream s) {
+  if (!ctx || !keys_in || !keys_out || !m_out) return fail(PGVD_ERR_INVALID, "null argument");
+  if (n < 0 || n >= (int64_t(1) << 30) - 1) return fail(PGVD_ERR_TOO_LARGE, "n must be in [0, 2^30-1)");
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  if (key_bytes == 4) return unique_impl<uint32_t>(ctx, static_cast<const uint32_t *>(keys_in), static_cast<uint32_t *>(keys_out), n, m_out, st);
+  if (key_bytes == 8) return unique_impl<uint64_t>(ctx, static_cast<const uint64_t *>(keys_in), static_cast<uint64_t *>(keys_out), n, m_out, st);
+  if (key_bytes == 16) return unique_impl<Key128>(ctx, static_cast<const Key128 *>(keys_in), static_cast<Key128 *>(keys_out), n, m_out, st);
+  return fail(PGVD_ERR_INVALID, "key_bytes must be 4, 8 or 16");
+}
+
+int pgvd_stream_scan(pgvd_ctx *ctx, const uint32_t *in, uint32_t *out, int64_t n, pgvd_stream s) {
+  if (!ctx || !in || !out) return fail(PGVD_ERR_INVALID, "null argument");
+  if (n < 0 || n >= (int64_t(1) << 30) - 1) return fail(PGVD_ERR_TOO_LARGE, "n must be in [0, 2^30-1)");
+  if (n == 0) return PGVD_OK;
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  const ZeroLayout z = zero_layout(n, 4, 0, false, false);
+  void *p;
+  CHK(pool_get(ctx, "zero_stage", z.total, &p));
+  char *zb = static_cast<char *>(p);
+  launch_prep(ctx->st, reinterpret_cast<uint32_t *>(zb), z.total / 4, st);
+  {
+    Stage t(ctx, "scan", st);
+    launch_scan_u32(in, out, n, reinterpret_cast<uint64_t *>(zb + z.lb_scan), ctx->st, st);
+  }
+  return launch_status("scan");
+}
+
+int pgvd_build_brt(pgvd_ctx *ctx, const void *keys, int64_t m, int key_bytes, int mbits,
+                   void *brt_out, int32_t *parent_out, pgvd_stream s) {
+  if (!ctx || !keys || (m > 1 && (!brt_out || !parent_out))) return fail(PGVD_ERR_INVALID, "null argument");
+  if (m < 1) return fail(PGVD_ERR_EMPTY, "no keys");
+  if (m >= (int64_t(1) << 30) - 1) return fail(PGVD_ERR_TOO_LARGE, "m must be below 2^30-1");
+  if (mbits < 1 || mbits > key_bytes * 8) return fail(PGVD_ERR_INVALID, "mbits out of range for key_bytes");
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  if (key_bytes == 4) return brt_impl<uint32_t>(ctx, static_cast<const uint32_t *>(keys), m, mbits, static_cast<BrtNodeT<uint32_t> *>(brt_out), parent_out, st);
+  if (key_bytes == 8) return brt_impl<uint64_t>(ctx, static_cast<const uint64_t *>(keys), m, mbits, static_cast<BrtNodeT<uint64_t> *>(brt_out), parent_out, st);
+  if (key_bytes == 16) return brt_impl<Key128>(ctx, static_cast<const Key128 *>(keys), m, mbits, static_cast<BrtNodeT<Key128> *>(brt_out), parent_out, st);
+  return fail(PGVD_ERR_INVALID, "key_bytes must be 4, 8 or 16");
+}
+
+int pgvd_local_splits(pgvd_ctx *ctx, const void *brt, int64_t m, int key_bytes, int dim,
+                      uint32_t *ls_out, pgvd_stream s) {
+  if (!ctx || !ls_out || (m > 1 && !brt)) return fail(PGVD_ERR_INVALID, "null argument");
+  if (dim != 2 && dim != 3) return fail(PGVD_ERR_INVALID, "dim must be 2 or 3");
+  if (m < 1) return fail(PGVD_ERR_EMPTY, "no keys");
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  if (key_bytes == 4) return local_splits_impl<uint32_t>(ctx, static_cast<const BrtNodeT<uint32_t> *>(brt), m, dim, ls_out, st);
+  if (key_bytes == 8) return local_splits_impl<uint64_t>(ctx, static_cast<const BrtNodeT<uint64_t> *>(brt), m, dim, ls_out, st);
+  if (key_bytes == 16) return local_splits_impl<Key128>(ctx, static_cast<const BrtNodeT<Key128> *>(brt), m, dim, ls_out, st);
+  return fail(PGVD_ERR_INVALID, "key_bytes must be 4, 8 or 16");
+}
+
+int pgvd_brt_to_octree(pgvd_ctx *ctx, const void *brt, const int32_t *parent, int64_t m,
+                       int key_bytes, int dim, void *octree_out, int64_t capacity,
+                       int64_t *octree_size, pgvd_stream s) {
+  if (!ctx || !octree_out || !octree_size || (m > 1 && (!brt || !parent)))
+    return fail(PGVD_ERR_INVALID, "null argument");
+  if (dim != 2 && dim != 3) return fail(PGVD_ERR_INVALID, "dim must be 2 or 3");
+  if (m < 1) return fail(PGVD_ERR_EMPTY, "no keys");
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  int32_t *o = static_cast<int32_t *>(octree_out);
+  if (key_bytes == 4) return brt_to_octree_impl<uint32_t>(ctx, static_cast<const BrtNodeT<uint32_t> *>(brt), parent, m, dim, o, capacity, octree_size, st);
+  if (key_bytes == 8) return brt_to_octree_impl<uint64_t>(ctx, static_cast<const BrtNodeT<uint64_t> *>(brt), parent, m, dim, o, capacity, octree_size, st);
+  if (key_bytes == 16) return brt_to_octree_impl<Key128>(ctx, static_cast<const BrtNodeT<Key128> *>(brt), parent, m, dim, o, capacity, octree_size, st);
+  return fail(PGVD_ERR_INVALID, "key_bytes must be 4, 8 or 16");
+}
+
+int pgvd_build_octree(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
+                      const uint32_t *labels, const pgvd_build_opts *opts, pgvd_stream s) {
+  if (!ctx || !pts) return fail(PGVD_ERR_INVALID, "null argument");
+  CHK(check_n(n));
+  CHK(check_dims(dim, bits));
+  if (opts && opts->quantized_input && bits > 31) return fail(PGVD_ERR_INVALID, "int32 grid input allows bits <= 31");
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  ctx->last.valid = false;
+  const int kb = key_bytes_for(dim * bits);
+  if (kb == 4) return build_impl<uint32_t>(ctx, pts, n, dim, bits, labels, opts, st);
+  if (kb == 8) return build_impl<uint64_t>(ctx, pts, n, dim, bits, labels, opts, st);
+  return build_impl<Key128>(ctx, pts, n, dim, bits, labels, opts, st);
+}
+
+int pgvd_build_octree_finish(pgvd_ctx *ctx, pgvd_octree_result *out) {
+  if (!ctx || !out) return fail(PGVD_ERR_INVALID, "null argument");
+  if (!ctx->last.valid) return fail(PGVD_ERR_INVALID, "no build in flight on this context");
+  CHK(use_device(ctx));
+  HIPCHK(hipEventSynchronize(ctx->done));
+  LastBuild &L = ctx->last;
+  for (int attempt = 0; attempt < 2 && (ctx->st_host->status & ST_CAPACITY); ++attempt) {
+    const int64_t need = (int64_t)ctx->st_host->octree_size;
+    if (L.kb == 4) CHK(refill_octree<uint32_t>(ctx, need + need / 4 + 64));
+    else if (L.kb == 8) CHK(refill_octree<uint64_t>(ctx, need + need / 4 + 64));
+    else CHK(refill_octree<Key128>(ctx, need + need / 4 + 64));
+  }
+  const RunState &R = *ctx->st_host;
+  if (R.status & ST_CAPACITY) return fail(PGVD_ERR_INTERNAL, "octree refill did not fit");
+  CHK(status_error(R.status));
+  std::memset(out, 0, sizeof(*out));
+  out->n = L.n;
+  out->m = R.m;
+  out->octree_size = R.octree_size;
+  out->dim = L.dim;
+  out->bits = L.bits;
+  out->mbits = L.mbits;
+  out->key_bytes = L.kb;
+  out->malformed = R.malformed ? 1 : 0;
+  out->keys = L.keys_u;
+  out->brt = L.brt;
+  out->parent = L.parent;
+  out->prefix_sums = L.ps;
+  out->octree = static_cast<const int32_t *>(ctx->pool["octree"].p);
+  out->key_conflict = L.conflict_bits;
+  out->node_conflict = L.labels ? static_cast<const uint32_t *>(ctx->pool["node_conflict"].p) : nullptr;
+  return PGVD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+// kernels.hpp -- host-side launchers for the pgvd HIP kernels and the
+// device-resident run state they share.  Internal to libpgvd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace pgvd {
+
+// Error bits collected on the device during a run (RunState::status).
+enum : uint32_t {
+  ST_NEG_COORD = 1u << 0,    // a coordinate quantized below zero
+  ST_CAPACITY = 1u << 1,     // octree larger than the output capacity
+  ST_STACK = 1u << 2,        // gather stack overflow (malformed tree)
+  ST_TOO_LARGE = 1u << 3,    // a count exceeded the 30-bit look-back field
+  ST_DUP_KEY = 1u << 4,      // BRT input had equal neighbouring keys
+  ST_WALK = 1u << 5,         // parent walk did not terminate within bound
+};
+
+constexpr int kMaxPasses = 16;    // 128-bit keys / 8-bit digits
+constexpr int kNumCounters = 24;  // tile counters: sort passes, unique, scans
+
+// Per-run scalars that kernels read and write on the device.  Cleared by the
+// prep kernel at the start of every run.
+struct RunState {
+  uint32_t bb_min[3];        // fenc-encoded bbox minimum
+  uint32_t bb_max[3];        // fenc-encoded bbox maximum
+  uint32_t status;           // ST_* bits
+  uint32_t malformed;        // 1 if some BRT node took an FP32-quirk split
+  uint32_t m;                // unique key count (incl. the padding zero)
+  uint32_t octree_size;      // O = ps[m-1]
+  uint32_t counters[kNumCounters];
+};
+
+// Fixed-layout BRT node in device memory (the reference's BrtNode,
+// C/BrtNode.h:10-21, minus `parent`, which is a separate int32 array).
+template <typename K>
+struct BrtNodeT {
+  typename StoredKey<K>::type lcp;
+  int32_t left;
+  uint16_t lcp_length;
+  uint8_t left_leaf;
+  uint8_t right_leaf;
+};
+
+// Launch configuration constants.
+constexpr int kBlock = 256;
+
+// ---- morton.hip ----
+void launch_prep(RunState *st, uint32_t *zero_words, size_t nzero, hipStream_t s);
+void launch_bbox(const float *pts, int64_t n, int dim, RunState *st, hipStream_t s);
+// Quantize + Morton encode (+ optional digit histograms for `passes` passes).
+// bb: if custom, (min, max) given; else taken from st (launch_bbox).
+template <typename K>
+void launch_encode(const float *pts, int64_t n, int dim, int bits, float effw,
+                   int custom_bb, const float *bbmin, const float *bbmax,
+                   K *keys, int64_t *qout, uint32_t *hist, int passes, RunState *st,
+                   hipStream_t s);
+// Morton encode from integer points (int32 or int64 coordinates).
+template <typename K>
+void launch_points_to_morton(const void *pts, int coord_bytes, int64_t n, int64_t padded_n,
+                             int dim, int bits, K *keys, hipStream_t s);
+template <typename K>
+void launch_histogram(const K *keys, int64_t n, uint32_t *hist, int passes, hipStream_t s);
+// Morton encode + digit histograms from int32 grid points (no padding).
+template <typename K>
+void launch_encode_int(const int32_t *pts, int64_t n, int dim, int bits, K *keys,
+                       uint32_t *hist, int passes, hipStream_t s);
+
+// ---- util.hip ----
+void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
+void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
+void launch_set_sizes(RunState *st, const uint32_t *ps, int64_t m, hipStream_t s);
+
+// ---- sort.hip ----
+// Onesweep LSD radix sort of n keys (+ optional u32 payload) over `passes`
+// 8-bit digits.  Returns 0 if the result is in (keys, vals), 1 if in
+// (keys_alt, vals_alt).  hist holds the digit histograms (passes x 256),
+// lookback `passes` x tiles x 256 words, cleared beforehand.
+// Pass 0 reads payload from vals_in (may equal vals); later passes ping-pong
+// vals/vals_alt in step with the keys.
+// pass_events (nullable): 2*passes events recorded around each pass.
+template <typename K>
+int launch_onesweep(K *keys, K *keys_alt, const uint32_t *vals_in, uint32_t *vals,
+                    uint32_t *vals_alt, int64_t n, int passes, const uint32_t *hist,
+                    uint32_t *lookback, RunState *st, hipStream_t s,
+                    hipEvent_t *pass_events = nullptr);
+int64_t onesweep_tiles(int key_bytes, bool payload, int64_t n);
+int onesweep_tile_size(int key_bytes, bool payload);
+
+// ---- scan.hip ----
+constexpr int kScanTile = 4096;
+inline int64_t scan_tiles(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
+// Unique over [zero?] + keys[0..n): writes out[0..m) and st->m.  With labels,
+// sets bit r of conflict_bits for every run r whose labels are not all equal.
+template <typename K>
+void launch_unique(const K *keys, const uint32_t *labels, int64_t n, int prepend_zero, K *out,
+                   uint32_t *conflict_bits, uint64_t *lookback, RunState *st, hipStream_t s);
+// Local splits (from the BRT) scanned inclusively into ps[0..m); rewrites
+// parent[] from claim form (claimer+1) to the reference form; writes ls[]
+// when non-null; st->octree_size = ps[m-1].
+template <typename K>
+void launch_splits_scan(const BrtNodeT<K> *brt, int32_t *parent, uint32_t *ls, uint32_t *ps,
+                        int64_t max_m, int dim, uint64_t *lookback, RunState *st,
+                        hipStream_t s);
+// Plain inclusive scan of n u32 values (StreamScan_p).
+void launch_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint64_t *lookback,
+                     RunState *st, hipStream_t s);
+
+// ---- brt.hip ----
+// Karras BRT over keys[0..m), m = st->m (or m_host >= 0).  parent[] must be
+// zero on entry; receives claimer+1 per claimed node (max over claimers).
+template <typename K>
+void launch_brt(const K *keys, int64_t max_m, int64_t m_host, int mbits, BrtNodeT<K> *brt,
+                int32_t *parent, RunState *st, hipStream_t s);
+// Rewrite parent[] from claim form to the reference form (BRT stage API).
+void launch_finalize_parents(int32_t *parent, int64_t max_m, int64_t m_host, RunState *st,
+                             hipStream_t s);
+
+// ---- octree.hip ----
+// Octree fill.  parent[] in the reference form.  force_scatter selects the
+// order-exact scatter path unconditionally (stage API, where the BRT comes
+// from the caller); otherwise st->malformed selects it on the device.
+template <typename K>
+void launch_octree(const BrtNodeT<K> *brt, const int32_t *parent, const uint32_t *ps,
+                   const K *keys, const uint32_t *conflict_bits, int64_t max_m, int dim,
+                   int mbits, int32_t *octree, uint32_t *node_conflict, int64_t capacity,
+                   int force_scatter, RunState *st, hipStream_t s);
+// Local splits the reference's way (each node writes its internal children's
+// entries, highest claimer last): ComputeLocalSplits_p for a caller's BRT.
+template <typename K>
+void launch_local_splits(const BrtNodeT<K> *brt, int64_t m, int dim, uint32_t *ls,
+                         uint64_t *tmp, hipStream_t s);
+
+}  // namespace pgvd

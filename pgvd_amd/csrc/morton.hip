@@ -1,0 +1,274 @@
+// morton.hip -- run-state prep, bounding box, FP32 quantize + Morton encode
+// with fused radix digit histograms.
+//
+// Reference behaviour followed:
+//   bbox        BoundingBox.h:166-175 / :80-87, Karras.cpp:75-83
+//   quantize    Karras.cpp:42-67 (FP32 expression :53-55), :84-88 (dwidth 0)
+//   Morton      C/z_order.h:14-47 (PointsToMortonKernel, kernels.cl:2-22)
+//   padding     opencl/Kernels.cpp:62-75 (zero keys past n)
+#include "kernels.hpp"
+
+namespace pgvd {
+
+namespace {
+
+constexpr int kMaxGrid = 2048;
+
+inline int grid_for(int64_t n, int per_block) {
+  int64_t g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > kMaxGrid) g = kMaxGrid;
+  return (int)g;
+}
+
+__global__ __launch_bounds__(kBlock) void prep_kernel(RunState *st, uint4 *zero, size_t nzero16) {
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid == 0) {
+    for (int k = 0; k < 3; ++k) { st->bb_min[k] = 0xFFFFFFFFu; st->bb_max[k] = 0u; }
+    st->status = 0; st->malformed = 0; st->m = 0; st->octree_size = 0;
+    for (int k = 0; k < kNumCounters; ++k) st->counters[k] = 0;
+  }
+  const uint4 z = make_uint4(0, 0, 0, 0);
+  for (size_t i = tid; i < nzero16; i += (size_t)gridDim.x * blockDim.x) zero[i] = z;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void bbox_kernel(const float *__restrict__ pts, int64_t n,
+                                                       RunState *st) {
+  float mn[DIM], mx[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) { mn[k] = __int_as_float(0x7f800000); mx[k] = -mn[k]; }
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      const float v = pts[i * DIM + k];
+      mn[k] = fminf(mn[k], v);
+      mx[k] = fmaxf(mx[k], v);
+    }
+  }
+  __shared__ float smn[4][DIM], smx[4][DIM];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    for (int d = 32; d >= 1; d >>= 1) {
+      mn[k] = fminf(mn[k], __shfl_xor(mn[k], d, 64));
+      mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], d, 64));
+    }
+    if (lane == 0) { smn[w][k] = mn[k]; smx[w][k] = mx[k]; }
+  }
+  __syncthreads();
+  if (threadIdx.x < DIM) {
+    const int k = threadIdx.x;
+    float a = smn[0][k], b = smx[0][k];
+    for (int j = 1; j < 4; ++j) { a = fminf(a, smn[j][k]); b = fmaxf(b, smx[j][k]); }
+    atomicMin(&st->bb_min[k], fenc(a));
+    atomicMax(&st->bb_max[k], fenc(b));
+  }
+}
+
+// One point -> quantized coordinates (Karras.cpp:50-65).  Each FP32 step is
+// an explicitly rounded op so no contraction or reassociation can change it.
+template <int DIM>
+__device__ inline bool quantize_point(const float *p, const float *mn, float dwidth, float effw,
+                                      int64_t *c) {
+  bool neg = false;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    if (dwidth == 0.0f) { c[k] = 0; continue; }
+    const float t = __fdiv_rn(__fsub_rn(p[k], mn[k]), dwidth);
+    const float f = __fmul_rn(effw, t);
+    const double d = __dadd_rn((double)f, 0.5);
+    const int64_t v = __double2ll_rz(d);
+    neg |= (v < 0);
+    c[k] = v;
+  }
+  return neg;
+}
+
+template <typename K, int DIM>
+__global__ __launch_bounds__(kBlock) void encode_kernel(
+    const float *__restrict__ pts, int64_t n, int bits, float effw, int custom, float bmin0,
+    float bmin1, float bmin2, float bmax0, float bmax1, float bmax2, K *__restrict__ keys,
+    int64_t *__restrict__ qout, uint32_t *__restrict__ hist, int passes, RunState *st) {
+  __shared__ uint32_t sh[kMaxPasses * 256];
+  for (int i = threadIdx.x; i < passes * 256; i += kBlock) sh[i] = 0;
+  float mn[3], mx[3];
+  if (custom) {
+    mn[0] = bmin0; mn[1] = bmin1; mn[2] = bmin2;
+    mx[0] = bmax0; mx[1] = bmax1; mx[2] = bmax2;
+  } else {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) { mn[k] = fdec(st->bb_min[k]); mx[k] = fdec(st->bb_max[k]); }
+  }
+  // BoundingBox::max_size (BoundingBox.h:80-87)
+  float dwidth = __fsub_rn(mx[0], mn[0]);
+#pragma unroll
+  for (int k = 1; k < DIM; ++k) {
+    const float s = __fsub_rn(mx[k], mn[k]);
+    dwidth = (s > dwidth) ? s : dwidth;
+  }
+  __syncthreads();
+  bool neg = false;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    float p[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) p[k] = pts[i * DIM + k];
+    int64_t c[DIM];
+    neg |= quantize_point<DIM>(p, mn, dwidth, effw, c);
+    if (qout) {
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) qout[i * DIM + k] = c[k];
+    }
+    uint64_t lo, hi;
+    morton128<DIM>(c, bits, lo, hi);
+    const K key = from128<K>(lo, hi);
+    if (keys) keys[i] = key;
+    for (int r = 0; r < passes; ++r) atomicAdd(&sh[r * 256 + kdigit(key, 8 * r)], 1u);
+  }
+  if (neg) atomicOr(&st->status, (uint32_t)ST_NEG_COORD);
+  if (hist) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < passes * 256; i += kBlock)
+      if (sh[i]) atomicAdd(&hist[i], sh[i]);
+  }
+}
+
+template <typename K, int DIM, typename C>
+__global__ __launch_bounds__(kBlock) void points_to_morton_kernel(const C *__restrict__ pts,
+                                                                  int64_t n, int64_t padded_n,
+                                                                  int bits, K *__restrict__ keys) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < padded_n;
+       i += (int64_t)gridDim.x * kBlock) {
+    K key = kzero_value<K>();
+    if (i < n) {
+      int64_t c[DIM];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) c[k] = (int64_t)pts[i * DIM + k];
+      uint64_t lo, hi;
+      morton128<DIM>(c, bits, lo, hi);
+      key = from128<K>(lo, hi);
+    }
+    keys[i] = key;
+  }
+}
+
+template <typename K, int DIM>
+__global__ __launch_bounds__(kBlock) void encode_int_kernel(const int32_t *__restrict__ pts,
+                                                            int64_t n, int bits,
+                                                            K *__restrict__ keys,
+                                                            uint32_t *__restrict__ hist,
+                                                            int passes) {
+  __shared__ uint32_t sh[kMaxPasses * 256];
+  for (int i = threadIdx.x; i < passes * 256; i += kBlock) sh[i] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    int64_t c[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) c[k] = (int64_t)pts[i * DIM + k];
+    uint64_t lo, hi;
+    morton128<DIM>(c, bits, lo, hi);
+    const K key = from128<K>(lo, hi);
+    keys[i] = key;
+    for (int r = 0; r < passes; ++r) atomicAdd(&sh[r * 256 + kdigit(key, 8 * r)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < passes * 256; i += kBlock)
+    if (sh[i]) atomicAdd(&hist[i], sh[i]);
+}
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void histogram_kernel(const K *__restrict__ keys, int64_t n,
+                                                           uint32_t *__restrict__ hist,
+                                                           int passes) {
+  __shared__ uint32_t sh[kMaxPasses * 256];
+  for (int i = threadIdx.x; i < passes * 256; i += kBlock) sh[i] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const K key = keys[i];
+    for (int r = 0; r < passes; ++r) atomicAdd(&sh[r * 256 + kdigit(key, 8 * r)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < passes * 256; i += kBlock)
+    if (sh[i]) atomicAdd(&hist[i], sh[i]);
+}
+
+}  // namespace
+
+void launch_prep(RunState *st, uint32_t *zero_words, size_t nzero, hipStream_t s) {
+  // zero_words is 16-byte aligned and nzero a multiple of 4 (the workspace
+  // layout guarantees both).
+  const size_t n16 = nzero / 4;
+  const int grid = grid_for((int64_t)(n16 ? n16 : 1), kBlock * 4);
+  prep_kernel<<<grid, kBlock, 0, s>>>(st, reinterpret_cast<uint4 *>(zero_words), n16);
+}
+
+void launch_bbox(const float *pts, int64_t n, int dim, RunState *st, hipStream_t s) {
+  const int grid = grid_for(n, kBlock * 8);
+  if (dim == 2) bbox_kernel<2><<<grid, kBlock, 0, s>>>(pts, n, st);
+  else bbox_kernel<3><<<grid, kBlock, 0, s>>>(pts, n, st);
+}
+
+template <typename K>
+void launch_encode(const float *pts, int64_t n, int dim, int bits, float effw, int custom_bb,
+                   const float *bbmin, const float *bbmax, K *keys, int64_t *qout,
+                   uint32_t *hist, int passes, RunState *st, hipStream_t s) {
+  const int grid = grid_for(n, kBlock * 4);
+  float a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+  if (custom_bb)
+    for (int k = 0; k < dim; ++k) { a[k] = bbmin[k]; b[k] = bbmax[k]; }
+  if (dim == 2)
+    encode_kernel<K, 2><<<grid, kBlock, 0, s>>>(pts, n, bits, effw, custom_bb, a[0], a[1], a[2],
+                                                b[0], b[1], b[2], keys, qout, hist, passes, st);
+  else
+    encode_kernel<K, 3><<<grid, kBlock, 0, s>>>(pts, n, bits, effw, custom_bb, a[0], a[1], a[2],
+                                                b[0], b[1], b[2], keys, qout, hist, passes, st);
+}
+
+template <typename K>
+void launch_points_to_morton(const void *pts, int coord_bytes, int64_t n, int64_t padded_n,
+                             int dim, int bits, K *keys, hipStream_t s) {
+  const int grid = grid_for(padded_n, kBlock * 4);
+  if (coord_bytes == 4) {
+    const int32_t *p = static_cast<const int32_t *>(pts);
+    if (dim == 2) points_to_morton_kernel<K, 2, int32_t><<<grid, kBlock, 0, s>>>(p, n, padded_n, bits, keys);
+    else points_to_morton_kernel<K, 3, int32_t><<<grid, kBlock, 0, s>>>(p, n, padded_n, bits, keys);
+  } else {
+    const int64_t *p = static_cast<const int64_t *>(pts);
+    if (dim == 2) points_to_morton_kernel<K, 2, int64_t><<<grid, kBlock, 0, s>>>(p, n, padded_n, bits, keys);
+    else points_to_morton_kernel<K, 3, int64_t><<<grid, kBlock, 0, s>>>(p, n, padded_n, bits, keys);
+  }
+}
+
+template <typename K>
+void launch_histogram(const K *keys, int64_t n, uint32_t *hist, int passes, hipStream_t s) {
+  const int grid = grid_for(n, kBlock * 8);
+  histogram_kernel<K><<<grid, kBlock, 0, s>>>(keys, n, hist, passes);
+}
+
+template <typename K>
+void launch_encode_int(const int32_t *pts, int64_t n, int dim, int bits, K *keys, uint32_t *hist,
+                       int passes, hipStream_t s) {
+  const int grid = grid_for(n, kBlock * 4);
+  if (dim == 2) encode_int_kernel<K, 2><<<grid, kBlock, 0, s>>>(pts, n, bits, keys, hist, passes);
+  else encode_int_kernel<K, 3><<<grid, kBlock, 0, s>>>(pts, n, bits, keys, hist, passes);
+}
+
+#define PGVD_INST(K)                                                                          \
+  template void launch_encode<K>(const float *, int64_t, int, int, float, int, const float *,  \
+                                 const float *, K *, int64_t *, uint32_t *, int, RunState *,   \
+                                 hipStream_t);                                                 \
+  template void launch_points_to_morton<K>(const void *, int, int64_t, int64_t, int, int, K *, \
+                                           hipStream_t);                                       \
+  template void launch_histogram<K>(const K *, int64_t, uint32_t *, int, hipStream_t);         \
+  template void launch_encode_int<K>(const int32_t *, int64_t, int, int, K *, uint32_t *, int,  \
+                                     hipStream_t);
+PGVD_INST(uint32_t)
+PGVD_INST(uint64_t)
+PGVD_INST(Key128)
+#undef PGVD_INST
+
+}  // namespace pgvd

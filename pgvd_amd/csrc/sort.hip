@@ -1,0 +1,211 @@
+// sort.hip -- onesweep LSD radix sort (8-bit digits) for 32/64/128-bit keys
+// with an optional u32 payload.
+//
+// Replaces the reference's bit-serial split sort (RadixSortBigUnsigned,
+// opencl/Kernels.cpp:219-247: BitPredicate + StreamScan + BUCompact per
+// bit).  Both are stable LSD sorts, so the sorted key sequence -- the only
+// thing the path consumes -- is identical.
+//
+// One kernel launch per digit.  Each workgroup claims a tile id from an
+// atomic counter (so every tile it waits on was claimed by a workgroup that
+// is already running), ranks its keys with wave64 ballot matching and
+// per-wave LDS digit counters, publishes its per-digit counts, and resolves
+// its global digit offsets with a per-digit decoupled look-back.  Keys are
+// then staged in LDS in digit order and written out in coalesced runs.
+#include "kernels.hpp"
+
+namespace pgvd {
+
+namespace {
+
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagIncl = 2u << 30;
+constexpr uint32_t kValMask = (1u << 30) - 1;
+constexpr int kWaves = kBlock / 64;
+
+template <typename K, bool PAY> struct SortCfg { static constexpr int IPT = 16; };
+template <> struct SortCfg<Key128, false> { static constexpr int IPT = 8; };
+template <> struct SortCfg<Key128, true> { static constexpr int IPT = 8; };
+
+// Exclusive scan of one u32 per thread over the 256-thread block.
+__device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t *s_wsum) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t incl = wave_incl_scan_add(x);
+  if (lane == 63) s_wsum[w] = incl;
+  __syncthreads();
+  uint32_t off = 0;
+  for (int j = 0; j < w; ++j) off += s_wsum[j];
+  return off + incl - x;
+}
+
+template <typename K, bool PAY, int IPT>
+__global__ __launch_bounds__(kBlock) void onesweep_kernel(
+    const K *__restrict__ kin, K *__restrict__ kout, const uint32_t *__restrict__ vin,
+    uint32_t *__restrict__ vout, uint32_t n, int shift, const uint32_t *__restrict__ ghist,
+    uint32_t *__restrict__ lookback, uint32_t *tile_counter) {
+  constexpr int TILE = kBlock * IPT;
+  __shared__ uint32_t s_cnt[kWaves * 256];
+  __shared__ uint32_t s_dstart[256];
+  __shared__ uint32_t s_gbase[256];
+  __shared__ uint32_t s_wsum_a[kWaves], s_wsum_b[kWaves];
+  __shared__ uint32_t s_tile;
+  __shared__ K s_keys[TILE];
+  __shared__ uint32_t s_vals[PAY ? TILE : 1];
+
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t == 0) s_tile = atomicAdd(tile_counter, 1u);
+  for (int i = t; i < kWaves * 256; i += kBlock) s_cnt[i] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t base = tile * TILE;
+  const uint32_t wbase = base + (uint32_t)w * 64u * IPT;
+
+  K k[IPT];
+  uint32_t v[IPT];
+  uint32_t rk[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t e = wbase + j * 64 + lane;
+    k[j] = (e < n) ? kin[e] : kzero_value<K>();
+    if (PAY) v[j] = (e < n) ? vin[e] : 0u;
+  }
+
+  // Rank: keys with equal digits within a wave found by 8 ballots; the
+  // lowest lane of each group bumps the wave's LDS counter for that digit.
+  // Items are processed in index order, so ranks are stable.
+  const uint64_t ltm = lanemask_lt();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t e = wbase + j * 64 + lane;
+    const bool valid = e < n;
+    const uint32_t d = kdigit(k[j], shift);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    if (valid) {
+      const uint32_t cnt = s_cnt[w * 256 + d];
+      const uint64_t below = peers & ltm;
+      rk[j] = cnt + (uint32_t)__popcll(below);
+      if (below == 0) s_cnt[w * 256 + d] = cnt + (uint32_t)__popcll(peers);
+    }
+  }
+  __syncthreads();
+
+  // Thread t owns digit t: per-wave exclusive offsets and the tile total.
+  uint32_t total = 0;
+#pragma unroll
+  for (int ww = 0; ww < kWaves; ++ww) {
+    const uint32_t c = s_cnt[ww * 256 + t];
+    s_cnt[ww * 256 + t] = total;
+    total += c;
+  }
+  uint32_t *lb = lookback + (size_t)tile * 256;
+  st_rel_u32(lb + t, (tile == 0 ? kFlagIncl : kFlagAgg) | total);
+
+  const uint32_t dstart = block_excl_scan(total, s_wsum_a);
+  s_dstart[t] = dstart;
+  const uint32_t gexcl = block_excl_scan(ghist[t], s_wsum_b);
+
+  uint32_t excl = 0;
+  if (tile > 0) {
+    int kk = (int)tile - 1;
+    for (;;) {
+      const uint32_t wv = ld_acq_u32(lookback + (size_t)kk * 256 + t);
+      const uint32_t f = wv & ~kValMask;
+      if (f == 0) { __builtin_amdgcn_s_sleep(1); continue; }
+      excl += wv & kValMask;
+      if (f == kFlagIncl) break;
+      --kk;
+    }
+    st_rel_u32(lb + t, kFlagIncl | (excl + total));
+  }
+  s_gbase[t] = gexcl + excl - dstart;
+  __syncthreads();
+
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t e = wbase + j * 64 + lane;
+    if (e < n) {
+      const uint32_t d = kdigit(k[j], shift);
+      const uint32_t pos = s_dstart[d] + s_cnt[w * 256 + d] + rk[j];
+      s_keys[pos] = k[j];
+      if (PAY) s_vals[pos] = v[j];
+    }
+  }
+  __syncthreads();
+
+  const uint32_t nvalid = (n - base) < (uint32_t)TILE ? (n - base) : (uint32_t)TILE;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t p = j * kBlock + t;
+    if (p < nvalid) {
+      const K key = s_keys[p];
+      const uint32_t g = s_gbase[kdigit(key, shift)] + p;
+      kout[g] = key;
+      if (PAY) vout[g] = s_vals[p];
+    }
+  }
+}
+
+}  // namespace
+
+int onesweep_tile_size(int key_bytes, bool payload) {
+  const int ipt = key_bytes == 16 ? SortCfg<Key128, false>::IPT : 16;
+  (void)payload;
+  return kBlock * ipt;
+}
+
+int64_t onesweep_tiles(int key_bytes, bool payload, int64_t n) {
+  const int ts = onesweep_tile_size(key_bytes, payload);
+  return (n + ts - 1) / ts;
+}
+
+template <typename K>
+int launch_onesweep(K *keys, K *keys_alt, const uint32_t *vals_in, uint32_t *vals,
+                    uint32_t *vals_alt, int64_t n, int passes, const uint32_t *hist,
+                    uint32_t *lookback, RunState *st, hipStream_t s, hipEvent_t *pass_events) {
+  if (n <= 1 || passes <= 0) return 0;
+  const bool pay = vals_in != nullptr;
+  const int64_t tiles = onesweep_tiles((int)sizeof(K), pay, n);
+  K *src = keys, *dst = keys_alt;
+  const uint32_t *vs = vals_in;
+  uint32_t *vd = vals_alt, *vother = vals;
+  for (int r = 0; r < passes; ++r) {
+    uint32_t *lb = lookback + (size_t)r * (size_t)tiles * 256;
+    uint32_t *ctr = &st->counters[r];
+    if (pass_events) (void)hipEventRecord(pass_events[2 * r], s);
+    if (pay) {
+      constexpr int IPT = SortCfg<K, true>::IPT;
+      onesweep_kernel<K, true, IPT><<<(unsigned)tiles, kBlock, 0, s>>>(
+          src, dst, vs, vd, (uint32_t)n, 8 * r, hist + 256 * r, lb, ctr);
+    } else {
+      constexpr int IPT = SortCfg<K, false>::IPT;
+      onesweep_kernel<K, false, IPT><<<(unsigned)tiles, kBlock, 0, s>>>(
+          src, dst, nullptr, nullptr, (uint32_t)n, 8 * r, hist + 256 * r, lb, ctr);
+    }
+    if (pass_events) (void)hipEventRecord(pass_events[2 * r + 1], s);
+    K *tk = src; src = dst; dst = tk;
+    // payload: the buffer just written becomes the next source; the next
+    // destination is the other work buffer.
+    uint32_t *written = vd;
+    vd = vother;
+    vother = written;
+    vs = written;
+  }
+  return passes & 1;
+}
+
+#define PGVD_INST(K)                                                                          \
+  template int launch_onesweep<K>(K *, K *, const uint32_t *, uint32_t *, uint32_t *, int64_t,  \
+                                  int, const uint32_t *, uint32_t *, RunState *, hipStream_t,   \
+                                  hipEvent_t *);
+PGVD_INST(uint32_t)
+PGVD_INST(uint64_t)
+PGVD_INST(Key128)
+#undef PGVD_INST
+
+}  // namespace pgvd
