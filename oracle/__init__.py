@@ -30,7 +30,7 @@ u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 
 
-def build(ref: bool = True) -> None:
+def compile_libs(ref: bool = True) -> None:
     """Compile liboracle.so (and _ref when the reference is present)."""
     target = "all" if ref else "oracle"
     subprocess.run(["make", "-s", "-C", HERE, target], check=True)
@@ -41,7 +41,7 @@ def lib():
     if _LIB is None:
         path = os.path.join(HERE, "liboracle.so")
         if not os.path.exists(path):
-            build(ref=False)
+            compile_libs(ref=False)
         L = C.CDLL(path)
         L.oracle_next_pow2.argtypes = [C.c_int]
         L.oracle_next_pow2.restype = C.c_int

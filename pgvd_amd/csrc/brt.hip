@@ -8,7 +8,7 @@
 //     the doubling goes, never l);
 //   * split by binary search whose step is the reference's FP32 expression
 //     t = (int)((l + (float)den - 1) / den) (:151) -- for ranges longer than
-//     ~5.6M keys that expression is not an exact ceil and selects a
+//     2^23 keys that expression can exceed the exact ceil and select a
 //     different split, which is reproduced;
 //   * lcp = key >> (mbits - lcp_length) (compute_lcp, :34-52);
 //   * parent links: the reference's serial loop lets the last (highest)
@@ -31,19 +31,18 @@ __global__ __launch_bounds__(kBlock) void brt_kernel(const K *__restrict__ keys,
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= m - 1) return;
   const K cur = keys[i];
+  const K nxt = keys[i + 1];
+  // The reference requires distinct keys (unique runs first); equal
+  // neighbours are reported, and every search below stays range-bounded.
+  if (keq(cur, nxt)) atomicOr(&st->status, (uint32_t)ST_DUP_KEY);
   int d, l;
   if (i == 0) {
     d = 1;
     l = m - 1;
   } else {
-    const int l_pos = delta(cur, keys[i + 1], mbits);
+    const int l_pos = delta(cur, nxt, mbits);
     const int l_neg = delta(cur, keys[i - 1], mbits);
-    if (l_pos == l_neg) {               // only possible with equal keys
-      atomicOr(&st->status, (uint32_t)ST_DUP_KEY);
-      d = 1;
-    } else {
-      d = l_pos > l_neg ? 1 : -1;
-    }
+    d = l_pos >= l_neg ? 1 : -1;
     const int lcp_min = (d == -1) ? l_pos : l_neg;
     int l_max = 2;
     for (;;) {
@@ -68,10 +67,12 @@ __global__ __launch_bounds__(kBlock) void brt_kernel(const K *__restrict__ keys,
     const int t = (int)__fdiv_rn(num, (float)den);
     if (s + t <= s_cutoff && delta(cur, keys[i + (s + t) * d], mbits) > lcp_node) s += t;
   }
-  if (3ll * l > (1ll << 24)) {
-    // The FP32 step can differ from an exact ceil only here; if it changed
-    // the split, the tree is not a well-formed radix tree and the octree
-    // stage must take the order-exact scatter path.
+  if (l > (1 << 23)) {
+    // For l <= 2^23 every intermediate of the FP32 step is an integer
+    // <= 2^24 (den < 2l is a power of two), so it equals the exact ceil.
+    // Above that it can differ; if it changed the split, the tree is not a
+    // well-formed radix tree and the octree stage must take the
+    // order-exact scatter path.
     int se = 0;
     for (int den = 2; den < 2 * l; den *= 2) {
       const int t = (int)(((long long)l + den - 1) / den);

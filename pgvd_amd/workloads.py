@@ -1,0 +1,82 @@
+"""Seeded synthetic point sets for the benchmark and parity tests.
+
+    uniform(n, dim, seed)        n points uniform in [-1,1]^dim
+    gaussian_mixture(n, ...)     32 seeded Gaussian clusters, tuned for ~30%
+                                 duplicate 63-bit keys at 21 bits/axis
+    star_loops(n, ...)           64 closed star-shaped loops in random planes,
+                                 sampled by arc length, with loop-id labels
+    batch_sizes(count, ...)      log-uniform instance sizes for batched builds
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def uniform(n: int, dim: int = 3, seed: int = 2) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return rng.random((n, dim), dtype=np.float32) * 2 - 1
+
+
+# Cluster widths.  At 21 bits/axis a grid cell is ~9.5e-7 wide, so clusters
+# of sigma 0.01-0.1 give almost no duplicate keys (m/n = 0.91 even at
+# 1e-5..1e-4).  Widths of 5e-6 .. 5e-5 (the 0.01-0.1 range scaled by 5e-4)
+# give m/n = 0.723 at 2^24 points, i.e. ~28% duplicate keys.
+GAUSS_SIGMA = (5e-6, 5e-5)
+
+
+def gaussian_mixture(n: int, dim: int = 3, seed: int = 3, k: int = 32,
+                     sigma=GAUSS_SIGMA) -> tuple[np.ndarray, str]:
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-0.9, 0.9, (k, dim))
+    sig = rng.uniform(sigma[0], sigma[1], k)
+    comp = rng.integers(0, k, n)
+    pts = centres[comp] + rng.standard_normal((n, dim)) * sig[comp, None]
+    pts = np.clip(pts, -1, 1).astype(np.float32)
+    # the bounding box spans the clusters; pin it to [-1,1]^dim with two
+    # corner points so the grid is the same as the uniform workload's
+    pts[0] = -1.0
+    pts[1] = 1.0
+    desc = (f"3D octree build, {n} float3 from a {k}-component Gaussian mixture "
+            f"(centres U[-0.9,0.9]^3 seed {seed}, sigma U[{sigma[0]:g},{sigma[1]:g}], "
+            f"corners pinned to [-1,1]^3)")
+    return pts, desc
+
+
+def star_loops(n: int, loops: int = 64, verts: int = 256, seed: int = 4):
+    """Closed loops r(t) = r0 (1 + 0.3 sin(k t + phi)) in random planes,
+    r0 ~ U[0.05, 0.3]; n points sampled uniformly by arc length over all
+    loops (equal share per loop); labels = loop id (uint32)."""
+    rng = np.random.default_rng(seed)
+    per = np.full(loops, n // loops)
+    per[: n - per.sum()] += 1
+    pts = np.empty((n, 3), np.float64)
+    labels = np.empty(n, np.uint32)
+    off = 0
+    for i in range(loops):
+        r0 = rng.uniform(0.05, 0.3)
+        kk = int(rng.integers(3, 9))
+        phi = rng.uniform(0, 2 * np.pi)
+        c = rng.uniform(-0.6, 0.6, 3)
+        a = rng.standard_normal(3)
+        a /= np.linalg.norm(a)
+        b = rng.standard_normal(3)
+        b -= a * (a @ b)
+        b /= np.linalg.norm(b)
+        t = np.arange(verts) * (2 * np.pi / verts)
+        r = r0 * (1 + 0.3 * np.sin(kk * t + phi))
+        V = c + (r * np.cos(t))[:, None] * a + (r * np.sin(t))[:, None] * b
+        W = np.roll(V, -1, axis=0)
+        seg = np.linalg.norm(W - V, axis=1)
+        cum = np.concatenate([[0.0], np.cumsum(seg)])
+        s = (np.arange(per[i]) + rng.random()) * (cum[-1] / per[i])
+        j = np.clip(np.searchsorted(cum, s, side="right") - 1, 0, verts - 1)
+        f = (s - cum[j]) / seg[j]
+        pts[off: off + per[i]] = V[j] + f[:, None] * (W[j] - V[j])
+        labels[off: off + per[i]] = i
+        off += per[i]
+    return pts.astype(np.float32), labels
+
+
+def batch_sizes(count: int = 4096, lo: float = 1e3, hi: float = 1e6, seed: int = 5) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(np.int64)

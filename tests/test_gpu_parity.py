@@ -1,0 +1,285 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and with the
+reference's golden fixtures.  Integer, key, index and tree outputs must be
+bit-exact; there is no floating-point output on this path."""
+from __future__ import annotations
+
+import hashlib
+
+import numpy as np
+import pytest
+
+import oracle
+import pgvd_amd as P
+from conftest import TREE_FIELDS, assert_tree_equal
+from pgvd_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+DTYPES = {"keys_lo": np.uint64, "keys_hi": np.uint64, "left": np.int32, "left_leaf": np.uint8,
+          "right_leaf": np.uint8, "lcp_lo": np.uint64, "lcp_hi": np.uint64,
+          "lcp_length": np.int32, "parent": np.int32, "prefix_sums": np.uint32,
+          "children": np.int32, "leaf": np.int32}
+
+
+def _digest(a, dtype):
+    return hashlib.sha256(np.ascontiguousarray(np.asarray(a).astype(dtype)).tobytes()).hexdigest()
+
+
+def _pts(n, dim, seed):
+    rng = np.random.default_rng(seed)
+    return rng.random((n, dim), dtype=np.float32) * 2 - 1
+
+
+# ------------------------------------------------------------ whole path ----
+@pytest.mark.parametrize("n,dim,bits", [
+    (1, 2, 16), (2, 2, 16), (3, 3, 21), (5, 2, 16), (7, 3, 21), (8, 2, 16), (9, 3, 21),
+    (1000, 2, 16), (4096, 3, 21), (4097, 3, 21), (65536, 2, 16), (100003, 3, 21),
+    (3000, 3, 42), (2000, 2, 64), (5000, 2, 30), (3000, 3, 10), (300, 2, 1), (50000, 3, 7),
+])
+def test_build_octree_matches_oracle(dev, n, dim, bits):
+    pts = _pts(n, dim, seed=n * 7 + dim * 3 + bits)
+    got = dev.build_octree(pts, bits)
+    want = oracle.build(oracle.quantize(pts, 1 << bits), bits)
+    assert_tree_equal(got, want, where=f"n={n} dim={dim} bits={bits}")
+
+
+def test_duplicates_and_degenerate_inputs(dev):
+    pts = _pts(20000, 3, 1)
+    pts[10000:] = pts[:10000]                                  # every point twice
+    want = oracle.build(oracle.quantize(pts, 1 << 21), 21)
+    assert_tree_equal(dev.build_octree(pts, 21), want, where="dupes")
+    same = np.full((777, 3), 0.3, np.float32)                  # zero-size box (Karras.cpp:84-88)
+    got = dev.build_octree(same, 21)
+    assert got["m"] == 1 and got["octree_size"] == 1
+    assert_tree_equal(got, oracle.build(oracle.quantize(same, 1 << 21), 21), where="same")
+    line = np.zeros((5000, 2), np.float32)                    # points on one axis
+    line[:, 0] = np.linspace(-1, 1, 5000, dtype=np.float32)
+    assert_tree_equal(dev.build_octree(line, 16),
+                      oracle.build(oracle.quantize(line, 1 << 16), 16), where="line")
+
+
+def test_custom_bbox_and_clamped(dev):
+    pts = _pts(5000, 2, 2) * np.float32(0.5)
+    bb = (np.array([-1, -1], np.float32), np.array([1, 1], np.float32))
+    for clamped in (False, True):
+        got = dev.build_octree(pts, 16, custom_bb=bb, clamped=clamped)
+        want = oracle.build(oracle.quantize(pts, 1 << 16, bb=bb, clamped=clamped), 16)
+        assert_tree_equal(got, want, where=f"custom clamped={clamped}")
+
+
+def test_bad_coordinate_is_an_error(dev):
+    """Karras.cpp:56-63 throws for a coordinate below the box; the C ABI
+    returns PGVD_ERR_BAD_COORD."""
+    pts = _pts(1000, 2, 3)
+    bb = (np.array([0, 0], np.float32), np.array([1, 1], np.float32))
+    with pytest.raises(P.PgvdError) as e:
+        dev.build_octree(pts, 16, custom_bb=bb)
+    assert e.value.code == 3
+    # the context stays usable
+    assert dev.build_octree(pts, 16)["m"] > 0
+
+
+def test_empty_input_is_an_error(dev):
+    with pytest.raises(P.PgvdError) as e:
+        dev.build_octree(np.zeros((0, 3), np.float32), 21)
+    assert e.value.code == 2                      # opencl/Kernels.cpp:389-391
+
+
+def test_int32_grid_input_matches_oracle(dev, golden_small):
+    """Kernels::BuildOctree_p takes integer points; the reference fixture at
+    bits = 30 through that entry."""
+    z = golden_small
+    q = z["int30/q"].astype(np.int32)
+    got = dev.build_octree(q, 30)
+    want = {k: z[f"int30/{k}"] for k in DTYPES if f"int30/{k}" in z}
+    want["m"] = want["keys_lo"].size
+    want["octree_size"] = want["children"].shape[0]
+    assert_tree_equal(got, want, where="int30 fixture")
+
+
+@pytest.mark.parametrize("name", ["n1000", "n5", "n1", "n2", "n4096", "n3000_dupes"])
+def test_reference_fixtures_2d(dev, golden_small, name):
+    z = golden_small
+    got = dev.build_octree(z[f"{name}/pts"], 16)
+    want = {k: z[f"{name}/{k}"] for k in DTYPES}
+    want["m"] = want["keys_lo"].size
+    want["octree_size"] = want["children"].shape[0]
+    assert_tree_equal(got, want, where=name)
+
+
+def test_reference_c1_digest(dev, golden_digests):
+    d = golden_digests["2d_c1_65536"]
+    got = dev.build_octree(_pts(d["n"], 2, d["seed"]), 16)
+    assert got["m"] == d["m"] and got["octree_size"] == d["octree_size"]
+    for k, dt in DTYPES.items():
+        assert _digest(got[k], dt) == d[k], k
+
+
+@pytest.mark.parametrize("name", ["c2_uniform", "c2_gauss"])
+def test_full_size_3d_digests(dev, golden_digests, name):
+    """2^24 points, 63-bit keys: every output bit-exact with the reference
+    (children, BRT, links) and the oracle (3D leaf)."""
+    d = golden_digests[name]
+    if name == "c2_uniform":
+        pts = workloads.uniform(1 << 24, 3, 2)
+    else:
+        pts, _ = workloads.gaussian_mixture(1 << 24)
+    assert _digest(pts, np.float32) == d["pts"]
+    got = dev.build_octree(pts, 21)
+    assert got["m"] == d["m"] and got["octree_size"] == d["octree_size"]
+    for k, dt in DTYPES.items():
+        assert _digest(got[k], dt) == d[k], k
+    # size-independent properties
+    k = got["keys_lo"]
+    assert (k[1:] > k[:-1]).all()
+    assert got["parent"][0] == -1
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fp32_split_quirk_stage_path(dev, golden_digests, dim):
+    """2^24 keys whose root split takes the reference's FP32 value: BRT via
+    the stage entry, then BinaryRadixToOctree through the order-exact path."""
+    from tests.golden.gen_golden import quirk_keys
+    d = golden_digests[f"quirk_{dim}d"]
+    keys, s_star = quirk_keys()
+    brt, parent = dev.build_brt(keys, None, 42, key_bytes=8)
+    assert int(brt["left"][0]) == d["root_left"] != s_star
+    assert _digest(brt["left"], np.int32) == d["left"]
+    assert _digest(parent, np.int32) == d["parent"]
+    assert _digest(brt["lcp_length"], np.int32) == d["lcp_length"]
+    assert _digest(brt["lcp"], np.uint64) == d["lcp_lo"]
+    ls = dev.local_splits(brt, keys.size, dim, 8)
+    assert _digest(ls, np.uint32) == d["local_splits"]
+    ch, lf = dev.brt_to_octree(brt, parent, keys.size, dim, 8)
+    assert ch.shape[0] == d["octree_size"]
+    assert _digest(ch, np.int32) == d["children"]
+    assert _digest(lf, np.int32) == d["leaf"]
+
+
+def test_labels_conflict_cells(dev):
+    """Point-label conflict pass: flags per unique key and per (node, octant)
+    slot against the oracle (itself pinned by brute force)."""
+    pts, labels = workloads.star_loops(1 << 16, loops=16, seed=9)
+    pts[:2000] = pts[5000:7000]                     # overlapping samples of two loops
+    for bits in (21, 42):
+        got = dev.build_octree(pts, bits, labels=labels)
+        want = oracle.build(oracle.quantize(pts, 1 << bits), bits, labels=labels)
+        assert_tree_equal(got, want, where=f"labels bits={bits}")
+        assert np.array_equal(got["key_conflict"], want["key_conflict"])
+        assert np.array_equal(got["node_conflict"], want["node_conflict"])
+        assert got["key_conflict"].sum() > 0
+
+
+# ------------------------------------------------------------ stage APIs ----
+def test_points_to_morton_reference_input(dev):
+    """PointsToMorton on (i, i), bits = 30 (tests/KernelTests.cpp:57-84),
+    including the zero-key padding to nextPow2."""
+    import json
+    import os
+    from conftest import GOLDEN
+    with open(os.path.join(GOLDEN, "morton_ref.json")) as f:
+        d = json.load(f)
+    q = np.stack([np.arange(1_000_000)] * 2, axis=1).astype(np.int32)
+    lo, hi = dev.points_to_morton(q, 30)
+    assert lo.size == oracle.next_pow2(1_000_000) == 1 << 20
+    assert _digest(lo[:1_000_000], np.uint64) == d["keys_lo_sha256"]
+    assert not lo[1_000_000:].any() and not hi.any()
+
+
+@pytest.mark.parametrize("dim,bits", [(2, 16), (3, 21), (3, 42), (2, 64)])
+def test_points_to_morton_matches_oracle(dev, dim, bits):
+    rng = np.random.default_rng(bits)
+    q = rng.integers(0, 2**62, (3001, dim), dtype=np.int64)
+    lo, hi = dev.points_to_morton(q, bits)
+    olo, ohi = oracle.morton(q, bits)
+    assert np.array_equal(lo[:3001], olo)
+    if dim * bits > 64:
+        assert np.array_equal(hi[:3001], ohi)
+    assert not lo[3001:].any()
+
+
+@pytest.mark.parametrize("kb,mbits,n", [(4, 32, 100003), (8, 63, 300001), (8, 40, 5),
+                                        (16, 126, 50001), (16, 90, 4096)])
+def test_radix_sort(dev, kb, mbits, n):
+    rng = np.random.default_rng(n)
+    lo = rng.integers(0, 2**63, n, dtype=np.uint64)
+    hi = rng.integers(0, 2**62, n, dtype=np.uint64)
+    if mbits < 64:
+        lo &= np.uint64((1 << mbits) - 1)
+    if kb < 16:
+        hi[:] = 0
+    else:
+        hi &= np.uint64((1 << (mbits - 64)) - 1)
+    lo[: n // 3] = lo[n // 3: 2 * (n // 3)]           # duplicates: stability is observable
+    hi[: n // 3] = hi[n // 3: 2 * (n // 3)]
+    vals = np.arange(n, dtype=np.uint32)
+    (slo, shi), svals = dev.radix_sort(lo, hi, mbits=mbits, vals=vals, key_bytes=kb)
+    order = np.lexsort((vals, lo, hi))              # stable: ties keep input order
+    assert np.array_equal(slo, lo[order])
+    if kb == 16:
+        assert np.array_equal(shi, hi[order])
+    assert np.array_equal(svals, vals[order])
+
+
+@pytest.mark.parametrize("kb", [4, 8, 16])
+def test_unique_sorted(dev, kb):
+    rng = np.random.default_rng(kb)
+    lo = np.sort(rng.integers(0, 5000, 200000, dtype=np.uint64))
+    hi = np.zeros_like(lo) if kb < 16 else (lo // np.uint64(7))
+    ulo, uhi = dev.unique_sorted(lo, hi, key_bytes=kb)
+    want = np.unique(lo)
+    assert np.array_equal(ulo, want)
+
+
+@pytest.mark.parametrize("n", [1, 2, 4095, 4096, 4097, 1 << 20, 3000017])
+def test_stream_scan(dev, n):
+    rng = np.random.default_rng(n)
+    x = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)   # wraps mod 2^32
+    assert np.array_equal(dev.stream_scan(x), oracle.stream_scan(x))
+
+
+@pytest.mark.parametrize("mbits,dim", [(32, 2), (63, 3), (126, 3), (40, 2)])
+def test_brt_and_octree_stages(dev, mbits, dim):
+    rng = np.random.default_rng(mbits)
+    m = 40000
+    if mbits <= 64:
+        lo = np.unique(rng.integers(0, 1 << min(mbits, 63), m, dtype=np.uint64))
+        hi = None
+    else:
+        lo = rng.integers(0, 2**63, m, dtype=np.uint64)
+        hi = rng.integers(0, 1 << (mbits - 64), m, dtype=np.uint64)
+        order = np.lexsort((lo, hi))
+        lo, hi = lo[order], hi[order]
+        keep = np.ones(m, bool)
+        keep[1:] = (lo[1:] != lo[:-1]) | (hi[1:] != hi[:-1])
+        lo, hi = lo[keep], hi[keep]
+    kb = P.key_bytes_for(mbits)
+    want = oracle.build_from_keys(lo, hi, dim, mbits)
+    brt, parent = dev.build_brt(lo, hi, mbits, key_bytes=kb)
+    assert np.array_equal(brt["left"], want["left"])
+    assert np.array_equal(brt["left_leaf"], want["left_leaf"])
+    assert np.array_equal(brt["right_leaf"], want["right_leaf"])
+    assert np.array_equal(brt["lcp_length"].astype(np.int32), want["lcp_length"])
+    assert np.array_equal(parent, want["parent"])
+    if kb == 16:
+        assert np.array_equal(brt["lcp_lo"], want["lcp_lo"]) and np.array_equal(brt["lcp_hi"], want["lcp_hi"])
+    else:
+        assert np.array_equal(brt["lcp"].astype(np.uint64), want["lcp_lo"])
+    ls = dev.local_splits(brt, lo.size, dim, kb)
+    assert np.array_equal(ls, want["local_splits"])
+    ch, lf = dev.brt_to_octree(brt, parent, lo.size, dim, kb)
+    assert np.array_equal(ch, want["children"]) and np.array_equal(lf, want["leaf"])
+
+
+def test_brt_rejects_duplicate_keys(dev):
+    with pytest.raises(P.PgvdError) as e:
+        dev.build_brt(np.array([1, 5, 5, 9], np.uint64), None, 32, key_bytes=8)
+    assert e.value.code == 9
+
+
+def test_octree_capacity_is_reported(dev):
+    keys = np.unique(np.random.default_rng(1).integers(0, 1 << 40, 5000, dtype=np.uint64))
+    brt, parent = dev.build_brt(keys, None, 40, key_bytes=8)
+    with pytest.raises(P.PgvdError) as e:
+        dev.brt_to_octree(brt, parent, keys.size, 2, 8, capacity=10)
+    assert e.value.code == 7

@@ -1,0 +1,292 @@
+"""Pins for the oracle that do not come from the reference's fixtures:
+closed forms, brute force on tiny inputs, and invariants a plausible mistake
+(dropped term, wrong sign or index, transposed operand) would break."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+
+
+def bitlen(x: int) -> int:
+    return x.bit_length()
+
+
+# ---------------------------------------------------------------- delta ----
+@pytest.mark.parametrize("mbits", [8, 32, 42, 63, 64, 87, 126, 128])
+def test_lcp_length_is_mbits_minus_bitlen_of_xor(mbits):
+    """compute_lcp_length's shift search (C/BuildBRT.c:54-88) equals
+    mbits - bitlen(a ^ b): 2000 random pairs, pairs sharing long prefixes,
+    and equal keys."""
+    rng = np.random.default_rng(mbits)
+    M = (1 << mbits) - 1
+    for _ in range(2000):
+        a = int(rng.integers(0, 2**62)) << 66 | int(rng.integers(0, 2**62)) << 4 | int(rng.integers(0, 16))
+        a &= M
+        share = int(rng.integers(0, mbits + 1))
+        b = (a ^ (int(rng.integers(1, 2**62)) >> share)) & M if share < mbits else a
+        assert oracle.lcp_length(a, b, mbits) == mbits - bitlen(a ^ b)
+    assert oracle.lcp_length(M, M, mbits) == mbits
+    assert oracle.lcp_length(0, M, mbits) == 0
+
+
+@pytest.mark.skipif(not oracle.ref_available(2), reason="reference checker not built here")
+def test_reference_lcp_length_matches_closed_form():
+    """The reference's own compute_lcp_length, on BigUnsigned keys <= 88 bits."""
+    rng = np.random.default_rng(3)
+    for mbits in (32, 60, 63, 87):
+        M = (1 << mbits) - 1
+        for _ in range(3000):
+            a = (int(rng.integers(0, 2**62)) << 30 | int(rng.integers(0, 2**30))) & M
+            b = (a ^ (1 << int(rng.integers(0, mbits)))) if rng.random() < .5 else \
+                (int(rng.integers(0, 2**62)) << 30 | int(rng.integers(0, 2**30))) & M
+            assert oracle.ref_lcp_length(a, b, mbits) == mbits - bitlen(a ^ b)
+
+
+# --------------------------------------------------------------- morton ----
+def _interleave_brute(c, dim, bits):
+    z = 0
+    for i in range(bits):
+        for k in range(dim):
+            if (c[k] >> i) & 1:
+                z |= 1 << (i * dim + k)
+    return z
+
+
+def _deinterleave(z, dim, bits):
+    c = [0] * dim
+    for i in range(bits):
+        for k in range(dim):
+            c[k] |= ((z >> (i * dim + k)) & 1) << i
+    return c
+
+
+@pytest.mark.parametrize("dim,bits", [(2, 16), (2, 30), (2, 64), (3, 21), (3, 42), (3, 10)])
+def test_morton_is_the_bit_interleave(dim, bits):
+    rng = np.random.default_rng(dim * 100 + bits)
+    n = 500
+    q = rng.integers(0, 2**62, (n, dim), dtype=np.int64)
+    lo, hi = oracle.morton(q, bits)
+    for i in range(n):
+        z = int(lo[i]) | int(hi[i]) << 64
+        c = [int(x) & ((1 << bits) - 1) for x in q[i]]     # bits >= `bits` ignored
+        assert z == _interleave_brute(c, dim, bits)
+        assert _deinterleave(z, dim, bits) == c
+    # axis order: x is the lowest bit of each digit
+    lo, _ = oracle.morton(np.array([[1] + [0] * (dim - 1)], np.int64), bits)
+    assert int(lo[0]) == 1
+    lo, _ = oracle.morton(np.array([[0] * (dim - 1) + [1]], np.int64), bits)
+    assert int(lo[0]) == 1 << (dim - 1)
+
+
+@pytest.mark.skipif(not oracle.ref_available(2), reason="reference checker not built here")
+def test_reference_xyz2z_matches_oracle():
+    rng = np.random.default_rng(4)
+    q = rng.integers(0, 2**30, (300, 2))
+    lo, hi = oracle.morton(q.astype(np.int64), 30)
+    for i in range(300):
+        assert oracle.ref_xyz2z_2d(int(q[i, 0]), int(q[i, 1]), 30) == int(lo[i])
+
+
+# ------------------------------------------------------------ next_pow2 ----
+def test_next_pow2():
+    for n in list(range(1, 70)) + [1000, 65535, 65536, 65537, (1 << 24) - 1, 1 << 24, (1 << 28) + 1]:
+        want = max(8, 1 << math.ceil(math.log2(n))) if n > 1 else 8
+        assert oracle.next_pow2(n) == want, n
+
+
+# ----------------------------------------------------------------- scan ----
+def test_hillis_steele_scan_equals_prefix_sum():
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 3, 7, 8, 9, 1000, 4097, 65536):
+        x = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+        assert np.array_equal(oracle.stream_scan(x), np.cumsum(x, dtype=np.uint64).astype(np.uint32))
+
+
+# ------------------------------------------------------------- quantize ----
+def test_quantize_endpoints_and_rounding():
+    # bbox [-1, 1] in x, W' = 2^16 - 1
+    pts = np.array([[-1, -1], [1, 1], [0, 0], [-1, 1]], np.float32)
+    q = oracle.quantize(pts, 1 << 16)
+    assert q.tolist() == [[0, 0], [65535, 65535], [32768, 32768], [0, 65535]]
+    # clamped uses W instead of W - 1 (Karras.cpp:46-49)
+    q = oracle.quantize(pts, 1 << 16, clamped=True)
+    assert q[1].tolist() == [65536, 65536]
+    # half-way values round up (+0.5 then truncate), not to even:
+    # t = 2.5/65535 * ... choose p so that W' * t is exactly k + 0.5
+    W1 = 65535
+    for k in (2, 3, 1000):
+        p = np.float32(-1 + 2 * (k + 0.5) / W1)
+        t = (np.float32(p) - np.float32(-1)) / np.float32(2)
+        f = np.float32(W1) * np.float32(t)
+        if float(f) == k + 0.5:
+            assert oracle.quantize(np.array([[-1, -1], [1, 1], [p, p]], np.float32), 1 << 16)[2, 0] == k + 1
+
+
+def test_quantize_is_float32_not_float64():
+    """A point whose float32 product lands on .5 while the exact product
+    is below it: the FP32 expression of Karras.cpp:53-55 rounds up, a
+    float64 evaluation would not."""
+    W1 = (1 << 21) - 1
+    rng = np.random.default_rng(6)
+    found = 0
+    for p in rng.random(200000).astype(np.float32) * 2 - 1:
+        t32 = (np.float32(p) - np.float32(-1)) / np.float32(2)
+        f32 = np.float32(W1) * t32
+        exact = W1 * (float(p) + 1) / 2
+        v32 = int(float(f32) + 0.5)
+        v64 = int(exact + 0.5)
+        if v32 != v64:
+            got = oracle.quantize(np.array([[-1, -1, -1], [1, 1, 1], [p, p, p]], np.float32), 1 << 21)
+            assert got[2, 0] == v32 != v64
+            found += 1
+            if found >= 5:
+                break
+    assert found >= 1
+
+
+def test_quantize_degenerate_and_errors():
+    pts = np.full((10, 3), 0.25, np.float32)
+    q = oracle.quantize(pts, 1 << 21)
+    assert q.shape == (1, 3) and not q.any()           # Karras.cpp:84-88
+    with pytest.raises(ValueError):                     # Karras.cpp:56-63
+        oracle.quantize(np.array([[-5, 0], [0.5, 0.5]], np.float32), 1 << 16,
+                        bb=(np.array([0, 0], np.float32), np.array([1, 1], np.float32)))
+    # slightly below min still rounds to 0 (truncation toward zero of d+0.5)
+    q = oracle.quantize(np.array([[-1e-6, 0.5]], np.float32), 1 << 16,
+                        bb=(np.array([0, 0], np.float32), np.array([1, 1], np.float32)))
+    assert q[0, 0] == 0
+
+
+# ------------------------------------------------------ tree invariants ----
+def _tree(n, dim, bits, seed, dupes=False):
+    rng = np.random.default_rng(seed)
+    pts = rng.random((n, dim), dtype=np.float32) * 2 - 1
+    if dupes:
+        pts[n // 2:] = pts[: n - n // 2]
+    q = oracle.quantize(pts, 1 << bits)
+    return q, oracle.build(q, bits)
+
+
+def _delta(a, b, mbits):
+    return mbits - (a ^ b).bit_length()
+
+
+@pytest.mark.parametrize("n,dim,bits,seed", [(3000, 2, 16, 1), (2000, 3, 21, 2), (500, 3, 42, 3),
+                                             (1000, 2, 16, 4)])
+def test_brt_structure(n, dim, bits, seed):
+    """Karras 2012: every internal node but the root is claimed exactly once,
+    leaf children cover every key exactly once, a node's lcp is the common
+    prefix of its range, and its split is at the first bit where the range
+    differs (keys below the tree's FP32 zone, l <= 2^23)."""
+    _, t = _tree(n, dim, bits, seed)
+    m, mbits = t["m"], dim * bits
+    keys = [int(a) | int(b) << 64 for a, b in zip(t["keys_lo"], t["keys_hi"])]
+    claims = np.zeros(m, int)
+    leaves = np.zeros(m, int)
+    for i in range(m - 1):
+        L = int(t["left"][i])
+        for side, leaf in ((0, t["left_leaf"][i]), (1, t["right_leaf"][i])):
+            (leaves if leaf else claims)[L + side] += 1
+    assert claims[0] == 0 and (claims[1:m - 1] == 1).all()
+    assert (leaves == 1).all()
+    assert t["parent"][0] == -1
+    for i in range(m - 1):
+        L = int(t["left"][i])
+        # range of i: scan children ranges via lcp: keys of the range share lcp_length bits
+        ln = int(t["lcp_length"][i])
+        assert _delta(keys[L], keys[L + 1], mbits) == ln       # split point differs right below lcp
+        lcp = int(t["lcp_lo"][i]) | int(t["lcp_hi"][i]) << 64
+        assert lcp == (keys[i] >> (mbits - ln) if ln else 0)
+
+
+@pytest.mark.parametrize("n,dim,bits,seed,dupes", [(3000, 2, 16, 1, False), (2000, 3, 21, 2, True),
+                                                   (700, 3, 42, 3, False), (9, 2, 16, 4, False)])
+def test_octree_descent_reaches_every_key(n, dim, bits, seed, dupes):
+    """Top-down walk of the octree by key digits, starting from node 0 at the
+    root's level, ends in an empty slot for every unique key, no two keys
+    share a slot, and `leaf` marks exactly the empty slots."""
+    _, t = _tree(n, dim, bits, seed, dupes)
+    mbits, nc = dim * bits, 1 << dim
+    ch, lf = t["children"], t["leaf"]
+    for o in range(t["octree_size"]):
+        assert lf[o] == sum(1 << c for c in range(nc) if ch[o, c] == -1)
+    if t["m"] < 2:
+        return
+    L0 = int(t["lcp_length"][0]) // dim
+    seen = set()
+    for lo, hi in zip(t["keys_lo"], t["keys_hi"]):
+        z = int(lo) | int(hi) << 64
+        node, lvl = 0, L0
+        while True:
+            o = (z >> (mbits - (lvl + 1) * dim)) & (nc - 1)
+            c = int(ch[node, o])
+            if c == -1:
+                break
+            node, lvl = c, lvl + 1
+        assert (node, o) not in seen
+        seen.add((node, o))
+
+
+def test_conflict_flags_brute_force():
+    """A unique key is in conflict when its points carry more than one label;
+    the (node, octant) mask matches the descent slot of each such key."""
+    rng = np.random.default_rng(8)
+    n, dim, bits = 4000, 3, 12
+    q = rng.integers(0, 40, (n, dim)).astype(np.int64)       # many duplicate keys
+    labels = rng.integers(0, 3, n).astype(np.uint32)
+    labels[q[:, 0] % 2 == 0] = 7                             # some keys single-labelled
+    t = oracle.build(q, bits, labels=labels)
+    lo, _ = oracle.morton(q, bits)
+    groups = {}
+    for z, lab in zip(lo.tolist(), labels.tolist()):
+        groups.setdefault(z, set()).add(lab)
+    keys = t["keys_lo"].tolist()
+    for r, z in enumerate(keys):
+        want = len(groups.get(z, set())) > 1
+        assert bool(t["key_conflict"][r]) == want
+    mbits, nc = dim * bits, 1 << dim
+    ch = t["children"]
+    L0 = int(t["lcp_length"][0]) // dim
+    mask = np.zeros(t["octree_size"], np.uint32)
+    for r, z in enumerate(keys):
+        if not t["key_conflict"][r]:
+            continue
+        node, lvl = 0, L0
+        while True:
+            o = (z >> (mbits - (lvl + 1) * dim)) & (nc - 1)
+            if ch[node, o] == -1:
+                break
+            node, lvl = int(ch[node, o]), lvl + 1
+        mask[node] |= 1 << o
+    assert np.array_equal(mask, t["node_conflict"])
+    assert t["key_conflict"].sum() > 0
+
+
+def test_padding_key_zero_present_iff_not_power_of_two():
+    """Zero-key padding (opencl/Kernels.cpp:62-75): key 0 is among the unique
+    keys when n is not a power of two or n < 8, even if no point maps to 0."""
+    for n in (5, 8, 9, 16, 17):
+        q = np.full((n, 2), 7, np.int64)
+        q[:, 0] = np.arange(1, n + 1)
+        t = oracle.build(q, 16)
+        has0 = int(t["keys_lo"][0]) == 0
+        assert has0 == (oracle.next_pow2(n) != n), n
+
+
+def test_quirk_fixture_distinguishes_fp32_from_exact_ceil(golden_digests):
+    """Planted mistake: an exact-ceil split search on the fixture's root finds
+    the exact split, which the reference (and the fixture) do not."""
+    from tests.golden.gen_golden import quirk_keys
+    keys, s_star = quirk_keys()
+    l = keys.size - 1
+    s, den = 0, 2
+    while den < 2 * l:
+        t = (l + den - 1) // den
+        if s + t <= s_star:
+            s += t
+        den *= 2
+    assert s == s_star != golden_digests["quirk_2d"]["root_left"]
