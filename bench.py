@@ -118,6 +118,9 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--cpu-sample", type=int, default=1 << 23)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="only run the plain copy kernel over MiB mebibytes (5 launches), "
+                         "the reference point for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); this build has no "
@@ -125,6 +128,21 @@ def main() -> int:
         return 2
 
     import pgvd_amd as P
+
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        dev = P.Device(0)
+        a, b = dev.alloc(nbytes), dev.alloc(nbytes)
+        dev.timing(True)
+        for _ in range(5):
+            dev.copy_kernel(b, a, nbytes)
+        ms, cnt = dev.timing_read("copy")
+        print(json.dumps({"calibration": "copy_kernel", "bytes_read": nbytes,
+                          "bytes_written": nbytes, "launches": cnt,
+                          "avg_ms": ms / max(cnt, 1),
+                          "GBps": 2 * nbytes / (ms / max(cnt, 1) * 1e-3) / 1e9}))
+        dev.close()
+        return 0
 
     dim, bits = 3, 21
     n = args.n
@@ -136,7 +154,11 @@ def main() -> int:
         dev.build_octree_async(dpts, n, dim, bits)
         res = dev.build_octree_finish()
 
-    dev.timing(True)
+    # Timed region.  HIP events cost ~10 us of queue time each, so the only
+    # events inside it are one pair per step around the radix-sort passes
+    # (the dominant kernel, launched back to back).
+    passes = (dim * bits + 7) // 8
+    dev.timing(True, sort_span_only=True)
     dev.timing_reset()
     dev.synchronize()
     t0 = time.perf_counter()
@@ -146,14 +168,23 @@ def main() -> int:
     dev.synchronize()
     elapsed = time.perf_counter() - t0
     ms_step = elapsed * 1e3 / args.steps
+    sort_ms, sort_cnt = dev.timing_read("sort")
+    pass_ms = sort_ms / max(sort_cnt, 1) / passes
 
-    passes = (dim * bits + 7) // 8
+    # Per-stage breakdown from separate steps with an event pair per stage
+    # (these steps are slower by the event overhead and are not the metric).
+    dev.timing(True)
+    dev.timing_reset()
+    prof_steps = 3
+    for _ in range(prof_steps):
+        dev.build_octree_async(dpts, n, dim, bits)
+    dev.build_octree_finish()
     stage_ms = {}
-    for st in ["prep", "bbox", "encode", "sort_pass", "unique", "brt", "splits_scan", "octree"]:
+    for st in ["prep", "bbox", "encode", "sort_pass", "unique", "brt", "splits_scan",
+               "malformed", "octree"]:
         tot, cnt = dev.timing_read(st)
         stage_ms[st] = (tot, cnt)
     dev.timing(False)
-    pass_ms = stage_ms["sort_pass"][0] / max(stage_ms["sort_pass"][1], 1)
 
     m, O = int(res.m), int(res.octree_size)
     ab = algorithmic_bytes(n, m, O, dim, int(res.key_bytes), passes)
@@ -189,13 +220,14 @@ def main() -> int:
             "frac": pass_gbps / HBM_PEAK_GBPS,
             "traffic": traffic,
             "avg_launch_ms": pass_ms,
-            "launches": stage_ms["sort_pass"][1],
+            "launches": sort_cnt * passes,
         },
         "pipeline": {
             "algorithmic_bytes": ab["total"],
             "achieved_GBps": total_gbps,
             "frac_of_hbm_peak": total_gbps / HBM_PEAK_GBPS,
-            "stage_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in stage_ms.items()},
+            "stage_ms_per_step": {k: v[0] / prof_steps for k, v in stage_ms.items()},
+            "stage_note": "separate steps with an event pair around every stage",
         },
     }
     dev.close()

@@ -64,10 +64,13 @@ int pgvd_memcpy_dtoh(pgvd_ctx *ctx, void *dst, const void *src, size_t bytes, pg
  * kernel for HBM byte counters). */
 int pgvd_copy_kernel(pgvd_ctx *ctx, void *dst, const void *src, size_t bytes, pgvd_stream s);
 
-/* Per-stage kernel timing with HIP events on the launch stream.
- * pgvd_timing_read returns the summed milliseconds and launch count of one
- * stage ("bbox", "encode", "sort_pass", "unique", "brt", "splits_scan",
- * "octree") since the last reset; it synchronizes. */
+/* Kernel timing with HIP events on the launch stream.  on = 1 records an
+ * event pair around every stage ("prep", "bbox", "encode", each
+ * "sort_pass", "unique", "brt", "splits_scan", "malformed", "octree");
+ * on = 2 records one pair around all radix-sort passes of a build ("sort")
+ * and nothing else (each event adds ~10 us of queue time).
+ * pgvd_timing_read returns the summed milliseconds and record count of one
+ * stage since the last reset; it synchronizes. */
 int pgvd_timing_enable(pgvd_ctx *ctx, int on);
 int pgvd_timing_reset(pgvd_ctx *ctx);
 int pgvd_timing_read(pgvd_ctx *ctx, const char *stage, double *total_ms, int64_t *count);
@@ -114,6 +117,29 @@ int pgvd_unique_sorted(pgvd_ctx *ctx, const void *keys_in, void *keys_out, int64
 /* Kernels::StreamScan_p / StreamScan_s (opencl/Kernels.cpp:102-159):
  * inclusive scan of n u32 values, modulo 2^32. */
 int pgvd_stream_scan(pgvd_ctx *ctx, const uint32_t *in, uint32_t *out, int64_t n, pgvd_stream s);
+
+/* The elementary steps of the reference's bit-serial sort and compaction
+ * (the whole-path entry does not use them):
+ *   Kernels::BitPredicate (opencl/Kernels.cpp:77-89, C/ParallelAlgorithms.c:13-18):
+ *     pred[i] = (bit `index` of key i == compared)
+ *   Kernels::UniquePredicate (opencl/Kernels.cpp:91-100, C/ParallelAlgorithms.c:22-31):
+ *     pred[0] = 1, pred[i] = (key i != key i-1)
+ *   Kernels::SingleCompact (opencl/Kernels.cpp:161-172, C/ParallelAlgorithms.c:73-81):
+ *     out[addr[i]-1] = in[i] where pred[i] == 1
+ *   Kernels::DoubleCompact (opencl/Kernels.cpp:174-196, C/ParallelAlgorithms.c:54-71):
+ *     keys with pred 1 to addr[i]-1, the rest after them in order
+ *   Kernels::InitOctree (opencl/Kernels.cpp:315-331, C/BuildOctree.c:134-139):
+ *     children = -1, leaf = all octant bits, for `nodes` nodes
+ * addr is the inclusive scan of pred (pgvd_stream_scan). */
+int pgvd_bit_predicate(pgvd_ctx *ctx, const void *keys, int key_bytes, uint32_t *pred, int index,
+                       int compared, int64_t n, pgvd_stream s);
+int pgvd_unique_predicate(pgvd_ctx *ctx, const void *keys, int key_bytes, uint32_t *pred,
+                          int64_t n, pgvd_stream s);
+int pgvd_single_compact(pgvd_ctx *ctx, const void *in, void *out, int key_bytes,
+                        const uint32_t *pred, const uint32_t *addr, int64_t n, pgvd_stream s);
+int pgvd_double_compact(pgvd_ctx *ctx, const void *in, void *out, int key_bytes,
+                        const uint32_t *pred, const uint32_t *addr, int64_t n, pgvd_stream s);
+int pgvd_init_octree(pgvd_ctx *ctx, void *octree, int64_t nodes, int dim, pgvd_stream s);
 
 /* BRT node records (C/BrtNode.h:10-21 without `parent`, which is a separate
  * int32 array with the reference's values: -1 for the root, 0 for a node no

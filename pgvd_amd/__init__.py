@@ -232,8 +232,9 @@ class Device:
     def copy_kernel(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int):
         self._check(lib().pgvd_copy_kernel(self.ctx, dst.ptr, src.ptr, nbytes, None))
 
-    def timing(self, on: bool = True):
-        self._check(lib().pgvd_timing_enable(self.ctx, int(on)))
+    def timing(self, on: bool = True, sort_span_only: bool = False):
+        mode = (2 if sort_span_only else 1) if on else 0
+        self._check(lib().pgvd_timing_enable(self.ctx, mode))
 
     def timing_reset(self):
         self._check(lib().pgvd_timing_reset(self.ctx))

@@ -91,7 +91,7 @@ struct pgvd_ctx {
   RunState *st = nullptr;       // device
   RunState *st_host = nullptr;  // pinned mirror
   hipEvent_t done = nullptr;
-  bool timing = false;
+  int timing = 0;               // 0 off, 1 every stage, 2 sort span only
   std::vector<TimingRec> recs;
   std::vector<hipEvent_t> spare;
   LastBuild last;
@@ -151,8 +151,8 @@ struct Stage {
   hipStream_t s;
   hipEvent_t a = nullptr;
   const char *name;
-  Stage(pgvd_ctx *c, const char *nm, hipStream_t st) : ctx(c), s(st), name(nm) {
-    if (ctx->timing) {
+  Stage(pgvd_ctx *c, const char *nm, hipStream_t st, int mode = 1) : ctx(c), s(st), name(nm) {
+    if (ctx->timing == mode) {
       a = take_event(ctx);
       if (a) (void)hipEventRecord(a, s);
     }
@@ -174,7 +174,7 @@ struct PassTimer {
   pgvd_ctx *ctx;
   std::vector<hipEvent_t> ev;
   PassTimer(pgvd_ctx *c, int passes) : ctx(c) {
-    if (!ctx->timing) return;
+    if (ctx->timing != 1) return;
     for (int i = 0; i < 2 * passes; ++i) {
       hipEvent_t e = take_event(ctx);
       if (!e) { ev.clear(); return; }
@@ -195,11 +195,11 @@ int launch_status(const char *what) {
 
 // Workspace that must be zero at the start of a run, as one block.
 struct ZeroLayout {
-  size_t hist = 0, sort_lb = 0, lb_unique = 0, lb_splits = 0, lb_scan = 0, parent = 0,
+  size_t hist = 0, sort_lb = 0, lb_unique = 0, lb_splits = 0, lb_scan = 0, lb_brt = 0,
          cbits = 0, total = 0;
 };
 
-ZeroLayout zero_layout(int64_t n, int kb, int passes, bool labels, bool with_tree) {
+ZeroLayout zero_layout(int64_t n, int kb, int passes, bool labels) {
   ZeroLayout z;
   size_t off = 0;
   z.hist = off;
@@ -213,8 +213,8 @@ ZeroLayout zero_layout(int64_t n, int kb, int passes, bool labels, bool with_tre
   off += align256((size_t)scan_tiles(n + 1) * 8);
   z.lb_scan = off;
   off += align256((size_t)scan_tiles(n + 1) * 8);
-  z.parent = off;
-  if (with_tree) off += align256((size_t)(n + 1) * 4);
+  z.lb_brt = off;
+  off += align256((size_t)brt_scan_tiles(n + 1) * 8);
   z.cbits = off;
   if (labels) off += align256((size_t)((n + 1 + 31) / 32) * 4);
   z.total = off;
@@ -295,14 +295,16 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   BrtNodeT<K> *brt = static_cast<BrtNodeT<K> *>(p);
   CHK(pool_get(ctx, "ps", (size_t)nk * 4, &p));
   uint32_t *ps = static_cast<uint32_t *>(p);
-  const ZeroLayout z = zero_layout(n, kb, passes, lab, true);
+  CHK(pool_get(ctx, "parent", (size_t)nk * 4, &p));
+  int32_t *parent = static_cast<int32_t *>(p);
+  const ZeroLayout z = zero_layout(n, kb, passes, lab);
   CHK(pool_get(ctx, "zero", z.total, &p));
   char *zb = static_cast<char *>(p);
   uint32_t *hist = reinterpret_cast<uint32_t *>(zb + z.hist);
   uint32_t *sort_lb = reinterpret_cast<uint32_t *>(zb + z.sort_lb);
   uint64_t *lb_u = reinterpret_cast<uint64_t *>(zb + z.lb_unique);
   uint64_t *lb_s = reinterpret_cast<uint64_t *>(zb + z.lb_splits);
-  int32_t *parent = reinterpret_cast<int32_t *>(zb + z.parent);
+  uint64_t *lb_b = reinterpret_cast<uint64_t *>(zb + z.lb_brt);
   uint32_t *cbits = lab ? reinterpret_cast<uint32_t *>(zb + z.cbits) : nullptr;
   int32_t *octree;
   uint32_t *nconf;
@@ -340,6 +342,7 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   CHK(launch_status("encode"));
   int parity;
   {
+    Stage span(ctx, "sort", s, 2);
     PassTimer pt(ctx, n > 1 ? passes : 0);
     parity = launch_onesweep<K>(kA, kB, labels, vA, vB, n, passes, hist, sort_lb, ctx->st, s,
                                 pt.events());
@@ -355,14 +358,24 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   CHK(launch_status("unique"));
   {
     Stage t(ctx, "brt", s);
-    launch_brt<K>(uniq, nk, -1, mbits, brt, parent, ctx->st, s);
+    // local splits go to the sort's spare key buffer (free after unique)
+    launch_brt_ls<K>(uniq, nk, mbits, dim, brt, parent, reinterpret_cast<uint32_t *>(sorted),
+                     ctx->st, s);
   }
   CHK(launch_status("brt"));
   {
     Stage t(ctx, "splits_scan", s);
-    launch_splits_scan<K>(brt, parent, nullptr, ps, nk, dim, lb_s, ctx->st, s);
+    launch_ls_scan(reinterpret_cast<const uint32_t *>(sorted), ps, nk, lb_b, ctx->st, s);
   }
   CHK(launch_status("splits_scan"));
+  {
+    // order-exact claims, local splits and scan; no-ops unless some BRT
+    // split took the FP32-quirk value
+    Stage t(ctx, "malformed", s);
+    launch_claims_if_malformed<K>(brt, parent, nk, ctx->st, s);
+    launch_splits_scan<K>(brt, parent, nullptr, ps, nk, dim, 1, lb_s, ctx->st, s);
+  }
+  CHK(launch_status("malformed path"));
   {
     Stage t(ctx, "octree", s);
     launch_octree<K>(brt, parent, ps, uniq, cbits, nk, dim, mbits, octree, nconf,
@@ -419,7 +432,7 @@ int radix_sort_impl(pgvd_ctx *ctx, K *keys, uint32_t *vals, int64_t n, int mbits
     CHK(pool_get(ctx, "sort_alt_v", (size_t)n * 4, &p));
     altv = static_cast<uint32_t *>(p);
   }
-  const ZeroLayout z = zero_layout(n, (int)sizeof(K), passes, vals != nullptr, false);
+  const ZeroLayout z = zero_layout(n, (int)sizeof(K), passes, vals != nullptr);
   CHK(pool_get(ctx, "zero_stage", z.total, &p));
   char *zb = static_cast<char *>(p);
   uint32_t *hist = reinterpret_cast<uint32_t *>(zb + z.hist);
@@ -442,7 +455,7 @@ int radix_sort_impl(pgvd_ctx *ctx, K *keys, uint32_t *vals, int64_t n, int mbits
 
 template <typename K>
 int unique_impl(pgvd_ctx *ctx, const K *in, K *out, int64_t n, int64_t *m_out, hipStream_t s) {
-  const ZeroLayout z = zero_layout(n, (int)sizeof(K), 0, false, false);
+  const ZeroLayout z = zero_layout(n, (int)sizeof(K), 0, false);
   void *p;
   CHK(pool_get(ctx, "zero_stage", z.total, &p));
   char *zb = static_cast<char *>(p);
@@ -492,7 +505,7 @@ int brt_to_octree_impl(pgvd_ctx *ctx, const BrtNodeT<K> *brt, const int32_t *par
   uint32_t *ls = static_cast<uint32_t *>(p);
   CHK(pool_get(ctx, "stage_ps", (size_t)m * 4, &p));
   uint32_t *ps = static_cast<uint32_t *>(p);
-  const ZeroLayout z = zero_layout(m, (int)sizeof(K), 0, false, false);
+  const ZeroLayout z = zero_layout(m, (int)sizeof(K), 0, false);
   CHK(pool_get(ctx, "zero_stage", z.total, &p));
   char *zb = static_cast<char *>(p);
   launch_prep(ctx->st, reinterpret_cast<uint32_t *>(zb), z.total / 4, s);
@@ -627,7 +640,7 @@ int pgvd_copy_kernel(pgvd_ctx *ctx, void *dst, const void *src, size_t bytes, pg
 
 int pgvd_timing_enable(pgvd_ctx *ctx, int on) {
   if (!ctx) return fail(PGVD_ERR_INVALID, "null context");
-  ctx->timing = on != 0;
+  ctx->timing = (on == 2) ? 2 : (on != 0 ? 1 : 0);
   return PGVD_OK;
 }
 
@@ -767,7 +780,7 @@ int pgvd_stream_scan(pgvd_ctx *ctx, const uint32_t *in, uint32_t *out, int64_t n
   if (n == 0) return PGVD_OK;
   CHK(use_device(ctx));
   hipStream_t st = pick(ctx, s);
-  const ZeroLayout z = zero_layout(n, 4, 0, false, false);
+  const ZeroLayout z = zero_layout(n, 4, 0, false);
   void *p;
   CHK(pool_get(ctx, "zero_stage", z.total, &p));
   char *zb = static_cast<char *>(p);
@@ -777,6 +790,63 @@ int pgvd_stream_scan(pgvd_ctx *ctx, const uint32_t *in, uint32_t *out, int64_t n
     launch_scan_u32(in, out, n, reinterpret_cast<uint64_t *>(zb + z.lb_scan), ctx->st, st);
   }
   return launch_status("scan");
+}
+
+#define PGVD_BY_KEY(kb, CALL)                                                   \
+  do {                                                                          \
+    if ((kb) == 4) { using K = uint32_t; CALL; }                                \
+    else if ((kb) == 8) { using K = uint64_t; CALL; }                           \
+    else if ((kb) == 16) { using K = Key128; CALL; }                            \
+    else return fail(PGVD_ERR_INVALID, "key_bytes must be 4, 8 or 16");        \
+  } while (0)
+
+int pgvd_bit_predicate(pgvd_ctx *ctx, const void *keys, int key_bytes, uint32_t *pred, int index,
+                       int compared, int64_t n, pgvd_stream s) {
+  if (!ctx || (n > 0 && (!keys || !pred))) return fail(PGVD_ERR_INVALID, "null argument");
+  if (n <= 0) return PGVD_OK;
+  CHK(use_device(ctx));
+  PGVD_BY_KEY(key_bytes, launch_bit_predicate<K>(static_cast<const K *>(keys), pred, index,
+                                                 (uint32_t)(compared != 0), n, pick(ctx, s)));
+  return launch_status("bit predicate");
+}
+
+int pgvd_unique_predicate(pgvd_ctx *ctx, const void *keys, int key_bytes, uint32_t *pred,
+                          int64_t n, pgvd_stream s) {
+  if (!ctx || (n > 0 && (!keys || !pred))) return fail(PGVD_ERR_INVALID, "null argument");
+  if (n <= 0) return PGVD_OK;
+  CHK(use_device(ctx));
+  PGVD_BY_KEY(key_bytes, launch_unique_predicate<K>(static_cast<const K *>(keys), pred, n,
+                                                    pick(ctx, s)));
+  return launch_status("unique predicate");
+}
+
+int pgvd_single_compact(pgvd_ctx *ctx, const void *in, void *out, int key_bytes,
+                        const uint32_t *pred, const uint32_t *addr, int64_t n, pgvd_stream s) {
+  if (!ctx || (n > 0 && (!in || !out || !pred || !addr))) return fail(PGVD_ERR_INVALID, "null argument");
+  if (n <= 0) return PGVD_OK;
+  CHK(use_device(ctx));
+  PGVD_BY_KEY(key_bytes, launch_single_compact<K>(static_cast<const K *>(in), static_cast<K *>(out),
+                                                  pred, addr, n, pick(ctx, s)));
+  return launch_status("single compact");
+}
+
+int pgvd_double_compact(pgvd_ctx *ctx, const void *in, void *out, int key_bytes,
+                        const uint32_t *pred, const uint32_t *addr, int64_t n, pgvd_stream s) {
+  if (!ctx || (n > 0 && (!in || !out || !pred || !addr))) return fail(PGVD_ERR_INVALID, "null argument");
+  if (n <= 0) return PGVD_OK;
+  CHK(use_device(ctx));
+  PGVD_BY_KEY(key_bytes, launch_double_compact<K>(static_cast<const K *>(in), static_cast<K *>(out),
+                                                  pred, addr, n, pick(ctx, s)));
+  return launch_status("double compact");
+}
+
+int pgvd_init_octree(pgvd_ctx *ctx, void *octree, int64_t nodes, int dim, pgvd_stream s) {
+  if (!ctx || (nodes > 0 && !octree)) return fail(PGVD_ERR_INVALID, "null argument");
+  if (dim != 2 && dim != 3) return fail(PGVD_ERR_INVALID, "dim must be 2 or 3");
+  if (nodes <= 0) return PGVD_OK;
+  CHK(use_device(ctx));
+  launch_init_octree(static_cast<int32_t *>(octree), nodes, dim, pick(ctx, s));
+  return launch_status("init octree");
 }
 
 int pgvd_build_brt(pgvd_ctx *ctx, const void *keys, int64_t m, int key_bytes, int mbits,

@@ -69,6 +69,20 @@ template <typename K>
 void launch_encode_int(const int32_t *pts, int64_t n, int dim, int bits, K *keys,
                        uint32_t *hist, int passes, hipStream_t s);
 
+// ---- stages.hip (the reference's elementary stage kernels) ----
+template <typename K>
+void launch_bit_predicate(const K *in, uint32_t *pred, int index, uint32_t compared, int64_t n,
+                          hipStream_t s);
+template <typename K>
+void launch_unique_predicate(const K *in, uint32_t *pred, int64_t n, hipStream_t s);
+template <typename K>
+void launch_single_compact(const K *in, K *out, const uint32_t *pred, const uint32_t *addr,
+                           int64_t n, hipStream_t s);
+template <typename K>
+void launch_double_compact(const K *in, K *out, const uint32_t *pred, const uint32_t *addr,
+                           int64_t n, hipStream_t s);
+void launch_init_octree(int32_t *octree, int64_t nodes, int dim, hipStream_t s);
+
 // ---- util.hip ----
 void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
@@ -101,10 +115,11 @@ void launch_unique(const K *keys, const uint32_t *labels, int64_t n, int prepend
 // Local splits (from the BRT) scanned inclusively into ps[0..m); rewrites
 // parent[] from claim form (claimer+1) to the reference form; writes ls[]
 // when non-null; st->octree_size = ps[m-1].
+// only_if_malformed: the kernel returns at once unless st->malformed.
 template <typename K>
 void launch_splits_scan(const BrtNodeT<K> *brt, int32_t *parent, uint32_t *ls, uint32_t *ps,
-                        int64_t max_m, int dim, uint64_t *lookback, RunState *st,
-                        hipStream_t s);
+                        int64_t max_m, int dim, int only_if_malformed, uint64_t *lookback,
+                        RunState *st, hipStream_t s);
 // Plain inclusive scan of n u32 values (StreamScan_p).
 void launch_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint64_t *lookback,
                      RunState *st, hipStream_t s);
@@ -118,6 +133,26 @@ void launch_brt(const K *keys, int64_t max_m, int64_t m_host, int mbits, BrtNode
 // Rewrite parent[] from claim form to the reference form (BRT stage API).
 void launch_finalize_parents(int32_t *parent, int64_t max_m, int64_t m_host, RunState *st,
                              hipStream_t s);
+// Pipeline: BRT + local splits + inclusive scan in one pass (m = st->m);
+// parent[] in the reference form; ps[0..m); st->octree_size.  lookback has
+// brt_scan_tiles(max_m) zeroed words.
+int64_t brt_scan_tiles(int64_t max_m);
+template <typename K>
+void launch_brt_scan(const K *keys, int64_t max_m, int mbits, int dim, BrtNodeT<K> *brt,
+                     int32_t *parent, uint32_t *ps, uint64_t *lookback, RunState *st,
+                     hipStream_t s);
+// Pipeline: one BRT node per lane with its local split into ls[0..m) and
+// parent links as plain stores (well-formed tree; see brt.hip).
+template <typename K>
+void launch_brt_ls(const K *keys, int64_t max_m, int mbits, int dim, BrtNodeT<K> *brt,
+                   int32_t *parent, uint32_t *ls, RunState *st, hipStream_t s);
+// Inclusive scan of ls[0..st->m) into ps; st->octree_size = total.
+void launch_ls_scan(const uint32_t *ls, uint32_t *ps, int64_t max_m, uint64_t *lookback,
+                    RunState *st, hipStream_t s);
+// When st->malformed: parent[] <- claim form (zero, then atomicMax claims).
+template <typename K>
+void launch_claims_if_malformed(const BrtNodeT<K> *brt, int32_t *parent, int64_t max_m,
+                                RunState *st, hipStream_t s);
 
 // ---- octree.hip ----
 // Octree fill.  parent[] in the reference form.  force_scatter selects the

@@ -1,0 +1,464 @@
+// kernels_api.hip -- C++ host layer (include/pgvd_kernels.hpp): the
+// reference's namespace Kernels stage API (opencl/Kernels.h:26-50) and Karras
+// facade (Karras.h:16-36) over the C ABI.  Host code only.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/pgvd_kernels.hpp"
+
+namespace pgvd {
+
+namespace {
+
+int key_bytes_for(int mbits) { return mbits <= 32 ? 4 : (mbits <= 64 ? 8 : 16); }
+size_t brt_node_bytes(int kb) { return kb == 4 ? 12 : (kb == 8 ? 16 : 24); }
+
+std::string err() { return pgvd_last_error(); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- Resln ---
+Resln make_resln(int64_t width, int dim) {
+  if (width <= 0) throw std::logic_error("No support for width of 0");
+  Resln r;
+  r.width = width;
+  r.bits = 0;
+  int64_t w = width;
+  while (!(w & 1)) {
+    ++r.bits;
+    w >>= 1;
+  }
+  r.mbits = r.bits * dim;
+  return r;
+}
+
+// ------------------------------------------------------- Buffer/Context ---
+Buffer::Mem::~Mem() {
+  if (ptr && ctx) pgvd_free(ctx.get(), ptr);
+}
+
+Buffer Buffer::parent_links() const {
+  Buffer b;
+  b.mem_ = aux_;
+  return b;
+}
+
+Context::Context(int device) {
+  pgvd_ctx *c = nullptr;
+  const int rc = pgvd_ctx_create(device, &c);
+  if (rc != PGVD_OK) throw std::runtime_error("pgvd_ctx_create: " + err());
+  ctx_ = std::shared_ptr<pgvd_ctx>(c, [](pgvd_ctx *p) { pgvd_ctx_destroy(p); });
+}
+
+Context::~Context() = default;
+
+Context &Context::Default() {
+  static Context ctx(0);
+  return ctx;
+}
+
+Buffer Context::alloc(size_t bytes) {
+  Buffer b;
+  auto m = std::make_shared<Buffer::Mem>();
+  m->ctx = ctx_;
+  if (pgvd_alloc(ctx_.get(), bytes ? bytes : 1, &m->ptr) != PGVD_OK)
+    throw std::runtime_error("pgvd_alloc: " + err());
+  m->bytes = bytes;
+  b.mem_ = m;
+  return b;
+}
+
+Buffer Context::alloc_aux(Buffer &owner, size_t bytes) {
+  if (!owner.aux_ || owner.aux_->bytes < bytes) {
+    Buffer a = alloc(bytes);
+    owner.aux_ = a.mem_;
+  }
+  return owner.parent_links();
+}
+
+int Context::get(Buffer &buffer, const std::string &key, size_t bytes) {
+  for (auto &kv : pool_) {
+    if (kv.first != key) continue;
+    if (kv.second->bytes < bytes) {
+      Buffer fresh = alloc(bytes);
+      kv.second = fresh.mem_;
+    }
+    buffer.mem_ = kv.second;
+    return PGVD_OK;
+  }
+  Buffer fresh = alloc(bytes);
+  pool_.emplace_back(key, fresh.mem_);
+  buffer.mem_ = fresh.mem_;
+  return PGVD_OK;
+}
+
+int Context::upload(Buffer &dst, const void *src, size_t bytes) {
+  return pgvd_memcpy_htod(ctx_.get(), dst.ptr(), src, bytes, nullptr);
+}
+
+int Context::download(void *dst, const Buffer &src, size_t bytes) {
+  return pgvd_memcpy_dtoh(ctx_.get(), dst, src.ptr(), bytes, nullptr);
+}
+
+int Context::synchronize() { return pgvd_synchronize(ctx_.get(), nullptr); }
+
+namespace Kernels {
+
+namespace {
+bool g_benchmarking = false;
+std::string g_bench_name;
+std::chrono::steady_clock::time_point g_bench_t0;
+Context &ctx() { return Context::Default(); }
+}  // namespace
+
+// opencl/Kernels.cpp:5-18: wall time (the reference used clock(), CPU time)
+// with the queue drained on both sides.
+void setBenchmarking(bool on) { g_benchmarking = on; }
+void startBenchmark(std::string benchmarkName) {
+  if (!g_benchmarking) return;
+  ctx().synchronize();
+  g_bench_name = benchmarkName;
+  g_bench_t0 = std::chrono::steady_clock::now();
+}
+void stopBenchmark() {
+  if (!g_benchmarking) return;
+  ctx().synchronize();
+  const double ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - g_bench_t0).count();
+  std::printf("%s: %.3f ms\n", g_bench_name.c_str(), ms);
+}
+
+int nextPow2(int num) { return pgvd_next_pow2(num); }
+
+template <typename P>
+static int upload_points(const std::vector<P> &points, Buffer &pointsBuffer, int dim) {
+  startBenchmark("Uploading points");
+  const int round = nextPow2((int)points.size());
+  int rc = ctx().get(pointsBuffer, "pointsBuffer", sizeof(P) * (size_t)round);
+  if (rc == PGVD_OK) rc = ctx().upload(pointsBuffer, points.data(), sizeof(P) * points.size());
+  pointsBuffer.dim = dim;
+  pointsBuffer.coord_bytes = 4;
+  stopBenchmark();
+  return rc;
+}
+int UploadPoints(const std::vector<int2> &points, Buffer &pointsBuffer) {
+  return upload_points(points, pointsBuffer, 2);
+}
+int UploadPoints(const std::vector<int3> &points, Buffer &pointsBuffer) {
+  return upload_points(points, pointsBuffer, 3);
+}
+
+int PointsToMorton_p(Buffer &points, Buffer &zpoints, int size, int bits) {
+  if (points.dim != 2 && points.dim != 3) return PGVD_ERR_INVALID;
+  const int kb = key_bytes_for(bits * points.dim);
+  const int global = nextPow2(size);
+  int rc = ctx().get(zpoints, "zpoints", (size_t)global * kb);
+  if (rc != PGVD_OK) return rc;
+  startBenchmark("PointsToMorton_p");
+  rc = pgvd_points_to_morton(ctx().raw(), points.ptr(), points.coord_bytes ? points.coord_bytes : 4,
+                             size, points.dim, bits, zpoints.ptr(), kb, nullptr);
+  stopBenchmark();
+  zpoints.key_bytes = kb;
+  zpoints.dim = points.dim;
+  return rc;
+}
+
+int BitPredicate(Buffer &input, Buffer &predicate, unsigned int &index, unsigned char compared,
+                 int globalSize) {
+  int rc = ctx().get(predicate, "predicate", sizeof(uint32_t) * (size_t)globalSize);
+  if (rc != PGVD_OK) return rc;
+  return pgvd_bit_predicate(ctx().raw(), input.ptr(), input.key_bytes,
+                            static_cast<uint32_t *>(predicate.ptr()), (int)index, compared,
+                            globalSize, nullptr);
+}
+
+int UniquePredicate(Buffer &input, Buffer &predicate, int globalSize) {
+  if (predicate.bytes() < sizeof(uint32_t) * (size_t)globalSize) {
+    const int rc = ctx().get(predicate, "predicate", sizeof(uint32_t) * (size_t)globalSize);
+    if (rc != PGVD_OK) return rc;
+  }
+  return pgvd_unique_predicate(ctx().raw(), input.ptr(), input.key_bytes,
+                               static_cast<uint32_t *>(predicate.ptr()), globalSize, nullptr);
+}
+
+int StreamScan_p(Buffer &input, Buffer &result, int globalSize) {
+  if (result.bytes() < sizeof(uint32_t) * (size_t)globalSize) {
+    const int rc = ctx().get(result, "scanResult", sizeof(uint32_t) * (size_t)globalSize);
+    if (rc != PGVD_OK) return rc;
+  }
+  return pgvd_stream_scan(ctx().raw(), static_cast<const uint32_t *>(input.ptr()),
+                          static_cast<uint32_t *>(result.ptr()), globalSize, nullptr);
+}
+
+static int compact(Buffer &input, Buffer &result, Buffer &predicate, Buffer &address,
+                   int globalSize, bool dbl) {
+  const int kb = input.key_bytes;
+  if (result.bytes() < (size_t)kb * globalSize) {
+    const int rc = ctx().get(result, dbl ? "doubleCompact" : "singleCompact", (size_t)kb * globalSize);
+    if (rc != PGVD_OK) return rc;
+  }
+  result.key_bytes = kb;
+  result.dim = input.dim;
+  auto *p = static_cast<const uint32_t *>(predicate.ptr());
+  auto *a = static_cast<const uint32_t *>(address.ptr());
+  return dbl ? pgvd_double_compact(ctx().raw(), input.ptr(), result.ptr(), kb, p, a, globalSize, nullptr)
+             : pgvd_single_compact(ctx().raw(), input.ptr(), result.ptr(), kb, p, a, globalSize, nullptr);
+}
+int SingleCompact(Buffer &input, Buffer &result, Buffer &predicate, Buffer &address,
+                  int globalSize) {
+  return compact(input, result, predicate, address, globalSize, false);
+}
+int DoubleCompact(Buffer &input, Buffer &result, Buffer &predicate, Buffer &address,
+                  int globalSize) {
+  return compact(input, result, predicate, address, globalSize, true);
+}
+
+// opencl/Kernels.cpp:198-217: unique over the padded array; rebinds input.
+int UniqueSorted(Buffer &input, int &size) {
+  startBenchmark("UniqueSorted");
+  const int global = nextPow2(size);
+  Buffer result;
+  int rc = ctx().get(result, "result", (size_t)input.key_bytes * global);
+  if (rc != PGVD_OK) return rc;
+  int64_t m = 0;
+  rc = pgvd_unique_sorted(ctx().raw(), input.ptr(), result.ptr(), global, input.key_bytes, &m,
+                          nullptr);
+  result.key_bytes = input.key_bytes;
+  result.dim = input.dim;
+  input = result;
+  size = (int)m;
+  stopBenchmark();
+  return rc;
+}
+
+// opencl/Kernels.cpp:219-247: sorts the nextPow2(size) keys of the buffer.
+int RadixSortBigUnsigned(Buffer &input, int size, int mbits) {
+  const int kb = input.key_bytes;
+  if (kb != 4 && kb != 8 && kb != 16) return PGVD_ERR_INVALID;
+  const int bits = mbits < kb * 8 ? mbits : kb * 8;   // bits past the key width are zero
+  startBenchmark("RadixSortBigUnsigned");
+  const int rc = pgvd_radix_sort(ctx().raw(), input.ptr(), nullptr, nextPow2(size), kb, bits, nullptr);
+  stopBenchmark();
+  return rc;
+}
+
+int BuildBinaryRadixTree_p(Buffer &zpoints, Buffer &internalBRTNodes, int size, int mbits) {
+  startBenchmark("BuildBinaryRadixTree_p");
+  const int kb = zpoints.key_bytes;
+  const int global = nextPow2(size);
+  int rc = ctx().get(internalBRTNodes, "internalBRTNodes", brt_node_bytes(kb) * (size_t)global);
+  if (rc != PGVD_OK) return rc;
+  Buffer parent = ctx().alloc_aux(internalBRTNodes, sizeof(int32_t) * (size_t)global);
+  rc = pgvd_build_brt(ctx().raw(), zpoints.ptr(), size, kb, mbits, internalBRTNodes.ptr(),
+                      static_cast<int32_t *>(parent.ptr()), nullptr);
+  internalBRTNodes.key_bytes = kb;
+  internalBRTNodes.dim = zpoints.dim;
+  stopBenchmark();
+  return rc;
+}
+
+int ComputeLocalSplits_p(Buffer &internalBRTNodes, Buffer &localSplits, int size) {
+  startBenchmark("ComputeLocalSplits_p");
+  int rc = ctx().get(localSplits, "localSplits", sizeof(uint32_t) * (size_t)nextPow2(size));
+  if (rc != PGVD_OK) return rc;
+  rc = pgvd_local_splits(ctx().raw(), internalBRTNodes.ptr(), size, internalBRTNodes.key_bytes,
+                         internalBRTNodes.dim, static_cast<uint32_t *>(localSplits.ptr()), nullptr);
+  stopBenchmark();
+  return rc;
+}
+
+int InitOctree(Buffer &internalBRTNodes, Buffer &octree, Buffer &, Buffer &, int, int octreeSize) {
+  startBenchmark("InitOctree");
+  const int dim = internalBRTNodes.dim;
+  const size_t stride = sizeof(int32_t) * ((1u << dim) + 1);
+  if (octree.bytes() < stride * (size_t)octreeSize) {
+    const int rc = ctx().get(octree, "octree", stride * (size_t)nextPow2(octreeSize));
+    if (rc != PGVD_OK) return rc;
+  }
+  const int rc = pgvd_init_octree(ctx().raw(), octree.ptr(), octreeSize, dim, nullptr);
+  stopBenchmark();
+  return rc;
+}
+
+template <int DIM>
+static int brt_to_octree(Buffer &I, std::vector<OctNodeT<DIM>> &octree_vec, int size) {
+  if (I.dim != DIM) return PGVD_ERR_INVALID;
+  startBenchmark("BinaryRadixToOctree_p");
+  int64_t cap = 2 * (int64_t)size + 16, osz = 0;
+  Buffer oct;
+  int rc;
+  for (;;) {
+    rc = ctx().get(oct, "octree", sizeof(OctNodeT<DIM>) * (size_t)cap);
+    if (rc != PGVD_OK) return rc;
+    rc = pgvd_brt_to_octree(ctx().raw(), I.ptr(), static_cast<const int32_t *>(I.parent_links().ptr()),
+                            size, I.key_bytes, DIM, oct.ptr(), cap, &osz, nullptr);
+    if (rc != PGVD_ERR_CAPACITY) break;
+    cap = osz + osz / 8 + 16;
+  }
+  if (rc != PGVD_OK) return rc;
+  octree_vec.resize((size_t)osz);
+  rc = ctx().download(octree_vec.data(), oct, sizeof(OctNodeT<DIM>) * (size_t)osz);
+  stopBenchmark();
+  return rc;
+}
+int BinaryRadixToOctree_p(Buffer &I, std::vector<OctNode2> &octree_vec, int size) {
+  return brt_to_octree<2>(I, octree_vec, size);
+}
+int BinaryRadixToOctree_p(Buffer &I, std::vector<OctNode3> &octree_vec, int size) {
+  return brt_to_octree<3>(I, octree_vec, size);
+}
+
+// opencl/Kernels.cpp:413-429, all on the device.
+template <typename P, int DIM>
+static int build_octree(const std::vector<P> &points, std::vector<OctNodeT<DIM>> &octree,
+                        int bits, int mbits) {
+  if (points.empty()) throw std::logic_error("Zero points not supported");
+  if (mbits != bits * DIM) return PGVD_ERR_INVALID;
+  Buffer pts;
+  int rc = upload_points(points, pts, DIM);
+  if (rc != PGVD_OK) return rc;
+  pgvd_build_opts opts;
+  std::memset(&opts, 0, sizeof(opts));
+  opts.quantized_input = 1;
+  rc = pgvd_build_octree(ctx().raw(), pts.ptr(), (int64_t)points.size(), DIM, bits, nullptr, &opts,
+                         nullptr);
+  if (rc != PGVD_OK) return rc;
+  pgvd_octree_result res;
+  rc = pgvd_build_octree_finish(ctx().raw(), &res);
+  if (rc != PGVD_OK) return rc;
+  octree.resize((size_t)res.octree_size);
+  return pgvd_memcpy_dtoh(ctx().raw(), octree.data(), res.octree,
+                          sizeof(OctNodeT<DIM>) * (size_t)res.octree_size, nullptr);
+}
+int BuildOctree_p(const std::vector<int2> &points, std::vector<OctNode2> &octree, int bits,
+                  int mbits) {
+  return build_octree<int2, 2>(points, octree, bits, mbits);
+}
+int BuildOctree_p(const std::vector<int3> &points, std::vector<OctNode3> &octree, int bits,
+                  int mbits) {
+  return build_octree<int3, 3>(points, octree, bits, mbits);
+}
+
+int ReadKeys(const Buffer &keys, int count, std::vector<uint64_t> &lo, std::vector<uint64_t> &hi) {
+  const int kb = keys.key_bytes;
+  std::vector<unsigned char> raw((size_t)kb * count);
+  const int rc = ctx().download(raw.data(), keys, raw.size());
+  lo.assign(count, 0);
+  hi.assign(count, 0);
+  for (int i = 0; i < count; ++i) {
+    if (kb == 4) { uint32_t v; std::memcpy(&v, &raw[(size_t)4 * i], 4); lo[i] = v; }
+    else if (kb == 8) std::memcpy(&lo[i], &raw[(size_t)8 * i], 8);
+    else { std::memcpy(&lo[i], &raw[(size_t)16 * i], 8); std::memcpy(&hi[i], &raw[(size_t)16 * i + 8], 8); }
+  }
+  return rc;
+}
+
+int ReadBrt(const Buffer &I, int size, std::vector<BrtNode> &nodes) {
+  const int kb = I.key_bytes;
+  const size_t nb = brt_node_bytes(kb);
+  const int n = size > 1 ? size - 1 : 0;
+  std::vector<unsigned char> raw(nb * (size_t)n);
+  std::vector<int32_t> parent((size_t)n);
+  int rc = ctx().download(raw.data(), I, raw.size());
+  if (rc == PGVD_OK) rc = ctx().download(parent.data(), I.parent_links(), sizeof(int32_t) * (size_t)n);
+  nodes.resize((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const unsigned char *r = &raw[nb * i];
+    BrtNode &b = nodes[i];
+    b.lcp_lo = b.lcp_hi = 0;
+    size_t off;
+    if (kb == 4) { uint32_t v; std::memcpy(&v, r, 4); b.lcp_lo = v; off = 4; }
+    else if (kb == 8) { std::memcpy(&b.lcp_lo, r, 8); off = 8; }
+    else { std::memcpy(&b.lcp_lo, r, 8); std::memcpy(&b.lcp_hi, r + 8, 8); off = 16; }
+    uint16_t len;
+    std::memcpy(&b.left, r + off, 4);
+    std::memcpy(&len, r + off + 4, 2);
+    b.left_leaf = r[off + 6] != 0;
+    b.right_leaf = r[off + 7] != 0;
+    b.lcp_length = len;
+    b.parent = parent[i];
+  }
+  return rc;
+}
+
+}  // namespace Kernels
+
+namespace Karras {
+
+template <typename F, typename I, int DIM>
+static std::vector<I> quantize(const std::vector<F> &points, const Resln &resln, const F *bbMin,
+                               const F *bbMax, bool clamped) {
+  std::vector<I> out;
+  if (points.empty()) return out;
+  if (resln.bits > 31) throw std::logic_error("int grid points need bits <= 31");
+  Context &c = Context::Default();
+  Buffer pts = c.alloc(sizeof(F) * points.size());
+  c.upload(pts, points.data(), sizeof(F) * points.size());
+  Buffer q = c.alloc(sizeof(int64_t) * DIM * points.size());
+  float bb[6];
+  const float *custom = nullptr;
+  if (bbMin && bbMax) {
+    std::memcpy(bb, bbMin, sizeof(F));
+    std::memcpy(bb + DIM, bbMax, sizeof(F));
+    custom = bb;
+  }
+  int64_t nout = 0;
+  const int rc = pgvd_quantize(c.raw(), static_cast<const float *>(pts.ptr()), (int64_t)points.size(),
+                               DIM, resln.bits, custom, clamped ? 1 : 0,
+                               static_cast<int64_t *>(q.ptr()), &nout, nullptr);
+  if (rc == PGVD_ERR_BAD_COORD) throw std::logic_error("bad coordinate");
+  if (rc != PGVD_OK) throw std::runtime_error("pgvd_quantize: " + std::string(pgvd_last_error()));
+  std::vector<int64_t> q64((size_t)nout * DIM);
+  c.download(q64.data(), q, sizeof(int64_t) * q64.size());
+  out.resize((size_t)nout);
+  for (int64_t i = 0; i < nout; ++i) {
+    int32_t v[DIM];
+    for (int k = 0; k < DIM; ++k) v[k] = (int32_t)q64[(size_t)i * DIM + k];
+    std::memcpy(&out[(size_t)i], v, sizeof(I));
+  }
+  return out;
+}
+
+std::vector<int2> Quantize(const std::vector<float2> &points, const Resln &resln,
+                           const float2 *bbMin, const float2 *bbMax, bool clamped) {
+  return quantize<float2, int2, 2>(points, resln, bbMin, bbMax, clamped);
+}
+std::vector<int3> Quantize(const std::vector<float3> &points, const Resln &resln,
+                           const float3 *bbMin, const float3 *bbMax, bool clamped) {
+  return quantize<float3, int3, 3>(points, resln, bbMin, bbMax, clamped);
+}
+
+std::vector<OctNode2> BuildOctreeInParallel(const std::vector<int2> &points, const Resln &r, bool) {
+  std::vector<OctNode2> octree;
+  const int rc = Kernels::BuildOctree_p(points, octree, r.bits, r.mbits);
+  if (rc != PGVD_OK) throw std::runtime_error("BuildOctree_p: " + std::string(pgvd_last_error()));
+  return octree;
+}
+std::vector<OctNode3> BuildOctreeInParallel(const std::vector<int3> &points, const Resln &r, bool) {
+  std::vector<OctNode3> octree;
+  const int rc = Kernels::BuildOctree_p(points, octree, r.bits, r.mbits);
+  if (rc != PGVD_OK) throw std::runtime_error("BuildOctree_p: " + std::string(pgvd_last_error()));
+  return octree;
+}
+
+std::vector<OctNode3> BuildOctreeFromPoints(const std::vector<float3> &points, const Resln &r) {
+  if (points.empty()) throw std::logic_error("Zero points not supported");
+  Context &c = Context::Default();
+  Buffer pts = c.alloc(sizeof(float3) * points.size());
+  c.upload(pts, points.data(), sizeof(float3) * points.size());
+  int rc = pgvd_build_octree(c.raw(), pts.ptr(), (int64_t)points.size(), 3, r.bits, nullptr,
+                             nullptr, nullptr);
+  pgvd_octree_result res;
+  if (rc == PGVD_OK) rc = pgvd_build_octree_finish(c.raw(), &res);
+  if (rc == PGVD_ERR_BAD_COORD) throw std::logic_error("bad coordinate");
+  if (rc != PGVD_OK) throw std::runtime_error("pgvd_build_octree: " + std::string(pgvd_last_error()));
+  std::vector<OctNode3> octree((size_t)res.octree_size);
+  pgvd_memcpy_dtoh(c.raw(), octree.data(), res.octree, sizeof(OctNode3) * octree.size(), nullptr);
+  return octree;
+}
+
+}  // namespace Karras
+
+}  // namespace pgvd

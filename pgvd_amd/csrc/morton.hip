@@ -135,6 +135,121 @@ __global__ __launch_bounds__(kBlock) void encode_kernel(
   }
 }
 
+// ---- vectorised forms: a chunk is 4 points = DIM float4 loads ------------
+template <int DIM>
+__device__ inline void load_chunk(const float *__restrict__ pts, int64_t ch, float (&f)[4 * DIM]) {
+  const float4 *p4 = reinterpret_cast<const float4 *>(pts) + ch * DIM;
+#pragma unroll
+  for (int v = 0; v < DIM; ++v) {
+    const float4 a = p4[v];
+    f[4 * v + 0] = a.x; f[4 * v + 1] = a.y; f[4 * v + 2] = a.z; f[4 * v + 3] = a.w;
+  }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void bbox_vec_kernel(const float *__restrict__ pts, int64_t n,
+                                                          RunState *st) {
+  float mn[DIM], mx[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) { mn[k] = __int_as_float(0x7f800000); mx[k] = -mn[k]; }
+  const int64_t nch = n / 4;
+  const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t ch = tid; ch < nch; ch += stride) {
+    float f[4 * DIM];
+    load_chunk<DIM>(pts, ch, f);
+#pragma unroll
+    for (int j = 0; j < 4 * DIM; ++j) {
+      mn[j % DIM] = fminf(mn[j % DIM], f[j]);
+      mx[j % DIM] = fmaxf(mx[j % DIM], f[j]);
+    }
+  }
+  for (int64_t i = nch * 4 + tid; i < n; i += stride) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      mn[k] = fminf(mn[k], pts[i * DIM + k]);
+      mx[k] = fmaxf(mx[k], pts[i * DIM + k]);
+    }
+  }
+  __shared__ float smn[4][DIM], smx[4][DIM];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    for (int d = 32; d >= 1; d >>= 1) {
+      mn[k] = fminf(mn[k], __shfl_xor(mn[k], d, 64));
+      mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], d, 64));
+    }
+    if (lane == 0) { smn[w][k] = mn[k]; smx[w][k] = mx[k]; }
+  }
+  __syncthreads();
+  if (threadIdx.x < DIM) {
+    const int k = threadIdx.x;
+    float a = smn[0][k], b = smx[0][k];
+    for (int j = 1; j < 4; ++j) { a = fminf(a, smn[j][k]); b = fmaxf(b, smx[j][k]); }
+    atomicMin(&st->bb_min[k], fenc(a));
+    atomicMax(&st->bb_max[k], fenc(b));
+  }
+}
+
+template <typename K>
+__device__ inline void store4(K *__restrict__ keys, int64_t ch, const K (&k)[4]) {
+  static_assert((4 * sizeof(K)) % 16 == 0, "4 keys are whole 16-byte words");
+  uint4 *dst = reinterpret_cast<uint4 *>(keys + ch * 4);
+  const uint4 *src = reinterpret_cast<const uint4 *>(k);
+#pragma unroll
+  for (int v = 0; v < (int)(4 * sizeof(K) / 16); ++v) dst[v] = src[v];
+}
+
+template <typename K, int DIM>
+__global__ __launch_bounds__(kBlock) void encode_vec_kernel(
+    const float *__restrict__ pts, int64_t n, int bits, float effw, int custom, float bmin0,
+    float bmin1, float bmin2, float bmax0, float bmax1, float bmax2, K *__restrict__ keys,
+    uint32_t *__restrict__ hist, int passes, RunState *st) {
+  __shared__ uint32_t sh[kMaxPasses * 256];
+  for (int i = threadIdx.x; i < passes * 256; i += kBlock) sh[i] = 0;
+  float mn[3], mx[3];
+  if (custom) {
+    mn[0] = bmin0; mn[1] = bmin1; mn[2] = bmin2;
+    mx[0] = bmax0; mx[1] = bmax1; mx[2] = bmax2;
+  } else {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) { mn[k] = fdec(st->bb_min[k]); mx[k] = fdec(st->bb_max[k]); }
+  }
+  float dwidth = __fsub_rn(mx[0], mn[0]);
+#pragma unroll
+  for (int k = 1; k < DIM; ++k) {
+    const float s = __fsub_rn(mx[k], mn[k]);
+    dwidth = (s > dwidth) ? s : dwidth;
+  }
+  __syncthreads();
+  bool neg = false;
+  const int64_t nch = n / 4;
+  const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  auto encode_one = [&](const float *p) -> K {
+    int64_t c[DIM];
+    neg |= quantize_point<DIM>(p, mn, dwidth, effw, c);
+    uint64_t lo, hi;
+    morton128<DIM>(c, bits, lo, hi);
+    const K key = from128<K>(lo, hi);
+    for (int r = 0; r < passes; ++r) atomicAdd(&sh[r * 256 + kdigit(key, 8 * r)], 1u);
+    return key;
+  };
+  for (int64_t ch = tid; ch < nch; ch += stride) {
+    float f[4 * DIM];
+    load_chunk<DIM>(pts, ch, f);
+    K kk[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) kk[j] = encode_one(f + j * DIM);
+    store4(keys, ch, kk);
+  }
+  for (int64_t i = nch * 4 + tid; i < n; i += stride) keys[i] = encode_one(pts + i * DIM);
+  if (neg) atomicOr(&st->status, (uint32_t)ST_NEG_COORD);
+  __syncthreads();
+  for (int i = threadIdx.x; i < passes * 256; i += kBlock)
+    if (sh[i]) atomicAdd(&hist[i], sh[i]);
+}
+
 template <typename K, int DIM, typename C>
 __global__ __launch_bounds__(kBlock) void points_to_morton_kernel(const C *__restrict__ pts,
                                                                   int64_t n, int64_t padded_n,
@@ -206,7 +321,15 @@ void launch_prep(RunState *st, uint32_t *zero_words, size_t nzero, hipStream_t s
   prep_kernel<<<grid, kBlock, 0, s>>>(st, reinterpret_cast<uint4 *>(zero_words), n16);
 }
 
+inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
 void launch_bbox(const float *pts, int64_t n, int dim, RunState *st, hipStream_t s) {
+  if (aligned16(pts)) {
+    const int grid = grid_for(n / 4 + 1, kBlock * 2);
+    if (dim == 2) bbox_vec_kernel<2><<<grid, kBlock, 0, s>>>(pts, n, st);
+    else bbox_vec_kernel<3><<<grid, kBlock, 0, s>>>(pts, n, st);
+    return;
+  }
   const int grid = grid_for(n, kBlock * 8);
   if (dim == 2) bbox_kernel<2><<<grid, kBlock, 0, s>>>(pts, n, st);
   else bbox_kernel<3><<<grid, kBlock, 0, s>>>(pts, n, st);
@@ -220,6 +343,18 @@ void launch_encode(const float *pts, int64_t n, int dim, int bits, float effw, i
   float a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
   if (custom_bb)
     for (int k = 0; k < dim; ++k) { a[k] = bbmin[k]; b[k] = bbmax[k]; }
+  if (keys && hist && !qout && aligned16(pts) && aligned16(keys)) {
+    const int vgrid = grid_for(n / 4 + 1, kBlock);
+    if (dim == 2)
+      encode_vec_kernel<K, 2><<<vgrid, kBlock, 0, s>>>(pts, n, bits, effw, custom_bb, a[0], a[1],
+                                                       a[2], b[0], b[1], b[2], keys, hist, passes,
+                                                       st);
+    else
+      encode_vec_kernel<K, 3><<<vgrid, kBlock, 0, s>>>(pts, n, bits, effw, custom_bb, a[0], a[1],
+                                                       a[2], b[0], b[1], b[2], keys, hist, passes,
+                                                       st);
+    return;
+  }
   if (dim == 2)
     encode_kernel<K, 2><<<grid, kBlock, 0, s>>>(pts, n, bits, effw, custom_bb, a[0], a[1], a[2],
                                                 b[0], b[1], b[2], keys, qout, hist, passes, st);

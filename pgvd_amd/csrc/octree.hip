@@ -63,6 +63,11 @@ __device__ inline void write_node(int32_t *octree, uint32_t idx, const int32_t *
   o[NC] = leaf;
 }
 
+// Octree nodes staged per workgroup: the nodes owned by BRT nodes
+// [q0, q0+256) are the contiguous range [ps[q0-1], ps[q0+255]); when it fits
+// here it is built in LDS and written out in coalesced runs.
+constexpr int kStageNodes = 256;   // small: this kernel is latency bound
+
 template <typename K, int DIM>
 __global__ __launch_bounds__(kBlock) void gather_kernel(
     const BrtNodeT<K> *__restrict__ brt, const uint32_t *__restrict__ ps,
@@ -71,9 +76,12 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(
     RunState *st) {
   if (st->malformed) return;
   constexpr int NC = 1 << DIM;
+  constexpr int STRIDE = NC + 1;
+  __shared__ int32_t s_nodes[kStageNodes * STRIDE];
   const uint32_t m = st->m;
   const uint32_t O = st->octree_size;
-  const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t q0 = blockIdx.x * kBlock;
+  const uint32_t q = q0 + threadIdx.x;
   if ((int64_t)O > capacity) {
     if (q == 0) atomicOr(&st->status, (uint32_t)ST_CAPACITY);
     return;
@@ -88,12 +96,23 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(
     }
     return;
   }
-  if (q >= m - 1) return;
-  const uint32_t psq = ps[q];
-  const uint32_t excl = q ? ps[q - 1] : 0u;
-  const uint32_t lsq = psq - excl;
-  if (q != 0 && lsq == 0) return;
-
+  if (q0 >= m - 1) return;
+  const uint32_t qlast = min(q0 + kBlock - 1, m - 2);
+  const uint32_t lo = q0 ? ps[q0 - 1] : 0u;
+  const uint32_t span = ps[qlast] - lo;
+  const bool staged = span <= (uint32_t)kStageNodes;   // uniform over the block
+  auto put = [&](uint32_t idx, const int32_t *c) {
+    if (staged) write_node<DIM>(s_nodes, idx - lo, c);
+    else write_node<DIM>(octree, idx, c);
+  };
+  bool active = q <= qlast;
+  uint32_t excl = 0, lsq = 0;
+  if (active) {
+    excl = q ? ps[q - 1] : 0u;
+    lsq = ps[q] - excl;
+    active = (q == 0) || lsq > 0;
+  }
+  if (active) {
   const BrtNodeT<K> nq = brt[q];
   const int lvl = nq.lcp_length / DIM;
   uint32_t cmask = 0;
@@ -130,14 +149,21 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(
       }
     }
   }
-  write_node<DIM>(octree, excl, ch);
+  put(excl, ch);
   if (node_conflict) node_conflict[excl] = cmask;
   for (uint32_t k = 1; k < lsq; ++k) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) ch[c] = -1;
     if (q != 0) ch[quadrant<K, DIM>(nq, (int)k - 1)] = (int32_t)(excl + k - 1);
-    write_node<DIM>(octree, excl + k, ch);
+    put(excl + k, ch);
     if (node_conflict) node_conflict[excl + k] = 0;
+  }
+  }  // active
+  if (staged) {
+    __syncthreads();
+    const uint32_t words = span * STRIDE;
+    int32_t *out = octree + (size_t)lo * STRIDE;
+    for (uint32_t w = threadIdx.x; w < words; w += kBlock) out[w] = s_nodes[w];
   }
 }
 
@@ -162,6 +188,14 @@ __global__ __launch_bounds__(kBlock) void scatter_init_kernel(int32_t *__restric
   }
 }
 
+template <typename K, int DIM>
+__device__ inline void scatter_one(uint32_t i, uint32_t m, const BrtNodeT<K> *__restrict__ brt,
+                                   const int32_t *__restrict__ parent,
+                                   const uint32_t *__restrict__ ps, const K *__restrict__ keys,
+                                   const uint32_t *__restrict__ conflict_bits, int mbits,
+                                   int32_t *__restrict__ octree,
+                                   uint32_t *__restrict__ node_conflict, RunState *st);
+
 // parent[] is in the reference form here (root -1, unclaimed 0).
 template <typename K, int DIM>
 __global__ __launch_bounds__(kBlock) void scatter_kernel(
@@ -170,11 +204,21 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(
     const uint32_t *__restrict__ conflict_bits, int mbits, int32_t *__restrict__ octree,
     uint32_t *__restrict__ node_conflict, int64_t capacity, int force, RunState *st) {
   if (!force && !st->malformed) return;
-  constexpr int NC = 1 << DIM;
   const uint32_t m = st->m;
   if ((int64_t)st->octree_size > capacity) return;
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (m < 2 || i >= m - 1) return;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i + 1 < m; i += gridDim.x * kBlock)
+    scatter_one<K, DIM>(i, m, brt, parent, ps, keys, conflict_bits, mbits, octree,
+                        node_conflict, st);
+}
+
+template <typename K, int DIM>
+__device__ inline void scatter_one(uint32_t i, uint32_t m, const BrtNodeT<K> *__restrict__ brt,
+                                   const int32_t *__restrict__ parent,
+                                   const uint32_t *__restrict__ ps, const K *__restrict__ keys,
+                                   const uint32_t *__restrict__ conflict_bits, int mbits,
+                                   int32_t *__restrict__ octree,
+                                   uint32_t *__restrict__ node_conflict, RunState *st) {
+  constexpr int NC = 1 << DIM;
   const BrtNodeT<K> ni = brt[i];
   // Bound on every parent walk: a walk visits each BRT node at most once.
   const uint32_t bound = m;
@@ -222,12 +266,14 @@ void launch_dim(const BrtNodeT<K> *brt, const int32_t *parent, const uint32_t *p
   if (!force_scatter)
     gather_kernel<K, DIM><<<grid, kBlock, 0, s>>>(brt, ps, keys, conflict_bits, mbits, octree,
                                                   node_conflict, capacity, st);
-  const unsigned igrid = grid < 4096 ? grid : 4096;
-  scatter_init_kernel<DIM><<<igrid, kBlock, 0, s>>>(octree, node_conflict, capacity,
+  // The scatter path is a no-op unless forced or st->malformed; its kernels
+  // loop over their ranges, so a capped grid keeps the no-op cheap.
+  const unsigned sgrid = grid < 1024 ? grid : 1024;
+  scatter_init_kernel<DIM><<<sgrid, kBlock, 0, s>>>(octree, node_conflict, capacity,
                                                     force_scatter, st);
-  scatter_kernel<K, DIM><<<grid, kBlock, 0, s>>>(brt, parent, ps, keys, conflict_bits, mbits,
-                                                 octree, node_conflict, capacity, force_scatter,
-                                                 st);
+  scatter_kernel<K, DIM><<<sgrid, kBlock, 0, s>>>(brt, parent, ps, keys, conflict_bits, mbits,
+                                                  octree, node_conflict, capacity, force_scatter,
+                                                  st);
 }
 
 // ComputeLocalSplits (C/BuildOctree.c:17-27) as the reference runs it: node

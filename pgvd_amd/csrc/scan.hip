@@ -1,4 +1,6 @@
-// scan.hip -- single-pass decoupled look-back scans.
+// scan.hip -- single-pass decoupled look-back scans over striped tiles
+// (thread t of a 256-thread workgroup holds elements r*256 + t, so every
+// load and store is coalesced).
 //
 //   unique        UniqueSorted (opencl/Kernels.cpp:198-217): UniquePredicate
 //                 (C/ParallelAlgorithms.c:22-31) + scan + BUSingleCompact
@@ -6,88 +8,31 @@
 //                 array with zero keys (opencl/Kernels.cpp:62-75); here the
 //                 padding is one virtual zero key in front of the sorted keys.
 //   splits scan   ComputeLocalSplits (C/BuildOctree.c:17-40) fused into the
-//                 inclusive scan of the local splits (opencl/Kernels.cpp:376).
+//                 inclusive scan of the local splits (opencl/Kernels.cpp:376),
+//                 from parent claims -- the order-exact form used when the
+//                 BRT is not a well-formed radix tree.
 //   scan_u32      StreamScan_p / StreamScan_s (opencl/Kernels.cpp:102-159):
 //                 inclusive scan of u32 values (mod 2^32).
-//
-// Each workgroup claims a tile id from an atomic counter, reduces its tile,
-// publishes the aggregate, and one wave resolves the exclusive prefix by
-// looking back over up to 64 predecessor tiles at a time.
-#include "kernels.hpp"
+#include "tilescan.hpp"
 
 namespace pgvd {
 
 namespace {
 
-constexpr uint64_t kSAgg = 1ull << 62;
-constexpr uint64_t kSIncl = 2ull << 62;
-constexpr uint64_t kSVal = (1ull << 62) - 1;
-constexpr int kIPT = kScanTile / kBlock;  // 16
+constexpr int kRows = kScanTile / kBlock;  // 16
 
-enum { CTR_UNIQUE = kMaxPasses, CTR_SPLITS = kMaxPasses + 1, CTR_SCAN = kMaxPasses + 2 };
+enum {
+  CTR_UNIQUE = kMaxPasses,
+  CTR_SPLITS = kMaxPasses + 1,
+  CTR_SCAN = kMaxPasses + 2,
+  CTR_LS = kMaxPasses + 4
+};
 
-// Inclusive scan of one value per thread across the block; returns the
-// thread's inclusive prefix and (via *total) the block total.
-__device__ inline uint64_t block_incl_scan64(uint64_t x, uint64_t *s_wsum, uint64_t *total) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint64_t incl = wave_incl_scan_add(x);
-  if (lane == 63) s_wsum[w] = incl;
-  __syncthreads();
-  uint64_t off = 0, tot = 0;
-  for (int j = 0; j < kBlock / 64; ++j) {
-    if (j < w) off += s_wsum[j];
-    tot += s_wsum[j];
-  }
-  *total = tot;
-  return off + incl;
-}
-
-// Called by every lane of wave 0 of the tile's workgroup after the tile's
-// aggregate has been published.  Returns the exclusive prefix of the tile.
-__device__ inline uint64_t lookback_excl(uint64_t *status, int tile) {
-  const int lane = threadIdx.x & 63;
-  uint64_t excl = 0;
-  int base = tile - 1;
-  while (base >= 0) {
-    const int k = base - lane;
-    uint64_t wv = kSIncl;  // before tile 0: an inclusive zero
-    if (k >= 0) {
-      for (;;) {
-        wv = ld_acq_u64(status + k);
-        if (wv & ~kSVal) break;
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    const uint64_t incl = __ballot((wv & ~kSVal) == kSIncl);
-    uint64_t val = wv & kSVal;
-    if (incl) {
-      const int first = __ffsll((long long)incl) - 1;
-      if (lane > first) val = 0;
-      excl += wave_reduce_add(val);
-      break;
-    }
-    excl += wave_reduce_add(val);
-    base -= 64;
-  }
-  return excl;
-}
-
-// Publishes the tile aggregate, resolves the exclusive prefix and publishes
-// the inclusive value.  Must be called by all threads of the block.
-__device__ inline uint64_t tile_prefix(uint64_t *status, int tile, uint64_t total,
-                                       uint64_t *s_excl) {
-  if (threadIdx.x == 0) st_rel_u64(status + tile, (tile == 0 ? kSIncl : kSAgg) | total);
-  if (tile > 0 && threadIdx.x < 64) {
-    const uint64_t e = lookback_excl(status, tile);
-    if (threadIdx.x == 0) {
-      st_rel_u64(status + tile, kSIncl | (e + total));
-      *s_excl = e;
-    }
-  } else if (threadIdx.x == 0) {
-    *s_excl = 0;
-  }
-  __syncthreads();
-  return *s_excl;
+template <typename K> __device__ inline K shfl_up1(K v);
+template <> __device__ inline uint32_t shfl_up1<uint32_t>(uint32_t v) { return __shfl_up(v, 1, 64); }
+template <> __device__ inline uint64_t shfl_up1<uint64_t>(uint64_t v) { return __shfl_up(v, 1, 64); }
+template <> __device__ inline Key128 shfl_up1<Key128>(Key128 v) {
+  return {__shfl_up(v.lo, 1, 64), __shfl_up(v.hi, 1, 64)};
 }
 
 template <typename K, bool LAB>
@@ -97,66 +42,47 @@ __global__ __launch_bounds__(kBlock) void unique_kernel(const K *__restrict__ ke
                                                         K *__restrict__ out,
                                                         uint32_t *__restrict__ conflict_bits,
                                                         uint64_t *lookback, RunState *st) {
-  __shared__ K s_k[kScanTile];
-  __shared__ uint64_t s_wsum[kBlock / 64];
+  __shared__ uint32_t s_w[kRows * 4];
   __shared__ uint64_t s_excl;
   __shared__ uint32_t s_tile;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63;
   if (t == 0) s_tile = atomicAdd(&st->counters[CTR_UNIQUE], 1u);
   __syncthreads();
   const int tile = (int)s_tile;
   const int ntiles = n ? (int)((n + kScanTile - 1) / kScanTile) : 1;
   const uint32_t base = (uint32_t)tile * kScanTile;
+  K k[kRows];
+  bool head[kRows];
 #pragma unroll
-  for (int j = 0; j < kIPT; ++j) {
-    const uint32_t e = base + j * kBlock + t;
-    if (e < n) s_k[j * kBlock + t] = keys[e];
+  for (int r = 0; r < kRows; ++r) {
+    const uint32_t e = base + r * kBlock + t;
+    k[r] = (e < n) ? keys[e] : kzero_value<K>();
+    K prev = shfl_up1(k[r]);
+    if (lane == 0 && e > 0 && e < n) prev = keys[e - 1];
+    head[r] = (e < n) && ((e == 0) ? (!prepend_zero || !kzero(k[r])) : !keq(k[r], prev));
   }
-  __syncthreads();
-
-  // Thread t owns elements [t*16, t*16+16) of the tile.
-  const uint32_t e0 = base + t * kIPT;
-  K prev = kzero_value<K>();
-  if (e0 > 0 && e0 < n) prev = (t > 0) ? s_k[t * kIPT - 1] : keys[e0 - 1];
-  uint32_t heads = 0, cnt = 0;
-#pragma unroll
-  for (int j = 0; j < kIPT; ++j) {
-    const uint32_t e = e0 + j;
-    if (e < n) {
-      const K cur = s_k[t * kIPT + j];
-      const bool h = (e == 0) ? (!prepend_zero || !kzero(cur)) : !keq(cur, prev);
-      heads |= (uint32_t)h << j;
-      cnt += h;
-      prev = cur;
-    }
-  }
-  uint64_t total;
-  const uint64_t incl = block_incl_scan64(cnt, s_wsum, &total);
+  uint32_t ex[kRows];
+  const uint32_t total = striped_flag_scan<kRows>(head, ex, s_w);
   const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl);
-
-  uint32_t pos = (uint32_t)(prepend_zero + excl + incl - cnt);
+  const uint32_t obase = (uint32_t)(prepend_zero + excl);
 #pragma unroll
-  for (int j = 0; j < kIPT; ++j) {
-    const uint32_t e = e0 + j;
-    if (e >= n) continue;
-    if (heads & (1u << j)) {
-      out[pos] = s_k[t * kIPT + j];
-      ++pos;
-    } else if (LAB && e > 0) {
-      // Not a head: same key as element e-1 (the virtual zero has no label).
-      if (labels[e] != labels[e - 1]) {
-        const uint32_t r = pos - 1;
-        atomicOr(&conflict_bits[r >> 5], 1u << (r & 31));
-      }
+  for (int r = 0; r < kRows; ++r) {
+    const uint32_t e = base + r * kBlock + t;
+    if (head[r]) out[obase + ex[r]] = k[r];
+    if (LAB && e < n && e > 0 && !head[r] && labels[e] != labels[e - 1]) {
+      // not a head: same key as e-1, in run (heads up to e) - 1; the
+      // virtual zero key carries no label, so e = 0 never compares.
+      const uint32_t run = obase + ex[r] - 1;
+      atomicOr(&conflict_bits[run >> 5], 1u << (run & 31));
     }
   }
   if (tile == 0 && t == 0 && prepend_zero) out[0] = kzero_value<K>();
   if (tile == ntiles - 1 && t == 0) st->m = (uint32_t)(prepend_zero + excl + total);
 }
 
-// Local split of BRT node c (C/BuildOctree.c:17-40 / opencl/Kernels.cpp:303-313),
-// computed from the node's own parent claim instead of being written by the
-// parent.  parent[c] holds claimer+1 (0 = unclaimed).
+// Local split of BRT node c computed from its own parent claim
+// (parent[c] = claimer + 1, 0 = unclaimed): C/BuildOctree.c:17-40 lets the
+// last claimer write the child's entry; the claim array holds that claimer.
 template <typename K>
 __device__ inline uint32_t local_split(const BrtNodeT<K> *brt, const int32_t *parent, uint32_t c,
                                        uint32_t m, int dim) {
@@ -172,60 +98,57 @@ __global__ __launch_bounds__(kBlock) void splits_scan_kernel(const BrtNodeT<K> *
                                                              int32_t *__restrict__ parent,
                                                              uint32_t *__restrict__ ls_out,
                                                              uint32_t *__restrict__ ps, int dim,
+                                                             int only_if_malformed,
                                                              uint64_t *lookback, RunState *st) {
-  __shared__ uint64_t s_wsum[kBlock / 64];
+  if (only_if_malformed && !st->malformed) return;
+  __shared__ uint32_t s_w[kRows * 4];
   __shared__ uint64_t s_excl;
   __shared__ uint32_t s_tile;
   const int t = threadIdx.x;
   const uint32_t m = st->m;
   const int ntiles = (int)((m + kScanTile - 1) / kScanTile);
-  if (t == 0) s_tile = atomicAdd(&st->counters[CTR_SPLITS], 1u);
-  __syncthreads();
-  const int tile = (int)s_tile;
-  if (tile >= ntiles) return;
-  const uint32_t e0 = (uint32_t)tile * kScanTile + t * kIPT;
-  uint32_t v[kIPT];
-  uint32_t sum = 0;
+  for (;;) {
+    __syncthreads();   // s_tile / s_w reuse across loop iterations
+    if (t == 0) s_tile = atomicAdd(&st->counters[CTR_SPLITS], 1u);
+    __syncthreads();
+    const int tile = (int)s_tile;
+    if (tile >= ntiles) return;
+    const uint32_t base = (uint32_t)tile * kScanTile;
+    uint32_t v[kRows], ex[kRows];
 #pragma unroll
-  for (int j = 0; j < kIPT; ++j) {
-    const uint32_t c = e0 + j;
-    v[j] = (c < m) ? local_split(brt, parent, c, m, dim) : 0u;
-    sum += v[j];
-  }
-  uint64_t total;
-  const uint64_t incl = block_incl_scan64(sum, s_wsum, &total);
-  const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl);
-  uint32_t run = (uint32_t)(excl + incl - sum);
-#pragma unroll
-  for (int j = 0; j < kIPT; ++j) {
-    const uint32_t c = e0 + j;
-    if (c < m) {
-      run += v[j];
-      ps[c] = run;
-      if (ls_out) ls_out[c] = v[j];
+    for (int r = 0; r < kRows; ++r) {
+      const uint32_t c = base + r * kBlock + t;
+      v[r] = (c < m) ? local_split(brt, parent, c, m, dim) : 0u;
     }
-  }
-  // Reference form of the parent links: root -1, unclaimed 0
-  // (opencl/Kernels.cpp:405 zero-initialises the BRT), else the claimer.
-  // Every local_split read of parent[] in this tile happened above the
-  // block barrier inside tile_prefix, and other tiles read only their own
-  // elements.
+    const uint32_t total = striped_excl_scan<kRows, uint32_t>(v, ex, s_w);
+    const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl);
 #pragma unroll
-  for (int j = 0; j < kIPT; ++j) {
-    const uint32_t c = e0 + j;
-    if (c + 1 < m) {
-      const int32_t pc = parent[c];
-      parent[c] = (c == 0) ? -1 : (pc ? pc - 1 : 0);
+    for (int r = 0; r < kRows; ++r) {
+      const uint32_t c = base + r * kBlock + t;
+      if (c < m) {
+        ps[c] = (uint32_t)(excl + ex[r] + v[r]);
+        if (ls_out) ls_out[c] = v[r];
+      }
     }
+    // Reference form of the parent links: root -1, unclaimed 0
+    // (opencl/Kernels.cpp:405 zero-initialises the BRT), else the claimer.
+    // local_split above read parent[c] for this thread's own c only.
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const uint32_t c = base + r * kBlock + t;
+      if (c + 1 < m) {
+        const int32_t pc = parent[c];
+        parent[c] = (c == 0) ? -1 : (pc ? pc - 1 : 0);
+      }
+    }
+    if (tile == ntiles - 1 && t == 0) st->octree_size = (uint32_t)(excl + total);
   }
-  if (tile == ntiles - 1 && t == 0) st->octree_size = (uint32_t)(excl + total);
 }
 
 __global__ __launch_bounds__(kBlock) void scan_u32_kernel(const uint32_t *__restrict__ in,
                                                           uint32_t *__restrict__ out, uint32_t n,
                                                           uint64_t *lookback, RunState *st) {
-  __shared__ uint32_t s_v[kScanTile];
-  __shared__ uint64_t s_wsum[kBlock / 64];
+  __shared__ uint64_t s_w[kRows * 4];
   __shared__ uint64_t s_excl;
   __shared__ uint32_t s_tile;
   const int t = threadIdx.x;
@@ -233,31 +156,60 @@ __global__ __launch_bounds__(kBlock) void scan_u32_kernel(const uint32_t *__rest
   __syncthreads();
   const int tile = (int)s_tile;
   const uint32_t base = (uint32_t)tile * kScanTile;
+  uint64_t v[kRows], ex[kRows];
 #pragma unroll
-  for (int j = 0; j < kIPT; ++j) {
-    const uint32_t e = base + j * kBlock + t;
-    s_v[j * kBlock + t] = (e < n) ? in[e] : 0u;
+  for (int r = 0; r < kRows; ++r) {
+    const uint32_t e = base + r * kBlock + t;
+    v[r] = (e < n) ? in[e] : 0u;
   }
-  __syncthreads();
-  uint32_t v[kIPT];
-  uint64_t sum = 0;
-#pragma unroll
-  for (int j = 0; j < kIPT; ++j) { v[j] = s_v[t * kIPT + j]; sum += v[j]; }
-  uint64_t total;
-  const uint64_t incl = block_incl_scan64(sum, s_wsum, &total);
+  const uint64_t total = striped_excl_scan<kRows, uint64_t>(v, ex, s_w);
   const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl);
-  uint32_t run = (uint32_t)(excl + incl - sum);
 #pragma unroll
-  for (int j = 0; j < kIPT; ++j) { run += v[j]; s_v[t * kIPT + j] = run; }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < kIPT; ++j) {
-    const uint32_t e = base + j * kBlock + t;
-    if (e < n) out[e] = s_v[j * kBlock + t];
+  for (int r = 0; r < kRows; ++r) {
+    const uint32_t e = base + r * kBlock + t;
+    if (e < n) out[e] = (uint32_t)(excl + ex[r] + v[r]);
   }
 }
 
+// Inclusive scan of the local splits ls[0..m), m = st->m, into ps; the total
+// is the octree size (opencl/Kernels.cpp:376-378).
+__global__ __launch_bounds__(kBlock) void ls_scan_kernel(const uint32_t *__restrict__ ls,
+                                                         uint32_t *__restrict__ ps,
+                                                         uint64_t *lookback, RunState *st) {
+  __shared__ uint32_t s_w[kRows * 4];
+  __shared__ uint64_t s_excl;
+  __shared__ uint32_t s_tile;
+  const int t = threadIdx.x;
+  const uint32_t m = st->m;
+  const int ntiles = (int)((m + kScanTile - 1) / kScanTile);
+  if (t == 0) s_tile = atomicAdd(&st->counters[CTR_LS], 1u);
+  __syncthreads();
+  const int tile = (int)s_tile;
+  if (tile >= ntiles) return;
+  const uint32_t base = (uint32_t)tile * kScanTile;
+  uint32_t v[kRows], ex[kRows];
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const uint32_t e = base + r * kBlock + t;
+    v[r] = (e < m) ? ls[e] : 0u;
+  }
+  const uint32_t total = striped_excl_scan<kRows, uint32_t>(v, ex, s_w);
+  const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl);
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const uint32_t e = base + r * kBlock + t;
+    if (e < m) ps[e] = (uint32_t)(excl + ex[r] + v[r]);
+  }
+  if (tile == ntiles - 1 && t == 0) st->octree_size = (uint32_t)(excl + total);
+}
+
 }  // namespace
+
+void launch_ls_scan(const uint32_t *ls, uint32_t *ps, int64_t max_m, uint64_t *lookback,
+                    RunState *st, hipStream_t s) {
+  const int64_t tiles = scan_tiles(max_m > 0 ? max_m : 1);
+  ls_scan_kernel<<<(unsigned)tiles, kBlock, 0, s>>>(ls, ps, lookback, st);
+}
 
 template <typename K>
 void launch_unique(const K *keys, const uint32_t *labels, int64_t n, int prepend_zero, K *out,
@@ -275,10 +227,12 @@ void launch_unique(const K *keys, const uint32_t *labels, int64_t n, int prepend
 
 template <typename K>
 void launch_splits_scan(const BrtNodeT<K> *brt, int32_t *parent, uint32_t *ls, uint32_t *ps,
-                        int64_t max_m, int dim, uint64_t *lookback, RunState *st,
-                        hipStream_t s) {
-  const int64_t tiles = scan_tiles(max_m > 0 ? max_m : 1);
-  splits_scan_kernel<K><<<(unsigned)tiles, kBlock, 0, s>>>(brt, parent, ls, ps, dim, lookback, st);
+                        int64_t max_m, int dim, int only_if_malformed, uint64_t *lookback,
+                        RunState *st, hipStream_t s) {
+  int64_t grid = scan_tiles(max_m > 0 ? max_m : 1);
+  if (grid > 2048) grid = 2048;      // persistent: each workgroup loops over tiles
+  splits_scan_kernel<K><<<(unsigned)grid, kBlock, 0, s>>>(brt, parent, ls, ps, dim,
+                                                          only_if_malformed, lookback, st);
 }
 
 void launch_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint64_t *lookback,
@@ -291,7 +245,7 @@ void launch_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint64_t *loo
   template void launch_unique<K>(const K *, const uint32_t *, int64_t, int, K *, uint32_t *,   \
                                  uint64_t *, RunState *, hipStream_t);                         \
   template void launch_splits_scan<K>(const BrtNodeT<K> *, int32_t *, uint32_t *, uint32_t *,  \
-                                      int64_t, int, uint64_t *, RunState *, hipStream_t);
+                                      int64_t, int, int, uint64_t *, RunState *, hipStream_t);
 PGVD_INST(uint32_t)
 PGVD_INST(uint64_t)
 PGVD_INST(Key128)

@@ -110,16 +110,32 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   s_dstart[t] = dstart;
   const uint32_t gexcl = block_excl_scan(ghist[t], s_wsum_b);
 
+  // Look-back for digit t: read up to kBatch predecessor tiles per round
+  // trip, consume them nearest first until an inclusive prefix; a tile that
+  // has not published yet is re-read on the next round.
   uint32_t excl = 0;
   if (tile > 0) {
+    constexpr int kBatch = 8;
     int kk = (int)tile - 1;
     for (;;) {
-      const uint32_t wv = ld_acq_u32(lookback + (size_t)kk * 256 + t);
-      const uint32_t f = wv & ~kValMask;
-      if (f == 0) { __builtin_amdgcn_s_sleep(1); continue; }
-      excl += wv & kValMask;
-      if (f == kFlagIncl) break;
-      --kk;
+      uint32_t wv[kBatch];
+#pragma unroll
+      for (int b = 0; b < kBatch; ++b)
+        wv[b] = (kk - b >= 0) ? ld_acq_u32(lookback + (size_t)(kk - b) * 256 + t) : kFlagIncl;
+      int used = 0;
+      bool done = false;
+#pragma unroll
+      for (int b = 0; b < kBatch; ++b) {
+        if (done || used != b) continue;
+        const uint32_t f = wv[b] & ~kValMask;
+        if (f == 0) continue;            // not published: stop consuming here
+        excl += wv[b] & kValMask;
+        used = b + 1;
+        done = (f == kFlagIncl);
+      }
+      if (done) break;
+      kk -= used;
+      if (used < kBatch) __builtin_amdgcn_s_sleep(1);
     }
     st_rel_u32(lb + t, kFlagIncl | (excl + total));
   }

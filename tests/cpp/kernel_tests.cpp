@@ -1,0 +1,265 @@
+// kernel_tests.cpp -- the reference's stage tests (tests/KernelTests.cpp)
+// restated against the C++ host layer (include/pgvd_kernels.hpp).  Where
+// the reference compares each `_p` stage with its serial `_s` twin, this
+// compares with the oracle (oracle/pgvd_oracle.h).  Inputs are seeded
+// (the reference used srand(time(NULL))).  Prints one line per scenario and
+// exits non-zero on the first failure.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+#include "pgvd_kernels.hpp"
+#include "pgvd_oracle.h"
+
+using namespace pgvd;
+
+static int failures = 0;
+#define CHECK(cond, what)                                                   \
+  do {                                                                      \
+    if (!(cond)) {                                                          \
+      std::printf("FAIL %s (%s:%d)\n", what, __FILE__, __LINE__);           \
+      ++failures;                                                           \
+      return;                                                               \
+    }                                                                       \
+  } while (0)
+
+static const int kBits = 30;          // tests/KernelTests.cpp:8
+static const int kMbits = kBits * 2;  // :9 (DIM 2)
+
+// "Points can be mapped to a Z-Order curve" (:47-92): (i, i), i < 10^6.
+static void morton_scenario() {
+  const int n = 1000000;
+  std::vector<int2> pts(n);
+  for (int i = 0; i < n; ++i) pts[i] = {i, i};
+  Buffer p, z;
+  CHECK(Kernels::UploadPoints(pts, p) == 0, "upload");
+  CHECK(Kernels::PointsToMorton_p(p, z, n, kBits) == 0, "PointsToMorton_p");
+  const int g = Kernels::nextPow2(n);
+  std::vector<uint64_t> lo, hi;
+  CHECK(Kernels::ReadKeys(z, g, lo, hi) == 0, "read keys");
+  for (int i = 0; i < g; ++i) {
+    uint64_t elo = 0, ehi = 0;
+    if (i < n) {
+      const int64_t c[2] = {i, i};
+      oracle_xyz2z(c, 2, kBits, &elo, &ehi);
+    }
+    CHECK(lo[i] == elo && hi[i] == ehi, "Morton keys equal the serial xyz2z (padding zero)");
+  }
+  std::printf("PASS Points can be mapped to a Z-Order curve\n");
+}
+
+static std::vector<uint64_t> random_keys(int n, int mbits, unsigned seed) {
+  std::mt19937_64 rng(seed);
+  std::vector<uint64_t> k(n);
+  for (auto &x : k) x = rng() & ((mbits >= 64) ? ~0ull : ((1ull << mbits) - 1));
+  return k;
+}
+
+static Buffer keys_buffer(const std::vector<uint64_t> &keys, int global) {
+  Context &c = Context::Default();
+  std::vector<uint64_t> host(keys);
+  host.resize(global, 0);                 // zero-key padding
+  Buffer b = c.alloc(sizeof(uint64_t) * global);
+  c.upload(b, host.data(), sizeof(uint64_t) * global);
+  b.key_bytes = 8;
+  b.dim = 2;
+  return b;
+}
+
+// "Big Unsigneds can be sorted using a parallel radix sort" (:96-146)
+static void sort_scenario() {
+  const int n = 1000000, g = Kernels::nextPow2(n);
+  auto keys = random_keys(n, kMbits, 1);
+  Buffer b = keys_buffer(keys, g);
+  CHECK(Kernels::RadixSortBigUnsigned(b, n, 88) == 0, "RadixSortBigUnsigned");
+  std::vector<uint64_t> lo, hi;
+  Kernels::ReadKeys(b, g, lo, hi);
+  keys.resize(g, 0);
+  std::sort(keys.begin(), keys.end());
+  CHECK(lo == keys, "sorted keys equal std::sort");
+  std::printf("PASS Big Unsigneds can be sorted using a parallel radix sort\n");
+}
+
+// The reference's own bit-serial sort (opencl/Kernels.cpp:219-247) driven
+// through the elementary stages: BitPredicate, StreamScan_p, DoubleCompact.
+static void bit_serial_scenario() {
+  const int n = 50000, g = Kernels::nextPow2(n), mbits = 20;
+  auto keys = random_keys(n, mbits, 2);
+  Buffer in = keys_buffer(keys, g), pred, addr, tmp;
+  for (unsigned index = 0; index < (unsigned)mbits; ++index) {
+    CHECK(Kernels::BitPredicate(in, pred, index, 0, g) == 0, "BitPredicate");
+    CHECK(Kernels::StreamScan_p(pred, addr, g) == 0, "StreamScan_p");
+    CHECK(Kernels::DoubleCompact(in, tmp, pred, addr, g) == 0, "DoubleCompact");
+    std::swap(in, tmp);
+  }
+  std::vector<uint64_t> lo, hi;
+  Kernels::ReadKeys(in, g, lo, hi);
+  keys.resize(g, 0);
+  std::sort(keys.begin(), keys.end());
+  CHECK(lo == keys, "bit-serial split sort equals std::sort");
+  // UniquePredicate + StreamScan_p + SingleCompact = UniqueSorted's steps
+  Buffer up, ua, uo;
+  CHECK(Kernels::UniquePredicate(in, up, g) == 0, "UniquePredicate");
+  CHECK(Kernels::StreamScan_p(up, ua, g) == 0, "StreamScan_p");
+  CHECK(Kernels::SingleCompact(in, uo, up, ua, g) == 0, "SingleCompact");
+  const int m = (int)(std::unique(keys.begin(), keys.end()) - keys.begin());
+  Kernels::ReadKeys(uo, m, lo, hi);
+  CHECK(std::equal(lo.begin(), lo.end(), keys.begin()), "single compaction equals std::unique");
+  std::printf("PASS the bit-serial split sort and compaction stages\n");
+}
+
+// "Sorted BigUnsigneds can be unique'd in parallel" (:148-207)
+static void unique_scenario() {
+  const int n = 1000000, g = Kernels::nextPow2(n);
+  auto keys = random_keys(n, 24, 3);        // many duplicates
+  keys.resize(g, 0);
+  std::sort(keys.begin(), keys.end());
+  Buffer b = keys_buffer(keys, g);
+  int size = n;
+  CHECK(Kernels::UniqueSorted(b, size) == 0, "UniqueSorted");
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  CHECK(size == (int)keys.size(), "unique count");
+  std::vector<uint64_t> lo, hi;
+  Kernels::ReadKeys(b, size, lo, hi);
+  CHECK(lo == keys, "unique keys equal std::unique");
+  std::printf("PASS Sorted BigUnsigneds can be unique'd in parallel\n");
+}
+
+// "binary radix tree" (:209-277), "local splits" (:279-315), "octree" (:317-350)
+static void tree_scenarios() {
+  auto keys = random_keys(1000000, kMbits, 4);
+  keys.push_back(0);
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  const int m = (int)keys.size();
+  Buffer z = keys_buffer(keys, m), I, ls;
+  CHECK(Kernels::BuildBinaryRadixTree_p(z, I, m, kMbits) == 0, "BuildBinaryRadixTree_p");
+  oracle_tree *t = oracle_build_from_keys(keys.data(), nullptr, m, 2, kMbits);
+  std::vector<int32_t> left(m - 1), len(m - 1), par(m - 1);
+  std::vector<uint8_t> ll(m - 1), rl(m - 1);
+  std::vector<uint64_t> lcp_lo(m - 1), lcp_hi(m - 1);
+  oracle_tree_brt(t, left.data(), ll.data(), rl.data(), lcp_lo.data(), lcp_hi.data(), len.data(), par.data());
+  std::vector<BrtNode> nodes;
+  Kernels::ReadBrt(I, m, nodes);
+  bool ok = true;
+  for (int i = 0; i < m - 1 && ok; ++i)
+    ok = nodes[i].left == left[i] && nodes[i].left_leaf == (bool)ll[i] &&
+         nodes[i].right_leaf == (bool)rl[i] && nodes[i].lcp_lo == lcp_lo[i] &&
+         nodes[i].lcp_hi == lcp_hi[i] && nodes[i].lcp_length == len[i] && nodes[i].parent == par[i];
+  CHECK(ok, "BRT nodes equal the serial BRT");
+  std::printf("PASS Sorted Z-Order numbers can be used to construct a binary radix tree\n");
+
+  CHECK(Kernels::ComputeLocalSplits_p(I, ls, m) == 0, "ComputeLocalSplits_p");
+  std::vector<uint32_t> gls(m), ols(m), ops(m);
+  Context::Default().download(gls.data(), ls, sizeof(uint32_t) * m);
+  oracle_tree_splits(t, ols.data(), ops.data());
+  CHECK(gls == ols, "local splits equal the serial local splits");
+  std::printf("PASS A binary radix tree can be used to calculate a vector of local splits\n");
+
+  std::vector<OctNode2> oct;
+  CHECK(Kernels::BinaryRadixToOctree_p(I, oct, m) == 0, "BinaryRadixToOctree_p");
+  const int64_t O = oracle_tree_octree_size(t);
+  CHECK((int64_t)oct.size() == O, "octree size");
+  std::vector<int32_t> ch(O * 4), lf(O);
+  oracle_tree_octree(t, ch.data(), lf.data());
+  for (int64_t i = 0; i < O && ok; ++i) {
+    for (int c = 0; c < 4; ++c) ok &= oct[i].children[c] == ch[i * 4 + c];
+    ok &= oct[i].leaf == lf[i];
+  }
+  CHECK(ok, "octree equals the serial octree");
+  oracle_tree_free(t);
+  std::printf("PASS A binary radix tree can be used to construct a Octree/Quadtree\n");
+}
+
+static bool same_octree(const std::vector<OctNode2> &a, oracle_tree *t) {
+  const int64_t O = oracle_tree_octree_size(t);
+  if ((int64_t)a.size() != O) return false;
+  std::vector<int32_t> ch(O * 4), lf(O);
+  oracle_tree_octree(t, ch.data(), lf.data());
+  for (int64_t i = 0; i < O; ++i) {
+    for (int c = 0; c < 4; ++c)
+      if (a[i].children[c] != ch[i * 4 + c]) return false;
+    if (a[i].leaf != lf[i]) return false;
+  }
+  return true;
+}
+
+// "An octree can be build using a bunch of points" (:352-446) and the
+// stress test (:448-490), which in the reference compared an empty octree.
+static void build_scenarios() {
+  std::mt19937 rng(5);
+  for (int round = 0; round < 60; ++round) {
+    const int n = round == 0 ? 1000000 : 1000 - round * 7;
+    std::vector<int2> pts(n);
+    std::vector<int64_t> q(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+      pts[i] = {(int32_t)(rng() & 0x7fffffff), (int32_t)(rng() & 0x7fffffff)};
+      q[2 * i] = pts[i].x;
+      q[2 * i + 1] = pts[i].y;
+    }
+    std::vector<OctNode2> oct;
+    CHECK(Kernels::BuildOctree_p(pts, oct, kBits, kMbits) == 0, "BuildOctree_p");
+    oracle_tree *t = oracle_build(q.data(), n, 2, kBits, nullptr);
+    const bool ok = same_octree(oct, t);
+    oracle_tree_free(t);
+    CHECK(ok, "octree equals BuildOctree_s");
+  }
+  std::printf("PASS An octree can be build using a bunch of points / stress test\n");
+  bool threw = false;
+  try {
+    std::vector<OctNode2> oct;
+    Kernels::BuildOctree_p(std::vector<int2>(), oct, kBits, kMbits);
+  } catch (const std::logic_error &) {
+    threw = true;
+  }
+  CHECK(threw, "zero points throw logic_error");
+  std::printf("PASS Zero points are rejected like the reference\n");
+}
+
+// Karras facade: float points -> Quantize -> BuildOctreeInParallel.
+static void karras_scenario() {
+  std::mt19937 rng(6);
+  std::uniform_real_distribution<float> u(-1.f, 1.f);
+  const int n = 100000;
+  std::vector<float3> pts(n);
+  std::vector<float> flat(3 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    pts[i] = {u(rng), u(rng), u(rng)};
+    flat[3 * i] = pts[i].x; flat[3 * i + 1] = pts[i].y; flat[3 * i + 2] = pts[i].z;
+  }
+  const Resln r = make_resln(1 << 21, 3);
+  auto q = Karras::Quantize(pts, r);
+  std::vector<int64_t> oq(3 * (size_t)n);
+  CHECK(oracle_quantize(flat.data(), n, 3, r.width, 0, nullptr, nullptr, 0, oq.data()) == n, "oracle quantize");
+  bool ok = (int)q.size() == n;
+  for (int i = 0; i < n && ok; ++i) ok = q[i].x == oq[3 * i] && q[i].y == oq[3 * i + 1] && q[i].z == oq[3 * i + 2];
+  CHECK(ok, "Quantize equals the serial quantize");
+  auto oct = Karras::BuildOctreeInParallel(q, r);
+  auto oct2 = Karras::BuildOctreeFromPoints(pts, r);
+  oracle_tree *t = oracle_build(oq.data(), n, 3, r.bits, nullptr);
+  const int64_t O = oracle_tree_octree_size(t);
+  std::vector<int32_t> ch(O * 8), lf(O);
+  oracle_tree_octree(t, ch.data(), lf.data());
+  ok = (int64_t)oct.size() == O && (int64_t)oct2.size() == O;
+  for (int64_t i = 0; i < O && ok; ++i) {
+    for (int c = 0; c < 8; ++c) ok &= oct[i].children[c] == ch[i * 8 + c] && oct2[i].children[c] == ch[i * 8 + c];
+    ok &= oct[i].leaf == lf[i] && oct2[i].leaf == lf[i];
+  }
+  oracle_tree_free(t);
+  CHECK(ok, "3D octree equals the serial octree");
+  std::printf("PASS Karras facade (Quantize, BuildOctreeInParallel) in 3D\n");
+}
+
+int main() {
+  morton_scenario();
+  sort_scenario();
+  bit_serial_scenario();
+  unique_scenario();
+  tree_scenarios();
+  build_scenarios();
+  karras_scenario();
+  std::printf(failures ? "FAILED %d\n" : "ALL PASSED\n", failures);
+  return failures ? 1 : 0;
+}

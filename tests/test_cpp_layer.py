@@ -1,0 +1,40 @@
+"""The C++ host layer (include/pgvd_kernels.hpp): it compiles against the
+installed headers, and on the GPU its stage tests -- the reference's
+tests/KernelTests.cpp scenarios restated against the oracle -- pass."""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(HERE, "cpp", "kernel_tests")
+
+
+def test_cpp_layer_compiles():
+    from tests.cpp.build_tests import build
+    assert os.path.exists(build())
+
+
+def test_cpp_layer_symbols_exported():
+    import pgvd_amd
+    out = subprocess.run(["nm", "-DC", "--defined-only", pgvd_amd.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    for name in ["pgvd::Kernels::BuildOctree_p", "pgvd::Kernels::RadixSortBigUnsigned",
+                 "pgvd::Kernels::UniqueSorted", "pgvd::Kernels::BuildBinaryRadixTree_p",
+                 "pgvd::Kernels::ComputeLocalSplits_p", "pgvd::Kernels::BinaryRadixToOctree_p",
+                 "pgvd::Kernels::StreamScan_p", "pgvd::Kernels::PointsToMorton_p",
+                 "pgvd::Kernels::BitPredicate", "pgvd::Kernels::DoubleCompact",
+                 "pgvd::Karras::Quantize", "pgvd::Karras::BuildOctreeInParallel"]:
+        assert name in out, name
+
+
+@pytest.mark.gpu
+def test_cpp_kernel_scenarios():
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cpp/kernel_tests was not built (run __graft_entry__.build())")
+    p = subprocess.run([BIN], capture_output=True, text=True, timeout=110)
+    print(p.stdout)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "ALL PASSED" in p.stdout
