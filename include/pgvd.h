@@ -63,6 +63,11 @@ int pgvd_memcpy_dtoh(pgvd_ctx *ctx, void *dst, const void *src, size_t bytes, pg
 /* Device-to-device copy through a plain copy kernel (also the calibration
  * kernel for HBM byte counters). */
 int pgvd_copy_kernel(pgvd_ctx *ctx, void *dst, const void *src, size_t bytes, pgvd_stream s);
+/* Position-dependent 64-bit digest of a device range (bytes a multiple of 4,
+ * 4-byte aligned): sum over its 32-bit words w_i of splitmix64(w_i | i<<32).
+ * Waits for the result.  Compares outputs on the device without a download;
+ * no counterpart in the reference. */
+int pgvd_digest(pgvd_ctx *ctx, const void *p, size_t bytes, pgvd_stream s, uint64_t *out);
 
 /* Kernel timing with HIP events on the launch stream.  on = 1 records an
  * event pair around every stage ("prep", "bbox", "encode", each
@@ -226,6 +231,60 @@ int pgvd_build_octree(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bi
  * refills it if it was too small, and fills *out.  Errors the device
  * detected (PGVD_ERR_BAD_COORD, PGVD_ERR_DUP_KEY) are returned here. */
 int pgvd_build_octree_finish(pgvd_ctx *ctx, pgvd_octree_result *out);
+
+/* ---- batched form -------------------------------------------------------
+ * Many independent octree builds of uneven size through one segmented
+ * pipeline: one launch per stage for the whole batch, replayed as a HIP
+ * graph.  The reference has no counterpart: its callers loop over
+ * Kernels::BuildOctree_p (opencl/Kernels.cpp:413-429) once per point set.
+ * Every instance's results are bit-identical to pgvd_build_octree on that
+ * instance alone.  Instances hold 1 .. 2^23 points; all share dim and bits;
+ * labels are not carried (use pgvd_build_octree for conflict cells). */
+typedef struct pgvd_batch pgvd_batch;
+
+typedef struct pgvd_batch_opts {
+  int quantized_input;  /* points are int32 grid coordinates (no bbox)          */
+  int clamped;          /* quantize to width instead of width-1 (Karras.cpp:71) */
+  int use_graph;        /* replay the pipeline as a captured HIP graph (default) */
+  int sort_ipt;         /* keys per thread of the sort tile (0: default)        */
+  int brt_rows;         /* BRT nodes per thread of a workgroup window (0: def.) */
+} pgvd_batch_opts;
+
+typedef struct pgvd_batch_result {
+  int64_t count;                 /* instances                                    */
+  int dim, bits, mbits, key_bytes, octree_stride;
+  int64_t total_octree;          /* nodes over all instances                     */
+  /* host arrays owned by the plan, count entries each:                        */
+  const int64_t *m;              /* unique keys per instance (incl. padding)    */
+  const int64_t *octree_size;    /* octree nodes per instance                   */
+  const int64_t *key_offset;     /* first key slot of each instance             */
+  const int64_t *octree_offset;  /* first octree node of each instance          */
+  /* device arrays owned by the plan, valid until its next run:                */
+  const void *keys;              /* instance i: keys[key_offset[i] .. +m[i])     */
+  const void *brt;               /* same slots, m[i]-1 nodes                     */
+  const int32_t *parent;         /* same slots                                   */
+  const uint32_t *prefix_sums;   /* same slots, m[i] entries                     */
+  const int32_t *octree;         /* instance i: nodes octree_offset[i] .. with   */
+                                 /* child indices local to the instance          */
+} pgvd_batch_result;
+
+/* Plans a batch of `count` instances with sizes[i] points each: builds the
+ * tile tables, allocates every device buffer once and (with use_graph)
+ * captures the pipeline on the first run.  opts may be NULL. */
+int pgvd_batch_create(pgvd_ctx *ctx, const int64_t *sizes, int64_t count, int dim, int bits,
+                      const pgvd_batch_opts *opts, pgvd_batch **out);
+int pgvd_batch_destroy(pgvd_batch *b);
+/* Enqueues the pipeline for points pts (device; the instances' points one
+ * after another, float or int32 per the plan) on the plan's stream. */
+int pgvd_batch_run(pgvd_batch *b, const void *pts);
+/* Waits for the last run, grows the octree buffer and refills it if it was
+ * too small, and fills *out. */
+int pgvd_batch_finish(pgvd_batch *b, pgvd_batch_result *out);
+/* Times every supported (sort_ipt, brt_rows) pair on pts, keeps the fastest
+ * for this plan and reports it.  Either output may be NULL. */
+int pgvd_batch_tune(pgvd_batch *b, const void *pts, int *sort_ipt, int *brt_rows);
+/* Milliseconds of the last pgvd_batch_run, from events around it. */
+int pgvd_batch_last_ms(pgvd_batch *b, float *ms);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,22 @@ class BuildOpts(C.Structure):
     ]
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("quantized_input", C.c_int), ("clamped", C.c_int), ("use_graph", C.c_int),
+                ("sort_ipt", C.c_int), ("brt_rows", C.c_int)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [
+        ("count", C.c_int64), ("dim", C.c_int), ("bits", C.c_int), ("mbits", C.c_int),
+        ("key_bytes", C.c_int), ("octree_stride", C.c_int), ("total_octree", C.c_int64),
+        ("m", C.POINTER(C.c_int64)), ("octree_size", C.POINTER(C.c_int64)),
+        ("key_offset", C.POINTER(C.c_int64)), ("octree_offset", C.POINTER(C.c_int64)),
+        ("keys", C.c_void_p), ("brt", C.c_void_p), ("parent", C.c_void_p),
+        ("prefix_sums", C.c_void_p), ("octree", C.c_void_p),
+    ]
+
+
 def lib():
     """Load libpgvd.so; raises if it has not been built."""
     global _LIB
@@ -88,6 +104,7 @@ def lib():
             "pgvd_memcpy_htod": ([vp, vp, vp, C.c_size_t, vp], i32),
             "pgvd_memcpy_dtoh": ([vp, vp, vp, C.c_size_t, vp], i32),
             "pgvd_copy_kernel": ([vp, vp, vp, C.c_size_t, vp], i32),
+            "pgvd_digest": ([vp, vp, C.c_size_t, vp, C.POINTER(C.c_uint64)], i32),
             "pgvd_timing_enable": ([vp, i32], i32),
             "pgvd_timing_reset": ([vp], i32),
             "pgvd_timing_read": ([vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(i64)], i32),
@@ -102,6 +119,13 @@ def lib():
             "pgvd_brt_to_octree": ([vp, vp, vp, i64, i32, i32, vp, i64, C.POINTER(i64), vp], i32),
             "pgvd_build_octree": ([vp, vp, i64, i32, i32, vp, C.POINTER(BuildOpts), vp], i32),
             "pgvd_build_octree_finish": ([vp, C.POINTER(OctreeResult)], i32),
+            "pgvd_batch_create": ([vp, C.POINTER(i64), i64, i32, i32, C.POINTER(BatchOpts),
+                                   C.POINTER(vp)], i32),
+            "pgvd_batch_destroy": ([vp], i32),
+            "pgvd_batch_run": ([vp, vp], i32),
+            "pgvd_batch_finish": ([vp, C.POINTER(BatchResult)], i32),
+            "pgvd_batch_tune": ([vp, vp, C.POINTER(i32), C.POINTER(i32)], i32),
+            "pgvd_batch_last_ms": ([vp, C.POINTER(C.c_float)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -228,6 +252,13 @@ class Device:
 
     def synchronize(self):
         self._check(lib().pgvd_synchronize(self.ctx, None))
+
+    def digest(self, ptr, nbytes: int) -> int:
+        """Position-dependent 64-bit digest of a device range (pgvd_digest)."""
+        p = ptr.ptr if isinstance(ptr, DeviceBuffer) else int(ptr or 0)
+        out = C.c_uint64()
+        self._check(lib().pgvd_digest(self.ctx, p, int(nbytes), None, C.byref(out)))
+        return out.value
 
     def copy_kernel(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int):
         self._check(lib().pgvd_copy_kernel(self.ctx, dst.ptr, src.ptr, nbytes, None))
@@ -448,3 +479,95 @@ class Device:
             dp.free()
             if dl:
                 dl.free()
+
+
+class Batch:
+    """A batched-form plan (pgvd_batch_*): many independent point sets of
+    uneven size through one segmented pipeline on one Device."""
+
+    def __init__(self, dev: "Device", sizes, dim: int, bits: int, quantized: bool = False,
+                 clamped: bool = False, use_graph: bool = True, sort_ipt: int = 0,
+                 brt_rows: int = 0):
+        self.dev = dev
+        self.sizes = np.ascontiguousarray(sizes, np.int64)
+        opts = BatchOpts(int(quantized), int(clamped), int(use_graph), int(sort_ipt),
+                         int(brt_rows))
+        h = C.c_void_p()
+        dev._check(lib().pgvd_batch_create(
+            dev.ctx, self.sizes.ctypes.data_as(C.POINTER(C.c_int64)), self.sizes.size, dim, bits,
+            C.byref(opts), C.byref(h)))
+        self.h = h
+        self.dim, self.bits = dim, bits
+
+    def close(self):
+        if self.h:
+            lib().pgvd_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def run(self, pts):
+        p = pts.ptr if isinstance(pts, DeviceBuffer) else int(pts)
+        self.dev._check(lib().pgvd_batch_run(self.h, p))
+
+    def finish(self) -> BatchResult:
+        r = BatchResult()
+        self.dev._check(lib().pgvd_batch_finish(self.h, C.byref(r)))
+        return r
+
+    def last_ms(self) -> float:
+        ms = C.c_float()
+        self.dev._check(lib().pgvd_batch_last_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def tune(self, pts) -> tuple[int, int]:
+        p = pts.ptr if isinstance(pts, DeviceBuffer) else int(pts)
+        a, b = C.c_int(), C.c_int()
+        self.dev._check(lib().pgvd_batch_tune(self.h, p, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    @staticmethod
+    def counts(r: BatchResult) -> dict:
+        n = r.count
+        return dict(m=np.ctypeslib.as_array(r.m, (n,)).copy(),
+                    octree_size=np.ctypeslib.as_array(r.octree_size, (n,)).copy(),
+                    key_offset=np.ctypeslib.as_array(r.key_offset, (n,)).copy(),
+                    octree_offset=np.ctypeslib.as_array(r.octree_offset, (n,)).copy())
+
+    def fetch_all(self, r: BatchResult) -> dict:
+        """Download the whole batch output: per-slot arrays and the octree."""
+        d, kb, nc = self.dev, r.key_bytes, 1 << r.dim
+        slots = int(self.sizes.sum() + self.sizes.size)
+        words = d.download(r.keys, key_dtype(kb), slots * (2 if kb == 16 else 1))
+        brt = d.download(r.brt, np.uint8, slots * BRT_DTYPES[kb].itemsize).view(BRT_DTYPES[kb])
+        parent = d.download(r.parent, np.int32, slots)
+        ps = d.download(r.prefix_sums, np.uint32, slots)
+        oct_ = d.download(r.octree, np.int32, r.total_octree * (nc + 1)).reshape(-1, nc + 1)
+        return dict(words=words, brt=brt, parent=parent, ps=ps, octree=oct_, **self.counts(r))
+
+    @staticmethod
+    def instance(all_: dict, i: int, key_bytes: int, dim: int) -> dict:
+        """Instance i of fetch_all() in the layout of Device.fetch()."""
+        nc = 1 << dim
+        k0, m = int(all_["key_offset"][i]), int(all_["m"][i])
+        o0, O = int(all_["octree_offset"][i]), int(all_["octree_size"][i])
+        w = all_["words"]
+        words = w[2 * k0:2 * (k0 + m)] if key_bytes == 16 else w[k0:k0 + m]
+        lo, hi = words_to_keys(words, key_bytes)
+        ni = max(m - 1, 0)
+        brt = all_["brt"][k0:k0 + ni]
+        raw = all_["octree"][o0:o0 + O]
+        out = dict(m=m, octree_size=O, keys_lo=lo, keys_hi=hi, brt=brt,
+                   parent=all_["parent"][k0:k0 + ni].copy(), prefix_sums=all_["ps"][k0:k0 + m].copy(),
+                   children=raw[:, :nc].copy(), leaf=raw[:, nc].copy(),
+                   left=brt["left"].copy(), left_leaf=brt["left_leaf"].copy(),
+                   right_leaf=brt["right_leaf"].copy(), lcp_length=brt["lcp_length"].astype(np.int32))
+        if key_bytes == 16:
+            out["lcp_lo"], out["lcp_hi"] = brt["lcp_lo"].copy(), brt["lcp_hi"].copy()
+        else:
+            out["lcp_lo"], out["lcp_hi"] = brt["lcp"].astype(np.uint64), np.zeros(ni, np.uint64)
+        return out

@@ -22,54 +22,59 @@ namespace pgvd {
 
 namespace {
 
-enum { CTR_BRT = kMaxPasses + 3 };
+// Result of one BRT node.
+template <typename K>
+struct BrtOut {
+  BrtNodeT<K> nd;
+  int lcp_min;    // smaller neighbour delta (-1 for node 0)
+  bool dup;       // keys[i] == keys[i+1]
+  bool quirk;     // the FP32 split step chose a split the exact ceil does not
+};
 
-// One BRT node.  Writes brt[i]; returns lcp_min (the smaller of the two
-// neighbour deltas, which is the parent's lcp_length in a well-formed
-// radix tree) through *lcp_min_out and the node's lcp_length as result.
-// Claims are made by `claim(child, i)`.
-template <typename K, typename Claim>
-__device__ inline int brt_node(const K *__restrict__ keys, int mbits, int m, int i,
-                               BrtNodeT<K> *__restrict__ brt, RunState *st, int *lcp_min_out,
-                               Claim claim) {
-  const K cur = keys[i];
-  const K nxt = keys[i + 1];
+// One BRT node from a delta oracle: dl(x) = delta(keys[i], keys[x], mbits)
+// for 0 <= x < m.  An oracle may answer -1 (below every real delta), which
+// ends each search at that probe; the caller then discards the result.
+template <typename K, typename Dl>
+__device__ inline BrtOut<K> brt_eval(K cur, Dl dl, int mbits, int m, int i) {
+  BrtOut<K> o;
   // The reference requires distinct keys (unique runs first); equal
-  // neighbours are reported, and every search below stays range-bounded.
-  if (keq(cur, nxt)) atomicOr(&st->status, (uint32_t)ST_DUP_KEY);
+  // neighbours (delta == mbits) are reported, and every search below stays
+  // range-bounded.
+  o.dup = dl(i + 1) == mbits;
+  o.quirk = false;
   int d, l;
-  *lcp_min_out = -1;
+  o.lcp_min = -1;
   if (i == 0) {
     d = 1;
     l = m - 1;
   } else {
-    const int l_pos = delta(cur, nxt, mbits);
-    const int l_neg = delta(cur, keys[i - 1], mbits);
+    const int l_pos = dl(i + 1);
+    const int l_neg = dl(i - 1);
     d = l_pos >= l_neg ? 1 : -1;
     const int lcp_min = (d == -1) ? l_pos : l_neg;
-    *lcp_min_out = lcp_min;
+    o.lcp_min = lcp_min;
     int l_max = 2;
     for (;;) {
       const long long idx = (long long)i + (long long)l_max * d;
       if (idx < 0 || idx > m - 1) break;
-      if (delta(cur, keys[idx], mbits) <= lcp_min) break;
+      if (dl((int)idx) <= lcp_min) break;
       l_max <<= 1;
     }
     const int l_cutoff = (d == -1) ? i : m - i - 1;
     l = 0;
     for (int t = l_max >> 1; t >= 1; t >>= 1) {
-      if (l + t <= l_cutoff && delta(cur, keys[i + (l + t) * d], mbits) > lcp_min) l += t;
+      if (l + t <= l_cutoff && dl(i + (l + t) * d) > lcp_min) l += t;
     }
   }
   const int j = i + l * d;
-  const int lcp_node = delta(cur, keys[j], mbits);
+  const int lcp_node = dl(j);
   const int s_cutoff = (d == -1) ? i - 1 : m - i - 2;
   int s = 0;
   for (int den = 2; den < 2 * l; den *= 2) {
     // (l + (float)den - 1) / den, every operation rounded as in C.
     const float num = __fadd_rn(__fadd_rn((float)l, (float)den), -1.0f);
     const int t = (int)__fdiv_rn(num, (float)den);
-    if (s + t <= s_cutoff && delta(cur, keys[i + (s + t) * d], mbits) > lcp_node) s += t;
+    if (s + t <= s_cutoff && dl(i + (s + t) * d) > lcp_node) s += t;
   }
   if (l > (1 << 23)) {
     // For l <= 2^23 every intermediate of the FP32 step is an integer
@@ -80,21 +85,23 @@ __device__ inline int brt_node(const K *__restrict__ keys, int mbits, int m, int
     int se = 0;
     for (int den = 2; den < 2 * l; den *= 2) {
       const int t = (int)(((long long)l + den - 1) / den);
-      if (se + t <= s_cutoff && delta(cur, keys[i + (se + t) * d], mbits) > lcp_node) se += t;
+      if (se + t <= s_cutoff && dl(i + (se + t) * d) > lcp_node) se += t;
     }
-    if (se != s) st->malformed = 1u;
+    o.quirk = se != s;
   }
   const int split = i + s * d + (d < 0 ? d : 0);
-  BrtNodeT<K> nd;
-  nd.left = split;
-  nd.left_leaf = (min(i, j) == split);
-  nd.right_leaf = (max(i, j) == split + 1);
-  nd.lcp = kshr(cur, mbits - lcp_node);
-  nd.lcp_length = (uint16_t)lcp_node;
-  brt[i] = nd;
-  if (!nd.left_leaf) claim(split, i);
-  if (!nd.right_leaf) claim(split + 1, i);
-  return lcp_node;
+  o.nd.left = split;
+  o.nd.left_leaf = (min(i, j) == split);
+  o.nd.right_leaf = (max(i, j) == split + 1);
+  o.nd.lcp = kshr(cur, mbits - lcp_node);
+  o.nd.lcp_length = (uint16_t)lcp_node;
+  return o;
+}
+
+template <typename K>
+__device__ inline void brt_flags(const BrtOut<K> &o, RunState *st) {
+  if (o.dup) atomicOr(&st->status, (uint32_t)ST_DUP_KEY);
+  if (o.quirk) st->malformed = 1u;
 }
 
 // Stage form (BuildBinaryRadixTree_p): claims as atomicMax(claimer + 1) into
@@ -107,86 +114,180 @@ __global__ __launch_bounds__(kBlock) void brt_kernel(const K *__restrict__ keys,
   const int m = m_host >= 0 ? (int)m_host : (int)st->m;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= m - 1) return;
-  int lcp_min;
-  brt_node(keys, mbits, m, i, brt, st, &lcp_min,
-           [&](int c, int p) { atomicMax(&parent[c], p + 1); });
+  const K cur = keys[i];
+  const BrtOut<K> o = brt_eval(cur, [&](int x) { return delta(cur, keys[x], mbits); }, mbits, m, i);
+  brt_flags(o, st);
+  brt[i] = o.nd;
+  if (!o.nd.left_leaf) atomicMax(&parent[o.nd.left], i + 1);
+  if (!o.nd.right_leaf) atomicMax(&parent[o.nd.left + 1], i + 1);
 }
 
-// Pipeline form: BRT node, its local split and the inclusive scan of the
-// local splits in one pass over striped tiles of kRows*256 nodes.
-//  * In a well-formed radix tree the parent's lcp_length is the node's
-//    lcp_min, so ls[c] = lcp_length(c)/DIM - lcp_min(c)/DIM needs no parent
-//    lookup (C/BuildOctree.c:17-27 computes the same from the parent).
-//  * Every non-root node has exactly one claimer, so parent links are
-//    plain stores in the reference form (root -1).
+// Pipeline store of node c: the record, its children's parent links as
+// plain stores (every non-root node of a well-formed radix tree has exactly
+// one claimer; root -1), and lsx[c] = ls | octant << 8 where
+//  * ls = lcp_length(c)/DIM - lcp_min(c)/DIM: in a well-formed radix tree
+//    the parent's lcp_length is the node's lcp_min, so this is the local
+//    split C/BuildOctree.c:17-27 computes from the parent (root: 1 + len/DIM);
+//  * octant = digit of keys[c] at the parent's octree level: key c lies in
+//    node c's range, so this is quadrantInLcp(c, ls - 1)
+//    (C/BuildOctree.c:47-55), the octant c's chain attaches in.
 // If some node took an FP32-quirk split (st->malformed), parent links, local
 // splits and prefix sums are recomputed by the order-exact claim path.
 template <typename K, int DIM>
-__global__ __launch_bounds__(kBlock) void brt_scan_kernel(const K *__restrict__ keys, int mbits,
-                                                          BrtNodeT<K> *__restrict__ brt,
-                                                          int32_t *__restrict__ parent,
-                                                          uint32_t *__restrict__ ps,
-                                                          uint64_t *lookback, RunState *st) {
-  constexpr int R = 8;
-  constexpr int TILE = R * kBlock;
-  __shared__ uint32_t s_w[R * 4];
-  __shared__ uint64_t s_excl;
-  __shared__ uint32_t s_tile;
-  const int t = threadIdx.x;
-  const int m = (int)st->m;
-  const int ntiles = (m + TILE - 1) / TILE;
-  if (t == 0) s_tile = atomicAdd(&st->counters[CTR_BRT], 1u);
-  __syncthreads();
-  const int tile = (int)s_tile;
-  if (tile >= ntiles) return;
-  const int base = tile * TILE;
-  uint32_t v[R], ex[R];
-#pragma unroll 1
-  for (int r = 0; r < R; ++r) {
-    const int c = base + r * kBlock + t;
-    uint32_t ls = 0;
-    if (c < m - 1) {
-      int lcp_min;
-      const int len = brt_node(keys, mbits, m, c, brt, st, &lcp_min,
-                               [&](int child, int p) { parent[child] = p; });
-      ls = (c == 0) ? 1u + (uint32_t)(len / DIM) : (uint32_t)(len / DIM - lcp_min / DIM);
-      if (c == 0) parent[0] = -1;
-    } else if (c == m - 1 && m == 1) {
-      ls = 1;     // single key: one octree node (the reference reads I[0] of
-                  // an empty vector here, opencl/Kernels.cpp:305-306)
-    }
-    v[r] = ls;
+__device__ inline void brt_store(int c, K cur, const BrtOut<K> &o, int mbits,
+                                 BrtNodeT<K> *__restrict__ brt, int32_t *__restrict__ parent,
+                                 uint32_t *__restrict__ lsx) {
+  brt[c] = o.nd;
+  if (!o.nd.left_leaf) parent[o.nd.left] = c;
+  if (!o.nd.right_leaf) parent[o.nd.left + 1] = c;
+  const int len = o.nd.lcp_length;
+  if (c == 0) {
+    parent[0] = -1;
+    lsx[0] = 1u + (uint32_t)(len / DIM);
+  } else {
+    const int plvl = o.lcp_min / DIM;
+    lsx[c] = (uint32_t)(len / DIM - plvl) | ((uint32_t)digit_at<K, DIM>(cur, plvl, mbits) << 8);
   }
-  const uint32_t total = striped_excl_scan<R, uint32_t>(v, ex, s_w);
-  const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl);
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int c = base + r * kBlock + t;
-    if (c < m) ps[c] = (uint32_t)(excl + ex[r] + v[r]);
-  }
-  if (tile == ntiles - 1 && t == 0) st->octree_size = (uint32_t)(excl + total);
 }
 
-// Pipeline form, one node per lane: BRT node + its local split, parent links
-// as plain stores (see brt_scan_kernel for why both are exact in a
-// well-formed tree).  ls[m-1] = 0; m == 1 gives ls[0] = 1.
+// Keys staged per workgroup: its SEG = R*256 nodes plus H keys either side.
+template <typename K> struct BrtWin { static constexpr int R = 4, H = 512; };
+template <> struct BrtWin<Key128> { static constexpr int R = 2, H = 256; };
+
+// Pipeline, first pass.  A workgroup stages keys [b0-H, b0+SEG+H) of one
+// key array keys[0..m) in LDS and evaluates nodes [b0, b0+SEG) against that
+// window only.  Most nodes span a few keys, so their searches (a chain of
+// dependent probes) run at LDS latency; a node whose search leaves the
+// window is handed to defer(true, c) and left to the global-memory pass.
+// lsx[m-1] = 0; m == 1 gives lsx[0] = 1.  Every lane calls defer once per
+// row, converged.
+template <typename K, int DIM, int R, typename Defer>
+__device__ inline void brt_window(const K *__restrict__ keys, int m, int b0, int mbits,
+                                  BrtNodeT<K> *__restrict__ brt, int32_t *__restrict__ parent,
+                                  uint32_t *__restrict__ lsx, RunState *st, Defer defer) {
+  constexpr int H = BrtWin<K>::H, SEG = R * kBlock, WN = SEG + 2 * H;
+  __shared__ K s_keys[WN];
+  const int w0 = b0 - H;
+  for (int r = threadIdx.x; r < WN; r += kBlock) {
+    const int g = w0 + r;
+    if (g >= 0 && g < m) s_keys[r] = keys[g];
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int r = 0; r < R; ++r) {
+    const int c = b0 + r * kBlock + (int)threadIdx.x;
+    bool miss = false;
+    if (c < m - 1) {
+      const K cur = s_keys[c - w0];
+      const BrtOut<K> o = brt_eval(
+          cur,
+          [&](int x) -> int {
+            const unsigned k = (unsigned)(x - w0);
+            if (k >= (unsigned)WN) { miss = true; return -1; }
+            return delta(cur, s_keys[k], mbits);
+          },
+          mbits, m, c);
+      if (!miss) {
+        brt_flags(o, st);
+        brt_store<K, DIM>(c, cur, o, mbits, brt, parent, lsx);
+      }
+    } else if (c == m - 1) {
+      lsx[c] = (m == 1) ? 1u : 0u;
+    }
+    defer(miss, c);
+  }
+}
+
 template <typename K, int DIM>
 __global__ __launch_bounds__(kBlock) void brt_ls_kernel(const K *__restrict__ keys, int mbits,
                                                         BrtNodeT<K> *__restrict__ brt,
                                                         int32_t *__restrict__ parent,
-                                                        uint32_t *__restrict__ ls, RunState *st) {
+                                                        uint32_t *__restrict__ lsx,
+                                                        uint32_t *__restrict__ defer,
+                                                        RunState *st) {
+  constexpr int R = BrtWin<K>::R, SEG = R * kBlock;
   const int m = (int)st->m;
-  const int c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= m) return;
-  if (c == m - 1) {
-    ls[c] = (m == 1) ? 1u : 0u;
-    return;
+  const int b0 = (int)xcd_segment(blockIdx.x, gridDim.x) * SEG;
+  if (b0 >= m) return;                      // uniform over the workgroup
+  brt_window<K, DIM, R>(keys, m, b0, mbits, brt, parent, lsx, st, [&](bool miss, int c) {
+    wave_append(miss, (uint32_t)c, &st->counters[CTR_BRT_DEFER], defer);
+  });
+}
+
+// Pipeline, second pass: the nodes the window could not hold, against keys
+// in global memory.
+template <typename K, int DIM>
+__global__ __launch_bounds__(kBlock) void brt_ls_defer_kernel(const K *__restrict__ keys,
+                                                              int mbits,
+                                                              BrtNodeT<K> *__restrict__ brt,
+                                                              int32_t *__restrict__ parent,
+                                                              uint32_t *__restrict__ lsx,
+                                                              const uint32_t *__restrict__ defer,
+                                                              RunState *st) {
+  const int m = (int)st->m;
+  const uint32_t cnt = st->counters[CTR_BRT_DEFER];
+  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
+    const int c = (int)defer[k];
+    const K cur = keys[c];
+    const BrtOut<K> o =
+        brt_eval(cur, [&](int x) { return delta(cur, keys[x], mbits); }, mbits, m, c);
+    brt_flags(o, st);
+    brt_store<K, DIM>(c, cur, o, mbits, brt, parent, lsx);
   }
-  int lcp_min;
-  const int len = brt_node(keys, mbits, m, c, brt, st, &lcp_min,
-                           [&](int child, int p) { parent[child] = p; });
-  if (c == 0) parent[0] = -1;
-  ls[c] = (c == 0) ? 1u + (uint32_t)(len / DIM) : (uint32_t)(len / DIM - lcp_min / DIM);
+}
+
+// Batched form: BRT tiles of R*256 key slots per instance (n+1 slots each);
+// instance s's arrays start at key slot kofs, m = ss[s].m.  Deferred nodes
+// are recorded by key slot.
+template <typename K, int DIM, int R>
+__global__ __launch_bounds__(kBlock) void brt_seg_kernel(const K *__restrict__ keys, int mbits,
+                                                         const SegInfo *__restrict__ seg,
+                                                         TileTable tt,
+                                                         const SegState *__restrict__ ss,
+                                                         BrtNodeT<K> *__restrict__ brt,
+                                                         int32_t *__restrict__ parent,
+                                                         uint32_t *__restrict__ lsx,
+                                                         uint32_t *__restrict__ defer,
+                                                         RunState *st) {
+  constexpr int SEG = R * kBlock;
+  const uint32_t tile = xcd_segment(blockIdx.x, gridDim.x);
+  const uint32_t s = tt.seg[tile];
+  const int m = (int)ss[s].m;
+  const int b0 = (int)(tile - tt.first[s]) * SEG;
+  if (b0 >= m) return;
+  const uint32_t kofs = seg[s].kofs;
+  brt_window<K, DIM, R>(keys + kofs, m, b0, mbits, brt + kofs, parent + kofs, lsx + kofs, st,
+                        [&](bool miss, int c) {
+                          wave_append(miss, kofs + (uint32_t)c, &st->counters[CTR_BRT_DEFER],
+                                      defer);
+                        });
+}
+
+template <typename K, int DIM>
+__global__ __launch_bounds__(kBlock) void brt_seg_defer_kernel(const K *__restrict__ keys,
+                                                               int mbits,
+                                                               const uint32_t *__restrict__ kofs_all,
+                                                               uint32_t B,
+                                                               const SegState *__restrict__ ss,
+                                                               BrtNodeT<K> *__restrict__ brt,
+                                                               int32_t *__restrict__ parent,
+                                                               uint32_t *__restrict__ lsx,
+                                                               const uint32_t *__restrict__ defer,
+                                                               RunState *st) {
+  const uint32_t cnt = st->counters[CTR_BRT_DEFER];
+  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
+    const uint32_t slot = defer[k];
+    const uint32_t s = find_seg(kofs_all, B, slot);
+    const uint32_t kofs = kofs_all[s];
+    const int c = (int)(slot - kofs);
+    const int m = (int)ss[s].m;
+    const K *kk = keys + kofs;
+    const K cur = kk[c];
+    const BrtOut<K> o =
+        brt_eval(cur, [&](int x) { return delta(cur, kk[x], mbits); }, mbits, m, c);
+    brt_flags(o, st);
+    brt_store<K, DIM>(c, cur, o, mbits, brt + kofs, parent + kofs, lsx + kofs);
+  }
 }
 
 // Order-exact recomputation of the parent claims when some split was an
@@ -235,23 +336,55 @@ void launch_finalize_parents(int32_t *parent, int64_t max_m, int64_t m_host, Run
   finalize_parents_kernel<<<grid, kBlock, 0, s>>>(parent, st, m_host);
 }
 
-int64_t brt_scan_tiles(int64_t max_m) { return (max_m + 8 * kBlock - 1) / (8 * kBlock); }
-
-template <typename K>
-void launch_brt_scan(const K *keys, int64_t max_m, int mbits, int dim, BrtNodeT<K> *brt,
-                     int32_t *parent, uint32_t *ps, uint64_t *lookback, RunState *st,
-                     hipStream_t s) {
-  const unsigned grid = (unsigned)brt_scan_tiles(max_m > 0 ? max_m : 1);
-  if (dim == 2) brt_scan_kernel<K, 2><<<grid, kBlock, 0, s>>>(keys, mbits, brt, parent, ps, lookback, st);
-  else brt_scan_kernel<K, 3><<<grid, kBlock, 0, s>>>(keys, mbits, brt, parent, ps, lookback, st);
+template <typename K, int DIM>
+void launch_brt_ls_dim(const K *keys, int64_t max_m, int mbits, BrtNodeT<K> *brt,
+                       int32_t *parent, uint32_t *lsx, uint32_t *defer, RunState *st,
+                       hipStream_t s) {
+  constexpr int SEG = BrtWin<K>::R * kBlock;
+  const unsigned grid = (unsigned)((max_m + SEG - 1) / SEG);
+  brt_ls_kernel<K, DIM><<<grid, kBlock, 0, s>>>(keys, mbits, brt, parent, lsx, defer, st);
+  // The overflow pass grid-strides over however many nodes were deferred.
+  const int64_t g2 = (max_m + kBlock - 1) / kBlock;
+  const unsigned grid2 = (unsigned)(g2 < 2048 ? g2 : 2048);
+  brt_ls_defer_kernel<K, DIM><<<grid2, kBlock, 0, s>>>(keys, mbits, brt, parent, lsx, defer, st);
 }
 
 template <typename K>
 void launch_brt_ls(const K *keys, int64_t max_m, int mbits, int dim, BrtNodeT<K> *brt,
-                   int32_t *parent, uint32_t *ls, RunState *st, hipStream_t s) {
-  const unsigned grid = (unsigned)((max_m + kBlock - 1) / kBlock);
-  if (dim == 2) brt_ls_kernel<K, 2><<<grid, kBlock, 0, s>>>(keys, mbits, brt, parent, ls, st);
-  else brt_ls_kernel<K, 3><<<grid, kBlock, 0, s>>>(keys, mbits, brt, parent, ls, st);
+                   int32_t *parent, uint32_t *lsx, uint32_t *defer, RunState *st,
+                   hipStream_t s) {
+  if (max_m < 1) max_m = 1;
+  if (dim == 2) launch_brt_ls_dim<K, 2>(keys, max_m, mbits, brt, parent, lsx, defer, st, s);
+  else launch_brt_ls_dim<K, 3>(keys, max_m, mbits, brt, parent, lsx, defer, st, s);
+}
+
+bool brt_seg_rows_ok(int key_bytes, int rows) {
+  return key_bytes == 16 ? (rows == 1 || rows == 2) : (rows == 2 || rows == 4);
+}
+
+template <typename K, int DIM, int R>
+void brt_seg_dim(const K *keys, int mbits, const BatchDev &b, BrtNodeT<K> *brt, int32_t *parent,
+                 uint32_t *lsx, uint32_t *defer, RunState *st, hipStream_t s) {
+  brt_seg_kernel<K, DIM, R><<<(unsigned)b.n_brt, kBlock, 0, s>>>(keys, mbits, b.seg, b.brt, b.ss,
+                                                                 brt, parent, lsx, defer, st);
+  brt_seg_defer_kernel<K, DIM><<<2048, kBlock, 0, s>>>(keys, mbits, b.kofs, (uint32_t)b.B, b.ss,
+                                                       brt, parent, lsx, defer, st);
+}
+
+template <typename K>
+void launch_brt_seg(const K *keys, int mbits, int dim, const BatchDev &b, BrtNodeT<K> *brt,
+                    int32_t *parent, uint32_t *lsx, uint32_t *defer, RunState *st,
+                    hipStream_t s) {
+  if (b.n_brt <= 0) return;
+  constexpr bool wide = sizeof(K) == 16;
+  const bool small = b.brt_rows == (wide ? 1 : 2);
+  if (dim == 2) {
+    if (small) brt_seg_dim<K, 2, wide ? 1 : 2>(keys, mbits, b, brt, parent, lsx, defer, st, s);
+    else brt_seg_dim<K, 2, wide ? 2 : 4>(keys, mbits, b, brt, parent, lsx, defer, st, s);
+  } else {
+    if (small) brt_seg_dim<K, 3, wide ? 1 : 2>(keys, mbits, b, brt, parent, lsx, defer, st, s);
+    else brt_seg_dim<K, 3, wide ? 2 : 4>(keys, mbits, b, brt, parent, lsx, defer, st, s);
+  }
 }
 
 template <typename K>
@@ -267,12 +400,12 @@ void launch_claims_if_malformed(const BrtNodeT<K> *brt, int32_t *parent, int64_t
 #define PGVD_INST(K)                                                                          \
   template void launch_brt<K>(const K *, int64_t, int64_t, int, BrtNodeT<K> *, int32_t *,      \
                               RunState *, hipStream_t);                                        \
-  template void launch_brt_scan<K>(const K *, int64_t, int, int, BrtNodeT<K> *, int32_t *,     \
-                                   uint32_t *, uint64_t *, RunState *, hipStream_t);           \
   template void launch_claims_if_malformed<K>(const BrtNodeT<K> *, int32_t *, int64_t,         \
                                               RunState *, hipStream_t);                        \
   template void launch_brt_ls<K>(const K *, int64_t, int, int, BrtNodeT<K> *, int32_t *,       \
-                                 uint32_t *, RunState *, hipStream_t);
+                                 uint32_t *, uint32_t *, RunState *, hipStream_t);             \
+  template void launch_brt_seg<K>(const K *, int, int, const BatchDev &, BrtNodeT<K> *,        \
+                                  int32_t *, uint32_t *, uint32_t *, RunState *, hipStream_t);
 PGVD_INST(uint32_t)
 PGVD_INST(uint64_t)
 PGVD_INST(Key128)

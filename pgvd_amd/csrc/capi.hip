@@ -20,130 +20,11 @@
 
 using namespace pgvd;
 
-namespace {
+#include "capi_internal.hpp"
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string &msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HIPCHK(x)                                                                         \
-  do {                                                                                    \
-    hipError_t e_ = (x);                                                                  \
-    if (e_ != hipSuccess)                                                                 \
-      return fail(PGVD_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));          \
-  } while (0)
-
-#define CHK(x)                    \
-  do {                            \
-    int r_ = (x);                 \
-    if (r_ != PGVD_OK) return r_; \
-  } while (0)
-
-inline float fdec_host(uint32_t e) {
-  const uint32_t b = (e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e;
-  float f;
-  std::memcpy(&f, &b, 4);
-  return f;
-}
-
-inline int key_bytes_for(int mbits) { return mbits <= 32 ? 4 : (mbits <= 64 ? 8 : 16); }
-
-inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-// opencl/Kernels.cpp:20
-int ref_next_pow2(int num) {
-  const int p = (int)pow(2, ceil(log(num) / log(2)));
-  return p > 8 ? p : 8;
-}
-
-struct PoolBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-};
-
-struct TimingRec {
-  std::string stage;
-  hipEvent_t a, b;
-};
-
-struct LastBuild {
-  bool valid = false;
-  int64_t n = 0;
-  int dim = 0, bits = 0, mbits = 0, kb = 0;
-  bool labels = false;
-  void *keys_u = nullptr;
-  void *brt = nullptr;
-  int32_t *parent = nullptr;
-  uint32_t *ps = nullptr;
-  uint32_t *conflict_bits = nullptr;
-  hipStream_t stream = nullptr;
-};
-
-}  // namespace
-
-struct pgvd_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::unordered_map<std::string, PoolBuf> pool;
-  RunState *st = nullptr;       // device
-  RunState *st_host = nullptr;  // pinned mirror
-  hipEvent_t done = nullptr;
-  int timing = 0;               // 0 off, 1 every stage, 2 sort span only
-  std::vector<TimingRec> recs;
-  std::vector<hipEvent_t> spare;
-  LastBuild last;
-  int64_t octree_cap = 0;       // nodes the pooled octree buffer holds
-  int octree_stride = 0;        // ints per node of that buffer
-};
+using namespace pgvd_capi;
 
 namespace {
-
-int use_device(pgvd_ctx *ctx) {
-  HIPCHK(hipSetDevice(ctx->device));
-  return PGVD_OK;
-}
-
-hipStream_t pick(pgvd_ctx *ctx, pgvd_stream s) {
-  return s ? static_cast<hipStream_t>(s) : ctx->stream;
-}
-
-// Keyed pool (CLFW::get, CLFW/src/clfw.cpp:212-226): reuse when large
-// enough, else reallocate with headroom.
-int pool_get(pgvd_ctx *ctx, const char *key, size_t bytes, void **out) {
-  if (bytes == 0) bytes = 256;
-  PoolBuf &b = ctx->pool[key];
-  if (b.bytes < bytes) {
-    if (b.p) {
-      // the old buffer may be in use on any stream the caller passed
-      HIPCHK(hipDeviceSynchronize());
-      HIPCHK(hipFree(b.p));
-      b.p = nullptr;
-      b.bytes = 0;
-    }
-    const size_t want = align256(bytes + bytes / 8);
-    if (hipMalloc(&b.p, want) != hipSuccess) {
-      b.p = nullptr;
-      return fail(PGVD_ERR_ALLOC, std::string("hipMalloc failed for pool buffer ") + key);
-    }
-    b.bytes = want;
-  }
-  *out = b.p;
-  return PGVD_OK;
-}
-
-hipEvent_t take_event(pgvd_ctx *ctx) {
-  if (!ctx->spare.empty()) {
-    hipEvent_t e = ctx->spare.back();
-    ctx->spare.pop_back();
-    return e;
-  }
-  hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
-  return e;
-}
 
 // Records HIP events around a stage's launches when timing is on.
 struct Stage {
@@ -214,7 +95,7 @@ ZeroLayout zero_layout(int64_t n, int kb, int passes, bool labels) {
   z.lb_scan = off;
   off += align256((size_t)scan_tiles(n + 1) * 8);
   z.lb_brt = off;
-  off += align256((size_t)brt_scan_tiles(n + 1) * 8);
+  off += align256((size_t)scan_tiles(n + 1) * 8);
   z.cbits = off;
   if (labels) off += align256((size_t)((n + 1 + 31) / 32) * 4);
   z.total = off;
@@ -297,6 +178,8 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   uint32_t *ps = static_cast<uint32_t *>(p);
   CHK(pool_get(ctx, "parent", (size_t)nk * 4, &p));
   int32_t *parent = static_cast<int32_t *>(p);
+  CHK(pool_get(ctx, "defer", (size_t)nk * 4, &p));
+  uint32_t *defer = static_cast<uint32_t *>(p);
   const ZeroLayout z = zero_layout(n, kb, passes, lab);
   CHK(pool_get(ctx, "zero", z.total, &p));
   char *zb = static_cast<char *>(p);
@@ -350,6 +233,7 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   CHK(launch_status("onesweep"));
   K *sorted = parity ? kB : kA;
   K *uniq = parity ? kA : kB;
+  uint32_t *lsx = reinterpret_cast<uint32_t *>(sorted);
   const uint32_t *sortedv = lab ? (n <= 1 ? labels : (parity ? vB : vA)) : nullptr;
   {
     Stage t(ctx, "unique", s);
@@ -358,14 +242,14 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   CHK(launch_status("unique"));
   {
     Stage t(ctx, "brt", s);
-    // local splits go to the sort's spare key buffer (free after unique)
-    launch_brt_ls<K>(uniq, nk, mbits, dim, brt, parent, reinterpret_cast<uint32_t *>(sorted),
-                     ctx->st, s);
+    // packed local splits go to the sort's spare key buffer (free after
+    // unique)
+    launch_brt_ls<K>(uniq, nk, mbits, dim, brt, parent, lsx, defer, ctx->st, s);
   }
   CHK(launch_status("brt"));
   {
     Stage t(ctx, "splits_scan", s);
-    launch_ls_scan(reinterpret_cast<const uint32_t *>(sorted), ps, nk, lb_b, ctx->st, s);
+    launch_ls_scan(lsx, ps, nk, lb_b, ctx->st, s);
   }
   CHK(launch_status("splits_scan"));
   {
@@ -378,8 +262,8 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   CHK(launch_status("malformed path"));
   {
     Stage t(ctx, "octree", s);
-    launch_octree<K>(brt, parent, ps, uniq, cbits, nk, dim, mbits, octree, nconf,
-                     ctx->octree_cap, 0, ctx->st, s);
+    launch_octree<K>(brt, parent, ps, lsx, uniq, cbits, nk, dim, mbits, octree, nconf,
+                     ctx->octree_cap, 0, defer, ctx->st, s);
   }
   CHK(launch_status("octree"));
   HIPCHK(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(RunState), hipMemcpyDeviceToHost, s));
@@ -396,6 +280,8 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   L.brt = brt;
   L.parent = parent;
   L.ps = ps;
+  L.lsx = lsx;
+  L.defer = defer;
   L.conflict_bits = cbits;
   L.stream = s;
   return PGVD_OK;
@@ -411,9 +297,9 @@ int refill_octree(pgvd_ctx *ctx, int64_t nodes) {
   ctx->st_host->status &= ~(uint32_t)ST_CAPACITY;
   HIPCHK(hipMemcpyAsync(&ctx->st->status, &ctx->st_host->status, 4, hipMemcpyHostToDevice,
                         L.stream));
-  launch_octree<K>(static_cast<const BrtNodeT<K> *>(L.brt), L.parent, L.ps,
+  launch_octree<K>(static_cast<const BrtNodeT<K> *>(L.brt), L.parent, L.ps, L.lsx,
                    static_cast<const K *>(L.keys_u), L.conflict_bits, L.n + 1, L.dim, L.mbits,
-                   octree, nconf, ctx->octree_cap, 0, ctx->st, L.stream);
+                   octree, nconf, ctx->octree_cap, 0, L.defer, ctx->st, L.stream);
   CHK(launch_status("octree refill"));
   return copy_state_and_wait(ctx, L.stream);
 }
@@ -520,8 +406,8 @@ int brt_to_octree_impl(pgvd_ctx *ctx, const BrtNodeT<K> *brt, const int32_t *par
   {
     Stage t(ctx, "octree", s);
     // mbits is not needed without conflict flags; keys are not read.
-    launch_octree<K>(brt, parent, ps, nullptr, nullptr, m, dim, 0, octree, nullptr, capacity, 1,
-                     ctx->st, s);
+    launch_octree<K>(brt, parent, ps, nullptr, nullptr, nullptr, m, dim, 0, octree, nullptr,
+                     capacity, 1, nullptr, ctx->st, s);
   }
   CHK(launch_status("octree"));
   CHK(copy_state_and_wait(ctx, s));
@@ -636,6 +522,20 @@ int pgvd_copy_kernel(pgvd_ctx *ctx, void *dst, const void *src, size_t bytes, pg
     launch_copy(dst, src, bytes, st);
   }
   return launch_status("copy");
+}
+
+int pgvd_digest(pgvd_ctx *ctx, const void *p, size_t bytes, pgvd_stream s, uint64_t *out) {
+  if (!ctx || !out || (!p && bytes)) return fail(PGVD_ERR_INVALID, "null argument");
+  if ((bytes & 3) || ((uintptr_t)p & 3)) return fail(PGVD_ERR_INVALID, "digest needs 4-byte aligned whole words");
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  void *d = nullptr;
+  CHK(pool_get(ctx, "digest", 8, &d));
+  launch_digest(p, bytes / 4, static_cast<uint64_t *>(d), true, st);
+  CHK(launch_status("digest"));
+  HIPCHK(hipMemcpyAsync(out, d, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return PGVD_OK;
 }
 
 int pgvd_timing_enable(pgvd_ctx *ctx, int on) {

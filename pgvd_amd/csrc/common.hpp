@@ -172,6 +172,46 @@ __device__ inline T wave_reduce_add(T v) {
   return v;
 }
 
+// Appends `idx` to list[] for every lane with `take` set, one atomic per wave
+// on *count.  Call with the whole wave converged.
+template <typename T>
+__device__ inline void wave_append(bool take, T idx, uint32_t *count, T *list) {
+  const uint64_t mb = __ballot(take);
+  if (!mb) return;
+  const int leader = __ffsll((unsigned long long)mb) - 1;
+  uint32_t base = 0;
+  if ((int)__lane_id() == leader) base = atomicAdd(count, (uint32_t)__popcll(mb));
+  base = __shfl(base, leader, 64);
+  if (take) list[base + (uint32_t)__popcll(mb & lanemask_lt())] = idx;
+}
+
+// Instance owning key slot `slot`: the largest s with kofs[s] <= slot
+// (kofs strictly increasing, kofs[B] = total slots).
+__device__ inline uint32_t find_seg(const uint32_t *kofs, uint32_t B, uint32_t slot) {
+  uint32_t lo = 0, hi = B;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (kofs[mid] <= slot) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Workgroup b of a grid of nb runs on XCD b % 8 (round-robin dispatch).  This
+// maps b to a segment index so that each XCD walks one contiguous run of
+// segments: neighbouring segments, whose halos overlap, then share an L2.
+__device__ inline uint32_t xcd_segment(uint32_t b, uint32_t nb) {
+  const uint32_t q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// Octant of `key` at octree level lvl: the DIM key bits right below a prefix
+// of lvl digits (0 if the level is below the key's last digit).
+template <typename K, int DIM>
+__device__ inline int digit_at(K key, int lvl, int mbits) {
+  const int sh = mbits - (lvl + 1) * DIM;
+  return sh < 0 ? 0 : (int)(klow32(kshr(key, sh)) & ((1u << DIM) - 1));
+}
+
 // Encodes a float so that unsigned order equals float order (non-NaN).
 __device__ inline uint32_t fenc(float f) {
   const uint32_t b = __float_as_uint(f);

@@ -21,6 +21,15 @@ enum : uint32_t {
 
 constexpr int kMaxPasses = 16;    // 128-bit keys / 8-bit digits
 constexpr int kNumCounters = 24;  // tile counters: sort passes, unique, scans
+// Counter slots after the sort passes' kMaxPasses.
+enum {
+  CTR_UNIQUE = kMaxPasses,
+  CTR_SPLITS = kMaxPasses + 1,
+  CTR_SCAN = kMaxPasses + 2,
+  CTR_LS = kMaxPasses + 3,
+  CTR_BRT_DEFER = kMaxPasses + 4,   // BRT nodes left to the global-memory pass
+  CTR_OCT_DEFER = kMaxPasses + 5,   // octree gathers left to the global pass
+};
 
 // Per-run scalars that kernels read and write on the device.  Cleared by the
 // prep kernel at the start of every run.
@@ -47,6 +56,39 @@ struct BrtNodeT {
 
 // Launch configuration constants.
 constexpr int kBlock = 256;
+
+// ---- batched form ----
+// One instance of a batch: its points pts[pofs .. pofs+n) (DIM floats or
+// ints each) and its key slots [kofs, kofs+n+1) in every per-key array
+// (keys, BRT, prefix sums, parent links).  Set once per batch plan.
+struct SegInfo {
+  uint64_t pofs;       // first point
+  uint32_t n;          // points (1 <= n <= kMaxSegPoints)
+  uint32_t kofs;       // first key slot
+  uint32_t prepend;    // 1 if the zero-key padding applies (nextPow2(n) != n)
+  uint32_t pad_;
+};
+// Per-instance results written on the device.
+struct SegState {
+  uint32_t bb_min[3];  // fenc-encoded bbox
+  uint32_t bb_max[3];
+  uint32_t m;          // unique keys (incl. the padding zero)
+  uint32_t octree_size;
+  uint32_t oct_off;    // first octree node of the instance in the batch output
+  uint32_t pad_[3];
+};
+// Tiles of one stage over the instances: tile t belongs to instance seg[t];
+// instance s owns tiles [first[s], first[s+1]).
+struct TileTable {
+  const uint32_t *seg;
+  const uint32_t *first;
+};
+// Largest instance the batched form takes: ranges stay <= 2^23 keys, where
+// the BRT's FP32 split step is exact, so every instance's radix tree is well
+// formed.
+constexpr int64_t kMaxSegPoints = int64_t(1) << 23;
+// Points per bbox / encode chunk.
+constexpr int kEncodeChunk = 65536;
 
 // ---- morton.hip ----
 void launch_prep(RunState *st, uint32_t *zero_words, size_t nzero, hipStream_t s);
@@ -87,6 +129,8 @@ void launch_init_octree(int32_t *octree, int64_t nodes, int dim, hipStream_t s);
 void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 void launch_set_sizes(RunState *st, const uint32_t *ps, int64_t m, hipStream_t s);
+// *out (+)= digest of `words` 32-bit words at p: sum of mix64(w_i | i << 32).
+void launch_digest(const void *p, size_t words, uint64_t *out, bool zero_first, hipStream_t s);
 
 // ---- sort.hip ----
 // Onesweep LSD radix sort of n keys (+ optional u32 payload) over `passes`
@@ -133,20 +177,14 @@ void launch_brt(const K *keys, int64_t max_m, int64_t m_host, int mbits, BrtNode
 // Rewrite parent[] from claim form to the reference form (BRT stage API).
 void launch_finalize_parents(int32_t *parent, int64_t max_m, int64_t m_host, RunState *st,
                              hipStream_t s);
-// Pipeline: BRT + local splits + inclusive scan in one pass (m = st->m);
-// parent[] in the reference form; ps[0..m); st->octree_size.  lookback has
-// brt_scan_tiles(max_m) zeroed words.
-int64_t brt_scan_tiles(int64_t max_m);
-template <typename K>
-void launch_brt_scan(const K *keys, int64_t max_m, int mbits, int dim, BrtNodeT<K> *brt,
-                     int32_t *parent, uint32_t *ps, uint64_t *lookback, RunState *st,
-                     hipStream_t s);
-// Pipeline: one BRT node per lane with its local split into ls[0..m) and
-// parent links as plain stores (well-formed tree; see brt.hip).
+// Pipeline: every BRT node with its local split and attach octant packed as
+// lsx[c] = ls | octant << 8 (lsx[0..m)), parent links as plain stores
+// (well-formed tree; see brt.hip).  defer[] holds up to max_m indices.
 template <typename K>
 void launch_brt_ls(const K *keys, int64_t max_m, int mbits, int dim, BrtNodeT<K> *brt,
-                   int32_t *parent, uint32_t *ls, RunState *st, hipStream_t s);
-// Inclusive scan of ls[0..st->m) into ps; st->octree_size = total.
+                   int32_t *parent, uint32_t *lsx, uint32_t *defer, RunState *st,
+                   hipStream_t s);
+// Inclusive scan of (lsx[0..st->m) & 0xff) into ps; st->octree_size = total.
 void launch_ls_scan(const uint32_t *ls, uint32_t *ps, int64_t max_m, uint64_t *lookback,
                     RunState *st, hipStream_t s);
 // When st->malformed: parent[] <- claim form (zero, then atomicMax claims).
@@ -157,16 +195,73 @@ void launch_claims_if_malformed(const BrtNodeT<K> *brt, int32_t *parent, int64_t
 // ---- octree.hip ----
 // Octree fill.  parent[] in the reference form.  force_scatter selects the
 // order-exact scatter path unconditionally (stage API, where the BRT comes
-// from the caller); otherwise st->malformed selects it on the device.
+// from the caller; lsx and defer may then be null); otherwise st->malformed
+// selects it on the device and the gather path reads lsx from
+// launch_brt_ls and uses defer[] (max_m entries) as its overflow list.
 template <typename K>
 void launch_octree(const BrtNodeT<K> *brt, const int32_t *parent, const uint32_t *ps,
-                   const K *keys, const uint32_t *conflict_bits, int64_t max_m, int dim,
-                   int mbits, int32_t *octree, uint32_t *node_conflict, int64_t capacity,
-                   int force_scatter, RunState *st, hipStream_t s);
+                   const uint32_t *lsx, const K *keys, const uint32_t *conflict_bits,
+                   int64_t max_m, int dim, int mbits, int32_t *octree,
+                   uint32_t *node_conflict, int64_t capacity, int force_scatter,
+                   uint32_t *defer, RunState *st, hipStream_t s);
 // Local splits the reference's way (each node writes its internal children's
 // entries, highest claimer last): ComputeLocalSplits_p for a caller's BRT.
 template <typename K>
 void launch_local_splits(const BrtNodeT<K> *brt, int64_t m, int dim, uint32_t *ls,
                          uint64_t *tmp, hipStream_t s);
+
+// ---- batched stages: one launch per stage for every instance of a batch ----
+// Device-side description of a batch plan.  Tile tables:
+//   chunk: kEncodeChunk points      sort: sort_ipt*256 keys
+//   scan:  kScanTile key slots      brt:  brt_rows*256 key slots
+//   node:  256 key slots
+// (key-slot tables cover n+1 slots per instance).
+struct BatchDev {
+  const SegInfo *seg = nullptr;
+  const uint32_t *kofs = nullptr;   // B+1 entries: first key slot per instance, then the total
+  SegState *ss = nullptr;
+  int64_t B = 0;
+  int sort_ipt = 16, brt_rows = 4;
+  TileTable chunk{}, sort{}, scan{}, brt{}, node{};
+  int64_t n_chunk = 0, n_sort = 0, n_scan = 0, n_brt = 0, n_node = 0;
+};
+// Sort tile configurations the batched sort is built for (keys per thread).
+bool onesweep_seg_ipt_ok(int key_bytes, int ipt);
+// BRT window configurations (nodes per thread).
+bool brt_seg_rows_ok(int key_bytes, int rows);
+
+void launch_seg_init(SegState *ss, int64_t B, hipStream_t s);                   // morton.hip
+void launch_bbox_seg(const float *pts, int dim, const BatchDev &b, hipStream_t s);
+// Quantize (float points, per-instance bbox) or take int32 grid points, Morton
+// encode into keys[kofs..), per-instance digit histograms bhist[s][pass][256].
+template <typename K>
+void launch_encode_seg(const void *pts, int quantized, int dim, int bits, float effw,
+                       const BatchDev &b, K *keys, uint32_t *bhist, int passes, RunState *st,
+                       hipStream_t s);
+// Per-instance onesweep sort; returns 1 if the result is in keys_alt.
+template <typename K>
+int launch_onesweep_seg(K *keys, K *keys_alt, const BatchDev &b, int passes,
+                        const uint32_t *bhist, uint32_t *lookback, RunState *st, hipStream_t s);
+// Per-instance unique (with its zero-key padding) into out[kofs..); ss[s].m.
+template <typename K>
+void launch_unique_seg(const K *keys, K *out, const BatchDev &b, uint64_t *lookback,
+                       RunState *st, hipStream_t s);
+// Per-instance BRT + packed local splits (as launch_brt_ls); defer holds
+// up to one key-slot index per key slot.
+template <typename K>
+void launch_brt_seg(const K *keys, int mbits, int dim, const BatchDev &b, BrtNodeT<K> *brt,
+                    int32_t *parent, uint32_t *lsx, uint32_t *defer, RunState *st,
+                    hipStream_t s);
+// Per-instance inclusive scan of the local splits; ss[s].octree_size.
+void launch_ls_scan_seg(const uint32_t *lsx, uint32_t *ps, const BatchDev &b,
+                        uint64_t *lookback, RunState *st, hipStream_t s);
+// ss[s].oct_off = exclusive scan of the octree sizes; st->octree_size = total.
+void launch_oct_offsets(const BatchDev &b, RunState *st, hipStream_t s);
+// Per-instance octree gather into octree[oct_off[s]..) (instance-local child
+// indices); flags ST_CAPACITY if the total exceeds capacity.
+template <typename K>
+void launch_gather_seg(const BrtNodeT<K> *brt, const uint32_t *ps, const uint32_t *lsx, int dim,
+                       const BatchDev &b, int32_t *octree, int64_t capacity, uint32_t *defer,
+                       RunState *st, hipStream_t s);
 
 }  // namespace pgvd

@@ -311,6 +311,115 @@ __global__ __launch_bounds__(kBlock) void histogram_kernel(const K *__restrict__
     if (sh[i]) atomicAdd(&hist[i], sh[i]);
 }
 
+// ---- batched form: one workgroup per chunk of kEncodeChunk points of one
+// instance ----
+__global__ __launch_bounds__(kBlock) void seg_init_kernel(SegState *ss, int64_t B) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= B) return;
+  SegState z;
+  for (int k = 0; k < 3; ++k) { z.bb_min[k] = 0xFFFFFFFFu; z.bb_max[k] = 0u; }
+  z.m = 0; z.octree_size = 0; z.oct_off = 0;
+  for (int k = 0; k < 3; ++k) z.pad_[k] = 0;
+  ss[i] = z;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void bbox_seg_kernel(const float *__restrict__ pts,
+                                                          const SegInfo *__restrict__ seg,
+                                                          TileTable ct, SegState *ss) {
+  const uint32_t chunk = blockIdx.x;
+  const uint32_t s = ct.seg[chunk];
+  const SegInfo si = seg[s];
+  const uint32_t start = (chunk - ct.first[s]) * (uint32_t)kEncodeChunk;
+  const uint32_t cnt = min((uint32_t)kEncodeChunk, si.n - start);
+  const float *p = pts + (si.pofs + start) * DIM;
+  float mn[DIM], mx[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) { mn[k] = __int_as_float(0x7f800000); mx[k] = -mn[k]; }
+  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      const float v = p[(size_t)i * DIM + k];
+      mn[k] = fminf(mn[k], v);
+      mx[k] = fmaxf(mx[k], v);
+    }
+  }
+  __shared__ float smn[4][DIM], smx[4][DIM];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    for (int d = 32; d >= 1; d >>= 1) {
+      mn[k] = fminf(mn[k], __shfl_xor(mn[k], d, 64));
+      mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], d, 64));
+    }
+    if (lane == 0) { smn[w][k] = mn[k]; smx[w][k] = mx[k]; }
+  }
+  __syncthreads();
+  if (threadIdx.x < DIM) {
+    const int k = threadIdx.x;
+    float a = smn[0][k], b = smx[0][k];
+    for (int j = 1; j < 4; ++j) { a = fminf(a, smn[j][k]); b = fmaxf(b, smx[j][k]); }
+    atomicMin(&ss[s].bb_min[k], fenc(a));
+    atomicMax(&ss[s].bb_max[k], fenc(b));
+  }
+}
+
+template <typename K, int DIM, bool INT>
+__global__ __launch_bounds__(kBlock) void encode_seg_kernel(
+    const void *__restrict__ pts, const SegInfo *__restrict__ seg, TileTable ct,
+    const SegState *__restrict__ ss, int bits, float effw, K *__restrict__ keys,
+    uint32_t *__restrict__ bhist, int passes, RunState *st) {
+  __shared__ uint32_t sh[kMaxPasses * 256];
+  for (int i = threadIdx.x; i < passes * 256; i += kBlock) sh[i] = 0;
+  const uint32_t chunk = blockIdx.x;
+  const uint32_t s = ct.seg[chunk];
+  const SegInfo si = seg[s];
+  const uint32_t start = (chunk - ct.first[s]) * (uint32_t)kEncodeChunk;
+  const uint32_t cnt = min((uint32_t)kEncodeChunk, si.n - start);
+  float mn[3] = {0.f, 0.f, 0.f};
+  float dwidth = 0.f;
+  if (!INT) {
+    float mx[3];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) { mn[k] = fdec(ss[s].bb_min[k]); mx[k] = fdec(ss[s].bb_max[k]); }
+    // BoundingBox::max_size (BoundingBox.h:80-87)
+    dwidth = __fsub_rn(mx[0], mn[0]);
+#pragma unroll
+    for (int k = 1; k < DIM; ++k) {
+      const float e = __fsub_rn(mx[k], mn[k]);
+      dwidth = (e > dwidth) ? e : dwidth;
+    }
+  }
+  __syncthreads();
+  bool neg = false;
+  K *out = keys + si.kofs + start;
+  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
+    const size_t pi = (size_t)(si.pofs + start + i) * DIM;
+    int64_t c[DIM];
+    if (INT) {
+      const int32_t *q = static_cast<const int32_t *>(pts) + pi;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) c[k] = q[k];
+    } else {
+      const float *q = static_cast<const float *>(pts) + pi;
+      float f[DIM];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) f[k] = q[k];
+      neg |= quantize_point<DIM>(f, mn, dwidth, effw, c);
+    }
+    uint64_t lo, hi;
+    morton128<DIM>(c, bits, lo, hi);
+    const K key = from128<K>(lo, hi);
+    out[i] = key;
+    for (int r = 0; r < passes; ++r) atomicAdd(&sh[r * 256 + kdigit(key, 8 * r)], 1u);
+  }
+  if (neg) atomicOr(&st->status, (uint32_t)ST_NEG_COORD);
+  __syncthreads();
+  uint32_t *h = bhist + (size_t)s * passes * 256;
+  for (int i = threadIdx.x; i < passes * 256; i += kBlock)
+    if (sh[i]) atomicAdd(&h[i], sh[i]);
+}
+
 }  // namespace
 
 void launch_prep(RunState *st, uint32_t *zero_words, size_t nzero, hipStream_t s) {
@@ -392,6 +501,32 @@ void launch_encode_int(const int32_t *pts, int64_t n, int dim, int bits, K *keys
   else encode_int_kernel<K, 3><<<grid, kBlock, 0, s>>>(pts, n, bits, keys, hist, passes);
 }
 
+void launch_seg_init(SegState *ss, int64_t B, hipStream_t s) {
+  const unsigned grid = (unsigned)((B + kBlock - 1) / kBlock);
+  seg_init_kernel<<<grid ? grid : 1, kBlock, 0, s>>>(ss, B);
+}
+
+void launch_bbox_seg(const float *pts, int dim, const BatchDev &b, hipStream_t s) {
+  if (b.n_chunk <= 0) return;
+  if (dim == 2) bbox_seg_kernel<2><<<(unsigned)b.n_chunk, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss);
+  else bbox_seg_kernel<3><<<(unsigned)b.n_chunk, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss);
+}
+
+template <typename K>
+void launch_encode_seg(const void *pts, int quantized, int dim, int bits, float effw,
+                       const BatchDev &b, K *keys, uint32_t *bhist, int passes, RunState *st,
+                       hipStream_t s) {
+  if (b.n_chunk <= 0) return;
+  const unsigned g = (unsigned)b.n_chunk;
+  if (quantized) {
+    if (dim == 2) encode_seg_kernel<K, 2, true><<<g, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss, bits, effw, keys, bhist, passes, st);
+    else encode_seg_kernel<K, 3, true><<<g, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss, bits, effw, keys, bhist, passes, st);
+  } else {
+    if (dim == 2) encode_seg_kernel<K, 2, false><<<g, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss, bits, effw, keys, bhist, passes, st);
+    else encode_seg_kernel<K, 3, false><<<g, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss, bits, effw, keys, bhist, passes, st);
+  }
+}
+
 #define PGVD_INST(K)                                                                          \
   template void launch_encode<K>(const float *, int64_t, int, int, float, int, const float *,  \
                                  const float *, K *, int64_t *, uint32_t *, int, RunState *,   \
@@ -400,7 +535,9 @@ void launch_encode_int(const int32_t *pts, int64_t n, int dim, int bits, K *keys
                                            hipStream_t);                                       \
   template void launch_histogram<K>(const K *, int64_t, uint32_t *, int, hipStream_t);         \
   template void launch_encode_int<K>(const int32_t *, int64_t, int, int, K *, uint32_t *, int,  \
-                                     hipStream_t);
+                                     hipStream_t);                                             \
+  template void launch_encode_seg<K>(const void *, int, int, int, float, const BatchDev &, K *, \
+                                     uint32_t *, int, RunState *, hipStream_t);
 PGVD_INST(uint32_t)
 PGVD_INST(uint64_t)
 PGVD_INST(Key128)

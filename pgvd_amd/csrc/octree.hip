@@ -34,14 +34,6 @@ __device__ inline int quadrant(const BrtNodeT<K> &b, int i) {
   return (int)(klow32(kshr((K)b.lcp, rshift)) & ((1u << DIM) - 1));
 }
 
-// Octant of `key` at octree level lvl (the DIM bits right below a prefix of
-// lvl digits).
-template <typename K, int DIM>
-__device__ inline int digit_at(K key, int lvl, int mbits) {
-  const int sh = mbits - (lvl + 1) * DIM;
-  return sh < 0 ? 0 : (int)(klow32(kshr(key, sh)) & ((1u << DIM) - 1));
-}
-
 __device__ inline bool test_bit(const uint32_t *bits, uint32_t i) {
   return (bits[i >> 5] >> (i & 31)) & 1u;
 }
@@ -63,91 +55,55 @@ __device__ inline void write_node(int32_t *octree, uint32_t idx, const int32_t *
   o[NC] = leaf;
 }
 
-// Octree nodes staged per workgroup: the nodes owned by BRT nodes
-// [q0, q0+256) are the contiguous range [ps[q0-1], ps[q0+255]); when it fits
-// here it is built in LDS and written out in coalesced runs.
-constexpr int kStageNodes = 256;   // small: this kernel is latency bound
-
-template <typename K, int DIM>
-__global__ __launch_bounds__(kBlock) void gather_kernel(
-    const BrtNodeT<K> *__restrict__ brt, const uint32_t *__restrict__ ps,
-    const K *__restrict__ keys, const uint32_t *__restrict__ conflict_bits, int mbits,
-    int32_t *__restrict__ octree, uint32_t *__restrict__ node_conflict, int64_t capacity,
-    RunState *st) {
-  if (st->malformed) return;
+// Octree nodes owned by BRT node q: the lowest, excl = ps[q-1], takes the
+// children found by descending through the same-level (ls == 0) BRT nodes
+// below q; chain nodes excl+k (1 <= k < ls_q) each take excl+k-1.
+//   links(x, left, ll, rl) -> false if x's record is unavailable
+//   child(c, ps_c, lsx_c)  -> false if c's entries are unavailable
+// Returns false, writing nothing, when a read was unavailable.
+template <typename K, int DIM, typename Links, typename Child, typename Put>
+__device__ inline bool gather_one(uint32_t q, const BrtNodeT<K> &nq, uint32_t excl, uint32_t lsq,
+                                  Links links, Child child, const K *__restrict__ keys,
+                                  const uint32_t *__restrict__ conflict_bits, int mbits, Put put,
+                                  uint32_t *__restrict__ node_conflict, RunState *st) {
   constexpr int NC = 1 << DIM;
-  constexpr int STRIDE = NC + 1;
-  __shared__ int32_t s_nodes[kStageNodes * STRIDE];
-  const uint32_t m = st->m;
-  const uint32_t O = st->octree_size;
-  const uint32_t q0 = blockIdx.x * kBlock;
-  const uint32_t q = q0 + threadIdx.x;
-  if ((int64_t)O > capacity) {
-    if (q == 0) atomicOr(&st->status, (uint32_t)ST_CAPACITY);
-    return;
-  }
   int32_t ch[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) ch[c] = -1;
-  if (m == 1) {             // single unique key: one node, no children
-    if (q == 0) {
-      write_node<DIM>(octree, 0, ch);
-      if (node_conflict) node_conflict[0] = 0;
-    }
-    return;
-  }
-  if (q0 >= m - 1) return;
-  const uint32_t qlast = min(q0 + kBlock - 1, m - 2);
-  const uint32_t lo = q0 ? ps[q0 - 1] : 0u;
-  const uint32_t span = ps[qlast] - lo;
-  const bool staged = span <= (uint32_t)kStageNodes;   // uniform over the block
-  auto put = [&](uint32_t idx, const int32_t *c) {
-    if (staged) write_node<DIM>(s_nodes, idx - lo, c);
-    else write_node<DIM>(octree, idx, c);
-  };
-  bool active = q <= qlast;
-  uint32_t excl = 0, lsq = 0;
-  if (active) {
-    excl = q ? ps[q - 1] : 0u;
-    lsq = ps[q] - excl;
-    active = (q == 0) || lsq > 0;
-  }
-  if (active) {
-  const BrtNodeT<K> nq = brt[q];
   const int lvl = nq.lcp_length / DIM;
   uint32_t cmask = 0;
   uint32_t stack[8];
   int sp = 0;
-  stack[sp++] = q;
+  uint32_t xl = (uint32_t)nq.left;
+  bool xll = nq.left_leaf, xrl = nq.right_leaf;
   // A well-formed tree visits at most 2^DIM - 1 nodes here; the cap only
   // guards against a malformed tree the BRT stage did not flag.
   int visits = 0;
-  while (sp > 0) {
-    if (++visits > 64) {
-      atomicOr(&st->status, (uint32_t)ST_STACK);
-      break;
-    }
-    const uint32_t x = stack[--sp];
-    const BrtNodeT<K> nx = (x == q) ? nq : brt[x];
+  for (;;) {
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
-      const uint32_t c = (uint32_t)nx.left + side;
-      const bool leaf = side ? nx.right_leaf : nx.left_leaf;
-      if (leaf) {
+      const uint32_t c = xl + side;
+      if (side ? xrl : xll) {
         if (conflict_bits && test_bit(conflict_bits, c))
           cmask |= 1u << digit_at<K, DIM>(keys[c], lvl, mbits);
         continue;
       }
-      const uint32_t psc = ps[c];
-      const uint32_t lsc = psc - ps[c - 1];
-      if (lsc > 0) {
-        ch[quadrant<K, DIM>(brt[c], (int)lsc - 1)] = (int32_t)(psc - 1);
+      uint32_t psc, lxc;
+      if (!child(c, psc, lxc)) return false;
+      if ((lxc & 0xffu) > 0) {
+        ch[lxc >> 8] = (int32_t)(psc - 1);     // c's chain top, in its octant
       } else if (sp < 8) {
         stack[sp++] = c;
       } else {
         atomicOr(&st->status, (uint32_t)ST_STACK);
       }
     }
+    if (sp == 0) break;
+    if (++visits > 64) {
+      atomicOr(&st->status, (uint32_t)ST_STACK);
+      break;
+    }
+    if (!links(stack[--sp], xl, xll, xrl)) return false;
   }
   put(excl, ch);
   if (node_conflict) node_conflict[excl] = cmask;
@@ -158,12 +114,226 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(
     put(excl + k, ch);
     if (node_conflict) node_conflict[excl + k] = 0;
   }
-  }  // active
+  return true;
+}
+
+// Octree nodes staged per workgroup: the nodes owned by BRT nodes
+// [q0, q0+256) are the contiguous range [ps[q0-1], ps[q0+255]); when it fits
+// here it is built in LDS and written out in coalesced runs.
+constexpr int kStageNodes = 256;
+// BRT entries (links, ps, lsx) staged either side of a workgroup's nodes.
+constexpr int kGatherHalo = 256;
+
+// Gather, first pass over the BRT nodes [q0, q0+256) of one tree with m
+// keys (m >= 2).  The workgroup stages links, ps and lsx of BRT nodes
+// [q0-H, q0+256+H) in LDS; a node whose descent leaves that window is handed
+// to defer(true, q) and left to the global-memory pass (whose writes land
+// after this kernel's, so the window pass's placeholder output for it is
+// overwritten).  Every lane calls defer once, converged.
+template <typename K, int DIM, typename Defer>
+__device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
+                                     const uint32_t *__restrict__ ps,
+                                     const uint32_t *__restrict__ lsx, const K *__restrict__ keys,
+                                     const uint32_t *__restrict__ conflict_bits, int mbits,
+                                     uint32_t m, uint32_t q0, int32_t *__restrict__ octree,
+                                     uint32_t *__restrict__ node_conflict, RunState *st,
+                                     Defer defer) {
+  constexpr int NC = 1 << DIM;
+  constexpr int STRIDE = NC + 1;
+  constexpr int H = kGatherHalo, WN = kBlock + 2 * H;
+  __shared__ int32_t s_nodes[kStageNodes * STRIDE];
+  __shared__ uint32_t s_lnk[WN], s_ps[WN], s_lsx[WN];
+  const uint32_t q = q0 + threadIdx.x;
+  const int w0 = (int)q0 - H;
+  for (int r = threadIdx.x; r < WN; r += kBlock) {
+    const int g = w0 + r;
+    if (g >= 0 && g < (int)m) {
+      s_ps[r] = ps[g];
+      s_lsx[r] = lsx[g];
+      if (g < (int)m - 1) {
+        const BrtNodeT<K> &b = brt[g];
+        s_lnk[r] = (uint32_t)b.left | ((uint32_t)b.left_leaf << 30) | ((uint32_t)b.right_leaf << 31);
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t qlast = min(q0 + kBlock - 1, m - 2);
+  const uint32_t lo = q0 ? s_ps[q0 - 1 - w0] : 0u;
+  const uint32_t span = s_ps[qlast - w0] - lo;
+  const bool staged = span <= (uint32_t)kStageNodes;   // uniform over the workgroup
+  bool active = q <= qlast, miss = false;
+  uint32_t excl = 0, lsq = 0;
+  if (active) {
+    excl = q ? s_ps[q - 1 - w0] : 0u;
+    lsq = s_ps[q - w0] - excl;
+    active = (q == 0) || lsq > 0;
+  }
+  if (active) {
+    auto links = [&](uint32_t x, uint32_t &left, bool &ll, bool &rl) -> bool {
+      const unsigned k = (unsigned)((int)x - w0);
+      if (k >= (unsigned)WN) return false;
+      const uint32_t v = s_lnk[k];
+      left = v & 0x3fffffffu;
+      ll = (v >> 30) & 1u;
+      rl = v >> 31;
+      return true;
+    };
+    auto child = [&](uint32_t c, uint32_t &psc, uint32_t &lxc) -> bool {
+      const unsigned k = (unsigned)((int)c - w0);
+      if (k >= (unsigned)WN) return false;
+      psc = s_ps[k];
+      lxc = s_lsx[k];
+      return true;
+    };
+    auto put = [&](uint32_t idx, const int32_t *c) {
+      if (staged) write_node<DIM>(s_nodes, idx - lo, c);
+      else write_node<DIM>(octree, idx, c);
+    };
+    miss = !gather_one<K, DIM>(q, brt[q], excl, lsq, links, child, keys, conflict_bits, mbits,
+                               put, node_conflict, st);
+  }
+  defer(miss, q);
   if (staged) {
     __syncthreads();
     const uint32_t words = span * STRIDE;
     int32_t *out = octree + (size_t)lo * STRIDE;
     for (uint32_t w = threadIdx.x; w < words; w += kBlock) out[w] = s_nodes[w];
+  }
+}
+
+template <int DIM>
+__device__ inline void single_node(int32_t *octree, uint32_t *node_conflict) {
+  constexpr int NC = 1 << DIM;
+  int32_t ch[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) ch[c] = -1;
+  write_node<DIM>(octree, 0, ch);
+  if (node_conflict) node_conflict[0] = 0;
+}
+
+// Deferred entries are appended only by a run that fills the octree, so the
+// counter is still zero when a refill after a capacity failure runs this
+// again.
+template <typename K, int DIM>
+__global__ __launch_bounds__(kBlock) void gather_kernel(
+    const BrtNodeT<K> *__restrict__ brt, const uint32_t *__restrict__ ps,
+    const uint32_t *__restrict__ lsx, const K *__restrict__ keys,
+    const uint32_t *__restrict__ conflict_bits, int mbits, int32_t *__restrict__ octree,
+    uint32_t *__restrict__ node_conflict, int64_t capacity, uint32_t *__restrict__ defer,
+    RunState *st) {
+  if (st->malformed) return;
+  const uint32_t m = st->m;
+  const uint32_t q0 = xcd_segment(blockIdx.x, gridDim.x) * kBlock;
+  if ((int64_t)st->octree_size > capacity) {
+    if (q0 == 0 && threadIdx.x == 0) atomicOr(&st->status, (uint32_t)ST_CAPACITY);
+    return;
+  }
+  if (m == 1) {             // single unique key: one node, no children
+    if (q0 == 0 && threadIdx.x == 0) single_node<DIM>(octree, node_conflict);
+    return;
+  }
+  if (q0 >= m - 1) return;
+  gather_window<K, DIM>(brt, ps, lsx, keys, conflict_bits, mbits, m, q0, octree, node_conflict,
+                        st, [&](bool miss, uint32_t q) {
+                          wave_append(miss, q, &st->counters[CTR_OCT_DEFER], defer);
+                        });
+}
+
+// Batched form: gather tiles of 256 key slots per instance; instance s writes
+// its nodes at octree[oct_off[s]..) with instance-local child indices.
+template <typename K, int DIM>
+__global__ __launch_bounds__(kBlock) void gather_seg_kernel(
+    const BrtNodeT<K> *__restrict__ brt, const uint32_t *__restrict__ ps,
+    const uint32_t *__restrict__ lsx, const SegInfo *__restrict__ seg, TileTable tt,
+    const SegState *__restrict__ ss, int32_t *__restrict__ octree, int64_t capacity,
+    uint32_t *__restrict__ defer, RunState *st) {
+  constexpr int STRIDE = (1 << DIM) + 1;
+  if ((int64_t)st->octree_size > capacity) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&st->status, (uint32_t)ST_CAPACITY);
+    return;
+  }
+  const uint32_t tile = xcd_segment(blockIdx.x, gridDim.x);
+  const uint32_t s = tt.seg[tile];
+  const SegState sst = ss[s];
+  const uint32_t m = sst.m;
+  const uint32_t q0 = (tile - tt.first[s]) * kBlock;
+  int32_t *oct = octree + (size_t)sst.oct_off * STRIDE;
+  if (m == 1) {
+    if (q0 == 0 && threadIdx.x == 0) single_node<DIM>(oct, nullptr);
+    return;
+  }
+  if (q0 >= m - 1) return;
+  const uint32_t kofs = seg[s].kofs;
+  gather_window<K, DIM>(brt + kofs, ps + kofs, lsx + kofs, (const K *)nullptr, nullptr, 0, m, q0,
+                        oct, nullptr, st, [&](bool miss, uint32_t q) {
+                          wave_append(miss, kofs + q, &st->counters[CTR_OCT_DEFER], defer);
+                        });
+}
+
+// Gather, second pass: the deferred BRT nodes, reading global memory.
+template <typename K, int DIM>
+__global__ __launch_bounds__(kBlock) void gather_defer_kernel(
+    const BrtNodeT<K> *__restrict__ brt, const uint32_t *__restrict__ ps,
+    const uint32_t *__restrict__ lsx, const K *__restrict__ keys,
+    const uint32_t *__restrict__ conflict_bits, int mbits, int32_t *__restrict__ octree,
+    uint32_t *__restrict__ node_conflict, int64_t capacity, const uint32_t *__restrict__ defer,
+    RunState *st) {
+  if (st->malformed || (int64_t)st->octree_size > capacity) return;
+  const uint32_t cnt = st->counters[CTR_OCT_DEFER];
+  auto links = [&](uint32_t x, uint32_t &left, bool &ll, bool &rl) -> bool {
+    const BrtNodeT<K> &b = brt[x];
+    left = (uint32_t)b.left;
+    ll = b.left_leaf;
+    rl = b.right_leaf;
+    return true;
+  };
+  auto child = [&](uint32_t c, uint32_t &psc, uint32_t &lxc) -> bool {
+    psc = ps[c];
+    lxc = lsx[c];
+    return true;
+  };
+  auto put = [&](uint32_t idx, const int32_t *c) { write_node<DIM>(octree, idx, c); };
+  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
+    const uint32_t q = defer[k];
+    const uint32_t excl = q ? ps[q - 1] : 0u;
+    (void)gather_one<K, DIM>(q, brt[q], excl, ps[q] - excl, links, child, keys, conflict_bits,
+                             mbits, put, node_conflict, st);
+  }
+}
+
+template <typename K, int DIM>
+__global__ __launch_bounds__(kBlock) void gather_seg_defer_kernel(
+    const BrtNodeT<K> *__restrict__ brt, const uint32_t *__restrict__ ps,
+    const uint32_t *__restrict__ lsx, const uint32_t *__restrict__ kofs_all, uint32_t B,
+    const SegState *__restrict__ ss, int32_t *__restrict__ octree, int64_t capacity,
+    const uint32_t *__restrict__ defer, RunState *st) {
+  constexpr int STRIDE = (1 << DIM) + 1;
+  if ((int64_t)st->octree_size > capacity) return;
+  const uint32_t cnt = st->counters[CTR_OCT_DEFER];
+  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
+    const uint32_t slot = defer[k];
+    const uint32_t s = find_seg(kofs_all, B, slot);
+    const uint32_t kofs = kofs_all[s];
+    const uint32_t q = slot - kofs;
+    const BrtNodeT<K> *b = brt + kofs;
+    const uint32_t *p = ps + kofs;
+    const uint32_t *lx = lsx + kofs;
+    int32_t *oct = octree + (size_t)ss[s].oct_off * STRIDE;
+    auto links = [&](uint32_t x, uint32_t &left, bool &ll, bool &rl) -> bool {
+      left = (uint32_t)b[x].left;
+      ll = b[x].left_leaf;
+      rl = b[x].right_leaf;
+      return true;
+    };
+    auto child = [&](uint32_t c, uint32_t &psc, uint32_t &lxc) -> bool {
+      psc = p[c];
+      lxc = lx[c];
+      return true;
+    };
+    auto put = [&](uint32_t idx, const int32_t *c) { write_node<DIM>(oct, idx, c); };
+    const uint32_t excl = q ? p[q - 1] : 0u;
+    (void)gather_one<K, DIM>(q, b[q], excl, p[q] - excl, links, child, (const K *)nullptr,
+                             nullptr, 0, put, nullptr, st);
   }
 }
 
@@ -257,18 +427,24 @@ __device__ inline void scatter_one(uint32_t i, uint32_t m, const BrtNodeT<K> *__
 }
 
 template <typename K, int DIM>
-void launch_dim(const BrtNodeT<K> *brt, const int32_t *parent, const uint32_t *ps, const K *keys,
-                const uint32_t *conflict_bits, int64_t max_m, int mbits, int32_t *octree,
-                uint32_t *node_conflict, int64_t capacity, int force_scatter, RunState *st,
+void launch_dim(const BrtNodeT<K> *brt, const int32_t *parent, const uint32_t *ps,
+                const uint32_t *lsx, const K *keys, const uint32_t *conflict_bits,
+                int64_t max_m, int mbits, int32_t *octree, uint32_t *node_conflict,
+                int64_t capacity, int force_scatter, uint32_t *defer, RunState *st,
                 hipStream_t s) {
   const int64_t nodes = max_m > 1 ? max_m - 1 : 1;
   const unsigned grid = (unsigned)((nodes + kBlock - 1) / kBlock);
-  if (!force_scatter)
-    gather_kernel<K, DIM><<<grid, kBlock, 0, s>>>(brt, ps, keys, conflict_bits, mbits, octree,
-                                                  node_conflict, capacity, st);
+  const unsigned sgrid = grid < 1024 ? grid : 1024;
+  if (!force_scatter) {
+    gather_kernel<K, DIM><<<grid, kBlock, 0, s>>>(brt, ps, lsx, keys, conflict_bits, mbits,
+                                                  octree, node_conflict, capacity, defer, st);
+    const unsigned dgrid = grid < 2048 ? grid : 2048;
+    gather_defer_kernel<K, DIM><<<dgrid, kBlock, 0, s>>>(brt, ps, lsx, keys, conflict_bits,
+                                                         mbits, octree, node_conflict,
+                                                         capacity, defer, st);
+  }
   // The scatter path is a no-op unless forced or st->malformed; its kernels
   // loop over their ranges, so a capped grid keeps the no-op cheap.
-  const unsigned sgrid = grid < 1024 ? grid : 1024;
   scatter_init_kernel<DIM><<<sgrid, kBlock, 0, s>>>(octree, node_conflict, capacity,
                                                     force_scatter, st);
   scatter_kernel<K, DIM><<<sgrid, kBlock, 0, s>>>(brt, parent, ps, keys, conflict_bits, mbits,
@@ -321,21 +497,45 @@ void launch_local_splits(const BrtNodeT<K> *brt, int64_t m, int dim, uint32_t *l
 
 template <typename K>
 void launch_octree(const BrtNodeT<K> *brt, const int32_t *parent, const uint32_t *ps,
-                   const K *keys, const uint32_t *conflict_bits, int64_t max_m, int dim,
-                   int mbits, int32_t *octree, uint32_t *node_conflict, int64_t capacity,
-                   int force_scatter, RunState *st, hipStream_t s) {
+                   const uint32_t *lsx, const K *keys, const uint32_t *conflict_bits,
+                   int64_t max_m, int dim, int mbits, int32_t *octree,
+                   uint32_t *node_conflict, int64_t capacity, int force_scatter,
+                   uint32_t *defer, RunState *st, hipStream_t s) {
   if (dim == 2)
-    launch_dim<K, 2>(brt, parent, ps, keys, conflict_bits, max_m, mbits, octree, node_conflict,
-                     capacity, force_scatter, st, s);
+    launch_dim<K, 2>(brt, parent, ps, lsx, keys, conflict_bits, max_m, mbits, octree,
+                     node_conflict, capacity, force_scatter, defer, st, s);
   else
-    launch_dim<K, 3>(brt, parent, ps, keys, conflict_bits, max_m, mbits, octree, node_conflict,
-                     capacity, force_scatter, st, s);
+    launch_dim<K, 3>(brt, parent, ps, lsx, keys, conflict_bits, max_m, mbits, octree,
+                     node_conflict, capacity, force_scatter, defer, st, s);
+}
+
+template <typename K, int DIM>
+void gather_seg_dim(const BrtNodeT<K> *brt, const uint32_t *ps, const uint32_t *lsx,
+                    const BatchDev &b, int32_t *octree, int64_t capacity, uint32_t *defer,
+                    RunState *st, hipStream_t s) {
+  gather_seg_kernel<K, DIM><<<(unsigned)b.n_node, kBlock, 0, s>>>(brt, ps, lsx, b.seg, b.node, b.ss,
+                                                                  octree, capacity, defer, st);
+  gather_seg_defer_kernel<K, DIM><<<2048, kBlock, 0, s>>>(brt, ps, lsx, b.kofs, (uint32_t)b.B,
+                                                          b.ss, octree, capacity, defer, st);
+}
+
+template <typename K>
+void launch_gather_seg(const BrtNodeT<K> *brt, const uint32_t *ps, const uint32_t *lsx, int dim,
+                       const BatchDev &b, int32_t *octree, int64_t capacity, uint32_t *defer,
+                       RunState *st, hipStream_t s) {
+  if (b.n_node <= 0) return;
+  if (dim == 2) gather_seg_dim<K, 2>(brt, ps, lsx, b, octree, capacity, defer, st, s);
+  else gather_seg_dim<K, 3>(brt, ps, lsx, b, octree, capacity, defer, st, s);
 }
 
 #define PGVD_INST(K)                                                                          \
   template void launch_octree<K>(const BrtNodeT<K> *, const int32_t *, const uint32_t *,       \
-                                 const K *, const uint32_t *, int64_t, int, int, int32_t *,    \
-                                 uint32_t *, int64_t, int, RunState *, hipStream_t);   \
+                                 const uint32_t *, const K *, const uint32_t *, int64_t, int,  \
+                                 int, int32_t *, uint32_t *, int64_t, int, uint32_t *,         \
+                                 RunState *, hipStream_t);                                     \
+  template void launch_gather_seg<K>(const BrtNodeT<K> *, const uint32_t *, const uint32_t *,  \
+                                     int, const BatchDev &, int32_t *, int64_t, uint32_t *,    \
+                                     RunState *, hipStream_t);                                 \
   template void launch_local_splits<K>(const BrtNodeT<K> *, int64_t, int, uint32_t *,          \
                                        uint64_t *, hipStream_t);
 PGVD_INST(uint32_t)

@@ -21,13 +21,6 @@ namespace {
 
 constexpr int kRows = kScanTile / kBlock;  // 16
 
-enum {
-  CTR_UNIQUE = kMaxPasses,
-  CTR_SPLITS = kMaxPasses + 1,
-  CTR_SCAN = kMaxPasses + 2,
-  CTR_LS = kMaxPasses + 4
-};
-
 template <typename K> __device__ inline K shfl_up1(K v);
 template <> __device__ inline uint32_t shfl_up1<uint32_t>(uint32_t v) { return __shfl_up(v, 1, 64); }
 template <> __device__ inline uint64_t shfl_up1<uint64_t>(uint64_t v) { return __shfl_up(v, 1, 64); }
@@ -191,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void ls_scan_kernel(const uint32_t *__restr
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
     const uint32_t e = base + r * kBlock + t;
-    v[r] = (e < m) ? ls[e] : 0u;
+    v[r] = (e < m) ? (ls[e] & 0xffu) : 0u;
   }
   const uint32_t total = striped_excl_scan<kRows, uint32_t>(v, ex, s_w);
   const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl);
@@ -201,6 +194,115 @@ __global__ __launch_bounds__(kBlock) void ls_scan_kernel(const uint32_t *__restr
     if (e < m) ps[e] = (uint32_t)(excl + ex[r] + v[r]);
   }
   if (tile == ntiles - 1 && t == 0) st->octree_size = (uint32_t)(excl + total);
+}
+
+// ---- batched form ----
+// Per-instance unique: instance s reads keys[kofs .. kofs+n) (sorted) and
+// writes [zero?] + its distinct keys to out[kofs..); scan tiles cover n+1
+// slots per instance, the last one writes ss[s].m.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void unique_seg_kernel(const K *__restrict__ keys,
+                                                            K *__restrict__ out,
+                                                            const SegInfo *__restrict__ seg,
+                                                            TileTable tt, SegState *ss,
+                                                            uint64_t *lookback, RunState *st) {
+  __shared__ uint32_t s_w[kRows * 4];
+  __shared__ uint64_t s_excl;
+  __shared__ uint32_t s_tile;
+  const int t = threadIdx.x, lane = t & 63;
+  if (t == 0) s_tile = atomicAdd(&st->counters[CTR_UNIQUE], 1u);
+  __syncthreads();
+  const int tile = (int)s_tile;
+  const uint32_t s = tt.seg[tile];
+  const int lo = (int)tt.first[s];
+  const SegInfo si = seg[s];
+  const uint32_t n = si.n;
+  const K *kin = keys + si.kofs;
+  K *kout = out + si.kofs;
+  const uint32_t base = (uint32_t)(tile - lo) * kScanTile;
+  K k[kRows];
+  bool head[kRows];
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const uint32_t e = base + r * kBlock + t;
+    k[r] = (e < n) ? kin[e] : kzero_value<K>();
+    K prev = shfl_up1(k[r]);
+    if (lane == 0 && e > 0 && e < n) prev = kin[e - 1];
+    head[r] = (e < n) && ((e == 0) ? (!si.prepend || !kzero(k[r])) : !keq(k[r], prev));
+  }
+  uint32_t ex[kRows];
+  const uint32_t total = striped_flag_scan<kRows>(head, ex, s_w);
+  const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl, lo);
+  const uint32_t obase = (uint32_t)(si.prepend + excl);
+#pragma unroll
+  for (int r = 0; r < kRows; ++r)
+    if (head[r]) kout[obase + ex[r]] = k[r];
+  if (tile == lo && t == 0 && si.prepend) kout[0] = kzero_value<K>();
+  if (tile == (int)tt.first[s + 1] - 1 && t == 0) ss[s].m = (uint32_t)(si.prepend + excl + total);
+}
+
+// Per-instance inclusive scan of the local splits (low byte of lsx) over
+// [0, m_s); the last tile in use writes ss[s].octree_size.
+__global__ __launch_bounds__(kBlock) void ls_scan_seg_kernel(const uint32_t *__restrict__ lsx,
+                                                             uint32_t *__restrict__ ps,
+                                                             const SegInfo *__restrict__ seg,
+                                                             TileTable tt, SegState *ss,
+                                                             uint64_t *lookback, RunState *st) {
+  __shared__ uint32_t s_w[kRows * 4];
+  __shared__ uint64_t s_excl;
+  __shared__ uint32_t s_tile;
+  const int t = threadIdx.x;
+  if (t == 0) s_tile = atomicAdd(&st->counters[CTR_LS], 1u);
+  __syncthreads();
+  const int tile = (int)s_tile;
+  const uint32_t s = tt.seg[tile];
+  const int lo = (int)tt.first[s];
+  const uint32_t m = ss[s].m;
+  const int used = (int)((m + kScanTile - 1) / kScanTile);
+  if (tile - lo >= used) return;    // no later tile of this instance depends on it
+  const uint32_t kofs = seg[s].kofs;
+  const uint32_t base = (uint32_t)(tile - lo) * kScanTile;
+  uint32_t v[kRows], ex[kRows];
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const uint32_t e = base + r * kBlock + t;
+    v[r] = (e < m) ? (lsx[kofs + e] & 0xffu) : 0u;
+  }
+  const uint32_t total = striped_excl_scan<kRows, uint32_t>(v, ex, s_w);
+  const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl, lo);
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const uint32_t e = base + r * kBlock + t;
+    if (e < m) ps[kofs + e] = (uint32_t)(excl + ex[r] + v[r]);
+  }
+  if (tile - lo == used - 1 && t == 0) ss[s].octree_size = (uint32_t)(excl + total);
+}
+
+// One workgroup: oct_off = exclusive scan of the instances' octree sizes.
+__global__ __launch_bounds__(kBlock) void oct_offsets_kernel(SegState *ss, int64_t B,
+                                                             RunState *st) {
+  __shared__ uint32_t s_w[kRows * 4];
+  uint64_t carry = 0;
+  for (int64_t b0 = 0; b0 < B; b0 += kScanTile) {
+    uint32_t v[kRows], ex[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int64_t i = b0 + r * kBlock + threadIdx.x;
+      v[r] = (i < B) ? ss[i].octree_size : 0u;
+    }
+    const uint32_t total = striped_excl_scan<kRows, uint32_t>(v, ex, s_w);
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int64_t i = b0 + r * kBlock + threadIdx.x;
+      if (i < B) ss[i].oct_off = (uint32_t)(carry + ex[r]);
+    }
+    carry += total;
+    __syncthreads();    // s_w is reused by the next chunk
+  }
+  if (threadIdx.x == 0) {
+    if (carry > 0xFFFFFFFFull) atomicOr(&st->status, (uint32_t)ST_TOO_LARGE);
+    st->octree_size = (uint32_t)carry;
+  }
 }
 
 }  // namespace
@@ -241,11 +343,32 @@ void launch_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint64_t *loo
   scan_u32_kernel<<<(unsigned)tiles, kBlock, 0, s>>>(in, out, (uint32_t)n, lookback, st);
 }
 
+template <typename K>
+void launch_unique_seg(const K *keys, K *out, const BatchDev &b, uint64_t *lookback,
+                       RunState *st, hipStream_t s) {
+  if (b.n_scan <= 0) return;
+  unique_seg_kernel<K><<<(unsigned)b.n_scan, kBlock, 0, s>>>(keys, out, b.seg, b.scan, b.ss,
+                                                            lookback, st);
+}
+
+void launch_ls_scan_seg(const uint32_t *lsx, uint32_t *ps, const BatchDev &b,
+                        uint64_t *lookback, RunState *st, hipStream_t s) {
+  if (b.n_scan <= 0) return;
+  ls_scan_seg_kernel<<<(unsigned)b.n_scan, kBlock, 0, s>>>(lsx, ps, b.seg, b.scan, b.ss,
+                                                          lookback, st);
+}
+
+void launch_oct_offsets(const BatchDev &b, RunState *st, hipStream_t s) {
+  oct_offsets_kernel<<<1, kBlock, 0, s>>>(b.ss, b.B, st);
+}
+
 #define PGVD_INST(K)                                                                          \
   template void launch_unique<K>(const K *, const uint32_t *, int64_t, int, K *, uint32_t *,   \
                                  uint64_t *, RunState *, hipStream_t);                         \
   template void launch_splits_scan<K>(const BrtNodeT<K> *, int32_t *, uint32_t *, uint32_t *,  \
-                                      int64_t, int, int, uint64_t *, RunState *, hipStream_t);
+                                      int64_t, int, int, uint64_t *, RunState *, hipStream_t);  \
+  template void launch_unique_seg<K>(const K *, K *, const BatchDev &, uint64_t *, RunState *, \
+                                     hipStream_t);
 PGVD_INST(uint32_t)
 PGVD_INST(uint64_t)
 PGVD_INST(Key128)

@@ -38,36 +38,37 @@ __device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t *s_wsum) {
   return off + incl - x;
 }
 
+// One onesweep tile: the nvalid (<= IPT*256) keys at kin[base..) of a run
+// of tiles [lo, ...) whose keys go to kout[seg0..) in digit order.  `tile` is
+// the tile's index in the look-back array (tiles lo..tile-1 precede it in the
+// same run); ghist is the run's digit histogram for this pass.
 template <typename K, bool PAY, int IPT>
-__global__ __launch_bounds__(kBlock) void onesweep_kernel(
-    const K *__restrict__ kin, K *__restrict__ kout, const uint32_t *__restrict__ vin,
-    uint32_t *__restrict__ vout, uint32_t n, int shift, const uint32_t *__restrict__ ghist,
-    uint32_t *__restrict__ lookback, uint32_t *tile_counter) {
+__device__ inline void onesweep_tile(const K *__restrict__ kin, K *__restrict__ kout,
+                                     const uint32_t *__restrict__ vin, uint32_t *__restrict__ vout,
+                                     uint32_t base, uint32_t nvalid, uint32_t tile, uint32_t lo,
+                                     uint32_t seg0, int shift, const uint32_t *__restrict__ ghist,
+                                     uint32_t *__restrict__ lookback) {
   constexpr int TILE = kBlock * IPT;
   __shared__ uint32_t s_cnt[kWaves * 256];
   __shared__ uint32_t s_dstart[256];
   __shared__ uint32_t s_gbase[256];
   __shared__ uint32_t s_wsum_a[kWaves], s_wsum_b[kWaves];
-  __shared__ uint32_t s_tile;
   __shared__ K s_keys[TILE];
   __shared__ uint32_t s_vals[PAY ? TILE : 1];
 
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  if (t == 0) s_tile = atomicAdd(tile_counter, 1u);
   for (int i = t; i < kWaves * 256; i += kBlock) s_cnt[i] = 0;
   __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint32_t base = tile * TILE;
-  const uint32_t wbase = base + (uint32_t)w * 64u * IPT;
+  const uint32_t wofs = (uint32_t)w * 64u * IPT;
 
   K k[IPT];
   uint32_t v[IPT];
   uint32_t rk[IPT];
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
-    const uint32_t e = wbase + j * 64 + lane;
-    k[j] = (e < n) ? kin[e] : kzero_value<K>();
-    if (PAY) v[j] = (e < n) ? vin[e] : 0u;
+    const uint32_t le = wofs + j * 64 + lane;
+    k[j] = (le < nvalid) ? kin[base + le] : kzero_value<K>();
+    if (PAY) v[j] = (le < nvalid) ? vin[base + le] : 0u;
   }
 
   // Rank: keys with equal digits within a wave found by 8 ballots; the
@@ -76,8 +77,8 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   const uint64_t ltm = lanemask_lt();
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
-    const uint32_t e = wbase + j * 64 + lane;
-    const bool valid = e < n;
+    const uint32_t le = wofs + j * 64 + lane;
+    const bool valid = le < nvalid;
     const uint32_t d = kdigit(k[j], shift);
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     total += c;
   }
   uint32_t *lb = lookback + (size_t)tile * 256;
-  st_rel_u32(lb + t, (tile == 0 ? kFlagIncl : kFlagAgg) | total);
+  st_rel_u32(lb + t, (tile == lo ? kFlagIncl : kFlagAgg) | total);
 
   const uint32_t dstart = block_excl_scan(total, s_wsum_a);
   s_dstart[t] = dstart;
@@ -114,14 +115,14 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   // trip, consume them nearest first until an inclusive prefix; a tile that
   // has not published yet is re-read on the next round.
   uint32_t excl = 0;
-  if (tile > 0) {
+  if (tile > lo) {
     constexpr int kBatch = 8;
     int kk = (int)tile - 1;
     for (;;) {
       uint32_t wv[kBatch];
 #pragma unroll
       for (int b = 0; b < kBatch; ++b)
-        wv[b] = (kk - b >= 0) ? ld_acq_u32(lookback + (size_t)(kk - b) * 256 + t) : kFlagIncl;
+        wv[b] = (kk - b >= (int)lo) ? ld_acq_u32(lookback + (size_t)(kk - b) * 256 + t) : kFlagIncl;
       int used = 0;
       bool done = false;
 #pragma unroll
@@ -139,13 +140,13 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     }
     st_rel_u32(lb + t, kFlagIncl | (excl + total));
   }
-  s_gbase[t] = gexcl + excl - dstart;
+  s_gbase[t] = seg0 + gexcl + excl - dstart;
   __syncthreads();
 
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
-    const uint32_t e = wbase + j * 64 + lane;
-    if (e < n) {
+    const uint32_t le = wofs + j * 64 + lane;
+    if (le < nvalid) {
       const uint32_t d = kdigit(k[j], shift);
       const uint32_t pos = s_dstart[d] + s_cnt[w * 256 + d] + rk[j];
       s_keys[pos] = k[j];
@@ -154,7 +155,6 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   }
   __syncthreads();
 
-  const uint32_t nvalid = (n - base) < (uint32_t)TILE ? (n - base) : (uint32_t)TILE;
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t p = j * kBlock + t;
@@ -165,6 +165,48 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
       if (PAY) vout[g] = s_vals[p];
     }
   }
+}
+
+// Tile ids come from an atomic counter, so every tile a workgroup looks
+// back at was claimed by a workgroup that is already running.
+__device__ inline uint32_t claim_tile(uint32_t *counter) {
+  __shared__ uint32_t s_tile;
+  if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+  __syncthreads();
+  return s_tile;
+}
+
+template <typename K, bool PAY, int IPT>
+__global__ __launch_bounds__(kBlock) void onesweep_kernel(
+    const K *__restrict__ kin, K *__restrict__ kout, const uint32_t *__restrict__ vin,
+    uint32_t *__restrict__ vout, uint32_t n, int shift, const uint32_t *__restrict__ ghist,
+    uint32_t *__restrict__ lookback, uint32_t *tile_counter) {
+  constexpr uint32_t TILE = kBlock * IPT;
+  const uint32_t tile = claim_tile(tile_counter);
+  const uint32_t base = tile * TILE;
+  const uint32_t nvalid = (n - base) < TILE ? (n - base) : TILE;
+  onesweep_tile<K, PAY, IPT>(kin, kout, vin, vout, base, nvalid, tile, 0u, 0u, shift, ghist,
+                             lookback);
+}
+
+// Segmented form: instance s owns keys [seg.kofs, +seg.n) and sort tiles
+// [tt.first[s], tt.first[s+1]); each instance is sorted on its own, in one
+// launch per digit for the whole batch.
+template <typename K, int IPT>
+__global__ __launch_bounds__(kBlock) void onesweep_seg_kernel(
+    const K *__restrict__ kin, K *__restrict__ kout, const SegInfo *__restrict__ seg,
+    TileTable tt, int passes, int pass, const uint32_t *__restrict__ bhist,
+    uint32_t *__restrict__ lookback, uint32_t *tile_counter) {
+  constexpr uint32_t TILE = kBlock * IPT;
+  const uint32_t tile = claim_tile(tile_counter);
+  const uint32_t s = tt.seg[tile];
+  const uint32_t lo = tt.first[s];
+  const SegInfo si = seg[s];
+  const uint32_t lofs = (tile - lo) * TILE;
+  const uint32_t nvalid = (si.n - lofs) < TILE ? (si.n - lofs) : TILE;
+  onesweep_tile<K, false, IPT>(kin, kout, nullptr, nullptr, si.kofs + lofs, nvalid, tile, lo,
+                               si.kofs, 8 * pass, bhist + ((size_t)s * passes + pass) * 256,
+                               lookback);
 }
 
 }  // namespace
@@ -215,10 +257,43 @@ int launch_onesweep(K *keys, K *keys_alt, const uint32_t *vals_in, uint32_t *val
   return passes & 1;
 }
 
+bool onesweep_seg_ipt_ok(int key_bytes, int ipt) {
+  return key_bytes == 16 ? (ipt == 4 || ipt == 8) : (ipt == 8 || ipt == 12 || ipt == 16);
+}
+
+template <typename K, int IPT>
+void seg_passes(K *&src, K *&dst, const BatchDev &b, int passes, const uint32_t *bhist,
+                uint32_t *lookback, RunState *st, hipStream_t s) {
+  for (int r = 0; r < passes; ++r) {
+    uint32_t *lb = lookback + (size_t)r * (size_t)b.n_sort * 256;
+    onesweep_seg_kernel<K, IPT><<<(unsigned)b.n_sort, kBlock, 0, s>>>(
+        src, dst, b.seg, b.sort, passes, r, bhist, lb, &st->counters[r]);
+    K *tk = src; src = dst; dst = tk;
+  }
+}
+
+template <typename K>
+int launch_onesweep_seg(K *keys, K *keys_alt, const BatchDev &b, int passes,
+                        const uint32_t *bhist, uint32_t *lookback, RunState *st, hipStream_t s) {
+  if (passes <= 0 || b.n_sort <= 0) return 0;
+  K *src = keys, *dst = keys_alt;
+  if constexpr (sizeof(K) == 16) {
+    if (b.sort_ipt == 4) seg_passes<K, 4>(src, dst, b, passes, bhist, lookback, st, s);
+    else seg_passes<K, 8>(src, dst, b, passes, bhist, lookback, st, s);
+  } else {
+    if (b.sort_ipt == 8) seg_passes<K, 8>(src, dst, b, passes, bhist, lookback, st, s);
+    else if (b.sort_ipt == 12) seg_passes<K, 12>(src, dst, b, passes, bhist, lookback, st, s);
+    else seg_passes<K, 16>(src, dst, b, passes, bhist, lookback, st, s);
+  }
+  return passes & 1;
+}
+
 #define PGVD_INST(K)                                                                          \
   template int launch_onesweep<K>(K *, K *, const uint32_t *, uint32_t *, uint32_t *, int64_t,  \
                                   int, const uint32_t *, uint32_t *, RunState *, hipStream_t,   \
-                                  hipEvent_t *);
+                                  hipEvent_t *);                                              \
+  template int launch_onesweep_seg<K>(K *, K *, const BatchDev &, int, const uint32_t *,       \
+                                      uint32_t *, RunState *, hipStream_t);
 PGVD_INST(uint32_t)
 PGVD_INST(uint64_t)
 PGVD_INST(Key128)

@@ -16,16 +16,16 @@ constexpr uint64_t kSIncl = 2ull << 62;
 constexpr uint64_t kSVal = (1ull << 62) - 1;
 
 // Called by every lane of one wave after the tile's aggregate is published.
-// Returns the exclusive prefix of `tile`, resolving up to 64 predecessors per
-// step.
-__device__ inline uint64_t lookback_excl(uint64_t *status, int tile) {
+// Returns the exclusive prefix of `tile` within the run of tiles starting at
+// `lo`, resolving up to 64 predecessors per step.
+__device__ inline uint64_t lookback_excl(uint64_t *status, int tile, int lo = 0) {
   const int lane = threadIdx.x & 63;
   uint64_t excl = 0;
   int base = tile - 1;
-  while (base >= 0) {
+  while (base >= lo) {
     const int k = base - lane;
-    uint64_t wv = kSIncl;  // before tile 0: an inclusive zero
-    if (k >= 0) {
+    uint64_t wv = kSIncl;  // before the run's first tile: an inclusive zero
+    if (k >= lo) {
       for (;;) {
         wv = ld_acq_u64(status + k);
         if (wv & ~kSVal) break;
@@ -47,13 +47,13 @@ __device__ inline uint64_t lookback_excl(uint64_t *status, int tile) {
 }
 
 // Publishes the tile aggregate, resolves and publishes the inclusive
-// prefix; returns the tile's exclusive prefix to every thread.  All threads
-// of the block must call it.
+// prefix; returns the tile's exclusive prefix (within the run of tiles
+// starting at `lo`) to every thread.  All threads of the block must call it.
 __device__ inline uint64_t tile_prefix(uint64_t *status, int tile, uint64_t total,
-                                       uint64_t *s_excl) {
-  if (threadIdx.x == 0) st_rel_u64(status + tile, (tile == 0 ? kSIncl : kSAgg) | total);
-  if (tile > 0 && threadIdx.x < 64) {
-    const uint64_t e = lookback_excl(status, tile);
+                                       uint64_t *s_excl, int lo = 0) {
+  if (threadIdx.x == 0) st_rel_u64(status + tile, (tile == lo ? kSIncl : kSAgg) | total);
+  if (tile > lo && threadIdx.x < 64) {
+    const uint64_t e = lookback_excl(status, tile, lo);
     if (threadIdx.x == 0) {
       st_rel_u64(status + tile, kSIncl | (e + total));
       *s_excl = e;

@@ -1,7 +1,8 @@
 // util.hip -- small utility kernels: word fill (used instead of
 // hipMemsetAsync so that captured graphs replay correctly), a plain copy
-// kernel (also the calibration kernel for HBM byte counters), and setting
-// the run sizes from a prefix-sum array.
+// kernel (also the calibration kernel for HBM byte counters), setting the
+// run sizes from a prefix-sum array, and a position-dependent digest of a
+// device range (result comparison without a download).
 #include "kernels.hpp"
 
 namespace pgvd {
@@ -28,6 +29,24 @@ __global__ void set_sizes_kernel(RunState *st, const uint32_t *ps, int64_t m) {
   st->octree_size = ps[m - 1];
 }
 
+__device__ inline uint64_t mix64(uint64_t z) {   // splitmix64 finaliser
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// out += sum over 32-bit words w_i of mix64(w_i | i << 32), mod 2^64.
+__global__ __launch_bounds__(kBlock) void digest_kernel(const uint32_t *__restrict__ p, size_t n,
+                                                        unsigned long long *out) {
+  uint64_t acc = 0;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+    acc += mix64((uint64_t)p[i] | ((uint64_t)i << 32));
+  acc = wave_reduce_add(acc);
+  if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
+
 inline unsigned grid_for(size_t n) {
   size_t g = (n + kBlock * 4 - 1) / (kBlock * 4);
   if (g < 1) g = 1;
@@ -48,6 +67,12 @@ void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
   copy_kernel<<<grid_for(n16 ? n16 : 1), kBlock, 0, s>>>(
       static_cast<uint4 *>(dst), static_cast<const uint4 *>(src), n16,
       static_cast<uint8_t *>(dst) + n16 * 16, static_cast<const uint8_t *>(src) + n16 * 16, tail);
+}
+
+void launch_digest(const void *p, size_t words, uint64_t *out, bool zero_first, hipStream_t s) {
+  unsigned long long *o = reinterpret_cast<unsigned long long *>(out);
+  if (zero_first) zero_u64_kernel<<<1, 1, 0, s>>>(o);
+  if (words) digest_kernel<<<grid_for(words), kBlock, 0, s>>>(static_cast<const uint32_t *>(p), words, o);
 }
 
 void launch_set_sizes(RunState *st, const uint32_t *ps, int64_t m, hipStream_t s) {

@@ -6,6 +6,9 @@
     star_loops(n, ...)           64 closed star-shaped loops in random planes,
                                  sampled by arc length, with loop-id labels
     batch_sizes(count, ...)      log-uniform instance sizes for batched builds
+    batch_instance(i, n, seed)   instance i of a batch: uniform (even i) or
+                                 8 Gaussian clusters (odd i)
+    batch_points(sizes, seed)    all instances, concatenated
 """
 from __future__ import annotations
 
@@ -80,3 +83,29 @@ def star_loops(n: int, loops: int = 64, verts: int = 256, seed: int = 4):
 def batch_sizes(count: int = 4096, lo: float = 1e3, hi: float = 1e6, seed: int = 5) -> np.ndarray:
     rng = np.random.default_rng(seed)
     return np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(np.int64)
+
+
+def batch_instance(i: int, n: int, seed: int = 5, dim: int = 3) -> np.ndarray:
+    """Instance i of a batch: even i uniform in [-1,1]^dim, odd i a mixture
+    of 8 Gaussian clusters (centres U[-0.9,0.9], sigma U[0.01,0.1]),
+    clipped to [-1,1]; seeded by (seed, i)."""
+    rng = np.random.default_rng([seed, i])
+    if i % 2 == 0:
+        return rng.random((n, dim), dtype=np.float32) * 2 - 1
+    centres = rng.uniform(-0.9, 0.9, (8, dim)).astype(np.float32)
+    sig = rng.uniform(0.01, 0.1, 8).astype(np.float32)
+    comp = rng.integers(0, 8, n)
+    pts = centres[comp] + rng.standard_normal((n, dim), dtype=np.float32) * sig[comp, None]
+    return np.clip(pts, -1, 1)
+
+
+def batch_points(sizes, seed: int = 5, dim: int = 3) -> np.ndarray:
+    """All instances of a batch, one after another (float32, sum(sizes) x dim)."""
+    sizes = np.asarray(sizes, np.int64)
+    out = np.empty((int(sizes.sum()), dim), np.float32)
+    off = 0
+    for i, n in enumerate(sizes):
+        out[off: off + n] = batch_instance(i, int(n), seed, dim)
+        off += int(n)
+    return out
+
