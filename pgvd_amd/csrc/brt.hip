@@ -16,86 +16,36 @@
 //     (claimer + 1) into a zeroed array, which gives the same winner.
 // delta is mbits - bitlen(a ^ b), the value compute_lcp_length's shift
 // search (:54-88) returns for every pair of distinct keys.
+#include "brt_eval.hpp"
 #include "tilescan.hpp"
 
 namespace pgvd {
 
 namespace {
 
-// Result of one BRT node.
-template <typename K>
-struct BrtOut {
-  BrtNodeT<K> nd;
-  int lcp_min;    // smaller neighbour delta (-1 for node 0)
-  bool dup;       // keys[i] == keys[i+1]
-  bool quirk;     // the FP32 split step chose a split the exact ceil does not
-};
-
-// One BRT node from a delta oracle: dl(x) = delta(keys[i], keys[x], mbits)
-// for 0 <= x < m.  An oracle may answer -1 (below every real delta), which
-// ends each search at that probe; the caller then discards the result.
-template <typename K, typename Dl>
-__device__ inline BrtOut<K> brt_eval(K cur, Dl dl, int mbits, int m, int i) {
-  BrtOut<K> o;
-  // The reference requires distinct keys (unique runs first); equal
-  // neighbours (delta == mbits) are reported, and every search below stays
-  // range-bounded.
-  o.dup = dl(i + 1) == mbits;
-  o.quirk = false;
-  int d, l;
-  o.lcp_min = -1;
-  if (i == 0) {
-    d = 1;
-    l = m - 1;
-  } else {
-    const int l_pos = dl(i + 1);
-    const int l_neg = dl(i - 1);
-    d = l_pos >= l_neg ? 1 : -1;
-    const int lcp_min = (d == -1) ? l_pos : l_neg;
-    o.lcp_min = lcp_min;
-    int l_max = 2;
-    for (;;) {
-      const long long idx = (long long)i + (long long)l_max * d;
-      if (idx < 0 || idx > m - 1) break;
-      if (dl((int)idx) <= lcp_min) break;
-      l_max <<= 1;
-    }
-    const int l_cutoff = (d == -1) ? i : m - i - 1;
-    l = 0;
-    for (int t = l_max >> 1; t >= 1; t >>= 1) {
-      if (l + t <= l_cutoff && dl(i + (l + t) * d) > lcp_min) l += t;
-    }
-  }
-  const int j = i + l * d;
-  const int lcp_node = dl(j);
-  const int s_cutoff = (d == -1) ? i - 1 : m - i - 2;
-  int s = 0;
-  for (int den = 2; den < 2 * l; den *= 2) {
-    // (l + (float)den - 1) / den, every operation rounded as in C.
-    const float num = __fadd_rn(__fadd_rn((float)l, (float)den), -1.0f);
-    const int t = (int)__fdiv_rn(num, (float)den);
-    if (s + t <= s_cutoff && dl(i + (s + t) * d) > lcp_node) s += t;
-  }
-  if (l > (1 << 23)) {
-    // For l <= 2^23 every intermediate of the FP32 step is an integer
-    // <= 2^24 (den < 2l is a power of two), so it equals the exact ceil.
-    // Above that it can differ; if it changed the split, the tree is not a
-    // well-formed radix tree and the octree stage must take the
-    // order-exact scatter path.
-    int se = 0;
-    for (int den = 2; den < 2 * l; den *= 2) {
-      const int t = (int)(((long long)l + den - 1) / den);
-      if (se + t <= s_cutoff && dl(i + (se + t) * d) > lcp_node) se += t;
-    }
-    o.quirk = se != s;
-  }
-  const int split = i + s * d + (d < 0 ? d : 0);
-  o.nd.left = split;
-  o.nd.left_leaf = (min(i, j) == split);
-  o.nd.right_leaf = (max(i, j) == split + 1);
-  o.nd.lcp = kshr(cur, mbits - lcp_node);
-  o.nd.lcp_length = (uint16_t)lcp_node;
-  return o;
+// Stores a BRT record as whole words (one 16-byte store for 64-bit keys)
+// instead of the field-by-field byte and short stores of a struct copy.
+__device__ inline void store_brt(BrtNodeT<uint32_t> *p, const BrtNodeT<uint32_t> &n) {
+  const uint32_t tail = (uint32_t)n.lcp_length | ((uint32_t)n.left_leaf << 16) |
+                        ((uint32_t)n.right_leaf << 24);
+  uint32_t *q = reinterpret_cast<uint32_t *>(p);   // 12-byte records: 4-byte aligned
+  q[0] = n.lcp;
+  q[1] = (uint32_t)n.left;
+  q[2] = tail;
+}
+__device__ inline void store_brt(BrtNodeT<uint64_t> *p, const BrtNodeT<uint64_t> &n) {
+  const uint32_t tail = (uint32_t)n.lcp_length | ((uint32_t)n.left_leaf << 16) |
+                        ((uint32_t)n.right_leaf << 24);
+  *reinterpret_cast<uint4 *>(p) =   // 16-byte records from a 256-byte aligned base
+      make_uint4((uint32_t)n.lcp, (uint32_t)(n.lcp >> 32), (uint32_t)n.left, tail);
+}
+__device__ inline void store_brt(BrtNodeT<Key128> *p, const BrtNodeT<Key128> &n) {
+  const uint32_t tail = (uint32_t)n.lcp_length | ((uint32_t)n.left_leaf << 16) |
+                        ((uint32_t)n.right_leaf << 24);
+  uint2 *q = reinterpret_cast<uint2 *>(p);         // 24-byte records: 8-byte aligned
+  q[0] = make_uint2((uint32_t)n.lcp.lo, (uint32_t)(n.lcp.lo >> 32));
+  q[1] = make_uint2((uint32_t)n.lcp.hi, (uint32_t)(n.lcp.hi >> 32));
+  q[2] = make_uint2((uint32_t)n.left, tail);
 }
 
 template <typename K>
@@ -115,7 +65,7 @@ __global__ __launch_bounds__(kBlock) void brt_kernel(const K *__restrict__ keys,
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= m - 1) return;
   const K cur = keys[i];
-  const BrtOut<K> o = brt_eval(cur, [&](int x) { return delta(cur, keys[x], mbits); }, mbits, m, i);
+  const BrtOut<K> o = brt_eval(cur, [&](int x) { return keys[x]; }, mbits, m, i);
   brt_flags(o, st);
   brt[i] = o.nd;
   if (!o.nd.left_leaf) atomicMax(&parent[o.nd.left], i + 1);
@@ -137,7 +87,7 @@ template <typename K, int DIM>
 __device__ inline void brt_store(int c, K cur, const BrtOut<K> &o, int mbits,
                                  BrtNodeT<K> *__restrict__ brt, int32_t *__restrict__ parent,
                                  uint32_t *__restrict__ lsx) {
-  brt[c] = o.nd;
+  store_brt(brt + c, o.nd);
   if (!o.nd.left_leaf) parent[o.nd.left] = c;
   if (!o.nd.right_leaf) parent[o.nd.left + 1] = c;
   const int len = o.nd.lcp_length;
@@ -179,12 +129,13 @@ __device__ inline void brt_window(const K *__restrict__ keys, int m, int b0, int
     bool miss = false;
     if (c < m - 1) {
       const K cur = s_keys[c - w0];
+      const K far = kflip(cur, mbits - 1);
       const BrtOut<K> o = brt_eval(
           cur,
-          [&](int x) -> int {
+          [&](int x) -> K {
             const unsigned k = (unsigned)(x - w0);
-            if (k >= (unsigned)WN) { miss = true; return -1; }
-            return delta(cur, s_keys[k], mbits);
+            if (k >= (unsigned)WN) { miss = true; return far; }
+            return s_keys[k];
           },
           mbits, m, c);
       if (!miss) {
@@ -229,8 +180,7 @@ __global__ __launch_bounds__(kBlock) void brt_ls_defer_kernel(const K *__restric
   for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
     const int c = (int)defer[k];
     const K cur = keys[c];
-    const BrtOut<K> o =
-        brt_eval(cur, [&](int x) { return delta(cur, keys[x], mbits); }, mbits, m, c);
+    const BrtOut<K> o = brt_eval(cur, [&](int x) { return keys[x]; }, mbits, m, c);
     brt_flags(o, st);
     brt_store<K, DIM>(c, cur, o, mbits, brt, parent, lsx);
   }
@@ -283,8 +233,7 @@ __global__ __launch_bounds__(kBlock) void brt_seg_defer_kernel(const K *__restri
     const int m = (int)ss[s].m;
     const K *kk = keys + kofs;
     const K cur = kk[c];
-    const BrtOut<K> o =
-        brt_eval(cur, [&](int x) { return delta(cur, kk[x], mbits); }, mbits, m, c);
+    const BrtOut<K> o = brt_eval(cur, [&](int x) { return kk[x]; }, mbits, m, c);
     brt_flags(o, st);
     brt_store<K, DIM>(c, cur, o, mbits, brt + kofs, parent + kofs, lsx + kofs);
   }

@@ -35,16 +35,16 @@ __host__ __device__ inline uint32_t kxor(uint32_t a, uint32_t b) { return a ^ b;
 __host__ __device__ inline uint64_t kxor(uint64_t a, uint64_t b) { return a ^ b; }
 __host__ __device__ inline Key128 kxor(Key128 a, Key128 b) { return {a.lo ^ b.lo, a.hi ^ b.hi}; }
 
-__device__ inline int bitlen(uint32_t x) { return 32 - __clz((int)x); }
-__device__ inline int bitlen(uint64_t x) { return 64 - __clzll((long long)x); }
-__device__ inline int bitlen(Key128 x) { return x.hi ? 128 - __clzll((long long)x.hi) : bitlen(x.lo); }
+__host__ __device__ inline int bitlen(uint32_t x) { return x ? 32 - __builtin_clz(x) : 0; }
+__host__ __device__ inline int bitlen(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+__host__ __device__ inline int bitlen(Key128 x) { return x.hi ? 128 - __builtin_clzll(x.hi) : bitlen(x.lo); }
 
 // Longest common prefix length of two mbits-wide keys.  The reference finds
 // it by a binary search over BigUnsigned right shifts (C/BuildBRT.c:54-88);
 // that search returns mbits - bitlen(a ^ b) for every pair (equal keys give
 // mbits), which is what this computes with one count-leading-zeros.
 template <typename K>
-__device__ inline int delta(K a, K b, int mbits) { return mbits - bitlen(kxor(a, b)); }
+__host__ __device__ inline int delta(K a, K b, int mbits) { return mbits - bitlen(kxor(a, b)); }
 
 // Logical right shift; shifts at or past the word width give 0, as a
 // BigUnsigned shift does (C/BigUnsigned.c:417-471).
@@ -67,6 +67,13 @@ __device__ inline uint32_t kdigit(uint32_t a, int shift) { return (a >> shift) &
 __device__ inline uint32_t kdigit(uint64_t a, int shift) { return (uint32_t)(a >> shift) & 0xffu; }
 __device__ inline uint32_t kdigit(Key128 a, int shift) {
   return shift >= 64 ? (uint32_t)(a.hi >> (shift - 64)) & 0xffu : (uint32_t)(a.lo >> shift) & 0xffu;
+}
+
+// Key with bit b (0 <= b < width) flipped.
+__host__ __device__ inline uint32_t kflip(uint32_t a, int b) { return a ^ (1u << b); }
+__host__ __device__ inline uint64_t kflip(uint64_t a, int b) { return a ^ (1ull << b); }
+__host__ __device__ inline Key128 kflip(Key128 a, int b) {
+  return b >= 64 ? Key128{a.lo, a.hi ^ (1ull << (b - 64))} : Key128{a.lo ^ (1ull << b), a.hi};
 }
 
 __host__ __device__ inline bool kzero(uint32_t a) { return a == 0; }

@@ -42,6 +42,25 @@ __device__ inline uint32_t ls_of(const uint32_t *ps, uint32_t c) {
   return c == 0 ? ps[0] : ps[c] - ps[c - 1];
 }
 
+// A BRT record's child links packed as left | left_leaf << 30 | right_leaf
+// << 31 (left < 2^30), read as one word pair where the record allows.
+template <typename K>
+__device__ inline uint32_t pack_links(uint32_t left, uint32_t tail) {
+  return left | (((tail >> 16) & 1u) << 30) | (((tail >> 24) & 1u) << 31);
+}
+__device__ inline uint32_t load_links(const BrtNodeT<uint32_t> *p) {
+  const uint32_t *q = reinterpret_cast<const uint32_t *>(p);
+  return pack_links<uint32_t>(q[1], q[2]);
+}
+__device__ inline uint32_t load_links(const BrtNodeT<uint64_t> *p) {
+  const uint2 v = reinterpret_cast<const uint2 *>(p)[1];
+  return pack_links<uint64_t>(v.x, v.y);
+}
+__device__ inline uint32_t load_links(const BrtNodeT<Key128> *p) {
+  const uint2 v = reinterpret_cast<const uint2 *>(p)[2];
+  return pack_links<Key128>(v.x, v.y);
+}
+
 template <int DIM>
 __device__ inline void write_node(int32_t *octree, uint32_t idx, const int32_t *ch) {
   constexpr int NC = 1 << DIM;
@@ -117,6 +136,81 @@ __device__ inline bool gather_one(uint32_t q, const BrtNodeT<K> &nq, uint32_t ex
   return true;
 }
 
+// gather_one for the window pass with its output staged in LDS: the
+// workgroup has filled the staged nodes with -1 children and full leaf
+// masks, so only found children and leaf words are written.  The pending
+// same-level nodes are window indices (< 1024) packed 10 bits each into one
+// 64-bit register (at most 2^DIM - 2 <= 6 of them).  node0 is q's lowest
+// staged node.  Returns false, writing nothing, when the descent leaves the
+// window.
+template <typename K, int DIM>
+__device__ inline bool gather_one_lds(uint32_t q, const BrtNodeT<K> &nq, uint32_t excl,
+                                      uint32_t lsq, int w0, int wn, const uint32_t *s_lnk,
+                                      const uint32_t *s_ps, const uint32_t *s_lsx,
+                                      const K *__restrict__ keys,
+                                      const uint32_t *__restrict__ conflict_bits, int mbits,
+                                      int32_t *node0, uint32_t *__restrict__ node_conflict,
+                                      RunState *st) {
+  constexpr int NC = 1 << DIM;
+  constexpr int STRIDE = NC + 1;
+  constexpr int32_t kFull = (1 << NC) - 1;
+  const int lvl = nq.lcp_length / DIM;
+  uint32_t cmask = 0, found = 0;
+  uint64_t stk = 0;
+  int sp = 0, visits = 0;
+  uint32_t xl = (uint32_t)nq.left;
+  bool xll = nq.left_leaf, xrl = nq.right_leaf;
+  for (;;) {
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const uint32_t c = xl + side;
+      if (side ? xrl : xll) {
+        if (conflict_bits && test_bit(conflict_bits, c))
+          cmask |= 1u << digit_at<K, DIM>(keys[c], lvl, mbits);
+        continue;
+      }
+      const unsigned kc = (unsigned)((int)c - w0);
+      if (kc >= (unsigned)wn) return false;
+      const uint32_t lxc = s_lsx[kc];
+      if (lxc & 0xffu) {
+        const uint32_t o = lxc >> 8;
+        node0[o] = (int32_t)(s_ps[kc] - 1);   // c's chain top, in its octant
+        found |= 1u << o;
+      } else if (sp < 6) {
+        stk = (stk << 10) | kc;
+        ++sp;
+      } else {
+        atomicOr(&st->status, (uint32_t)ST_STACK);
+      }
+    }
+    if (sp == 0) break;
+    // A well-formed tree visits at most 2^DIM - 1 nodes here; the cap only
+    // guards against a malformed tree the BRT stage did not flag.
+    if (++visits > 64) {
+      atomicOr(&st->status, (uint32_t)ST_STACK);
+      break;
+    }
+    const uint32_t v = s_lnk[(uint32_t)stk & 0x3ffu];
+    stk >>= 10;
+    --sp;
+    xl = v & 0x3fffffffu;
+    xll = (v >> 30) & 1u;
+    xrl = v >> 31;
+  }
+  node0[NC] = kFull & ~(int32_t)found;
+  if (node_conflict) node_conflict[excl] = cmask;
+  for (uint32_t k = 1; k < lsq; ++k) {
+    if (q != 0) {
+      const int o = quadrant<K, DIM>(nq, (int)k - 1);
+      int32_t *nd = node0 + k * STRIDE;
+      nd[o] = (int32_t)(excl + k - 1);
+      nd[NC] = kFull & ~(1 << o);
+    }
+    if (node_conflict) node_conflict[excl + k] = 0;
+  }
+  return true;
+}
+
 // Octree nodes staged per workgroup: the nodes owned by BRT nodes
 // [q0, q0+256) are the contiguous range [ps[q0-1], ps[q0+255]); when it fits
 // here it is built in LDS and written out in coalesced runs.
@@ -150,10 +244,7 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
     if (g >= 0 && g < (int)m) {
       s_ps[r] = ps[g];
       s_lsx[r] = lsx[g];
-      if (g < (int)m - 1) {
-        const BrtNodeT<K> &b = brt[g];
-        s_lnk[r] = (uint32_t)b.left | ((uint32_t)b.left_leaf << 30) | ((uint32_t)b.right_leaf << 31);
-      }
+      if (g < (int)m - 1) s_lnk[r] = load_links(brt + g);
     }
   }
   __syncthreads();
@@ -161,6 +252,12 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
   const uint32_t lo = q0 ? s_ps[q0 - 1 - w0] : 0u;
   const uint32_t span = s_ps[qlast - w0] - lo;
   const bool staged = span <= (uint32_t)kStageNodes;   // uniform over the workgroup
+  if (staged) {
+    // every staged node starts with no children and a full leaf mask
+    for (uint32_t w = threadIdx.x; w < span * STRIDE; w += kBlock)
+      s_nodes[w] = (w % STRIDE == (uint32_t)NC) ? (1 << NC) - 1 : -1;
+    __syncthreads();
+  }
   bool active = q <= qlast, miss = false;
   uint32_t excl = 0, lsq = 0;
   if (active) {
@@ -168,7 +265,11 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
     lsq = s_ps[q - w0] - excl;
     active = (q == 0) || lsq > 0;
   }
-  if (active) {
+  if (active && staged) {
+    miss = !gather_one_lds<K, DIM>(q, brt[q], excl, lsq, w0, WN, s_lnk, s_ps, s_lsx, keys,
+                                   conflict_bits, mbits, s_nodes + (excl - lo) * STRIDE,
+                                   node_conflict, st);
+  } else if (active) {
     auto links = [&](uint32_t x, uint32_t &left, bool &ll, bool &rl) -> bool {
       const unsigned k = (unsigned)((int)x - w0);
       if (k >= (unsigned)WN) return false;
@@ -185,10 +286,7 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
       lxc = s_lsx[k];
       return true;
     };
-    auto put = [&](uint32_t idx, const int32_t *c) {
-      if (staged) write_node<DIM>(s_nodes, idx - lo, c);
-      else write_node<DIM>(octree, idx, c);
-    };
+    auto put = [&](uint32_t idx, const int32_t *c) { write_node<DIM>(octree, idx, c); };
     miss = !gather_one<K, DIM>(q, brt[q], excl, lsq, links, child, keys, conflict_bits, mbits,
                                put, node_conflict, st);
   }

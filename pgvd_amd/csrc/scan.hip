@@ -1,6 +1,8 @@
-// scan.hip -- single-pass decoupled look-back scans over striped tiles
-// (thread t of a 256-thread workgroup holds elements r*256 + t, so every
-// load and store is coalesced).
+// scan.hip -- single-pass decoupled look-back scans.  The unique and
+// local-split scans give each thread 16 consecutive elements (16-byte vector
+// loads and stores, a serial scan in registers, one workgroup scan of the
+// per-thread totals); the order-exact splits scan and the stage scan use
+// striped tiles (element r*256 + t in thread t).
 //
 //   unique        UniqueSorted (opencl/Kernels.cpp:198-217): UniquePredicate
 //                 (C/ParallelAlgorithms.c:22-31) + scan + BUSingleCompact
@@ -28,6 +30,76 @@ template <> __device__ inline Key128 shfl_up1<Key128>(Key128 v) {
   return {__shfl_up(v.lo, 1, 64), __shfl_up(v.hi, 1, 64)};
 }
 
+// One unique tile: elements [base, base+4096) of kin[0..n) (sorted), thread t
+// holding base + 16t .. +16.  Writes [zero?] + the distinct keys to
+// kout[prepend + rank]; with labels, flags run r in conflict_bits when the
+// labels of its keys are not all equal.  *m_out (last tile only) = m.
+template <typename K, bool LAB>
+__device__ inline void unique_tile(const K *__restrict__ kin, const uint32_t *__restrict__ lab,
+                                   uint32_t n, uint32_t prepend, K *__restrict__ kout,
+                                   uint32_t *__restrict__ conflict_bits, uint64_t *lookback,
+                                   int tile, int lo, uint32_t base, uint32_t *m_out) {
+  constexpr int R = kRows;
+  __shared__ uint32_t s_w[4];
+  const int t = threadIdx.x, lane = t & 63;
+  const uint32_t e0 = base + (uint32_t)t * R;
+  const uint32_t valid = e0 < n ? min((uint32_t)R, n - e0) : 0u;
+  K k[R];
+  load_run<K, R>(kin + e0, valid, k);
+  K prev = shfl_up1(k[R - 1]);
+  if (lane == 0 && valid > 0 && e0 > 0) prev = kin[e0 - 1];
+  uint32_t heads = 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const uint32_t e = e0 + j;
+    const bool h = (j < (int)valid) &&
+                   ((e == 0) ? (!prepend || !kzero(k[0])) : !keq(k[j], j ? k[j - 1] : prev));
+    heads |= (uint32_t)h << j;
+  }
+  uint32_t total;
+  const uint32_t texcl = block_excl_sum<uint32_t>((uint32_t)__popc(heads), s_w, &total);
+  const uint64_t excl = tile_prefix(lookback, tile, total, lo);
+  const uint32_t obase = (uint32_t)(prepend + excl + texcl);
+  if constexpr (sizeof(K) <= 8) {
+    // Compacted keys are staged in LDS (index r + r/16 against bank
+    // conflicts) so that the tile's output is written in coalesced runs.
+    __shared__ K s_out[kScanTile + kScanTile / 16];
+    uint32_t o = texcl;
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if ((heads >> j) & 1u) {
+        s_out[o + (o >> 4)] = k[j];
+        ++o;
+      }
+    __syncthreads();
+    K *dst = kout + prepend + excl;
+    for (uint32_t r = t; r < total; r += kBlock) dst[r] = s_out[r + (r >> 4)];
+  } else {
+    uint32_t o = obase;
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if ((heads >> j) & 1u) kout[o++] = k[j];
+  }
+  if (LAB) {
+    uint32_t l[R];
+    load_run<uint32_t, R>(lab + e0, valid, l);
+    uint32_t lprev = __shfl_up(l[R - 1], 1, 64);
+    if (lane == 0 && valid > 0 && e0 > 0) lprev = lab[e0 - 1];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const uint32_t e = e0 + j;
+      if (j < (int)valid && e > 0 && !((heads >> j) & 1u) && l[j] != (j ? l[j - 1] : lprev)) {
+        // not a head: same key as e-1, in run (heads up to e) - 1; the
+        // virtual zero key carries no label, so e = 0 never compares.
+        const uint32_t run = obase + (uint32_t)__popc(heads & ((1u << j) - 1u)) - 1u;
+        atomicOr(&conflict_bits[run >> 5], 1u << (run & 31));
+      }
+    }
+  }
+  if (tile == lo && t == 0 && prepend) kout[0] = kzero_value<K>();
+  if (m_out && t == 0) *m_out = (uint32_t)(prepend + excl + total);
+}
+
 template <typename K, bool LAB>
 __global__ __launch_bounds__(kBlock) void unique_kernel(const K *__restrict__ keys,
                                                         const uint32_t *__restrict__ labels,
@@ -35,42 +107,13 @@ __global__ __launch_bounds__(kBlock) void unique_kernel(const K *__restrict__ ke
                                                         K *__restrict__ out,
                                                         uint32_t *__restrict__ conflict_bits,
                                                         uint64_t *lookback, RunState *st) {
-  __shared__ uint32_t s_w[kRows * 4];
-  __shared__ uint64_t s_excl;
   __shared__ uint32_t s_tile;
-  const int t = threadIdx.x, lane = t & 63;
-  if (t == 0) s_tile = atomicAdd(&st->counters[CTR_UNIQUE], 1u);
+  if (threadIdx.x == 0) s_tile = atomicAdd(&st->counters[CTR_UNIQUE], 1u);
   __syncthreads();
   const int tile = (int)s_tile;
   const int ntiles = n ? (int)((n + kScanTile - 1) / kScanTile) : 1;
-  const uint32_t base = (uint32_t)tile * kScanTile;
-  K k[kRows];
-  bool head[kRows];
-#pragma unroll
-  for (int r = 0; r < kRows; ++r) {
-    const uint32_t e = base + r * kBlock + t;
-    k[r] = (e < n) ? keys[e] : kzero_value<K>();
-    K prev = shfl_up1(k[r]);
-    if (lane == 0 && e > 0 && e < n) prev = keys[e - 1];
-    head[r] = (e < n) && ((e == 0) ? (!prepend_zero || !kzero(k[r])) : !keq(k[r], prev));
-  }
-  uint32_t ex[kRows];
-  const uint32_t total = striped_flag_scan<kRows>(head, ex, s_w);
-  const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl);
-  const uint32_t obase = (uint32_t)(prepend_zero + excl);
-#pragma unroll
-  for (int r = 0; r < kRows; ++r) {
-    const uint32_t e = base + r * kBlock + t;
-    if (head[r]) out[obase + ex[r]] = k[r];
-    if (LAB && e < n && e > 0 && !head[r] && labels[e] != labels[e - 1]) {
-      // not a head: same key as e-1, in run (heads up to e) - 1; the
-      // virtual zero key carries no label, so e = 0 never compares.
-      const uint32_t run = obase + ex[r] - 1;
-      atomicOr(&conflict_bits[run >> 5], 1u << (run & 31));
-    }
-  }
-  if (tile == 0 && t == 0 && prepend_zero) out[0] = kzero_value<K>();
-  if (tile == ntiles - 1 && t == 0) st->m = (uint32_t)(prepend_zero + excl + total);
+  unique_tile<K, LAB>(keys, labels, n, (uint32_t)prepend_zero, out, conflict_bits, lookback, tile,
+                      0, (uint32_t)tile * kScanTile, tile == ntiles - 1 ? &st->m : nullptr);
 }
 
 // Local split of BRT node c computed from its own parent claim
@@ -95,7 +138,6 @@ __global__ __launch_bounds__(kBlock) void splits_scan_kernel(const BrtNodeT<K> *
                                                              uint64_t *lookback, RunState *st) {
   if (only_if_malformed && !st->malformed) return;
   __shared__ uint32_t s_w[kRows * 4];
-  __shared__ uint64_t s_excl;
   __shared__ uint32_t s_tile;
   const int t = threadIdx.x;
   const uint32_t m = st->m;
@@ -114,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void splits_scan_kernel(const BrtNodeT<K> *
       v[r] = (c < m) ? local_split(brt, parent, c, m, dim) : 0u;
     }
     const uint32_t total = striped_excl_scan<kRows, uint32_t>(v, ex, s_w);
-    const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl);
+    const uint64_t excl = tile_prefix(lookback, tile, total);
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
       const uint32_t c = base + r * kBlock + t;
@@ -142,7 +184,6 @@ __global__ __launch_bounds__(kBlock) void scan_u32_kernel(const uint32_t *__rest
                                                           uint32_t *__restrict__ out, uint32_t n,
                                                           uint64_t *lookback, RunState *st) {
   __shared__ uint64_t s_w[kRows * 4];
-  __shared__ uint64_t s_excl;
   __shared__ uint32_t s_tile;
   const int t = threadIdx.x;
   if (t == 0) s_tile = atomicAdd(&st->counters[CTR_SCAN], 1u);
@@ -156,7 +197,7 @@ __global__ __launch_bounds__(kBlock) void scan_u32_kernel(const uint32_t *__rest
     v[r] = (e < n) ? in[e] : 0u;
   }
   const uint64_t total = striped_excl_scan<kRows, uint64_t>(v, ex, s_w);
-  const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl);
+  const uint64_t excl = tile_prefix(lookback, tile, total);
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
     const uint32_t e = base + r * kBlock + t;
@@ -164,36 +205,48 @@ __global__ __launch_bounds__(kBlock) void scan_u32_kernel(const uint32_t *__rest
   }
 }
 
+// One local-split scan tile: inclusive scan of (lsx & 0xff) over
+// [base, base+4096) of [0, m) into ps, thread t holding base + 16t .. +16;
+// *o_out (last tile only) = the octree size.
+__device__ inline void ls_scan_tile(const uint32_t *__restrict__ lsx, uint32_t *__restrict__ ps,
+                                    uint32_t m, uint64_t *lookback, int tile, int lo,
+                                    uint32_t base, uint32_t *o_out) {
+  constexpr int R = kRows;
+  __shared__ uint32_t s_w[4];
+  const uint32_t e0 = base + threadIdx.x * R;
+  const uint32_t valid = e0 < m ? min((uint32_t)R, m - e0) : 0u;
+  uint32_t v[R];
+  load_run<uint32_t, R>(lsx + e0, valid, v);
+  uint32_t run = 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    run += (j < (int)valid) ? (v[j] & 0xffu) : 0u;
+    v[j] = run;
+  }
+  uint32_t total;
+  const uint32_t texcl = block_excl_sum<uint32_t>(run, s_w, &total);
+  const uint64_t excl = tile_prefix(lookback, tile, total, lo);
+  const uint32_t add = (uint32_t)excl + texcl;
+#pragma unroll
+  for (int j = 0; j < R; ++j) v[j] += add;
+  store_run<uint32_t, R>(ps + e0, valid, v);
+  if (o_out && threadIdx.x == 0) *o_out = (uint32_t)(excl + total);
+}
+
 // Inclusive scan of the local splits ls[0..m), m = st->m, into ps; the total
 // is the octree size (opencl/Kernels.cpp:376-378).
 __global__ __launch_bounds__(kBlock) void ls_scan_kernel(const uint32_t *__restrict__ ls,
                                                          uint32_t *__restrict__ ps,
                                                          uint64_t *lookback, RunState *st) {
-  __shared__ uint32_t s_w[kRows * 4];
-  __shared__ uint64_t s_excl;
   __shared__ uint32_t s_tile;
-  const int t = threadIdx.x;
   const uint32_t m = st->m;
   const int ntiles = (int)((m + kScanTile - 1) / kScanTile);
-  if (t == 0) s_tile = atomicAdd(&st->counters[CTR_LS], 1u);
+  if (threadIdx.x == 0) s_tile = atomicAdd(&st->counters[CTR_LS], 1u);
   __syncthreads();
   const int tile = (int)s_tile;
   if (tile >= ntiles) return;
-  const uint32_t base = (uint32_t)tile * kScanTile;
-  uint32_t v[kRows], ex[kRows];
-#pragma unroll
-  for (int r = 0; r < kRows; ++r) {
-    const uint32_t e = base + r * kBlock + t;
-    v[r] = (e < m) ? (ls[e] & 0xffu) : 0u;
-  }
-  const uint32_t total = striped_excl_scan<kRows, uint32_t>(v, ex, s_w);
-  const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl);
-#pragma unroll
-  for (int r = 0; r < kRows; ++r) {
-    const uint32_t e = base + r * kBlock + t;
-    if (e < m) ps[e] = (uint32_t)(excl + ex[r] + v[r]);
-  }
-  if (tile == ntiles - 1 && t == 0) st->octree_size = (uint32_t)(excl + total);
+  ls_scan_tile(ls, ps, m, lookback, tile, 0, (uint32_t)tile * kScanTile,
+               tile == ntiles - 1 ? &st->octree_size : nullptr);
 }
 
 // ---- batched form ----
@@ -206,39 +259,16 @@ __global__ __launch_bounds__(kBlock) void unique_seg_kernel(const K *__restrict_
                                                             const SegInfo *__restrict__ seg,
                                                             TileTable tt, SegState *ss,
                                                             uint64_t *lookback, RunState *st) {
-  __shared__ uint32_t s_w[kRows * 4];
-  __shared__ uint64_t s_excl;
   __shared__ uint32_t s_tile;
-  const int t = threadIdx.x, lane = t & 63;
-  if (t == 0) s_tile = atomicAdd(&st->counters[CTR_UNIQUE], 1u);
+  if (threadIdx.x == 0) s_tile = atomicAdd(&st->counters[CTR_UNIQUE], 1u);
   __syncthreads();
   const int tile = (int)s_tile;
   const uint32_t s = tt.seg[tile];
   const int lo = (int)tt.first[s];
   const SegInfo si = seg[s];
-  const uint32_t n = si.n;
-  const K *kin = keys + si.kofs;
-  K *kout = out + si.kofs;
-  const uint32_t base = (uint32_t)(tile - lo) * kScanTile;
-  K k[kRows];
-  bool head[kRows];
-#pragma unroll
-  for (int r = 0; r < kRows; ++r) {
-    const uint32_t e = base + r * kBlock + t;
-    k[r] = (e < n) ? kin[e] : kzero_value<K>();
-    K prev = shfl_up1(k[r]);
-    if (lane == 0 && e > 0 && e < n) prev = kin[e - 1];
-    head[r] = (e < n) && ((e == 0) ? (!si.prepend || !kzero(k[r])) : !keq(k[r], prev));
-  }
-  uint32_t ex[kRows];
-  const uint32_t total = striped_flag_scan<kRows>(head, ex, s_w);
-  const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl, lo);
-  const uint32_t obase = (uint32_t)(si.prepend + excl);
-#pragma unroll
-  for (int r = 0; r < kRows; ++r)
-    if (head[r]) kout[obase + ex[r]] = k[r];
-  if (tile == lo && t == 0 && si.prepend) kout[0] = kzero_value<K>();
-  if (tile == (int)tt.first[s + 1] - 1 && t == 0) ss[s].m = (uint32_t)(si.prepend + excl + total);
+  unique_tile<K, false>(keys + si.kofs, nullptr, si.n, si.prepend, out + si.kofs, nullptr,
+                        lookback, tile, lo, (uint32_t)(tile - lo) * kScanTile,
+                        tile == (int)tt.first[s + 1] - 1 ? &ss[s].m : nullptr);
 }
 
 // Per-instance inclusive scan of the local splits (low byte of lsx) over
@@ -248,11 +278,8 @@ __global__ __launch_bounds__(kBlock) void ls_scan_seg_kernel(const uint32_t *__r
                                                              const SegInfo *__restrict__ seg,
                                                              TileTable tt, SegState *ss,
                                                              uint64_t *lookback, RunState *st) {
-  __shared__ uint32_t s_w[kRows * 4];
-  __shared__ uint64_t s_excl;
   __shared__ uint32_t s_tile;
-  const int t = threadIdx.x;
-  if (t == 0) s_tile = atomicAdd(&st->counters[CTR_LS], 1u);
+  if (threadIdx.x == 0) s_tile = atomicAdd(&st->counters[CTR_LS], 1u);
   __syncthreads();
   const int tile = (int)s_tile;
   const uint32_t s = tt.seg[tile];
@@ -261,21 +288,8 @@ __global__ __launch_bounds__(kBlock) void ls_scan_seg_kernel(const uint32_t *__r
   const int used = (int)((m + kScanTile - 1) / kScanTile);
   if (tile - lo >= used) return;    // no later tile of this instance depends on it
   const uint32_t kofs = seg[s].kofs;
-  const uint32_t base = (uint32_t)(tile - lo) * kScanTile;
-  uint32_t v[kRows], ex[kRows];
-#pragma unroll
-  for (int r = 0; r < kRows; ++r) {
-    const uint32_t e = base + r * kBlock + t;
-    v[r] = (e < m) ? (lsx[kofs + e] & 0xffu) : 0u;
-  }
-  const uint32_t total = striped_excl_scan<kRows, uint32_t>(v, ex, s_w);
-  const uint64_t excl = tile_prefix(lookback, tile, total, &s_excl, lo);
-#pragma unroll
-  for (int r = 0; r < kRows; ++r) {
-    const uint32_t e = base + r * kBlock + t;
-    if (e < m) ps[kofs + e] = (uint32_t)(excl + ex[r] + v[r]);
-  }
-  if (tile - lo == used - 1 && t == 0) ss[s].octree_size = (uint32_t)(excl + total);
+  ls_scan_tile(lsx + kofs, ps + kofs, m, lookback, tile, lo, (uint32_t)(tile - lo) * kScanTile,
+               tile - lo == used - 1 ? &ss[s].octree_size : nullptr);
 }
 
 // One workgroup: oct_off = exclusive scan of the instances' octree sizes.

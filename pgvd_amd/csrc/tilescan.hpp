@@ -1,5 +1,5 @@
 // tilescan.hpp -- single-pass decoupled look-back building blocks shared by
-// the scan, unique and fused BRT kernels (256-thread workgroups, wave64).
+// the scan and unique kernels (256-thread workgroups, wave64).
 //
 // A tile's status word is 64 bits: flag in bits 62-63 (1 = aggregate only,
 // 2 = inclusive prefix), value in bits 0-61.  Status arrays must be zero
@@ -15,54 +15,101 @@ constexpr uint64_t kSAgg = 1ull << 62;
 constexpr uint64_t kSIncl = 2ull << 62;
 constexpr uint64_t kSVal = (1ull << 62) - 1;
 
-// Called by every lane of one wave after the tile's aggregate is published.
-// Returns the exclusive prefix of `tile` within the run of tiles starting at
-// `lo`, resolving up to 64 predecessors per step.
-__device__ inline uint64_t lookback_excl(uint64_t *status, int tile, int lo = 0) {
-  const int lane = threadIdx.x & 63;
+// Publishes the tile aggregate, resolves the tile's exclusive prefix within
+// the run of tiles starting at `lo` and publishes its inclusive prefix;
+// returns the exclusive prefix to every thread.  All 256 threads of the
+// workgroup call it.  Each round reads the status of the 256 nearest
+// unresolved predecessors at once (one per thread) and stops at the nearest
+// inclusive one, so a tile k tiles past the last resolved one waits about
+// k/256 round trips.
+__device__ inline uint64_t tile_prefix(uint64_t *status, int tile, uint64_t total, int lo = 0) {
+  __shared__ int s_first[4];
+  __shared__ uint64_t s_part[4];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t == 0) st_rel_u64(status + tile, (tile == lo ? kSIncl : kSAgg) | total);
   uint64_t excl = 0;
-  int base = tile - 1;
-  while (base >= lo) {
-    const int k = base - lane;
-    uint64_t wv = kSIncl;  // before the run's first tile: an inclusive zero
-    if (k >= lo) {
-      for (;;) {
-        wv = ld_acq_u64(status + k);
-        if (wv & ~kSVal) break;
-        __builtin_amdgcn_s_sleep(1);
+  if (tile > lo) {
+    int base = tile - 1;
+    for (;;) {
+      const int k = base - t;
+      uint64_t wv = kSIncl;  // before the run's first tile: an inclusive zero
+      if (k >= lo) {
+        for (;;) {
+          wv = ld_acq_u64(status + k);
+          if (wv & ~kSVal) break;
+          __builtin_amdgcn_s_sleep(1);
+        }
       }
+      const uint64_t b = __ballot((wv & ~kSVal) == kSIncl);
+      if (lane == 0) s_first[w] = b ? __ffsll((long long)b) - 1 : 64;
+      __syncthreads();
+      int first = kBlock;
+      for (int j = 3; j >= 0; --j)
+        if (s_first[j] < 64) first = j * 64 + s_first[j];
+      uint64_t val = (t <= first) ? (wv & kSVal) : 0;
+      val = wave_reduce_add(val);
+      if (lane == 0) s_part[w] = val;
+      __syncthreads();
+      excl += s_part[0] + s_part[1] + s_part[2] + s_part[3];
+      __syncthreads();  // s_first / s_part are reused by the next round
+      if (first < kBlock) break;
+      base -= kBlock;
     }
-    const uint64_t incl = __ballot((wv & ~kSVal) == kSIncl);
-    uint64_t val = wv & kSVal;
-    if (incl) {
-      const int first = __ffsll((long long)incl) - 1;
-      if (lane > first) val = 0;
-      excl += wave_reduce_add(val);
-      break;
-    }
-    excl += wave_reduce_add(val);
-    base -= 64;
+    if (t == 0) st_rel_u64(status + tile, kSIncl | (excl + total));
   }
   return excl;
 }
 
-// Publishes the tile aggregate, resolves and publishes the inclusive
-// prefix; returns the tile's exclusive prefix (within the run of tiles
-// starting at `lo`) to every thread.  All threads of the block must call it.
-__device__ inline uint64_t tile_prefix(uint64_t *status, int tile, uint64_t total,
-                                       uint64_t *s_excl, int lo = 0) {
-  if (threadIdx.x == 0) st_rel_u64(status + tile, (tile == lo ? kSIncl : kSAgg) | total);
-  if (tile > lo && threadIdx.x < 64) {
-    const uint64_t e = lookback_excl(status, tile, lo);
-    if (threadIdx.x == 0) {
-      st_rel_u64(status + tile, kSIncl | (e + total));
-      *s_excl = e;
-    }
-  } else if (threadIdx.x == 0) {
-    *s_excl = 0;
-  }
+// Exclusive scan of one value per thread over the 256-thread workgroup;
+// *total receives the workgroup sum.  s_w: 4 words of LDS.
+template <typename T>
+__device__ inline T block_excl_sum(T x, T *s_w, T *total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const T incl = wave_incl_scan_add(x);
+  if (lane == 63) s_w[w] = incl;
   __syncthreads();
-  return *s_excl;
+  T before = 0, all = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const T y = s_w[j];
+    if (j < w) before += y;
+    all += y;
+  }
+  *total = all;
+  return before + incl - x;
+}
+
+// R consecutive elements starting at p (valid of them in range): 16-byte
+// vector loads when the run is whole and aligned, else element loads.
+template <typename T, int R>
+__device__ inline void load_run(const T *__restrict__ p, uint32_t valid, T (&v)[R]) {
+  constexpr int NV = (int)(sizeof(T) * R / 16);
+  if ((sizeof(T) * R) % 16 == 0 && valid >= (uint32_t)R && ((uintptr_t)p & 15) == 0) {
+    uint4 q[NV];
+    const uint4 *src = reinterpret_cast<const uint4 *>(p);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) q[i] = src[i];
+    __builtin_memcpy(&v[0], &q[0], sizeof(v));
+  } else {
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = (j < (int)valid) ? p[j] : T{};
+  }
+}
+
+template <typename T, int R>
+__device__ inline void store_run(T *__restrict__ p, uint32_t valid, const T (&v)[R]) {
+  constexpr int NV = (int)(sizeof(T) * R / 16);
+  if ((sizeof(T) * R) % 16 == 0 && valid >= (uint32_t)R && ((uintptr_t)p & 15) == 0) {
+    uint4 q[NV];
+    __builtin_memcpy(&q[0], &v[0], sizeof(v));
+    uint4 *dst = reinterpret_cast<uint4 *>(p);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) dst[i] = q[i];
+  } else {
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (j < (int)valid) p[j] = v[j];
+  }
 }
 
 // Exclusive scan of a striped tile: thread t holds v[r] for element
