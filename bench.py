@@ -233,7 +233,7 @@ def main() -> int:
     ap.add_argument("--workload", default="uniform", choices=["uniform", "gauss", "batch"])
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--n", type=int, default=1 << 24)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 23)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="only run the plain copy kernel over MiB mebibytes (5 launches), "

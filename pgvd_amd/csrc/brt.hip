@@ -100,47 +100,59 @@ __device__ inline void brt_store(int c, K cur, const BrtOut<K> &o, int mbits,
   }
 }
 
-// Keys staged per workgroup: its SEG = R*256 nodes plus H keys either side.
-template <typename K> struct BrtWin { static constexpr int R = 4, H = 512; };
-template <> struct BrtWin<Key128> { static constexpr int R = 2, H = 256; };
+// Adjacent-key deltas staged per workgroup: its SEG = R*256 nodes plus H
+// positions either side.
+template <typename K> struct BrtWin { static constexpr int R = 4, H = 256; };
 
-// Pipeline, first pass.  A workgroup stages keys [b0-H, b0+SEG+H) of one
-// key array keys[0..m) in LDS and evaluates nodes [b0, b0+SEG) against that
-// window only.  Most nodes span a few keys, so their searches (a chain of
-// dependent probes) run at LDS latency; a node whose search leaves the
-// window is handed to defer(true, c) and left to the global-memory pass.
-// lsx[m-1] = 0; m == 1 gives lsx[0] = 1.  Every lane calls defer once per
-// row, converged.
+// Pipeline, first pass.  A workgroup builds, in LDS, the sparse table of
+// adjacent-key deltas over positions [b0-H, b0+SEG+H) of keys[0..m) (one
+// byte per entry and level) and resolves nodes [b0, b0+SEG) from it
+// (brt_node_window): each node is a few table descents instead of a chain of
+// key probes.  A node whose range or split leaves the window is handed to
+// defer(true, c) and left to the global-memory pass.  lsx[m-1] = 0;
+// m == 1 gives lsx[0] = 1.  Every lane calls defer once per row, converged.
 template <typename K, int DIM, int R, typename Defer>
 __device__ inline void brt_window(const K *__restrict__ keys, int m, int b0, int mbits,
                                   BrtNodeT<K> *__restrict__ brt, int32_t *__restrict__ parent,
                                   uint32_t *__restrict__ lsx, RunState *st, Defer defer) {
   constexpr int H = BrtWin<K>::H, SEG = R * kBlock, WN = SEG + 2 * H;
-  __shared__ K s_keys[WN];
+  constexpr int LV = (WN >= 2048) ? 12 : (WN >= 1024 ? 11 : (WN >= 512 ? 10 : 9));
+  static_assert((1 << (LV - 1)) <= WN && WN < (1 << LV), "levels cover the window");
+  __shared__ uint8_t s_mn[LV * WN];
   const int w0 = b0 - H;
+  // The nodes' own keys are loaded first so that their latency overlaps the
+  // table build instead of starting each row.
+  K cur[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int c = b0 + r * kBlock + (int)threadIdx.x;
+    cur[r] = c < m - 1 ? keys[c] : kzero_value<K>();
+  }
   for (int r = threadIdx.x; r < WN; r += kBlock) {
     const int g = w0 + r;
-    if (g >= 0 && g < m) s_keys[r] = keys[g];
+    s_mn[r] = (g >= 0 && g < m - 1) ? (uint8_t)(1 + delta(keys[g], keys[g + 1], mbits)) : 0;
   }
   __syncthreads();
 #pragma unroll 1
+  for (int j = 1; j < LV; ++j) {
+    const int half = 1 << (j - 1);
+    for (int r = threadIdx.x; r + 2 * half <= WN; r += kBlock) {
+      const uint8_t x = s_mn[(j - 1) * WN + r], y = s_mn[(j - 1) * WN + r + half];
+      s_mn[j * WN + r] = x < y ? x : y;
+    }
+    __syncthreads();
+  }
+  auto mn = [&](int j, int k) -> int { return s_mn[j * WN + k]; };
+#pragma unroll
   for (int r = 0; r < R; ++r) {
     const int c = b0 + r * kBlock + (int)threadIdx.x;
     bool miss = false;
     if (c < m - 1) {
-      const K cur = s_keys[c - w0];
-      const K far = kflip(cur, mbits - 1);
-      const BrtOut<K> o = brt_eval(
-          cur,
-          [&](int x) -> K {
-            const unsigned k = (unsigned)(x - w0);
-            if (k >= (unsigned)WN) { miss = true; return far; }
-            return s_keys[k];
-          },
-          mbits, m, c);
+      BrtOut<K> o;
+      miss = !brt_node_window(cur[r], mn, LV, w0, WN, mbits, m, c, o);
       if (!miss) {
         brt_flags(o, st);
-        brt_store<K, DIM>(c, cur, o, mbits, brt, parent, lsx);
+        brt_store<K, DIM>(c, cur[r], o, mbits, brt, parent, lsx);
       }
     } else if (c == m - 1) {
       lsx[c] = (m == 1) ? 1u : 0u;
@@ -308,7 +320,8 @@ void launch_brt_ls(const K *keys, int64_t max_m, int mbits, int dim, BrtNodeT<K>
 }
 
 bool brt_seg_rows_ok(int key_bytes, int rows) {
-  return key_bytes == 16 ? (rows == 1 || rows == 2) : (rows == 2 || rows == 4);
+  (void)key_bytes;
+  return rows == 2 || rows == 4;
 }
 
 template <typename K, int DIM, int R>
@@ -325,14 +338,13 @@ void launch_brt_seg(const K *keys, int mbits, int dim, const BatchDev &b, BrtNod
                     int32_t *parent, uint32_t *lsx, uint32_t *defer, RunState *st,
                     hipStream_t s) {
   if (b.n_brt <= 0) return;
-  constexpr bool wide = sizeof(K) == 16;
-  const bool small = b.brt_rows == (wide ? 1 : 2);
+  const bool small = b.brt_rows == 2;
   if (dim == 2) {
-    if (small) brt_seg_dim<K, 2, wide ? 1 : 2>(keys, mbits, b, brt, parent, lsx, defer, st, s);
-    else brt_seg_dim<K, 2, wide ? 2 : 4>(keys, mbits, b, brt, parent, lsx, defer, st, s);
+    if (small) brt_seg_dim<K, 2, 2>(keys, mbits, b, brt, parent, lsx, defer, st, s);
+    else brt_seg_dim<K, 2, 4>(keys, mbits, b, brt, parent, lsx, defer, st, s);
   } else {
-    if (small) brt_seg_dim<K, 3, wide ? 1 : 2>(keys, mbits, b, brt, parent, lsx, defer, st, s);
-    else brt_seg_dim<K, 3, wide ? 2 : 4>(keys, mbits, b, brt, parent, lsx, defer, st, s);
+    if (small) brt_seg_dim<K, 3, 2>(keys, mbits, b, brt, parent, lsx, defer, st, s);
+    else brt_seg_dim<K, 3, 4>(keys, mbits, b, brt, parent, lsx, defer, st, s);
   }
 }
 

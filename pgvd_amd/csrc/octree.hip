@@ -239,6 +239,10 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
   __shared__ uint32_t s_lnk[WN], s_ps[WN], s_lsx[WN];
   const uint32_t q = q0 + threadIdx.x;
   const int w0 = (int)q0 - H;
+  // q's own record is loaded first so that its latency overlaps the window
+  // load instead of following it.
+  BrtNodeT<K> nq{};
+  if (q <= m - 2) nq = brt[q];
   for (int r = threadIdx.x; r < WN; r += kBlock) {
     const int g = w0 + r;
     if (g >= 0 && g < (int)m) {
@@ -266,7 +270,7 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
     active = (q == 0) || lsq > 0;
   }
   if (active && staged) {
-    miss = !gather_one_lds<K, DIM>(q, brt[q], excl, lsq, w0, WN, s_lnk, s_ps, s_lsx, keys,
+    miss = !gather_one_lds<K, DIM>(q, nq, excl, lsq, w0, WN, s_lnk, s_ps, s_lsx, keys,
                                    conflict_bits, mbits, s_nodes + (excl - lo) * STRIDE,
                                    node_conflict, st);
   } else if (active) {
@@ -287,7 +291,7 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
       return true;
     };
     auto put = [&](uint32_t idx, const int32_t *c) { write_node<DIM>(octree, idx, c); };
-    miss = !gather_one<K, DIM>(q, brt[q], excl, lsq, links, child, keys, conflict_bits, mbits,
+    miss = !gather_one<K, DIM>(q, nq, excl, lsq, links, child, keys, conflict_bits, mbits,
                                put, node_conflict, st);
   }
   defer(miss, q);

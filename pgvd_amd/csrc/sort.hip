@@ -43,7 +43,7 @@ __device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t *s_wsum) {
 // the tile's index in the look-back array (tiles lo..tile-1 precede it in the
 // same run); ghist is the run's digit histogram for this pass.
 template <typename K, bool PAY, int IPT>
-__device__ inline void onesweep_tile(const K *__restrict__ kin, K *__restrict__ kout,
+__device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__restrict__ kin, K *__restrict__ kout,
                                      const uint32_t *__restrict__ vin, uint32_t *__restrict__ vout,
                                      uint32_t base, uint32_t nvalid, uint32_t tile, uint32_t lo,
                                      uint32_t seg0, int shift, const uint32_t *__restrict__ ghist,
