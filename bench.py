@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload uniform|gauss]
     python bench.py --workload batch [--batch-count 4096]   (batched form)
+    python bench.py --workload serve [--serve-rate 2000]    (served form)
 
 One step = the whole path (bounding box, FP32 quantize, 63-bit Morton keys,
 onesweep radix sort, unique, Karras BRT, local splits + scan, octree fill)
@@ -225,12 +226,114 @@ def bench_batch(args) -> int:
     return 0 if (same_counts and not mism) else 1
 
 
+def bench_serve(args) -> int:
+    """Served form: a Poisson stream of octree-build requests (mean
+    --serve-rate per second for --serve-seconds), each 2D (16 bits/axis,
+    32-bit keys) or 3D (21 bits/axis, 63-bit keys) with equal probability,
+    sizes log-uniform in [1e4, 4e6] points (seed 6), points in pinned host
+    memory; admitted against a 64 GB HBM budget onto --serve-slots contexts
+    whose pooled buffers stay on the device.  Latency = completion - arrival.
+    Every 50th request is rebuilt alone and compared (m, octree size, octree
+    digest)."""
+    import pgvd_amd as P
+
+    rng = np.random.default_rng(6)
+    t, arrivals = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / args.serve_rate)
+        if t > args.serve_seconds:
+            break
+        arrivals.append(t)
+    count = len(arrivals)
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(4e6), count)).astype(np.int64)
+    dims = rng.choice([2, 3], count)
+    pool_n, per_dim = 4_000_000, 8
+    pools = {}
+    for d in (2, 3):
+        for k in range(per_dim):
+            pa = P.PinnedArray((pool_n, d))
+            pa.array[:] = rng.random((pool_n, d), dtype=np.float32) * 2 - 1
+            pools[(d, k)] = pa
+    reqs = (P.Request * count)()
+    for i in range(count):
+        d = int(dims[i])
+        r = reqs[i]
+        r.host_pts = pools[(d, i % per_dim)].ptr
+        r.n, r.dim, r.bits, r.arrival_s = int(sizes[i]), d, 16 if d == 2 else 21, arrivals[i]
+    budget = 64 << 30
+    print(f"serve: {count} requests over {args.serve_seconds} s, {int(sizes.sum())} points",
+          file=sys.stderr, flush=True)
+    with P.Server(0, slots=args.serve_slots, hbm_budget=budget) as srv:
+        # warm the slots' pools with the largest requests of each dimension
+        warm = (P.Request * (2 * args.serve_slots))()
+        for j in range(len(warm)):
+            d = 2 + (j % 2)
+            warm[j].host_pts = pools[(d, 0)].ptr
+            warm[j].n, warm[j].dim, warm[j].bits, warm[j].arrival_s = pool_n, d, 16 if d == 2 else 21, 0.0
+        srv.run(warm)
+        st = srv.run(reqs)
+    lat = np.array([reqs[i].done_s - reqs[i].arrival_s for i in range(count)])
+    wait = np.array([reqs[i].admit_s - reqs[i].arrival_s for i in range(count)])
+    bad = [i for i in range(count) if reqs[i].status != 0]
+    # alone rebuilds of every 50th request
+    dev = P.Device(0)
+    mism = []
+    checked = 0
+    for i in range(0, count, 50):
+        r = reqs[i]
+        pts = np.array(pools[(r.dim, i % per_dim)].array[: r.n])
+        dpts = dev.upload(pts)
+        dev.build_octree_async(dpts, r.n, r.dim, r.bits)
+        rr = dev.build_octree_finish()
+        dg = dev.digest(rr.octree, rr.octree_size * ((1 << r.dim) + 1) * 4)
+        if (rr.m, rr.octree_size, dg) != (r.m, r.octree_size, r.octree_digest):
+            mism.append(i)
+        checked += 1
+        dpts.free()
+    dev.close()
+    for pa in pools.values():
+        pa.free()
+    total_pts = int(sizes.sum())
+    out = {
+        "metric": "served octree builds: throughput with p50/p99 latency (Poisson arrivals)",
+        "value": count / st.elapsed_s,
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "data": "synthetic",
+        "dtype": "u32/u64",
+        "config": {
+            "workload": f"Poisson arrivals at {args.serve_rate}/s for {args.serve_seconds} s; 2D "
+                        f"(16 bits/axis) or 3D (21 bits/axis) with p=1/2; sizes log-uniform in "
+                        f"[1e4, 4e6] (seed 6); points uniform in pinned host memory, copied per "
+                        f"request; 64 GB HBM budget; {args.serve_slots} slots",
+            "requests": count, "total_points": total_pts,
+        },
+        "offered_requests_per_s": args.serve_rate,
+        "points_per_s": total_pts / st.elapsed_s,
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                       "p99": float(np.percentile(lat, 99) * 1e3),
+                       "mean": float(lat.mean() * 1e3), "max": float(lat.max() * 1e3)},
+        "queue_wait_ms": {"p50": float(np.percentile(wait, 50) * 1e3),
+                          "p99": float(np.percentile(wait, 99) * 1e3)},
+        "max_in_flight": st.max_in_flight,
+        "max_in_flight_GB": st.max_in_flight_bytes / 1e9,
+        "failed": bad,
+        "alone_vs_mix": {"checked": checked, "mismatches": mism},
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if not bad and not mism else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="uniform", choices=["uniform", "gauss", "batch"])
+    ap.add_argument("--workload", default="uniform", choices=["uniform", "gauss", "batch", "serve"])
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
@@ -248,6 +351,8 @@ def main() -> int:
 
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20

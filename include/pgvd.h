@@ -286,6 +286,48 @@ int pgvd_batch_tune(pgvd_batch *b, const void *pts, int *sort_ipt, int *brt_rows
 /* Milliseconds of the last pgvd_batch_run, from events around it. */
 int pgvd_batch_last_ms(pgvd_batch *b, float *ms);
 
+/* ---- served form ---------------------------------------------------------
+ * A request loop over whole-path builds of different sizes and dimensions.
+ * It keeps `slots` contexts (one stream and one pooled set of buffers each,
+ * device-resident across requests).  A request is admitted, in arrival
+ * order, as soon as a slot is free and its device footprint fits the HBM
+ * budget beside the requests in flight; the builds of admitted requests
+ * overlap on the device.  No counterpart in the reference. */
+typedef struct pgvd_server pgvd_server;
+
+typedef struct pgvd_request {
+  /* in */
+  const float *host_pts;    /* n*dim floats in host memory (pinned: pgvd_host_alloc) */
+  int64_t n;
+  int dim, bits;
+  double arrival_s;         /* arrival time, seconds after the trace starts */
+  /* out */
+  double admit_s, done_s;   /* admission and completion times, same clock */
+  int64_t m, octree_size;
+  uint64_t octree_digest;   /* pgvd_digest of the octree nodes */
+  int status;               /* pgvd_status of the build */
+  int slot;
+} pgvd_request;
+
+typedef struct pgvd_server_stats {
+  double elapsed_s;          /* first arrival to last completion */
+  int max_in_flight;         /* most requests on the device at once */
+  size_t max_in_flight_bytes;
+  int64_t completed;
+} pgvd_server_stats;
+
+int pgvd_server_create(int device, int slots, size_t hbm_budget_bytes, pgvd_server **out);
+int pgvd_server_destroy(pgvd_server *s);
+/* Replays the requests (sorted by arrival_s) in real time and returns when
+ * all are done; per-request results go into reqs[i], totals into *stats
+ * (nullable).  Returns the first failing request's status, if any. */
+int pgvd_server_run(pgvd_server *s, pgvd_request *reqs, int64_t count, pgvd_server_stats *stats);
+/* Device bytes one whole-path build of n points occupies (its admission size). */
+size_t pgvd_build_footprint(int64_t n, int dim, int bits);
+/* Pinned host memory for request points. */
+int pgvd_host_alloc(size_t bytes, void **out);
+int pgvd_host_free(void *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,20 @@ class BatchResult(C.Structure):
     ]
 
 
+class Request(C.Structure):
+    _fields_ = [
+        ("host_pts", C.c_void_p), ("n", C.c_int64), ("dim", C.c_int), ("bits", C.c_int),
+        ("arrival_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double),
+        ("m", C.c_int64), ("octree_size", C.c_int64), ("octree_digest", C.c_uint64),
+        ("status", C.c_int), ("slot", C.c_int),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("elapsed_s", C.c_double), ("max_in_flight", C.c_int),
+                ("max_in_flight_bytes", C.c_size_t), ("completed", C.c_int64)]
+
+
 def lib():
     """Load libpgvd.so; raises if it has not been built."""
     global _LIB
@@ -126,6 +140,12 @@ def lib():
             "pgvd_batch_finish": ([vp, C.POINTER(BatchResult)], i32),
             "pgvd_batch_tune": ([vp, vp, C.POINTER(i32), C.POINTER(i32)], i32),
             "pgvd_batch_last_ms": ([vp, C.POINTER(C.c_float)], i32),
+            "pgvd_server_create": ([i32, i32, C.c_size_t, C.POINTER(vp)], i32),
+            "pgvd_server_destroy": ([vp], i32),
+            "pgvd_server_run": ([vp, C.POINTER(Request), i64, C.POINTER(ServerStats)], i32),
+            "pgvd_build_footprint": ([i64, i32, i32], C.c_size_t),
+            "pgvd_host_alloc": ([C.c_size_t, C.POINTER(vp)], i32),
+            "pgvd_host_free": ([vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -571,3 +591,60 @@ class Batch:
         else:
             out["lcp_lo"], out["lcp_hi"] = brt["lcp"].astype(np.uint64), np.zeros(ni, np.uint64)
         return out
+
+
+class PinnedArray:
+    """A numpy view of pinned host memory (pgvd_host_alloc)."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.dtype = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        rc = lib().pgvd_host_alloc(max(nbytes, 1), C.byref(p))
+        if rc != 0:
+            raise PgvdError(rc, lib().pgvd_last_error().decode())
+        self.ptr = p.value
+        buf = (C.c_char * max(nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            lib().pgvd_host_free(self.ptr)
+            self.ptr = None
+            self.array = None
+
+
+class Server:
+    """The served form (pgvd_server_*): `slots` contexts with pooled buffers,
+    admission in arrival order against an HBM budget, overlapped builds."""
+
+    def __init__(self, device: int = 0, slots: int = 4, hbm_budget: int = 64 << 30):
+        h = C.c_void_p()
+        rc = lib().pgvd_server_create(device, slots, hbm_budget, C.byref(h))
+        if rc != 0:
+            raise PgvdError(rc, lib().pgvd_last_error().decode())
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().pgvd_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def footprint(n: int, dim: int, bits: int) -> int:
+        return int(lib().pgvd_build_footprint(n, dim, bits))
+
+    def run(self, reqs) -> ServerStats:
+        """reqs: a ctypes array of Request sorted by arrival_s (filled in place)."""
+        st = ServerStats()
+        rc = lib().pgvd_server_run(self.h, reqs, len(reqs), C.byref(st))
+        if rc != 0:
+            raise PgvdError(rc, lib().pgvd_last_error().decode())
+        return st
+

@@ -7,6 +7,7 @@
 // the reference's two blocking reads (unique count, :214; octree size,
 // :347) are replaced by device-side counts that later kernels read, so the
 // whole path is one stream of launches and one small readback at the end.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -101,6 +102,29 @@ ZeroLayout zero_layout(int64_t n, int kb, int passes, bool labels) {
   z.total = off;
   return z;
 }
+
+}  // namespace
+
+size_t pgvd_capi::plain_footprint_bytes(int64_t n, int dim, int bits, bool labels) {
+  const int mbits = dim * bits;
+  const int kb = key_bytes_for(mbits);
+  const int64_t nk = n + 1;
+  const size_t node = kb == 4 ? sizeof(BrtNodeT<uint32_t>)
+                              : (kb == 8 ? sizeof(BrtNodeT<uint64_t>) : sizeof(BrtNodeT<Key128>));
+  const int64_t oct_nodes = std::max<int64_t>(2 * nk, 1024);
+  size_t b = 0;
+  b += 2 * (size_t)nk * kb;                              // keys_a, keys_b
+  if (labels) b += 2 * (size_t)n * 4;                    // vals_a, vals_b
+  b += (size_t)nk * node;                                // brt
+  b += 3 * (size_t)nk * 4;                               // ps, parent, defer
+  b += zero_layout(n, kb, (mbits + 7) / 8, labels).total;
+  b += (size_t)oct_nodes * ((1u << dim) + 1) * 4;        // octree
+  if (labels) b += (size_t)oct_nodes * 4;                // node_conflict
+  b += b / 8;                                            // pool headroom
+  return b + (size_t)n * dim * 4;                        // input points
+}
+
+namespace {
 
 int copy_state_and_wait(pgvd_ctx *ctx, hipStream_t s) {
   HIPCHK(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(RunState), hipMemcpyDeviceToHost, s));

@@ -143,4 +143,8 @@ inline hipEvent_t take_event(pgvd_ctx *ctx) {
   return e;
 }
 
+// Device bytes a whole-path build of n points takes in a context's pool
+// (every pooled buffer with its headroom, plus the input points).
+size_t plain_footprint_bytes(int64_t n, int dim, int bits, bool labels);
+
 }  // namespace pgvd_capi

@@ -131,6 +131,10 @@ void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 void launch_set_sizes(RunState *st, const uint32_t *ps, int64_t m, hipStream_t s);
 // *out (+)= digest of `words` 32-bit words at p: sum of mix64(w_i | i << 32).
 void launch_digest(const void *p, size_t words, uint64_t *out, bool zero_first, hipStream_t s);
+// *out = digest of count[0] * per_item words at p (at most n_max), the count
+// read on the device.
+void launch_digest_count(const void *p, const uint32_t *count, uint32_t per_item, size_t n_max,
+                         uint64_t *out, hipStream_t s);
 
 // ---- sort.hip ----
 // Onesweep LSD radix sort of n keys (+ optional u32 payload) over `passes`

@@ -47,6 +47,21 @@ __global__ __launch_bounds__(kBlock) void digest_kernel(const uint32_t *__restri
 
 __global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
 
+// Same digest over count[0] * per_item 32-bit words, the count read on the
+// device (a size only the device knows yet); n_max bounds the grid.
+__global__ __launch_bounds__(kBlock) void digest_count_kernel(const uint32_t *__restrict__ p,
+                                                              const uint32_t *count,
+                                                              uint32_t per_item, size_t n_max,
+                                                              unsigned long long *out) {
+  size_t n = (size_t)(*count) * per_item;
+  if (n > n_max) n = n_max;
+  uint64_t acc = 0;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+    acc += mix64((uint64_t)p[i] | ((uint64_t)i << 32));
+  acc = wave_reduce_add(acc);
+  if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+}
+
 inline unsigned grid_for(size_t n) {
   size_t g = (n + kBlock * 4 - 1) / (kBlock * 4);
   if (g < 1) g = 1;
@@ -73,6 +88,14 @@ void launch_digest(const void *p, size_t words, uint64_t *out, bool zero_first, 
   unsigned long long *o = reinterpret_cast<unsigned long long *>(out);
   if (zero_first) zero_u64_kernel<<<1, 1, 0, s>>>(o);
   if (words) digest_kernel<<<grid_for(words), kBlock, 0, s>>>(static_cast<const uint32_t *>(p), words, o);
+}
+
+void launch_digest_count(const void *p, const uint32_t *count, uint32_t per_item, size_t n_max,
+                         uint64_t *out, hipStream_t s) {
+  unsigned long long *o = reinterpret_cast<unsigned long long *>(out);
+  zero_u64_kernel<<<1, 1, 0, s>>>(o);
+  digest_count_kernel<<<grid_for(n_max ? n_max : 1), kBlock, 0, s>>>(
+      static_cast<const uint32_t *>(p), count, per_item, n_max, o);
 }
 
 void launch_set_sizes(RunState *st, const uint32_t *ps, int64_t m, hipStream_t s) {
