@@ -31,7 +31,14 @@ METRIC = "sorted Morton keys/sec, end-to-end octree build at 2^24 pts; % HBM roo
 
 
 def make_points(workload: str, n: int) -> tuple[np.ndarray, str]:
-    """Synthetic inputs (seeded)."""
+    """Synthetic inputs (seeded).  The conflict-cell workload returns labels
+    through make_points.labels."""
+    if workload == "conflict":
+        from pgvd_amd.workloads import star_loops
+        pts, labels = star_loops(n, loops=64, seed=4)
+        make_points.labels = labels
+        return pts, (f"3D conflict-cell build, {n} float3 on 64 star-shaped loops (256 vertices, "
+                     f"radii 0.05-0.3, seed 4) with uint32 loop ids")
     if workload == "uniform":
         rng = np.random.default_rng(2)
         pts = rng.random((n, 3), dtype=np.float32) * 2 - 1
@@ -64,7 +71,7 @@ def algorithmic_bytes(n: int, m: int, O: int, dim: int, K: int, R: int, L: int =
     return parts
 
 
-def cpu_baseline(pts: np.ndarray, bits: int, sample: int) -> dict:
+def cpu_baseline(pts: np.ndarray, bits: int, sample: int, labels=None) -> dict:
     import oracle
     L = oracle.lib()
     sub = np.ascontiguousarray(pts[:sample])
@@ -75,7 +82,11 @@ def cpu_baseline(pts: np.ndarray, bits: int, sample: int) -> dict:
         pinned = False
     m = C.c_int64()
     t0 = time.perf_counter()
-    osz = L.oracle_run_pipeline(sub.reshape(-1), sub.shape[0], sub.shape[1], bits, C.byref(m))
+    if labels is None:
+        osz = L.oracle_run_pipeline(sub.reshape(-1), sub.shape[0], sub.shape[1], bits, C.byref(m))
+    else:     # quantize, then the serial path with the point-label conflict pass
+        osz = oracle.build(oracle.quantize(sub, 1 << bits), bits,
+                           labels=np.ascontiguousarray(labels[:sample]))["octree_size"]
     dt = time.perf_counter() - t0
     if osz < 0:
         raise RuntimeError("oracle pipeline failed")
@@ -95,7 +106,8 @@ def cpu_baseline(pts: np.ndarray, bits: int, sample: int) -> dict:
         "kind": "port",
         "sample": (f"first {sub.shape[0]} points of the same workload, whole serial path "
                    f"(bbox, quantize, bit-serial Morton, comparison sort, unique, per-node BRT, "
-                   f"Hillis-Steele scan, serial octree) in {dt:.2f} s on 1 core "
+                   f"Hillis-Steele scan, serial octree"
+                   f"{', point-label conflict pass' if labels is not None else ''}) in {dt:.2f} s on 1 core "
                    f"({'pinned' if pinned else 'unpinned'}) of {os.cpu_count()} x {cpu}"),
     }
 
@@ -330,12 +342,13 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="uniform", choices=["uniform", "gauss", "batch", "serve"])
+    ap.add_argument("--workload", default="uniform",
+                    choices=["uniform", "gauss", "conflict", "batch", "serve"])
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--batch-count", type=int, default=4096)
-    ap.add_argument("--n", type=int, default=1 << 24)
+    ap.add_argument("--n", type=int, default=0, help="points (default 2^24; 2^22 for conflict)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -369,14 +382,17 @@ def main() -> int:
         dev.close()
         return 0
 
-    dim, bits = 3, 21
-    n = args.n
+    conflict = args.workload == "conflict"
+    dim, bits = 3, (42 if conflict else 21)
+    n = args.n or (1 << 22 if conflict else 1 << 24)
     pts, desc = make_points(args.workload, n)
+    labels = make_points.labels if conflict else None
     dev = P.Device(0)
     dpts = dev.upload(pts)
+    dlab = dev.upload(labels) if conflict else None
 
     for _ in range(args.warmup):
-        dev.build_octree_async(dpts, n, dim, bits)
+        dev.build_octree_async(dpts, n, dim, bits, dlab)
         res = dev.build_octree_finish()
 
     # Timed region.  HIP events cost ~10 us of queue time each, so the only
@@ -388,7 +404,7 @@ def main() -> int:
     dev.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        dev.build_octree_async(dpts, n, dim, bits)
+        dev.build_octree_async(dpts, n, dim, bits, dlab)
     res = dev.build_octree_finish()
     dev.synchronize()
     elapsed = time.perf_counter() - t0
@@ -402,8 +418,13 @@ def main() -> int:
     dev.timing_reset()
     prof_steps = 3
     for _ in range(prof_steps):
-        dev.build_octree_async(dpts, n, dim, bits)
-    dev.build_octree_finish()
+        dev.build_octree_async(dpts, n, dim, bits, dlab)
+    prof = dev.build_octree_finish()
+    flagged = None
+    if conflict:
+        got = dev.fetch(prof)
+        flagged = {"conflict_keys": int(got["key_conflict"].sum()),
+                   "nodes_with_conflict_octants": int((got["node_conflict"] != 0).sum())}
     stage_ms = {}
     for st in ["prep", "bbox", "encode", "sort_pass", "unique", "brt", "splits_scan",
                "malformed", "octree"]:
@@ -412,13 +433,18 @@ def main() -> int:
     dev.timing(False)
 
     m, O = int(res.m), int(res.octree_size)
-    ab = algorithmic_bytes(n, m, O, dim, int(res.key_bytes), passes)
+    lb = 4 if conflict else 0
+    ab = algorithmic_bytes(n, m, O, dim, int(res.key_bytes), passes, lb)
+    if conflict:      # the conflict-flag term of the byte model: labels read + m/8 flag bits
+        ab["total"] += lb * n + m // 8
     pass_gbps = ab["sort_pass"] / (pass_ms * 1e-3) / 1e9
     total_gbps = ab["total"] / (ms_step * 1e-3) / 1e9
     traffic = load_traffic("sort_pass", args.workload)
 
     out = {
-        "metric": METRIC,
+        "metric": (METRIC if not conflict else
+                   "sorted Morton keys/sec, conflict-cell octree build at 2^22 pts "
+                   "(126-bit keys + uint32 labels); % HBM roofline"),
         "value": n / (ms_step * 1e-3),
         "unit": "keys/s",
         "n_gpus": 1,
@@ -428,16 +454,18 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u64",
+        "dtype": "u128" if conflict else "u64",
         "data": "synthetic",
         "config": {
             "workload": f"{desc}, {bits} bits/axis, {dim * bits}-bit keys; full path "
                         "points-in-HBM to octree-in-HBM",
             "n_points": n, "dim": dim, "bits": bits, "distribution": args.workload,
             "unique_keys": m, "octree_nodes": O, "malformed_brt": int(res.malformed),
+            **({"labels": "uint32 loop id per point", "flags": flagged} if conflict else {}),
         },
         "roofline": {
-            "kernel": "onesweep_kernel (one 8-bit radix pass of 2^24 u64 keys)",
+            "kernel": ("onesweep_kernel (one 8-bit radix pass of 2^22 u128 keys with u32 labels)"
+                       if conflict else "onesweep_kernel (one 8-bit radix pass of 2^24 u64 keys)"),
             "bound": "hbm",
             "achieved": pass_gbps,
             "peak": HBM_PEAK_GBPS,
@@ -457,7 +485,7 @@ def main() -> int:
     }
     dev.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pts, bits, min(args.cpu_sample, n))
+        out["cpu_baseline"] = cpu_baseline(pts, bits, min(args.cpu_sample, n), labels)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -170,6 +170,27 @@ def test_labels_conflict_cells(dev):
         assert got["key_conflict"].sum() > 0
 
 
+def test_conflict_cells_full_size(dev):
+    """The conflict-cell workload at full size: 2^22 points on 64 star-shaped
+    loops, 42 bits/axis (126-bit keys), loop ids as labels -- every output
+    against the oracle.  At this resolution no two loops share a cell, so a
+    second run copies 50,000 samples of some loops onto others (the same
+    coordinates under another label) to exercise the flags at full size."""
+    pts, labels = workloads.star_loops(1 << 22, loops=64, seed=4)
+    rng = np.random.default_rng(11)
+    shared = pts.copy()
+    src = rng.choice(pts.shape[0], 50_000, replace=False)
+    dst = rng.choice(pts.shape[0], 50_000, replace=False)
+    shared[dst] = pts[src]
+    for name, p in (("loops", pts), ("shared samples", shared)):
+        got = dev.build_octree(p, 42, labels=labels)
+        want = oracle.build(oracle.quantize(p, 1 << 42), 42, labels=labels)
+        assert_tree_equal(got, want, where=name)
+        assert np.array_equal(got["key_conflict"], want["key_conflict"]), name
+        assert np.array_equal(got["node_conflict"], want["node_conflict"]), name
+    assert want["key_conflict"].sum() > 1000
+
+
 # ------------------------------------------------------------ stage APIs ----
 def test_points_to_morton_reference_input(dev):
     """PointsToMorton on (i, i), bits = 30 (tests/KernelTests.cpp:57-84),
