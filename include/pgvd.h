@@ -246,7 +246,8 @@ typedef struct pgvd_batch_opts {
   int quantized_input;  /* points are int32 grid coordinates (no bbox)          */
   int clamped;          /* quantize to width instead of width-1 (Karras.cpp:71) */
   int use_graph;        /* replay the pipeline as a captured HIP graph (default) */
-  int sort_ipt;         /* keys per thread of the sort tile (0: default)        */
+  int sort_ipt;         /* keys per thread of the sort tile: 8, 12, 15 or 16    */
+                        /* (4, 7 or 8 for 128-bit keys); 0: default           */
   int brt_rows;         /* BRT nodes per thread of a workgroup window (0: def.) */
 } pgvd_batch_opts;
 

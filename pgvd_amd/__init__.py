@@ -28,7 +28,8 @@ from dataclasses import dataclass
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpgvd.so")
+# PGVD_LIB selects another build of the library (e.g. to compare tile shapes).
+LIB_PATH = os.environ.get("PGVD_LIB") or os.path.join(HERE, "libpgvd.so")
 ROOT = os.path.dirname(HERE)
 HEADER = os.path.join(ROOT, "include", "pgvd.h")
 

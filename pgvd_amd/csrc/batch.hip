@@ -306,7 +306,7 @@ int pgvd_batch_create(pgvd_ctx *ctx, const int64_t *sizes, int64_t count, int di
   b->B = count;
   b->total_pts = total;
   b->total_slots = slots;
-  b->sort_ipt = b->kb == 16 ? 8 : 16;
+  b->sort_ipt = b->kb == 16 ? 7 : 15;
   b->brt_rows = 4;
   if (opts && opts->sort_ipt) {
     if (!onesweep_seg_ipt_ok(b->kb, opts->sort_ipt)) { delete b; return fail(PGVD_ERR_INVALID, "unsupported sort_ipt for this key width"); }
@@ -437,7 +437,7 @@ int pgvd_batch_tune(pgvd_batch *b, const void *pts, int *sort_ipt, int *brt_rows
   CHK(use_device(b->ctx));
   const bool graph = b->use_graph;
   b->use_graph = false;
-  const std::vector<int> ipts = b->kb == 16 ? std::vector<int>{4, 8} : std::vector<int>{8, 12, 16};
+  const std::vector<int> ipts = b->kb == 16 ? std::vector<int>{4, 7, 8} : std::vector<int>{8, 12, 15, 16};
   const std::vector<int> rows = {2, 4};
   float best = 1e30f;
   int bi = b->sort_ipt, br = b->brt_rows;

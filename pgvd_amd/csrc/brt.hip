@@ -84,20 +84,23 @@ __global__ __launch_bounds__(kBlock) void brt_kernel(const K *__restrict__ keys,
 // If some node took an FP32-quirk split (st->malformed), parent links, local
 // splits and prefix sums are recomputed by the order-exact claim path.
 template <typename K, int DIM>
-__device__ inline void brt_store(int c, K cur, const BrtOut<K> &o, int mbits,
-                                 BrtNodeT<K> *__restrict__ brt, int32_t *__restrict__ parent,
-                                 uint32_t *__restrict__ lsx) {
+__device__ inline uint32_t brt_store(int c, K cur, const BrtOut<K> &o, int mbits,
+                                     BrtNodeT<K> *__restrict__ brt, int32_t *__restrict__ parent,
+                                     uint32_t *__restrict__ lsx) {
   store_brt(brt + c, o.nd);
   if (!o.nd.left_leaf) parent[o.nd.left] = c;
   if (!o.nd.right_leaf) parent[o.nd.left + 1] = c;
   const int len = o.nd.lcp_length;
   if (c == 0) {
     parent[0] = -1;
-    lsx[0] = 1u + (uint32_t)(len / DIM);
-  } else {
-    const int plvl = o.lcp_min / DIM;
-    lsx[c] = (uint32_t)(len / DIM - plvl) | ((uint32_t)digit_at<K, DIM>(cur, plvl, mbits) << 8);
+    const uint32_t ls = 1u + (uint32_t)(len / DIM);
+    lsx[0] = ls;
+    return ls;
   }
+  const int plvl = o.lcp_min / DIM;
+  const uint32_t ls = (uint32_t)(len / DIM - plvl);
+  lsx[c] = ls | ((uint32_t)digit_at<K, DIM>(cur, plvl, mbits) << 8);
+  return ls;
 }
 
 // Adjacent-key deltas staged per workgroup: its SEG = R*256 nodes plus H
@@ -111,10 +114,14 @@ template <typename K> struct BrtWin { static constexpr int R = 4, H = 256; };
 // key probes.  A node whose range or split leaves the window is handed to
 // defer(true, c) and left to the global-memory pass.  lsx[m-1] = 0;
 // m == 1 gives lsx[0] = 1.  Every lane calls defer once per row, converged.
+// With chunk_tot, each row of 256 nodes (a chunk) also stores the sum of
+// its local splits, deferred nodes left out (their pass adds them).
 template <typename K, int DIM, int R, typename Defer>
 __device__ inline void brt_window(const K *__restrict__ keys, int m, int b0, int mbits,
                                   BrtNodeT<K> *__restrict__ brt, int32_t *__restrict__ parent,
-                                  uint32_t *__restrict__ lsx, RunState *st, Defer defer) {
+                                  uint32_t *__restrict__ lsx, uint32_t *__restrict__ chunk_tot,
+                                  RunState *st, Defer defer) {
+  __shared__ uint32_t s_rowsum[R * (kBlock / 64)];
   constexpr int H = BrtWin<K>::H, SEG = R * kBlock, WN = SEG + 2 * H;
   constexpr int LV = (WN >= 2048) ? 12 : (WN >= 1024 ? 11 : (WN >= 512 ? 10 : 9));
   static_assert((1 << (LV - 1)) <= WN && WN < (1 << LV), "levels cover the window");
@@ -147,17 +154,31 @@ __device__ inline void brt_window(const K *__restrict__ keys, int m, int b0, int
   for (int r = 0; r < R; ++r) {
     const int c = b0 + r * kBlock + (int)threadIdx.x;
     bool miss = false;
+    uint32_t ls = 0;
     if (c < m - 1) {
       BrtOut<K> o;
       miss = !brt_node_window(cur[r], mn, LV, w0, WN, mbits, m, c, o);
       if (!miss) {
         brt_flags(o, st);
-        brt_store<K, DIM>(c, cur[r], o, mbits, brt, parent, lsx);
+        ls = brt_store<K, DIM>(c, cur[r], o, mbits, brt, parent, lsx);
       }
     } else if (c == m - 1) {
-      lsx[c] = (m == 1) ? 1u : 0u;
+      ls = (m == 1) ? 1u : 0u;
+      lsx[c] = ls;
     }
     defer(miss, c);
+    if (chunk_tot) {
+      const uint32_t sum = wave_reduce_add(ls);
+      if ((threadIdx.x & 63) == 0) s_rowsum[r * (kBlock / 64) + (threadIdx.x >> 6)] = sum;
+    }
+  }
+  if (chunk_tot) {
+    __syncthreads();
+    if (threadIdx.x < R && b0 + (int)threadIdx.x * kBlock < m) {
+      uint32_t tot = 0;
+      for (int w = 0; w < kBlock / 64; ++w) tot += s_rowsum[threadIdx.x * (kBlock / 64) + w];
+      chunk_tot[(b0 >> 8) + threadIdx.x] = tot;
+    }
   }
 }
 
@@ -166,15 +187,17 @@ __global__ __launch_bounds__(kBlock) void brt_ls_kernel(const K *__restrict__ ke
                                                         BrtNodeT<K> *__restrict__ brt,
                                                         int32_t *__restrict__ parent,
                                                         uint32_t *__restrict__ lsx,
+                                                        uint32_t *__restrict__ chunk_tot,
                                                         uint32_t *__restrict__ defer,
                                                         RunState *st) {
   constexpr int R = BrtWin<K>::R, SEG = R * kBlock;
   const int m = (int)st->m;
   const int b0 = (int)xcd_segment(blockIdx.x, gridDim.x) * SEG;
   if (b0 >= m) return;                      // uniform over the workgroup
-  brt_window<K, DIM, R>(keys, m, b0, mbits, brt, parent, lsx, st, [&](bool miss, int c) {
-    wave_append(miss, (uint32_t)c, &st->counters[CTR_BRT_DEFER], defer);
-  });
+  brt_window<K, DIM, R>(keys, m, b0, mbits, brt, parent, lsx, chunk_tot, st,
+                        [&](bool miss, int c) {
+                          wave_append(miss, (uint32_t)c, &st->counters[CTR_BRT_DEFER], defer);
+                        });
 }
 
 // Pipeline, second pass: the nodes the window could not hold, against keys
@@ -185,6 +208,7 @@ __global__ __launch_bounds__(kBlock) void brt_ls_defer_kernel(const K *__restric
                                                               BrtNodeT<K> *__restrict__ brt,
                                                               int32_t *__restrict__ parent,
                                                               uint32_t *__restrict__ lsx,
+                                                              uint32_t *__restrict__ chunk_tot,
                                                               const uint32_t *__restrict__ defer,
                                                               RunState *st) {
   const int m = (int)st->m;
@@ -194,7 +218,8 @@ __global__ __launch_bounds__(kBlock) void brt_ls_defer_kernel(const K *__restric
     const K cur = keys[c];
     const BrtOut<K> o = brt_eval(cur, [&](int x) { return keys[x]; }, mbits, m, c);
     brt_flags(o, st);
-    brt_store<K, DIM>(c, cur, o, mbits, brt, parent, lsx);
+    const uint32_t ls = brt_store<K, DIM>(c, cur, o, mbits, brt, parent, lsx);
+    if (ls) atomicAdd(&chunk_tot[c >> 8], ls);
   }
 }
 
@@ -218,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void brt_seg_kernel(const K *__restrict__ k
   const int b0 = (int)(tile - tt.first[s]) * SEG;
   if (b0 >= m) return;
   const uint32_t kofs = seg[s].kofs;
-  brt_window<K, DIM, R>(keys + kofs, m, b0, mbits, brt + kofs, parent + kofs, lsx + kofs, st,
+  brt_window<K, DIM, R>(keys + kofs, m, b0, mbits, brt + kofs, parent + kofs, lsx + kofs, nullptr, st,
                         [&](bool miss, int c) {
                           wave_append(miss, kofs + (uint32_t)c, &st->counters[CTR_BRT_DEFER],
                                       defer);
@@ -247,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void brt_seg_defer_kernel(const K *__restri
     const K cur = kk[c];
     const BrtOut<K> o = brt_eval(cur, [&](int x) { return kk[x]; }, mbits, m, c);
     brt_flags(o, st);
-    brt_store<K, DIM>(c, cur, o, mbits, brt + kofs, parent + kofs, lsx + kofs);
+    (void)brt_store<K, DIM>(c, cur, o, mbits, brt + kofs, parent + kofs, lsx + kofs);
   }
 }
 
@@ -299,24 +324,26 @@ void launch_finalize_parents(int32_t *parent, int64_t max_m, int64_t m_host, Run
 
 template <typename K, int DIM>
 void launch_brt_ls_dim(const K *keys, int64_t max_m, int mbits, BrtNodeT<K> *brt,
-                       int32_t *parent, uint32_t *lsx, uint32_t *defer, RunState *st,
-                       hipStream_t s) {
+                       int32_t *parent, uint32_t *lsx, uint32_t *chunk_tot, uint32_t *defer,
+                       RunState *st, hipStream_t s) {
   constexpr int SEG = BrtWin<K>::R * kBlock;
   const unsigned grid = (unsigned)((max_m + SEG - 1) / SEG);
-  brt_ls_kernel<K, DIM><<<grid, kBlock, 0, s>>>(keys, mbits, brt, parent, lsx, defer, st);
+  brt_ls_kernel<K, DIM><<<grid, kBlock, 0, s>>>(keys, mbits, brt, parent, lsx, chunk_tot, defer,
+                                                st);
   // The overflow pass grid-strides over however many nodes were deferred.
   const int64_t g2 = (max_m + kBlock - 1) / kBlock;
   const unsigned grid2 = (unsigned)(g2 < 2048 ? g2 : 2048);
-  brt_ls_defer_kernel<K, DIM><<<grid2, kBlock, 0, s>>>(keys, mbits, brt, parent, lsx, defer, st);
+  brt_ls_defer_kernel<K, DIM><<<grid2, kBlock, 0, s>>>(keys, mbits, brt, parent, lsx, chunk_tot,
+                                                       defer, st);
 }
 
 template <typename K>
 void launch_brt_ls(const K *keys, int64_t max_m, int mbits, int dim, BrtNodeT<K> *brt,
-                   int32_t *parent, uint32_t *lsx, uint32_t *defer, RunState *st,
-                   hipStream_t s) {
+                   int32_t *parent, uint32_t *lsx, uint32_t *chunk_tot, uint32_t *defer,
+                   RunState *st, hipStream_t s) {
   if (max_m < 1) max_m = 1;
-  if (dim == 2) launch_brt_ls_dim<K, 2>(keys, max_m, mbits, brt, parent, lsx, defer, st, s);
-  else launch_brt_ls_dim<K, 3>(keys, max_m, mbits, brt, parent, lsx, defer, st, s);
+  if (dim == 2) launch_brt_ls_dim<K, 2>(keys, max_m, mbits, brt, parent, lsx, chunk_tot, defer, st, s);
+  else launch_brt_ls_dim<K, 3>(keys, max_m, mbits, brt, parent, lsx, chunk_tot, defer, st, s);
 }
 
 bool brt_seg_rows_ok(int key_bytes, int rows) {
@@ -364,7 +391,7 @@ void launch_claims_if_malformed(const BrtNodeT<K> *brt, int32_t *parent, int64_t
   template void launch_claims_if_malformed<K>(const BrtNodeT<K> *, int32_t *, int64_t,         \
                                               RunState *, hipStream_t);                        \
   template void launch_brt_ls<K>(const K *, int64_t, int, int, BrtNodeT<K> *, int32_t *,       \
-                                 uint32_t *, uint32_t *, RunState *, hipStream_t);             \
+                                 uint32_t *, uint32_t *, uint32_t *, RunState *, hipStream_t); \
   template void launch_brt_seg<K>(const K *, int, int, const BatchDev &, BrtNodeT<K> *,        \
                                   int32_t *, uint32_t *, uint32_t *, RunState *, hipStream_t);
 PGVD_INST(uint32_t)

@@ -117,6 +117,7 @@ size_t pgvd_capi::plain_footprint_bytes(int64_t n, int dim, int bits, bool label
   if (labels) b += 2 * (size_t)n * 4;                    // vals_a, vals_b
   b += (size_t)nk * node;                                // brt
   b += 3 * (size_t)nk * 4;                               // ps, parent, defer
+  b += (size_t)((nk + kBlock - 1) / kBlock) * 4;         // chunks
   b += zero_layout(n, kb, (mbits + 7) / 8, labels).total;
   b += (size_t)oct_nodes * ((1u << dim) + 1) * 4;        // octree
   if (labels) b += (size_t)oct_nodes * 4;                // node_conflict
@@ -204,6 +205,8 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   int32_t *parent = static_cast<int32_t *>(p);
   CHK(pool_get(ctx, "defer", (size_t)nk * 4, &p));
   uint32_t *defer = static_cast<uint32_t *>(p);
+  CHK(pool_get(ctx, "chunks", (size_t)((nk + kBlock - 1) / kBlock) * 4, &p));
+  uint32_t *chunks = static_cast<uint32_t *>(p);
   const ZeroLayout z = zero_layout(n, kb, passes, lab);
   CHK(pool_get(ctx, "zero", z.total, &p));
   char *zb = static_cast<char *>(p);
@@ -267,13 +270,14 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   {
     Stage t(ctx, "brt", s);
     // packed local splits go to the sort's spare key buffer (free after
-    // unique)
-    launch_brt_ls<K>(uniq, nk, mbits, dim, brt, parent, lsx, defer, ctx->st, s);
+    // unique); their totals per chunk of 256 nodes to chunks
+    launch_brt_ls<K>(uniq, nk, mbits, dim, brt, parent, lsx, chunks, defer, ctx->st, s);
   }
   CHK(launch_status("brt"));
   {
+    // chunk bases and the octree size; the gather writes ps from them
     Stage t(ctx, "splits_scan", s);
-    launch_ls_scan(lsx, ps, nk, lb_b, ctx->st, s);
+    launch_chunk_scan(chunks, nk, lb_b, ctx->st, s);
   }
   CHK(launch_status("splits_scan"));
   {
@@ -286,7 +290,7 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   CHK(launch_status("malformed path"));
   {
     Stage t(ctx, "octree", s);
-    launch_octree<K>(brt, parent, ps, lsx, uniq, cbits, nk, dim, mbits, octree, nconf,
+    launch_octree<K>(brt, parent, ps, lsx, chunks, uniq, cbits, nk, dim, mbits, octree, nconf,
                      ctx->octree_cap, 0, defer, ctx->st, s);
   }
   CHK(launch_status("octree"));
@@ -305,6 +309,7 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   L.parent = parent;
   L.ps = ps;
   L.lsx = lsx;
+  L.chunks = chunks;
   L.defer = defer;
   L.conflict_bits = cbits;
   L.stream = s;
@@ -321,7 +326,7 @@ int refill_octree(pgvd_ctx *ctx, int64_t nodes) {
   ctx->st_host->status &= ~(uint32_t)ST_CAPACITY;
   HIPCHK(hipMemcpyAsync(&ctx->st->status, &ctx->st_host->status, 4, hipMemcpyHostToDevice,
                         L.stream));
-  launch_octree<K>(static_cast<const BrtNodeT<K> *>(L.brt), L.parent, L.ps, L.lsx,
+  launch_octree<K>(static_cast<const BrtNodeT<K> *>(L.brt), L.parent, L.ps, L.lsx, L.chunks,
                    static_cast<const K *>(L.keys_u), L.conflict_bits, L.n + 1, L.dim, L.mbits,
                    octree, nconf, ctx->octree_cap, 0, L.defer, ctx->st, L.stream);
   CHK(launch_status("octree refill"));
@@ -430,8 +435,8 @@ int brt_to_octree_impl(pgvd_ctx *ctx, const BrtNodeT<K> *brt, const int32_t *par
   {
     Stage t(ctx, "octree", s);
     // mbits is not needed without conflict flags; keys are not read.
-    launch_octree<K>(brt, parent, ps, nullptr, nullptr, nullptr, m, dim, 0, octree, nullptr,
-                     capacity, 1, nullptr, ctx->st, s);
+    launch_octree<K>(brt, parent, ps, nullptr, nullptr, nullptr, nullptr, m, dim, 0, octree,
+                     nullptr, capacity, 1, nullptr, ctx->st, s);
   }
   CHK(launch_status("octree"));
   CHK(copy_state_and_wait(ctx, s));

@@ -72,6 +72,7 @@ struct LastBuild {
   int32_t *parent = nullptr;
   uint32_t *ps = nullptr;
   uint32_t *lsx = nullptr;
+  uint32_t *chunks = nullptr;
   uint32_t *defer = nullptr;
   uint32_t *conflict_bits = nullptr;
   hipStream_t stream = nullptr;

@@ -183,14 +183,16 @@ void launch_finalize_parents(int32_t *parent, int64_t max_m, int64_t m_host, Run
                              hipStream_t s);
 // Pipeline: every BRT node with its local split and attach octant packed as
 // lsx[c] = ls | octant << 8 (lsx[0..m)), parent links as plain stores
-// (well-formed tree; see brt.hip).  defer[] holds up to max_m indices.
+// (well-formed tree; see brt.hip), and chunk_tot[k] = the local splits of
+// nodes [256k, 256k+256) summed.  defer[] holds up to max_m indices.
 template <typename K>
 void launch_brt_ls(const K *keys, int64_t max_m, int mbits, int dim, BrtNodeT<K> *brt,
-                   int32_t *parent, uint32_t *lsx, uint32_t *defer, RunState *st,
-                   hipStream_t s);
-// Inclusive scan of (lsx[0..st->m) & 0xff) into ps; st->octree_size = total.
-void launch_ls_scan(const uint32_t *ls, uint32_t *ps, int64_t max_m, uint64_t *lookback,
-                    RunState *st, hipStream_t s);
+                   int32_t *parent, uint32_t *lsx, uint32_t *chunk_tot, uint32_t *defer,
+                   RunState *st, hipStream_t s);
+// chunk_tot[0..ceil(m/256)) -> exclusive prefix sums in place;
+// st->octree_size = total (the octree size).
+void launch_chunk_scan(uint32_t *chunk_tot, int64_t max_m, uint64_t *lookback, RunState *st,
+                       hipStream_t s);
 // When st->malformed: parent[] <- claim form (zero, then atomicMax claims).
 template <typename K>
 void launch_claims_if_malformed(const BrtNodeT<K> *brt, int32_t *parent, int64_t max_m,
@@ -202,12 +204,14 @@ void launch_claims_if_malformed(const BrtNodeT<K> *brt, int32_t *parent, int64_t
 // from the caller; lsx and defer may then be null); otherwise st->malformed
 // selects it on the device and the gather path reads lsx from
 // launch_brt_ls and uses defer[] (max_m entries) as its overflow list.
+// The gather path also writes ps[0..m) from chunk_excl (launch_chunk_scan)
+// and lsx.
 template <typename K>
-void launch_octree(const BrtNodeT<K> *brt, const int32_t *parent, const uint32_t *ps,
-                   const uint32_t *lsx, const K *keys, const uint32_t *conflict_bits,
-                   int64_t max_m, int dim, int mbits, int32_t *octree,
-                   uint32_t *node_conflict, int64_t capacity, int force_scatter,
-                   uint32_t *defer, RunState *st, hipStream_t s);
+void launch_octree(const BrtNodeT<K> *brt, const int32_t *parent, uint32_t *ps,
+                   const uint32_t *lsx, const uint32_t *chunk_excl, const K *keys,
+                   const uint32_t *conflict_bits, int64_t max_m, int dim, int mbits,
+                   int32_t *octree, uint32_t *node_conflict, int64_t capacity,
+                   int force_scatter, uint32_t *defer, RunState *st, hipStream_t s);
 // Local splits the reference's way (each node writes its internal children's
 // entries, highest claimer last): ComputeLocalSplits_p for a caller's BRT.
 template <typename K>

@@ -23,6 +23,7 @@
 //           resolved by atomicMax -- the top-of-chain index ps[c]-1 grows
 //           with c, so the largest value is the serial loop's last writer.
 #include "kernels.hpp"
+#include "tilescan.hpp"
 
 namespace pgvd {
 
@@ -224,10 +225,17 @@ constexpr int kGatherHalo = 256;
 // to defer(true, q) and left to the global-memory pass (whose writes land
 // after this kernel's, so the window pass's placeholder output for it is
 // overwritten).  Every lane calls defer once, converged.
+// With chunk_excl (exclusive prefix sums of the local splits per chunk of
+// 256 nodes), the window's prefix sums are computed here from lsx instead
+// of read from ps, and the workgroup writes them to ps_out for its own
+// nodes (and for node m-1 when that starts the next chunk).
 template <typename K, int DIM, typename Defer>
 __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
                                      const uint32_t *__restrict__ ps,
-                                     const uint32_t *__restrict__ lsx, const K *__restrict__ keys,
+                                     const uint32_t *__restrict__ lsx,
+                                     const uint32_t *__restrict__ chunk_excl,
+                                     uint32_t *__restrict__ ps_out,
+                                     const K *__restrict__ keys,
                                      const uint32_t *__restrict__ conflict_bits, int mbits,
                                      uint32_t m, uint32_t q0, int32_t *__restrict__ octree,
                                      uint32_t *__restrict__ node_conflict, RunState *st,
@@ -243,12 +251,38 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
   // load instead of following it.
   BrtNodeT<K> nq{};
   if (q <= m - 2) nq = brt[q];
-  for (int r = threadIdx.x; r < WN; r += kBlock) {
-    const int g = w0 + r;
-    if (g >= 0 && g < (int)m) {
-      s_ps[r] = ps[g];
-      s_lsx[r] = lsx[g];
-      if (g < (int)m - 1) s_lnk[r] = load_links(brt + g);
+  if (chunk_excl) {
+    // The window is three whole chunks (q0 is a multiple of 256), so the
+    // base of the first one plus a running scan gives every prefix sum.
+    static_assert(WN == 3 * kBlock && H == kBlock, "window of three chunks");
+    __shared__ uint32_t s_w[3 * 4];
+    uint32_t v[3], ex[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int k = r * kBlock + (int)threadIdx.x;
+      const int g = w0 + k;
+      uint32_t lx = 0;
+      if (g >= 0 && g < (int)m) {
+        lx = lsx[g];
+        if (g < (int)m - 1) s_lnk[k] = load_links(brt + g);
+      }
+      s_lsx[k] = lx;
+      v[r] = lx & 0xffu;
+    }
+    const uint32_t base = q0 ? chunk_excl[(q0 >> 8) - 1] : 0u;
+    (void)striped_excl_scan<3, uint32_t>(v, ex, s_w);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) s_ps[r * kBlock + threadIdx.x] = base + ex[r] + v[r];
+    if (q < m) ps_out[q] = base + ex[1] + v[1];
+    if (threadIdx.x == 0 && q0 + kBlock == m - 1) ps_out[m - 1] = base + ex[2] + v[2];
+  } else {
+    for (int r = threadIdx.x; r < WN; r += kBlock) {
+      const int g = w0 + r;
+      if (g >= 0 && g < (int)m) {
+        s_ps[r] = ps[g];
+        s_lsx[r] = lsx[g];
+        if (g < (int)m - 1) s_lnk[r] = load_links(brt + g);
+      }
     }
   }
   __syncthreads();
@@ -316,13 +350,15 @@ __device__ inline void single_node(int32_t *octree, uint32_t *node_conflict) {
 // Deferred entries are appended only by a run that fills the octree, so the
 // counter is still zero when a refill after a capacity failure runs this
 // again.
+// With chunk_excl the prefix sums ps[0..m) are written here (see
+// gather_window); without it they are read.
 template <typename K, int DIM>
 __global__ __launch_bounds__(kBlock) void gather_kernel(
-    const BrtNodeT<K> *__restrict__ brt, const uint32_t *__restrict__ ps,
-    const uint32_t *__restrict__ lsx, const K *__restrict__ keys,
-    const uint32_t *__restrict__ conflict_bits, int mbits, int32_t *__restrict__ octree,
-    uint32_t *__restrict__ node_conflict, int64_t capacity, uint32_t *__restrict__ defer,
-    RunState *st) {
+    const BrtNodeT<K> *__restrict__ brt, uint32_t *__restrict__ ps,
+    const uint32_t *__restrict__ lsx, const uint32_t *__restrict__ chunk_excl,
+    const K *__restrict__ keys, const uint32_t *__restrict__ conflict_bits, int mbits,
+    int32_t *__restrict__ octree, uint32_t *__restrict__ node_conflict, int64_t capacity,
+    uint32_t *__restrict__ defer, RunState *st) {
   if (st->malformed) return;
   const uint32_t m = st->m;
   const uint32_t q0 = xcd_segment(blockIdx.x, gridDim.x) * kBlock;
@@ -331,11 +367,15 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(
     return;
   }
   if (m == 1) {             // single unique key: one node, no children
-    if (q0 == 0 && threadIdx.x == 0) single_node<DIM>(octree, node_conflict);
+    if (q0 == 0 && threadIdx.x == 0) {
+      single_node<DIM>(octree, node_conflict);
+      if (chunk_excl) ps[0] = 1;
+    }
     return;
   }
   if (q0 >= m - 1) return;
-  gather_window<K, DIM>(brt, ps, lsx, keys, conflict_bits, mbits, m, q0, octree, node_conflict,
+  gather_window<K, DIM>(brt, ps, lsx, chunk_excl, ps, keys, conflict_bits, mbits, m, q0, octree,
+                        node_conflict,
                         st, [&](bool miss, uint32_t q) {
                           wave_append(miss, q, &st->counters[CTR_OCT_DEFER], defer);
                         });
@@ -366,7 +406,8 @@ __global__ __launch_bounds__(kBlock) void gather_seg_kernel(
   }
   if (q0 >= m - 1) return;
   const uint32_t kofs = seg[s].kofs;
-  gather_window<K, DIM>(brt + kofs, ps + kofs, lsx + kofs, (const K *)nullptr, nullptr, 0, m, q0,
+  gather_window<K, DIM>(brt + kofs, ps + kofs, lsx + kofs, nullptr, nullptr,
+                        (const K *)nullptr, nullptr, 0, m, q0,
                         oct, nullptr, st, [&](bool miss, uint32_t q) {
                           wave_append(miss, kofs + q, &st->counters[CTR_OCT_DEFER], defer);
                         });
@@ -529,17 +570,18 @@ __device__ inline void scatter_one(uint32_t i, uint32_t m, const BrtNodeT<K> *__
 }
 
 template <typename K, int DIM>
-void launch_dim(const BrtNodeT<K> *brt, const int32_t *parent, const uint32_t *ps,
-                const uint32_t *lsx, const K *keys, const uint32_t *conflict_bits,
-                int64_t max_m, int mbits, int32_t *octree, uint32_t *node_conflict,
-                int64_t capacity, int force_scatter, uint32_t *defer, RunState *st,
-                hipStream_t s) {
+void launch_dim(const BrtNodeT<K> *brt, const int32_t *parent, uint32_t *ps,
+                const uint32_t *lsx, const uint32_t *chunk_excl, const K *keys,
+                const uint32_t *conflict_bits, int64_t max_m, int mbits, int32_t *octree,
+                uint32_t *node_conflict, int64_t capacity, int force_scatter, uint32_t *defer,
+                RunState *st, hipStream_t s) {
   const int64_t nodes = max_m > 1 ? max_m - 1 : 1;
   const unsigned grid = (unsigned)((nodes + kBlock - 1) / kBlock);
   const unsigned sgrid = grid < 1024 ? grid : 1024;
   if (!force_scatter) {
-    gather_kernel<K, DIM><<<grid, kBlock, 0, s>>>(brt, ps, lsx, keys, conflict_bits, mbits,
-                                                  octree, node_conflict, capacity, defer, st);
+    gather_kernel<K, DIM><<<grid, kBlock, 0, s>>>(brt, ps, lsx, chunk_excl, keys, conflict_bits,
+                                                  mbits, octree, node_conflict, capacity, defer,
+                                                  st);
     const unsigned dgrid = grid < 2048 ? grid : 2048;
     gather_defer_kernel<K, DIM><<<dgrid, kBlock, 0, s>>>(brt, ps, lsx, keys, conflict_bits,
                                                          mbits, octree, node_conflict,
@@ -598,17 +640,17 @@ void launch_local_splits(const BrtNodeT<K> *brt, int64_t m, int dim, uint32_t *l
 }
 
 template <typename K>
-void launch_octree(const BrtNodeT<K> *brt, const int32_t *parent, const uint32_t *ps,
-                   const uint32_t *lsx, const K *keys, const uint32_t *conflict_bits,
-                   int64_t max_m, int dim, int mbits, int32_t *octree,
-                   uint32_t *node_conflict, int64_t capacity, int force_scatter,
-                   uint32_t *defer, RunState *st, hipStream_t s) {
+void launch_octree(const BrtNodeT<K> *brt, const int32_t *parent, uint32_t *ps,
+                   const uint32_t *lsx, const uint32_t *chunk_excl, const K *keys,
+                   const uint32_t *conflict_bits, int64_t max_m, int dim, int mbits,
+                   int32_t *octree, uint32_t *node_conflict, int64_t capacity,
+                   int force_scatter, uint32_t *defer, RunState *st, hipStream_t s) {
   if (dim == 2)
-    launch_dim<K, 2>(brt, parent, ps, lsx, keys, conflict_bits, max_m, mbits, octree,
-                     node_conflict, capacity, force_scatter, defer, st, s);
+    launch_dim<K, 2>(brt, parent, ps, lsx, chunk_excl, keys, conflict_bits, max_m, mbits,
+                     octree, node_conflict, capacity, force_scatter, defer, st, s);
   else
-    launch_dim<K, 3>(brt, parent, ps, lsx, keys, conflict_bits, max_m, mbits, octree,
-                     node_conflict, capacity, force_scatter, defer, st, s);
+    launch_dim<K, 3>(brt, parent, ps, lsx, chunk_excl, keys, conflict_bits, max_m, mbits,
+                     octree, node_conflict, capacity, force_scatter, defer, st, s);
 }
 
 template <typename K, int DIM>
@@ -631,10 +673,10 @@ void launch_gather_seg(const BrtNodeT<K> *brt, const uint32_t *ps, const uint32_
 }
 
 #define PGVD_INST(K)                                                                          \
-  template void launch_octree<K>(const BrtNodeT<K> *, const int32_t *, const uint32_t *,       \
-                                 const uint32_t *, const K *, const uint32_t *, int64_t, int,  \
-                                 int, int32_t *, uint32_t *, int64_t, int, uint32_t *,         \
-                                 RunState *, hipStream_t);                                     \
+  template void launch_octree<K>(const BrtNodeT<K> *, const int32_t *, uint32_t *,             \
+                                 const uint32_t *, const uint32_t *, const K *,                \
+                                 const uint32_t *, int64_t, int, int, int32_t *, uint32_t *,   \
+                                 int64_t, int, uint32_t *, RunState *, hipStream_t);           \
   template void launch_gather_seg<K>(const BrtNodeT<K> *, const uint32_t *, const uint32_t *,  \
                                      int, const BatchDev &, int32_t *, int64_t, uint32_t *,    \
                                      RunState *, hipStream_t);                                 \

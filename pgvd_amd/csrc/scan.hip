@@ -233,20 +233,45 @@ __device__ inline void ls_scan_tile(const uint32_t *__restrict__ lsx, uint32_t *
   if (o_out && threadIdx.x == 0) *o_out = (uint32_t)(excl + total);
 }
 
-// Inclusive scan of the local splits ls[0..m), m = st->m, into ps; the total
-// is the octree size (opencl/Kernels.cpp:376-378).
-__global__ __launch_bounds__(kBlock) void ls_scan_kernel(const uint32_t *__restrict__ ls,
-                                                         uint32_t *__restrict__ ps,
-                                                         uint64_t *lookback, RunState *st) {
+// Exclusive scan, in place, of the per-chunk local-split totals
+// chunk[0..ceil(m/256)) written by the BRT pass; the grand total is the
+// octree size (opencl/Kernels.cpp:376-378).  Each tile covers 4096 chunks,
+// thread t holding 16 consecutive ones.
+__global__ __launch_bounds__(kBlock) void chunk_scan_kernel(uint32_t *__restrict__ chunk,
+                                                            uint64_t *lookback, RunState *st) {
+  constexpr int R = kRows;
   __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_w[4];
   const uint32_t m = st->m;
-  const int ntiles = (int)((m + kScanTile - 1) / kScanTile);
+  const uint32_t nc = (m + kBlock - 1) / kBlock;
+  const int ntiles = (int)((nc + kScanTile - 1) / kScanTile);
   if (threadIdx.x == 0) s_tile = atomicAdd(&st->counters[CTR_LS], 1u);
   __syncthreads();
   const int tile = (int)s_tile;
   if (tile >= ntiles) return;
-  ls_scan_tile(ls, ps, m, lookback, tile, 0, (uint32_t)tile * kScanTile,
-               tile == ntiles - 1 ? &st->octree_size : nullptr);
+  const uint32_t e0 = (uint32_t)tile * kScanTile + threadIdx.x * R;
+  const uint32_t valid = e0 < nc ? min((uint32_t)R, nc - e0) : 0u;
+  uint32_t v[R];
+  load_run<uint32_t, R>(chunk + e0, valid, v);
+  uint64_t run = 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const uint32_t x = (j < (int)valid) ? v[j] : 0u;
+    v[j] = (uint32_t)run;
+    run += x;
+  }
+  uint64_t total;
+  const uint64_t texcl = block_excl_sum<uint64_t>(run, s_w, &total);
+  const uint64_t excl = tile_prefix(lookback, tile, total);
+  const uint32_t add = (uint32_t)(excl + texcl);
+#pragma unroll
+  for (int j = 0; j < R; ++j) v[j] += add;
+  store_run<uint32_t, R>(chunk + e0, valid, v);
+  if (tile == ntiles - 1 && threadIdx.x == 0) {
+    const uint64_t all = excl + total;
+    if (all > 0xFFFFFFFFull) atomicOr(&st->status, (uint32_t)ST_TOO_LARGE);
+    st->octree_size = (uint32_t)all;
+  }
 }
 
 // ---- batched form ----
@@ -321,10 +346,10 @@ __global__ __launch_bounds__(kBlock) void oct_offsets_kernel(SegState *ss, int64
 
 }  // namespace
 
-void launch_ls_scan(const uint32_t *ls, uint32_t *ps, int64_t max_m, uint64_t *lookback,
-                    RunState *st, hipStream_t s) {
-  const int64_t tiles = scan_tiles(max_m > 0 ? max_m : 1);
-  ls_scan_kernel<<<(unsigned)tiles, kBlock, 0, s>>>(ls, ps, lookback, st);
+void launch_chunk_scan(uint32_t *chunk_tot, int64_t max_m, uint64_t *lookback, RunState *st,
+                       hipStream_t s) {
+  const int64_t chunks = ((max_m > 0 ? max_m : 1) + kBlock - 1) / kBlock;
+  chunk_scan_kernel<<<(unsigned)scan_tiles(chunks), kBlock, 0, s>>>(chunk_tot, lookback, st);
 }
 
 template <typename K>

@@ -23,9 +23,15 @@ constexpr uint32_t kFlagIncl = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1;
 constexpr int kWaves = kBlock / 64;
 
-template <typename K, bool PAY> struct SortCfg { static constexpr int IPT = 16; };
-template <> struct SortCfg<Key128, false> { static constexpr int IPT = 8; };
-template <> struct SortCfg<Key128, true> { static constexpr int IPT = 8; };
+// Keys per thread of a tile.  The digit-ordered staging buffer reuses the
+// per-wave counters' space; payload tiles are kept near 30 KB of LDS.
+#ifndef PGVD_SORT_IPT64
+#define PGVD_SORT_IPT64 24   // keys per thread for 32/64-bit keys without payload
+#endif
+template <typename K, bool PAY> struct SortCfg { static constexpr int IPT = PGVD_SORT_IPT64; };
+template <> struct SortCfg<uint64_t, true> { static constexpr int IPT = 10; };
+template <> struct SortCfg<Key128, false> { static constexpr int IPT = 7; };
+template <> struct SortCfg<Key128, true> { static constexpr int IPT = 6; };
 
 // Exclusive scan of one u32 per thread over the 256-thread block.
 __device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t *s_wsum) {
@@ -49,12 +55,18 @@ __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__r
                                      uint32_t seg0, int shift, const uint32_t *__restrict__ ghist,
                                      uint32_t *__restrict__ lookback) {
   constexpr int TILE = kBlock * IPT;
-  __shared__ uint32_t s_cnt[kWaves * 256];
-  __shared__ uint32_t s_dstart[256];
-  __shared__ uint32_t s_gbase[256];
+  // The per-wave digit counters (ranking) and the digit-ordered staging of
+  // keys and payload (write-out) are never live together: one LDS area.
+  constexpr int kStageBytes = TILE * (int)sizeof(K) + (PAY ? TILE * 4 : 0);
+  constexpr int kCntBytes = kWaves * 256 * 4;
+  __shared__ __attribute__((aligned(16))) unsigned char s_raw[kStageBytes > kCntBytes ? kStageBytes : kCntBytes];
+  uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_raw);
+  K *s_keys = reinterpret_cast<K *>(s_raw);
+  uint32_t *s_vals = reinterpret_cast<uint32_t *>(s_raw + TILE * sizeof(K));
+  // Digit starts within the tile, then (once every key's slot is known)
+  // each digit's global base.
+  __shared__ uint32_t s_dig[256];
   __shared__ uint32_t s_wsum_a[kWaves], s_wsum_b[kWaves];
-  __shared__ K s_keys[TILE];
-  __shared__ uint32_t s_vals[PAY ? TILE : 1];
 
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   for (int i = t; i < kWaves * 256; i += kBlock) s_cnt[i] = 0;
@@ -108,7 +120,7 @@ __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__r
   st_rel_u32(lb + t, (tile == lo ? kFlagIncl : kFlagAgg) | total);
 
   const uint32_t dstart = block_excl_scan(total, s_wsum_a);
-  s_dstart[t] = dstart;
+  s_dig[t] = dstart;
   const uint32_t gexcl = block_excl_scan(ghist[t], s_wsum_b);
 
   // Look-back for digit t: read up to kBatch predecessor tiles per round
@@ -140,17 +152,26 @@ __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__r
     }
     st_rel_u32(lb + t, kFlagIncl | (excl + total));
   }
-  s_gbase[t] = seg0 + gexcl + excl - dstart;
   __syncthreads();
 
+  // Each key's slot in the digit-ordered staging, taken from the counters
+  // and digit starts before their space is reused.
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t le = wofs + j * 64 + lane;
     if (le < nvalid) {
       const uint32_t d = kdigit(k[j], shift);
-      const uint32_t pos = s_dstart[d] + s_cnt[w * 256 + d] + rk[j];
-      s_keys[pos] = k[j];
-      if (PAY) s_vals[pos] = v[j];
+      rk[j] += s_dig[d] + s_cnt[w * 256 + d];
+    }
+  }
+  __syncthreads();
+  s_dig[t] = seg0 + gexcl + excl - dstart;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t le = wofs + j * 64 + lane;
+    if (le < nvalid) {
+      s_keys[rk[j]] = k[j];
+      if (PAY) s_vals[rk[j]] = v[j];
     }
   }
   __syncthreads();
@@ -160,7 +181,7 @@ __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__r
     const uint32_t p = j * kBlock + t;
     if (p < nvalid) {
       const K key = s_keys[p];
-      const uint32_t g = s_gbase[kdigit(key, shift)] + p;
+      const uint32_t g = s_dig[kdigit(key, shift)] + p;
       kout[g] = key;
       if (PAY) vout[g] = s_vals[p];
     }
@@ -212,8 +233,10 @@ __global__ __launch_bounds__(kBlock) void onesweep_seg_kernel(
 }  // namespace
 
 int onesweep_tile_size(int key_bytes, bool payload) {
-  const int ipt = key_bytes == 16 ? SortCfg<Key128, false>::IPT : 16;
-  (void)payload;
+  int ipt;
+  if (key_bytes == 16) ipt = payload ? SortCfg<Key128, true>::IPT : SortCfg<Key128, false>::IPT;
+  else if (key_bytes == 8) ipt = payload ? SortCfg<uint64_t, true>::IPT : SortCfg<uint64_t, false>::IPT;
+  else ipt = payload ? SortCfg<uint32_t, true>::IPT : SortCfg<uint32_t, false>::IPT;
   return kBlock * ipt;
 }
 
@@ -258,7 +281,8 @@ int launch_onesweep(K *keys, K *keys_alt, const uint32_t *vals_in, uint32_t *val
 }
 
 bool onesweep_seg_ipt_ok(int key_bytes, int ipt) {
-  return key_bytes == 16 ? (ipt == 4 || ipt == 8) : (ipt == 8 || ipt == 12 || ipt == 16);
+  return key_bytes == 16 ? (ipt == 4 || ipt == 7 || ipt == 8)
+                         : (ipt == 8 || ipt == 12 || ipt == 15 || ipt == 16);
 }
 
 template <typename K, int IPT>
@@ -279,10 +303,12 @@ int launch_onesweep_seg(K *keys, K *keys_alt, const BatchDev &b, int passes,
   K *src = keys, *dst = keys_alt;
   if constexpr (sizeof(K) == 16) {
     if (b.sort_ipt == 4) seg_passes<K, 4>(src, dst, b, passes, bhist, lookback, st, s);
+    else if (b.sort_ipt == 7) seg_passes<K, 7>(src, dst, b, passes, bhist, lookback, st, s);
     else seg_passes<K, 8>(src, dst, b, passes, bhist, lookback, st, s);
   } else {
     if (b.sort_ipt == 8) seg_passes<K, 8>(src, dst, b, passes, bhist, lookback, st, s);
     else if (b.sort_ipt == 12) seg_passes<K, 12>(src, dst, b, passes, bhist, lookback, st, s);
+    else if (b.sort_ipt == 15) seg_passes<K, 15>(src, dst, b, passes, bhist, lookback, st, s);
     else seg_passes<K, 16>(src, dst, b, passes, bhist, lookback, st, s);
   }
   return passes & 1;

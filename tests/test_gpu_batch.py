@@ -148,7 +148,7 @@ def test_batch_tuner_keeps_results(dev):
     b = pgvd_amd.Batch(dev, [len(p) for p in parts], dim, bits)
     try:
         ipt, rows = b.tune(d)
-        assert ipt in (8, 12, 16) and rows in (2, 4)
+        assert ipt in (8, 12, 15, 16) and rows in (2, 4)
         b.run(d)
         all_ = b.fetch_all(b.finish())
         for i, p in enumerate(parts):
@@ -160,7 +160,7 @@ def test_batch_tuner_keeps_results(dev):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("ipt,rows", [(8, 2), (12, 4), (16, 2)])
+@pytest.mark.parametrize("ipt,rows", [(8, 2), (12, 4), (15, 4), (16, 2)])
 def test_batch_every_config(dev, ipt, rows):
     dim, bits = 3, 21
     parts = make_batch([1, 4097, 50000, 2], dim, seed=ipt * 10 + rows)

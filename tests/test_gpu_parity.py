@@ -85,6 +85,32 @@ def test_empty_input_is_an_error(dev):
     assert e.value.code == 2                      # opencl/Kernels.cpp:389-391
 
 
+@pytest.mark.parametrize("m", [2, 3, 256, 257, 258, 513, 769, 1024, 1025, 4097, 65537])
+def test_tree_sizes_at_chunk_edges(dev, m):
+    """m distinct grid points including the origin (no zero key prepended):
+    trees whose last node starts, ends or straddles a 256-node chunk, where
+    the gather derives the prefix sums from per-chunk totals."""
+    rng = np.random.default_rng(m)
+    flat = rng.choice(1 << 20, m - 1, replace=False) + 1
+    q = np.stack([flat & 1023, flat >> 10], axis=1).astype(np.int32)
+    q = np.concatenate([np.zeros((1, 2), np.int32), q])
+    want = oracle.build(q, 10)
+    assert want["m"] == m
+    assert_tree_equal(dev.build_octree(q, 10), want, where=f"m={m}")
+
+
+def test_octree_larger_than_first_buffer(dev):
+    """Pairs of grid points one step apart, scattered: each pair hangs off a
+    chain of single-child octree nodes, so the octree outgrows the 2m nodes
+    first allocated and the finish call refills it from the kept BRT."""
+    rng = np.random.default_rng(11)
+    c = rng.integers(0, 1 << 20, (20000, 3)) * 2
+    q = np.concatenate([c, c + np.array([1, 0, 0])]).astype(np.int32)
+    want = oracle.build(q, 21)
+    assert want["octree_size"] > 2 * (want["m"] + 1)
+    assert_tree_equal(dev.build_octree(q, 21), want, where="refill")
+
+
 def test_int32_grid_input_matches_oracle(dev, golden_small):
     """Kernels::BuildOctree_p takes integer points; the reference fixture at
     bits = 30 through that entry."""
