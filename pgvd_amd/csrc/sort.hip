@@ -28,6 +28,11 @@ constexpr int kWaves = kBlock / 64;
 #ifndef PGVD_SORT_IPT64
 #define PGVD_SORT_IPT64 24   // keys per thread for 32/64-bit keys without payload
 #endif
+// Waves per SIMD the sort kernels are compiled for (register budget); LDS
+// allows three 256-thread tiles of 24 64-bit keys per CU.
+#ifndef PGVD_SORT_WAVES
+#define PGVD_SORT_WAVES 3
+#endif
 template <typename K, bool PAY> struct SortCfg { static constexpr int IPT = PGVD_SORT_IPT64; };
 template <> struct SortCfg<uint64_t, true> { static constexpr int IPT = 10; };
 template <> struct SortCfg<Key128, false> { static constexpr int IPT = 7; };
@@ -44,29 +49,46 @@ __device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t *s_wsum) {
   return off + incl - x;
 }
 
+// LDS of one onesweep tile.  The per-wave digit counters (ranking) and the
+// digit-ordered staging of keys and payload (write-out) are never live
+// together: one area.  s_dig holds the digit starts within the tile, then
+// (once every key's slot is known) each digit's global base.
+template <typename K, bool PAY, int IPT>
+struct OnesweepLds {
+  static constexpr int kTile = kBlock * IPT;
+  static constexpr int kStageBytes = kTile * (int)sizeof(K) + (PAY ? kTile * 4 : 0);
+  static constexpr int kCntBytes = kWaves * 256 * 4;
+  __attribute__((aligned(16))) unsigned char raw[kStageBytes > kCntBytes ? kStageBytes : kCntBytes];
+  uint32_t dig[256];
+  uint32_t wsum_a[kWaves], wsum_b[kWaves];
+};
+
+// One instance per <K, PAY, IPT>, shared by the whole-tile and partial-tile
+// forms of onesweep_tile.
+template <typename K, bool PAY, int IPT>
+__device__ inline OnesweepLds<K, PAY, IPT> &onesweep_lds() {
+  __shared__ OnesweepLds<K, PAY, IPT> lds;
+  return lds;
+}
+
 // One onesweep tile: the nvalid (<= IPT*256) keys at kin[base..) of a run
 // of tiles [lo, ...) whose keys go to kout[seg0..) in digit order.  `tile` is
 // the tile's index in the look-back array (tiles lo..tile-1 precede it in the
-// same run); ghist is the run's digit histogram for this pass.
-template <typename K, bool PAY, int IPT>
+// same run); ghist is the run's digit histogram for this pass.  FULL: nvalid
+// is the whole tile, so no key needs a range test.
+template <typename K, bool PAY, int IPT, bool FULL>
 __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__restrict__ kin, K *__restrict__ kout,
                                      const uint32_t *__restrict__ vin, uint32_t *__restrict__ vout,
                                      uint32_t base, uint32_t nvalid, uint32_t tile, uint32_t lo,
                                      uint32_t seg0, int shift, const uint32_t *__restrict__ ghist,
                                      uint32_t *__restrict__ lookback) {
   constexpr int TILE = kBlock * IPT;
-  // The per-wave digit counters (ranking) and the digit-ordered staging of
-  // keys and payload (write-out) are never live together: one LDS area.
-  constexpr int kStageBytes = TILE * (int)sizeof(K) + (PAY ? TILE * 4 : 0);
-  constexpr int kCntBytes = kWaves * 256 * 4;
-  __shared__ __attribute__((aligned(16))) unsigned char s_raw[kStageBytes > kCntBytes ? kStageBytes : kCntBytes];
-  uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_raw);
-  K *s_keys = reinterpret_cast<K *>(s_raw);
-  uint32_t *s_vals = reinterpret_cast<uint32_t *>(s_raw + TILE * sizeof(K));
-  // Digit starts within the tile, then (once every key's slot is known)
-  // each digit's global base.
-  __shared__ uint32_t s_dig[256];
-  __shared__ uint32_t s_wsum_a[kWaves], s_wsum_b[kWaves];
+  OnesweepLds<K, PAY, IPT> &L = onesweep_lds<K, PAY, IPT>();
+  uint32_t *s_cnt = reinterpret_cast<uint32_t *>(L.raw);
+  K *s_keys = reinterpret_cast<K *>(L.raw);
+  uint32_t *s_vals = reinterpret_cast<uint32_t *>(L.raw + TILE * sizeof(K));
+  uint32_t *s_dig = L.dig;
+  uint32_t *s_wsum_a = L.wsum_a, *s_wsum_b = L.wsum_b;
 
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   for (int i = t; i < kWaves * 256; i += kBlock) s_cnt[i] = 0;
@@ -79,31 +101,41 @@ __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__r
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t le = wofs + j * 64 + lane;
-    k[j] = (le < nvalid) ? kin[base + le] : kzero_value<K>();
-    if (PAY) v[j] = (le < nvalid) ? vin[base + le] : 0u;
+    k[j] = (FULL || le < nvalid) ? kin[base + le] : kzero_value<K>();
+    if (PAY) v[j] = (FULL || le < nvalid) ? vin[base + le] : 0u;
   }
 
   // Rank: keys with equal digits within a wave found by 8 ballots; the
   // lowest lane of each group bumps the wave's LDS counter for that digit.
-  // Items are processed in index order, so ranks are stable.
-  const uint64_t ltm = lanemask_lt();
+  // Items are processed in index order, so ranks are stable.  Per digit bit
+  // b, s = -1 where the bit is set, else 0, and a lane keeps the lanes whose
+  // ballot bit equals its own: peers &= ~(ballot ^ s), in two 32-bit halves,
+  // one v_bitop3 each (truth table 0x90: operand 0 is the table index's
+  // high bit, so the result is 1 for (1,0,0) and (1,1,1)).
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t le = wofs + j * 64 + lane;
-    const bool valid = le < nvalid;
+    const bool valid = FULL || le < nvalid;
     const uint32_t d = kdigit(k[j], shift);
-    uint64_t peers = __ballot(valid);
+    uint32_t plo = ~0u, phi = ~0u;
+    if (!FULL) {
+      const uint64_t vb = __ballot(valid);
+      plo = (uint32_t)vb;
+      phi = (uint32_t)(vb >> 32);
+    }
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bb = __ballot(bit);
-      peers &= bit ? bb : ~bb;
+      const int32_t sb = __builtin_amdgcn_sbfe((int32_t)d, b, 1);
+      const uint64_t bb = __ballot(sb < 0);
+      plo = __builtin_amdgcn_bitop3_b32(plo, (uint32_t)bb, (uint32_t)sb, 0x90);
+      phi = __builtin_amdgcn_bitop3_b32(phi, (uint32_t)(bb >> 32), (uint32_t)sb, 0x90);
     }
     if (valid) {
       const uint32_t cnt = s_cnt[w * 256 + d];
-      const uint64_t below = peers & ltm;
-      rk[j] = cnt + (uint32_t)__popcll(below);
-      if (below == 0) s_cnt[w * 256 + d] = cnt + (uint32_t)__popcll(peers);
+      // peers in lower lanes, counted by mbcnt
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
+      rk[j] = cnt + below;
+      if (below == 0) s_cnt[w * 256 + d] = cnt + (uint32_t)(__popc(plo) + __popc(phi));
     }
   }
   __syncthreads();
@@ -159,7 +191,7 @@ __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__r
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t le = wofs + j * 64 + lane;
-    if (le < nvalid) {
+    if (FULL || le < nvalid) {
       const uint32_t d = kdigit(k[j], shift);
       rk[j] += s_dig[d] + s_cnt[w * 256 + d];
     }
@@ -169,7 +201,7 @@ __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__r
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t le = wofs + j * 64 + lane;
-    if (le < nvalid) {
+    if (FULL || le < nvalid) {
       s_keys[rk[j]] = k[j];
       if (PAY) s_vals[rk[j]] = v[j];
     }
@@ -179,7 +211,7 @@ __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__r
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t p = j * kBlock + t;
-    if (p < nvalid) {
+    if (FULL || p < nvalid) {
       const K key = s_keys[p];
       const uint32_t g = s_dig[kdigit(key, shift)] + p;
       kout[g] = key;
@@ -198,7 +230,7 @@ __device__ inline uint32_t claim_tile(uint32_t *counter) {
 }
 
 template <typename K, bool PAY, int IPT>
-__global__ __launch_bounds__(kBlock) void onesweep_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PGVD_SORT_WAVES))) void onesweep_kernel(
     const K *__restrict__ kin, K *__restrict__ kout, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ vout, uint32_t n, int shift, const uint32_t *__restrict__ ghist,
     uint32_t *__restrict__ lookback, uint32_t *tile_counter) {
@@ -206,15 +238,19 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   const uint32_t tile = claim_tile(tile_counter);
   const uint32_t base = tile * TILE;
   const uint32_t nvalid = (n - base) < TILE ? (n - base) : TILE;
-  onesweep_tile<K, PAY, IPT>(kin, kout, vin, vout, base, nvalid, tile, 0u, 0u, shift, ghist,
-                             lookback);
+  if (nvalid == TILE)
+    onesweep_tile<K, PAY, IPT, true>(kin, kout, vin, vout, base, nvalid, tile, 0u, 0u, shift,
+                                     ghist, lookback);
+  else
+    onesweep_tile<K, PAY, IPT, false>(kin, kout, vin, vout, base, nvalid, tile, 0u, 0u, shift,
+                                      ghist, lookback);
 }
 
 // Segmented form: instance s owns keys [seg.kofs, +seg.n) and sort tiles
 // [tt.first[s], tt.first[s+1]); each instance is sorted on its own, in one
 // launch per digit for the whole batch.
 template <typename K, int IPT>
-__global__ __launch_bounds__(kBlock) void onesweep_seg_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PGVD_SORT_WAVES))) void onesweep_seg_kernel(
     const K *__restrict__ kin, K *__restrict__ kout, const SegInfo *__restrict__ seg,
     TileTable tt, int passes, int pass, const uint32_t *__restrict__ bhist,
     uint32_t *__restrict__ lookback, uint32_t *tile_counter) {
@@ -225,9 +261,13 @@ __global__ __launch_bounds__(kBlock) void onesweep_seg_kernel(
   const SegInfo si = seg[s];
   const uint32_t lofs = (tile - lo) * TILE;
   const uint32_t nvalid = (si.n - lofs) < TILE ? (si.n - lofs) : TILE;
-  onesweep_tile<K, false, IPT>(kin, kout, nullptr, nullptr, si.kofs + lofs, nvalid, tile, lo,
-                               si.kofs, 8 * pass, bhist + ((size_t)s * passes + pass) * 256,
-                               lookback);
+  const uint32_t *gh = bhist + ((size_t)s * passes + pass) * 256;
+  if (nvalid == TILE)
+    onesweep_tile<K, false, IPT, true>(kin, kout, nullptr, nullptr, si.kofs + lofs, nvalid, tile,
+                                       lo, si.kofs, 8 * pass, gh, lookback);
+  else
+    onesweep_tile<K, false, IPT, false>(kin, kout, nullptr, nullptr, si.kofs + lofs, nvalid, tile,
+                                        lo, si.kofs, 8 * pass, gh, lookback);
 }
 
 }  // namespace
