@@ -124,43 +124,76 @@ __device__ inline void brt_window(const K *__restrict__ keys, int m, int b0, int
   __shared__ uint32_t s_rowsum[R * (kBlock / 64)];
   constexpr int H = BrtWin<K>::H, SEG = R * kBlock, WN = SEG + 2 * H;
   constexpr int LV = (WN >= 2048) ? 12 : (WN >= 1024 ? 11 : (WN >= 512 ? 10 : 9));
-  static_assert((1 << (LV - 1)) <= WN && WN < (1 << LV), "levels cover the window");
-  __shared__ uint8_t s_mn[LV * WN];
+  constexpr int PT = WN / kBlock;     // window positions per thread
+  static_assert(WN % kBlock == 0 && H == kBlock, "thread t holds positions t + 256 i");
+  using Tab = DeltaTab<WN, LV>;
+  constexpr int kWords = (Tab::kBytes + 3) / 4;
+  __shared__ __attribute__((aligned(16))) uint32_t s_tab[kWords];
+  uint8_t *s_mn = reinterpret_cast<uint8_t *>(s_tab);
   const int w0 = b0 - H;
-  // The nodes' own keys are loaded first so that their latency overlaps the
-  // table build instead of starting each row.
-  K cur[R];
+  const int t = threadIdx.x;
+  // Window keys: position t + 256 i needs keys[g] and keys[g+1], g = w0 + t +
+  // 256 i; all loads are issued before the first is used.  Node rows are
+  // positions 1 .. R, so the nodes' own keys come from the same loads.
+  K ka[PT], kb[PT];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int c = b0 + r * kBlock + (int)threadIdx.x;
-    cur[r] = c < m - 1 ? keys[c] : kzero_value<K>();
+  for (int i = 0; i < PT; ++i) {
+    const int g = w0 + t + i * kBlock;
+    const bool v = g >= 0 && g < m - 1;
+    ka[i] = keys[v ? g : 0];
+    kb[i] = keys[v ? g + 1 : 0];
   }
-  for (int r = threadIdx.x; r < WN; r += kBlock) {
-    const int g = w0 + r;
-    s_mn[r] = (g >= 0 && g < m - 1) ? (uint8_t)(1 + delta(keys[g], keys[g + 1], mbits)) : 0;
+  // sentinels: padding and the block ends of every level
+  for (int r = t; r < kWords; r += kBlock) s_tab[r] = 0;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < PT; ++i) {
+    const int g = w0 + t + i * kBlock;
+    s_mn[Tab::base(0) + t + i * kBlock] =
+        (g >= 0 && g < m - 1) ? (uint8_t)(1 + delta(ka[i], kb[i], mbits)) : 0;
   }
   __syncthreads();
-#pragma unroll 1
-  for (int j = 1; j < LV; ++j) {
-    const int half = 1 << (j - 1);
-    for (int r = threadIdx.x; r + 2 * half <= WN; r += kBlock) {
-      const uint8_t x = s_mn[(j - 1) * WN + r], y = s_mn[(j - 1) * WN + r + half];
-      s_mn[j * WN + r] = x < y ? x : y;
+  // Two levels per pass from the level below them: entry (j, r) is the min
+  // of two entries of level j-1, entry (j+1, r) of four; probes past WN read
+  // the level's last entry (0) and only feed entries left at 0.
+#pragma unroll
+  for (int j = 1; j < LV; j += 2) {
+    const int q = 1 << (j - 1);
+    const uint8_t *prev = s_mn + Tab::base(j - 1);
+    uint8_t x[PT][4];
+#pragma unroll
+    for (int i = 0; i < PT; ++i) {
+      const int r = t + i * kBlock;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) x[i][u] = prev[min(r + u * q, WN)];
+    }
+#pragma unroll
+    for (int i = 0; i < PT; ++i) {
+      const int r = t + i * kBlock;
+      const uint8_t a = x[i][0] < x[i][1] ? x[i][0] : x[i][1];
+      const uint8_t b = x[i][2] < x[i][3] ? x[i][2] : x[i][3];
+      if (r + 2 * q <= WN) s_mn[Tab::base(j) + r] = a;
+      if (j + 1 < LV && r + 4 * q <= WN) s_mn[Tab::base(j + 1) + r] = a < b ? a : b;
     }
     __syncthreads();
   }
-  auto mn = [&](int j, int k) -> int { return s_mn[j * WN + k]; };
+  const Tab tab{s_mn};
+  // every row's searches first (independent, so they overlap), then stores
+  BrtOut<K> o[R];
+  bool ok[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    ok[r] = brt_node_window(ka[1 + r], tab, w0, mbits, m, b0 + r * kBlock + t, o[r]);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int c = b0 + r * kBlock + (int)threadIdx.x;
+    const int c = b0 + r * kBlock + t;
     bool miss = false;
     uint32_t ls = 0;
     if (c < m - 1) {
-      BrtOut<K> o;
-      miss = !brt_node_window(cur[r], mn, LV, w0, WN, mbits, m, c, o);
+      miss = !ok[r];
       if (!miss) {
-        brt_flags(o, st);
-        ls = brt_store<K, DIM>(c, cur[r], o, mbits, brt, parent, lsx);
+        brt_flags(o[r], st);
+        ls = brt_store<K, DIM>(c, ka[1 + r], o[r], mbits, brt, parent, lsx);
       }
     } else if (c == m - 1) {
       ls = (m == 1) ? 1u : 0u;
@@ -169,7 +202,7 @@ __device__ inline void brt_window(const K *__restrict__ keys, int m, int b0, int
     defer(miss, c);
     if (chunk_tot) {
       const uint32_t sum = wave_reduce_add(ls);
-      if ((threadIdx.x & 63) == 0) s_rowsum[r * (kBlock / 64) + (threadIdx.x >> 6)] = sum;
+      if ((t & 63) == 0) s_rowsum[r * (kBlock / 64) + (t >> 6)] = sum;
     }
   }
   if (chunk_tot) {
@@ -188,20 +221,19 @@ __global__ __launch_bounds__(kBlock) void brt_ls_kernel(const K *__restrict__ ke
                                                         int32_t *__restrict__ parent,
                                                         uint32_t *__restrict__ lsx,
                                                         uint32_t *__restrict__ chunk_tot,
-                                                        uint32_t *__restrict__ defer,
+                                                        uint64_t *__restrict__ miss_words,
                                                         RunState *st) {
   constexpr int R = BrtWin<K>::R, SEG = R * kBlock;
   const int m = (int)st->m;
   const int b0 = (int)xcd_segment(blockIdx.x, gridDim.x) * SEG;
   if (b0 >= m) return;                      // uniform over the workgroup
   brt_window<K, DIM, R>(keys, m, b0, mbits, brt, parent, lsx, chunk_tot, st,
-                        [&](bool miss, int c) {
-                          wave_append(miss, (uint32_t)c, &st->counters[CTR_BRT_DEFER], defer);
-                        });
+                        [&](bool miss, int c) { wave_mask(miss, miss_words, (uint32_t)c >> 6); });
 }
 
-// Pipeline, second pass: the nodes the window could not hold, against keys
-// in global memory.
+// Pipeline, second pass: the nodes the window could not hold (set bits of
+// the first pass's miss words, one word per 64 nodes), against keys in
+// global memory.
 template <typename K, int DIM>
 __global__ __launch_bounds__(kBlock) void brt_ls_defer_kernel(const K *__restrict__ keys,
                                                               int mbits,
@@ -209,18 +241,18 @@ __global__ __launch_bounds__(kBlock) void brt_ls_defer_kernel(const K *__restric
                                                               int32_t *__restrict__ parent,
                                                               uint32_t *__restrict__ lsx,
                                                               uint32_t *__restrict__ chunk_tot,
-                                                              const uint32_t *__restrict__ defer,
+                                                              const uint64_t *__restrict__ miss_words,
                                                               RunState *st) {
   const int m = (int)st->m;
-  const uint32_t cnt = st->counters[CTR_BRT_DEFER];
-  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
-    const int c = (int)defer[k];
+  const uint32_t nwords = m > 1 ? (uint32_t)(m - 1 + 63) / 64u : 0u;
+  for_each_masked(miss_words, nwords, [&](uint32_t cu) {
+    const int c = (int)cu;
     const K cur = keys[c];
     const BrtOut<K> o = brt_eval(cur, [&](int x) { return keys[x]; }, mbits, m, c);
     brt_flags(o, st);
     const uint32_t ls = brt_store<K, DIM>(c, cur, o, mbits, brt, parent, lsx);
     if (ls) atomicAdd(&chunk_tot[c >> 8], ls);
-  }
+  });
 }
 
 // Batched form: BRT tiles of R*256 key slots per instance (n+1 slots each);
@@ -328,13 +360,14 @@ void launch_brt_ls_dim(const K *keys, int64_t max_m, int mbits, BrtNodeT<K> *brt
                        RunState *st, hipStream_t s) {
   constexpr int SEG = BrtWin<K>::R * kBlock;
   const unsigned grid = (unsigned)((max_m + SEG - 1) / SEG);
-  brt_ls_kernel<K, DIM><<<grid, kBlock, 0, s>>>(keys, mbits, brt, parent, lsx, chunk_tot, defer,
+  uint64_t *words = reinterpret_cast<uint64_t *>(defer);
+  brt_ls_kernel<K, DIM><<<grid, kBlock, 0, s>>>(keys, mbits, brt, parent, lsx, chunk_tot, words,
                                                 st);
-  // The overflow pass grid-strides over however many nodes were deferred.
-  const int64_t g2 = (max_m + kBlock - 1) / kBlock;
-  const unsigned grid2 = (unsigned)(g2 < 2048 ? g2 : 2048);
+  // The overflow pass: one thread per miss word.
+  const int64_t g2 = ((max_m + 63) / 64 + kBlock - 1) / kBlock;
+  const unsigned grid2 = (unsigned)(g2 < 4096 ? g2 : 4096);
   brt_ls_defer_kernel<K, DIM><<<grid2, kBlock, 0, s>>>(keys, mbits, brt, parent, lsx, chunk_tot,
-                                                       defer, st);
+                                                       words, st);
 }
 
 template <typename K>

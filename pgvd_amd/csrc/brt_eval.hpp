@@ -3,6 +3,8 @@
 // so that the search logic itself is checked against the oracle on the CPU.
 #pragma once
 
+#include <cstdlib>
+
 #include "kernels.hpp"
 
 namespace pgvd {
@@ -112,73 +114,79 @@ __host__ __device__ inline BrtOut<K> brt_eval(K cur, Key key, int mbits, int m, 
 // adjacent deltas D_k = delta(keys[k], keys[k+1]) for k between i and x, so
 // a node's range end is the nearest adjacent delta <= its lcp_min and its
 // split is the (unique) position of the range's minimum adjacent delta.  A
-// window holds mn(j, k) = 1 + min D over window positions [k, k + 2^j) of
-// global positions [w0, w0 + wn), with 0 for positions outside [0, m-2] (a
-// sentinel below every delta).
-
-// Smallest k >= pos with mn(0, k) <= v (dir = +1), or largest k <= pos with
-// mn(0, k) <= v (dir = -1); wn resp. -1 if the window holds none.  Greedy
-// descent over the levels; every step reads an in-range entry and moves by a
-// select, so lanes searching in either direction run one branch-free loop.
-template <typename MN>
-__host__ __device__ inline int find_le(MN mn, int levels, int wn, int pos, int v, bool fwd) {
-  for (int j = levels - 1; j >= 0; --j) {
-    const int step = 1 << j;
-    const int at = fwd ? pos : pos - step + 1;          // block [at, at + step)
-    const bool inside = fwd ? (pos + step <= wn) : (at >= 0);
-    const int atc = at < 0 ? 0 : (at > wn - step ? wn - step : at);
-    const bool skip = inside && mn(j, atc) > v;
-    pos += skip ? (fwd ? step : -step) : 0;
+// window over global positions [w0, w0 + WN) holds, per level j < LV,
+// mn(j, k) = 1 + min D over window positions [k, k + 2^j), and 0 (a
+// sentinel below every delta) where that block reaches outside the window
+// or outside [0, m-2].  Level j stores k = -2^j .. WN, so every probe of a
+// descent is in range without a test:
+//   byte offset of (j, k) = base(j) + k,  base(j) = 2^(j+1) - 1 + j (WN + 1),
+//   table size = 2^LV - 1 + LV (WN + 1) bytes.
+template <int WN, int LV>
+struct DeltaTab {
+  static_assert((1 << (LV - 1)) <= WN && WN < (1 << LV), "levels cover the window");
+  static constexpr int kBytes = (1 << LV) - 1 + LV * (WN + 1);
+  __host__ __device__ static constexpr int base(int j) { return (2 << j) - 1 + j * (WN + 1); }
+  const uint8_t *t;
+  __host__ __device__ int mn(int j, int k) const { return t[base(j) + k]; }
+  // Smallest k >= pos with mn(0, k) <= v (fwd), or largest k <= pos with
+  // mn(0, k) <= v (backward); WN resp. -1 if the window holds none.  Needs
+  // -1 <= pos <= WN.  Greedy descent: a block of 2^j entries all above v is
+  // skipped; blocks that reach outside the window read 0 and are not, so
+  // pos stays in [-1, WN] and every probe in [-2^j, WN].
+  __host__ __device__ int find(int pos, int v, bool fwd) const {
+#pragma unroll
+    for (int j = LV - 1; j >= 0; --j) {
+      const int step = 1 << j;
+      const int at = fwd ? pos : pos - step + 1;
+#if !defined(__HIP_DEVICE_COMPILE__)
+      if (at < -step || at > WN) std::abort();   // host builds check the probe bound
+#endif
+      const bool skip = t[base(j) + at] > v;
+      pos += skip ? (fwd ? step : -step) : 0;
+    }
+    return pos;
   }
-  return pos;
-}
+};
 
-template <typename MN>
-__host__ __device__ inline int next_le(MN mn, int levels, int wn, int pos, int v) {
-  return find_le(mn, levels, wn, pos, v, true);
-}
-
-// BRT node i (0 <= i < m-1) from the window (w0 <= i-1, i < w0 + wn).
+// BRT node i (0 <= i < m-1) from the window (w0 <= i-1, i < w0 + WN).
 // Returns false when the node's range or split is not inside the window; the
 // node then goes to brt_eval over the whole key array.  Well-formed trees
-// only: ranges here are far below 2^23 keys.
-template <typename K, typename MN>
-__host__ __device__ inline bool brt_node_window(K cur, MN mn, int levels, int w0, int wn,
+// only: ranges here are far below 2^23 keys.  Branch-free: every probe is
+// made (clamped into the window) whatever the outcome, so that the searches
+// of several nodes of one lane can overlap; o holds the node only when the
+// result is true.
+template <typename K, int WN, int LV>
+__host__ __device__ inline bool brt_node_window(K cur, const DeltaTab<WN, LV> &tab, int w0,
                                                 int mbits, int m, int i, BrtOut<K> &o) {
   const int wi = i - w0;
-  const int dpos = mn(0, wi) - 1;
-  o.dup = dpos == mbits;
-  o.quirk = false;
-  int lo, hi;
-  if (i == 0) {
-    lo = 0;
-    hi = m - 1;
-    o.lcp_min = -1;
-  } else {
-    const int dneg = mn(0, wi - 1) - 1;
-    const bool fwd = dpos >= dneg;
-    const int dmin = fwd ? dneg : dpos;
-    o.lcp_min = dmin;
-    const int k = find_le(mn, levels, wn, fwd ? wi : wi - 1, dmin + 1, fwd);
-    if (k < 0 || k >= wn) return false;
-    lo = fwd ? i : w0 + k + 1;
-    hi = fwd ? w0 + k : i;
-  }
+  const int dpos = tab.mn(0, wi) - 1;
+  const int dneg = tab.mn(0, wi - 1) - 1;
+  const bool root = i == 0;
+  const bool fwd = dpos >= dneg;
+  const int dmin = fwd ? dneg : dpos;
+  const int k = tab.find(fwd ? wi : wi - 1, dmin + 1, fwd);
+  bool ok = root || (k >= 0 && k < WN);
+  const int lo = root ? 0 : (fwd ? i : w0 + k + 1);
+  const int hi = root ? m - 1 : (fwd ? w0 + k : i);
   const int a = lo - w0, b = hi - 1 - w0;
-  if (a < 0 || b >= wn || b < a) return false;
-  const int jl = 31 - __builtin_clz((unsigned)(b - a + 1));
-  const int lc = mn(jl, a), rc = mn(jl, b - (1 << jl) + 1);
+  ok = ok && a >= 0 && b < WN && b >= a;
+  const int ac = ok ? a : 0, bc = ok ? b : 0;
+  const int jl = 31 - __builtin_clz((unsigned)(bc - ac + 1));
+  const int lc = tab.mn(jl, ac), rc = tab.mn(jl, bc - (1 << jl) + 1);
   const int lcp_node = (lc < rc ? lc : rc) - 1;
-  const int split = w0 + next_le(mn, levels, wn, a, lcp_node + 1);
+  const int split = w0 + tab.find(ac, lcp_node + 1, true);
   // Holds for distinct keys; equal keys (reported through o.dup) may break
   // it, and then the bounded search over the whole array takes the node.
-  if (hi <= lo || split < lo || split >= hi) return false;
+  ok = ok && hi > lo && split >= lo && split < hi;
+  o.dup = dpos == mbits;
+  o.quirk = false;
+  o.lcp_min = root ? -1 : dmin;
   o.nd.left = split;
   o.nd.left_leaf = (lo == split);
   o.nd.right_leaf = (hi == split + 1);
   o.nd.lcp = kshr(cur, mbits - lcp_node);
   o.nd.lcp_length = (uint16_t)lcp_node;
-  return true;
+  return ok;
 }
 
 }  // namespace pgvd

@@ -192,6 +192,15 @@ __device__ inline void wave_append(bool take, T idx, uint32_t *count, T *list) {
   if (take) list[base + (uint32_t)__popcll(mb & lanemask_lt())] = idx;
 }
 
+// Miss masks: a converged wave records which of its 64 lanes take a node as
+// one word, words[word] = ballot(take), stored by lane 0.  No atomics, so
+// the word must belong to this wave alone (word = first index >> 6 for 64
+// consecutive, 64-aligned indices).
+__device__ inline void wave_mask(bool take, uint64_t *words, uint32_t word) {
+  const uint64_t b = __ballot(take);
+  if (__lane_id() == 0) words[word] = b;
+}
+
 // Instance owning key slot `slot`: the largest s with kofs[s] <= slot
 // (kofs strictly increasing, kofs[B] = total slots).
 __device__ inline uint32_t find_seg(const uint32_t *kofs, uint32_t B, uint32_t slot) {

@@ -184,7 +184,8 @@ void launch_finalize_parents(int32_t *parent, int64_t max_m, int64_t m_host, Run
 // Pipeline: every BRT node with its local split and attach octant packed as
 // lsx[c] = ls | octant << 8 (lsx[0..m)), parent links as plain stores
 // (well-formed tree; see brt.hip), and chunk_tot[k] = the local splits of
-// nodes [256k, 256k+256) summed.  defer[] holds up to max_m indices.
+// nodes [256k, 256k+256) summed.  defer[] holds the miss words of the
+// overflow pass: (max_m + 63) / 64 64-bit words, 8-byte aligned.
 template <typename K>
 void launch_brt_ls(const K *keys, int64_t max_m, int mbits, int dim, BrtNodeT<K> *brt,
                    int32_t *parent, uint32_t *lsx, uint32_t *chunk_tot, uint32_t *defer,
@@ -203,7 +204,8 @@ void launch_claims_if_malformed(const BrtNodeT<K> *brt, int32_t *parent, int64_t
 // order-exact scatter path unconditionally (stage API, where the BRT comes
 // from the caller; lsx and defer may then be null); otherwise st->malformed
 // selects it on the device and the gather path reads lsx from
-// launch_brt_ls and uses defer[] (max_m entries) as its overflow list.
+// launch_brt_ls and uses defer[] for its miss words ((max_m + 63) / 64
+// 64-bit words, 8-byte aligned).
 // The gather path also writes ps[0..m) from chunk_excl (launch_chunk_scan)
 // and lsx.
 template <typename K>

@@ -347,9 +347,8 @@ __device__ inline void single_node(int32_t *octree, uint32_t *node_conflict) {
   if (node_conflict) node_conflict[0] = 0;
 }
 
-// Deferred entries are appended only by a run that fills the octree, so the
-// counter is still zero when a refill after a capacity failure runs this
-// again.
+// Nodes the window cannot resolve are marked in miss_words (one word per 64
+// BRT nodes) for the global-memory pass.
 // With chunk_excl the prefix sums ps[0..m) are written here (see
 // gather_window); without it they are read.
 template <typename K, int DIM>
@@ -358,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(
     const uint32_t *__restrict__ lsx, const uint32_t *__restrict__ chunk_excl,
     const K *__restrict__ keys, const uint32_t *__restrict__ conflict_bits, int mbits,
     int32_t *__restrict__ octree, uint32_t *__restrict__ node_conflict, int64_t capacity,
-    uint32_t *__restrict__ defer, RunState *st) {
+    uint64_t *__restrict__ miss_words, RunState *st) {
   if (st->malformed) return;
   const uint32_t m = st->m;
   const uint32_t q0 = xcd_segment(blockIdx.x, gridDim.x) * kBlock;
@@ -375,10 +374,8 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(
   }
   if (q0 >= m - 1) return;
   gather_window<K, DIM>(brt, ps, lsx, chunk_excl, ps, keys, conflict_bits, mbits, m, q0, octree,
-                        node_conflict,
-                        st, [&](bool miss, uint32_t q) {
-                          wave_append(miss, q, &st->counters[CTR_OCT_DEFER], defer);
-                        });
+                        node_conflict, st,
+                        [&](bool miss, uint32_t q) { wave_mask(miss, miss_words, q >> 6); });
 }
 
 // Batched form: gather tiles of 256 key slots per instance; instance s writes
@@ -413,16 +410,18 @@ __global__ __launch_bounds__(kBlock) void gather_seg_kernel(
                         });
 }
 
-// Gather, second pass: the deferred BRT nodes, reading global memory.
+// Gather, second pass: the BRT nodes marked in miss_words, reading global
+// memory.
 template <typename K, int DIM>
 __global__ __launch_bounds__(kBlock) void gather_defer_kernel(
     const BrtNodeT<K> *__restrict__ brt, const uint32_t *__restrict__ ps,
     const uint32_t *__restrict__ lsx, const K *__restrict__ keys,
     const uint32_t *__restrict__ conflict_bits, int mbits, int32_t *__restrict__ octree,
-    uint32_t *__restrict__ node_conflict, int64_t capacity, const uint32_t *__restrict__ defer,
-    RunState *st) {
+    uint32_t *__restrict__ node_conflict, int64_t capacity,
+    const uint64_t *__restrict__ miss_words, RunState *st) {
   if (st->malformed || (int64_t)st->octree_size > capacity) return;
-  const uint32_t cnt = st->counters[CTR_OCT_DEFER];
+  const uint32_t m = st->m;
+  const uint32_t nwords = m > 1 ? (m - 1 + 63) / 64u : 0u;
   auto links = [&](uint32_t x, uint32_t &left, bool &ll, bool &rl) -> bool {
     const BrtNodeT<K> &b = brt[x];
     left = (uint32_t)b.left;
@@ -436,12 +435,11 @@ __global__ __launch_bounds__(kBlock) void gather_defer_kernel(
     return true;
   };
   auto put = [&](uint32_t idx, const int32_t *c) { write_node<DIM>(octree, idx, c); };
-  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
-    const uint32_t q = defer[k];
+  for_each_masked(miss_words, nwords, [&](uint32_t q) {
     const uint32_t excl = q ? ps[q - 1] : 0u;
     (void)gather_one<K, DIM>(q, brt[q], excl, ps[q] - excl, links, child, keys, conflict_bits,
                              mbits, put, node_conflict, st);
-  }
+  });
 }
 
 template <typename K, int DIM>
@@ -579,13 +577,15 @@ void launch_dim(const BrtNodeT<K> *brt, const int32_t *parent, uint32_t *ps,
   const unsigned grid = (unsigned)((nodes + kBlock - 1) / kBlock);
   const unsigned sgrid = grid < 1024 ? grid : 1024;
   if (!force_scatter) {
+    uint64_t *words = reinterpret_cast<uint64_t *>(defer);
     gather_kernel<K, DIM><<<grid, kBlock, 0, s>>>(brt, ps, lsx, chunk_excl, keys, conflict_bits,
-                                                  mbits, octree, node_conflict, capacity, defer,
+                                                  mbits, octree, node_conflict, capacity, words,
                                                   st);
-    const unsigned dgrid = grid < 2048 ? grid : 2048;
+    const int64_t g2 = ((nodes + 63) / 64 + kBlock - 1) / kBlock;
+    const unsigned dgrid = (unsigned)(g2 < 4096 ? g2 : 4096);
     gather_defer_kernel<K, DIM><<<dgrid, kBlock, 0, s>>>(brt, ps, lsx, keys, conflict_bits,
                                                          mbits, octree, node_conflict,
-                                                         capacity, defer, st);
+                                                         capacity, words, st);
   }
   // The scatter path is a no-op unless forced or st->malformed; its kernels
   // loop over their ranges, so a capped grid keeps the no-op cheap.

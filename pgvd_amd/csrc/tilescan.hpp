@@ -79,6 +79,41 @@ __device__ inline T block_excl_sum(T x, T *s_w, T *total) {
   return before + incl - x;
 }
 
+// Calls f(index) for every bit set in words[0..nwords), index = word * 64 +
+// bit.  A workgroup takes 256 words at a time and spreads their set bits
+// over its threads (entry e goes to thread e % 256), so a word with many
+// bits does not hold one lane for all of them.  Uniform over the workgroup;
+// all 256 threads call it.
+template <typename F>
+__device__ inline void for_each_masked(const uint64_t *__restrict__ words, uint32_t nwords, F f) {
+  __shared__ uint64_t s_word[kBlock];
+  __shared__ uint32_t s_ofs[kBlock + 1];
+  __shared__ uint32_t s_w4[4];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t base = blockIdx.x * kBlock; base < nwords; base += gridDim.x * kBlock) {
+    const uint32_t wd = base + t;
+    const uint64_t b = wd < nwords ? words[wd] : 0ull;
+    uint32_t total;
+    const uint32_t ex = block_excl_sum<uint32_t>((uint32_t)__popcll(b), s_w4, &total);
+    s_word[t] = b;
+    s_ofs[t] = ex;
+    if (t == 0) s_ofs[kBlock] = total;
+    __syncthreads();
+    for (uint32_t e = t; e < total; e += kBlock) {
+      // the word holding entry e: s_ofs[lo] <= e < s_ofs[lo + 1]
+      uint32_t lo = 0, hi = kBlock;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_ofs[mid] <= e) lo = mid; else hi = mid;
+      }
+      uint64_t x = s_word[lo];
+      for (uint32_t k = e - s_ofs[lo]; k; --k) x &= x - 1;   // drop the lower set bits
+      f((base + lo) * 64u + (uint32_t)(__ffsll((unsigned long long)x) - 1));
+    }
+    __syncthreads();   // s_word, s_ofs and s_w4 are refilled next round
+  }
+}
+
 // R consecutive elements starting at p (valid of them in range): 16-byte
 // vector loads when the run is whole and aligned, else element loads.
 template <typename T, int R>

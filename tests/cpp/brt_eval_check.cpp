@@ -81,6 +81,10 @@ void make_keys(int mbits, int64_t m, uint64_t seed, bool clustered, std::vector<
   hi.swap(h2);
 }
 
+// the window shape of the BRT kernels: 1024 nodes and 256 positions of
+// halo either side
+constexpr int WN = 1536, LV = 11;
+
 template <typename K>
 int check(const char *name, int mbits, int64_t want_m, uint64_t seed, bool clustered, int seg,
           int halo) {
@@ -140,24 +144,35 @@ int check(const char *name, int mbits, int64_t want_m, uint64_t seed, bool clust
   // positions [b0 - halo, b0 + seg + halo)
   int64_t rmq_misses = 0;
   {
-    const int wn = seg + 2 * halo;
-    int levels = 1;
-    while ((2 << (levels - 1)) <= wn) ++levels;
-    std::vector<uint8_t> tab((size_t)levels * wn);
+    // the kernels' layout (DeltaTab), filled here from its definition: entry
+    // (j, k) for k = -2^j .. WN is 1 + min D over [k, k + 2^j) when that
+    // block lies in [0, WN) and in [0, m-2], else 0
+    constexpr int wn = WN, levels = LV;
+    using Tab = DeltaTab<WN, LV>;
+    std::vector<uint8_t> tab(Tab::kBytes, 0xEE);
     for (int64_t b0 = 0; b0 < n; b0 += seg) {
       const int64_t w0 = b0 - halo;
-      for (int k = 0; k < wn; ++k) {
+      auto d1 = [&](int k) -> int {     // level-0 value of window position k
         const int64_t g = w0 + k;
-        tab[k] = (g >= 0 && g < m - 1) ? (uint8_t)(1 + delta(keys[g], keys[g + 1], mbits)) : 0;
+        return (k >= 0 && k < wn && g >= 0 && g < m - 1) ? 1 + delta(keys[g], keys[g + 1], mbits)
+                                                         : 0;
+      };
+      // min over [k, k + 2^j) as the min of its two halves
+      std::vector<int> cur(wn + 1), nxt(wn + 1);
+      for (int k = 0; k <= wn; ++k) cur[k] = d1(k);
+      for (int j = 0; j < levels; ++j) {
+        for (int k = -(1 << j); k <= wn; ++k) {
+          const bool inside = k >= 0 && k + (1 << j) <= wn;
+          tab[(size_t)(Tab::base(j) + k)] = (uint8_t)(inside ? cur[k] : 0);
+        }
+        for (int k = 0; k <= wn; ++k)
+          nxt[k] = k + (2 << j) <= wn ? std::min(cur[k], cur[k + (1 << j)]) : 0;
+        cur.swap(nxt);
       }
-      for (int j = 1; j < levels; ++j)
-        for (int k = 0; k + (1 << j) <= wn; ++k)
-          tab[(size_t)j * wn + k] = std::min(tab[(size_t)(j - 1) * wn + k],
-                                             tab[(size_t)(j - 1) * wn + k + (1 << (j - 1))]);
-      auto mn = [&](int j, int k) -> int { return tab[(size_t)j * wn + k]; };
+      const Tab view{tab.data()};
       for (int64_t i = b0; i < std::min<int64_t>(b0 + seg, n); ++i) {
         BrtOut<K> o;
-        if (!brt_node_window(keys[i], mn, levels, (int)w0, wn, mbits, (int)m, (int)i, o)) {
+        if (!brt_node_window(keys[i], view, (int)w0, mbits, (int)m, (int)i, o)) {
           ++rmq_misses;
           continue;
         }
