@@ -117,6 +117,7 @@ size_t pgvd_capi::plain_footprint_bytes(int64_t n, int dim, int bits, bool label
   if (labels) b += 2 * (size_t)n * 4;                    // vals_a, vals_b
   b += (size_t)nk * node;                                // brt
   b += 3 * (size_t)nk * 4;                               // ps, parent, defer
+  b += kBboxPartBytes;                                   // bbox_part
   b += (size_t)((nk + kBlock - 1) / kBlock) * 4;         // chunks
   b += zero_layout(n, kb, (mbits + 7) / 8, labels).total;
   b += (size_t)oct_nodes * ((1u << dim) + 1) * 4;        // octree
@@ -240,7 +241,8 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   } else {
     if (!custom) {
       Stage t(ctx, "bbox", s);
-      launch_bbox(static_cast<const float *>(pts), n, dim, ctx->st, s);
+      CHK(pool_get(ctx, "bbox_part", kBboxPartBytes, &p));
+      launch_bbox(static_cast<const float *>(pts), n, dim, static_cast<float *>(p), ctx->st, s);
     }
     const int64_t width = (int64_t)1 << bits;
     const float effw = (float)(clamped ? width : width - 1);
@@ -610,9 +612,11 @@ int pgvd_bbox(pgvd_ctx *ctx, const float *pts, int64_t n, int dim, float *bb_out
   CHK(use_device(ctx));
   hipStream_t st = pick(ctx, s);
   launch_prep(ctx->st, nullptr, 0, st);
+  void *part;
+  CHK(pool_get(ctx, "bbox_part", kBboxPartBytes, &part));
   {
     Stage t(ctx, "bbox", st);
-    launch_bbox(pts, n, dim, ctx->st, st);
+    launch_bbox(pts, n, dim, static_cast<float *>(part), ctx->st, st);
   }
   CHK(launch_status("bbox"));
   CHK(copy_state_and_wait(ctx, st));
@@ -633,7 +637,11 @@ int pgvd_quantize(pgvd_ctx *ctx, const float *pts, int64_t n, int dim, int bits,
   CHK(use_device(ctx));
   hipStream_t st = pick(ctx, s);
   launch_prep(ctx->st, nullptr, 0, st);
-  if (!custom_bb) launch_bbox(pts, n, dim, ctx->st, st);
+  if (!custom_bb) {
+    void *part;
+    CHK(pool_get(ctx, "bbox_part", kBboxPartBytes, &part));
+    launch_bbox(pts, n, dim, static_cast<float *>(part), ctx->st, st);
+  }
   const int64_t width = (int64_t)1 << bits;
   const float effw = (float)(clamped ? width : width - 1);
   {

@@ -122,7 +122,9 @@ __device__ inline void or_shifted(uint64_t v, int s, uint64_t &lo, uint64_t &hi)
 // Morton key of DIM coordinates: key bit i*DIM+k = bit i of coordinate k for
 // i < bits (C/z_order.h:14-47).  Bits of a coordinate at or above `bits` are
 // ignored, as xyz2z ignores them.
-template <int DIM>
+// WIDE = false: the key has at most 64 bits (bits <= 32 in 2D, <= 21 in 3D),
+// so the coordinates' upper parts are zero after the mask and are skipped.
+template <int DIM, bool WIDE = true>
 __device__ inline void morton128(const int64_t *c, int bits, uint64_t &lo, uint64_t &hi) {
   lo = 0; hi = 0;
   const uint64_t cmask = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
@@ -131,10 +133,10 @@ __device__ inline void morton128(const int64_t *c, int bits, uint64_t &lo, uint6
     const uint64_t v = (uint64_t)c[k] & cmask;
     if (DIM == 2) {
       or_shifted(spread2((uint32_t)v), k, lo, hi);
-      or_shifted(spread2((uint32_t)(v >> 32)), 64 + k, lo, hi);
+      if (WIDE) or_shifted(spread2((uint32_t)(v >> 32)), 64 + k, lo, hi);
     } else {
       or_shifted(spread3((uint32_t)(v & 0x1FFFFF)), k, lo, hi);
-      or_shifted(spread3((uint32_t)((v >> 21) & 0x1FFFFF)), 63 + k, lo, hi);
+      if (WIDE) or_shifted(spread3((uint32_t)((v >> 21) & 0x1FFFFF)), 63 + k, lo, hi);
       // bits 42..63 of a coordinate would land at key bits >= 126 + k; the
       // widest supported key is 3*42 = 126 bits, so they are always masked.
     }

@@ -92,7 +92,11 @@ constexpr int kEncodeChunk = 65536;
 
 // ---- morton.hip ----
 void launch_prep(RunState *st, uint32_t *zero_words, size_t nzero, hipStream_t s);
-void launch_bbox(const float *pts, int64_t n, int dim, RunState *st, hipStream_t s);
+// partials: scratch of kBboxPartBytes (one 8-float record per workgroup).
+constexpr int kBboxParts = 2048;
+constexpr size_t kBboxPartBytes = (size_t)kBboxParts * 8 * sizeof(float);
+void launch_bbox(const float *pts, int64_t n, int dim, float *partials, RunState *st,
+                 hipStream_t s);
 // Quantize + Morton encode (+ optional digit histograms for `passes` passes).
 // bb: if custom, (min, max) given; else taken from st (launch_bbox).
 template <typename K>

@@ -69,7 +69,9 @@ __global__ __launch_bounds__(kBlock) void bbox_kernel(const float *__restrict__ 
 
 // One point -> quantized coordinates (Karras.cpp:50-65).  Each FP32 step is
 // an explicitly rounded op so no contraction or reassociation can change it.
-template <int DIM>
+// SMALL: values in [0, 2^32) take the one-instruction u32 conversion, which
+// truncates the same way; others the general 64-bit one.
+template <int DIM, bool SMALL = false>
 __device__ inline bool quantize_point(const float *p, const float *mn, float dwidth, float effw,
                                       int64_t *c) {
   bool neg = false;
@@ -79,7 +81,8 @@ __device__ inline bool quantize_point(const float *p, const float *mn, float dwi
     const float t = __fdiv_rn(__fsub_rn(p[k], mn[k]), dwidth);
     const float f = __fmul_rn(effw, t);
     const double d = __dadd_rn((double)f, 0.5);
-    const int64_t v = __double2ll_rz(d);
+    const int64_t v = (SMALL && d >= 0.0 && d < 4294967296.0) ? (int64_t)__double2uint_rz(d)
+                                                               : __double2ll_rz(d);
     neg |= (v < 0);
     c[k] = v;
   }
@@ -122,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void encode_kernel(
       for (int k = 0; k < DIM; ++k) qout[i * DIM + k] = c[k];
     }
     uint64_t lo, hi;
-    morton128<DIM>(c, bits, lo, hi);
+    morton128<DIM, (sizeof(K) > 8)>(c, bits, lo, hi);
     const K key = from128<K>(lo, hi);
     if (keys) keys[i] = key;
     for (int r = 0; r < passes; ++r) atomicAdd(&sh[r * 256 + kdigit(key, 8 * r)], 1u);
@@ -146,31 +149,103 @@ __device__ inline void load_chunk(const float *__restrict__ pts, int64_t ch, flo
   }
 }
 
+// Bounding box over the points as a flat float array: thread t reads float4
+// number f = B + 256 u + t (B a multiple of 3 * 256 * kBboxU4), so each load
+// instruction of a wave covers 1 KiB contiguously.  In 3D float f * 4 + j is
+// component (f + j) % 3 and f % 3 = (u + t) % 3: lanes are accumulated per
+// u % 3 and mapped to components once at the end; in 2D float f * 4 + j is
+// component j % 2.  Floats past the last whole float4 are read one by one.
+constexpr int kBboxU4 = 12;   // float4 loads in flight per thread (multiple of 3)
+
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void bbox_vec_kernel(const float *__restrict__ pts, int64_t n,
-                                                          RunState *st) {
+                                                          float *__restrict__ partials) {
+  static_assert(kBboxU4 % 3 == 0, "per-round span is a multiple of 3 float4s");
+  constexpr int NA = DIM == 3 ? 3 : 1;
+  const float kNan = __int_as_float(0x7fc00000);   // fminf / fmaxf skip it
+  const float kInf = __int_as_float(0x7f800000);
+  float amn[NA][4], amx[NA][4];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { amn[a][j] = kInf; amx[a][j] = -kInf; }
+  const int64_t nf4 = n * DIM / 4;
+  const float4 *p4 = reinterpret_cast<const float4 *>(pts);
+  const int64_t span = (int64_t)kBlock * kBboxU4;
+  for (int64_t B = (int64_t)blockIdx.x * span; B < nf4; B += (int64_t)gridDim.x * span) {
+    float4 v[kBboxU4];
+#pragma unroll
+    for (int u = 0; u < kBboxU4; ++u) {
+      const int64_t f = B + (int64_t)u * kBlock + threadIdx.x;
+      v[u] = p4[f < nf4 ? f : 0];
+      if (f >= nf4) v[u] = make_float4(kNan, kNan, kNan, kNan);
+    }
+#pragma unroll
+    for (int u = 0; u < kBboxU4; ++u) {
+      const int a = u % NA;
+      amn[a][0] = fminf(amn[a][0], v[u].x); amx[a][0] = fmaxf(amx[a][0], v[u].x);
+      amn[a][1] = fminf(amn[a][1], v[u].y); amx[a][1] = fmaxf(amx[a][1], v[u].y);
+      amn[a][2] = fminf(amn[a][2], v[u].z); amx[a][2] = fmaxf(amx[a][2], v[u].z);
+      amn[a][3] = fminf(amn[a][3], v[u].w); amx[a][3] = fmaxf(amx[a][3], v[u].w);
+    }
+  }
   float mn[DIM], mx[DIM];
 #pragma unroll
-  for (int k = 0; k < DIM; ++k) { mn[k] = __int_as_float(0x7f800000); mx[k] = -mn[k]; }
-  const int64_t nch = n / 4;
-  const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t ch = tid; ch < nch; ch += stride) {
-    float f[4 * DIM];
-    load_chunk<DIM>(pts, ch, f);
+  for (int k = 0; k < DIM; ++k) { mn[k] = kInf; mx[k] = -kInf; }
+  const int t3 = (int)(threadIdx.x % 3u);
 #pragma unroll
-    for (int j = 0; j < 4 * DIM; ++j) {
-      mn[j % DIM] = fminf(mn[j % DIM], f[j]);
-      mx[j % DIM] = fmaxf(mx[j % DIM], f[j]);
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int comp = DIM == 3 ? (a + j + t3) % 3 : (j & 1);
+#pragma unroll
+      for (int k = 0; k < DIM; ++k)
+        if (comp == k) { mn[k] = fminf(mn[k], amn[a][j]); mx[k] = fmaxf(mx[k], amx[a][j]); }
     }
+  // floats after the last whole float4
+  for (int64_t e = nf4 * 4 + (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n * DIM;
+       e += (int64_t)gridDim.x * kBlock) {
+    const int k = (int)(e % DIM);
+#pragma unroll
+    for (int kk = 0; kk < DIM; ++kk)
+      if (kk == k) { mn[kk] = fminf(mn[kk], pts[e]); mx[kk] = fmaxf(mx[kk], pts[e]); }
   }
-  for (int64_t i = nch * 4 + tid; i < n; i += stride) {
+  __shared__ float smn[4][DIM], smx[4][DIM];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    for (int d = 32; d >= 1; d >>= 1) {
+      mn[k] = fminf(mn[k], __shfl_xor(mn[k], d, 64));
+      mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], d, 64));
+    }
+    if (lane == 0) { smn[w][k] = mn[k]; smx[w][k] = mx[k]; }
+  }
+  __syncthreads();
+  // one record per workgroup (min[0..3), max[4..7)); no atomics, which would
+  // all meet on one cache line
+  if (threadIdx.x < DIM) {
+    const int k = threadIdx.x;
+    float a = smn[0][k], b = smx[0][k];
+    for (int j = 1; j < 4; ++j) { a = fminf(a, smn[j][k]); b = fmaxf(b, smx[j][k]); }
+    partials[blockIdx.x * 8 + k] = a;
+    partials[blockIdx.x * 8 + 4 + k] = b;
+  }
+}
+
+// Folds the per-workgroup records of bbox_vec_kernel into st's box.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void bbox_finish_kernel(const float *__restrict__ partials,
+                                                             int nparts, RunState *st) {
+  const float kInf = __int_as_float(0x7f800000);
+  float mn[DIM], mx[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) { mn[k] = kInf; mx[k] = -kInf; }
+  for (int b = threadIdx.x; b < nparts; b += kBlock)
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
-      mn[k] = fminf(mn[k], pts[i * DIM + k]);
-      mx[k] = fmaxf(mx[k], pts[i * DIM + k]);
+      mn[k] = fminf(mn[k], partials[b * 8 + k]);
+      mx[k] = fmaxf(mx[k], partials[b * 8 + 4 + k]);
     }
-  }
   __shared__ float smn[4][DIM], smx[4][DIM];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -186,8 +261,8 @@ __global__ __launch_bounds__(kBlock) void bbox_vec_kernel(const float *__restric
     const int k = threadIdx.x;
     float a = smn[0][k], b = smx[0][k];
     for (int j = 1; j < 4; ++j) { a = fminf(a, smn[j][k]); b = fmaxf(b, smx[j][k]); }
-    atomicMin(&st->bb_min[k], fenc(a));
-    atomicMax(&st->bb_max[k], fenc(b));
+    st->bb_min[k] = min(st->bb_min[k], fenc(a));
+    st->bb_max[k] = max(st->bb_max[k], fenc(b));
   }
 }
 
@@ -230,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void encode_vec_kernel(
     int64_t c[DIM];
     neg |= quantize_point<DIM>(p, mn, dwidth, effw, c);
     uint64_t lo, hi;
-    morton128<DIM>(c, bits, lo, hi);
+    morton128<DIM, (sizeof(K) > 8)>(c, bits, lo, hi);
     const K key = from128<K>(lo, hi);
     for (int r = 0; r < passes; ++r) atomicAdd(&sh[r * 256 + kdigit(key, 8 * r)], 1u);
     return key;
@@ -262,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void points_to_morton_kernel(const C *__res
 #pragma unroll
       for (int k = 0; k < DIM; ++k) c[k] = (int64_t)pts[i * DIM + k];
       uint64_t lo, hi;
-      morton128<DIM>(c, bits, lo, hi);
+      morton128<DIM, (sizeof(K) > 8)>(c, bits, lo, hi);
       key = from128<K>(lo, hi);
     }
     keys[i] = key;
@@ -284,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void encode_int_kernel(const int32_t *__res
 #pragma unroll
     for (int k = 0; k < DIM; ++k) c[k] = (int64_t)pts[i * DIM + k];
     uint64_t lo, hi;
-    morton128<DIM>(c, bits, lo, hi);
+    morton128<DIM, (sizeof(K) > 8)>(c, bits, lo, hi);
     const K key = from128<K>(lo, hi);
     keys[i] = key;
     for (int r = 0; r < passes; ++r) atomicAdd(&sh[r * 256 + kdigit(key, 8 * r)], 1u);
@@ -408,7 +483,7 @@ __global__ __launch_bounds__(kBlock) void encode_seg_kernel(
       neg |= quantize_point<DIM>(f, mn, dwidth, effw, c);
     }
     uint64_t lo, hi;
-    morton128<DIM>(c, bits, lo, hi);
+    morton128<DIM, (sizeof(K) > 8)>(c, bits, lo, hi);
     const K key = from128<K>(lo, hi);
     out[i] = key;
     for (int r = 0; r < passes; ++r) atomicAdd(&sh[r * 256 + kdigit(key, 8 * r)], 1u);
@@ -432,11 +507,18 @@ void launch_prep(RunState *st, uint32_t *zero_words, size_t nzero, hipStream_t s
 
 inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
-void launch_bbox(const float *pts, int64_t n, int dim, RunState *st, hipStream_t s) {
+void launch_bbox(const float *pts, int64_t n, int dim, float *partials, RunState *st,
+                 hipStream_t s) {
+  static_assert(kMaxGrid <= kBboxParts, "one partial record per workgroup");
   if (aligned16(pts)) {
-    const int grid = grid_for(n / 4 + 1, kBlock * 2);
-    if (dim == 2) bbox_vec_kernel<2><<<grid, kBlock, 0, s>>>(pts, n, st);
-    else bbox_vec_kernel<3><<<grid, kBlock, 0, s>>>(pts, n, st);
+    const int grid = grid_for(n * dim / 4 + 1, kBlock * kBboxU4 * 2);
+    if (dim == 2) {
+      bbox_vec_kernel<2><<<grid, kBlock, 0, s>>>(pts, n, partials);
+      bbox_finish_kernel<2><<<1, kBlock, 0, s>>>(partials, grid, st);
+    } else {
+      bbox_vec_kernel<3><<<grid, kBlock, 0, s>>>(pts, n, partials);
+      bbox_finish_kernel<3><<<1, kBlock, 0, s>>>(partials, grid, st);
+    }
     return;
   }
   const int grid = grid_for(n, kBlock * 8);

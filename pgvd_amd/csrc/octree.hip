@@ -249,41 +249,43 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
   const int w0 = (int)q0 - H;
   // q's own record is loaded first so that its latency overlaps the window
   // load instead of following it.
-  BrtNodeT<K> nq{};
-  if (q <= m - 2) nq = brt[q];
+  // Every load of the window is issued before the first is used: addresses
+  // outside [0, m) are clamped to entry 0 and the values dropped.
+  static_assert(WN == 3 * kBlock && H == kBlock, "window of three chunks");
+  const BrtNodeT<K> nq = brt[q <= m - 2 ? q : 0u];
+  uint32_t lx[3], lk[3], pv[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int g = w0 + r * kBlock + (int)threadIdx.x;
+    const bool in = g >= 0 && g < (int)m;
+    lx[r] = lsx[in ? g : 0];
+    lk[r] = load_links(brt + (in && g < (int)m - 1 ? g : 0));
+    if (!chunk_excl) pv[r] = ps[in ? g : 0];
+  }
   if (chunk_excl) {
     // The window is three whole chunks (q0 is a multiple of 256), so the
     // base of the first one plus a running scan gives every prefix sum.
-    static_assert(WN == 3 * kBlock && H == kBlock, "window of three chunks");
     __shared__ uint32_t s_w[3 * 4];
+    const uint32_t base = q0 ? chunk_excl[(q0 >> 8) - 1] : 0u;
     uint32_t v[3], ex[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      const int k = r * kBlock + (int)threadIdx.x;
-      const int g = w0 + k;
-      uint32_t lx = 0;
-      if (g >= 0 && g < (int)m) {
-        lx = lsx[g];
-        if (g < (int)m - 1) s_lnk[k] = load_links(brt + g);
-      }
-      s_lsx[k] = lx;
-      v[r] = lx & 0xffu;
+      const int g = w0 + r * kBlock + (int)threadIdx.x;
+      if (g < 0 || g >= (int)m) lx[r] = 0;
+      v[r] = lx[r] & 0xffu;
     }
-    const uint32_t base = q0 ? chunk_excl[(q0 >> 8) - 1] : 0u;
     (void)striped_excl_scan<3, uint32_t>(v, ex, s_w);
 #pragma unroll
-    for (int r = 0; r < 3; ++r) s_ps[r * kBlock + threadIdx.x] = base + ex[r] + v[r];
-    if (q < m) ps_out[q] = base + ex[1] + v[1];
-    if (threadIdx.x == 0 && q0 + kBlock == m - 1) ps_out[m - 1] = base + ex[2] + v[2];
-  } else {
-    for (int r = threadIdx.x; r < WN; r += kBlock) {
-      const int g = w0 + r;
-      if (g >= 0 && g < (int)m) {
-        s_ps[r] = ps[g];
-        s_lsx[r] = lsx[g];
-        if (g < (int)m - 1) s_lnk[r] = load_links(brt + g);
-      }
-    }
+    for (int r = 0; r < 3; ++r) pv[r] = base + ex[r] + v[r];
+    if (q < m) ps_out[q] = pv[1];
+    if (threadIdx.x == 0 && q0 + kBlock == m - 1) ps_out[m - 1] = pv[2];
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int k = r * kBlock + (int)threadIdx.x;
+    s_lsx[k] = lx[r];
+    s_lnk[k] = lk[r];
+    s_ps[k] = pv[r];
   }
   __syncthreads();
   const uint32_t qlast = min(q0 + kBlock - 1, m - 2);
