@@ -137,15 +137,65 @@ __device__ inline bool gather_one(uint32_t q, const BrtNodeT<K> &nq, uint32_t ex
   return true;
 }
 
-// gather_one for the window pass with its output staged in LDS: the
+// One breadth-first round of gather_one_bfs over W slots (BRT child
+// indices, leaf flags, live flags): a live internal slot inside the window
+// is either the top of a child chain (local split > 0: recorded in node0 at
+// its octant) or a same-level node, whose two children become slots of the
+// next round.  All LDS reads of a round are independent.
+template <typename K, int DIM, int W, bool LAST>
+__device__ inline void bfs_round(const uint32_t (&idx)[W], const bool (&lf)[W],
+                                 const bool (&live)[W], uint32_t (&nidx)[2 * W],
+                                 bool (&nlf)[2 * W], bool (&nlive)[2 * W], int w0, int wn,
+                                 const uint32_t *s_lnk, const uint32_t *s_ps,
+                                 const uint32_t *s_lsx, const K *__restrict__ keys,
+                                 const uint32_t *__restrict__ conflict_bits, int lvl, int mbits,
+                                 int32_t *node0, uint32_t &found, uint32_t &cmask, bool &miss,
+                                 RunState *st) {
+#pragma unroll
+  for (int s = 0; s < W; ++s) {
+    const bool internal = live[s] && !lf[s];
+    const unsigned k = (unsigned)((int)idx[s] - w0);
+    const bool inwin = k < (unsigned)wn;
+    miss |= internal && !inwin;
+    const bool use = internal && inwin;
+    const unsigned kk = use ? k : 0u;
+    const uint32_t lx = s_lsx[kk];
+    const bool term = use && (lx & 0xffu) != 0;
+    const bool expand = use && (lx & 0xffu) == 0;
+    if (term) {
+      const uint32_t o = lx >> 8;
+      node0[o] = (int32_t)(s_ps[kk] - 1);   // the child chain's top, in its octant
+      found |= 1u << o;
+    }
+    if (live[s] && lf[s] && conflict_bits && test_bit(conflict_bits, idx[s]))
+      cmask |= 1u << digit_at<K, DIM>(keys[idx[s]], lvl, mbits);
+    if (LAST) {
+      // a well-formed tree has no same-level node this deep; a malformed one
+      // the BRT stage did not flag is reported
+      if (expand) atomicOr(&st->status, (uint32_t)ST_STACK);
+    } else {
+      const uint32_t v = s_lnk[kk];
+      nidx[2 * s] = v & 0x3fffffffu;
+      nidx[2 * s + 1] = (v & 0x3fffffffu) + 1;
+      nlf[2 * s] = (v >> 30) & 1u;
+      nlf[2 * s + 1] = v >> 31;
+      nlive[2 * s] = expand;
+      nlive[2 * s + 1] = expand;
+    }
+  }
+}
+
+// gather_one for the window pass with its output staged in LDS (the
 // workgroup has filled the staged nodes with -1 children and full leaf
-// masks, so only found children and leaf words are written.  The pending
-// same-level nodes are window indices (< 1024) packed 10 bits each into one
-// 64-bit register (at most 2^DIM - 2 <= 6 of them).  node0 is q's lowest
-// staged node.  Returns false, writing nothing, when the descent leaves the
-// window.
+// masks, so only found children and leaf words are written).  The
+// same-level BRT nodes below q split on later bits of the digit q splits,
+// so they form a binary trie of fewer than DIM levels: DIM rounds over
+// fixed slots (2, 4, 8) find every child chain without a loop or stack.
+// node0 is q's lowest staged node.  Returns false when an internal node of
+// that trie lies outside the window; the global-memory pass then rewrites
+// q's nodes whole.
 template <typename K, int DIM>
-__device__ inline bool gather_one_lds(uint32_t q, const BrtNodeT<K> &nq, uint32_t excl,
+__device__ inline bool gather_one_bfs(uint32_t q, const BrtNodeT<K> &nq, uint32_t excl,
                                       uint32_t lsq, int w0, int wn, const uint32_t *s_lnk,
                                       const uint32_t *s_ps, const uint32_t *s_lsx,
                                       const K *__restrict__ keys,
@@ -156,48 +206,26 @@ __device__ inline bool gather_one_lds(uint32_t q, const BrtNodeT<K> &nq, uint32_
   constexpr int STRIDE = NC + 1;
   constexpr int32_t kFull = (1 << NC) - 1;
   const int lvl = nq.lcp_length / DIM;
-  uint32_t cmask = 0, found = 0;
-  uint64_t stk = 0;
-  int sp = 0, visits = 0;
-  uint32_t xl = (uint32_t)nq.left;
-  bool xll = nq.left_leaf, xrl = nq.right_leaf;
-  for (;;) {
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const uint32_t c = xl + side;
-      if (side ? xrl : xll) {
-        if (conflict_bits && test_bit(conflict_bits, c))
-          cmask |= 1u << digit_at<K, DIM>(keys[c], lvl, mbits);
-        continue;
-      }
-      const unsigned kc = (unsigned)((int)c - w0);
-      if (kc >= (unsigned)wn) return false;
-      const uint32_t lxc = s_lsx[kc];
-      if (lxc & 0xffu) {
-        const uint32_t o = lxc >> 8;
-        node0[o] = (int32_t)(s_ps[kc] - 1);   // c's chain top, in its octant
-        found |= 1u << o;
-      } else if (sp < 6) {
-        stk = (stk << 10) | kc;
-        ++sp;
-      } else {
-        atomicOr(&st->status, (uint32_t)ST_STACK);
-      }
-    }
-    if (sp == 0) break;
-    // A well-formed tree visits at most 2^DIM - 1 nodes here; the cap only
-    // guards against a malformed tree the BRT stage did not flag.
-    if (++visits > 64) {
-      atomicOr(&st->status, (uint32_t)ST_STACK);
-      break;
-    }
-    const uint32_t v = s_lnk[(uint32_t)stk & 0x3ffu];
-    stk >>= 10;
-    --sp;
-    xl = v & 0x3fffffffu;
-    xll = (v >> 30) & 1u;
-    xrl = v >> 31;
+  uint32_t found = 0, cmask = 0;
+  bool miss = false;
+  const uint32_t i1[2] = {(uint32_t)nq.left, (uint32_t)nq.left + 1};
+  const bool f1[2] = {(bool)nq.left_leaf, (bool)nq.right_leaf};
+  const bool l1[2] = {true, true};
+  uint32_t i2[4];
+  bool f2[4], l2[4];
+  bfs_round<K, DIM, 2, DIM == 1>(i1, f1, l1, i2, f2, l2, w0, wn, s_lnk, s_ps, s_lsx, keys,
+                                 conflict_bits, lvl, mbits, node0, found, cmask, miss, st);
+  uint32_t i3[8];
+  bool f3[8], l3[8];
+  bfs_round<K, DIM, 4, DIM == 2>(i2, f2, l2, i3, f3, l3, w0, wn, s_lnk, s_ps, s_lsx, keys,
+                                 conflict_bits, lvl, mbits, node0, found, cmask, miss, st);
+  if (DIM == 3) {
+    uint32_t i4[16];
+    bool f4[16], l4[16];
+    bfs_round<K, DIM, 8, true>(i3, f3, l3, i4, f4, l4, w0, wn, s_lnk, s_ps, s_lsx, keys,
+                               conflict_bits, lvl, mbits, node0, found, cmask, miss, st);
   }
+  if (miss) return false;
   node0[NC] = kFull & ~(int32_t)found;
   if (node_conflict) node_conflict[excl] = cmask;
   for (uint32_t k = 1; k < lsq; ++k) {
@@ -212,24 +240,28 @@ __device__ inline bool gather_one_lds(uint32_t q, const BrtNodeT<K> &nq, uint32_
   return true;
 }
 
-// Octree nodes staged per workgroup: the nodes owned by BRT nodes
-// [q0, q0+256) are the contiguous range [ps[q0-1], ps[q0+255]); when it fits
-// here it is built in LDS and written out in coalesced runs.
-constexpr int kStageNodes = 256;
 // BRT entries (links, ps, lsx) staged either side of a workgroup's nodes.
 constexpr int kGatherHalo = 256;
+// BRT nodes per workgroup of the single-tree gather: kGatherRows rows of 256.
+#ifndef PGVD_GATHER_ROWS
+#define PGVD_GATHER_ROWS 2
+#endif
+constexpr int kGatherRows = PGVD_GATHER_ROWS;
 
-// Gather, first pass over the BRT nodes [q0, q0+256) of one tree with m
+// Gather, first pass over the BRT nodes [q0, q0+R*256) of one tree with m
 // keys (m >= 2).  The workgroup stages links, ps and lsx of BRT nodes
-// [q0-H, q0+256+H) in LDS; a node whose descent leaves that window is handed
-// to defer(true, q) and left to the global-memory pass (whose writes land
-// after this kernel's, so the window pass's placeholder output for it is
-// overwritten).  Every lane calls defer once, converged.
+// [q0-H, q0+R*256+H) in LDS; a node whose descent leaves that window is
+// handed to defer(true, q) and left to the global-memory pass (whose writes
+// land after this kernel's, so the window pass's placeholder output for it
+// is overwritten).  Every lane calls defer once per row, converged.  The
+// octree nodes the workgroup's BRT nodes own are the contiguous range
+// [ps[q0-1], ps[q_last]); when it holds at most R*256 nodes it is built in
+// LDS and written out in coalesced runs.
 // With chunk_excl (exclusive prefix sums of the local splits per chunk of
 // 256 nodes), the window's prefix sums are computed here from lsx instead
 // of read from ps, and the workgroup writes them to ps_out for its own
 // nodes (and for node m-1 when that starts the next chunk).
-template <typename K, int DIM, typename Defer>
+template <typename K, int DIM, int R, typename Defer>
 __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
                                      const uint32_t *__restrict__ ps,
                                      const uint32_t *__restrict__ lsx,
@@ -242,100 +274,111 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
                                      Defer defer) {
   constexpr int NC = 1 << DIM;
   constexpr int STRIDE = NC + 1;
-  constexpr int H = kGatherHalo, WN = kBlock + 2 * H;
-  __shared__ int32_t s_nodes[kStageNodes * STRIDE];
+  constexpr int H = kGatherHalo, WR = R + 2, WN = WR * kBlock, STAGE = R * kBlock;
+  static_assert(H == kBlock, "the window is whole chunks of 256");
+  __shared__ int32_t s_nodes[STAGE * STRIDE];
   __shared__ uint32_t s_lnk[WN], s_ps[WN], s_lsx[WN];
-  const uint32_t q = q0 + threadIdx.x;
+  const int t = threadIdx.x;
   const int w0 = (int)q0 - H;
-  // q's own record is loaded first so that its latency overlaps the window
-  // load instead of following it.
   // Every load of the window is issued before the first is used: addresses
   // outside [0, m) are clamped to entry 0 and the values dropped.
-  static_assert(WN == 3 * kBlock && H == kBlock, "window of three chunks");
-  const BrtNodeT<K> nq = brt[q <= m - 2 ? q : 0u];
-  uint32_t lx[3], lk[3], pv[3];
+  BrtNodeT<K> nq[R];
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const int g = w0 + r * kBlock + (int)threadIdx.x;
+  for (int j = 0; j < R; ++j) {
+    const uint32_t q = q0 + j * kBlock + t;
+    nq[j] = brt[q <= m - 2 ? q : 0u];
+  }
+  uint32_t lx[WR], lk[WR], pv[WR];
+#pragma unroll
+  for (int r = 0; r < WR; ++r) {
+    const int g = w0 + r * kBlock + t;
     const bool in = g >= 0 && g < (int)m;
     lx[r] = lsx[in ? g : 0];
     lk[r] = load_links(brt + (in && g < (int)m - 1 ? g : 0));
     if (!chunk_excl) pv[r] = ps[in ? g : 0];
   }
   if (chunk_excl) {
-    // The window is three whole chunks (q0 is a multiple of 256), so the
-    // base of the first one plus a running scan gives every prefix sum.
-    __shared__ uint32_t s_w[3 * 4];
+    // The window is whole chunks (q0 is a multiple of 256), so the base of
+    // the first one plus a running scan gives every prefix sum.
+    __shared__ uint32_t s_w[WR * 4];
     const uint32_t base = q0 ? chunk_excl[(q0 >> 8) - 1] : 0u;
-    uint32_t v[3], ex[3];
+    uint32_t v[WR], ex[WR];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      const int g = w0 + r * kBlock + (int)threadIdx.x;
+    for (int r = 0; r < WR; ++r) {
+      const int g = w0 + r * kBlock + t;
       if (g < 0 || g >= (int)m) lx[r] = 0;
       v[r] = lx[r] & 0xffu;
     }
-    (void)striped_excl_scan<3, uint32_t>(v, ex, s_w);
+    (void)striped_excl_scan<WR, uint32_t>(v, ex, s_w);
 #pragma unroll
-    for (int r = 0; r < 3; ++r) pv[r] = base + ex[r] + v[r];
-    if (q < m) ps_out[q] = pv[1];
-    if (threadIdx.x == 0 && q0 + kBlock == m - 1) ps_out[m - 1] = pv[2];
+    for (int r = 0; r < WR; ++r) pv[r] = base + ex[r] + v[r];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const uint32_t q = q0 + j * kBlock + t;
+      if (q < m) ps_out[q] = pv[1 + j];
+    }
+    if (t == 0 && q0 + STAGE == m - 1) ps_out[m - 1] = pv[R + 1];
   }
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const int k = r * kBlock + (int)threadIdx.x;
+  for (int r = 0; r < WR; ++r) {
+    const int k = r * kBlock + t;
     s_lsx[k] = lx[r];
     s_lnk[k] = lk[r];
     s_ps[k] = pv[r];
   }
   __syncthreads();
-  const uint32_t qlast = min(q0 + kBlock - 1, m - 2);
+  const uint32_t qlast = min(q0 + STAGE - 1, m - 2);
   const uint32_t lo = q0 ? s_ps[q0 - 1 - w0] : 0u;
   const uint32_t span = s_ps[qlast - w0] - lo;
-  const bool staged = span <= (uint32_t)kStageNodes;   // uniform over the workgroup
+  const bool staged = span <= (uint32_t)STAGE;   // uniform over the workgroup
   if (staged) {
     // every staged node starts with no children and a full leaf mask
-    for (uint32_t w = threadIdx.x; w < span * STRIDE; w += kBlock)
+    for (uint32_t w = t; w < span * STRIDE; w += kBlock)
       s_nodes[w] = (w % STRIDE == (uint32_t)NC) ? (1 << NC) - 1 : -1;
     __syncthreads();
   }
-  bool active = q <= qlast, miss = false;
-  uint32_t excl = 0, lsq = 0;
-  if (active) {
-    excl = q ? s_ps[q - 1 - w0] : 0u;
-    lsq = s_ps[q - w0] - excl;
-    active = (q == 0) || lsq > 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const uint32_t q = q0 + j * kBlock + t;
+    bool active = q <= qlast, miss = false;
+    uint32_t excl = 0, lsq = 0;
+    if (active) {
+      excl = q ? s_ps[q - 1 - w0] : 0u;
+      lsq = s_ps[q - w0] - excl;
+      active = (q == 0) || lsq > 0;
+    }
+    if (active && staged) {
+      miss = !gather_one_bfs<K, DIM>(q, nq[j], excl, lsq, w0, WN, s_lnk, s_ps, s_lsx, keys,
+                                     conflict_bits, mbits, s_nodes + (excl - lo) * STRIDE,
+                                     node_conflict, st);
+    } else if (active) {
+      auto links = [&](uint32_t x, uint32_t &left, bool &ll, bool &rl) -> bool {
+        const unsigned k = (unsigned)((int)x - w0);
+        if (k >= (unsigned)WN) return false;
+        const uint32_t v = s_lnk[k];
+        left = v & 0x3fffffffu;
+        ll = (v >> 30) & 1u;
+        rl = v >> 31;
+        return true;
+      };
+      auto child = [&](uint32_t c, uint32_t &psc, uint32_t &lxc) -> bool {
+        const unsigned k = (unsigned)((int)c - w0);
+        if (k >= (unsigned)WN) return false;
+        psc = s_ps[k];
+        lxc = s_lsx[k];
+        return true;
+      };
+      auto put = [&](uint32_t idx, const int32_t *c) { write_node<DIM>(octree, idx, c); };
+      miss = !gather_one<K, DIM>(q, nq[j], excl, lsq, links, child, keys, conflict_bits, mbits,
+                                 put, node_conflict, st);
+    }
+    defer(miss, q);
   }
-  if (active && staged) {
-    miss = !gather_one_lds<K, DIM>(q, nq, excl, lsq, w0, WN, s_lnk, s_ps, s_lsx, keys,
-                                   conflict_bits, mbits, s_nodes + (excl - lo) * STRIDE,
-                                   node_conflict, st);
-  } else if (active) {
-    auto links = [&](uint32_t x, uint32_t &left, bool &ll, bool &rl) -> bool {
-      const unsigned k = (unsigned)((int)x - w0);
-      if (k >= (unsigned)WN) return false;
-      const uint32_t v = s_lnk[k];
-      left = v & 0x3fffffffu;
-      ll = (v >> 30) & 1u;
-      rl = v >> 31;
-      return true;
-    };
-    auto child = [&](uint32_t c, uint32_t &psc, uint32_t &lxc) -> bool {
-      const unsigned k = (unsigned)((int)c - w0);
-      if (k >= (unsigned)WN) return false;
-      psc = s_ps[k];
-      lxc = s_lsx[k];
-      return true;
-    };
-    auto put = [&](uint32_t idx, const int32_t *c) { write_node<DIM>(octree, idx, c); };
-    miss = !gather_one<K, DIM>(q, nq, excl, lsq, links, child, keys, conflict_bits, mbits,
-                               put, node_conflict, st);
-  }
-  defer(miss, q);
   if (staged) {
     __syncthreads();
     const uint32_t words = span * STRIDE;
     int32_t *out = octree + (size_t)lo * STRIDE;
-    for (uint32_t w = threadIdx.x; w < words; w += kBlock) out[w] = s_nodes[w];
+    for (uint32_t w = t; w < words; w += kBlock) out[w] = s_nodes[w];
   }
 }
 
@@ -362,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(
     uint64_t *__restrict__ miss_words, RunState *st) {
   if (st->malformed) return;
   const uint32_t m = st->m;
-  const uint32_t q0 = xcd_segment(blockIdx.x, gridDim.x) * kBlock;
+  const uint32_t q0 = xcd_segment(blockIdx.x, gridDim.x) * (kGatherRows * kBlock);
   if ((int64_t)st->octree_size > capacity) {
     if (q0 == 0 && threadIdx.x == 0) atomicOr(&st->status, (uint32_t)ST_CAPACITY);
     return;
@@ -375,7 +418,8 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(
     return;
   }
   if (q0 >= m - 1) return;
-  gather_window<K, DIM>(brt, ps, lsx, chunk_excl, ps, keys, conflict_bits, mbits, m, q0, octree,
+  gather_window<K, DIM, kGatherRows>(brt, ps, lsx, chunk_excl, ps, keys, conflict_bits, mbits, m,
+                                     q0, octree,
                         node_conflict, st,
                         [&](bool miss, uint32_t q) { wave_mask(miss, miss_words, q >> 6); });
 }
@@ -405,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void gather_seg_kernel(
   }
   if (q0 >= m - 1) return;
   const uint32_t kofs = seg[s].kofs;
-  gather_window<K, DIM>(brt + kofs, ps + kofs, lsx + kofs, nullptr, nullptr,
+  gather_window<K, DIM, 1>(brt + kofs, ps + kofs, lsx + kofs, nullptr, nullptr,
                         (const K *)nullptr, nullptr, 0, m, q0,
                         oct, nullptr, st, [&](bool miss, uint32_t q) {
                           wave_append(miss, kofs + q, &st->counters[CTR_OCT_DEFER], defer);
@@ -580,7 +624,9 @@ void launch_dim(const BrtNodeT<K> *brt, const int32_t *parent, uint32_t *ps,
   const unsigned sgrid = grid < 1024 ? grid : 1024;
   if (!force_scatter) {
     uint64_t *words = reinterpret_cast<uint64_t *>(defer);
-    gather_kernel<K, DIM><<<grid, kBlock, 0, s>>>(brt, ps, lsx, chunk_excl, keys, conflict_bits,
+    const unsigned ggrid =
+        (unsigned)((nodes + kGatherRows * kBlock - 1) / (kGatherRows * kBlock));
+    gather_kernel<K, DIM><<<ggrid, kBlock, 0, s>>>(brt, ps, lsx, chunk_excl, keys, conflict_bits,
                                                   mbits, octree, node_conflict, capacity, words,
                                                   st);
     const int64_t g2 = ((nodes + 63) / 64 + kBlock - 1) / kBlock;
