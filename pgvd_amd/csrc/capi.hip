@@ -113,7 +113,7 @@ size_t pgvd_capi::plain_footprint_bytes(int64_t n, int dim, int bits, bool label
                               : (kb == 8 ? sizeof(BrtNodeT<uint64_t>) : sizeof(BrtNodeT<Key128>));
   const int64_t oct_nodes = std::max<int64_t>(2 * nk, 1024);
   size_t b = 0;
-  b += 2 * (size_t)nk * kb;                              // keys_a, keys_b
+  b += 2 * (size_t)nk * kb + 32;                         // keys_a, keys_b
   if (labels) b += 2 * (size_t)n * 4;                    // vals_a, vals_b
   b += (size_t)nk * node;                                // brt
   b += 3 * (size_t)nk * 4;                               // ps, parent, defer
@@ -187,10 +187,14 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   const int64_t nk = n + 1;
   const bool lab = labels != nullptr;
   void *p;
-  CHK(pool_get(ctx, "keys_a", (size_t)nk * kb, &p));
-  K *kA = static_cast<K *>(p);
-  CHK(pool_get(ctx, "keys_b", (size_t)nk * kb, &p));
-  K *kB = static_cast<K *>(p);
+  // Each key buffer keeps 16 bytes before its keys: the distinct keys are
+  // compacted in place from the sorted ones, one slot to the left when a
+  // zero key is prepended.
+  constexpr int64_t kPad = 16 / (int64_t)sizeof(K);
+  CHK(pool_get(ctx, "keys_a", (size_t)(nk + kPad) * kb, &p));
+  K *kA = static_cast<K *>(p) + kPad;
+  CHK(pool_get(ctx, "keys_b", (size_t)(nk + kPad) * kb, &p));
+  K *kB = static_cast<K *>(p) + kPad;
   uint32_t *vA = nullptr, *vB = nullptr;
   if (lab) {
     CHK(pool_get(ctx, "vals_a", (size_t)n * 4, &p));
@@ -261,8 +265,9 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   }
   CHK(launch_status("onesweep"));
   K *sorted = parity ? kB : kA;
-  K *uniq = parity ? kA : kB;
-  uint32_t *lsx = reinterpret_cast<uint32_t *>(sorted);
+  K *uniq = sorted - prepend;   // in place (see unique_tile)
+  // packed local splits go to the sort's other buffer, free after the sort
+  uint32_t *lsx = reinterpret_cast<uint32_t *>(parity ? kA : kB);
   const uint32_t *sortedv = lab ? (n <= 1 ? labels : (parity ? vB : vA)) : nullptr;
   {
     Stage t(ctx, "unique", s);
@@ -271,8 +276,7 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   CHK(launch_status("unique"));
   {
     Stage t(ctx, "brt", s);
-    // packed local splits go to the sort's spare key buffer (free after
-    // unique); their totals per chunk of 256 nodes to chunks
+    // local-split totals per chunk of 256 nodes go to chunks
     launch_brt_ls<K>(uniq, nk, mbits, dim, brt, parent, lsx, chunks, defer, ctx->st, s);
   }
   CHK(launch_status("brt"));

@@ -34,6 +34,11 @@ template <> __device__ inline Key128 shfl_up1<Key128>(Key128 v) {
 // holding base + 16t .. +16.  Writes [zero?] + the distinct keys to
 // kout[prepend + rank]; with labels, flags run r in conflict_bits when the
 // labels of its keys are not all equal.  *m_out (last tile only) = m.
+// kout may be kin - prepend (in place): the r-th distinct key then goes to
+// kin[r], never past its own input slot, and a tile writes only after its
+// look-back has resolved, i.e. after every earlier tile has loaded its
+// input.  A tile whose keys are all distinct and already in place (no
+// duplicate before it) writes nothing.
 template <typename K, bool LAB>
 __device__ inline void unique_tile(const K *__restrict__ kin, const uint32_t *__restrict__ lab,
                                    uint32_t n, uint32_t prepend, K *__restrict__ kout,
@@ -60,7 +65,11 @@ __device__ inline void unique_tile(const K *__restrict__ kin, const uint32_t *__
   const uint32_t texcl = block_excl_sum<uint32_t>((uint32_t)__popc(heads), s_w, &total);
   const uint64_t excl = tile_prefix(lookback, tile, total, lo);
   const uint32_t obase = (uint32_t)(prepend + excl + texcl);
-  if constexpr (sizeof(K) <= 8) {
+  const uint32_t tile_n = n > base ? min((uint32_t)kScanTile, n - base) : 0u;
+  const bool in_place = total == tile_n && kout + prepend + excl == kin + base;   // uniform
+  if (in_place) {
+    // every key of the tile is already where it goes
+  } else if constexpr (sizeof(K) <= 8) {
     // Compacted keys are staged in LDS (index r + r/16 against bank
     // conflicts) so that the tile's output is written in coalesced runs.
     __shared__ K s_out[kScanTile + kScanTile / 16];
