@@ -19,8 +19,23 @@ __global__ __launch_bounds__(kBlock) void copy_kernel(uint4 *__restrict__ dst,
                                                       uint8_t *__restrict__ dtail,
                                                       const uint8_t *__restrict__ stail,
                                                       size_t ntail) {
+  // four 16-byte words per thread and round, all loads before the stores
+  constexpr int U = 4;
   const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x;
-  for (size_t i = tid; i < n16; i += (size_t)gridDim.x * kBlock) dst[i] = src[i];
+  const size_t stride = (size_t)gridDim.x * kBlock * U;
+  for (size_t base = (size_t)blockIdx.x * kBlock * U + threadIdx.x; base < n16; base += stride) {
+    uint4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t i = base + (size_t)u * kBlock;
+      if (i < n16) v[u] = src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t i = base + (size_t)u * kBlock;
+      if (i < n16) dst[i] = v[u];
+    }
+  }
   if (tid < ntail) dtail[tid] = stail[tid];
 }
 
