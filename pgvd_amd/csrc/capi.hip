@@ -21,6 +21,8 @@
 
 using namespace pgvd;
 
+#include <type_traits>
+
 #include "capi_internal.hpp"
 
 using namespace pgvd_capi;
@@ -78,7 +80,7 @@ int launch_status(const char *what) {
 // Workspace that must be zero at the start of a run, as one block.
 struct ZeroLayout {
   size_t hist = 0, sort_lb = 0, lb_unique = 0, lb_splits = 0, lb_scan = 0, lb_brt = 0,
-         cbits = 0, total = 0;
+         cbits = 0, msd_hist = 0, total = 0;
 };
 
 ZeroLayout zero_layout(int64_t n, int kb, int passes, bool labels) {
@@ -99,6 +101,8 @@ ZeroLayout zero_layout(int64_t n, int kb, int passes, bool labels) {
   off += align256((size_t)scan_tiles(n + 1) * 8);
   z.cbits = off;
   if (labels) off += align256((size_t)((n + 1 + 31) / 32) * 4);
+  z.msd_hist = off;
+  off += align256((size_t)kMsdMaxBuckets * 4);
   z.total = off;
   return z;
 }
@@ -118,6 +122,7 @@ size_t pgvd_capi::plain_footprint_bytes(int64_t n, int dim, int bits, bool label
   b += (size_t)nk * node;                                // brt
   b += 3 * (size_t)nk * 4;                               // ps, parent, defer
   b += kBboxPartBytes;                                   // bbox_part
+  b += (2 * (size_t)kMsdMaxBuckets + 1) * 4;             // msd_start, msd_cursor
   b += (size_t)((nk + kBlock - 1) / kBlock) * 4;         // chunks
   b += zero_layout(n, kb, (mbits + 7) / 8, labels).total;
   b += (size_t)oct_nodes * ((1u << dim) + 1) * 4;        // octree
@@ -221,6 +226,8 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   uint64_t *lb_s = reinterpret_cast<uint64_t *>(zb + z.lb_splits);
   uint64_t *lb_b = reinterpret_cast<uint64_t *>(zb + z.lb_brt);
   uint32_t *cbits = lab ? reinterpret_cast<uint32_t *>(zb + z.cbits) : nullptr;
+  uint32_t *msd_hist = reinterpret_cast<uint32_t *>(zb + z.msd_hist);
+  int msd_b = 0;
   int32_t *octree;
   uint32_t *nconf;
   int64_t want = 2 * nk;
@@ -250,18 +257,40 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
     }
     const int64_t width = (int64_t)1 << bits;
     const float effw = (float)(clamped ? width : width - 1);
+    // 64-bit keys of a large build without labels: bucket sort, with the LSD
+    // passes launched beside it for inputs whose buckets overflow
+    if (std::is_same<K, uint64_t>::value && !lab &&
+        encode_vectorised(static_cast<const float *>(pts), kA))
+      msd_b = msd_bits_for(n);
     Stage t(ctx, "encode", s);
-    launch_encode<K>(static_cast<const float *>(pts), n, dim, bits, effw, custom ? 1 : 0,
-                     custom ? opts->bb_min : nullptr, custom ? opts->bb_max : nullptr, kA,
-                     nullptr, hist, passes, ctx->st, s);
+    if (msd_b)
+      launch_encode<K>(static_cast<const float *>(pts), n, dim, bits, effw, custom ? 1 : 0,
+                       custom ? opts->bb_min : nullptr, custom ? opts->bb_max : nullptr, kA,
+                       nullptr, hist, 0, ctx->st, s, msd_hist, mbits - msd_b, 1 << msd_b);
+    else
+      launch_encode<K>(static_cast<const float *>(pts), n, dim, bits, effw, custom ? 1 : 0,
+                       custom ? opts->bb_min : nullptr, custom ? opts->bb_max : nullptr, kA,
+                       nullptr, hist, passes, ctx->st, s);
   }
   CHK(launch_status("encode"));
   int parity;
+  if constexpr (std::is_same<K, uint64_t>::value) {
+    if (msd_b) {
+      // the result lands where the LSD passes would leave it
+      Stage t(ctx, "msd", s);
+      CHK(pool_get(ctx, "msd_start", ((size_t)kMsdMaxBuckets + 1) * 4, &p));
+      uint32_t *mstart = static_cast<uint32_t *>(p);
+      CHK(pool_get(ctx, "msd_cursor", (size_t)kMsdMaxBuckets * 4, &p));
+      launch_msd_sort(kA, kB, (passes & 1) ? kB : kA, n, mbits, msd_b, msd_hist, mstart,
+                      static_cast<uint32_t *>(p), ctx->st, s);
+      launch_histogram<K>(kA, n, hist, passes, s, ctx->st);
+    }
+  }
   {
     Stage span(ctx, "sort", s, 2);
     PassTimer pt(ctx, n > 1 ? passes : 0);
     parity = launch_onesweep<K>(kA, kB, labels, vA, vB, n, passes, hist, sort_lb, ctx->st, s,
-                                pt.events());
+                                pt.events(), msd_b ? ctx->st : nullptr);
   }
   CHK(launch_status("onesweep"));
   K *sorted = parity ? kB : kA;

@@ -40,6 +40,7 @@ struct RunState {
   uint32_t malformed;        // 1 if some BRT node took an FP32-quirk split
   uint32_t m;                // unique key count (incl. the padding zero)
   uint32_t octree_size;      // O = ps[m-1]
+  uint32_t msd_ok;           // 1: the bucket sort (msd.hip) sorted the keys
   uint32_t counters[kNumCounters];
 };
 
@@ -99,17 +100,24 @@ void launch_bbox(const float *pts, int64_t n, int dim, float *partials, RunState
                  hipStream_t s);
 // Quantize + Morton encode (+ optional digit histograms for `passes` passes).
 // bb: if custom, (min, max) given; else taken from st (launch_bbox).
+// msd_hist (nullable): also the histogram of key >> msd_shift over
+// msd_nb buckets (the bucket sort's), in the vectorised form only.
 template <typename K>
 void launch_encode(const float *pts, int64_t n, int dim, int bits, float effw,
                    int custom_bb, const float *bbmin, const float *bbmax,
                    K *keys, int64_t *qout, uint32_t *hist, int passes, RunState *st,
-                   hipStream_t s);
+                   hipStream_t s, uint32_t *msd_hist = nullptr, int msd_shift = 0,
+                   int msd_nb = 0);
+// true when launch_encode takes the vectorised form (and so fills msd_hist)
+bool encode_vectorised(const float *pts, const void *keys);
 // Morton encode from integer points (int32 or int64 coordinates).
 template <typename K>
 void launch_points_to_morton(const void *pts, int coord_bytes, int64_t n, int64_t padded_n,
                              int dim, int bits, K *keys, hipStream_t s);
+// skip_if_msd (nullable): does nothing when skip_if_msd->msd_ok.
 template <typename K>
-void launch_histogram(const K *keys, int64_t n, uint32_t *hist, int passes, hipStream_t s);
+void launch_histogram(const K *keys, int64_t n, uint32_t *hist, int passes, hipStream_t s,
+                      const RunState *skip_if_msd = nullptr);
 // Morton encode + digit histograms from int32 grid points (no padding).
 template <typename K>
 void launch_encode_int(const int32_t *pts, int64_t n, int dim, int bits, K *keys,
@@ -148,11 +156,28 @@ void launch_digest_count(const void *p, const uint32_t *count, uint32_t per_item
 // Pass 0 reads payload from vals_in (may equal vals); later passes ping-pong
 // vals/vals_alt in step with the keys.
 // pass_events (nullable): 2*passes events recorded around each pass.
+// skip_if_msd (nullable): the passes do nothing when skip_if_msd->msd_ok.
 template <typename K>
 int launch_onesweep(K *keys, K *keys_alt, const uint32_t *vals_in, uint32_t *vals,
                     uint32_t *vals_alt, int64_t n, int passes, const uint32_t *hist,
                     uint32_t *lookback, RunState *st, hipStream_t s,
-                    hipEvent_t *pass_events = nullptr);
+                    hipEvent_t *pass_events = nullptr, const RunState *skip_if_msd = nullptr);
+
+// ---- msd.hip ----
+// Bucket sort of 64-bit keys without payload: scatter by the top b bits,
+// then each bucket sorted in one workgroup's LDS.  Active only when every
+// bucket holds at most kMsdCap keys (then st->msd_ok = 1); the caller runs
+// the LSD passes beside it with skip_if_msd = st.
+constexpr int kMsdCap = 6144;
+constexpr int kMsdMaxBuckets = 4096;
+// b for n keys (0: no bucket sort for this size).
+int msd_bits_for(int64_t n);
+// keys -> alt (scatter) -> out (buckets sorted; out may be alt).  hist: the
+// bucket histogram (2^b counters, from launch_encode); start 2^b + 1 and
+// cursor 2^b words of scratch.
+void launch_msd_sort(uint64_t *keys, uint64_t *alt, uint64_t *out, int64_t n, int mbits, int b,
+                     const uint32_t *hist, uint32_t *start, uint32_t *cursor, RunState *st,
+                     hipStream_t s);
 int64_t onesweep_tiles(int key_bytes, bool payload, int64_t n);
 int onesweep_tile_size(int key_bytes, bool payload);
 

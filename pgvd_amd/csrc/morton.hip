@@ -25,7 +25,7 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(RunState *st, uint4 *zero,
   const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid == 0) {
     for (int k = 0; k < 3; ++k) { st->bb_min[k] = 0xFFFFFFFFu; st->bb_max[k] = 0u; }
-    st->status = 0; st->malformed = 0; st->m = 0; st->octree_size = 0;
+    st->status = 0; st->malformed = 0; st->m = 0; st->octree_size = 0; st->msd_ok = 0;
     for (int k = 0; k < kNumCounters; ++k) st->counters[k] = 0;
   }
   const uint4 z = make_uint4(0, 0, 0, 0);
@@ -279,9 +279,14 @@ template <typename K, int DIM>
 __global__ __launch_bounds__(kBlock) void encode_vec_kernel(
     const float *__restrict__ pts, int64_t n, int bits, float effw, int custom, float bmin0,
     float bmin1, float bmin2, float bmax0, float bmax1, float bmax2, K *__restrict__ keys,
-    uint32_t *__restrict__ hist, int passes, RunState *st) {
+    uint32_t *__restrict__ hist, int passes, RunState *st, uint32_t *__restrict__ msd_hist,
+    int msd_shift, int msd_nb) {
   __shared__ uint32_t sh[kMaxPasses * 256];
+  __shared__ uint32_t s_msd[kMsdMaxBuckets];
   for (int i = threadIdx.x; i < passes * 256; i += kBlock) sh[i] = 0;
+  if (msd_hist)
+    for (int i = threadIdx.x; i < msd_nb; i += kBlock) s_msd[i] = 0;
+  const uint32_t msd_mask = (uint32_t)msd_nb - 1;
   float mn[3], mx[3];
   if (custom) {
     mn[0] = bmin0; mn[1] = bmin1; mn[2] = bmin2;
@@ -308,6 +313,7 @@ __global__ __launch_bounds__(kBlock) void encode_vec_kernel(
     morton128<DIM, (sizeof(K) > 8)>(c, bits, lo, hi);
     const K key = from128<K>(lo, hi);
     for (int r = 0; r < passes; ++r) atomicAdd(&sh[r * 256 + kdigit(key, 8 * r)], 1u);
+    if (msd_hist) atomicAdd(&s_msd[klow32(kshr(key, msd_shift)) & msd_mask], 1u);
     return key;
   };
   for (int64_t ch = tid; ch < nch; ch += stride) {
@@ -323,6 +329,9 @@ __global__ __launch_bounds__(kBlock) void encode_vec_kernel(
   __syncthreads();
   for (int i = threadIdx.x; i < passes * 256; i += kBlock)
     if (sh[i]) atomicAdd(&hist[i], sh[i]);
+  if (msd_hist)
+    for (int i = threadIdx.x; i < msd_nb; i += kBlock)
+      if (s_msd[i]) atomicAdd(&msd_hist[i], s_msd[i]);
 }
 
 template <typename K, int DIM, typename C>
@@ -372,7 +381,8 @@ __global__ __launch_bounds__(kBlock) void encode_int_kernel(const int32_t *__res
 template <typename K>
 __global__ __launch_bounds__(kBlock) void histogram_kernel(const K *__restrict__ keys, int64_t n,
                                                            uint32_t *__restrict__ hist,
-                                                           int passes) {
+                                                           int passes, const RunState *skip) {
+  if (skip && skip->msd_ok) return;
   __shared__ uint32_t sh[kMaxPasses * 256];
   for (int i = threadIdx.x; i < passes * 256; i += kBlock) sh[i] = 0;
   __syncthreads();
@@ -526,24 +536,30 @@ void launch_bbox(const float *pts, int64_t n, int dim, float *partials, RunState
   else bbox_kernel<3><<<grid, kBlock, 0, s>>>(pts, n, st);
 }
 
+bool encode_vectorised(const float *pts, const void *keys) {
+  return aligned16(pts) && aligned16(keys);
+}
+
 template <typename K>
 void launch_encode(const float *pts, int64_t n, int dim, int bits, float effw, int custom_bb,
                    const float *bbmin, const float *bbmax, K *keys, int64_t *qout,
-                   uint32_t *hist, int passes, RunState *st, hipStream_t s) {
+                   uint32_t *hist, int passes, RunState *st, hipStream_t s, uint32_t *msd_hist,
+                   int msd_shift, int msd_nb) {
   const int grid = grid_for(n, kBlock * 4);
   float a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
   if (custom_bb)
     for (int k = 0; k < dim; ++k) { a[k] = bbmin[k]; b[k] = bbmax[k]; }
-  if (keys && hist && !qout && aligned16(pts) && aligned16(keys)) {
+  if (keys && hist && !qout && encode_vectorised(pts, keys)) {
     const int vgrid = grid_for(n / 4 + 1, kBlock);
+    if (msd_nb > kMsdMaxBuckets) msd_hist = nullptr;
     if (dim == 2)
       encode_vec_kernel<K, 2><<<vgrid, kBlock, 0, s>>>(pts, n, bits, effw, custom_bb, a[0], a[1],
                                                        a[2], b[0], b[1], b[2], keys, hist, passes,
-                                                       st);
+                                                       st, msd_hist, msd_shift, msd_nb);
     else
       encode_vec_kernel<K, 3><<<vgrid, kBlock, 0, s>>>(pts, n, bits, effw, custom_bb, a[0], a[1],
                                                        a[2], b[0], b[1], b[2], keys, hist, passes,
-                                                       st);
+                                                       st, msd_hist, msd_shift, msd_nb);
     return;
   }
   if (dim == 2)
@@ -570,9 +586,10 @@ void launch_points_to_morton(const void *pts, int coord_bytes, int64_t n, int64_
 }
 
 template <typename K>
-void launch_histogram(const K *keys, int64_t n, uint32_t *hist, int passes, hipStream_t s) {
+void launch_histogram(const K *keys, int64_t n, uint32_t *hist, int passes, hipStream_t s,
+                      const RunState *skip_if_msd) {
   const int grid = grid_for(n, kBlock * 8);
-  histogram_kernel<K><<<grid, kBlock, 0, s>>>(keys, n, hist, passes);
+  histogram_kernel<K><<<grid, kBlock, 0, s>>>(keys, n, hist, passes, skip_if_msd);
 }
 
 template <typename K>
@@ -612,10 +629,11 @@ void launch_encode_seg(const void *pts, int quantized, int dim, int bits, float 
 #define PGVD_INST(K)                                                                          \
   template void launch_encode<K>(const float *, int64_t, int, int, float, int, const float *,  \
                                  const float *, K *, int64_t *, uint32_t *, int, RunState *,   \
-                                 hipStream_t);                                                 \
+                                 hipStream_t, uint32_t *, int, int);                           \
   template void launch_points_to_morton<K>(const void *, int, int64_t, int64_t, int, int, K *, \
                                            hipStream_t);                                       \
-  template void launch_histogram<K>(const K *, int64_t, uint32_t *, int, hipStream_t);         \
+  template void launch_histogram<K>(const K *, int64_t, uint32_t *, int, hipStream_t,          \
+                                    const RunState *);                                         \
   template void launch_encode_int<K>(const int32_t *, int64_t, int, int, K *, uint32_t *, int,  \
                                      hipStream_t);                                             \
   template void launch_encode_seg<K>(const void *, int, int, int, float, const BatchDev &, K *, \

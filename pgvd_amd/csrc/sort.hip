@@ -233,7 +233,8 @@ template <typename K, bool PAY, int IPT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PGVD_SORT_WAVES))) void onesweep_kernel(
     const K *__restrict__ kin, K *__restrict__ kout, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ vout, uint32_t n, int shift, const uint32_t *__restrict__ ghist,
-    uint32_t *__restrict__ lookback, uint32_t *tile_counter) {
+    uint32_t *__restrict__ lookback, uint32_t *tile_counter, const RunState *skip) {
+  if (skip && skip->msd_ok) return;   // uniform: the bucket sort has sorted the keys
   constexpr uint32_t TILE = kBlock * IPT;
   const uint32_t tile = claim_tile(tile_counter);
   const uint32_t base = tile * TILE;
@@ -288,7 +289,8 @@ int64_t onesweep_tiles(int key_bytes, bool payload, int64_t n) {
 template <typename K>
 int launch_onesweep(K *keys, K *keys_alt, const uint32_t *vals_in, uint32_t *vals,
                     uint32_t *vals_alt, int64_t n, int passes, const uint32_t *hist,
-                    uint32_t *lookback, RunState *st, hipStream_t s, hipEvent_t *pass_events) {
+                    uint32_t *lookback, RunState *st, hipStream_t s, hipEvent_t *pass_events,
+                    const RunState *skip_if_msd) {
   if (n <= 1 || passes <= 0) return 0;
   const bool pay = vals_in != nullptr;
   const int64_t tiles = onesweep_tiles((int)sizeof(K), pay, n);
@@ -302,11 +304,11 @@ int launch_onesweep(K *keys, K *keys_alt, const uint32_t *vals_in, uint32_t *val
     if (pay) {
       constexpr int IPT = SortCfg<K, true>::IPT;
       onesweep_kernel<K, true, IPT><<<(unsigned)tiles, kBlock, 0, s>>>(
-          src, dst, vs, vd, (uint32_t)n, 8 * r, hist + 256 * r, lb, ctr);
+          src, dst, vs, vd, (uint32_t)n, 8 * r, hist + 256 * r, lb, ctr, skip_if_msd);
     } else {
       constexpr int IPT = SortCfg<K, false>::IPT;
       onesweep_kernel<K, false, IPT><<<(unsigned)tiles, kBlock, 0, s>>>(
-          src, dst, nullptr, nullptr, (uint32_t)n, 8 * r, hist + 256 * r, lb, ctr);
+          src, dst, nullptr, nullptr, (uint32_t)n, 8 * r, hist + 256 * r, lb, ctr, skip_if_msd);
     }
     if (pass_events) (void)hipEventRecord(pass_events[2 * r + 1], s);
     K *tk = src; src = dst; dst = tk;
@@ -357,7 +359,7 @@ int launch_onesweep_seg(K *keys, K *keys_alt, const BatchDev &b, int passes,
 #define PGVD_INST(K)                                                                          \
   template int launch_onesweep<K>(K *, K *, const uint32_t *, uint32_t *, uint32_t *, int64_t,  \
                                   int, const uint32_t *, uint32_t *, RunState *, hipStream_t,   \
-                                  hipEvent_t *);                                              \
+                                  hipEvent_t *, const RunState *);                            \
   template int launch_onesweep_seg<K>(K *, K *, const BatchDev &, int, const uint32_t *,       \
                                       uint32_t *, RunState *, hipStream_t);
 PGVD_INST(uint32_t)
