@@ -122,7 +122,7 @@ size_t pgvd_capi::plain_footprint_bytes(int64_t n, int dim, int bits, bool label
   b += (size_t)nk * node;                                // brt
   b += 3 * (size_t)nk * 4;                               // ps, parent, defer
   b += kBboxPartBytes;                                   // bbox_part
-  b += (2 * (size_t)kMsdMaxBuckets + 1) * 4;             // msd_start, msd_cursor
+  b += ((size_t)kMsdMaxBuckets + 1 + kMsdCursorWords) * 4;   // msd_start, msd_cursor
   b += (size_t)((nk + kBlock - 1) / kBlock) * 4;         // chunks
   b += zero_layout(n, kb, (mbits + 7) / 8, labels).total;
   b += (size_t)oct_nodes * ((1u << dim) + 1) * 4;        // octree
@@ -280,7 +280,7 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
       Stage t(ctx, "msd", s);
       CHK(pool_get(ctx, "msd_start", ((size_t)kMsdMaxBuckets + 1) * 4, &p));
       uint32_t *mstart = static_cast<uint32_t *>(p);
-      CHK(pool_get(ctx, "msd_cursor", (size_t)kMsdMaxBuckets * 4, &p));
+      CHK(pool_get(ctx, "msd_cursor", (size_t)kMsdCursorWords * 4, &p));
       launch_msd_sort(kA, kB, (passes & 1) ? kB : kA, n, mbits, msd_b, msd_hist, mstart,
                       static_cast<uint32_t *>(p), ctx->st, s);
       launch_histogram<K>(kA, n, hist, passes, s, ctx->st);

@@ -168,13 +168,14 @@ int launch_onesweep(K *keys, K *keys_alt, const uint32_t *vals_in, uint32_t *val
 // then each bucket sorted in one workgroup's LDS.  Active only when every
 // bucket holds at most kMsdCap keys (then st->msd_ok = 1); the caller runs
 // the LSD passes beside it with skip_if_msd = st.
-constexpr int kMsdCap = 6144;
+constexpr int kMsdCap = 5120;
 constexpr int kMsdMaxBuckets = 4096;
+constexpr int kMsdCursorWords = kMsdMaxBuckets + 2 * 64 + 1;
 // b for n keys (0: no bucket sort for this size).
 int msd_bits_for(int64_t n);
 // keys -> alt (scatter) -> out (buckets sorted; out may be alt).  hist: the
 // bucket histogram (2^b counters, from launch_encode); start 2^b + 1 and
-// cursor 2^b words of scratch.
+// cursor kMsdCursorWords words of scratch.
 void launch_msd_sort(uint64_t *keys, uint64_t *alt, uint64_t *out, int64_t n, int mbits, int b,
                      const uint32_t *hist, uint32_t *start, uint32_t *cursor, RunState *st,
                      hipStream_t s);

@@ -1,16 +1,17 @@
-// msd.hip -- bucket sort for 64-bit keys of large single builds: one scatter
-// by the top b bits (b = 10..12) into 2^b buckets, then every bucket sorted
-// on its remaining low bits inside one workgroup's LDS.
+// msd.hip -- bucket sort for 64-bit keys of large single builds: the keys
+// are split by their top b bits (b = 10..12) into 2^b buckets in two
+// scatter passes of b/2 and b - b/2 bits, then every bucket is sorted on its
+// remaining low bits inside one workgroup's LDS.
 //
 // The output is the sorted key sequence, the same one the reference's LSD
 // split sort (RadixSortBigUnsigned, opencl/Kernels.cpp:219-247) produces:
 // keys without payload, so equal keys are indistinguishable and the order of
-// the scatter within a bucket does not matter.  Every kernel here does
+// the scatters within a bucket does not matter.  Every kernel here does
 // nothing unless st->msd_ok, which the plan kernel sets when every bucket
 // holds at most kMsdCap keys; otherwise the pipeline's LSD passes, launched
 // beside these and doing nothing when st->msd_ok, sort instead.  For
 // uniform-ish keys a bucket is ~4K keys: the LSD passes over HBM become
-// seven passes inside LDS.
+// passes inside LDS.
 #include "kernels.hpp"
 #include "tilescan.hpp"
 
@@ -18,21 +19,31 @@ namespace pgvd {
 
 namespace {
 
-constexpr int kMsdIpt = 24;                   // keys per thread of a bucket sort
-constexpr int kMsdScatterIpt = 16;            // keys per thread of the scatter
-constexpr int kMsdScatterTile = kBlock * kMsdScatterIpt;
-static_assert(kMsdCap == kBlock * kMsdIpt, "bucket capacity is one workgroup's keys");
+constexpr int kSplitIpt = 16;                  // keys per thread of a scatter tile
+constexpr int kSplitTile = kBlock * kSplitIpt; // 4096
+constexpr int kSplitMaxDigits = 64;            // 2^(b/2) and 2^(b - b/2) for b <= 12
+static_assert(kMsdCap == kBlock * 20, "a bucket sort holds 20 keys per thread at most");
 
-// Exclusive scan of the bucket histogram into start[0..nb] and the scatter
-// cursors; msd_ok = 1 when every bucket fits one workgroup.  One workgroup;
-// nb is a power of two in [256, 4096].
+// Words of msd_cursor: cursor[0..2^b) for the second scatter, then the
+// first scatter's 2^(b/2) cursors, then the second scatter's 2^(b/2) + 1
+// tile starts.
+constexpr int kCursor1 = kMsdMaxBuckets;
+constexpr int kTileStart = kMsdMaxBuckets + kSplitMaxDigits;
+static_assert(kTileStart + kSplitMaxDigits + 1 <= kMsdCursorWords, "msd_cursor holds them");
+
+// Plan, one workgroup: exclusive scan of the bucket histogram into
+// start[0..nb] and the second scatter's cursors; the first scatter's
+// cursors (the start of every group of nb2 buckets) and, per group, the
+// start of its tiles in the second scatter; msd_ok = 1 when every bucket
+// fits one workgroup.  nb is a power of two in [1024, 4096].
 __global__ __launch_bounds__(kBlock) void msd_plan_kernel(const uint32_t *__restrict__ hist,
-                                                          int nb, uint32_t *__restrict__ start,
+                                                          int nb, int nb2,
+                                                          uint32_t *__restrict__ start,
                                                           uint32_t *__restrict__ cursor,
                                                           RunState *st) {
   __shared__ uint32_t s_w[4], s_mx[4];
   const int t = threadIdx.x;
-  const int per = nb / kBlock;                  // 1..16 consecutive buckets per thread
+  const int per = nb / kBlock;                  // 4..16 consecutive buckets per thread
   uint32_t v[16];
   uint32_t run = 0, mx = 0;
 #pragma unroll
@@ -54,80 +65,126 @@ __global__ __launch_bounds__(kBlock) void msd_plan_kernel(const uint32_t *__rest
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
   if ((t & 63) == 0) s_mx[t >> 6] = mx;
-  __syncthreads();
+  __syncthreads();   // start[] is complete
   if (t == 0) {
     const uint32_t m4 = max(max(s_mx[0], s_mx[1]), max(s_mx[2], s_mx[3]));
     st->msd_ok = m4 <= (uint32_t)kMsdCap ? 1u : 0u;
+    const int nb1 = nb / nb2;
+    uint32_t tiles = 0;
+    for (int g = 0; g < nb1; ++g) {
+      const uint32_t lo = start[g * nb2], hi = start[(g + 1) * nb2];
+      cursor[kCursor1 + g] = lo;
+      cursor[kTileStart + g] = tiles;
+      tiles += (hi - lo + kSplitTile - 1) / kSplitTile;
+    }
+    cursor[kTileStart + nb1] = tiles;
   }
 }
 
-// Scatter of 4096 keys per workgroup to their buckets: a key's slot within
-// its tile and bucket comes from an LDS counter, the tile's run in each
-// bucket from one atomic on that bucket's cursor.
-__global__ __launch_bounds__(kBlock) void msd_scatter_kernel(const uint64_t *__restrict__ in,
-                                                             uint64_t *__restrict__ out,
-                                                             uint32_t n, int shift, int nb,
-                                                             uint32_t *__restrict__ cursor,
-                                                             const RunState *st) {
-  if (!st->msd_ok) return;
-  __shared__ uint32_t s_cnt[kMsdMaxBuckets], s_base[kMsdMaxBuckets];
+// One scatter tile: keys in[lo .. hi) (0 < hi - lo <= 4096) go to out in
+// runs by digit ld = (key >> dshift) & (nd - 1), the run of digit ld
+// starting at an atomic claim on gcursor[gbase + ld].  Keys are staged in
+// LDS in digit order so that each run is written by consecutive lanes.
+__device__ inline void split_tile(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                                  uint32_t n, uint32_t lo, uint32_t hi, int dshift, int nd,
+                                  uint32_t *__restrict__ gcursor, uint32_t gbase) {
+  __shared__ uint64_t s_keys[kSplitTile];
+  __shared__ uint32_t s_cnt[kSplitMaxDigits], s_lstart[kSplitMaxDigits],
+      s_gstart[kSplitMaxDigits];
   const int t = threadIdx.x;
-  for (int i = t; i < nb; i += kBlock) s_cnt[i] = 0;
-  __syncthreads();
-  const uint32_t base = blockIdx.x * (uint32_t)kMsdScatterTile;
-  const uint32_t dmask = (uint32_t)nb - 1;
-  uint64_t k[kMsdScatterIpt];
-  uint32_t dr[kMsdScatterIpt];
+  const uint32_t dmask = (uint32_t)nd - 1;
+  if (t < nd) s_cnt[t] = 0;
+  uint64_t k[kSplitIpt];
+  uint32_t dr[kSplitIpt];
 #pragma unroll
-  for (int j = 0; j < kMsdScatterIpt; ++j) {
-    const uint32_t i = base + j * kBlock + t;
-    k[j] = in[i < n ? i : 0u];
+  for (int j = 0; j < kSplitIpt; ++j) {
+    const uint32_t i = lo + j * kBlock + t;
+    k[j] = in[i < hi ? i : lo];
   }
+  __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kMsdScatterIpt; ++j) {
-    const uint32_t i = base + j * kBlock + t;
+  for (int j = 0; j < kSplitIpt; ++j) {
+    const uint32_t i = lo + j * kBlock + t;
     dr[j] = 0;
-    if (i < n) {
-      const uint32_t d = (uint32_t)(k[j] >> shift) & dmask;
+    if (i < hi) {
+      const uint32_t d = (uint32_t)(k[j] >> dshift) & dmask;
       dr[j] = atomicAdd(&s_cnt[d], 1u) | (d << 16);
     }
   }
   __syncthreads();
-  for (int d = t; d < nb; d += kBlock) {
-    const uint32_t c = s_cnt[d];
-    if (c) s_base[d] = atomicAdd(&cursor[d], c);
+  if (t < 64) {   // wave 0: digit starts in the tile and the runs' claims
+    const uint32_t c = t < nd ? s_cnt[t] : 0u;
+    const uint32_t incl = wave_incl_scan_add(c);
+    if (t < nd) {
+      s_lstart[t] = incl - c;
+      s_gstart[t] = c ? atomicAdd(&gcursor[gbase + t], c) : 0u;
+    }
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kMsdScatterIpt; ++j) {
-    const uint32_t i = base + j * kBlock + t;
-    if (i < n) {
-      const uint32_t pos = s_base[dr[j] >> 16] + (dr[j] & 0xffffu);
-      if (pos < n) out[pos] = k[j];   // always true when the histogram matches the keys
-    }
+  for (int j = 0; j < kSplitIpt; ++j) {
+    const uint32_t i = lo + j * kBlock + t;
+    if (i < hi) s_keys[s_lstart[dr[j] >> 16] + (dr[j] & 0xffffu)] = k[j];
+  }
+  __syncthreads();
+  const uint32_t cnt = hi - lo;
+  for (uint32_t p = t; p < cnt; p += kBlock) {
+    const uint64_t key = s_keys[p];
+    const uint32_t d = (uint32_t)(key >> dshift) & dmask;
+    const uint32_t pos = s_gstart[d] + (p - s_lstart[d]);
+    if (pos < n) out[pos] = key;   // always true when the histogram matches the keys
   }
 }
 
-// Sorts bucket b (keys in[start[b] .. start[b+1])) on bits [0, 8 passes) in
-// LDS and writes it to out at the same place.  The workgroup holds exactly
-// kMsdCap keys: slots past the bucket hold ~0, which stays behind every real
-// key (a larger or equal digit in every pass, and later in the stable
-// order).  Key index le = 64 IPT w + 64 j + lane lives in wave w, register
-// j; each pass ranks by the same ballot matching as the onesweep tile, so it
-// is stable.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void msd_bucket_kernel(const uint64_t *__restrict__ in,
+// First scatter: contiguous tiles of the whole array, by the top b1 bits.
+__global__ __launch_bounds__(kBlock) void msd_split1_kernel(const uint64_t *__restrict__ in,
                                                             uint64_t *__restrict__ out,
-                                                            const uint32_t *__restrict__ start,
-                                                            int passes, const RunState *st) {
+                                                            uint32_t n, int shift1, int nb1,
+                                                            uint32_t *__restrict__ cursor,
+                                                            const RunState *st) {
   if (!st->msd_ok) return;
-  const uint32_t s0 = start[blockIdx.x];
-  const uint32_t cnt = min(start[blockIdx.x + 1] - s0, (uint32_t)kMsdCap);
-  if (cnt == 0) return;
-  constexpr int IPT = kMsdIpt;
-  __shared__ uint64_t s_keys[kMsdCap];
-  __shared__ uint32_t s_cnt[4 * 256];
-  __shared__ uint32_t s_dig[256];
-  __shared__ uint32_t s_w[4];
+  const uint32_t lo = blockIdx.x * (uint32_t)kSplitTile;
+  const uint32_t hi = min(lo + (uint32_t)kSplitTile, n);
+  split_tile(in, out, n, lo, hi, shift1, nb1, cursor + kCursor1, 0u);
+}
+
+// Second scatter: tiles within each first-level group g (keys
+// [start[g nb2], start[(g+1) nb2])), by the next b2 bits into the group's
+// nb2 buckets.
+__global__ __launch_bounds__(kBlock) void msd_split2_kernel(const uint64_t *__restrict__ in,
+                                                            uint64_t *__restrict__ out,
+                                                            uint32_t n, int shift, int nb2,
+                                                            int nb1,
+                                                            const uint32_t *__restrict__ start,
+                                                            uint32_t *__restrict__ cursor,
+                                                            const RunState *st) {
+  if (!st->msd_ok) return;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t *ts = cursor + kTileStart;
+  if (tile >= ts[nb1]) return;
+  int g = 0;   // the group holding this tile: ts[g] <= tile < ts[g + 1]
+  for (int step = kSplitMaxDigits / 2; step >= 1; step >>= 1)
+    if (g + step < nb1 && ts[g + step] <= tile) g += step;
+  const uint32_t glo = start[g * nb2], ghi = start[(g + 1) * nb2];
+  const uint32_t lo = glo + (tile - ts[g]) * (uint32_t)kSplitTile;
+  const uint32_t hi = min(lo + (uint32_t)kSplitTile, ghi);
+  split_tile(in, out, n, lo, hi, shift, nb2, cursor, (uint32_t)(g * nb2));
+}
+
+// Sorts cnt <= 256 IPT keys of one bucket, in[s0 ..), on bits [0, 8 passes)
+// in LDS and writes them to out at the same place.  The workgroup holds 256
+// IPT keys: slots past the bucket hold ~0, which stays behind every real key
+// (a larger or equal digit in every pass, and later in the stable order).
+// Key index le = 64 IPT w + 64 j + lane lives in wave w, register j; each
+// pass ranks by the ballot matching of the onesweep tile, so it is stable.
+// The last pass ranks by its low last_bits digit bits only: above them
+// every real key has its bucket's bits, and ~0 falls in the top group,
+// behind the real keys there.
+template <int IPT>
+__device__ inline void bucket_sort(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                                   uint32_t s0, uint32_t cnt, int passes, int last_bits,
+                                   uint64_t *s_keys, uint32_t *s_cnt, uint32_t *s_dig,
+                                   uint32_t *s_w) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const uint32_t wofs = (uint32_t)w * 64u * IPT;
   uint64_t k[IPT];
@@ -138,24 +195,29 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   }
   for (int p = 0; p < passes; ++p) {
     const int sh = 8 * p;
+    const int nbits = p == passes - 1 ? last_bits : 8;   // uniform
+    const uint32_t dmask = (1u << nbits) - 1;
     for (int i = t; i < 4 * 256; i += kBlock) s_cnt[i] = 0;
     __syncthreads();
     uint32_t rk[IPT];   // rank within (wave, digit) | digit << 16
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
-      const uint32_t d = (uint32_t)(k[j] >> sh) & 0xffu;
+      const uint32_t d = (uint32_t)(k[j] >> sh) & dmask;
       uint32_t plo = ~0u, phi = ~0u;
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
-        const int32_t sb = __builtin_amdgcn_sbfe((int32_t)d, b, 1);
-        const uint64_t bb = __ballot(sb < 0);
-        plo = __builtin_amdgcn_bitop3_b32(plo, (uint32_t)bb, (uint32_t)sb, 0x90);
-        phi = __builtin_amdgcn_bitop3_b32(phi, (uint32_t)(bb >> 32), (uint32_t)sb, 0x90);
+        if (b < nbits) {
+          const int32_t sb = __builtin_amdgcn_sbfe((int32_t)d, b, 1);
+          const uint64_t bb = __ballot(sb < 0);
+          plo = __builtin_amdgcn_bitop3_b32(plo, (uint32_t)bb, (uint32_t)sb, 0x90);
+          phi = __builtin_amdgcn_bitop3_b32(phi, (uint32_t)(bb >> 32), (uint32_t)sb, 0x90);
+        }
       }
       const uint32_t c = s_cnt[w * 256 + d];
       const uint32_t below = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
       rk[j] = (c + below) | (d << 16);
-      if (below == 0) s_cnt[w * 256 + d] = c + (uint32_t)(__popc(plo) + __popc(phi));
+      // every lane of the group stores the same new count
+      s_cnt[w * 256 + d] = c + (uint32_t)(__popc(plo) + __popc(phi));
     }
     __syncthreads();
     // thread t owns digit t: wave offsets, then the digit's start in the tile
@@ -186,6 +248,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   }
 }
 
+// One workgroup per bucket; the tile is the smallest of 12, 16 or 20 keys
+// per thread that holds the bucket.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void
+msd_bucket_kernel(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                  const uint32_t *__restrict__ start, int passes, int last_bits,
+                  const RunState *st) {
+  if (!st->msd_ok) return;
+  const uint32_t s0 = start[blockIdx.x];
+  const uint32_t cnt = min(start[blockIdx.x + 1] - s0, (uint32_t)kMsdCap);
+  if (cnt == 0) return;
+  __shared__ uint64_t s_keys[kMsdCap];
+  __shared__ uint32_t s_cnt[4 * 256];
+  __shared__ uint32_t s_dig[256];
+  __shared__ uint32_t s_w[4];
+  if (cnt <= 12 * kBlock)
+    bucket_sort<12>(in, out, s0, cnt, passes, last_bits, s_keys, s_cnt, s_dig, s_w);
+  else if (cnt <= 16 * kBlock)
+    bucket_sort<16>(in, out, s0, cnt, passes, last_bits, s_keys, s_cnt, s_dig, s_w);
+  else
+    bucket_sort<20>(in, out, s0, cnt, passes, last_bits, s_keys, s_cnt, s_dig, s_w);
+}
+
 }  // namespace
 
 int msd_bits_for(int64_t n) {
@@ -198,12 +282,19 @@ int msd_bits_for(int64_t n) {
 void launch_msd_sort(uint64_t *keys, uint64_t *alt, uint64_t *out, int64_t n, int mbits, int b,
                      const uint32_t *hist, uint32_t *start, uint32_t *cursor, RunState *st,
                      hipStream_t s) {
-  const int nb = 1 << b;
-  const int shift = mbits - b;
-  msd_plan_kernel<<<1, kBlock, 0, s>>>(hist, nb, start, cursor, st);
-  const unsigned tiles = (unsigned)((n + kMsdScatterTile - 1) / kMsdScatterTile);
-  msd_scatter_kernel<<<tiles, kBlock, 0, s>>>(keys, alt, (uint32_t)n, shift, nb, cursor, st);
-  msd_bucket_kernel<<<(unsigned)nb, kBlock, 0, s>>>(alt, out, start, (shift + 7) / 8, st);
+  const int nb = 1 << b, b1 = b / 2, b2 = b - b1;
+  const int nb1 = 1 << b1, nb2 = 1 << b2;
+  const int shift = mbits - b;             // bits each bucket is sorted on
+  msd_plan_kernel<<<1, kBlock, 0, s>>>(hist, nb, nb2, start, cursor, st);
+  const unsigned tiles1 = (unsigned)((n + kSplitTile - 1) / kSplitTile);
+  msd_split1_kernel<<<tiles1, kBlock, 0, s>>>(keys, alt, (uint32_t)n, mbits - b1, nb1, cursor,
+                                              st);
+  // the groups' tiles: the sum of ceil(len / tile) is at most tiles1 + nb1
+  msd_split2_kernel<<<tiles1 + (unsigned)nb1, kBlock, 0, s>>>(alt, keys, (uint32_t)n, shift, nb2,
+                                                              nb1, start, cursor, st);
+  const int passes = (shift + 7) / 8;
+  const int last_bits = shift - 8 * (passes - 1);
+  msd_bucket_kernel<<<(unsigned)nb, kBlock, 0, s>>>(keys, out, start, passes, last_bits, st);
 }
 
 }  // namespace pgvd
