@@ -248,20 +248,103 @@ __device__ inline void bucket_sort(const uint64_t *__restrict__ in, uint64_t *__
   }
 }
 
-// One workgroup per bucket; the tile is the smallest of 12, 16 or 20 keys
-// per thread that holds the bucket.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void
+// Sub-buckets of the counting pass: the 12 bits below a bucket's own.
+constexpr int kSubBits = 12;
+constexpr int kSubBins = 1 << kSubBits;
+constexpr int kSubPer = kSubBins / kBlock;   // 16 consecutive bins per thread
+// Largest sub-bucket the counting pass finishes by insertion sort.
+constexpr uint32_t kSubMaxGroup = 32;
+
+// One workgroup per bucket (cnt <= kMsdCap keys in[s0 ..), all sharing the
+// bits from `shift` up).  Counting pass: every key gets a slot in its
+// sub-bucket (its next 12 bits, i.e. bits shift-12 .. shift-1) from an LDS
+// atomic, in any order -- keys without payload need no stable order -- and
+// each sub-bucket, ~1 key for uniform keys, is then put in order by an
+// insertion sort on the whole key.  A bucket with a sub-bucket of more than
+// kSubMaxGroup keys (clustered keys) is sorted instead by the stable
+// in-LDS radix passes, with the smallest of 12, 16 or 20 keys per thread
+// that holds it.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void
 msd_bucket_kernel(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
-                  const uint32_t *__restrict__ start, int passes, int last_bits,
+                  const uint32_t *__restrict__ start, int shift, int passes, int last_bits,
                   const RunState *st) {
   if (!st->msd_ok) return;
   const uint32_t s0 = start[blockIdx.x];
   const uint32_t cnt = min(start[blockIdx.x + 1] - s0, (uint32_t)kMsdCap);
   if (cnt == 0) return;
   __shared__ uint64_t s_keys[kMsdCap];
-  __shared__ uint32_t s_cnt[4 * 256];
-  __shared__ uint32_t s_dig[256];
-  __shared__ uint32_t s_w[4];
+  __shared__ uint32_t s_sub[kSubBins + 1];   // counts, then starts
+  __shared__ uint32_t s_w[4], s_mx[4];
+  // the radix passes' counters live in s_sub's space
+  uint32_t *s_cnt = s_sub, *s_dig = s_sub + 4 * 256;
+  const int t = threadIdx.x;
+  constexpr int IPT = kMsdCap / kBlock;
+  const int sub_shift = shift - kSubBits;
+  for (int i = t; i < kSubBins; i += kBlock) s_sub[i] = 0;
+  uint64_t k[IPT];
+  uint32_t rs[IPT];   // slot in sub-bucket | sub-bucket << 16
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t i = j * kBlock + t;
+    k[j] = in[s0 + (i < cnt ? i : 0u)];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t i = j * kBlock + t;
+    rs[j] = 0;
+    if (i < cnt) {
+      const uint32_t sb = (uint32_t)(k[j] >> sub_shift) & (kSubBins - 1);
+      rs[j] = atomicAdd(&s_sub[sb], 1u) | (sb << 16);
+    }
+  }
+  __syncthreads();
+  // starts of thread t's 16 consecutive sub-buckets, and the largest one
+  uint32_t c16[kSubPer], run = 0, mx = 0;
+#pragma unroll
+  for (int i = 0; i < kSubPer; ++i) {
+    c16[i] = s_sub[t * kSubPer + i];
+    run += c16[i];
+    mx = max(mx, c16[i]);
+  }
+  uint32_t total;
+  uint32_t ex = block_excl_sum<uint32_t>(run, s_w, &total);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
+  if ((t & 63) == 0) s_mx[t >> 6] = mx;
+  __syncthreads();   // every thread has read its counts
+#pragma unroll
+  for (int i = 0; i < kSubPer; ++i) {
+    s_sub[t * kSubPer + i] = ex;
+    ex += c16[i];
+  }
+  if (t == 0) s_sub[kSubBins] = total;
+  const uint32_t gmax = max(max(s_mx[0], s_mx[1]), max(s_mx[2], s_mx[3]));   // uniform
+  __syncthreads();
+  if (gmax <= kSubMaxGroup) {
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t i = j * kBlock + t;
+      if (i < cnt) s_keys[s_sub[rs[j] >> 16] + (rs[j] & 0xffffu)] = k[j];
+    }
+    __syncthreads();
+    // thread t puts its sub-buckets in order (insertion sort in LDS)
+    for (int i = 0; i < kSubPer; ++i) {
+      const uint32_t b0 = s_sub[t * kSubPer + i], b1 = s_sub[t * kSubPer + i + 1];
+      for (uint32_t x = b0 + 1; x < b1; ++x) {
+        const uint64_t v = s_keys[x];
+        uint32_t y = x;
+        while (y > b0 && s_keys[y - 1] > v) {
+          s_keys[y] = s_keys[y - 1];
+          --y;
+        }
+        s_keys[y] = v;
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < cnt; i += kBlock) out[s0 + i] = s_keys[i];
+    return;
+  }
   if (cnt <= 12 * kBlock)
     bucket_sort<12>(in, out, s0, cnt, passes, last_bits, s_keys, s_cnt, s_dig, s_w);
   else if (cnt <= 16 * kBlock)
@@ -294,7 +377,8 @@ void launch_msd_sort(uint64_t *keys, uint64_t *alt, uint64_t *out, int64_t n, in
                                                               nb1, start, cursor, st);
   const int passes = (shift + 7) / 8;
   const int last_bits = shift - 8 * (passes - 1);
-  msd_bucket_kernel<<<(unsigned)nb, kBlock, 0, s>>>(keys, out, start, passes, last_bits, st);
+  msd_bucket_kernel<<<(unsigned)nb, kBlock, 0, s>>>(keys, out, start, shift, passes, last_bits,
+                                                    st);
 }
 
 }  // namespace pgvd
