@@ -69,15 +69,20 @@ __global__ __launch_bounds__(kBlock) void msd_plan_kernel(const uint32_t *__rest
   if (t == 0) {
     const uint32_t m4 = max(max(s_mx[0], s_mx[1]), max(s_mx[2], s_mx[3]));
     st->msd_ok = m4 <= (uint32_t)kMsdCap ? 1u : 0u;
-    const int nb1 = nb / nb2;
-    uint32_t tiles = 0;
-    for (int g = 0; g < nb1; ++g) {
-      const uint32_t lo = start[g * nb2], hi = start[(g + 1) * nb2];
-      cursor[kCursor1 + g] = lo;
-      cursor[kTileStart + g] = tiles;
-      tiles += (hi - lo + kSplitTile - 1) / kSplitTile;
+  }
+  const int nb1 = nb / nb2;   // <= 64: wave 0, lane g = group g
+  if (t < 64) {
+    uint32_t lo = 0, tiles = 0;
+    if (t < nb1) {
+      lo = start[t * nb2];
+      tiles = (start[(t + 1) * nb2] - lo + kSplitTile - 1) / kSplitTile;
     }
-    cursor[kTileStart + nb1] = tiles;
+    const uint32_t incl = wave_incl_scan_add(tiles);
+    if (t < nb1) {
+      cursor[kCursor1 + t] = lo;
+      cursor[kTileStart + t] = incl - tiles;
+    }
+    if (t == nb1 - 1) cursor[kTileStart + nb1] = incl;
   }
 }
 

@@ -294,6 +294,7 @@ int launch_onesweep(K *keys, K *keys_alt, const uint32_t *vals_in, uint32_t *val
   if (n <= 1 || passes <= 0) return 0;
   const bool pay = vals_in != nullptr;
   const int64_t tiles = onesweep_tiles((int)sizeof(K), pay, n);
+
   K *src = keys, *dst = keys_alt;
   const uint32_t *vs = vals_in;
   uint32_t *vd = vals_alt, *vother = vals;
