@@ -5,11 +5,12 @@
     python bench.py --workload serve [--serve-rate 2000]    (served form)
 
 One step = the whole path (bounding box, FP32 quantize, 63-bit Morton keys,
-onesweep radix sort, unique, Karras BRT, local splits + scan, octree fill)
-over 2^24 float3 points that are already in HBM.  Prints one JSON line:
-sorted Morton keys/s (whole job), the per-launch roofline of the dominant
-kernel (an onesweep sort pass), the whole-path algorithmic-byte roofline,
-and the serial CPU path (the oracle, one core) timed on a bounded sample.
+radix sort, unique, Karras BRT with local splits, octree fill) over 2^24
+float3 points that are already in HBM.  Prints one JSON line: sorted Morton
+keys/s (whole job), the per-launch roofline of the dominant kernel (the
+octree gather when the bucket sort ran, else an onesweep sort pass), the
+whole-path algorithmic-byte roofline, and the serial CPU path (the oracle,
+one core) timed on a bounded sample.
 """
 from __future__ import annotations
 
@@ -49,25 +50,29 @@ def make_points(workload: str, n: int) -> tuple[np.ndarray, str]:
     raise SystemExit(f"unknown workload {workload}")
 
 
-def algorithmic_bytes(n: int, m: int, O: int, dim: int, K: int, R: int, L: int = 0) -> dict:
-    """Per-stage algorithmic HBM bytes (P point bytes, K key bytes, L label
-    bytes, R radix passes at 8-bit digits, B = 16+K BRT node bytes,
-    N = 4*(2^dim+1) octree node bytes)."""
+def algorithmic_bytes(n: int, m: int, O: int, dim: int, K: int, R: int, L: int = 0,
+                      bucket_sort: bool = False) -> dict:
+    """Per-stage algorithmic HBM bytes of the path as built (P point bytes,
+    K key bytes, L label bytes, R radix passes at 8-bit digits, B = 16+K BRT
+    node bytes, N = 4*(2^dim+1) octree node bytes).  The sort is R read+write
+    passes over keys and labels, or with the bucket sort two scatters and the
+    bucket pass over the keys.  The BRT pass writes records, parent links and
+    packed local splits; the gather reads records and local splits and writes
+    prefix sums and octree nodes."""
     P = 4 * dim
     B = 16 + K
     N = 4 * ((1 << dim) + 1)
     parts = {
         "bbox": P * n,
         "encode": (P + K + L) * n,
-        "sort": 2 * R * (K + L) * n,
+        "sort": 3 * 2 * K * n if bucket_sort else 2 * R * (K + L) * n,
         "unique": (K + L) * n + K * m,
-        "brt": K * m + B * (m - 1),
-        "splits": B * (m - 1) + 4 * m,
-        "scan": 8 * m,
-        "octree": (B + 8) * (m - 1) + N * O,
+        "brt": K * m + B * (m - 1) + 8 * m,
+        "octree": (B + 4) * (m - 1) + 4 * m + N * O,
     }
     parts["total"] = sum(parts.values())
     parts["sort_pass"] = 2 * (K + L) * n
+    parts["gather"] = parts["octree"]
     return parts
 
 
@@ -391,15 +396,18 @@ def main() -> int:
     dpts = dev.upload(pts)
     dlab = dev.upload(labels) if conflict else None
 
-    for _ in range(args.warmup):
+    res = None
+    for _ in range(max(args.warmup, 1)):
         dev.build_octree_async(dpts, n, dim, bits, dlab)
         res = dev.build_octree_finish()
+    bucket = bool(res.bucket_sort)
 
     # Timed region.  HIP events cost ~10 us of queue time each, so the only
-    # events inside it are one pair per step around the radix-sort passes
-    # (the dominant kernel, launched back to back).
+    # events inside it are one pair per step around the dominant kernel: the
+    # octree gather when the bucket sort ran, else the radix-sort passes
+    # (launched back to back).
     passes = (dim * bits + 7) // 8
-    dev.timing(True, sort_span_only=True)
+    dev.timing(True, sort_span_only=not bucket, gather_only=bucket)
     dev.timing_reset()
     dev.synchronize()
     t0 = time.perf_counter()
@@ -409,8 +417,12 @@ def main() -> int:
     dev.synchronize()
     elapsed = time.perf_counter() - t0
     ms_step = elapsed * 1e3 / args.steps
-    sort_ms, sort_cnt = dev.timing_read("sort")
-    pass_ms = sort_ms / max(sort_cnt, 1) / passes
+    if bucket:
+        dom_ms, dom_cnt = dev.timing_read("gather")
+        launch_ms, launches = dom_ms / max(dom_cnt, 1), dom_cnt
+    else:
+        dom_ms, dom_cnt = dev.timing_read("sort")
+        launch_ms, launches = dom_ms / max(dom_cnt, 1) / passes, dom_cnt * passes
 
     # Per-stage breakdown from separate steps with an event pair per stage
     # (these steps are slower by the event overhead and are not the metric).
@@ -426,7 +438,7 @@ def main() -> int:
         flagged = {"conflict_keys": int(got["key_conflict"].sum()),
                    "nodes_with_conflict_octants": int((got["node_conflict"] != 0).sum())}
     stage_ms = {}
-    for st in ["prep", "bbox", "encode", "sort_pass", "unique", "brt", "splits_scan",
+    for st in ["prep", "bbox", "encode", "msd", "sort_pass", "unique", "brt", "splits_scan",
                "malformed", "octree"]:
         tot, cnt = dev.timing_read(st)
         stage_ms[st] = (tot, cnt)
@@ -434,12 +446,20 @@ def main() -> int:
 
     m, O = int(res.m), int(res.octree_size)
     lb = 4 if conflict else 0
-    ab = algorithmic_bytes(n, m, O, dim, int(res.key_bytes), passes, lb)
+    ab = algorithmic_bytes(n, m, O, dim, int(res.key_bytes), passes, lb, bucket)
     if conflict:      # the conflict-flag term of the byte model: labels read + m/8 flag bits
         ab["total"] += lb * n + m // 8
-    pass_gbps = ab["sort_pass"] / (pass_ms * 1e-3) / 1e9
+    dom_key = "gather" if bucket else "sort_pass"
+    dom_gbps = ab[dom_key] / (launch_ms * 1e-3) / 1e9
     total_gbps = ab["total"] / (ms_step * 1e-3) / 1e9
-    traffic = load_traffic("sort_pass", args.workload)
+    traffic = load_traffic(dom_key, args.workload)
+    if bucket:
+        dom_name = (f"gather_kernel (octree fill of {m} BRT nodes into {O} octree nodes; "
+                    f"algorithmic bytes {ab['gather']})")
+    elif conflict:
+        dom_name = "onesweep_kernel (one 8-bit radix pass of 2^22 u128 keys with u32 labels)"
+    else:
+        dom_name = f"onesweep_kernel (one 8-bit radix pass of {n} u64 keys)"
 
     out = {
         "metric": (METRIC if not conflict else
@@ -461,19 +481,19 @@ def main() -> int:
                         "points-in-HBM to octree-in-HBM",
             "n_points": n, "dim": dim, "bits": bits, "distribution": args.workload,
             "unique_keys": m, "octree_nodes": O, "malformed_brt": int(res.malformed),
+            "sort": "bucket sort (2 scatters + in-LDS buckets)" if bucket else "LSD onesweep",
             **({"labels": "uint32 loop id per point", "flags": flagged} if conflict else {}),
         },
         "roofline": {
-            "kernel": ("onesweep_kernel (one 8-bit radix pass of 2^22 u128 keys with u32 labels)"
-                       if conflict else "onesweep_kernel (one 8-bit radix pass of 2^24 u64 keys)"),
+            "kernel": dom_name,
             "bound": "hbm",
-            "achieved": pass_gbps,
+            "achieved": dom_gbps,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
-            "frac": pass_gbps / HBM_PEAK_GBPS,
+            "frac": dom_gbps / HBM_PEAK_GBPS,
             "traffic": traffic,
-            "avg_launch_ms": pass_ms,
-            "launches": sort_cnt * passes,
+            "avg_launch_ms": launch_ms,
+            "launches": launches,
         },
         "pipeline": {
             "algorithmic_bytes": ab["total"],

@@ -70,10 +70,12 @@ int pgvd_copy_kernel(pgvd_ctx *ctx, void *dst, const void *src, size_t bytes, pg
 int pgvd_digest(pgvd_ctx *ctx, const void *p, size_t bytes, pgvd_stream s, uint64_t *out);
 
 /* Kernel timing with HIP events on the launch stream.  on = 1 records an
- * event pair around every stage ("prep", "bbox", "encode", each
- * "sort_pass", "unique", "brt", "splits_scan", "malformed", "octree");
- * on = 2 records one pair around all radix-sort passes of a build ("sort")
- * and nothing else (each event adds ~10 us of queue time).
+ * event pair around every stage ("prep", "bbox", "encode", "msd" (the
+ * bucket sort), each "sort_pass", "unique", "brt", "splits_scan",
+ * "malformed", "octree"); on = 2 records one pair around all radix-sort
+ * passes of a build ("sort") and nothing else; on = 3 one pair around the
+ * octree gather kernel of a build ("gather") and nothing else (each event
+ * adds ~10 us of queue time).
  * pgvd_timing_read returns the summed milliseconds and record count of one
  * stage since the last reset; it synchronizes. */
 int pgvd_timing_enable(pgvd_ctx *ctx, int on);
@@ -208,6 +210,8 @@ typedef struct pgvd_octree_result {
   int64_t octree_size;      /* octree nodes                                      */
   int dim, bits, mbits, key_bytes;
   int malformed;            /* 1 if some BRT split took the FP32-quirk value     */
+  int bucket_sort;          /* 1 if the bucket sort ordered the keys, 0 the
+                               LSD radix passes                               */
   /* device pointers owned by the context, valid until its next build:       */
   const void *keys;         /* m sorted unique keys                              */
   const void *brt;          /* m-1 pgvd_brt_node{32,64,128}                      */

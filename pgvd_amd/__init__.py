@@ -53,7 +53,7 @@ class OctreeResult(C.Structure):
     _fields_ = [
         ("n", C.c_int64), ("m", C.c_int64), ("octree_size", C.c_int64),
         ("dim", C.c_int), ("bits", C.c_int), ("mbits", C.c_int), ("key_bytes", C.c_int),
-        ("malformed", C.c_int),
+        ("malformed", C.c_int), ("bucket_sort", C.c_int),
         ("keys", C.c_void_p), ("brt", C.c_void_p), ("parent", C.c_void_p),
         ("prefix_sums", C.c_void_p), ("octree", C.c_void_p),
         ("key_conflict", C.c_void_p), ("node_conflict", C.c_void_p),
@@ -284,8 +284,11 @@ class Device:
     def copy_kernel(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int):
         self._check(lib().pgvd_copy_kernel(self.ctx, dst.ptr, src.ptr, nbytes, None))
 
-    def timing(self, on: bool = True, sort_span_only: bool = False):
-        mode = (2 if sort_span_only else 1) if on else 0
+    def timing(self, on: bool = True, sort_span_only: bool = False, gather_only: bool = False):
+        """Stage timing: every stage; or only one event pair per build around
+        the radix-sort passes (sort_span_only) or the octree gather kernel
+        (gather_only)."""
+        mode = (3 if gather_only else 2 if sort_span_only else 1) if on else 0
         self._check(lib().pgvd_timing_enable(self.ctx, mode))
 
     def timing_reset(self):
@@ -464,7 +467,8 @@ class Device:
         parent = self.download(r.parent, np.int32, ni)
         ps = self.download(r.prefix_sums, np.uint32, m)
         raw = self.download(r.octree, np.int32, O * (nc + 1)).reshape(O, nc + 1)
-        out = dict(m=m, octree_size=O, malformed=r.malformed, keys_lo=lo, keys_hi=hi,
+        out = dict(m=m, octree_size=O, malformed=r.malformed, bucket_sort=r.bucket_sort,
+                   keys_lo=lo, keys_hi=hi,
                    brt=brt, parent=parent, prefix_sums=ps,
                    children=raw[:, :nc].copy(), leaf=raw[:, nc].copy())
         if kb == 16:

@@ -325,8 +325,15 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   CHK(launch_status("malformed path"));
   {
     Stage t(ctx, "octree", s);
+    hipEvent_t gev[2] = {nullptr, nullptr};
+    if (ctx->timing == 3) {
+      gev[0] = take_event(ctx);
+      gev[1] = take_event(ctx);
+    }
+    const bool timed = gev[0] && gev[1];
     launch_octree<K>(brt, parent, ps, lsx, chunks, uniq, cbits, nk, dim, mbits, octree, nconf,
-                     ctx->octree_cap, 0, defer, ctx->st, s);
+                     ctx->octree_cap, 0, defer, ctx->st, s, timed ? gev : nullptr);
+    if (timed) ctx->recs.push_back({"gather", gev[0], gev[1]});
   }
   CHK(launch_status("octree"));
   HIPCHK(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(RunState), hipMemcpyDeviceToHost, s));
@@ -604,7 +611,7 @@ int pgvd_digest(pgvd_ctx *ctx, const void *p, size_t bytes, pgvd_stream s, uint6
 
 int pgvd_timing_enable(pgvd_ctx *ctx, int on) {
   if (!ctx) return fail(PGVD_ERR_INVALID, "null context");
-  ctx->timing = (on == 2) ? 2 : (on != 0 ? 1 : 0);
+  ctx->timing = (on == 2 || on == 3) ? on : (on != 0 ? 1 : 0);
   return PGVD_OK;
 }
 
@@ -901,6 +908,7 @@ int pgvd_build_octree_finish(pgvd_ctx *ctx, pgvd_octree_result *out) {
   out->mbits = L.mbits;
   out->key_bytes = L.kb;
   out->malformed = R.malformed ? 1 : 0;
+  out->bucket_sort = R.msd_ok ? 1 : 0;
   out->keys = L.keys_u;
   out->brt = L.brt;
   out->parent = L.parent;

@@ -237,13 +237,15 @@ void launch_claims_if_malformed(const BrtNodeT<K> *brt, int32_t *parent, int64_t
 // launch_brt_ls and uses defer[] for its miss words ((max_m + 63) / 64
 // 64-bit words, 8-byte aligned).
 // The gather path also writes ps[0..m) from chunk_excl (launch_chunk_scan)
-// and lsx.
+// and lsx.  gather_ev (nullable): two events recorded around the gather
+// kernel's launch.
 template <typename K>
 void launch_octree(const BrtNodeT<K> *brt, const int32_t *parent, uint32_t *ps,
                    const uint32_t *lsx, const uint32_t *chunk_excl, const K *keys,
                    const uint32_t *conflict_bits, int64_t max_m, int dim, int mbits,
                    int32_t *octree, uint32_t *node_conflict, int64_t capacity,
-                   int force_scatter, uint32_t *defer, RunState *st, hipStream_t s);
+                   int force_scatter, uint32_t *defer, RunState *st, hipStream_t s,
+                   hipEvent_t *gather_ev = nullptr);
 // Local splits the reference's way (each node writes its internal children's
 // entries, highest claimer last): ComputeLocalSplits_p for a caller's BRT.
 template <typename K>

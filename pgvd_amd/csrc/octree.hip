@@ -618,7 +618,7 @@ void launch_dim(const BrtNodeT<K> *brt, const int32_t *parent, uint32_t *ps,
                 const uint32_t *lsx, const uint32_t *chunk_excl, const K *keys,
                 const uint32_t *conflict_bits, int64_t max_m, int mbits, int32_t *octree,
                 uint32_t *node_conflict, int64_t capacity, int force_scatter, uint32_t *defer,
-                RunState *st, hipStream_t s) {
+                RunState *st, hipStream_t s, hipEvent_t *gather_ev) {
   const int64_t nodes = max_m > 1 ? max_m - 1 : 1;
   const unsigned grid = (unsigned)((nodes + kBlock - 1) / kBlock);
   const unsigned sgrid = grid < 1024 ? grid : 1024;
@@ -626,9 +626,11 @@ void launch_dim(const BrtNodeT<K> *brt, const int32_t *parent, uint32_t *ps,
     uint64_t *words = reinterpret_cast<uint64_t *>(defer);
     const unsigned ggrid =
         (unsigned)((nodes + kGatherRows * kBlock - 1) / (kGatherRows * kBlock));
+    if (gather_ev) (void)hipEventRecord(gather_ev[0], s);
     gather_kernel<K, DIM><<<ggrid, kBlock, 0, s>>>(brt, ps, lsx, chunk_excl, keys, conflict_bits,
                                                   mbits, octree, node_conflict, capacity, words,
                                                   st);
+    if (gather_ev) (void)hipEventRecord(gather_ev[1], s);
     const int64_t g2 = ((nodes + 63) / 64 + kBlock - 1) / kBlock;
     const unsigned dgrid = (unsigned)(g2 < 4096 ? g2 : 4096);
     gather_defer_kernel<K, DIM><<<dgrid, kBlock, 0, s>>>(brt, ps, lsx, keys, conflict_bits,
@@ -692,13 +694,14 @@ void launch_octree(const BrtNodeT<K> *brt, const int32_t *parent, uint32_t *ps,
                    const uint32_t *lsx, const uint32_t *chunk_excl, const K *keys,
                    const uint32_t *conflict_bits, int64_t max_m, int dim, int mbits,
                    int32_t *octree, uint32_t *node_conflict, int64_t capacity,
-                   int force_scatter, uint32_t *defer, RunState *st, hipStream_t s) {
+                   int force_scatter, uint32_t *defer, RunState *st, hipStream_t s,
+                   hipEvent_t *gather_ev) {
   if (dim == 2)
     launch_dim<K, 2>(brt, parent, ps, lsx, chunk_excl, keys, conflict_bits, max_m, mbits,
-                     octree, node_conflict, capacity, force_scatter, defer, st, s);
+                     octree, node_conflict, capacity, force_scatter, defer, st, s, gather_ev);
   else
     launch_dim<K, 3>(brt, parent, ps, lsx, chunk_excl, keys, conflict_bits, max_m, mbits,
-                     octree, node_conflict, capacity, force_scatter, defer, st, s);
+                     octree, node_conflict, capacity, force_scatter, defer, st, s, gather_ev);
 }
 
 template <typename K, int DIM>
@@ -724,7 +727,8 @@ void launch_gather_seg(const BrtNodeT<K> *brt, const uint32_t *ps, const uint32_
   template void launch_octree<K>(const BrtNodeT<K> *, const int32_t *, uint32_t *,             \
                                  const uint32_t *, const uint32_t *, const K *,                \
                                  const uint32_t *, int64_t, int, int, int32_t *, uint32_t *,   \
-                                 int64_t, int, uint32_t *, RunState *, hipStream_t);           \
+                                 int64_t, int, uint32_t *, RunState *, hipStream_t,            \
+                                 hipEvent_t *);                                                \
   template void launch_gather_seg<K>(const BrtNodeT<K> *, const uint32_t *, const uint32_t *,  \
                                      int, const BatchDev &, int32_t *, int64_t, uint32_t *,    \
                                      RunState *, hipStream_t);                                 \

@@ -152,6 +152,8 @@ def test_full_size_3d_digests(dev, golden_digests, name):
         pts, _ = workloads.gaussian_mixture(1 << 24)
     assert _digest(pts, np.float32) == d["pts"]
     got = dev.build_octree(pts, 21)
+    if name == "c2_uniform":
+        assert got["bucket_sort"] == 1      # uniform keys fill every bucket evenly
     assert got["m"] == d["m"] and got["octree_size"] == d["octree_size"]
     for k, dt in DTYPES.items():
         assert _digest(got[k], dt) == d[k], k
@@ -159,6 +161,25 @@ def test_full_size_3d_digests(dev, golden_digests, name):
     k = got["keys_lo"]
     assert (k[1:] > k[:-1]).all()
     assert got["parent"][0] == -1
+
+
+def test_bucket_sort_and_its_fallback(dev):
+    """Builds of 2^23 or more 64-bit keys scatter by the top key bits into
+    buckets that each sort inside one workgroup; when any bucket is too full
+    the LSD radix passes sort instead.  Both paths against the oracle at a
+    size that is not a power of two."""
+    n = (1 << 23) + 4097
+    pts = _pts(n, 3, seed=23)
+    got = dev.build_octree(pts, 21)
+    assert got["bucket_sort"] == 1
+    assert_tree_equal(got, oracle.build(oracle.quantize(pts, 1 << 21), 21), where="buckets")
+    clustered = pts.copy()
+    clustered[: n // 4] *= np.float32(1e-3)         # a quarter of the points near the centre
+    got = dev.build_octree(clustered, 21)
+    assert got["bucket_sort"] == 0
+    assert_tree_equal(got, oracle.build(oracle.quantize(clustered, 1 << 21), 21),
+                      where="fallback")
+    assert (got["keys_lo"][1:] > got["keys_lo"][:-1]).all()
 
 
 @pytest.mark.parametrize("dim", [2, 3])
