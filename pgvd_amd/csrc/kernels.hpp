@@ -41,6 +41,7 @@ struct RunState {
   uint32_t m;                // unique key count (incl. the padding zero)
   uint32_t octree_size;      // O = ps[m-1]
   uint32_t msd_ok;           // 1: the bucket sort (msd.hip) sorted the keys
+  uint32_t msd_nflag;        // buckets the bucket sort's fallback kernel sorts
   uint32_t counters[kNumCounters];
 };
 
@@ -170,7 +171,13 @@ int launch_onesweep(K *keys, K *keys_alt, const uint32_t *vals_in, uint32_t *val
 // the LSD passes beside it with skip_if_msd = st.
 constexpr int kMsdCap = 5120;
 constexpr int kMsdMaxBuckets = 4096;
-constexpr int kMsdCursorWords = kMsdMaxBuckets + 2 * 64 + 1;
+// Words between the first scatter's cursors (one per L2 line: the 4096-odd
+// scatter tiles all claim on these few words).
+#ifndef PGVD_MSD_CSTRIDE
+#define PGVD_MSD_CSTRIDE 32
+#endif
+constexpr int kMsdCStride = PGVD_MSD_CSTRIDE;
+constexpr int kMsdCursorWords = kMsdMaxBuckets + 64 * kMsdCStride + 64 + 1;
 // b for n keys (0: no bucket sort for this size).
 int msd_bits_for(int64_t n);
 // keys -> alt (scatter) -> out (buckets sorted; out may be alt).  hist: the

@@ -24,11 +24,12 @@ constexpr int kSplitTile = kBlock * kSplitIpt; // 4096
 constexpr int kSplitMaxDigits = 64;            // 2^(b/2) and 2^(b - b/2) for b <= 12
 static_assert(kMsdCap == kBlock * 20, "a bucket sort holds 20 keys per thread at most");
 
-// Words of msd_cursor: cursor[0..2^b) for the second scatter, then the
-// first scatter's 2^(b/2) cursors, then the second scatter's 2^(b/2) + 1
-// tile starts.
+// Words of msd_cursor: cursor[0..2^b) for the second scatter (then the
+// list of buckets left to the fallback kernel), then the first scatter's
+// 2^(b/2) cursors kMsdCStride words apart, then the second scatter's
+// 2^(b/2) + 1 tile starts.
 constexpr int kCursor1 = kMsdMaxBuckets;
-constexpr int kTileStart = kMsdMaxBuckets + kSplitMaxDigits;
+constexpr int kTileStart = kMsdMaxBuckets + kSplitMaxDigits * kMsdCStride;
 static_assert(kTileStart + kSplitMaxDigits + 1 <= kMsdCursorWords, "msd_cursor holds them");
 
 // Plan, one workgroup: exclusive scan of the bucket histogram into
@@ -69,6 +70,7 @@ __global__ __launch_bounds__(kBlock) void msd_plan_kernel(const uint32_t *__rest
   if (t == 0) {
     const uint32_t m4 = max(max(s_mx[0], s_mx[1]), max(s_mx[2], s_mx[3]));
     st->msd_ok = m4 <= (uint32_t)kMsdCap ? 1u : 0u;
+    st->msd_nflag = 0;
   }
   const int nb1 = nb / nb2;   // <= 64: wave 0, lane g = group g
   if (t < 64) {
@@ -79,7 +81,7 @@ __global__ __launch_bounds__(kBlock) void msd_plan_kernel(const uint32_t *__rest
     }
     const uint32_t incl = wave_incl_scan_add(tiles);
     if (t < nb1) {
-      cursor[kCursor1 + t] = lo;
+      cursor[kCursor1 + t * kMsdCStride] = lo;
       cursor[kTileStart + t] = incl - tiles;
     }
     if (t == nb1 - 1) cursor[kTileStart + nb1] = incl;
@@ -88,11 +90,12 @@ __global__ __launch_bounds__(kBlock) void msd_plan_kernel(const uint32_t *__rest
 
 // One scatter tile: keys in[lo .. hi) (0 < hi - lo <= 4096) go to out in
 // runs by digit ld = (key >> dshift) & (nd - 1), the run of digit ld
-// starting at an atomic claim on gcursor[gbase + ld].  Keys are staged in
+// starting at an atomic claim on gcursor[gbase + ld * cstride].  Keys are staged in
 // LDS in digit order so that each run is written by consecutive lanes.
 __device__ inline void split_tile(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
                                   uint32_t n, uint32_t lo, uint32_t hi, int dshift, int nd,
-                                  uint32_t *__restrict__ gcursor, uint32_t gbase) {
+                                  uint32_t *__restrict__ gcursor, uint32_t gbase,
+                                  int cstride) {
   __shared__ uint64_t s_keys[kSplitTile];
   __shared__ uint32_t s_cnt[kSplitMaxDigits], s_lstart[kSplitMaxDigits],
       s_gstart[kSplitMaxDigits];
@@ -122,7 +125,7 @@ __device__ inline void split_tile(const uint64_t *__restrict__ in, uint64_t *__r
     const uint32_t incl = wave_incl_scan_add(c);
     if (t < nd) {
       s_lstart[t] = incl - c;
-      s_gstart[t] = c ? atomicAdd(&gcursor[gbase + t], c) : 0u;
+      s_gstart[t] = c ? atomicAdd(&gcursor[gbase + t * cstride], c) : 0u;
     }
   }
   __syncthreads();
@@ -150,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void msd_split1_kernel(const uint64_t *__re
   if (!st->msd_ok) return;
   const uint32_t lo = blockIdx.x * (uint32_t)kSplitTile;
   const uint32_t hi = min(lo + (uint32_t)kSplitTile, n);
-  split_tile(in, out, n, lo, hi, shift1, nb1, cursor + kCursor1, 0u);
+  split_tile(in, out, n, lo, hi, shift1, nb1, cursor + kCursor1, 0u, kMsdCStride);
 }
 
 // Second scatter: tiles within each first-level group g (keys
@@ -173,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void msd_split2_kernel(const uint64_t *__re
   const uint32_t glo = start[g * nb2], ghi = start[(g + 1) * nb2];
   const uint32_t lo = glo + (tile - ts[g]) * (uint32_t)kSplitTile;
   const uint32_t hi = min(lo + (uint32_t)kSplitTile, ghi);
-  split_tile(in, out, n, lo, hi, shift, nb2, cursor, (uint32_t)(g * nb2));
+  split_tile(in, out, n, lo, hi, shift, nb2, cursor, (uint32_t)(g * nb2), 1);
 }
 
 // Sorts cnt <= 256 IPT keys of one bucket, in[s0 ..), on bits [0, 8 passes)
@@ -256,9 +259,17 @@ __device__ inline void bucket_sort(const uint64_t *__restrict__ in, uint64_t *__
 // Sub-buckets of the counting pass: the 12 bits below a bucket's own.
 constexpr int kSubBits = 12;
 constexpr int kSubBins = 1 << kSubBits;
-constexpr int kSubPer = kSubBins / kBlock;   // 16 consecutive bins per thread
 // Largest sub-bucket the counting pass finishes by insertion sort.
 constexpr uint32_t kSubMaxGroup = 32;
+// Threads of a counting-pass workgroup.  The LDS of one bucket (~57 KB)
+// allows two workgroups per CU, so the waves that hide the loads' latency
+// come from large workgroups.
+#ifndef PGVD_MSD_BLOCK
+#define PGVD_MSD_BLOCK 1024
+#endif
+constexpr int kCountBlock = PGVD_MSD_BLOCK;
+// Workgroups of the fallback kernel; each loops over the flagged buckets.
+constexpr int kFallbackGrid = 512;
 
 // One workgroup per bucket (cnt <= kMsdCap keys in[s0 ..), all sharing the
 // bits from `shift` up).  Counting pass: every key gets a slot in its
@@ -266,37 +277,38 @@ constexpr uint32_t kSubMaxGroup = 32;
 // atomic, in any order -- keys without payload need no stable order -- and
 // each sub-bucket, ~1 key for uniform keys, is then put in order by an
 // insertion sort on the whole key.  A bucket with a sub-bucket of more than
-// kSubMaxGroup keys (clustered keys) is sorted instead by the stable
-// in-LDS radix passes, with the smallest of 12, 16 or 20 keys per thread
-// that holds it.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void
-msd_bucket_kernel(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
-                  const uint32_t *__restrict__ start, int shift, int passes, int last_bits,
-                  const RunState *st) {
+// kSubMaxGroup keys (clustered keys) is appended to `flagged` and left to
+// msd_fallback_kernel.
+template <int BS>
+__global__ __launch_bounds__(BS) void msd_bucket_kernel(const uint64_t *__restrict__ in,
+                                                        uint64_t *__restrict__ out,
+                                                        const uint32_t *__restrict__ start,
+                                                        int shift, uint32_t *__restrict__ flagged,
+                                                        RunState *st) {
+  constexpr int IPT = (kMsdCap + BS - 1) / BS;
+  constexpr int NW = BS / 64;
+  constexpr int PER = kSubBins / BS;   // consecutive sub-buckets per thread
   if (!st->msd_ok) return;
   const uint32_t s0 = start[blockIdx.x];
   const uint32_t cnt = min(start[blockIdx.x + 1] - s0, (uint32_t)kMsdCap);
   if (cnt == 0) return;
   __shared__ uint64_t s_keys[kMsdCap];
   __shared__ uint32_t s_sub[kSubBins + 1];   // counts, then starts
-  __shared__ uint32_t s_w[4], s_mx[4];
-  // the radix passes' counters live in s_sub's space
-  uint32_t *s_cnt = s_sub, *s_dig = s_sub + 4 * 256;
+  __shared__ uint32_t s_w[NW], s_mx[NW];
   const int t = threadIdx.x;
-  constexpr int IPT = kMsdCap / kBlock;
   const int sub_shift = shift - kSubBits;
-  for (int i = t; i < kSubBins; i += kBlock) s_sub[i] = 0;
+  for (int i = t; i < kSubBins; i += BS) s_sub[i] = 0;
   uint64_t k[IPT];
   uint32_t rs[IPT];   // slot in sub-bucket | sub-bucket << 16
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
-    const uint32_t i = j * kBlock + t;
+    const uint32_t i = j * BS + t;
     k[j] = in[s0 + (i < cnt ? i : 0u)];
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
-    const uint32_t i = j * kBlock + t;
+    const uint32_t i = j * BS + t;
     rs[j] = 0;
     if (i < cnt) {
       const uint32_t sb = (uint32_t)(k[j] >> sub_shift) & (kSubBins - 1);
@@ -304,58 +316,86 @@ msd_bucket_kernel(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
     }
   }
   __syncthreads();
-  // starts of thread t's 16 consecutive sub-buckets, and the largest one
-  uint32_t c16[kSubPer], run = 0, mx = 0;
+  // starts of thread t's PER consecutive sub-buckets, and the largest one
+  uint32_t cp[PER], run = 0, mx = 0;
 #pragma unroll
-  for (int i = 0; i < kSubPer; ++i) {
-    c16[i] = s_sub[t * kSubPer + i];
-    run += c16[i];
-    mx = max(mx, c16[i]);
+  for (int i = 0; i < PER; ++i) {
+    cp[i] = s_sub[t * PER + i];
+    run += cp[i];
+    mx = max(mx, cp[i]);
   }
   uint32_t total;
-  uint32_t ex = block_excl_sum<uint32_t>(run, s_w, &total);
+  uint32_t ex = block_excl_sum<uint32_t, NW>(run, s_w, &total);
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
   if ((t & 63) == 0) s_mx[t >> 6] = mx;
   __syncthreads();   // every thread has read its counts
 #pragma unroll
-  for (int i = 0; i < kSubPer; ++i) {
-    s_sub[t * kSubPer + i] = ex;
-    ex += c16[i];
+  for (int i = 0; i < PER; ++i) {
+    s_sub[t * PER + i] = ex;
+    ex += cp[i];
   }
   if (t == 0) s_sub[kSubBins] = total;
-  const uint32_t gmax = max(max(s_mx[0], s_mx[1]), max(s_mx[2], s_mx[3]));   // uniform
-  __syncthreads();
-  if (gmax <= kSubMaxGroup) {
+  uint32_t gmax = 0;   // uniform
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const uint32_t i = j * kBlock + t;
-      if (i < cnt) s_keys[s_sub[rs[j] >> 16] + (rs[j] & 0xffffu)] = k[j];
-    }
-    __syncthreads();
-    // thread t puts its sub-buckets in order (insertion sort in LDS)
-    for (int i = 0; i < kSubPer; ++i) {
-      const uint32_t b0 = s_sub[t * kSubPer + i], b1 = s_sub[t * kSubPer + i + 1];
-      for (uint32_t x = b0 + 1; x < b1; ++x) {
-        const uint64_t v = s_keys[x];
-        uint32_t y = x;
-        while (y > b0 && s_keys[y - 1] > v) {
-          s_keys[y] = s_keys[y - 1];
-          --y;
-        }
-        s_keys[y] = v;
-      }
-    }
-    __syncthreads();
-    for (uint32_t i = t; i < cnt; i += kBlock) out[s0 + i] = s_keys[i];
+  for (int w = 0; w < NW; ++w) gmax = max(gmax, s_mx[w]);
+  if (gmax > kSubMaxGroup) {
+    if (t == 0) flagged[atomicAdd(&st->msd_nflag, 1u)] = blockIdx.x;
     return;
   }
-  if (cnt <= 12 * kBlock)
-    bucket_sort<12>(in, out, s0, cnt, passes, last_bits, s_keys, s_cnt, s_dig, s_w);
-  else if (cnt <= 16 * kBlock)
-    bucket_sort<16>(in, out, s0, cnt, passes, last_bits, s_keys, s_cnt, s_dig, s_w);
-  else
-    bucket_sort<20>(in, out, s0, cnt, passes, last_bits, s_keys, s_cnt, s_dig, s_w);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t i = j * BS + t;
+    if (i < cnt) s_keys[s_sub[rs[j] >> 16] + (rs[j] & 0xffffu)] = k[j];
+  }
+  __syncthreads();
+  // thread t puts its sub-buckets in order (insertion sort in LDS)
+  uint32_t b0 = s_sub[t * PER];
+  for (int i = 0; i < PER; ++i) {
+    const uint32_t b1 = s_sub[t * PER + i + 1];
+    for (uint32_t x = b0 + 1; x < b1; ++x) {
+      const uint64_t v = s_keys[x];
+      uint32_t y = x;
+      while (y > b0 && s_keys[y - 1] > v) {
+        s_keys[y] = s_keys[y - 1];
+        --y;
+      }
+      s_keys[y] = v;
+    }
+    b0 = b1;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t i = j * BS + t;
+    if (i < cnt) out[s0 + i] = s_keys[i];
+  }
+}
+
+// The buckets msd_bucket_kernel flagged (a sub-bucket of more than
+// kSubMaxGroup keys), sorted by the stable in-LDS radix passes with the
+// smallest of 12, 16 or 20 keys per thread that holds the bucket.  Every
+// workgroup loops over the list; with no flagged bucket it does nothing.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void
+msd_fallback_kernel(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                    const uint32_t *__restrict__ start, const uint32_t *__restrict__ flagged,
+                    int passes, int last_bits, const RunState *st) {
+  if (!st->msd_ok) return;
+  const uint32_t nf = st->msd_nflag;
+  __shared__ uint64_t s_keys[kMsdCap];
+  __shared__ uint32_t s_cnt[4 * 256], s_dig[256], s_w[4];
+  for (uint32_t f = blockIdx.x; f < nf; f += gridDim.x) {
+    const uint32_t b = flagged[f];
+    const uint32_t s0 = start[b];
+    const uint32_t cnt = min(start[b + 1] - s0, (uint32_t)kMsdCap);
+    if (cnt <= 12 * kBlock)
+      bucket_sort<12>(in, out, s0, cnt, passes, last_bits, s_keys, s_cnt, s_dig, s_w);
+    else if (cnt <= 16 * kBlock)
+      bucket_sort<16>(in, out, s0, cnt, passes, last_bits, s_keys, s_cnt, s_dig, s_w);
+    else
+      bucket_sort<20>(in, out, s0, cnt, passes, last_bits, s_keys, s_cnt, s_dig, s_w);
+  }
 }
 
 }  // namespace
@@ -382,8 +422,10 @@ void launch_msd_sort(uint64_t *keys, uint64_t *alt, uint64_t *out, int64_t n, in
                                                               nb1, start, cursor, st);
   const int passes = (shift + 7) / 8;
   const int last_bits = shift - 8 * (passes - 1);
-  msd_bucket_kernel<<<(unsigned)nb, kBlock, 0, s>>>(keys, out, start, shift, passes, last_bits,
-                                                    st);
+  msd_bucket_kernel<kCountBlock><<<(unsigned)nb, kCountBlock, 0, s>>>(keys, out, start, shift,
+                                                                      cursor, st);
+  msd_fallback_kernel<<<kFallbackGrid, kBlock, 0, s>>>(keys, out, start, cursor, passes,
+                                                       last_bits, st);
 }
 
 }  // namespace pgvd

@@ -61,8 +61,9 @@ __device__ inline uint64_t tile_prefix(uint64_t *status, int tile, uint64_t tota
 }
 
 // Exclusive scan of one value per thread over the 256-thread workgroup;
-// *total receives the workgroup sum.  s_w: 4 words of LDS.
-template <typename T>
+// *total receives the workgroup sum.  s_w: NW words of LDS, NW = waves per
+// workgroup.
+template <typename T, int NW = kBlock / 64>
 __device__ inline T block_excl_sum(T x, T *s_w, T *total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const T incl = wave_incl_scan_add(x);
@@ -70,7 +71,7 @@ __device__ inline T block_excl_sum(T x, T *s_w, T *total) {
   __syncthreads();
   T before = 0, all = 0;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < NW; ++j) {
     const T y = s_w[j];
     if (j < w) before += y;
     all += y;

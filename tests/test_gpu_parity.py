@@ -173,6 +173,14 @@ def test_bucket_sort_and_its_fallback(dev):
     got = dev.build_octree(pts, 21)
     assert got["bucket_sort"] == 1
     assert_tree_equal(got, oracle.build(oracle.quantize(pts, 1 << 21), 21), where="buckets")
+    # 150 copies of each of 8 points: their buckets hold a sub-bucket too
+    # large for the counting pass and are sorted by the in-LDS radix passes
+    dup = pts.copy()
+    for g in range(8):
+        dup[1000 * g + 1: 1000 * g + 150] = dup[1000 * g]
+    got = dev.build_octree(dup, 21)
+    assert got["bucket_sort"] == 1
+    assert_tree_equal(got, oracle.build(oracle.quantize(dup, 1 << 21), 21), where="dup buckets")
     clustered = pts.copy()
     clustered[: n // 4] *= np.float32(1e-3)         # a quarter of the points near the centre
     got = dev.build_octree(clustered, 21)
