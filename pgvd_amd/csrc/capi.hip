@@ -281,8 +281,8 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
       CHK(pool_get(ctx, "msd_start", ((size_t)kMsdMaxBuckets + 1) * 4, &p));
       uint32_t *mstart = static_cast<uint32_t *>(p);
       CHK(pool_get(ctx, "msd_cursor", (size_t)kMsdCursorWords * 4, &p));
-      launch_msd_sort(kA, kB, (passes & 1) ? kB : kA, n, mbits, msd_b, msd_hist, mstart,
-                      static_cast<uint32_t *>(p), ctx->st, s);
+      launch_msd_sort(kA, kB, (passes & 1) ? kB : kA, prepend, n, mbits, msd_b, msd_hist,
+                      mstart, static_cast<uint32_t *>(p), ctx->st, s);
       launch_histogram<K>(kA, n, hist, passes, s, ctx->st);
     }
   }
@@ -300,7 +300,8 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   const uint32_t *sortedv = lab ? (n <= 1 ? labels : (parity ? vB : vA)) : nullptr;
   {
     Stage t(ctx, "unique", s);
-    launch_unique<K>(sorted, sortedv, n, prepend, uniq, cbits, lb_u, ctx->st, s);
+    launch_unique<K>(sorted, sortedv, n, prepend, uniq, cbits, lb_u, ctx->st, s,
+                     msd_b ? ctx->st : nullptr);
   }
   CHK(launch_status("unique"));
   {

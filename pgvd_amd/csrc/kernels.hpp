@@ -41,7 +41,7 @@ struct RunState {
   uint32_t m;                // unique key count (incl. the padding zero)
   uint32_t octree_size;      // O = ps[m-1]
   uint32_t msd_ok;           // 1: the bucket sort (msd.hip) sorted the keys
-  uint32_t msd_nflag;        // buckets the bucket sort's fallback kernel sorts
+  uint32_t msd_dups;         // 1: the bucket sort saw equal keys (or a zero key to merge)
   uint32_t counters[kNumCounters];
 };
 
@@ -180,12 +180,14 @@ constexpr int kMsdCStride = PGVD_MSD_CSTRIDE;
 constexpr int kMsdCursorWords = kMsdMaxBuckets + 64 * kMsdCStride + 64 + 1;
 // b for n keys (0: no bucket sort for this size).
 int msd_bits_for(int64_t n);
-// keys -> alt (scatter) -> out (buckets sorted; out may be alt).  hist: the
-// bucket histogram (2^b counters, from launch_encode); start 2^b + 1 and
-// cursor kMsdCursorWords words of scratch.
-void launch_msd_sort(uint64_t *keys, uint64_t *alt, uint64_t *out, int64_t n, int mbits, int b,
-                     const uint32_t *hist, uint32_t *start, uint32_t *cursor, RunState *st,
-                     hipStream_t s);
+// keys -> alt -> keys (two scatters), then every bucket sorted into out
+// (may be keys); st->msd_dups = 1 when unique has work to do (equal keys,
+// or a zero key while a zero key is prepended).  hist: the bucket
+// histogram (2^b counters, from launch_encode); start 2^b + 1 and cursor
+// kMsdCursorWords words of scratch.
+void launch_msd_sort(uint64_t *keys, uint64_t *alt, uint64_t *out, int prepend, int64_t n,
+                     int mbits, int b, const uint32_t *hist, uint32_t *start, uint32_t *cursor,
+                     RunState *st, hipStream_t s);
 int64_t onesweep_tiles(int key_bytes, bool payload, int64_t n);
 int onesweep_tile_size(int key_bytes, bool payload);
 
@@ -194,9 +196,14 @@ constexpr int kScanTile = 4096;
 inline int64_t scan_tiles(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
 // Unique over [zero?] + keys[0..n): writes out[0..m) and st->m.  With labels,
 // sets bit r of conflict_bits for every run r whose labels are not all equal.
+// skip_if_msd (nullable): when skip_if_msd->msd_ok and not ->msd_dups (the
+// bucket sort found every key distinct and none to merge with the prepended
+// zero), the keys are already in place: only out[0] = 0 (prepend_zero) and
+// st->m = prepend_zero + n are written.
 template <typename K>
 void launch_unique(const K *keys, const uint32_t *labels, int64_t n, int prepend_zero, K *out,
-                   uint32_t *conflict_bits, uint64_t *lookback, RunState *st, hipStream_t s);
+                   uint32_t *conflict_bits, uint64_t *lookback, RunState *st, hipStream_t s,
+                   const RunState *skip_if_msd = nullptr);
 // Local splits (from the BRT) scanned inclusively into ps[0..m); rewrites
 // parent[] from claim form (claimer+1) to the reference form; writes ls[]
 // when non-null; st->octree_size = ps[m-1].

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void msd_plan_kernel(const uint32_t *__rest
   if (t == 0) {
     const uint32_t m4 = max(max(s_mx[0], s_mx[1]), max(s_mx[2], s_mx[3]));
     st->msd_ok = m4 <= (uint32_t)kMsdCap ? 1u : 0u;
-    st->msd_nflag = 0;
+    st->msd_dups = 0;
   }
   const int nb1 = nb / nb2;   // <= 64: wave 0, lane g = group g
   if (t < 64) {
@@ -179,33 +179,27 @@ __global__ __launch_bounds__(kBlock) void msd_split2_kernel(const uint64_t *__re
   split_tile(in, out, n, lo, hi, shift, nb2, cursor, (uint32_t)(g * nb2), 1);
 }
 
-// Sorts cnt <= 256 IPT keys of one bucket, in[s0 ..), on bits [0, 8 passes)
-// in LDS and writes them to out at the same place.  The workgroup holds 256
-// IPT keys: slots past the bucket hold ~0, which stays behind every real key
-// (a larger or equal digit in every pass, and later in the stable order).
-// Key index le = 64 IPT w + 64 j + lane lives in wave w, register j; each
+// Stable LSD passes, in LDS, over the BS IPT keys k[] of a workgroup: key
+// index le = 64 IPT w + 64 j + lane lives in wave w, register j, and each
 // pass ranks by the ballot matching of the onesweep tile, so it is stable.
-// The last pass ranks by its low last_bits digit bits only: above them
-// every real key has its bucket's bits, and ~0 falls in the top group,
-// behind the real keys there.
-template <int IPT>
-__device__ inline void bucket_sort(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
-                                   uint32_t s0, uint32_t cnt, int passes, int last_bits,
-                                   uint64_t *s_keys, uint32_t *s_cnt, uint32_t *s_dig,
-                                   uint32_t *s_w) {
+// On return s_keys[0 .. BS IPT) holds the keys in order of bits
+// [0, 8 (passes - 1) + last_bits).  The caller fills the slots past the
+// bucket with ~0 at the highest indices: ~0 then stays behind every real key
+// (a larger or equal digit in every pass, and later in the stable order).
+// s_cnt: NW 256 words, s_dig: 256 words, s_w: NW words.
+template <int BS, int IPT>
+__device__ inline void lds_radix_passes(uint64_t (&k)[IPT], int passes, int last_bits,
+                                        uint64_t *s_keys, uint32_t *s_cnt, uint32_t *s_dig,
+                                        uint32_t *s_w) {
+  constexpr int NW = BS / 64;
+  static_assert(64 * IPT < (1 << 16), "a rank fits 16 bits");
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const uint32_t wofs = (uint32_t)w * 64u * IPT;
-  uint64_t k[IPT];
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const uint32_t le = wofs + j * 64 + lane;
-    k[j] = le < cnt ? in[s0 + le] : ~0ull;
-  }
   for (int p = 0; p < passes; ++p) {
     const int sh = 8 * p;
     const int nbits = p == passes - 1 ? last_bits : 8;   // uniform
     const uint32_t dmask = (1u << nbits) - 1;
-    for (int i = t; i < 4 * 256; i += kBlock) s_cnt[i] = 0;
+    for (int i = t; i < NW * 256; i += BS) s_cnt[i] = 0;
     __syncthreads();
     uint32_t rk[IPT];   // rank within (wave, digit) | digit << 16
 #pragma unroll
@@ -228,16 +222,19 @@ __device__ inline void bucket_sort(const uint64_t *__restrict__ in, uint64_t *__
       s_cnt[w * 256 + d] = c + (uint32_t)(__popc(plo) + __popc(phi));
     }
     __syncthreads();
-    // thread t owns digit t: wave offsets, then the digit's start in the tile
+    // thread t < 256 owns digit t: wave offsets, then the digit's start
     uint32_t tot = 0;
+    if (t < 256) {
 #pragma unroll
-    for (int ww = 0; ww < 4; ++ww) {
-      const uint32_t c = s_cnt[ww * 256 + t];
-      s_cnt[ww * 256 + t] = tot;
-      tot += c;
+      for (int ww = 0; ww < NW; ++ww) {
+        const uint32_t c = s_cnt[ww * 256 + t];
+        s_cnt[ww * 256 + t] = tot;
+        tot += c;
+      }
     }
     uint32_t all;
-    s_dig[t] = block_excl_sum<uint32_t>(tot, s_w, &all);
+    const uint32_t dex = block_excl_sum<uint32_t, NW>(tot, s_w, &all);
+    if (t < 256) s_dig[t] = dex;
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
@@ -245,14 +242,11 @@ __device__ inline void bucket_sort(const uint64_t *__restrict__ in, uint64_t *__
       s_keys[(rk[j] & 0xffffu) + s_dig[d] + s_cnt[w * 256 + d]] = k[j];
     }
     __syncthreads();
+    if (p + 1 < passes) {
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) k[j] = s_keys[wofs + j * 64 + lane];
-    __syncthreads();   // s_keys and s_cnt are rewritten by the next pass
-  }
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const uint32_t le = wofs + j * 64 + lane;
-    if (le < cnt) out[s0 + le] = k[j];
+      for (int j = 0; j < IPT; ++j) k[j] = s_keys[wofs + j * 64 + lane];
+      __syncthreads();   // s_keys and s_cnt are rewritten by the next pass
+    }
   }
 }
 
@@ -261,51 +255,62 @@ constexpr int kSubBits = 12;
 constexpr int kSubBins = 1 << kSubBits;
 // Largest sub-bucket the counting pass finishes by insertion sort.
 constexpr uint32_t kSubMaxGroup = 32;
-// Threads of a counting-pass workgroup.  The LDS of one bucket (~57 KB)
-// allows two workgroups per CU, so the waves that hide the loads' latency
-// come from large workgroups.
+// Threads of a bucket workgroup.  The LDS of one bucket (~58 KB) allows two
+// workgroups per CU, so the waves that hide the loads' latency come from
+// large workgroups (and registers are capped so that two fit: BS/128 waves
+// per SIMD).
 #ifndef PGVD_MSD_BLOCK
 #define PGVD_MSD_BLOCK 1024
 #endif
-constexpr int kCountBlock = PGVD_MSD_BLOCK;
-// Workgroups of the fallback kernel; each loops over the flagged buckets.
-constexpr int kFallbackGrid = 512;
+constexpr int kBucketBlock = PGVD_MSD_BLOCK;
+#ifndef PGVD_MSD_WPE
+#define PGVD_MSD_WPE (PGVD_MSD_BLOCK / 128)
+#endif
 
-// One workgroup per bucket (cnt <= kMsdCap keys in[s0 ..), all sharing the
-// bits from `shift` up).  Counting pass: every key gets a slot in its
-// sub-bucket (its next 12 bits, i.e. bits shift-12 .. shift-1) from an LDS
-// atomic, in any order -- keys without payload need no stable order -- and
-// each sub-bucket, ~1 key for uniform keys, is then put in order by an
-// insertion sort on the whole key.  A bucket with a sub-bucket of more than
-// kSubMaxGroup keys (clustered keys) is appended to `flagged` and left to
-// msd_fallback_kernel.
+// One workgroup per bucket: bucket b holds cnt <= kMsdCap keys in[s0 ..),
+// all sharing the bits from `shift` up; they are sorted in LDS and written
+// to out[s0 ..).  Equal keys share a bucket (and a sub-bucket), so the
+// workgroup also sees every duplicate key: when it finds one -- or the
+// zero key at the front while a virtual zero key is prepended (prepend),
+// which UniqueSorted would merge -- it sets st->msd_dups, and only then does
+// unique_kernel run over the sorted keys.
+//
+// Sorting: every key gets a slot in its sub-bucket (its next 12 bits, bits
+// shift-12 .. shift-1) from an LDS atomic, in any order -- keys without
+// payload need no stable order -- and each sub-bucket, ~1 key for uniform
+// keys, is put in order by an insertion sort on the whole key.  A bucket
+// with a sub-bucket of more than kSubMaxGroup keys (clustered keys) is
+// sorted by stable LDS radix passes on its low `shift` bits instead.
 template <int BS>
-__global__ __launch_bounds__(BS) void msd_bucket_kernel(const uint64_t *__restrict__ in,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PGVD_MSD_WPE))) void
+msd_bucket_kernel(const uint64_t *__restrict__ in,
                                                         uint64_t *__restrict__ out,
                                                         const uint32_t *__restrict__ start,
-                                                        int shift, uint32_t *__restrict__ flagged,
-                                                        RunState *st) {
-  constexpr int IPT = (kMsdCap + BS - 1) / BS;
+                                                        int shift, int passes, int last_bits,
+                                                        uint32_t prepend, RunState *st) {
+  constexpr int IPT = kMsdCap / BS;
+  static_assert(IPT * BS == kMsdCap, "the workgroup holds a full bucket");
   constexpr int NW = BS / 64;
   constexpr int PER = kSubBins / BS;   // consecutive sub-buckets per thread
+  static_assert(NW * 256 <= kSubBins, "the radix counters fit the sub-bucket space");
   if (!st->msd_ok) return;
+  __shared__ uint64_t s_keys[kMsdCap];
+  __shared__ uint32_t s_sub[kSubBins + 1];   // counts, then starts; radix counters
+  __shared__ uint32_t s_dig[256], s_w[NW], s_mx[NW];
+  const int t = threadIdx.x;
   const uint32_t s0 = start[blockIdx.x];
   const uint32_t cnt = min(start[blockIdx.x + 1] - s0, (uint32_t)kMsdCap);
   if (cnt == 0) return;
-  __shared__ uint64_t s_keys[kMsdCap];
-  __shared__ uint32_t s_sub[kSubBins + 1];   // counts, then starts
-  __shared__ uint32_t s_w[NW], s_mx[NW];
-  const int t = threadIdx.x;
-  const int sub_shift = shift - kSubBits;
   for (int i = t; i < kSubBins; i += BS) s_sub[i] = 0;
+  __syncthreads();
+  const int sub_shift = shift - kSubBits;
   uint64_t k[IPT];
   uint32_t rs[IPT];   // slot in sub-bucket | sub-bucket << 16
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t i = j * BS + t;
-    k[j] = in[s0 + (i < cnt ? i : 0u)];
+    k[j] = i < cnt ? in[s0 + i] : 0ull;
   }
-  __syncthreads();
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t i = j * BS + t;
@@ -330,72 +335,62 @@ __global__ __launch_bounds__(BS) void msd_bucket_kernel(const uint64_t *__restri
   for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
   if ((t & 63) == 0) s_mx[t >> 6] = mx;
   __syncthreads();   // every thread has read its counts
+  uint32_t gmax = (t & 63) < NW ? s_mx[t & 63] : 0u;   // uniform after the reduction
 #pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    s_sub[t * PER + i] = ex;
-    ex += cp[i];
-  }
-  if (t == 0) s_sub[kSubBins] = total;
-  uint32_t gmax = 0;   // uniform
+  for (int d = 32; d >= 1; d >>= 1) gmax = max(gmax, (uint32_t)__shfl_xor((int)gmax, d, 64));
+  if (gmax <= kSubMaxGroup) {
 #pragma unroll
-  for (int w = 0; w < NW; ++w) gmax = max(gmax, s_mx[w]);
-  if (gmax > kSubMaxGroup) {
-    if (t == 0) flagged[atomicAdd(&st->msd_nflag, 1u)] = blockIdx.x;
-    return;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const uint32_t i = j * BS + t;
-    if (i < cnt) s_keys[s_sub[rs[j] >> 16] + (rs[j] & 0xffffu)] = k[j];
-  }
-  __syncthreads();
-  // thread t puts its sub-buckets in order (insertion sort in LDS)
-  uint32_t b0 = s_sub[t * PER];
-  for (int i = 0; i < PER; ++i) {
-    const uint32_t b1 = s_sub[t * PER + i + 1];
-    for (uint32_t x = b0 + 1; x < b1; ++x) {
-      const uint64_t v = s_keys[x];
-      uint32_t y = x;
-      while (y > b0 && s_keys[y - 1] > v) {
-        s_keys[y] = s_keys[y - 1];
-        --y;
-      }
-      s_keys[y] = v;
+    for (int i = 0; i < PER; ++i) {
+      s_sub[t * PER + i] = ex;
+      ex += cp[i];
     }
-    b0 = b1;
+    if (t == 0) s_sub[kSubBins] = total;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t i = j * BS + t;
+      if (i < cnt) s_keys[s_sub[rs[j] >> 16] + (rs[j] & 0xffffu)] = k[j];
+    }
+    __syncthreads();
+    // thread t puts its sub-buckets in order (insertion sort in LDS)
+    uint32_t b0 = s_sub[t * PER];
+    for (int i = 0; i < PER; ++i) {
+      const uint32_t b1 = s_sub[t * PER + i + 1];
+      for (uint32_t x = b0 + 1; x < b1; ++x) {
+        const uint64_t v = s_keys[x];
+        uint32_t y = x;
+        while (y > b0 && s_keys[y - 1] > v) {
+          s_keys[y] = s_keys[y - 1];
+          --y;
+        }
+        s_keys[y] = v;
+      }
+      b0 = b1;
+    }
+  } else {
+    // keys in the radix passes' layout, the slots past the bucket last
+    const int lane = t & 63, w = t >> 6;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t le = (uint32_t)w * 64u * IPT + j * 64 + lane;
+      k[j] = le < cnt ? in[s0 + le] : ~0ull;
+    }
+    lds_radix_passes<BS, IPT>(k, passes, last_bits, s_keys, s_sub, s_dig, s_w);
   }
   __syncthreads();
+  // write out, looking for equal neighbours
+  bool dup = false;
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t i = j * BS + t;
-    if (i < cnt) out[s0 + i] = s_keys[i];
+    if (i < cnt) {
+      const uint64_t v = s_keys[i];
+      dup |= i > 0 ? v == s_keys[i - 1] : (s0 == 0 && prepend && v == 0ull);
+      out[s0 + i] = v;
+    }
   }
-}
-
-// The buckets msd_bucket_kernel flagged (a sub-bucket of more than
-// kSubMaxGroup keys), sorted by the stable in-LDS radix passes with the
-// smallest of 12, 16 or 20 keys per thread that holds the bucket.  Every
-// workgroup loops over the list; with no flagged bucket it does nothing.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void
-msd_fallback_kernel(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
-                    const uint32_t *__restrict__ start, const uint32_t *__restrict__ flagged,
-                    int passes, int last_bits, const RunState *st) {
-  if (!st->msd_ok) return;
-  const uint32_t nf = st->msd_nflag;
-  __shared__ uint64_t s_keys[kMsdCap];
-  __shared__ uint32_t s_cnt[4 * 256], s_dig[256], s_w[4];
-  for (uint32_t f = blockIdx.x; f < nf; f += gridDim.x) {
-    const uint32_t b = flagged[f];
-    const uint32_t s0 = start[b];
-    const uint32_t cnt = min(start[b + 1] - s0, (uint32_t)kMsdCap);
-    if (cnt <= 12 * kBlock)
-      bucket_sort<12>(in, out, s0, cnt, passes, last_bits, s_keys, s_cnt, s_dig, s_w);
-    else if (cnt <= 16 * kBlock)
-      bucket_sort<16>(in, out, s0, cnt, passes, last_bits, s_keys, s_cnt, s_dig, s_w);
-    else
-      bucket_sort<20>(in, out, s0, cnt, passes, last_bits, s_keys, s_cnt, s_dig, s_w);
-  }
+  if (__any(dup) && (t & 63) == 0)
+    __hip_atomic_store(&st->msd_dups, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace
@@ -407,9 +402,9 @@ int msd_bits_for(int64_t n) {
   return 0;
 }
 
-void launch_msd_sort(uint64_t *keys, uint64_t *alt, uint64_t *out, int64_t n, int mbits, int b,
-                     const uint32_t *hist, uint32_t *start, uint32_t *cursor, RunState *st,
-                     hipStream_t s) {
+void launch_msd_sort(uint64_t *keys, uint64_t *alt, uint64_t *out, int prepend, int64_t n,
+                     int mbits, int b, const uint32_t *hist, uint32_t *start, uint32_t *cursor,
+                     RunState *st, hipStream_t s) {
   const int nb = 1 << b, b1 = b / 2, b2 = b - b1;
   const int nb1 = 1 << b1, nb2 = 1 << b2;
   const int shift = mbits - b;             // bits each bucket is sorted on
@@ -422,10 +417,8 @@ void launch_msd_sort(uint64_t *keys, uint64_t *alt, uint64_t *out, int64_t n, in
                                                               nb1, start, cursor, st);
   const int passes = (shift + 7) / 8;
   const int last_bits = shift - 8 * (passes - 1);
-  msd_bucket_kernel<kCountBlock><<<(unsigned)nb, kCountBlock, 0, s>>>(keys, out, start, shift,
-                                                                      cursor, st);
-  msd_fallback_kernel<<<kFallbackGrid, kBlock, 0, s>>>(keys, out, start, cursor, passes,
-                                                       last_bits, st);
+  msd_bucket_kernel<kBucketBlock><<<(unsigned)nb, kBucketBlock, 0, s>>>(
+      keys, out, start, shift, passes, last_bits, (uint32_t)prepend, st);
 }
 
 }  // namespace pgvd

@@ -115,7 +115,16 @@ __global__ __launch_bounds__(kBlock) void unique_kernel(const K *__restrict__ ke
                                                         uint32_t n, int prepend_zero,
                                                         K *__restrict__ out,
                                                         uint32_t *__restrict__ conflict_bits,
-                                                        uint64_t *lookback, RunState *st) {
+                                                        uint64_t *lookback, RunState *st,
+                                                        const RunState *skip) {
+  if (skip && skip->msd_ok && !skip->msd_dups) {
+    // distinct sorted keys, already where they go (see unique_tile)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (prepend_zero) out[0] = kzero_value<K>();
+      st->m = (uint32_t)prepend_zero + n;
+    }
+    return;
+  }
   __shared__ uint32_t s_tile;
   if (threadIdx.x == 0) s_tile = atomicAdd(&st->counters[CTR_UNIQUE], 1u);
   __syncthreads();
@@ -363,16 +372,17 @@ void launch_chunk_scan(uint32_t *chunk_tot, int64_t max_m, uint64_t *lookback, R
 
 template <typename K>
 void launch_unique(const K *keys, const uint32_t *labels, int64_t n, int prepend_zero, K *out,
-                   uint32_t *conflict_bits, uint64_t *lookback, RunState *st, hipStream_t s) {
+                   uint32_t *conflict_bits, uint64_t *lookback, RunState *st, hipStream_t s,
+                   const RunState *skip_if_msd) {
   const int64_t tiles = scan_tiles(n > 0 ? n : 1);
   if (labels)
     unique_kernel<K, true><<<(unsigned)tiles, kBlock, 0, s>>>(keys, labels, (uint32_t)n,
                                                               prepend_zero, out, conflict_bits,
-                                                              lookback, st);
+                                                              lookback, st, skip_if_msd);
   else
     unique_kernel<K, false><<<(unsigned)tiles, kBlock, 0, s>>>(keys, nullptr, (uint32_t)n,
                                                                prepend_zero, out, nullptr,
-                                                               lookback, st);
+                                                               lookback, st, skip_if_msd);
 }
 
 template <typename K>
@@ -412,7 +422,7 @@ void launch_oct_offsets(const BatchDev &b, RunState *st, hipStream_t s) {
 
 #define PGVD_INST(K)                                                                          \
   template void launch_unique<K>(const K *, const uint32_t *, int64_t, int, K *, uint32_t *,   \
-                                 uint64_t *, RunState *, hipStream_t);                         \
+                                 uint64_t *, RunState *, hipStream_t, const RunState *);       \
   template void launch_splits_scan<K>(const BrtNodeT<K> *, int32_t *, uint32_t *, uint32_t *,  \
                                       int64_t, int, int, uint64_t *, RunState *, hipStream_t);  \
   template void launch_unique_seg<K>(const K *, K *, const BatchDev &, uint64_t *, RunState *, \

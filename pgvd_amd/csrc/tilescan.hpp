@@ -1,5 +1,6 @@
 // tilescan.hpp -- single-pass decoupled look-back building blocks shared by
-// the scan and unique kernels (256-thread workgroups, wave64).
+// the scan and unique kernels (wave64; 256-thread workgroups unless a
+// template parameter says otherwise).
 //
 // A tile's status word is 64 bits: flag in bits 62-63 (1 = aggregate only,
 // 2 = inclusive prefix), value in bits 0-61.  Status arrays must be zero
@@ -17,14 +18,16 @@ constexpr uint64_t kSVal = (1ull << 62) - 1;
 
 // Publishes the tile aggregate, resolves the tile's exclusive prefix within
 // the run of tiles starting at `lo` and publishes its inclusive prefix;
-// returns the exclusive prefix to every thread.  All 256 threads of the
-// workgroup call it.  Each round reads the status of the 256 nearest
+// returns the exclusive prefix to every thread.  All BS threads of the
+// workgroup call it.  Each round reads the status of the BS nearest
 // unresolved predecessors at once (one per thread) and stops at the nearest
 // inclusive one, so a tile k tiles past the last resolved one waits about
-// k/256 round trips.
+// k/BS round trips.
+template <int BS = kBlock>
 __device__ inline uint64_t tile_prefix(uint64_t *status, int tile, uint64_t total, int lo = 0) {
-  __shared__ int s_first[4];
-  __shared__ uint64_t s_part[4];
+  constexpr int NW = BS / 64;
+  __shared__ int s_first[NW];
+  __shared__ uint64_t s_part[NW];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   if (t == 0) st_rel_u64(status + tile, (tile == lo ? kSIncl : kSAgg) | total);
   uint64_t excl = 0;
@@ -43,17 +46,28 @@ __device__ inline uint64_t tile_prefix(uint64_t *status, int tile, uint64_t tota
       const uint64_t b = __ballot((wv & ~kSVal) == kSIncl);
       if (lane == 0) s_first[w] = b ? __ffsll((long long)b) - 1 : 64;
       __syncthreads();
-      int first = kBlock;
-      for (int j = 3; j >= 0; --j)
-        if (s_first[j] < 64) first = j * 64 + s_first[j];
+      int first = BS;
+      if constexpr (NW <= 4) {
+        for (int j = NW - 1; j >= 0; --j)
+          if (s_first[j] < 64) first = j * 64 + s_first[j];
+      } else {
+        const int f = lane < NW ? s_first[lane] : 64;
+        first = f < 64 ? lane * 64 + f : BS;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) first = min(first, __shfl_xor(first, d, 64));
+      }
       uint64_t val = (t <= first) ? (wv & kSVal) : 0;
       val = wave_reduce_add(val);
       if (lane == 0) s_part[w] = val;
       __syncthreads();
-      excl += s_part[0] + s_part[1] + s_part[2] + s_part[3];
+      if constexpr (NW <= 4) {
+        for (int j = 0; j < NW; ++j) excl += s_part[j];
+      } else {
+        excl += wave_reduce_add<uint64_t>(lane < NW ? s_part[lane] : 0ull);
+      }
       __syncthreads();  // s_first / s_part are reused by the next round
-      if (first < kBlock) break;
-      base -= kBlock;
+      if (first < BS) break;
+      base -= BS;
     }
     if (t == 0) st_rel_u64(status + tile, kSIncl | (excl + total));
   }
@@ -70,11 +84,19 @@ __device__ inline T block_excl_sum(T x, T *s_w, T *total) {
   if (lane == 63) s_w[w] = incl;
   __syncthreads();
   T before = 0, all = 0;
+  if constexpr (NW <= 4) {
 #pragma unroll
-  for (int j = 0; j < NW; ++j) {
-    const T y = s_w[j];
-    if (j < w) before += y;
-    all += y;
+    for (int j = 0; j < NW; ++j) {
+      const T y = s_w[j];
+      if (j < w) before += y;
+      all += y;
+    }
+  } else {
+    // one wave total per lane, scanned across the wave (few registers)
+    const T y = wave_incl_scan_add(lane < NW ? s_w[lane] : T(0));
+    all = __shfl(y, NW - 1, 64);
+    const T b = __shfl(y, w > 0 ? w - 1 : 0, 64);
+    before = w > 0 ? b : T(0);
   }
   *total = all;
   return before + incl - x;

@@ -181,6 +181,13 @@ def test_bucket_sort_and_its_fallback(dev):
     got = dev.build_octree(dup, 21)
     assert got["bucket_sort"] == 1
     assert_tree_equal(got, oracle.build(oracle.quantize(dup, 1 << 21), 21), where="dup buckets")
+    # a point on the box's minimum corner has the zero key, which merges with
+    # the zero key prepended at this size (n is not a power of two)
+    corner = pts.copy()
+    corner[5] = pts.min(axis=0)
+    got = dev.build_octree(corner, 21)
+    assert got["bucket_sort"] == 1 and got["keys_lo"][0] == 0
+    assert_tree_equal(got, oracle.build(oracle.quantize(corner, 1 << 21), 21), where="zero key")
     clustered = pts.copy()
     clustered[: n // 4] *= np.float32(1e-3)         # a quarter of the points near the centre
     got = dev.build_octree(clustered, 21)
