@@ -51,12 +51,14 @@ def make_points(workload: str, n: int) -> tuple[np.ndarray, str]:
 
 
 def algorithmic_bytes(n: int, m: int, O: int, dim: int, K: int, R: int, L: int = 0,
-                      bucket_sort: bool = False) -> dict:
+                      bucket_sort: bool = False, prepend: int = 0) -> dict:
     """Per-stage algorithmic HBM bytes of the path as built (P point bytes,
     K key bytes, L label bytes, R radix passes at 8-bit digits, B = 16+K BRT
     node bytes, N = 4*(2^dim+1) octree node bytes).  The sort is R read+write
     passes over keys and labels, or with the bucket sort two scatters and the
-    bucket pass over the keys.  The BRT pass writes records, parent links and
+    bucket pass over the keys; the bucket pass also finds whether any two
+    keys are equal, so when none are (m = n + prepend) unique moves nothing.
+    The BRT pass writes records, parent links and
     packed local splits; the gather reads records and local splits and writes
     prefix sums and octree nodes."""
     P = 4 * dim
@@ -66,7 +68,7 @@ def algorithmic_bytes(n: int, m: int, O: int, dim: int, K: int, R: int, L: int =
         "bbox": P * n,
         "encode": (P + K + L) * n,
         "sort": 3 * 2 * K * n if bucket_sort else 2 * R * (K + L) * n,
-        "unique": (K + L) * n + K * m,
+        "unique": 0 if bucket_sort and m == n + prepend else (K + L) * n + K * m,
         "brt": K * m + B * (m - 1) + 8 * m,
         "octree": (B + 4) * (m - 1) + 4 * m + N * O,
     }
@@ -446,7 +448,8 @@ def main() -> int:
 
     m, O = int(res.m), int(res.octree_size)
     lb = 4 if conflict else 0
-    ab = algorithmic_bytes(n, m, O, dim, int(res.key_bytes), passes, lb, bucket)
+    prepend = 0 if n & (n - 1) == 0 else 1     # zero-key padding below the next power of two
+    ab = algorithmic_bytes(n, m, O, dim, int(res.key_bytes), passes, lb, bucket, prepend)
     if conflict:      # the conflict-flag term of the byte model: labels read + m/8 flag bits
         ab["total"] += lb * n + m // 8
     dom_key = "gather" if bucket else "sort_pass"

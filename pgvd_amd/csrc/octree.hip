@@ -82,7 +82,7 @@ __device__ inline void write_node(int32_t *octree, uint32_t idx, const int32_t *
 //   child(c, ps_c, lsx_c)  -> false if c's entries are unavailable
 // Returns false, writing nothing, when a read was unavailable.
 template <typename K, int DIM, typename Links, typename Child, typename Put>
-__device__ inline bool gather_one(uint32_t q, const BrtNodeT<K> &nq, uint32_t excl, uint32_t lsq,
+__device__ __forceinline__ bool gather_one(uint32_t q, const BrtNodeT<K> &nq, uint32_t excl, uint32_t lsq,
                                   Links links, Child child, const K *__restrict__ keys,
                                   const uint32_t *__restrict__ conflict_bits, int mbits, Put put,
                                   uint32_t *__restrict__ node_conflict, RunState *st) {
@@ -143,7 +143,7 @@ __device__ inline bool gather_one(uint32_t q, const BrtNodeT<K> &nq, uint32_t ex
 // its octant) or a same-level node, whose two children become slots of the
 // next round.  All LDS reads of a round are independent.
 template <typename K, int DIM, int W, bool LAST>
-__device__ inline void bfs_round(const uint32_t (&idx)[W], const bool (&lf)[W],
+__device__ __forceinline__ void bfs_round(const uint32_t (&idx)[W], const bool (&lf)[W],
                                  const bool (&live)[W], uint32_t (&nidx)[2 * W],
                                  bool (&nlf)[2 * W], bool (&nlive)[2 * W], int w0, int wn,
                                  const uint32_t *s_lnk, const uint32_t *s_ps,
@@ -162,11 +162,11 @@ __device__ inline void bfs_round(const uint32_t (&idx)[W], const bool (&lf)[W],
     const uint32_t lx = s_lsx[kk];
     const bool term = use && (lx & 0xffu) != 0;
     const bool expand = use && (lx & 0xffu) == 0;
-    if (term) {
-      const uint32_t o = lx >> 8;
-      node0[o] = (int32_t)(s_ps[kk] - 1);   // the child chain's top, in its octant
-      found |= 1u << o;
-    }
+    // the child chain's top, in its octant; otherwise a dummy store to the
+    // leaf word, which is written last
+    const uint32_t o = term ? lx >> 8 : (uint32_t)(1 << DIM);
+    node0[o] = (int32_t)(s_ps[kk] - 1);
+    found |= term ? 1u << o : 0u;
     if (live[s] && lf[s] && conflict_bits && test_bit(conflict_bits, idx[s]))
       cmask |= 1u << digit_at<K, DIM>(keys[idx[s]], lvl, mbits);
     if (LAST) {
@@ -185,17 +185,19 @@ __device__ inline void bfs_round(const uint32_t (&idx)[W], const bool (&lf)[W],
   }
 }
 
-// gather_one for the window pass with its output staged in LDS (the
-// workgroup has filled the staged nodes with -1 children and full leaf
-// masks, so only found children and leaf words are written).  The
+// gather_one for the window pass with its output staged in LDS: q's nodes
+// node0[0 .. lsq STRIDE) are written whole here (children -1 unless found,
+// leaf masks).  The
 // same-level BRT nodes below q split on later bits of the digit q splits,
 // so they form a binary trie of fewer than DIM levels: DIM rounds over
 // fixed slots (2, 4, 8) find every child chain without a loop or stack.
-// node0 is q's lowest staged node.  Returns false when an internal node of
-// that trie lies outside the window; the global-memory pass then rewrites
-// q's nodes whole.
+// node0 is q's lowest staged node; lk is q's packed child links.  q's record
+// is read from brt only for a chain (lsq > 1) or the conflict octants.
+// Returns false when an internal node of that trie lies outside the window;
+// the global-memory pass then rewrites q's nodes whole.
 template <typename K, int DIM>
-__device__ inline bool gather_one_bfs(uint32_t q, const BrtNodeT<K> &nq, uint32_t excl,
+__device__ __forceinline__ bool gather_one_bfs(uint32_t q, uint32_t lk,
+                                      const BrtNodeT<K> *__restrict__ brt, uint32_t excl,
                                       uint32_t lsq, int w0, int wn, const uint32_t *s_lnk,
                                       const uint32_t *s_ps, const uint32_t *s_lsx,
                                       const K *__restrict__ keys,
@@ -205,11 +207,14 @@ __device__ inline bool gather_one_bfs(uint32_t q, const BrtNodeT<K> &nq, uint32_
   constexpr int NC = 1 << DIM;
   constexpr int STRIDE = NC + 1;
   constexpr int32_t kFull = (1 << NC) - 1;
-  const int lvl = nq.lcp_length / DIM;
+  const int lvl = conflict_bits ? brt[q].lcp_length / DIM : 0;
   uint32_t found = 0, cmask = 0;
   bool miss = false;
-  const uint32_t i1[2] = {(uint32_t)nq.left, (uint32_t)nq.left + 1};
-  const bool f1[2] = {(bool)nq.left_leaf, (bool)nq.right_leaf};
+#pragma unroll
+  for (int c = 0; c < NC; ++c) node0[c] = -1;
+  const uint32_t left = lk & 0x3fffffffu;
+  const uint32_t i1[2] = {left, left + 1};
+  const bool f1[2] = {((lk >> 30) & 1u) != 0u, (lk >> 31) != 0u};
   const bool l1[2] = {true, true};
   uint32_t i2[4];
   bool f2[4], l2[4];
@@ -228,14 +233,20 @@ __device__ inline bool gather_one_bfs(uint32_t q, const BrtNodeT<K> &nq, uint32_
   if (miss) return false;
   node0[NC] = kFull & ~(int32_t)found;
   if (node_conflict) node_conflict[excl] = cmask;
-  for (uint32_t k = 1; k < lsq; ++k) {
-    if (q != 0) {
-      const int o = quadrant<K, DIM>(nq, (int)k - 1);
+  if (lsq > 1) {
+    const BrtNodeT<K> nq = brt[q];
+    for (uint32_t k = 1; k < lsq; ++k) {
       int32_t *nd = node0 + k * STRIDE;
-      nd[o] = (int32_t)(excl + k - 1);
-      nd[NC] = kFull & ~(1 << o);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) nd[c] = -1;
+      nd[NC] = kFull;
+      if (q != 0) {
+        const int o = quadrant<K, DIM>(nq, (int)k - 1);
+        nd[o] = (int32_t)(excl + k - 1);
+        nd[NC] = kFull & ~(1 << o);
+      }
+      if (node_conflict) node_conflict[excl + k] = 0;
     }
-    if (node_conflict) node_conflict[excl + k] = 0;
   }
   return true;
 }
@@ -276,18 +287,14 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
   constexpr int STRIDE = NC + 1;
   constexpr int H = kGatherHalo, WR = R + 2, WN = WR * kBlock, STAGE = R * kBlock;
   static_assert(H == kBlock, "the window is whole chunks of 256");
-  __shared__ int32_t s_nodes[STAGE * STRIDE];
+  // staged octree words, 16-byte aligned, plus up to 3 words in front so
+  // that LDS and global words agree modulo 4 (16-byte copies out)
+  __shared__ __align__(16) int32_t s_nodes[(STAGE * STRIDE + 3 + 3) & ~3];
   __shared__ uint32_t s_lnk[WN], s_ps[WN], s_lsx[WN];
   const int t = threadIdx.x;
   const int w0 = (int)q0 - H;
   // Every load of the window is issued before the first is used: addresses
   // outside [0, m) are clamped to entry 0 and the values dropped.
-  BrtNodeT<K> nq[R];
-#pragma unroll
-  for (int j = 0; j < R; ++j) {
-    const uint32_t q = q0 + j * kBlock + t;
-    nq[j] = brt[q <= m - 2 ? q : 0u];
-  }
   uint32_t lx[WR], lk[WR], pv[WR];
 #pragma unroll
   for (int r = 0; r < WR; ++r) {
@@ -331,27 +338,41 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
   const uint32_t lo = q0 ? s_ps[q0 - 1 - w0] : 0u;
   const uint32_t span = s_ps[qlast - w0] - lo;
   const bool staged = span <= (uint32_t)STAGE;   // uniform over the workgroup
-  if (staged) {
-    // every staged node starts with no children and a full leaf mask
-    for (uint32_t w = t; w < span * STRIDE; w += kBlock)
-      s_nodes[w] = (w % STRIDE == (uint32_t)NC) ? (1 << NC) - 1 : -1;
-    __syncthreads();
-  }
+  const uint32_t a0 = (uint32_t)(((uint64_t)lo * STRIDE) & 3u);
+  // The nodes that own octree nodes (about half), compacted so that each
+  // lane takes one: entry e = j * 256 + t of the rows.
+  __shared__ uint16_t s_aidx[STAGE];
+  __shared__ uint32_t s_miss[STAGE / 32], s_w4[4];
+  if (t < STAGE / 32) s_miss[t] = 0;
+  uint32_t amask = 0;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const uint32_t q = q0 + j * kBlock + t;
-    bool active = q <= qlast, miss = false;
-    uint32_t excl = 0, lsq = 0;
+    bool active = q <= qlast;
     if (active) {
-      excl = q ? s_ps[q - 1 - w0] : 0u;
-      lsq = s_ps[q - w0] - excl;
-      active = (q == 0) || lsq > 0;
+      const uint32_t excl = q ? s_ps[q - 1 - w0] : 0u;
+      active = (q == 0) || s_ps[q - w0] > excl;
     }
-    if (active && staged) {
-      miss = !gather_one_bfs<K, DIM>(q, nq[j], excl, lsq, w0, WN, s_lnk, s_ps, s_lsx, keys,
-                                     conflict_bits, mbits, s_nodes + (excl - lo) * STRIDE,
-                                     node_conflict, st);
-    } else if (active) {
+    amask |= (uint32_t)active << j;
+  }
+  uint32_t nact;
+  uint32_t pos = block_excl_sum<uint32_t>((uint32_t)__popc(amask), s_w4, &nact);
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+    if ((amask >> j) & 1u) s_aidx[pos++] = (uint16_t)(j * kBlock + t);
+  __syncthreads();
+  for (uint32_t a = t; a < nact; a += kBlock) {
+    const uint32_t e = s_aidx[a];
+    const uint32_t q = q0 + e;
+    const uint32_t excl = q ? s_ps[q - 1 - w0] : 0u;
+    const uint32_t lsq = s_ps[q - w0] - excl;
+    bool ok;
+    if (staged) {
+      ok = gather_one_bfs<K, DIM>(q, s_lnk[q - w0], brt, excl, lsq, w0, WN, s_lnk, s_ps, s_lsx,
+                                  keys,
+                                  conflict_bits, mbits, s_nodes + a0 + (excl - lo) * STRIDE,
+                                  node_conflict, st);
+    } else {
       auto links = [&](uint32_t x, uint32_t &left, bool &ll, bool &rl) -> bool {
         const unsigned k = (unsigned)((int)x - w0);
         if (k >= (unsigned)WN) return false;
@@ -369,16 +390,31 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
         return true;
       };
       auto put = [&](uint32_t idx, const int32_t *c) { write_node<DIM>(octree, idx, c); };
-      miss = !gather_one<K, DIM>(q, nq[j], excl, lsq, links, child, keys, conflict_bits, mbits,
-                                 put, node_conflict, st);
+      ok = gather_one<K, DIM>(q, brt[q], excl, lsq, links, child, keys, conflict_bits, mbits,
+                              put, node_conflict, st);
     }
-    defer(miss, q);
+    if (!ok) atomicOr(&s_miss[e >> 5], 1u << (e & 31));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const uint32_t e = j * kBlock + t;
+    defer(((s_miss[e >> 5] >> (e & 31)) & 1u) != 0u, q0 + e);
   }
   if (staged) {
-    __syncthreads();
+    // LDS words [a0, a0 + words) -> octree words [lo STRIDE, ..): whole
+    // 16-byte chunks as one store, the partial first and last chunks by word
     const uint32_t words = span * STRIDE;
-    int32_t *out = octree + (size_t)lo * STRIDE;
-    for (uint32_t w = t; w < words; w += kBlock) out[w] = s_nodes[w];
+    int32_t *out = octree + (size_t)lo * STRIDE - a0;   // out[i] <-> s_nodes[i]
+    const uint32_t end = a0 + words, c0 = a0 >> 2, c1 = (end + 3) >> 2;
+    for (uint32_t c = c0 + t; c < c1; c += kBlock) {
+      const uint32_t i = c * 4;
+      if (i >= a0 && i + 4 <= end) {
+        *reinterpret_cast<int4 *>(out + i) = *reinterpret_cast<const int4 *>(s_nodes + i);
+      } else {
+        for (uint32_t k = max(i, a0); k < min(i + 4, end); ++k) out[k] = s_nodes[k];
+      }
+    }
   }
 }
 
