@@ -9,8 +9,8 @@ radix sort, unique, Karras BRT with local splits, octree fill) over 2^24
 float3 points that are already in HBM.  Prints one JSON line: sorted Morton
 keys/s (whole job), the per-launch roofline of the dominant kernel (the
 octree gather when the bucket sort ran, else an onesweep sort pass), the
-whole-path algorithmic-byte roofline, and the serial CPU path (the oracle,
-one core) timed on a bounded sample.
+whole-path roofline against the survey's algorithmic byte model, and the
+serial CPU path (the oracle, one core) timed on a bounded sample.
 """
 from __future__ import annotations
 
@@ -50,31 +50,32 @@ def make_points(workload: str, n: int) -> tuple[np.ndarray, str]:
     raise SystemExit(f"unknown workload {workload}")
 
 
-def algorithmic_bytes(n: int, m: int, O: int, dim: int, K: int, R: int, L: int = 0,
-                      bucket_sort: bool = False, prepend: int = 0) -> dict:
-    """Per-stage algorithmic HBM bytes of the path as built (P point bytes,
-    K key bytes, L label bytes, R radix passes at 8-bit digits, B = 16+K BRT
-    node bytes, N = 4*(2^dim+1) octree node bytes).  The sort is R read+write
-    passes over keys and labels, or with the bucket sort two scatters and the
-    bucket pass over the keys; the bucket pass also finds whether any two
-    keys are equal, so when none are (m = n + prepend) unique moves nothing.
-    The BRT pass writes records, parent links and
-    packed local splits; the gather reads records and local splits and writes
-    prefix sums and octree nodes."""
+def algorithmic_bytes(n: int, m: int, O: int, dim: int, K: int, R: int, L: int = 0) -> dict:
+    """Algorithmic HBM bytes of the path, term by term as the survey's byte
+    model fixes them (whatever the kernels actually move): P = 4 dim point
+    bytes, K key bytes, L label bytes, R = ceil(mbits/8) radix passes (the
+    8-bit-digit convention, also when the bucket sort sorts in fewer passes
+    over HBM), B = 16 + K compact BRT node bytes, N = 4 (2^dim + 1) octree
+    node bytes.  The gather kernel performs the scan of local splits and the
+    octree fill, so its figure is those two terms."""
     P = 4 * dim
     B = 16 + K
     N = 4 * ((1 << dim) + 1)
     parts = {
         "bbox": P * n,
         "encode": (P + K + L) * n,
-        "sort": 3 * 2 * K * n if bucket_sort else 2 * R * (K + L) * n,
-        "unique": 0 if bucket_sort and m == n + prepend else (K + L) * n + K * m,
-        "brt": K * m + B * (m - 1) + 8 * m,
-        "octree": (B + 4) * (m - 1) + 4 * m + N * O,
+        "sort": 2 * R * (K + L) * n,
+        "unique": (K + L) * n + K * m,
+        "brt": K * m + B * (m - 1),
+        "splits": B * (m - 1) + 4 * m,
+        "scan": 8 * m,
+        "octree": (B + 8) * (m - 1) + N * O,
     }
+    if L:
+        parts["conflict"] = L * n + m // 8
     parts["total"] = sum(parts.values())
     parts["sort_pass"] = 2 * (K + L) * n
-    parts["gather"] = parts["octree"]
+    parts["gather"] = parts["scan"] + parts["octree"]
     return parts
 
 
@@ -448,17 +449,14 @@ def main() -> int:
 
     m, O = int(res.m), int(res.octree_size)
     lb = 4 if conflict else 0
-    prepend = 0 if n & (n - 1) == 0 else 1     # zero-key padding below the next power of two
-    ab = algorithmic_bytes(n, m, O, dim, int(res.key_bytes), passes, lb, bucket, prepend)
-    if conflict:      # the conflict-flag term of the byte model: labels read + m/8 flag bits
-        ab["total"] += lb * n + m // 8
+    ab = algorithmic_bytes(n, m, O, dim, int(res.key_bytes), passes, lb)
     dom_key = "gather" if bucket else "sort_pass"
     dom_gbps = ab[dom_key] / (launch_ms * 1e-3) / 1e9
     total_gbps = ab["total"] / (ms_step * 1e-3) / 1e9
     traffic = load_traffic(dom_key, args.workload)
     if bucket:
-        dom_name = (f"gather_kernel (octree fill of {m} BRT nodes into {O} octree nodes; "
-                    f"algorithmic bytes {ab['gather']})")
+        dom_name = (f"gather_kernel (scan of local splits and octree fill: {m} BRT nodes into "
+                    f"{O} octree nodes; algorithmic bytes {ab['gather']})")
     elif conflict:
         dom_name = "onesweep_kernel (one 8-bit radix pass of 2^22 u128 keys with u32 labels)"
     else:
@@ -500,6 +498,7 @@ def main() -> int:
         },
         "pipeline": {
             "algorithmic_bytes": ab["total"],
+            "byte_model": {k: v for k, v in ab.items() if k not in ("total", "sort_pass", "gather")},
             "achieved_GBps": total_gbps,
             "frac_of_hbm_peak": total_gbps / HBM_PEAK_GBPS,
             "stage_ms_per_step": {k: v[0] / prof_steps for k, v in stage_ms.items()},
