@@ -258,6 +258,13 @@ constexpr int kGatherHalo = 256;
 #define PGVD_GATHER_ROWS 2
 #endif
 constexpr int kGatherRows = PGVD_GATHER_ROWS;
+// Staged octree nodes per workgroup, in percent of its BRT nodes: 62% of
+// 512 keeps the LDS at 24.9 KB, six workgroups per CU (all of 512 would
+// leave five).
+#ifndef PGVD_GATHER_STAGE_PCT
+#define PGVD_GATHER_STAGE_PCT 62
+#endif
+constexpr int kGatherStagePct = PGVD_GATHER_STAGE_PCT;
 
 // Gather, first pass over the BRT nodes [q0, q0+R*256) of one tree with m
 // keys (m >= 2).  The workgroup stages links, ps and lsx of BRT nodes
@@ -286,10 +293,13 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
   constexpr int NC = 1 << DIM;
   constexpr int STRIDE = NC + 1;
   constexpr int H = kGatherHalo, WR = R + 2, WN = WR * kBlock, STAGE = R * kBlock;
+  // octree nodes staged in LDS: a window of STAGE BRT nodes owns ~0.47 STAGE
+  // of them for uniform keys; a larger span is written straight to HBM
+  constexpr int SN = STAGE * kGatherStagePct / 100;
   static_assert(H == kBlock, "the window is whole chunks of 256");
   // staged octree words, 16-byte aligned, plus up to 3 words in front so
   // that LDS and global words agree modulo 4 (16-byte copies out)
-  __shared__ __align__(16) int32_t s_nodes[(STAGE * STRIDE + 3 + 3) & ~3];
+  __shared__ __align__(16) int32_t s_nodes[(SN * STRIDE + 3 + 3) & ~3];
   __shared__ uint32_t s_lnk[WN], s_ps[WN], s_lsx[WN];
   const int t = threadIdx.x;
   const int w0 = (int)q0 - H;
@@ -337,7 +347,7 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
   const uint32_t qlast = min(q0 + STAGE - 1, m - 2);
   const uint32_t lo = q0 ? s_ps[q0 - 1 - w0] : 0u;
   const uint32_t span = s_ps[qlast - w0] - lo;
-  const bool staged = span <= (uint32_t)STAGE;   // uniform over the workgroup
+  const bool staged = span <= (uint32_t)SN;   // uniform over the workgroup
   const uint32_t a0 = (uint32_t)(((uint64_t)lo * STRIDE) & 3u);
   // The nodes that own octree nodes (about half), compacted so that each
   // lane takes one: entry e = j * 256 + t of the rows.
