@@ -215,8 +215,14 @@ __device__ inline void brt_window(const K *__restrict__ keys, int m, int b0, int
   }
 }
 
+// Waves per SIMD the BRT window kernel is compiled for: 6 caps it at 80
+// VGPRs (it needs 81 uncapped, which leaves 5), without spills.
+#ifndef PGVD_BRT_WPE
+#define PGVD_BRT_WPE 6
+#endif
 template <typename K, int DIM>
-__global__ __launch_bounds__(kBlock) void brt_ls_kernel(const K *__restrict__ keys, int mbits,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PGVD_BRT_WPE))) void
+brt_ls_kernel(const K *__restrict__ keys, int mbits,
                                                         BrtNodeT<K> *__restrict__ brt,
                                                         int32_t *__restrict__ parent,
                                                         uint32_t *__restrict__ lsx,

@@ -101,8 +101,9 @@ void launch_bbox(const float *pts, int64_t n, int dim, float *partials, RunState
                  hipStream_t s);
 // Quantize + Morton encode (+ optional digit histograms for `passes` passes).
 // bb: if custom, (min, max) given; else taken from st (launch_bbox).
-// msd_hist (nullable): also the histogram of key >> msd_shift over
-// msd_nb buckets (the bucket sort's), in the vectorised form only.
+// msd_hist (nullable): instead of the digit histograms (passes is taken as
+// 0), the histogram of key >> msd_shift over msd_nb buckets (the bucket
+// sort's), in the vectorised form only.
 template <typename K>
 void launch_encode(const float *pts, int64_t n, int dim, int bits, float effw,
                    int custom_bb, const float *bbmin, const float *bbmax,

@@ -281,8 +281,13 @@ __global__ __launch_bounds__(kBlock) void encode_vec_kernel(
     float bmin1, float bmin2, float bmax0, float bmax1, float bmax2, K *__restrict__ keys,
     uint32_t *__restrict__ hist, int passes, RunState *st, uint32_t *__restrict__ msd_hist,
     int msd_shift, int msd_nb) {
+  // One LDS histogram serves either the LSD digit counts (passes > 0) or the
+  // bucket counts (msd_hist, built with passes = 0): 16 KB instead of 32 KB
+  // keeps eight workgroups per CU.
+  static_assert(kMaxPasses * 256 == kMsdMaxBuckets, "the two histograms share their LDS");
   __shared__ uint32_t sh[kMaxPasses * 256];
-  __shared__ uint32_t s_msd[kMsdMaxBuckets];
+  uint32_t *s_msd = sh;
+  if (msd_hist) passes = 0;
   for (int i = threadIdx.x; i < passes * 256; i += kBlock) sh[i] = 0;
   if (msd_hist)
     for (int i = threadIdx.x; i < msd_nb; i += kBlock) s_msd[i] = 0;

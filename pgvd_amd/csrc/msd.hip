@@ -38,7 +38,7 @@ static_assert(kTileStart + kSplitMaxDigits + 1 <= kMsdCursorWords, "msd_cursor h
 // start of its tiles in the second scatter; msd_ok = 1 when every bucket
 // fits one workgroup.  nb is a power of two in [1024, 4096].
 __global__ __launch_bounds__(kBlock) void msd_plan_kernel(const uint32_t *__restrict__ hist,
-                                                          int nb, int nb2,
+                                                          uint32_t n, int nb, int nb2,
                                                           uint32_t *__restrict__ start,
                                                           uint32_t *__restrict__ cursor,
                                                           RunState *st) {
@@ -69,7 +69,9 @@ __global__ __launch_bounds__(kBlock) void msd_plan_kernel(const uint32_t *__rest
   __syncthreads();   // start[] is complete
   if (t == 0) {
     const uint32_t m4 = max(max(s_mx[0], s_mx[1]), max(s_mx[2], s_mx[3]));
-    st->msd_ok = m4 <= (uint32_t)kMsdCap ? 1u : 0u;
+    // (a histogram that does not count every key leaves the sort to the LSD
+    // passes)
+    st->msd_ok = m4 <= (uint32_t)kMsdCap && total == n ? 1u : 0u;
     st->msd_dups = 0;
   }
   const int nb1 = nb / nb2;   // <= 64: wave 0, lane g = group g
@@ -408,7 +410,7 @@ void launch_msd_sort(uint64_t *keys, uint64_t *alt, uint64_t *out, int prepend, 
   const int nb = 1 << b, b1 = b / 2, b2 = b - b1;
   const int nb1 = 1 << b1, nb2 = 1 << b2;
   const int shift = mbits - b;             // bits each bucket is sorted on
-  msd_plan_kernel<<<1, kBlock, 0, s>>>(hist, nb, nb2, start, cursor, st);
+  msd_plan_kernel<<<1, kBlock, 0, s>>>(hist, (uint32_t)n, nb, nb2, start, cursor, st);
   const unsigned tiles1 = (unsigned)((n + kSplitTile - 1) / kSplitTile);
   msd_split1_kernel<<<tiles1, kBlock, 0, s>>>(keys, alt, (uint32_t)n, mbits - b1, nb1, cursor,
                                               st);
