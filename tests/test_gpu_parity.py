@@ -218,6 +218,34 @@ def test_fp32_split_quirk_stage_path(dev, golden_digests, dim):
     assert _digest(lf, np.int32) == d["leaf"]
 
 
+def test_fp32_split_quirk_whole_build(dev, golden_digests):
+    """The same 2^24 keys as 2D grid points (21 bits/axis, x in the even
+    bits) through the whole build: the root split takes the reference's
+    FP32 value, the tree is malformed, and parents, local splits, their scan
+    and the octree come from the order-exact path inside the pipeline."""
+    from tests.golden.gen_golden import quirk_keys
+    d = golden_digests["quirk_2d"]
+    keys, s_star = quirk_keys()
+    q = np.zeros((keys.size, 2), np.int32)
+    for i in range(21):
+        for k in range(2):
+            bit = (keys >> np.uint64(2 * i + k)) & np.uint64(1)
+            q[:, k] |= bit.astype(np.int32) << i
+    got = dev.build_octree(q, 21)
+    assert got["malformed"] == 1
+    assert np.array_equal(got["keys_lo"], keys)
+    assert got["m"] == d["m"] and _digest(got["keys_lo"], np.uint64) == d["keys_lo"]
+    assert _digest(got["prefix_sums"], np.uint32) == d["prefix_sums"]
+    assert int(got["left"][0]) == d["root_left"] != s_star
+    assert _digest(got["left"], np.int32) == d["left"]
+    assert _digest(got["parent"], np.int32) == d["parent"]
+    assert _digest(got["lcp_length"], np.int32) == d["lcp_length"]
+    assert _digest(got["lcp_lo"], np.uint64) == d["lcp_lo"]
+    assert got["octree_size"] == d["octree_size"]
+    assert _digest(got["children"], np.int32) == d["children"]
+    assert _digest(got["leaf"], np.int32) == d["leaf"]
+
+
 def test_labels_conflict_cells(dev):
     """Point-label conflict pass: flags per unique key and per (node, octant)
     slot against the oracle (itself pinned by brute force)."""
