@@ -241,7 +241,10 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   const bool quantized = opts && opts->quantized_input;
   const bool custom = opts && opts->use_custom_bb;
   const bool clamped = opts && opts->clamped;
-  {
+  // the build's start (state reset, workspace zeroed) rides on the bbox pass
+  // when there is one
+  const bool bbox_pass = !quantized && !custom;
+  if (!bbox_pass) {
     Stage t(ctx, "prep", s);
     launch_prep(ctx->st, reinterpret_cast<uint32_t *>(zb), z.total / 4, s);
   }
@@ -250,10 +253,11 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
     Stage t(ctx, "encode", s);
     launch_encode_int<K>(static_cast<const int32_t *>(pts), n, dim, bits, kA, hist, passes, s);
   } else {
-    if (!custom) {
+    if (bbox_pass) {
       Stage t(ctx, "bbox", s);
       CHK(pool_get(ctx, "bbox_part", kBboxPartBytes, &p));
-      launch_bbox(static_cast<const float *>(pts), n, dim, static_cast<float *>(p), ctx->st, s);
+      launch_bbox(static_cast<const float *>(pts), n, dim, static_cast<float *>(p), ctx->st, s,
+                  reinterpret_cast<uint32_t *>(zb), z.total / 4, true);
     }
     const int64_t width = (int64_t)1 << bits;
     const float effw = (float)(clamped ? width : width - 1);

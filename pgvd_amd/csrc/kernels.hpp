@@ -97,8 +97,11 @@ void launch_prep(RunState *st, uint32_t *zero_words, size_t nzero, hipStream_t s
 // partials: scratch of kBboxPartBytes (one 8-float record per workgroup).
 constexpr int kBboxParts = 2048;
 constexpr size_t kBboxPartBytes = (size_t)kBboxParts * 8 * sizeof(float);
+// prep: also the build's start (launch_prep's work), in the same launch
+// where the points allow.
 void launch_bbox(const float *pts, int64_t n, int dim, float *partials, RunState *st,
-                 hipStream_t s);
+                 hipStream_t s, uint32_t *zero_words = nullptr, size_t nzero = 0,
+                 bool prep = false);
 // Quantize + Morton encode (+ optional digit histograms for `passes` passes).
 // bb: if custom, (min, max) given; else taken from st (launch_bbox).
 // msd_hist (nullable): instead of the digit histograms (passes is taken as

@@ -21,7 +21,8 @@ inline int grid_for(int64_t n, int per_block) {
   return (int)g;
 }
 
-__global__ __launch_bounds__(kBlock) void prep_kernel(RunState *st, uint4 *zero, size_t nzero16) {
+// A build's start: st reset, the zero-initialised workspace cleared.
+__device__ inline void prep_run(RunState *st, uint4 *zero, size_t nzero16) {
   const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid == 0) {
     for (int k = 0; k < 3; ++k) { st->bb_min[k] = 0xFFFFFFFFu; st->bb_max[k] = 0u; }
@@ -30,6 +31,10 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(RunState *st, uint4 *zero,
   }
   const uint4 z = make_uint4(0, 0, 0, 0);
   for (size_t i = tid; i < nzero16; i += (size_t)gridDim.x * blockDim.x) zero[i] = z;
+}
+
+__global__ __launch_bounds__(kBlock) void prep_kernel(RunState *st, uint4 *zero, size_t nzero16) {
+  prep_run(st, zero, nzero16);
 }
 
 template <int DIM>
@@ -157,9 +162,13 @@ __device__ inline void load_chunk(const float *__restrict__ pts, int64_t ch, flo
 // component j % 2.  Floats past the last whole float4 are read one by one.
 constexpr int kBboxU4 = 12;   // float4 loads in flight per thread (multiple of 3)
 
+// With prep_st, also the build's start (prep_run): one launch fewer.
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void bbox_vec_kernel(const float *__restrict__ pts, int64_t n,
-                                                          float *__restrict__ partials) {
+                                                          float *__restrict__ partials,
+                                                          RunState *prep_st, uint4 *zero,
+                                                          size_t nzero16) {
+  if (prep_st) prep_run(prep_st, zero, nzero16);
   static_assert(kBboxU4 % 3 == 0, "per-round span is a multiple of 3 float4s");
   constexpr int NA = DIM == 3 ? 3 : 1;
   const float kNan = __int_as_float(0x7fc00000);   // fminf / fmaxf skip it
@@ -523,19 +532,23 @@ void launch_prep(RunState *st, uint32_t *zero_words, size_t nzero, hipStream_t s
 inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 void launch_bbox(const float *pts, int64_t n, int dim, float *partials, RunState *st,
-                 hipStream_t s) {
+                 hipStream_t s, uint32_t *zero_words, size_t nzero, bool prep) {
   static_assert(kMaxGrid <= kBboxParts, "one partial record per workgroup");
   if (aligned16(pts)) {
     const int grid = grid_for(n * dim / 4 + 1, kBlock * kBboxU4 * 2);
+    RunState *pst = prep ? st : nullptr;
+    uint4 *z16 = reinterpret_cast<uint4 *>(zero_words);
+    const size_t n16 = prep ? nzero / 4 : 0;
     if (dim == 2) {
-      bbox_vec_kernel<2><<<grid, kBlock, 0, s>>>(pts, n, partials);
+      bbox_vec_kernel<2><<<grid, kBlock, 0, s>>>(pts, n, partials, pst, z16, n16);
       bbox_finish_kernel<2><<<1, kBlock, 0, s>>>(partials, grid, st);
     } else {
-      bbox_vec_kernel<3><<<grid, kBlock, 0, s>>>(pts, n, partials);
+      bbox_vec_kernel<3><<<grid, kBlock, 0, s>>>(pts, n, partials, pst, z16, n16);
       bbox_finish_kernel<3><<<1, kBlock, 0, s>>>(partials, grid, st);
     }
     return;
   }
+  if (prep) launch_prep(st, zero_words, nzero, s);
   const int grid = grid_for(n, kBlock * 8);
   if (dim == 2) bbox_kernel<2><<<grid, kBlock, 0, s>>>(pts, n, st);
   else bbox_kernel<3><<<grid, kBlock, 0, s>>>(pts, n, st);

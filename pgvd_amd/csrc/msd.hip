@@ -97,8 +97,7 @@ __global__ __launch_bounds__(kBlock) void msd_plan_kernel(const uint32_t *__rest
 __device__ inline void split_tile(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
                                   uint32_t n, uint32_t lo, uint32_t hi, int dshift, int nd,
                                   uint32_t *__restrict__ gcursor, uint32_t gbase,
-                                  int cstride) {
-  __shared__ uint64_t s_keys[kSplitTile];
+                                  int cstride, uint64_t *s_keys) {
   __shared__ uint32_t s_cnt[kSplitMaxDigits], s_lstart[kSplitMaxDigits],
       s_gstart[kSplitMaxDigits];
   const int t = threadIdx.x;
@@ -153,9 +152,10 @@ __global__ __launch_bounds__(kBlock) void msd_split1_kernel(const uint64_t *__re
                                                             uint32_t *__restrict__ cursor,
                                                             const RunState *st) {
   if (!st->msd_ok) return;
+  __shared__ uint64_t s_keys[kSplitTile];
   const uint32_t lo = blockIdx.x * (uint32_t)kSplitTile;
   const uint32_t hi = min(lo + (uint32_t)kSplitTile, n);
-  split_tile(in, out, n, lo, hi, shift1, nb1, cursor + kCursor1, 0u, kMsdCStride);
+  split_tile(in, out, n, lo, hi, shift1, nb1, cursor + kCursor1, 0u, kMsdCStride, s_keys);
 }
 
 // Second scatter: tiles within each first-level group g (keys
@@ -169,6 +169,7 @@ __global__ __launch_bounds__(kBlock) void msd_split2_kernel(const uint64_t *__re
                                                             uint32_t *__restrict__ cursor,
                                                             const RunState *st) {
   if (!st->msd_ok) return;
+  __shared__ uint64_t s_keys[kSplitTile];
   const uint32_t tile = blockIdx.x;
   const uint32_t *ts = cursor + kTileStart;
   if (tile >= ts[nb1]) return;
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void msd_split2_kernel(const uint64_t *__re
   const uint32_t glo = start[g * nb2], ghi = start[(g + 1) * nb2];
   const uint32_t lo = glo + (tile - ts[g]) * (uint32_t)kSplitTile;
   const uint32_t hi = min(lo + (uint32_t)kSplitTile, ghi);
-  split_tile(in, out, n, lo, hi, shift, nb2, cursor, (uint32_t)(g * nb2), 1);
+  split_tile(in, out, n, lo, hi, shift, nb2, cursor, (uint32_t)(g * nb2), 1, s_keys);
 }
 
 // Stable LSD passes, in LDS, over the BS IPT keys k[] of a workgroup: key

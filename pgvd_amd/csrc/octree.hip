@@ -146,8 +146,8 @@ template <typename K, int DIM, int W, bool LAST>
 __device__ __forceinline__ void bfs_round(const uint32_t (&idx)[W], const bool (&lf)[W],
                                  const bool (&live)[W], uint32_t (&nidx)[2 * W],
                                  bool (&nlf)[2 * W], bool (&nlive)[2 * W], int w0, int wn,
-                                 const uint32_t *s_lnk, const uint32_t *s_ps,
-                                 const uint32_t *s_lsx, const K *__restrict__ keys,
+                                 const uint32_t *s_lnk, const uint2 *s_pl,
+                                 const K *__restrict__ keys,
                                  const uint32_t *__restrict__ conflict_bits, int lvl, int mbits,
                                  int32_t *node0, uint32_t &found, uint32_t &cmask, bool &miss,
                                  RunState *st) {
@@ -159,13 +159,14 @@ __device__ __forceinline__ void bfs_round(const uint32_t (&idx)[W], const bool (
     miss |= internal && !inwin;
     const bool use = internal && inwin;
     const unsigned kk = use ? k : 0u;
-    const uint32_t lx = s_lsx[kk];
+    const uint2 pl = s_pl[kk];   // prefix sum, packed local split
+    const uint32_t lx = pl.y;
     const bool term = use && (lx & 0xffu) != 0;
     const bool expand = use && (lx & 0xffu) == 0;
     // the child chain's top, in its octant; otherwise a dummy store to the
     // leaf word, which is written last
     const uint32_t o = term ? lx >> 8 : (uint32_t)(1 << DIM);
-    node0[o] = (int32_t)(s_ps[kk] - 1);
+    node0[o] = (int32_t)(pl.x - 1);
     found |= term ? 1u << o : 0u;
     if (live[s] && lf[s] && conflict_bits && test_bit(conflict_bits, idx[s]))
       cmask |= 1u << digit_at<K, DIM>(keys[idx[s]], lvl, mbits);
@@ -199,7 +200,7 @@ template <typename K, int DIM>
 __device__ __forceinline__ bool gather_one_bfs(uint32_t q, uint32_t lk,
                                       const BrtNodeT<K> *__restrict__ brt, uint32_t excl,
                                       uint32_t lsq, int w0, int wn, const uint32_t *s_lnk,
-                                      const uint32_t *s_ps, const uint32_t *s_lsx,
+                                      const uint2 *s_pl,
                                       const K *__restrict__ keys,
                                       const uint32_t *__restrict__ conflict_bits, int mbits,
                                       int32_t *node0, uint32_t *__restrict__ node_conflict,
@@ -218,16 +219,16 @@ __device__ __forceinline__ bool gather_one_bfs(uint32_t q, uint32_t lk,
   const bool l1[2] = {true, true};
   uint32_t i2[4];
   bool f2[4], l2[4];
-  bfs_round<K, DIM, 2, DIM == 1>(i1, f1, l1, i2, f2, l2, w0, wn, s_lnk, s_ps, s_lsx, keys,
+  bfs_round<K, DIM, 2, DIM == 1>(i1, f1, l1, i2, f2, l2, w0, wn, s_lnk, s_pl, keys,
                                  conflict_bits, lvl, mbits, node0, found, cmask, miss, st);
   uint32_t i3[8];
   bool f3[8], l3[8];
-  bfs_round<K, DIM, 4, DIM == 2>(i2, f2, l2, i3, f3, l3, w0, wn, s_lnk, s_ps, s_lsx, keys,
+  bfs_round<K, DIM, 4, DIM == 2>(i2, f2, l2, i3, f3, l3, w0, wn, s_lnk, s_pl, keys,
                                  conflict_bits, lvl, mbits, node0, found, cmask, miss, st);
   if (DIM == 3) {
     uint32_t i4[16];
     bool f4[16], l4[16];
-    bfs_round<K, DIM, 8, true>(i3, f3, l3, i4, f4, l4, w0, wn, s_lnk, s_ps, s_lsx, keys,
+    bfs_round<K, DIM, 8, true>(i3, f3, l3, i4, f4, l4, w0, wn, s_lnk, s_pl, keys,
                                conflict_bits, lvl, mbits, node0, found, cmask, miss, st);
   }
   if (miss) return false;
@@ -300,7 +301,10 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
   // staged octree words, 16-byte aligned, plus up to 3 words in front so
   // that LDS and global words agree modulo 4 (16-byte copies out)
   __shared__ __align__(16) int32_t s_nodes[(SN * STRIDE + 3 + 3) & ~3];
-  __shared__ uint32_t s_lnk[WN], s_ps[WN], s_lsx[WN];
+  // per window position: packed links, and (inclusive prefix sum of local
+  // splits, packed local split) side by side so that one read gets both
+  __shared__ uint32_t s_lnk[WN];
+  __shared__ uint2 s_pl[WN];
   const int t = threadIdx.x;
   const int w0 = (int)q0 - H;
   // Every load of the window is issued before the first is used: addresses
@@ -339,14 +343,13 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
 #pragma unroll
   for (int r = 0; r < WR; ++r) {
     const int k = r * kBlock + t;
-    s_lsx[k] = lx[r];
+    s_pl[k] = make_uint2(pv[r], lx[r]);
     s_lnk[k] = lk[r];
-    s_ps[k] = pv[r];
   }
   __syncthreads();
   const uint32_t qlast = min(q0 + STAGE - 1, m - 2);
-  const uint32_t lo = q0 ? s_ps[q0 - 1 - w0] : 0u;
-  const uint32_t span = s_ps[qlast - w0] - lo;
+  const uint32_t lo = q0 ? s_pl[q0 - 1 - w0].x : 0u;
+  const uint32_t span = s_pl[qlast - w0].x - lo;
   const bool staged = span <= (uint32_t)SN;   // uniform over the workgroup
   const uint32_t a0 = (uint32_t)(((uint64_t)lo * STRIDE) & 3u);
   // The nodes that own octree nodes (about half), compacted so that each
@@ -360,8 +363,8 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
     const uint32_t q = q0 + j * kBlock + t;
     bool active = q <= qlast;
     if (active) {
-      const uint32_t excl = q ? s_ps[q - 1 - w0] : 0u;
-      active = (q == 0) || s_ps[q - w0] > excl;
+      const uint32_t excl = q ? s_pl[q - 1 - w0].x : 0u;
+      active = (q == 0) || s_pl[q - w0].x > excl;
     }
     amask |= (uint32_t)active << j;
   }
@@ -374,12 +377,11 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
   for (uint32_t a = t; a < nact; a += kBlock) {
     const uint32_t e = s_aidx[a];
     const uint32_t q = q0 + e;
-    const uint32_t excl = q ? s_ps[q - 1 - w0] : 0u;
-    const uint32_t lsq = s_ps[q - w0] - excl;
+    const uint32_t excl = q ? s_pl[q - 1 - w0].x : 0u;
+    const uint32_t lsq = s_pl[q - w0].x - excl;
     bool ok;
     if (staged) {
-      ok = gather_one_bfs<K, DIM>(q, s_lnk[q - w0], brt, excl, lsq, w0, WN, s_lnk, s_ps, s_lsx,
-                                  keys,
+      ok = gather_one_bfs<K, DIM>(q, s_lnk[q - w0], brt, excl, lsq, w0, WN, s_lnk, s_pl, keys,
                                   conflict_bits, mbits, s_nodes + a0 + (excl - lo) * STRIDE,
                                   node_conflict, st);
     } else {
@@ -395,8 +397,8 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
       auto child = [&](uint32_t c, uint32_t &psc, uint32_t &lxc) -> bool {
         const unsigned k = (unsigned)((int)c - w0);
         if (k >= (unsigned)WN) return false;
-        psc = s_ps[k];
-        lxc = s_lsx[k];
+        psc = s_pl[k].x;
+        lxc = s_pl[k].y;
         return true;
       };
       auto put = [&](uint32_t idx, const int32_t *c) { write_node<DIM>(octree, idx, c); };
