@@ -298,6 +298,13 @@ __global__ __launch_bounds__(kBlock) void encode_vec_kernel(
   uint32_t *s_msd = sh;
   if (msd_hist) passes = 0;
   for (int i = threadIdx.x; i < passes * 256; i += kBlock) sh[i] = 0;
+  // 3D keys of at most 64 bits: Morton spread by table, byte by byte
+  // (s_spread3[v] = bit i of v at bit 3i), instead of 64-bit mask steps.
+  constexpr bool kTable = DIM == 3 && sizeof(K) == 8;
+  __shared__ uint32_t s_spread3[kTable ? 256 : 1];
+  if (kTable) s_spread3[threadIdx.x] = (uint32_t)spread3(threadIdx.x);
+  static_assert(!kTable || kBlock == 256, "one table entry per thread");
+  const uint32_t cmask32 = bits >= 32 ? ~0u : (1u << bits) - 1u;
   if (msd_hist)
     for (int i = threadIdx.x; i < msd_nb; i += kBlock) s_msd[i] = 0;
   const uint32_t msd_mask = (uint32_t)msd_nb - 1;
@@ -323,8 +330,22 @@ __global__ __launch_bounds__(kBlock) void encode_vec_kernel(
   auto encode_one = [&](const float *p) -> K {
     int64_t c[DIM];
     neg |= quantize_point<DIM>(p, mn, dwidth, effw, c);
-    uint64_t lo, hi;
-    morton128<DIM, (sizeof(K) > 8)>(c, bits, lo, hi);
+    uint64_t lo, hi = 0;
+    if constexpr (kTable) {
+      // bits <= 21: the three coordinates' bytes j give key bits 24j ..
+      const uint32_t x = (uint32_t)c[0] & cmask32, y = (uint32_t)c[1] & cmask32,
+                     z = (uint32_t)c[2] & cmask32;
+      uint64_t w[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int sh = 8 * j;
+        w[j] = s_spread3[(x >> sh) & 255u] | s_spread3[(y >> sh) & 255u] << 1 |
+               s_spread3[(z >> sh) & 255u] << 2;
+      }
+      lo = w[0] | w[1] << 24 | w[2] << 48;
+    } else {
+      morton128<DIM, (sizeof(K) > 8)>(c, bits, lo, hi);
+    }
     const K key = from128<K>(lo, hi);
     for (int r = 0; r < passes; ++r) atomicAdd(&sh[r * 256 + kdigit(key, 8 * r)], 1u);
     if (msd_hist) atomicAdd(&s_msd[klow32(kshr(key, msd_shift)) & msd_mask], 1u);
