@@ -256,7 +256,7 @@ __device__ inline void lds_radix_passes(uint64_t (&k)[IPT], int passes, int last
 // Sub-buckets of the counting pass: the 12 bits below a bucket's own.
 constexpr int kSubBits = 12;
 constexpr int kSubBins = 1 << kSubBits;
-// Largest sub-bucket the counting pass finishes by insertion sort.
+// Largest sub-bucket the counting pass puts in order by ranking its keys.
 constexpr uint32_t kSubMaxGroup = 32;
 // Threads of a bucket workgroup.  The LDS of one bucket (~58 KB) allows two
 // workgroups per CU, so the waves that hide the loads' latency come from
@@ -280,8 +280,8 @@ constexpr int kBucketBlock = PGVD_MSD_BLOCK;
 //
 // Sorting: every key gets a slot in its sub-bucket (its next 12 bits, bits
 // shift-12 .. shift-1) from an LDS atomic, in any order -- keys without
-// payload need no stable order -- and each sub-bucket, ~1 key for uniform
-// keys, is put in order by an insertion sort on the whole key.  A bucket
+// payload need no stable order -- and then its place from its rank on the
+// whole key within the sub-bucket (~1 key for uniform keys).  A bucket
 // with a sub-bucket of more than kSubMaxGroup keys (clustered keys) is
 // sorted by stable LDS radix passes on its low `shift` bits instead.
 template <int BS>
@@ -355,21 +355,30 @@ msd_bucket_kernel(const uint64_t *__restrict__ in,
       if (i < cnt) s_keys[s_sub[rs[j] >> 16] + (rs[j] & 0xffffu)] = k[j];
     }
     __syncthreads();
-    // thread t puts its sub-buckets in order (insertion sort in LDS)
-    uint32_t b0 = s_sub[t * PER];
-    for (int i = 0; i < PER; ++i) {
-      const uint32_t b1 = s_sub[t * PER + i + 1];
-      for (uint32_t x = b0 + 1; x < b1; ++x) {
-        const uint64_t v = s_keys[x];
-        uint32_t y = x;
-        while (y > b0 && s_keys[y - 1] > v) {
-          s_keys[y] = s_keys[y - 1];
-          --y;
+    // every key's rank within its sub-bucket [b0, b1) (at most kSubMaxGroup
+    // keys; ties by position), then one permutation in LDS
+    uint64_t v[IPT];
+    uint32_t dst[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t i = j * BS + t;
+      dst[j] = i;
+      if (i < cnt) {
+        v[j] = s_keys[i];
+        const uint32_t sb = (uint32_t)(v[j] >> sub_shift) & (kSubBins - 1);
+        const uint32_t b0 = s_sub[sb], b1 = s_sub[sb + 1];
+        uint32_t r = b0;
+        for (uint32_t x = b0; x < b1; ++x) {
+          const uint64_t w = s_keys[x];
+          r += (w < v[j] || (w == v[j] && x < i)) ? 1u : 0u;
         }
-        s_keys[y] = v;
+        dst[j] = r;
       }
-      b0 = b1;
     }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+      if (j * BS + t < cnt) s_keys[dst[j]] = v[j];
   } else {
     // keys in the radix passes' layout, the slots past the bucket last
     const int lane = t & 63, w = t >> 6;
