@@ -88,9 +88,6 @@ ZeroLayout zero_layout(int64_t n, int kb, int passes, bool labels) {
   size_t off = 0;
   z.hist = off;
   off += align256((size_t)kMaxPasses * 256 * 4);
-  z.sort_lb = off;
-  const int64_t tiles = onesweep_tiles(kb, labels, n > 0 ? n : 1);
-  off += align256((size_t)passes * (size_t)tiles * 256 * 4);
   z.lb_unique = off;
   off += align256((size_t)scan_tiles(n + 1) * 8);
   z.lb_splits = off;
@@ -103,6 +100,11 @@ ZeroLayout zero_layout(int64_t n, int kb, int passes, bool labels) {
   if (labels) off += align256((size_t)((n + 1 + 31) / 32) * 4);
   z.msd_hist = off;
   off += align256((size_t)kMsdMaxBuckets * 4);
+  // the sort passes' look-back words last: beside the bucket sort they are
+  // zeroed only if the passes run (launch_histogram), not at the start
+  z.sort_lb = off;
+  const int64_t tiles = onesweep_tiles(kb, labels, n > 0 ? n : 1);
+  off += align256((size_t)passes * (size_t)tiles * 256 * 4);
   z.total = off;
   return z;
 }
@@ -241,6 +243,12 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   const bool quantized = opts && opts->quantized_input;
   const bool custom = opts && opts->use_custom_bb;
   const bool clamped = opts && opts->clamped;
+  // 64-bit keys of a large build without labels: bucket sort, with the LSD
+  // passes launched beside it for inputs whose buckets overflow
+  if (!quantized && std::is_same<K, uint64_t>::value && !lab &&
+      encode_vectorised(static_cast<const float *>(pts), kA))
+    msd_b = msd_bits_for(n);
+  const bool msd_first = msd_b != 0;
   // the build's start (state reset, workspace zeroed) rides on the bbox pass
   // when there is one
   const bool bbox_pass = !quantized && !custom;
@@ -257,15 +265,11 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
       Stage t(ctx, "bbox", s);
       CHK(pool_get(ctx, "bbox_part", kBboxPartBytes, &p));
       launch_bbox(static_cast<const float *>(pts), n, dim, static_cast<float *>(p), ctx->st, s,
-                  reinterpret_cast<uint32_t *>(zb), z.total / 4, true);
+                  reinterpret_cast<uint32_t *>(zb), (msd_first ? z.sort_lb : z.total) / 4,
+                  true);
     }
     const int64_t width = (int64_t)1 << bits;
     const float effw = (float)(clamped ? width : width - 1);
-    // 64-bit keys of a large build without labels: bucket sort, with the LSD
-    // passes launched beside it for inputs whose buckets overflow
-    if (std::is_same<K, uint64_t>::value && !lab &&
-        encode_vectorised(static_cast<const float *>(pts), kA))
-      msd_b = msd_bits_for(n);
     Stage t(ctx, "encode", s);
     if (msd_b)
       launch_encode<K>(static_cast<const float *>(pts), n, dim, bits, effw, custom ? 1 : 0,
@@ -287,7 +291,7 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
       CHK(pool_get(ctx, "msd_cursor", (size_t)kMsdCursorWords * 4, &p));
       launch_msd_sort(kA, kB, (passes & 1) ? kB : kA, prepend, n, mbits, msd_b, msd_hist,
                       mstart, static_cast<uint32_t *>(p), ctx->st, s);
-      launch_histogram<K>(kA, n, hist, passes, s, ctx->st);
+      launch_histogram<K>(kA, n, hist, passes, s, ctx->st, sort_lb, (z.total - z.sort_lb) / 4);
     }
   }
   {

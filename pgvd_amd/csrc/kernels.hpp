@@ -119,10 +119,14 @@ bool encode_vectorised(const float *pts, const void *keys);
 template <typename K>
 void launch_points_to_morton(const void *pts, int coord_bytes, int64_t n, int64_t padded_n,
                              int dim, int bits, K *keys, hipStream_t s);
-// skip_if_msd (nullable): does nothing when skip_if_msd->msd_ok.
+// skip_if_msd (nullable): does nothing when skip_if_msd->msd_ok.  Otherwise
+// also zeroes zero_words[0 .. nzero) (16-byte aligned, nzero a multiple of
+// 4): the passes' look-back words, which the build's start leaves alone
+// beside the bucket sort.
 template <typename K>
 void launch_histogram(const K *keys, int64_t n, uint32_t *hist, int passes, hipStream_t s,
-                      const RunState *skip_if_msd = nullptr);
+                      const RunState *skip_if_msd = nullptr, uint32_t *zero_words = nullptr,
+                      size_t nzero = 0);
 // Morton encode + digit histograms from int32 grid points (no padding).
 template <typename K>
 void launch_encode_int(const int32_t *pts, int64_t n, int dim, int bits, K *keys,

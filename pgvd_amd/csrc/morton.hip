@@ -416,8 +416,14 @@ __global__ __launch_bounds__(kBlock) void encode_int_kernel(const int32_t *__res
 template <typename K>
 __global__ __launch_bounds__(kBlock) void histogram_kernel(const K *__restrict__ keys, int64_t n,
                                                            uint32_t *__restrict__ hist,
-                                                           int passes, const RunState *skip) {
+                                                           int passes, const RunState *skip,
+                                                           uint4 *zero, size_t nzero16) {
   if (skip && skip->msd_ok) return;
+  // words the passes need zeroed that the build's start left alone
+  const uint4 z = make_uint4(0, 0, 0, 0);
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < nzero16;
+       i += (size_t)gridDim.x * kBlock)
+    zero[i] = z;
   __shared__ uint32_t sh[kMaxPasses * 256];
   for (int i = threadIdx.x; i < passes * 256; i += kBlock) sh[i] = 0;
   __syncthreads();
@@ -626,9 +632,10 @@ void launch_points_to_morton(const void *pts, int coord_bytes, int64_t n, int64_
 
 template <typename K>
 void launch_histogram(const K *keys, int64_t n, uint32_t *hist, int passes, hipStream_t s,
-                      const RunState *skip_if_msd) {
+                      const RunState *skip_if_msd, uint32_t *zero_words, size_t nzero) {
   const int grid = grid_for(n, kBlock * 8);
-  histogram_kernel<K><<<grid, kBlock, 0, s>>>(keys, n, hist, passes, skip_if_msd);
+  histogram_kernel<K><<<grid, kBlock, 0, s>>>(keys, n, hist, passes, skip_if_msd,
+                                              reinterpret_cast<uint4 *>(zero_words), nzero / 4);
 }
 
 template <typename K>
@@ -672,7 +679,7 @@ void launch_encode_seg(const void *pts, int quantized, int dim, int bits, float 
   template void launch_points_to_morton<K>(const void *, int, int64_t, int64_t, int, int, K *, \
                                            hipStream_t);                                       \
   template void launch_histogram<K>(const K *, int64_t, uint32_t *, int, hipStream_t,          \
-                                    const RunState *);                                         \
+                                    const RunState *, uint32_t *, size_t);                     \
   template void launch_encode_int<K>(const int32_t *, int64_t, int, int, K *, uint32_t *, int,  \
                                      hipStream_t);                                             \
   template void launch_encode_seg<K>(const void *, int, int, int, float, const BatchDev &, K *, \
