@@ -197,6 +197,18 @@ def test_bucket_sort_and_its_fallback(dev):
     assert (got["keys_lo"][1:] > got["keys_lo"][:-1]).all()
 
 
+@pytest.mark.parametrize("dim,bits", [(2, 24), (3, 13)])
+def test_bucket_sort_other_key_widths(dev, dim, bits):
+    """The bucket sort on 48-bit 2D keys and 39-bit 3D keys (fewer bits left
+    below the buckets' own), against the oracle."""
+    n = (1 << 23) + 999
+    pts = _pts(n, dim, seed=dim * 100 + bits)
+    got = dev.build_octree(pts, bits)
+    assert got["bucket_sort"] == 1
+    assert_tree_equal(got, oracle.build(oracle.quantize(pts, 1 << bits), bits),
+                      where=f"dim={dim} bits={bits}")
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_fp32_split_quirk_stage_path(dev, golden_digests, dim):
     """2^24 keys whose root split takes the reference's FP32 value: BRT via
