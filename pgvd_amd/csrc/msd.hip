@@ -90,10 +90,12 @@ __global__ __launch_bounds__(kBlock) void msd_plan_kernel(const uint32_t *__rest
   }
 }
 
-// One scatter tile: keys in[lo .. hi) (0 < hi - lo <= 4096) go to out in
-// runs by digit ld = (key >> dshift) & (nd - 1), the run of digit ld
-// starting at an atomic claim on gcursor[gbase + ld * cstride].  Keys are staged in
-// LDS in digit order so that each run is written by consecutive lanes.
+// One scatter tile of a BS-thread workgroup: keys in[lo .. hi) (0 < hi - lo
+// <= 16 BS) go to out in runs by digit ld = (key >> dshift) & (nd - 1), the
+// run of digit ld starting at an atomic claim on gcursor[gbase + ld *
+// cstride].  Keys are staged in LDS (s_keys, 16 BS words) in digit order so
+// that each run is written by consecutive lanes.
+template <int BS>
 __device__ inline void split_tile(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
                                   uint32_t n, uint32_t lo, uint32_t hi, int dshift, int nd,
                                   uint32_t *__restrict__ gcursor, uint32_t gbase,
@@ -107,13 +109,13 @@ __device__ inline void split_tile(const uint64_t *__restrict__ in, uint64_t *__r
   uint32_t dr[kSplitIpt];
 #pragma unroll
   for (int j = 0; j < kSplitIpt; ++j) {
-    const uint32_t i = lo + j * kBlock + t;
+    const uint32_t i = lo + j * BS + t;
     k[j] = in[i < hi ? i : lo];
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kSplitIpt; ++j) {
-    const uint32_t i = lo + j * kBlock + t;
+    const uint32_t i = lo + j * BS + t;
     dr[j] = 0;
     if (i < hi) {
       const uint32_t d = (uint32_t)(k[j] >> dshift) & dmask;
@@ -132,12 +134,12 @@ __device__ inline void split_tile(const uint64_t *__restrict__ in, uint64_t *__r
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kSplitIpt; ++j) {
-    const uint32_t i = lo + j * kBlock + t;
+    const uint32_t i = lo + j * BS + t;
     if (i < hi) s_keys[s_lstart[dr[j] >> 16] + (dr[j] & 0xffffu)] = k[j];
   }
   __syncthreads();
   const uint32_t cnt = hi - lo;
-  for (uint32_t p = t; p < cnt; p += kBlock) {
+  for (uint32_t p = t; p < cnt; p += BS) {
     const uint64_t key = s_keys[p];
     const uint32_t d = (uint32_t)(key >> dshift) & dmask;
     const uint32_t pos = s_gstart[d] + (p - s_lstart[d]);
@@ -145,17 +147,24 @@ __device__ inline void split_tile(const uint64_t *__restrict__ in, uint64_t *__r
   }
 }
 
-// First scatter: contiguous tiles of the whole array, by the top b1 bits.
-__global__ __launch_bounds__(kBlock) void msd_split1_kernel(const uint64_t *__restrict__ in,
-                                                            uint64_t *__restrict__ out,
-                                                            uint32_t n, int shift1, int nb1,
-                                                            uint32_t *__restrict__ cursor,
-                                                            const RunState *st) {
+// First scatter: contiguous tiles of the whole array, by the top b1 bits,
+// in 512-thread workgroups of 8192 keys: every tile claims on the same 2^b1
+// cursors, so fewer, larger tiles wait less on them (two such workgroups
+// per CU hold as many waves as four of 256 threads).
+#ifndef PGVD_SPLIT1_BLOCK
+#define PGVD_SPLIT1_BLOCK 512
+#endif
+constexpr int kSplit1Block = PGVD_SPLIT1_BLOCK;
+constexpr int kSplit1Tile = kSplit1Block * kSplitIpt;
+__global__ __launch_bounds__(kSplit1Block) void msd_split1_kernel(
+    const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint32_t n, int shift1, int nb1,
+    uint32_t *__restrict__ cursor, const RunState *st) {
   if (!st->msd_ok) return;
-  __shared__ uint64_t s_keys[kSplitTile];
-  const uint32_t lo = blockIdx.x * (uint32_t)kSplitTile;
-  const uint32_t hi = min(lo + (uint32_t)kSplitTile, n);
-  split_tile(in, out, n, lo, hi, shift1, nb1, cursor + kCursor1, 0u, kMsdCStride, s_keys);
+  __shared__ uint64_t s_keys[kSplit1Tile];
+  const uint32_t lo = blockIdx.x * (uint32_t)kSplit1Tile;
+  const uint32_t hi = min(lo + (uint32_t)kSplit1Tile, n);
+  split_tile<kSplit1Block>(in, out, n, lo, hi, shift1, nb1, cursor + kCursor1, 0u, kMsdCStride,
+                           s_keys);
 }
 
 // Second scatter: tiles within each first-level group g (keys
@@ -179,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void msd_split2_kernel(const uint64_t *__re
   const uint32_t glo = start[g * nb2], ghi = start[(g + 1) * nb2];
   const uint32_t lo = glo + (tile - ts[g]) * (uint32_t)kSplitTile;
   const uint32_t hi = min(lo + (uint32_t)kSplitTile, ghi);
-  split_tile(in, out, n, lo, hi, shift, nb2, cursor, (uint32_t)(g * nb2), 1, s_keys);
+  split_tile<kBlock>(in, out, n, lo, hi, shift, nb2, cursor, (uint32_t)(g * nb2), 1, s_keys);
 }
 
 // Stable LSD passes, in LDS, over the BS IPT keys k[] of a workgroup: key
@@ -421,11 +430,13 @@ void launch_msd_sort(uint64_t *keys, uint64_t *alt, uint64_t *out, int prepend, 
   const int nb1 = 1 << b1, nb2 = 1 << b2;
   const int shift = mbits - b;             // bits each bucket is sorted on
   msd_plan_kernel<<<1, kBlock, 0, s>>>(hist, (uint32_t)n, nb, nb2, start, cursor, st);
-  const unsigned tiles1 = (unsigned)((n + kSplitTile - 1) / kSplitTile);
-  msd_split1_kernel<<<tiles1, kBlock, 0, s>>>(keys, alt, (uint32_t)n, mbits - b1, nb1, cursor,
-                                              st);
-  // the groups' tiles: the sum of ceil(len / tile) is at most tiles1 + nb1
-  msd_split2_kernel<<<tiles1 + (unsigned)nb1, kBlock, 0, s>>>(alt, keys, (uint32_t)n, shift, nb2,
+  const unsigned tiles1 = (unsigned)((n + kSplit1Tile - 1) / kSplit1Tile);
+  msd_split1_kernel<<<tiles1, kSplit1Block, 0, s>>>(keys, alt, (uint32_t)n, mbits - b1, nb1,
+                                                    cursor, st);
+  // the second scatter's tiles (kSplitTile keys) within each group: the sum
+  // of ceil(len / tile) over the groups is at most tiles2 + nb1
+  const unsigned tiles2 = (unsigned)((n + kSplitTile - 1) / kSplitTile);
+  msd_split2_kernel<<<tiles2 + (unsigned)nb1, kBlock, 0, s>>>(alt, keys, (uint32_t)n, shift, nb2,
                                                               nb1, start, cursor, st);
   const int passes = (shift + 7) / 8;
   const int last_bits = shift - 8 * (passes - 1);
