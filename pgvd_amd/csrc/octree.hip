@@ -137,6 +137,11 @@ __device__ __forceinline__ bool gather_one(uint32_t q, const BrtNodeT<K> &nq, ui
   return true;
 }
 
+// A window position's prefix sum (relative to the window's base) and its
+// packed local split (local split | attach octant << 8 < 2^11) in one word.
+constexpr int kPlShift = 11;
+constexpr uint32_t kPlLsx = (1u << kPlShift) - 1;
+
 // One breadth-first round of gather_one_bfs over W slots (BRT child
 // indices, leaf flags, live flags): a live internal slot inside the window
 // is either the top of a child chain (local split > 0: recorded in node0 at
@@ -146,7 +151,7 @@ template <typename K, int DIM, int W, bool LAST>
 __device__ __forceinline__ void bfs_round(const uint32_t (&idx)[W], const bool (&lf)[W],
                                  const bool (&live)[W], uint32_t (&nidx)[2 * W],
                                  bool (&nlf)[2 * W], bool (&nlive)[2 * W], int w0, int wn,
-                                 const uint32_t *s_lnk, const uint2 *s_pl,
+                                 const uint32_t *s_lnk, const uint32_t *s_pl, uint32_t pbase,
                                  const K *__restrict__ keys,
                                  const uint32_t *__restrict__ conflict_bits, int lvl, int mbits,
                                  int32_t *node0, uint32_t &found, uint32_t &cmask, bool &miss,
@@ -159,14 +164,14 @@ __device__ __forceinline__ void bfs_round(const uint32_t (&idx)[W], const bool (
     miss |= internal && !inwin;
     const bool use = internal && inwin;
     const unsigned kk = use ? k : 0u;
-    const uint2 pl = s_pl[kk];   // prefix sum, packed local split
-    const uint32_t lx = pl.y;
+    const uint32_t pl = s_pl[kk];   // prefix sum - pbase, packed local split
+    const uint32_t lx = pl & kPlLsx;
     const bool term = use && (lx & 0xffu) != 0;
     const bool expand = use && (lx & 0xffu) == 0;
     // the child chain's top, in its octant; otherwise a dummy store to the
     // leaf word, which is written last
     const uint32_t o = term ? lx >> 8 : (uint32_t)(1 << DIM);
-    node0[o] = (int32_t)(pl.x - 1);
+    node0[o] = (int32_t)(pbase + (pl >> kPlShift) - 1);
     found |= term ? 1u << o : 0u;
     if (live[s] && lf[s] && conflict_bits && test_bit(conflict_bits, idx[s]))
       cmask |= 1u << digit_at<K, DIM>(keys[idx[s]], lvl, mbits);
@@ -200,7 +205,7 @@ template <typename K, int DIM>
 __device__ __forceinline__ bool gather_one_bfs(uint32_t q, uint32_t lk,
                                       const BrtNodeT<K> *__restrict__ brt, uint32_t excl,
                                       uint32_t lsq, int w0, int wn, const uint32_t *s_lnk,
-                                      const uint2 *s_pl,
+                                      const uint32_t *s_pl, uint32_t pbase,
                                       const K *__restrict__ keys,
                                       const uint32_t *__restrict__ conflict_bits, int mbits,
                                       int32_t *node0, uint32_t *__restrict__ node_conflict,
@@ -219,16 +224,16 @@ __device__ __forceinline__ bool gather_one_bfs(uint32_t q, uint32_t lk,
   const bool l1[2] = {true, true};
   uint32_t i2[4];
   bool f2[4], l2[4];
-  bfs_round<K, DIM, 2, DIM == 1>(i1, f1, l1, i2, f2, l2, w0, wn, s_lnk, s_pl, keys,
+  bfs_round<K, DIM, 2, DIM == 1>(i1, f1, l1, i2, f2, l2, w0, wn, s_lnk, s_pl, pbase, keys,
                                  conflict_bits, lvl, mbits, node0, found, cmask, miss, st);
   uint32_t i3[8];
   bool f3[8], l3[8];
-  bfs_round<K, DIM, 4, DIM == 2>(i2, f2, l2, i3, f3, l3, w0, wn, s_lnk, s_pl, keys,
+  bfs_round<K, DIM, 4, DIM == 2>(i2, f2, l2, i3, f3, l3, w0, wn, s_lnk, s_pl, pbase, keys,
                                  conflict_bits, lvl, mbits, node0, found, cmask, miss, st);
   if (DIM == 3) {
     uint32_t i4[16];
     bool f4[16], l4[16];
-    bfs_round<K, DIM, 8, true>(i3, f3, l3, i4, f4, l4, w0, wn, s_lnk, s_pl, keys,
+    bfs_round<K, DIM, 8, true>(i3, f3, l3, i4, f4, l4, w0, wn, s_lnk, s_pl, pbase, keys,
                                conflict_bits, lvl, mbits, node0, found, cmask, miss, st);
   }
   if (miss) return false;
@@ -301,10 +306,10 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
   // staged octree words, 16-byte aligned, plus up to 3 words in front so
   // that LDS and global words agree modulo 4 (16-byte copies out)
   __shared__ __align__(16) int32_t s_nodes[(SN * STRIDE + 3 + 3) & ~3];
-  // per window position: packed links, and (inclusive prefix sum of local
-  // splits, packed local split) side by side so that one read gets both
-  __shared__ uint32_t s_lnk[WN];
-  __shared__ uint2 s_pl[WN];
+  // per window position: packed links, and the inclusive prefix sum of
+  // local splits (relative to the window's base) beside the packed local
+  // split in one word, so that one read gets both
+  __shared__ uint32_t s_lnk[WN], s_pl[WN];
   const int t = threadIdx.x;
   const int w0 = (int)q0 - H;
   // Every load of the window is issued before the first is used: addresses
@@ -318,11 +323,12 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
     lk[r] = load_links(brt + (in && g < (int)m - 1 ? g : 0));
     if (!chunk_excl) pv[r] = ps[in ? g : 0];
   }
+  uint32_t base;   // the prefix sum before the window
   if (chunk_excl) {
     // The window is whole chunks (q0 is a multiple of 256), so the base of
     // the first one plus a running scan gives every prefix sum.
     __shared__ uint32_t s_w[WR * 4];
-    const uint32_t base = q0 ? chunk_excl[(q0 >> 8) - 1] : 0u;
+    base = q0 ? chunk_excl[(q0 >> 8) - 1] : 0u;
     uint32_t v[WR], ex[WR];
 #pragma unroll
     for (int r = 0; r < WR; ++r) {
@@ -339,19 +345,23 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
       if (q < m) ps_out[q] = pv[1 + j];
     }
     if (t == 0 && q0 + STAGE == m - 1) ps_out[m - 1] = pv[R + 1];
+  } else {
+    base = w0 > 0 ? ps[w0 - 1] : 0u;
   }
 #pragma unroll
   for (int r = 0; r < WR; ++r) {
     const int k = r * kBlock + t;
-    s_pl[k] = make_uint2(pv[r], lx[r]);
+    s_pl[k] = (pv[r] - base) << kPlShift | (lx[r] & kPlLsx);
     s_lnk[k] = lk[r];
   }
   __syncthreads();
   const uint32_t qlast = min(q0 + STAGE - 1, m - 2);
-  const uint32_t lo = q0 ? s_pl[q0 - 1 - w0].x : 0u;
-  const uint32_t span = s_pl[qlast - w0].x - lo;
+  // prefix sums below are relative to base (they fit 21 bits: a window's
+  // 1024 positions own at most 1024 * 42 octree nodes)
+  const uint32_t lo = q0 ? s_pl[q0 - 1 - w0] >> kPlShift : 0u;
+  const uint32_t span = (s_pl[qlast - w0] >> kPlShift) - lo;
   const bool staged = span <= (uint32_t)SN;   // uniform over the workgroup
-  const uint32_t a0 = (uint32_t)(((uint64_t)lo * STRIDE) & 3u);
+  const uint32_t a0 = (uint32_t)(((uint64_t)(base + lo) * STRIDE) & 3u);
   // The nodes that own octree nodes (about half), compacted so that each
   // lane takes one: entry e = j * 256 + t of the rows.
   __shared__ uint16_t s_aidx[STAGE];
@@ -363,8 +373,8 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
     const uint32_t q = q0 + j * kBlock + t;
     bool active = q <= qlast;
     if (active) {
-      const uint32_t excl = q ? s_pl[q - 1 - w0].x : 0u;
-      active = (q == 0) || s_pl[q - w0].x > excl;
+      const uint32_t excl = q ? s_pl[q - 1 - w0] >> kPlShift : 0u;
+      active = (q == 0) || (s_pl[q - w0] >> kPlShift) > excl;
     }
     amask |= (uint32_t)active << j;
   }
@@ -377,12 +387,14 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
   for (uint32_t a = t; a < nact; a += kBlock) {
     const uint32_t e = s_aidx[a];
     const uint32_t q = q0 + e;
-    const uint32_t excl = q ? s_pl[q - 1 - w0].x : 0u;
-    const uint32_t lsq = s_pl[q - w0].x - excl;
+    const uint32_t excl_r = q ? s_pl[q - 1 - w0] >> kPlShift : 0u;
+    const uint32_t lsq = (s_pl[q - w0] >> kPlShift) - excl_r;
+    const uint32_t excl = base + excl_r;
     bool ok;
     if (staged) {
-      ok = gather_one_bfs<K, DIM>(q, s_lnk[q - w0], brt, excl, lsq, w0, WN, s_lnk, s_pl, keys,
-                                  conflict_bits, mbits, s_nodes + a0 + (excl - lo) * STRIDE,
+      ok = gather_one_bfs<K, DIM>(q, s_lnk[q - w0], brt, excl, lsq, w0, WN, s_lnk, s_pl, base,
+                                  keys, conflict_bits, mbits,
+                                  s_nodes + a0 + (excl_r - lo) * STRIDE,
                                   node_conflict, st);
     } else {
       auto links = [&](uint32_t x, uint32_t &left, bool &ll, bool &rl) -> bool {
@@ -397,8 +409,8 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
       auto child = [&](uint32_t c, uint32_t &psc, uint32_t &lxc) -> bool {
         const unsigned k = (unsigned)((int)c - w0);
         if (k >= (unsigned)WN) return false;
-        psc = s_pl[k].x;
-        lxc = s_pl[k].y;
+        psc = base + (s_pl[k] >> kPlShift);
+        lxc = s_pl[k] & kPlLsx;
         return true;
       };
       auto put = [&](uint32_t idx, const int32_t *c) { write_node<DIM>(octree, idx, c); };
@@ -417,7 +429,7 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
     // LDS words [a0, a0 + words) -> octree words [lo STRIDE, ..): whole
     // 16-byte chunks as one store, the partial first and last chunks by word
     const uint32_t words = span * STRIDE;
-    int32_t *out = octree + (size_t)lo * STRIDE - a0;   // out[i] <-> s_nodes[i]
+    int32_t *out = octree + (size_t)(base + lo) * STRIDE - a0;   // out[i] <-> s_nodes[i]
     const uint32_t end = a0 + words, c0 = a0 >> 2, c1 = (end + 3) >> 2;
     for (uint32_t c = c0 + t; c < c1; c += kBlock) {
       const uint32_t i = c * 4;
