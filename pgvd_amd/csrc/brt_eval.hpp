@@ -88,15 +88,16 @@ __host__ __device__ inline BrtOut<K> brt_eval(K cur, Key key, int mbits, int m, 
   } else {
     // Above 2^23 the FP32 step can exceed the exact ceil; if that changes
     // the split, the tree is not a well-formed radix tree and the octree
-    // stage must take the order-exact scatter path.
-    for (long long den = 2; den < 2ll * l; den *= 2) {
-      const int t = fp32_split_step(l, (int)den);
-      if (s + t <= s_cutoff && above_node(s + t)) s += t;
-    }
+    // stage must take the order-exact scatter path.  The two searches step
+    // side by side, so that their probes do not wait on each other.
     int se = 0;
     for (long long den = 2; den < 2ll * l; den *= 2) {
-      const int t = (int)(((long long)l + den - 1) / den);
-      if (se + t <= s_cutoff && above_node(se + t)) se += t;
+      const int t = fp32_split_step(l, (int)den);
+      const int te = (int)(((long long)l + den - 1) / den);
+      const bool up = s + t <= s_cutoff && above_node(s + t);
+      const bool upe = se + te <= s_cutoff && above_node(se + te);
+      if (up) s += t;
+      if (upe) se += te;
     }
     o.quirk = se != s;
   }
