@@ -9,7 +9,7 @@ radix sort, unique, Karras BRT with local splits, octree fill) over 2^24
 float3 points that are already in HBM.  Prints one JSON line: sorted Morton
 keys/s (whole job), the per-launch roofline of the dominant kernel (the
 octree gather when the bucket sort ran, else an onesweep sort pass), the
-whole-path roofline against the survey's algorithmic byte model, and the
+whole-path roofline against the path's algorithmic byte model, and the
 serial CPU path (the oracle, one core) timed on a bounded sample.
 """
 from __future__ import annotations
@@ -51,8 +51,8 @@ def make_points(workload: str, n: int) -> tuple[np.ndarray, str]:
 
 
 def algorithmic_bytes(n: int, m: int, O: int, dim: int, K: int, R: int, L: int = 0) -> dict:
-    """Algorithmic HBM bytes of the path, term by term as the survey's byte
-    model fixes them (whatever the kernels actually move): P = 4 dim point
+    """Algorithmic HBM bytes of the path, term by term as a fixed byte model
+    of its stages (whatever the kernels actually move): P = 4 dim point
     bytes, K key bytes, L label bytes, R = ceil(mbits/8) radix passes (the
     8-bit-digit convention, also when the bucket sort sorts in fewer passes
     over HBM), B = 16 + K compact BRT node bytes, N = 4 (2^dim + 1) octree
@@ -364,8 +364,8 @@ def main() -> int:
                          "the reference point for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); this build has no "
-              "multi-GPU path -- see DESIGN.md for how the path would shard.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): the path runs on one "
+              "GPU, and its sharding over several is a design, not built.", file=sys.stderr)
         return 2
 
     import pgvd_amd as P

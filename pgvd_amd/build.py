@@ -27,7 +27,8 @@ FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-ffp-contract=off",
-    # packed FP32 instructions stay off (GPU-box traps)
+    # packed FP32 instructions stay off: some of their forms have given
+    # wrong results on this hardware
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
     f"-I{os.path.join(ROOT, 'include')}",
