@@ -3,6 +3,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload uniform|gauss]
     python bench.py --workload batch [--batch-count 4096]   (batched form)
     python bench.py --workload serve [--serve-rate 2000]    (served form)
+    python bench.py --workload findleaf                     (point location)
 
 One step = the whole path (bounding box, FP32 quantize, 63-bit Morton keys,
 radix sort, unique, Karras BRT with local splits, octree fill) over 2^24
@@ -345,13 +346,103 @@ def bench_serve(args) -> int:
     return 0 if not bad and not mism else 1
 
 
+def bench_findleaf(args) -> int:
+    """Batched point location, OctreeUtils::FindLeaf (OctreeUtils.cpp:18-71):
+    2^24 uniform 3D grid points (21 bits/axis, seed 7) located in the octree
+    of the 2^24-point uniform build; queries and octree already in HBM.  One
+    step = one pgvd_find_leaf call over all queries.  Algorithmic bytes per
+    call: 12 B in and 20 B out per query (point; parent, octant, data, level,
+    status) plus one read of the octree (36 B per node)."""
+    import pgvd_amd as P
+
+    dim, bits = 3, 21
+    nq = args.n or (1 << 24)
+    pts, desc = make_points("uniform", 1 << 24)
+    dev = P.Device(0)
+    dpts = dev.upload(pts)
+    dev.build_octree_async(dpts, pts.shape[0], dim, bits)
+    res = dev.build_octree_finish()
+    O = int(res.octree_size)
+    q = np.random.default_rng(7).integers(0, 1 << bits, (nq, dim), dtype=np.int32)
+    dq = dev.upload(q)
+    outs = {k: dev.alloc(nq * 4) for k in ("parent", "octant", "data", "level", "status")}
+    for _ in range(max(args.warmup, 1)):
+        dev.find_leaf_device(dq, nq, dim, bits, res.octree, O, outs)
+    dev.timing(True)
+    dev.timing_reset()
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev.find_leaf_device(dq, nq, dim, bits, res.octree, O, outs)
+    dev.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    kms, kcnt = dev.timing_read("find_leaf")
+    launch_ms = kms / max(kcnt, 1)
+    dev.timing(False)
+    got = {k: dev.download(b, np.uint32 if k == "status" else np.int32, nq) for k, b in outs.items()}
+    octree = dev.download(res.octree, np.int32, O * ((1 << dim) + 1)).reshape(O, -1)
+    levels = np.bincount(got["level"], minlength=bits + 1)
+    dev.close()
+    abytes = nq * (12 + 20) + O * ((1 << dim) + 1) * 4
+    gbps = abytes / (launch_ms * 1e-3) / 1e9
+    out = {
+        "metric": "octree point location (FindLeaf) queries/sec on the 2^24-point octree",
+        "value": nq / (ms_step * 1e-3),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{nq} uniform grid queries (seed 7) in the octree of {desc}, "
+                        f"{bits} bits/axis ({O} octree nodes)",
+            "queries": nq, "octree_nodes": O, "dim": dim, "bits": bits,
+            "failed_queries": int((got["status"] != 0).sum()),
+            "level_histogram": {int(l): int(c) for l, c in enumerate(levels) if c},
+        },
+        "roofline": {
+            "kernel": "find_leaf_kernel (one lane per query, one dependent node read per level)",
+            "bound": "hbm",
+            "achieved": gbps,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": gbps / HBM_PEAK_GBPS,
+            "traffic": load_traffic("find_leaf", "findleaf"),
+            "avg_launch_ms": launch_ms,
+            "launches": kcnt,
+            "algorithmic_bytes": abytes,
+        },
+    }
+    if not args.no_cpu:
+        import oracle
+        s = min(args.cpu_sample, nq, 1 << 22)
+        t0 = time.perf_counter()
+        want = oracle.find_leaf(q[:s], bits, octree[:, :-1], octree[:, -1])
+        dt = time.perf_counter() - t0
+        mism = sum(int((got[k][:s].astype(np.int64) != want[k].astype(np.int64)).sum())
+                   for k in want)
+        out["cpu_baseline"] = {
+            "value": s / dt, "unit": "queries/s", "cores": 1, "kind": "port",
+            "sample": f"first {s} queries, serial FindLeaf restatement on 1 core in {dt:.2f} s",
+            "mismatches_vs_gpu": mism,
+        }
+    print(json.dumps(out), flush=True)
+    return 0 if out["config"]["failed_queries"] == 0 and \
+        out.get("cpu_baseline", {}).get("mismatches_vs_gpu", 0) == 0 else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="uniform",
-                    choices=["uniform", "gauss", "conflict", "batch", "serve"])
+                    choices=["uniform", "gauss", "conflict", "batch", "serve", "findleaf"])
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -374,6 +465,8 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "findleaf":
+        return bench_findleaf(args)
 
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20

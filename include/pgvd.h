@@ -108,6 +108,26 @@ int pgvd_quantize(pgvd_ctx *ctx, const float *pts, int64_t n, int dim, int bits,
 int pgvd_points_to_morton(pgvd_ctx *ctx, const void *pts, int coord_bytes, int64_t n, int dim,
                           int bits, void *keys_out, int key_bytes, pgvd_stream s);
 
+/* OctreeUtils::FindLeaf (OctreeUtils.cpp:18-71), batched: for each of q
+ * grid points (pts: device, q*dim int32; a coordinate's bits from `bits` up
+ * are ignored, as xyz2z ignores them) descend the octree (device,
+ * octree_size nodes of 2^dim children + leaf mask, as pgvd_build_octree
+ * writes it) from node 0 by the point's Morton digits, the octant masked to
+ * dim bits (the reference leaves that mask commented out, :37-41), until the
+ * octant's leaf bit is set.  Per query (device arrays of q):
+ *   parent  the node holding the leaf octant (the OctCell's parent_idx);
+ *   octant  that octant;
+ *   data    children[octant] of that node (-1: an empty cell);
+ *   level   the digits descended: the cell's width is 2^(bits - level), its
+ *           origin the point with its low bits - level bits cleared;
+ *   status  (nullable) 0; 1 when no leaf was reached (the reference
+ *           throws); 2 when a child index leaves the octree (not one the
+ *           build wrote).
+ * 1 <= bits <= 31. */
+int pgvd_find_leaf(pgvd_ctx *ctx, const int32_t *pts, int64_t q, int dim, int bits,
+                   const int32_t *octree, int64_t octree_size, int32_t *parent, int32_t *octant,
+                   int32_t *data, int32_t *level, uint32_t *status, pgvd_stream s);
+
 /* Kernels::RadixSortBigUnsigned (opencl/Kernels.cpp:219-247): ascending
  * stable sort of n keys in place, over the low mbits bits; vals (nullable,
  * u32 per key) is permuted with the keys. */

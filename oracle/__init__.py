@@ -54,6 +54,8 @@ def lib():
         L.oracle_compute_lcp_length.argtypes = [C.c_uint64] * 4 + [C.c_int]
         L.oracle_compute_lcp_length.restype = C.c_int
         L.oracle_stream_scan.argtypes = [u32p, u32p, C.c_int]
+        L.oracle_find_leaf.argtypes = [i64p, C.c_int64, C.c_int, C.c_int, i32p, C.c_int64,
+                                       i32p, i32p, i32p, i32p, i32p]
         L.oracle_build.argtypes = [i64p, C.c_int64, C.c_int, C.c_int, C.c_void_p]
         L.oracle_build.restype = C.c_void_p
         L.oracle_build_from_keys.argtypes = [u64p, u64p, C.c_int64, C.c_int, C.c_int]
@@ -157,6 +159,20 @@ def morton(q: np.ndarray, bits: int):
 def lcp_length(a: int, b: int, mbits: int) -> int:
     M = (1 << 64) - 1
     return lib().oracle_compute_lcp_length(a & M, a >> 64, b & M, b >> 64, mbits)
+
+
+def find_leaf(q: np.ndarray, bits: int, children: np.ndarray, leaf: np.ndarray) -> dict:
+    """OctreeUtils::FindLeaf for every row of q (grid points) on an octree
+    given as children (O x 2^dim) and leaf (O)."""
+    q = np.ascontiguousarray(q, dtype=np.int64)
+    n, dim = q.shape
+    O = children.shape[0]
+    oct_ = np.ascontiguousarray(np.concatenate([children.reshape(O, -1), leaf.reshape(O, 1)],
+                                               axis=1).astype(np.int32))
+    out = {k: np.zeros(n, np.int32) for k in ("parent", "octant", "data", "level", "status")}
+    lib().oracle_find_leaf(q.reshape(-1), n, dim, bits, oct_.reshape(-1), O, out["parent"],
+                           out["octant"], out["data"], out["level"], out["status"])
+    return out
 
 
 def stream_scan(x: np.ndarray) -> np.ndarray:

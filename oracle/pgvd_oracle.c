@@ -492,3 +492,42 @@ int64_t oracle_run_pipeline(const float *pts, int64_t n, int dim, int bits, int6
   oracle_tree_free(t);
   return O;
 }
+
+/* OctreeUtils::FindLeaf (OctreeUtils.cpp:18-71): the point's key digit by
+ * digit from the top (:33-40, the octant masked to dim bits, the fix the
+ * commented-out line :37 intends), width halved per digit (:41), and the
+ * descent stops at the first octant whose leaf bit is set (:47-51); a node
+ * visited without one is an error (:55-69). */
+void oracle_find_leaf(const int64_t *p, int64_t n, int dim, int bits, const int32_t *octree,
+                      int64_t O, int32_t *parent, int32_t *octant, int32_t *data,
+                      int32_t *level, int32_t *status) {
+  const int nc = 1 << dim, stride = nc + 1, mbits = bits * dim;
+  for (int64_t q = 0; q < n; ++q) {
+    uint64_t lo, hi;
+    oracle_xyz2z(p + q * dim, dim, bits, &lo, &hi);
+    const u128 z = mk(lo, hi);
+    int64_t idx = 0;
+    int o = 0, lev = 0, st = 1;
+    int32_t d = -1;
+    for (int i = mbits - dim; i >= 0; i -= dim) {
+      o = (int)(shr(z, i) & (u128)(nc - 1));
+      ++lev;
+      const int32_t *node = octree + idx * stride;
+      if (node[nc] & (1 << o)) {
+        d = node[o];
+        st = 0;
+        break;
+      }
+      if (node[o] < 0 || node[o] >= O) {
+        st = 2;
+        break;
+      }
+      idx = node[o];
+    }
+    parent[q] = (int32_t)idx;
+    octant[q] = o;
+    data[q] = d;
+    level[q] = lev;
+    status[q] = st;
+  }
+}

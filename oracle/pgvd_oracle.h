@@ -90,6 +90,19 @@ int oracle_tree_conflicts(const oracle_tree *t, uint8_t *key_flags,
                           uint32_t *node_masks);
 void oracle_tree_free(oracle_tree *t);
 
+/* OctreeUtils::FindLeaf (OctreeUtils.cpp:18-71) for each of n grid points
+ * p (n*dim int64, coordinates below 2^bits), with the octant masked to dim
+ * bits as the commented-out line there (:37-41) intends.  octree: O nodes of
+ * 2^dim + 1 int32 (children, leaf).  Per point: parent = the node whose
+ * octant is the leaf cell (the OctCell's parent_idx), octant, data =
+ * children[octant] of that node, level = the number of digits descended
+ * (the cell's width is 2^(bits - level)), status = 0, 1 when no leaf is
+ * reached (the reference throws) or 2 when a child index leaves the octree
+ * (not an octree the build produced). */
+void oracle_find_leaf(const int64_t *p, int64_t n, int dim, int bits, const int32_t *octree,
+                      int64_t O, int32_t *parent, int32_t *octant, int32_t *data,
+                      int32_t *level, int32_t *status);
+
 /* Timing helper for the CPU baseline: runs the whole serial path from
  * float points (bbox, quantize, BuildOctree_s) and returns the octree size,
  * or -1 on error.  *m_out receives the unique key count. */

@@ -126,6 +126,7 @@ def lib():
             "pgvd_bbox": ([vp, vp, i64, i32, vp, vp], i32),
             "pgvd_quantize": ([vp, vp, i64, i32, i32, vp, i32, vp, C.POINTER(i64), vp], i32),
             "pgvd_points_to_morton": ([vp, vp, i32, i64, i32, i32, vp, i32, vp], i32),
+            "pgvd_find_leaf": ([vp, vp, i64, i32, i32, vp, i64, vp, vp, vp, vp, vp, vp], i32),
             "pgvd_radix_sort": ([vp, vp, vp, i64, i32, i32, vp], i32),
             "pgvd_unique_sorted": ([vp, vp, vp, i64, i32, C.POINTER(i64), vp], i32),
             "pgvd_stream_scan": ([vp, vp, vp, i64, vp], i32),
@@ -345,6 +346,36 @@ class Device:
             d.free()
             k.free()
         return words_to_keys(words, kb)
+
+    def find_leaf(self, q: np.ndarray, bits: int, children: np.ndarray, leaf: np.ndarray) -> dict:
+        """OctreeUtils::FindLeaf for every row of q (int32 grid points) on an
+        octree given as children (O x 2^dim) and leaf (O) host arrays."""
+        q = np.ascontiguousarray(q, dtype=np.int32)
+        n, dim = q.shape
+        O = children.shape[0]
+        oct_ = np.ascontiguousarray(np.concatenate([children.reshape(O, -1), leaf.reshape(O, 1)],
+                                                   axis=1).astype(np.int32))
+        dq, do = self.upload(q), self.upload(oct_)
+        outs = {k: self.alloc(max(n, 1) * 4) for k in ("parent", "octant", "data", "level", "status")}
+        try:
+            self.find_leaf_device(dq, n, dim, bits, do, O, outs)
+            res = {k: self.download(b, np.uint32 if k == "status" else np.int32, n)
+                   for k, b in outs.items()}
+        finally:
+            for b in [dq, do, *outs.values()]:
+                b.free()
+        return res
+
+    def find_leaf_device(self, dq: DeviceBuffer, n: int, dim: int, bits: int, octree,
+                         octree_size: int, outs: dict) -> None:
+        """pgvd_find_leaf on device buffers (asynchronous); octree is a
+        DeviceBuffer or a device pointer (e.g. OctreeResult.octree)."""
+        optr = octree.ptr if isinstance(octree, DeviceBuffer) else octree
+        st = outs.get("status")
+        self._check(lib().pgvd_find_leaf(self.ctx, dq.ptr, n, dim, bits, optr, octree_size,
+                                         outs["parent"].ptr, outs["octant"].ptr,
+                                         outs["data"].ptr, outs["level"].ptr,
+                                         st.ptr if st is not None else None, None))
 
     def radix_sort(self, lo, hi=None, mbits: int = 64, vals=None, key_bytes: int | None = None):
         kb = key_bytes or key_bytes_for(mbits)

@@ -734,6 +734,24 @@ int pgvd_points_to_morton(pgvd_ctx *ctx, const void *pts, int coord_bytes, int64
   return launch_status("points_to_morton");
 }
 
+int pgvd_find_leaf(pgvd_ctx *ctx, const int32_t *pts, int64_t q, int dim, int bits,
+                   const int32_t *octree, int64_t octree_size, int32_t *parent, int32_t *octant,
+                   int32_t *data, int32_t *level, uint32_t *status, pgvd_stream s) {
+  if (!ctx || !pts || !octree || !parent || !octant || !data || !level)
+    return fail(PGVD_ERR_INVALID, "null argument");
+  if (dim != 2 && dim != 3) return fail(PGVD_ERR_INVALID, "dim must be 2 or 3");
+  if (bits < 1 || bits > 31) return fail(PGVD_ERR_INVALID, "bits must be in 1..31 (int32 grid points)");
+  if (q < 0) return fail(PGVD_ERR_INVALID, "q must be >= 0");
+  if (octree_size < 1) return fail(PGVD_ERR_INVALID, "octree_size must be >= 1");
+  if (q == 0) return PGVD_OK;
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  Stage t(ctx, "find_leaf", st);
+  launch_find_leaf(pts, q, dim, bits, octree, octree_size, parent, octant, data, level, status,
+                   st);
+  return launch_status("find_leaf");
+}
+
 int pgvd_radix_sort(pgvd_ctx *ctx, void *keys, uint32_t *vals, int64_t n, int key_bytes, int mbits,
                     pgvd_stream s) {
   if (!ctx || !keys) return fail(PGVD_ERR_INVALID, "null argument");

@@ -406,3 +406,35 @@ def test_octree_capacity_is_reported(dev):
     with pytest.raises(P.PgvdError) as e:
         dev.brt_to_octree(brt, parent, keys.size, 2, 8, capacity=10)
     assert e.value.code == 7
+
+
+# ------------------------------------------------------------ find leaf ----
+@pytest.mark.parametrize("dim,bits,n", [(2, 16, 65536), (3, 21, 1 << 20), (3, 10, 200000),
+                                        (2, 30, 50000)])
+def test_find_leaf_matches_oracle(dev, dim, bits, n):
+    """OctreeUtils::FindLeaf batched on the GPU against the oracle's
+    restatement (pinned by brute force), on the octree the GPU built: the
+    build's own points and random grid points, every output field."""
+    rng = np.random.default_rng(dim * 31 + bits)
+    pts = _pts(n, dim, seed=dim * 7 + bits)
+    tree = dev.build_octree(pts, bits)
+    q = oracle.quantize(pts, 1 << bits)
+    queries = np.concatenate([q, rng.integers(0, 1 << bits, (n // 4, dim))]).astype(np.int32)
+    got = dev.find_leaf(queries, bits, tree["children"], tree["leaf"])
+    want = oracle.find_leaf(queries, bits, tree["children"], tree["leaf"])
+    for k in ("parent", "octant", "data", "level", "status"):
+        assert np.array_equal(got[k].astype(np.int64), want[k].astype(np.int64)), k
+    assert (want["status"] == 0).all() and (want["level"] > 1).any()
+
+
+def test_find_leaf_rejects_foreign_child_index(dev):
+    """A child index outside the octree stops the descent with status 2 (the
+    oracle agrees) instead of reading past the nodes."""
+    children = np.array([[3, -1, -1, -1], [-1, -1, -1, -1]], np.int32)   # node 0 -> node 3 of 2
+    leaf = np.array([0b1110, 0b1111], np.int32)
+    queries = np.array([[0, 0], [1, 0], [1, 1]], np.int32)
+    got = dev.find_leaf(queries, 1, children, leaf)
+    want = oracle.find_leaf(queries, 1, children, leaf)
+    for k in want:
+        assert np.array_equal(got[k].astype(np.int64), want[k].astype(np.int64)), k
+    assert list(want["status"]) == [2, 0, 0]

@@ -290,3 +290,59 @@ def test_quirk_fixture_distinguishes_fp32_from_exact_ceil(golden_digests):
             s += t
         den *= 2
     assert s == s_star != golden_digests["quirk_2d"]["root_left"]
+
+
+# ------------------------------------------------------------- find leaf ----
+def _reachable_cells(children, dim):
+    """Level and cell (the octant digits from node 0 down, as a tuple) of
+    every node reachable from node 0, by walking child links top-down."""
+    nc = 1 << dim
+    cells = {0: (0, ())}
+    stack = [0]
+    while stack:
+        node = stack.pop()
+        lvl, path = cells[node]
+        for o in range(nc):
+            c = int(children[node, o])
+            if c >= 0:
+                cells[c] = (lvl + 1, path + (o,))
+                stack.append(c)
+    return cells
+
+
+@pytest.mark.parametrize("dim,bits,n", [(2, 16, 3000), (3, 21, 2000), (3, 6, 1500), (2, 30, 800)])
+def test_find_leaf_against_brute_force(dim, bits, n):
+    """OctreeUtils::FindLeaf (oracle) against a brute force: label every
+    reachable node with its level and the digits of its cell, and take for
+    each query the deepest node whose cell holds it; its octant for the query
+    must be a leaf, and that is the reference's result."""
+    rng = np.random.default_rng(dim * 1000 + bits)
+    pts = (rng.random((n, dim), dtype=np.float32) * 2 - 1)
+    pts[: n // 3] *= np.float32(0.01)                      # a dense cluster too
+    q = oracle.quantize(pts, 1 << bits)
+    t = oracle.build(q, bits)
+    queries = np.concatenate([q, rng.integers(0, 1 << bits, (400, dim))]).astype(np.int64)
+    got = oracle.find_leaf(queries, bits, t["children"], t["leaf"])
+    nc = 1 << dim
+    cells = _reachable_cells(t["children"], dim)
+    by_path = {path: node for node, (lvl, path) in cells.items()}
+
+    def digit(p, lvl):   # the point's octant below a node of level lvl
+        return sum(((int(p[k]) >> (bits - 1 - lvl)) & 1) << k for k in range(dim))
+
+    for i, p in enumerate(queries):
+        path, node, lvl = (), 0, 0
+        while lvl < bits:
+            nxt = path + (digit(p, lvl),)
+            if nxt not in by_path:
+                break
+            path, node, lvl = nxt, by_path[nxt], lvl + 1
+        o = digit(p, lvl)
+        assert (int(t["leaf"][node]) >> o) & 1, (i, node, o)
+        assert got["status"][i] == 0
+        assert (got["parent"][i], got["octant"][i], got["level"][i]) == (node, o, lvl + 1), i
+        assert got["data"][i] == t["children"][node, o] == -1
+    # the octant mask matters: below the first digit the unmasked value of
+    # the reference's line 37 would exceed 2^dim - 1 (an index past the
+    # node); most queries descend further than that
+    assert int((got["level"] > 2).sum()) > queries.shape[0] // 2
