@@ -406,7 +406,9 @@ def bench_findleaf(args) -> int:
             "level_histogram": {int(l): int(c) for l, c in enumerate(levels) if c},
         },
         "roofline": {
-            "kernel": "find_leaf_kernel (one lane per query, one dependent node read per level)",
+            "kernel": ("find_leaf_table_kernel + find_leaf_kernel (the octree's cells at depth 6 "
+                       "tabulated, then one lane per query: one table read, then one dependent "
+                       "node read per level; timed together)"),
             "bound": "hbm",
             "achieved": gbps,
             "peak": HBM_PEAK_GBPS,

@@ -746,9 +746,12 @@ int pgvd_find_leaf(pgvd_ctx *ctx, const int32_t *pts, int64_t q, int dim, int bi
   if (q == 0) return PGVD_OK;
   CHK(use_device(ctx));
   hipStream_t st = pick(ctx, s);
+  const int D = find_leaf_table_depth(q, dim, bits);
+  void *table = nullptr;
+  if (D > 0) CHK(pool_get(ctx, "find_leaf_table", size_t(8) << (D * dim), &table));
   Stage t(ctx, "find_leaf", st);
-  launch_find_leaf(pts, q, dim, bits, octree, octree_size, parent, octant, data, level, status,
-                   st);
+  launch_find_leaf(pts, q, dim, bits, octree, octree_size, table, D, parent, octant, data, level,
+                   status, st);
   return launch_status("find_leaf");
 }
 

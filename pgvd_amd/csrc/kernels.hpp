@@ -173,10 +173,14 @@ int launch_onesweep(K *keys, K *keys_alt, const uint32_t *vals_in, uint32_t *val
                     hipEvent_t *pass_events = nullptr, const RunState *skip_if_msd = nullptr);
 
 // ---- query.hip ----
-// OctreeUtils::FindLeaf for q int32 grid points (see pgvd_find_leaf).
+// OctreeUtils::FindLeaf for q int32 grid points (see pgvd_find_leaf).  With
+// D > 0 the launch first fills `table` (8 << (D * dim) bytes) with the
+// octree's cells at depth D; find_leaf_table_depth picks D (0: no table).
+constexpr int kFindLeafTableBits = 18;
+int find_leaf_table_depth(int64_t q, int dim, int bits);
 void launch_find_leaf(const int32_t *pts, int64_t q, int dim, int bits, const int32_t *octree,
-                      int64_t octree_size, int32_t *parent, int32_t *octant, int32_t *data,
-                      int32_t *level, uint32_t *status, hipStream_t s);
+                      int64_t octree_size, void *table, int D, int32_t *parent, int32_t *octant,
+                      int32_t *data, int32_t *level, uint32_t *status, hipStream_t s);
 
 // ---- msd.hip ----
 // Bucket sort of 64-bit keys without payload: scatter by the top b bits,

@@ -410,11 +410,16 @@ def test_octree_capacity_is_reported(dev):
 
 # ------------------------------------------------------------ find leaf ----
 @pytest.mark.parametrize("dim,bits,n", [(2, 16, 65536), (3, 21, 1 << 20), (3, 10, 200000),
-                                        (2, 30, 50000)])
+                                        (2, 30, 50000), (2, 16, 1 << 19), (3, 5, 100000),
+                                        (2, 2, 40)])
 def test_find_leaf_matches_oracle(dev, dim, bits, n):
     """OctreeUtils::FindLeaf batched on the GPU against the oracle's
     restatement (pinned by brute force), on the octree the GPU built: the
-    build's own points and random grid points, every output field."""
+    build's own points and random grid points, every output field.  Batches
+    of at least twice the cell count at the table depth (2^18 cells, fewer
+    when bits * dim is below 18) start from the depth table: here 3D 21 bits,
+    2D 16 bits at 2^19 points, and 3D 5 bits and 2D 2 bits, where the table
+    covers every level."""
     rng = np.random.default_rng(dim * 31 + bits)
     pts = _pts(n, dim, seed=dim * 7 + bits)
     tree = dev.build_octree(pts, bits)
@@ -427,14 +432,16 @@ def test_find_leaf_matches_oracle(dev, dim, bits, n):
     assert (want["status"] == 0).all() and (want["level"] > 1).any()
 
 
-def test_find_leaf_rejects_foreign_child_index(dev):
+@pytest.mark.parametrize("reps", [1, 4])
+def test_find_leaf_rejects_foreign_child_index(dev, reps):
     """A child index outside the octree stops the descent with status 2 (the
-    oracle agrees) instead of reading past the nodes."""
+    oracle agrees) instead of reading past the nodes; with 12 queries the
+    descent starts from the depth table, which records the stop."""
     children = np.array([[3, -1, -1, -1], [-1, -1, -1, -1]], np.int32)   # node 0 -> node 3 of 2
     leaf = np.array([0b1110, 0b1111], np.int32)
-    queries = np.array([[0, 0], [1, 0], [1, 1]], np.int32)
+    queries = np.tile(np.array([[0, 0], [1, 0], [1, 1]], np.int32), (reps, 1))
     got = dev.find_leaf(queries, 1, children, leaf)
     want = oracle.find_leaf(queries, 1, children, leaf)
     for k in want:
         assert np.array_equal(got[k].astype(np.int64), want[k].astype(np.int64)), k
-    assert list(want["status"]) == [2, 0, 0]
+    assert list(want["status"]) == [2, 0, 0] * reps
