@@ -4,6 +4,7 @@
     python bench.py --workload batch [--batch-count 4096]   (batched form)
     python bench.py --workload serve [--serve-rate 2000]    (served form)
     python bench.py --workload findleaf                     (point location)
+    python bench.py --workload octree_text                  (text form of the octree)
 
 One step = the whole path (bounding box, FP32 quantize, 63-bit Morton keys,
 radix sort, unique, Karras BRT with local splits, octree fill) over 2^24
@@ -438,13 +439,131 @@ def bench_findleaf(args) -> int:
         out.get("cpu_baseline", {}).get("mismatches_vs_gpu", 0) == 0 else 1
 
 
+def bench_text(args) -> int:
+    """The octree's text form (operator<< / operator>>, C/OctNode.h:58-98) on
+    the device: the octree of the 2^24-point uniform 3D build (21 bits/axis)
+    formatted to text in HBM, and that text parsed back to records.  One step
+    = one pgvd_octree_format_text call (the metric) and one
+    pgvd_octree_parse_text call, each returning once its length or count is
+    read back.  Algorithmic bytes: format reads 36 B per node and writes the
+    text; parse reads the text and writes 36 B per node."""
+    import pgvd_amd as P
+
+    dim, bits = 3, 21
+    pts, desc = make_points("uniform", 1 << 24)
+    dev = P.Device(0)
+    dpts = dev.upload(pts)
+    dev.build_octree_async(dpts, pts.shape[0], dim, bits)
+    res = dev.build_octree_finish()
+    O = int(res.octree_size)
+    rec_bytes = O * ((1 << dim) + 1) * 4
+    L = P.lib()
+    ln, o2 = C.c_int64(), C.c_int64()
+    dev._check(L.pgvd_octree_format_text(dev.ctx, res.octree, O, dim, None, 0, C.byref(ln), None))
+    nt = ln.value
+    tbuf, obuf = dev.alloc(nt), dev.alloc(rec_bytes)
+
+    def fmt():
+        dev._check(L.pgvd_octree_format_text(dev.ctx, res.octree, O, dim, tbuf.ptr, nt,
+                                             C.byref(ln), None))
+
+    def prs():
+        dev._check(L.pgvd_octree_parse_text(dev.ctx, tbuf.ptr, nt, dim, obuf.ptr, O,
+                                            C.byref(o2), None))
+
+    for _ in range(max(args.warmup, 1)):
+        fmt()
+        prs()
+    dev.timing(True)
+    dev.timing_reset()
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fmt()
+    fmt_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        prs()
+    prs_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    f_tot, f_cnt = dev.timing_read("format_text")
+    p_tot, p_cnt = dev.timing_read("parse_text")
+    dev.timing(False)
+    f_launch, p_launch = f_tot / max(f_cnt, 1), p_tot / max(p_cnt, 1)
+    same = dev.digest(obuf, rec_bytes) == dev.digest(res.octree, rec_bytes)
+    text_host = dev.download(tbuf, np.uint8, nt).tobytes() if not args.no_cpu else None
+    octree_host = dev.download(res.octree, np.int32, O * ((1 << dim) + 1)) if not args.no_cpu else None
+    dev.close()
+    f_bytes, p_bytes = rec_bytes + nt, nt + rec_bytes
+    f_gbps = f_bytes / (f_launch * 1e-3) / 1e9
+    out = {
+        "metric": "octree text form (operator<<) bytes/s formatted, 2^24-point octree",
+        "value": nt / (fmt_ms * 1e-3),
+        "unit": "bytes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fmt_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i32/u8",
+        "data": "synthetic",
+        "config": {
+            "workload": f"octree of {desc}, {bits} bits/axis: {O} nodes formatted to {nt} bytes "
+                        "of text in HBM, then parsed back",
+            "octree_nodes": O, "text_bytes": nt, "parsed_equal_by_digest": bool(same),
+        },
+        "parse": {"ms_per_step": prs_ms, "bytes_per_s": nt / (prs_ms * 1e-3),
+                  "avg_launch_ms": p_launch,
+                  "achieved_GBps": p_bytes / (p_launch * 1e-3) / 1e9,
+                  "frac": p_bytes / (p_launch * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+                  "traffic": load_traffic("parse_text", "octree_text")},
+        "roofline": {
+            "kernel": "format_text_kernel (256 nodes per workgroup printed into LDS, look-back "
+                      "offsets; with its workspace clear)",
+            "bound": "hbm",
+            "achieved": f_gbps,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": f_gbps / HBM_PEAK_GBPS,
+            "traffic": load_traffic("format_text", "octree_text"),
+            "avg_launch_ms": f_launch,
+            "launches": f_cnt,
+            "algorithmic_bytes": f_bytes,
+        },
+    }
+    if not args.no_cpu:
+        import oracle
+        stride = (1 << dim) + 1
+        s = min(O, 1 << 20)
+        sub = octree_host[: s * stride]
+        t0 = time.perf_counter()
+        txt = oracle.octree_format_text(sub, dim)
+        dt = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        back = oracle.octree_parse_text(txt, dim, cap=s)
+        dtp = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": len(txt) / dt, "unit": "bytes/s", "cores": 1, "kind": "port",
+            "sample": f"first {s} nodes formatted by the serial restatement on 1 core in {dt:.2f} s "
+                      f"(parse: {len(txt) / dtp:.3g} bytes/s)",
+            "gpu_text_equal": bool(text_host == oracle.octree_format_text(
+                octree_host.reshape(O, stride), dim)),
+            "parse_equal": bool(back is not None and np.array_equal(back.reshape(-1), sub)),
+        }
+    print(json.dumps(out), flush=True)
+    ok = same and out.get("cpu_baseline", {}).get("gpu_text_equal", True)
+    return 0 if ok else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="uniform",
-                    choices=["uniform", "gauss", "conflict", "batch", "serve", "findleaf"])
+                    choices=["uniform", "gauss", "conflict", "batch", "serve", "findleaf",
+                             "octree_text"])
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -469,6 +588,8 @@ def main() -> int:
         return bench_serve(args)
     if args.workload == "findleaf":
         return bench_findleaf(args)
+    if args.workload == "octree_text":
+        return bench_text(args)
 
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20

@@ -131,6 +131,42 @@ int pgvd_find_leaf(pgvd_ctx *ctx, const int32_t *pts, int64_t q, int dim, int bi
                    const int32_t *octree, int64_t octree_size, int32_t *parent, int32_t *octant,
                    int32_t *data, int32_t *level, uint32_t *status, pgvd_stream s);
 
+/* The octree's text form: operator<< for std::vector<OctNode>
+ * (C/OctNode.h:58-87, commented out in the reference), formatted on the
+ * device.  The text is the node count and a space, then for every node each
+ * of its 2^dim children and its leaf mask in decimal, each followed by one
+ * space.  octree: device, O records; text: device, capacity bytes, or NULL
+ * to measure.  *len_out (host) receives the text's length;
+ * PGVD_ERR_CAPACITY if it exceeds capacity (the buffer's contents are then
+ * unspecified).  Synchronous: returns when *len_out is known. */
+int pgvd_octree_format_text(pgvd_ctx *ctx, const int32_t *octree, int64_t O, int dim, char *text,
+                            int64_t capacity, int64_t *len_out, pgvd_stream s);
+
+/* operator>> for std::vector<OctNode> (C/OctNode.h:69-98), parsed on the
+ * device: the node count N, then N records of 2^dim + 1 integers separated
+ * by any whitespace, the leaf mask cast to unsigned char as the reference
+ * reads it; tokens after the last record are ignored.  text: device, len
+ * bytes; octree: device, room for cap records.  *O_out (host) receives N;
+ * PGVD_ERR_CAPACITY if N > cap.  PGVD_ERR_INVALID when the count is not in
+ * the first 4096 bytes, a record token is not an optional sign and 1 to 10
+ * digits within int32 (where operator>> would set failbit), or the text
+ * ends early.  Synchronous. */
+int pgvd_octree_parse_text(pgvd_ctx *ctx, const char *text, int64_t len, int dim, int32_t *octree,
+                           int64_t cap, int64_t *O_out, pgvd_stream s);
+
+/* Octree files, replacing the reference's (commented-out) stream operators
+ * on files.  PGVD_OCTREE_TEXT is the text form above; PGVD_OCTREE_BINARY is
+ * a 32-byte header ("PGVDOCT1", uint32 dim, uint32 record bytes, int64 node
+ * count, 8 zero bytes) followed by the records as the device holds them
+ * (little-endian int32).  Load tells the two apart by the header.  Both run
+ * on the context's stream and return when done; the octree pointer is
+ * device memory. */
+enum { PGVD_OCTREE_TEXT = 0, PGVD_OCTREE_BINARY = 1 };
+int pgvd_octree_save(pgvd_ctx *ctx, const int32_t *octree, int64_t O, int dim, const char *path,
+                     int format);
+int pgvd_octree_load(pgvd_ctx *ctx, const char *path, int dim, int32_t *octree, int64_t cap,
+                     int64_t *O_out);
+
 /* Kernels::RadixSortBigUnsigned (opencl/Kernels.cpp:219-247): ascending
  * stable sort of n keys in place, over the low mbits bits; vals (nullable,
  * u32 per key) is permuted with the keys. */

@@ -54,6 +54,10 @@ def lib():
         L.oracle_compute_lcp_length.argtypes = [C.c_uint64] * 4 + [C.c_int]
         L.oracle_compute_lcp_length.restype = C.c_int
         L.oracle_stream_scan.argtypes = [u32p, u32p, C.c_int]
+        L.oracle_octree_format_text.argtypes = [i32p, C.c_int64, C.c_int, C.c_char_p]
+        L.oracle_octree_format_text.restype = C.c_int64
+        L.oracle_octree_parse_text.argtypes = [C.c_char_p, C.c_int64, C.c_int, i32p, C.c_int64,
+                                               C.POINTER(C.c_int64)]
         L.oracle_find_leaf.argtypes = [i64p, C.c_int64, C.c_int, C.c_int, i32p, C.c_int64,
                                        i32p, i32p, i32p, i32p, i32p]
         L.oracle_build.argtypes = [i64p, C.c_int64, C.c_int, C.c_int, C.c_void_p]
@@ -173,6 +177,28 @@ def find_leaf(q: np.ndarray, bits: int, children: np.ndarray, leaf: np.ndarray) 
     lib().oracle_find_leaf(q.reshape(-1), n, dim, bits, oct_.reshape(-1), O, out["parent"],
                            out["octant"], out["data"], out["level"], out["status"])
     return out
+
+
+def octree_format_text(octree: np.ndarray, dim: int) -> bytes:
+    """operator<< of the octree (O x (2^dim + 1) int32 records) as bytes."""
+    o = np.ascontiguousarray(octree, dtype=np.int32).reshape(-1)
+    O = o.size // ((1 << dim) + 1)
+    n = lib().oracle_octree_format_text(o, O, dim, None)
+    buf = C.create_string_buffer(int(n))
+    lib().oracle_octree_format_text(o, O, dim, buf)
+    return buf.raw[:n]
+
+
+def octree_parse_text(text: bytes, dim: int, cap: int | None = None):
+    """operator>> of an octree; returns the O x (2^dim + 1) records, or None
+    when the text is malformed (-1) or holds more than cap nodes (-2)."""
+    cap = len(text) if cap is None else cap
+    out = np.zeros(max(cap, 1) * ((1 << dim) + 1), np.int32)
+    O = C.c_int64()
+    rc = lib().oracle_octree_parse_text(text, len(text), dim, out, cap, C.byref(O))
+    if rc:
+        return None
+    return out[: O.value * ((1 << dim) + 1)].reshape(O.value, -1)
 
 
 def stream_scan(x: np.ndarray) -> np.ndarray:

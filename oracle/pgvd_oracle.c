@@ -531,3 +531,71 @@ void oracle_find_leaf(const int64_t *p, int64_t n, int dim, int bits, const int3
     status[q] = st;
   }
 }
+
+/* C/OctNode.h:58-87 (operator<< for OctNode and for the vector): the size,
+ * a space, then every node followed by a space, a node being its children
+ * and its leaf (as int) separated by spaces. */
+static int64_t put_int(char *out, int64_t at, int32_t v) {
+  char tmp[12];
+  int k = 0;
+  uint32_t u = v < 0 ? (uint32_t)0 - (uint32_t)v : (uint32_t)v;
+  do { tmp[k++] = (char)('0' + u % 10); u /= 10; } while (u);
+  if (v < 0) tmp[k++] = '-';
+  for (int j = 0; j < k; ++j) { if (out) out[at + j] = tmp[k - 1 - j]; }
+  if (out) out[at + k] = ' ';
+  return at + k + 1;
+}
+
+int64_t oracle_octree_format_text(const int32_t *octree, int64_t O, int dim, char *out) {
+  const int stride = (1 << dim) + 1;
+  int64_t at = 0;
+  char hdr[24];
+  int h = 0;
+  int64_t n = O;
+  do { hdr[h++] = (char)('0' + n % 10); n /= 10; } while (n);
+  for (int j = 0; j < h; ++j) { if (out) out[at + j] = hdr[h - 1 - j]; }
+  at += h;
+  if (out) out[at] = ' ';
+  ++at;
+  for (int64_t i = 0; i < O * stride; ++i) at = put_int(out, at, octree[i]);
+  return at;
+}
+
+/* C/OctNode.h:69-98 (operator>>): whitespace-separated integers. */
+static int is_ws(char c) { return c == ' ' || c == '\n' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+
+static int next_int(const char *t, int64_t len, int64_t *at, int64_t *v) {
+  int64_t i = *at;
+  while (i < len && is_ws(t[i])) ++i;
+  if (i >= len) return -1;
+  int neg = 0;
+  if (t[i] == '+' || t[i] == '-') { neg = t[i] == '-'; ++i; }
+  int64_t x = 0;
+  int nd = 0;
+  while (i < len && !is_ws(t[i])) {
+    if (t[i] < '0' || t[i] > '9' || nd >= 10) return -1;
+    x = x * 10 + (t[i] - '0');
+    ++nd;
+    ++i;
+  }
+  if (nd == 0) return -1;
+  if (neg) x = -x;
+  *at = i;
+  *v = x;
+  return 0;
+}
+
+int oracle_octree_parse_text(const char *text, int64_t len, int dim, int32_t *octree,
+                             int64_t cap, int64_t *O_out) {
+  const int stride = (1 << dim) + 1;
+  int64_t at = 0, n;
+  if (next_int(text, len, &at, &n) || n < 0) return -1;
+  *O_out = n;
+  if (n > cap) return -2;
+  for (int64_t i = 0; i < n * stride; ++i) {
+    int64_t v;
+    if (next_int(text, len, &at, &v) || v < INT32_MIN || v > INT32_MAX) return -1;
+    octree[i] = (i % stride == stride - 1) ? (int32_t)(uint8_t)v : (int32_t)v;
+  }
+  return 0;
+}

@@ -103,6 +103,18 @@ void oracle_find_leaf(const int64_t *p, int64_t n, int dim, int bits, const int3
                       int64_t O, int32_t *parent, int32_t *octant, int32_t *data,
                       int32_t *level, int32_t *status);
 
+/* Text form of an octree: operator<< / operator>> for OctNode and
+ * std::vector<OctNode> (C/OctNode.h:58-98, commented out in the reference).
+ * format writes "N " then, per node, each child and the leaf mask in
+ * decimal, each followed by one space; returns the length (out may be NULL
+ * to measure).  parse reads N and then N nodes of 2^dim + 1 integers
+ * separated by whitespace (the leaf cast to unsigned char, :76-77); a token
+ * that is not an optional sign and 1..10 digits in int32 range, or too few
+ * tokens, returns -1; more than `cap` nodes returns -2. */
+int64_t oracle_octree_format_text(const int32_t *octree, int64_t O, int dim, char *out);
+int oracle_octree_parse_text(const char *text, int64_t len, int dim, int32_t *octree,
+                             int64_t cap, int64_t *O_out);
+
 /* Timing helper for the CPU baseline: runs the whole serial path from
  * float points (bbox, quantize, BuildOctree_s) and returns the octree size,
  * or -1 on error.  *m_out receives the unique key count. */

@@ -127,6 +127,10 @@ def lib():
             "pgvd_quantize": ([vp, vp, i64, i32, i32, vp, i32, vp, C.POINTER(i64), vp], i32),
             "pgvd_points_to_morton": ([vp, vp, i32, i64, i32, i32, vp, i32, vp], i32),
             "pgvd_find_leaf": ([vp, vp, i64, i32, i32, vp, i64, vp, vp, vp, vp, vp, vp], i32),
+            "pgvd_octree_format_text": ([vp, vp, i64, i32, vp, i64, C.POINTER(i64), vp], i32),
+            "pgvd_octree_parse_text": ([vp, vp, i64, i32, vp, i64, C.POINTER(i64), vp], i32),
+            "pgvd_octree_save": ([vp, vp, i64, i32, C.c_char_p, i32], i32),
+            "pgvd_octree_load": ([vp, C.c_char_p, i32, vp, i64, C.POINTER(i64)], i32),
             "pgvd_radix_sort": ([vp, vp, vp, i64, i32, i32, vp], i32),
             "pgvd_unique_sorted": ([vp, vp, vp, i64, i32, C.POINTER(i64), vp], i32),
             "pgvd_stream_scan": ([vp, vp, vp, i64, vp], i32),
@@ -376,6 +380,54 @@ class Device:
                                          outs["parent"].ptr, outs["octant"].ptr,
                                          outs["data"].ptr, outs["level"].ptr,
                                          st.ptr if st is not None else None, None))
+
+    # ---- the octree's text and binary forms (C/OctNode.h:58-98) ----
+    def format_octree_text(self, octree, O: int, dim: int) -> bytes:
+        """operator<< of O device records (a DeviceBuffer or pointer) as bytes."""
+        optr = octree.ptr if isinstance(octree, DeviceBuffer) else octree
+        n = C.c_int64()
+        self._check(lib().pgvd_octree_format_text(self.ctx, optr, O, dim, None, 0, C.byref(n), None))
+        buf = self.alloc(n.value)
+        try:
+            self._check(lib().pgvd_octree_format_text(self.ctx, optr, O, dim, buf.ptr, n.value,
+                                                      C.byref(n), None))
+            return self.download(buf, np.uint8, n.value).tobytes()
+        finally:
+            buf.free()
+
+    def parse_octree_text(self, text: bytes, dim: int) -> np.ndarray:
+        """operator>> of text; returns the O x (2^dim + 1) int32 records."""
+        stride = (1 << dim) + 1
+        cap = len(text) // (2 * stride) + 1      # a record token takes two bytes at least
+        dt = self.upload(np.frombuffer(text, np.uint8))
+        do = self.alloc(cap * stride * 4)
+        O = C.c_int64()
+        try:
+            self._check(lib().pgvd_octree_parse_text(self.ctx, dt.ptr, len(text), dim, do.ptr, cap,
+                                                     C.byref(O), None))
+            return self.download(do, np.int32, O.value * stride).reshape(O.value, stride)
+        finally:
+            dt.free()
+            do.free()
+
+    def save_octree(self, octree, O: int, dim: int, path: str, binary: bool = False) -> None:
+        optr = octree.ptr if isinstance(octree, DeviceBuffer) else octree
+        self._check(lib().pgvd_octree_save(self.ctx, optr, O, dim, os.fsencode(path),
+                                           1 if binary else 0))
+
+    def load_octree(self, path: str, dim: int) -> np.ndarray:
+        """Reads an octree file (either form) to the device and returns its records."""
+        stride = (1 << dim) + 1
+        size = os.path.getsize(path)
+        cap = max(size // (2 * stride), size // (4 * stride)) + 1
+        do = self.alloc(cap * stride * 4)
+        O = C.c_int64()
+        try:
+            self._check(lib().pgvd_octree_load(self.ctx, os.fsencode(path), dim, do.ptr, cap,
+                                               C.byref(O)))
+            return self.download(do, np.int32, O.value * stride).reshape(O.value, stride)
+        finally:
+            do.free()
 
     def radix_sort(self, lo, hi=None, mbits: int = 64, vals=None, key_bytes: int | None = None):
         kb = key_bytes or key_bytes_for(mbits)

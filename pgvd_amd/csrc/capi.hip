@@ -755,6 +755,197 @@ int pgvd_find_leaf(pgvd_ctx *ctx, const int32_t *pts, int64_t q, int dim, int bi
   return launch_status("find_leaf");
 }
 
+int pgvd_octree_format_text(pgvd_ctx *ctx, const int32_t *octree, int64_t O, int dim, char *text,
+                            int64_t capacity, int64_t *len_out, pgvd_stream s) {
+  if (!ctx || !len_out || (O > 0 && !octree)) return fail(PGVD_ERR_INVALID, "null argument");
+  if (dim != 2 && dim != 3) return fail(PGVD_ERR_INVALID, "dim must be 2 or 3");
+  if (O < 0 || O > INT32_MAX) return fail(PGVD_ERR_INVALID, "O must be in [0, 2^31)");
+  if (capacity < 0) return fail(PGVD_ERR_INVALID, "capacity must be >= 0");
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  char hdr[24];
+  const int hl = std::snprintf(hdr, sizeof hdr, "%lld ", static_cast<long long>(O));
+  if (O == 0) {
+    *len_out = hl;
+    if (!text) return PGVD_OK;
+    if (capacity < hl) return fail(PGVD_ERR_CAPACITY, "text buffer too small (*len_out bytes needed)");
+    HIPCHK(hipMemcpy(text, hdr, hl, hipMemcpyHostToDevice));
+    return PGVD_OK;
+  }
+  void *ws = nullptr;
+  CHK(pool_get(ctx, "text_ws", 8 * text_ws_words(format_text_tiles(O)), &ws));
+  uint64_t total = 0;
+  {
+    Stage t(ctx, "format_text", st);
+    launch_format_text(octree, O, dim, hl, reinterpret_cast<uint8_t *>(text), capacity,
+                       static_cast<uint64_t *>(ws), st);
+    CHK(launch_status("format_text"));
+  }
+  HIPCHK(hipMemcpyAsync(&total, static_cast<uint64_t *>(ws) + 1, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *len_out = static_cast<int64_t>(total);
+  if (text && static_cast<int64_t>(total) > capacity)
+    return fail(PGVD_ERR_CAPACITY, "text buffer too small (*len_out bytes needed)");
+  return PGVD_OK;
+}
+
+namespace {
+inline bool text_space(char c) {
+  return c == ' ' || c == '\n' || c == '\t' || c == '\r' || c == '\v' || c == '\f';
+}
+}  // namespace
+
+int pgvd_octree_parse_text(pgvd_ctx *ctx, const char *text, int64_t len, int dim, int32_t *octree,
+                           int64_t cap, int64_t *O_out, pgvd_stream s) {
+  if (!ctx || !text || !O_out || (cap > 0 && !octree)) return fail(PGVD_ERR_INVALID, "null argument");
+  if (dim != 2 && dim != 3) return fail(PGVD_ERR_INVALID, "dim must be 2 or 3");
+  if (len < 1 || len >= (int64_t(1) << 42)) return fail(PGVD_ERR_INVALID, "len must be in [1, 2^42)");
+  if (cap < 0) return fail(PGVD_ERR_INVALID, "cap must be >= 0");
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  // the node count, the first token, is read here
+  std::vector<char> head(static_cast<size_t>(std::min<int64_t>(len, 4096)));
+  HIPCHK(hipMemcpyAsync(head.data(), text, head.size(), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  size_t i = 0;
+  while (i < head.size() && text_space(head[i])) ++i;
+  if (i < head.size() && head[i] == '+') ++i;
+  int64_t N = 0;
+  int nd = 0;
+  while (i < head.size() && !text_space(head[i])) {
+    if (head[i] < '0' || head[i] > '9' || nd >= 10) return fail(PGVD_ERR_INVALID, "the node count is not a number");
+    N = N * 10 + (head[i] - '0');
+    ++nd;
+    ++i;
+  }
+  if (nd == 0 || (i == head.size() && static_cast<int64_t>(i) < len))
+    return fail(PGVD_ERR_INVALID, "no node count in the first 4096 bytes");
+  if (N > INT32_MAX) return fail(PGVD_ERR_INVALID, "node count above 2^31 - 1");
+  *O_out = N;
+  if (N > cap) return fail(PGVD_ERR_CAPACITY, "octree buffer too small (*O_out nodes needed)");
+  if (N == 0) return PGVD_OK;
+  const int64_t tiles = parse_text_tiles(len);
+  void *ws = nullptr;
+  CHK(pool_get(ctx, "text_ws", 8 * text_ws_words(tiles), &ws));
+  uint64_t res[2] = {0, 0};
+  {
+    Stage t(ctx, "parse_text", st);
+    launch_parse_text(reinterpret_cast<const uint8_t *>(text), len, dim, N, octree,
+                      static_cast<uint64_t *>(ws), st);
+    CHK(launch_status("parse_text"));
+  }
+  HIPCHK(hipMemcpyAsync(res, static_cast<uint64_t *>(ws) + 1, 16, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const uint64_t need = static_cast<uint64_t>(N) * ((1u << dim) + 1) + 1;
+  if (res[1] != ~0ull)
+    return fail(PGVD_ERR_INVALID, "malformed integer at token " + std::to_string(res[1] - 1));
+  if (res[0] < need)
+    return fail(PGVD_ERR_INVALID, "text ends after " + std::to_string(res[0]) + " of " +
+                                      std::to_string(need) + " tokens");
+  return PGVD_OK;
+}
+
+namespace {
+// Binary octree file: this header, then O records of 2^dim + 1 int32
+// (children, leaf), little-endian, as the device holds them.
+struct OctreeFileHeader {
+  char magic[8];        // "PGVDOCT1"
+  uint32_t dim;
+  uint32_t record_bytes;
+  int64_t O;
+  uint64_t reserved;
+};
+static_assert(sizeof(OctreeFileHeader) == 32, "32-byte header");
+constexpr char kOctMagic[8] = {'P', 'G', 'V', 'D', 'O', 'C', 'T', '1'};
+
+struct File {
+  FILE *f;
+  explicit File(FILE *p) : f(p) {}
+  ~File() {
+    if (f) std::fclose(f);
+  }
+};
+}  // namespace
+
+int pgvd_octree_save(pgvd_ctx *ctx, const int32_t *octree, int64_t O, int dim, const char *path,
+                     int format) {
+  if (!ctx || !path || (O > 0 && !octree)) return fail(PGVD_ERR_INVALID, "null argument");
+  if (dim != 2 && dim != 3) return fail(PGVD_ERR_INVALID, "dim must be 2 or 3");
+  if (format != PGVD_OCTREE_TEXT && format != PGVD_OCTREE_BINARY)
+    return fail(PGVD_ERR_INVALID, "format must be PGVD_OCTREE_TEXT or PGVD_OCTREE_BINARY");
+  if (O < 0 || O > INT32_MAX) return fail(PGVD_ERR_INVALID, "O must be in [0, 2^31)");
+  CHK(use_device(ctx));
+  hipStream_t st = ctx->stream;
+  const size_t rec = size_t(4) * ((1u << dim) + 1);
+  std::vector<char> host;
+  if (format == PGVD_OCTREE_TEXT) {
+    int64_t len = 0;
+    CHK(pgvd_octree_format_text(ctx, octree, O, dim, nullptr, 0, &len, nullptr));
+    void *buf = nullptr;
+    CHK(pool_get(ctx, "text_buf", static_cast<size_t>(len), &buf));
+    CHK(pgvd_octree_format_text(ctx, octree, O, dim, static_cast<char *>(buf), len, &len, nullptr));
+    host.resize(static_cast<size_t>(len));
+    HIPCHK(hipMemcpyAsync(host.data(), buf, host.size(), hipMemcpyDeviceToHost, st));
+  } else {
+    OctreeFileHeader h{};
+    std::memcpy(h.magic, kOctMagic, 8);
+    h.dim = static_cast<uint32_t>(dim);
+    h.record_bytes = static_cast<uint32_t>(rec);
+    h.O = O;
+    host.resize(sizeof h + rec * static_cast<size_t>(O));
+    std::memcpy(host.data(), &h, sizeof h);
+    if (O) HIPCHK(hipMemcpyAsync(host.data() + sizeof h, octree, rec * O, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  File f(std::fopen(path, "wb"));
+  if (!f.f) return fail(PGVD_ERR_INVALID, std::string("cannot open for writing: ") + path);
+  if (!host.empty() && std::fwrite(host.data(), 1, host.size(), f.f) != host.size())
+    return fail(PGVD_ERR_INVALID, std::string("short write: ") + path);
+  if (std::fclose(f.f) != 0) {
+    f.f = nullptr;
+    return fail(PGVD_ERR_INVALID, std::string("cannot close: ") + path);
+  }
+  f.f = nullptr;
+  return PGVD_OK;
+}
+
+int pgvd_octree_load(pgvd_ctx *ctx, const char *path, int dim, int32_t *octree, int64_t cap,
+                     int64_t *O_out) {
+  if (!ctx || !path || !O_out || (cap > 0 && !octree)) return fail(PGVD_ERR_INVALID, "null argument");
+  if (dim != 2 && dim != 3) return fail(PGVD_ERR_INVALID, "dim must be 2 or 3");
+  if (cap < 0) return fail(PGVD_ERR_INVALID, "cap must be >= 0");
+  CHK(use_device(ctx));
+  hipStream_t st = ctx->stream;
+  File f(std::fopen(path, "rb"));
+  if (!f.f) return fail(PGVD_ERR_INVALID, std::string("cannot open: ") + path);
+  std::vector<char> host;
+  char chunk[1 << 16];
+  size_t got;
+  while ((got = std::fread(chunk, 1, sizeof chunk, f.f)) > 0) host.insert(host.end(), chunk, chunk + got);
+  if (std::ferror(f.f)) return fail(PGVD_ERR_INVALID, std::string("read error: ") + path);
+  const size_t rec = size_t(4) * ((1u << dim) + 1);
+  if (host.size() >= 8 && std::memcmp(host.data(), kOctMagic, 8) == 0) {
+    OctreeFileHeader h;
+    if (host.size() < sizeof h) return fail(PGVD_ERR_INVALID, "truncated octree header");
+    std::memcpy(&h, host.data(), sizeof h);
+    if (h.dim != static_cast<uint32_t>(dim) || h.record_bytes != rec)
+      return fail(PGVD_ERR_INVALID, "octree file holds another dimension");
+    if (h.O < 0 || host.size() != sizeof h + rec * static_cast<size_t>(h.O))
+      return fail(PGVD_ERR_INVALID, "octree file size does not match its node count");
+    *O_out = h.O;
+    if (h.O > cap) return fail(PGVD_ERR_CAPACITY, "octree buffer too small (*O_out nodes needed)");
+    if (h.O) HIPCHK(hipMemcpyAsync(octree, host.data() + sizeof h, rec * h.O, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return PGVD_OK;
+  }
+  if (host.empty()) return fail(PGVD_ERR_INVALID, std::string("empty file: ") + path);
+  void *buf = nullptr;
+  CHK(pool_get(ctx, "text_buf", host.size(), &buf));
+  HIPCHK(hipMemcpyAsync(buf, host.data(), host.size(), hipMemcpyHostToDevice, st));
+  return pgvd_octree_parse_text(ctx, static_cast<const char *>(buf), static_cast<int64_t>(host.size()),
+                                dim, octree, cap, O_out, nullptr);
+}
+
 int pgvd_radix_sort(pgvd_ctx *ctx, void *keys, uint32_t *vals, int64_t n, int key_bytes, int mbits,
                     pgvd_stream s) {
   if (!ctx || !keys) return fail(PGVD_ERR_INVALID, "null argument");

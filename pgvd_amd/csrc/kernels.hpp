@@ -182,6 +182,19 @@ void launch_find_leaf(const int32_t *pts, int64_t q, int dim, int bits, const in
                       int64_t octree_size, void *table, int D, int32_t *parent, int32_t *octant,
                       int32_t *data, int32_t *level, uint32_t *status, hipStream_t s);
 
+// ---- textio.hip ----
+// The octree's text form (operator<< / operator>>, C/OctNode.h:58-98).  ws
+// holds text_ws_words(tiles) words: word 1 receives the total (bytes
+// formatted, or tokens seen), word 2 the first malformed record token + 1
+// (all ones: none).
+int64_t text_ws_words(int64_t tiles);
+int64_t format_text_tiles(int64_t O);
+int64_t parse_text_tiles(int64_t len);
+void launch_format_text(const int32_t *octree, int64_t O, int dim, int hdr_len, uint8_t *text,
+                        int64_t capacity, uint64_t *ws, hipStream_t s);
+void launch_parse_text(const uint8_t *text, int64_t len, int dim, int64_t N, int32_t *octree,
+                       uint64_t *ws, hipStream_t s);
+
 // ---- msd.hip ----
 // Bucket sort of 64-bit keys without payload: scatter by the top b bits,
 // then each bucket sorted in one workgroup's LDS.  Active only when every

@@ -1,0 +1,242 @@
+// textio.hip -- the octree's text form on the device: operator<< and
+// operator>> for OctNode and std::vector<OctNode> (C/OctNode.h:58-98,
+// commented out in the reference).  The text is "N " followed, for every
+// node, by each child and the leaf mask in decimal, each followed by one
+// space.
+//
+// Both directions are one pass with a decoupled look-back over tiles:
+//   format: a workgroup takes 256 nodes, sums their text lengths, scans them
+//           in the workgroup and across tiles, prints its nodes into LDS and
+//           copies the run out at its offset;
+//   parse:  a workgroup takes 4096 bytes, marks where tokens start (a
+//           non-space after a space), scans the counts to token numbers, and
+//           every start lane parses its token into record k - 1 of the
+//           octree (token 0 is N, read by the host).
+#include "kernels.hpp"
+#include "tilescan.hpp"
+
+namespace pgvd {
+
+namespace {
+
+// workspace words (uint64): [0] tile counter (low 32 bits), [1] total
+// (bytes or tokens, written by the last tile), [2] first malformed token + 1
+// (atomic min over ~0: none), then the tile status array from word 4.
+constexpr int kWsCounter = 0, kWsTotal = 1, kWsBad = 2, kWsStatus = 4;
+constexpr int kFmtBlock = 256;        // nodes per format tile
+constexpr int kMaxNodeText = 9 * 12;  // 3D node: 9 values of at most 11 chars + space
+constexpr int kParseBytes = 16;       // bytes per thread in a parse tile
+constexpr int kParseTile = kBlock * kParseBytes;
+
+__device__ inline int dec_len(int32_t v) {
+  uint32_t u = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
+  int n = 1;
+  if (u >= 10u) n = 2;
+  if (u >= 100u) n = 3;
+  if (u >= 1000u) n = 4;
+  if (u >= 10000u) n = 5;
+  if (u >= 100000u) n = 6;
+  if (u >= 1000000u) n = 7;
+  if (u >= 10000000u) n = 8;
+  if (u >= 100000000u) n = 9;
+  if (u >= 1000000000u) n = 10;
+  return n + (v < 0);
+}
+
+// Prints v and one space at s[at..]; returns the position after the space.
+__device__ inline int put_dec(uint8_t *s, int at, int32_t v) {
+  const int n = dec_len(v);
+  uint32_t u = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
+  int e = at + n;
+  s[e] = ' ';
+  do {
+    s[--e] = (uint8_t)('0' + u % 10u);
+    u /= 10u;
+  } while (u);
+  if (v < 0) s[at] = '-';
+  return at + n + 1;
+}
+
+__global__ void ws_zero_kernel(uint64_t *ws, int64_t words) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < words;
+       i += (int64_t)gridDim.x * kBlock)
+    ws[i] = i == kWsBad ? ~0ull : 0ull;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kFmtBlock) void format_text_kernel(
+    const int32_t *__restrict__ octree, int64_t O, int64_t N_hdr, int hdr_len,
+    uint8_t *__restrict__ text, int64_t capacity, uint64_t *__restrict__ ws, int ntiles) {
+  constexpr int STRIDE = (1 << DIM) + 1;
+  __shared__ uint8_t s_txt[kFmtBlock * kMaxNodeText];
+  __shared__ uint32_t s_w[kFmtBlock / 64];
+  __shared__ int s_tile;
+  const int t = threadIdx.x;
+  if (t == 0) s_tile = (int)atomicAdd(reinterpret_cast<unsigned int *>(ws + kWsCounter), 1u);
+  __syncthreads();
+  const int tile = s_tile;
+  const int64_t i = (int64_t)tile * kFmtBlock + t;
+  int32_t v[STRIDE];
+  uint32_t len = 0;
+  if (i < O) {
+#pragma unroll
+    for (int k = 0; k < STRIDE; ++k) {
+      v[k] = octree[i * STRIDE + k];
+      len += (uint32_t)dec_len(v[k]) + 1u;
+    }
+  }
+  uint32_t total;
+  const uint32_t ex = block_excl_sum<uint32_t, kFmtBlock / 64>(len, s_w, &total);
+  const uint64_t before = tile_prefix<kFmtBlock>(ws + kWsStatus, tile, total);
+  if (tile == ntiles - 1 && t == 0) ws[kWsTotal] = (uint64_t)hdr_len + before + total;
+  if (!text) return;
+  const int64_t g0 = hdr_len + (int64_t)before;
+  if (g0 + (int64_t)total > capacity) return;
+  if (tile == 0 && t == 0) {
+    // "N " at the start: N printed from the least significant digit
+    uint64_t u = (uint64_t)N_hdr;
+    int e = hdr_len - 1;
+    text[e] = ' ';
+    do {
+      text[--e] = (uint8_t)('0' + u % 10u);
+      u /= 10u;
+    } while (u);
+  }
+  if (i < O) {
+    int at = (int)ex;
+#pragma unroll
+    for (int k = 0; k < STRIDE; ++k) at = put_dec(s_txt, at, v[k]);
+  }
+  __syncthreads();
+  // the tile's run: bytes up to the first 4-byte boundary, whole words, tail
+  uint8_t *dst = text + g0;
+  int head = (int)((4 - ((uintptr_t)dst & 3)) & 3);
+  if (head > (int)total) head = (int)total;
+  if (t < head) dst[t] = s_txt[t];
+  const int nw = ((int)total - head) >> 2;
+  uint32_t *dw = reinterpret_cast<uint32_t *>(dst + head);
+  for (int w = t; w < nw; w += kFmtBlock) {
+    const int b = head + 4 * w;
+    dw[w] = (uint32_t)s_txt[b] | ((uint32_t)s_txt[b + 1] << 8) | ((uint32_t)s_txt[b + 2] << 16) |
+            ((uint32_t)s_txt[b + 3] << 24);
+  }
+  const int tail0 = head + 4 * nw;
+  if (t < (int)total - tail0) dst[tail0 + t] = s_txt[tail0 + t];
+}
+
+__device__ inline bool is_space(uint8_t c) {
+  return c == ' ' || c == '\n' || c == '\t' || c == '\r' || c == '\v' || c == '\f';
+}
+
+// Parses the token at text[p..) (p < len, text[p] not a space): an optional
+// sign and 1..10 digits ending at a space or at len, within int32.
+__device__ inline bool parse_token(const uint8_t *__restrict__ text, int64_t len, int64_t p,
+                                   int32_t *out) {
+  bool neg = false;
+  uint8_t c = text[p];
+  if (c == '+' || c == '-') {
+    neg = c == '-';
+    ++p;
+  }
+  int64_t x = 0;
+  int nd = 0;
+  while (p < len) {
+    c = text[p];
+    if (is_space(c)) break;
+    if (c < '0' || c > '9' || nd >= 10) return false;
+    x = x * 10 + (c - '0');
+    ++nd;
+    ++p;
+  }
+  if (nd == 0) return false;
+  if (neg) x = -x;
+  if (x < INT32_MIN || x > INT32_MAX) return false;
+  *out = (int32_t)x;
+  return true;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void parse_text_kernel(
+    const uint8_t *__restrict__ text, int64_t len, int64_t N, int32_t *__restrict__ octree,
+    uint64_t *__restrict__ ws, int ntiles) {
+  constexpr int STRIDE = (1 << DIM) + 1;
+  __shared__ uint32_t s_w[kBlock / 64];
+  __shared__ int s_tile;
+  const int t = threadIdx.x;
+  if (t == 0) s_tile = (int)atomicAdd(reinterpret_cast<unsigned int *>(ws + kWsCounter), 1u);
+  __syncthreads();
+  const int tile = s_tile;
+  const int64_t p0 = (int64_t)tile * kParseTile + (int64_t)t * kParseBytes;
+  uint8_t b[kParseBytes];
+  if (p0 + kParseBytes <= len && (((uintptr_t)(text + p0)) & 15) == 0) {
+    const uint4 q = *reinterpret_cast<const uint4 *>(text + p0);
+    const uint32_t wv[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int k = 0; k < kParseBytes; ++k) b[k] = (uint8_t)(wv[k >> 2] >> (8 * (k & 3)));
+  } else {
+#pragma unroll
+    for (int k = 0; k < kParseBytes; ++k) b[k] = p0 + k < len ? text[p0 + k] : (uint8_t)' ';
+  }
+  bool prev_space = p0 == 0 || p0 > len || is_space(text[p0 - 1]);
+  uint32_t starts = 0;
+#pragma unroll
+  for (int k = 0; k < kParseBytes; ++k) {
+    const bool sp = is_space(b[k]);
+    if (!sp && prev_space) starts |= 1u << k;
+    prev_space = sp;
+  }
+  uint32_t total;
+  const uint32_t ex = block_excl_sum<uint32_t>((uint32_t)__popc(starts), s_w, &total);
+  const uint64_t before = tile_prefix<kBlock>(ws + kWsStatus, tile, total);
+  if (tile == ntiles - 1 && t == 0) ws[kWsTotal] = before + total;
+  uint64_t k = before + ex;
+  const uint64_t last = (uint64_t)N * STRIDE;   // tokens 1..last are the records
+  while (starts) {
+    const int j = __ffs(starts) - 1;
+    starts &= starts - 1;
+    if (k >= 1 && k <= last) {
+      int32_t val;
+      if (parse_token(text, len, p0 + j, &val)) {
+        const uint64_t r = k - 1;
+        octree[r] = (r % STRIDE == STRIDE - 1) ? (int32_t)(uint8_t)val : val;
+      } else {
+        atomicMin(reinterpret_cast<unsigned long long *>(ws + kWsBad), (unsigned long long)k + 1);
+      }
+    }
+    ++k;
+  }
+}
+
+}  // namespace
+
+int64_t text_ws_words(int64_t tiles) { return kWsStatus + tiles; }
+int64_t format_text_tiles(int64_t O) { return (O + kFmtBlock - 1) / kFmtBlock; }
+int64_t parse_text_tiles(int64_t len) { return (len + kParseTile - 1) / kParseTile; }
+
+void launch_format_text(const int32_t *octree, int64_t O, int dim, int hdr_len, uint8_t *text,
+                        int64_t capacity, uint64_t *ws, hipStream_t s) {
+  const int64_t tiles = format_text_tiles(O);
+  const int64_t words = text_ws_words(tiles);
+  ws_zero_kernel<<<(unsigned)std::min<int64_t>((words + kBlock - 1) / kBlock, 1024), kBlock, 0,
+                   s>>>(ws, words);
+  if (dim == 2)
+    format_text_kernel<2><<<(unsigned)tiles, kFmtBlock, 0, s>>>(octree, O, O, hdr_len, text,
+                                                                capacity, ws, (int)tiles);
+  else
+    format_text_kernel<3><<<(unsigned)tiles, kFmtBlock, 0, s>>>(octree, O, O, hdr_len, text,
+                                                                capacity, ws, (int)tiles);
+}
+
+void launch_parse_text(const uint8_t *text, int64_t len, int dim, int64_t N, int32_t *octree,
+                       uint64_t *ws, hipStream_t s) {
+  const int64_t tiles = parse_text_tiles(len);
+  const int64_t words = text_ws_words(tiles);
+  ws_zero_kernel<<<(unsigned)std::min<int64_t>((words + kBlock - 1) / kBlock, 1024), kBlock, 0,
+                   s>>>(ws, words);
+  if (dim == 2)
+    parse_text_kernel<2><<<(unsigned)tiles, kBlock, 0, s>>>(text, len, N, octree, ws, (int)tiles);
+  else
+    parse_text_kernel<3><<<(unsigned)tiles, kBlock, 0, s>>>(text, len, N, octree, ws, (int)tiles);
+}
+
+}  // namespace pgvd

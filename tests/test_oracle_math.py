@@ -346,3 +346,39 @@ def test_find_leaf_against_brute_force(dim, bits, n):
     # the reference's line 37 would exceed 2^dim - 1 (an index past the
     # node); most queries descend further than that
     assert int((got["level"] > 2).sum()) > queries.shape[0] // 2
+
+
+# ------------------------------------------------------ octree text form ----
+def test_octree_text_known_answer():
+    """operator<< (C/OctNode.h:58-87): the size and a space, then every node
+    as its children and leaf, each value followed by one space."""
+    octree = np.array([[1, -1, -1, -1, 14], [-1, -1, -1, -1, 15]], np.int32)
+    assert oracle.octree_format_text(octree, 2) == b"2 1 -1 -1 -1 14 -1 -1 -1 -1 15 "
+    ext = np.array([[-2147483648, 2147483647, 0, 10, 255]], np.int32)
+    assert oracle.octree_format_text(ext, 2) == b"1 -2147483648 2147483647 0 10 255 "
+    assert oracle.octree_format_text(np.zeros((0, 9), np.int32), 3) == b"0 "
+
+
+def test_octree_text_parse_rules():
+    """operator>> (C/OctNode.h:69-98): whitespace of any kind between
+    integers, an optional sign, the leaf cast to unsigned char, and trailing
+    tokens ignored; malformed or missing tokens are rejected."""
+    want = np.array([[1, -1, -1, -1, 14], [-1, -1, -1, -1, 15]], np.int32)
+    for text in (b"2 1 -1 -1 -1 14 -1 -1 -1 -1 15 ", b"2\n1\t-1  -1 -1 +14\r\n-1 -1 -1 -1 15",
+                 b"  2 1 -1 -1 -1 14 -1 -1 -1 -1 15 7 7"):
+        assert np.array_equal(oracle.octree_parse_text(text, 2), want), text
+    got = oracle.octree_parse_text(b"1 0 0 0 0 300 ", 2)
+    assert got[0, 4] == 300 % 256
+    for bad in (b"2 1 -1 -1 -1 14 -1 -1 -1 -1", b"1 1 x -1 -1 14", b"1 2147483648 0 0 0 0",
+                b"1 - 0 0 0 0", b"", b"-1 "):
+        assert oracle.octree_parse_text(bad, 2) is None, bad
+    assert oracle.octree_parse_text(b"1 -2147483648 0 0 0 0 ", 2)[0, 0] == -2147483648
+
+
+@pytest.mark.parametrize("dim,n", [(2, 5000), (3, 20000)])
+def test_octree_text_round_trip(dim, n):
+    rng = np.random.default_rng(dim)
+    t = oracle.build(rng.integers(0, 1 << 16, (n, dim)), 16)
+    octree = np.concatenate([t["children"], t["leaf"][:, None]], axis=1)
+    text = oracle.octree_format_text(octree, dim)
+    assert np.array_equal(oracle.octree_parse_text(text, dim), octree)
