@@ -8,10 +8,10 @@
 //   format: a workgroup takes 256 nodes, sums their text lengths, scans them
 //           in the workgroup and across tiles, prints its nodes into LDS and
 //           copies the run out at its offset;
-//   parse:  a workgroup takes 4096 bytes, marks where tokens start (a
-//           non-space after a space), scans the counts to token numbers, and
-//           every start lane parses its token into record k - 1 of the
-//           octree (token 0 is N, read by the host).
+//   parse:  a workgroup stages 4096 bytes in LDS, marks where tokens start
+//           (a non-space after a space), scans the counts to token numbers,
+//           parses every token from LDS and writes the tile's records
+//           k - 1 as one run (token 0 is N, read by the host).
 #include "kernels.hpp"
 #include "tilescan.hpp"
 
@@ -128,25 +128,23 @@ __device__ inline bool is_space(uint8_t c) {
   return c == ' ' || c == '\n' || c == '\t' || c == '\r' || c == '\v' || c == '\f';
 }
 
-// Parses the token at text[p..) (p < len, text[p] not a space): an optional
-// sign and 1..10 digits ending at a space or at len, within int32.
-__device__ inline bool parse_token(const uint8_t *__restrict__ text, int64_t len, int64_t p,
-                                   int32_t *out) {
+// Token parse from the staged bytes: an optional sign and 1..10 digits,
+// ending at a space (bytes past the text are staged as spaces), within int32.
+__device__ inline bool parse_staged(const uint8_t *s, int p, int32_t *out) {
   bool neg = false;
-  uint8_t c = text[p];
+  uint8_t c = s[p];
   if (c == '+' || c == '-') {
     neg = c == '-';
     ++p;
   }
   int64_t x = 0;
   int nd = 0;
-  while (p < len) {
-    c = text[p];
+  for (;; ++p) {
+    c = s[p];
     if (is_space(c)) break;
     if (c < '0' || c > '9' || nd >= 10) return false;
     x = x * 10 + (c - '0');
     ++nd;
-    ++p;
   }
   if (nd == 0) return false;
   if (neg) x = -x;
@@ -155,33 +153,47 @@ __device__ inline bool parse_token(const uint8_t *__restrict__ text, int64_t len
   return true;
 }
 
+// A tile's bytes and the 16 after it are staged in LDS: a token starting in
+// the tile is decided within 12 bytes (sign, 10 digits, one more byte).  Its
+// values go to LDS in token order and out as one coalesced run of records.
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void parse_text_kernel(
     const uint8_t *__restrict__ text, int64_t len, int64_t N, int32_t *__restrict__ octree,
     uint64_t *__restrict__ ws, int ntiles) {
   constexpr int STRIDE = (1 << DIM) + 1;
+  constexpr int kHalo = 16;
+  __shared__ __align__(16) uint8_t s_txt[kParseTile + kHalo];
+  __shared__ int32_t s_val[kParseTile / 2];
   __shared__ uint32_t s_w[kBlock / 64];
   __shared__ int s_tile;
   const int t = threadIdx.x;
   if (t == 0) s_tile = (int)atomicAdd(reinterpret_cast<unsigned int *>(ws + kWsCounter), 1u);
   __syncthreads();
   const int tile = s_tile;
-  const int64_t p0 = (int64_t)tile * kParseTile + (int64_t)t * kParseBytes;
-  uint8_t b[kParseBytes];
+  const int64_t tb = (int64_t)tile * kParseTile;
+  const int64_t p0 = tb + (int64_t)t * kParseBytes;
+  uint4 q;
   if (p0 + kParseBytes <= len && (((uintptr_t)(text + p0)) & 15) == 0) {
-    const uint4 q = *reinterpret_cast<const uint4 *>(text + p0);
-    const uint32_t wv[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-    for (int k = 0; k < kParseBytes; ++k) b[k] = (uint8_t)(wv[k >> 2] >> (8 * (k & 3)));
+    q = *reinterpret_cast<const uint4 *>(text + p0);
   } else {
+    uint32_t wv[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int k = 0; k < kParseBytes; ++k) b[k] = p0 + k < len ? text[p0 + k] : (uint8_t)' ';
+    for (int k = 0; k < kParseBytes; ++k)
+      wv[k >> 2] |= (uint32_t)(p0 + k < len ? text[p0 + k] : (uint8_t)' ') << (8 * (k & 3));
+    q = make_uint4(wv[0], wv[1], wv[2], wv[3]);
   }
-  bool prev_space = p0 == 0 || p0 > len || is_space(text[p0 - 1]);
+  *reinterpret_cast<uint4 *>(s_txt + t * kParseBytes) = q;
+  if (t < kHalo) {
+    const int64_t p = tb + kParseTile + t;
+    s_txt[kParseTile + t] = p < len ? text[p] : (uint8_t)' ';
+  }
+  const bool prev0 = tb == 0 || tb > len || is_space(text[tb - 1]);
+  __syncthreads();
+  bool prev_space = t == 0 ? prev0 : is_space(s_txt[t * kParseBytes - 1]);
   uint32_t starts = 0;
 #pragma unroll
   for (int k = 0; k < kParseBytes; ++k) {
-    const bool sp = is_space(b[k]);
+    const bool sp = is_space(s_txt[t * kParseBytes + k]);
     if (!sp && prev_space) starts |= 1u << k;
     prev_space = sp;
   }
@@ -189,21 +201,28 @@ __global__ __launch_bounds__(kBlock) void parse_text_kernel(
   const uint32_t ex = block_excl_sum<uint32_t>((uint32_t)__popc(starts), s_w, &total);
   const uint64_t before = tile_prefix<kBlock>(ws + kWsStatus, tile, total);
   if (tile == ntiles - 1 && t == 0) ws[kWsTotal] = before + total;
-  uint64_t k = before + ex;
   const uint64_t last = (uint64_t)N * STRIDE;   // tokens 1..last are the records
+  uint32_t li = ex;
   while (starts) {
     const int j = __ffs(starts) - 1;
     starts &= starts - 1;
+    const uint64_t k = before + li;
     if (k >= 1 && k <= last) {
-      int32_t val;
-      if (parse_token(text, len, p0 + j, &val)) {
-        const uint64_t r = k - 1;
-        octree[r] = (r % STRIDE == STRIDE - 1) ? (int32_t)(uint8_t)val : val;
-      } else {
+      int32_t val = 0;
+      if (!parse_staged(s_txt, t * kParseBytes + j, &val))
         atomicMin(reinterpret_cast<unsigned long long *>(ws + kWsBad), (unsigned long long)k + 1);
-      }
+      s_val[li] = val;
     }
-    ++k;
+    ++li;
+  }
+  __syncthreads();
+  for (uint32_t i = t; i < total; i += kBlock) {
+    const uint64_t k = before + i;
+    if (k >= 1 && k <= last) {
+      const uint64_t r = k - 1;
+      const int32_t v = s_val[i];
+      octree[r] = (r % STRIDE == STRIDE - 1) ? (int32_t)(uint8_t)v : v;
+    }
   }
 }
 
