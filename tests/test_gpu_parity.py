@@ -4,6 +4,7 @@ bit-exact; there is no floating-point output on this path."""
 from __future__ import annotations
 
 import hashlib
+import os
 
 import numpy as np
 import pytest
@@ -445,3 +446,25 @@ def test_find_leaf_rejects_foreign_child_index(dev, reps):
     for k in want:
         assert np.array_equal(got[k].astype(np.int64), want[k].astype(np.int64)), k
     assert list(want["status"]) == [2, 0, 0] * reps
+
+
+# ----------------------------------------------------- reference data ----
+def _data_points():
+    path = os.path.join(os.path.dirname(__file__), "golden", "data_points.npz")
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("name", ["data2_test1", "data2_test2", "data2_test3", "data2_test4",
+                                  "data2_test5", "data2_test6", "data2_test7", "data2_test8",
+                                  "data3_simple1", "data3_simple2", "data3_simple3", "data3_bun1"])
+def test_reference_data_builds(dev, name):
+    """The reference's own point sets (data2 polyline scenes, data3 meshes,
+    read by pgvd_amd.io into the committed fixture), built at the viewer's
+    resolution and at the widest key of each dimension."""
+    pts = _data_points()[name]
+    dim = pts.shape[1]
+    for bits in ((16, 30) if dim == 2 else (21, 42)):
+        got = dev.build_octree(pts, bits)
+        want = oracle.build(oracle.quantize(pts, 1 << bits), bits)
+        assert_tree_equal(got, want, where=f"{name} bits={bits}")
