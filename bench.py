@@ -5,6 +5,7 @@
     python bench.py --workload serve [--serve-rate 2000]    (served form)
     python bench.py --workload findleaf                     (point location)
     python bench.py --workload octree_text                  (text form of the octree)
+    python bench.py --workload cellwalk                     (batched cell walks)
 
 One step = the whole path (bounding box, FP32 quantize, 63-bit Morton keys,
 radix sort, unique, Karras BRT with local splits, octree fill) over 2^24
@@ -439,6 +440,111 @@ def bench_findleaf(args) -> int:
         out.get("cpu_baseline", {}).get("mismatches_vs_gpu", 0) == 0 else 1
 
 
+def bench_cellwalk(args) -> int:
+    """Batched CellWalk (viewer/Octree2.cpp:265-313): 2^20 2D segments (seed
+    8; a uniform in the grid, b within 1/256 of the grid's width of a) walked
+    through the quadtree of 2^22 uniform 2D points at 20 bits/axis; segments
+    and quadtree already in HBM.  One step = one pgvd_cell_walk call with a
+    cell buffer (count pass, offset scan, write pass).  Algorithmic bytes:
+    16 B in and 8 B of offset out per segment, 20 B out per visited cell."""
+    import pgvd_amd as P
+
+    bits, npts = 20, 1 << 22
+    nseg = args.n or (1 << 20)
+    rng = np.random.default_rng(8)
+    pts = (rng.random((npts, 2), dtype=np.float32) * 2 - 1)
+    W = 1 << bits
+    a = rng.random((nseg, 2)) * (W - 1e-3)
+    b = np.clip(a + (rng.random((nseg, 2)) * 2 - 1) * (W / 256), 0, W - 1e-3)
+    segs = np.concatenate([a, b], axis=1).astype(np.float32)
+    dev = P.Device(0)
+    dpts = dev.upload(pts)
+    dev.build_octree_async(dpts, npts, 2, bits)
+    res = dev.build_octree_finish()
+    O = int(res.octree_size)
+    dseg = dev.upload(segs)
+    doff = dev.alloc(8 * (nseg + 1))
+    L = P.lib()
+    total = C.c_int64()
+    dev._check(L.pgvd_cell_walk(dev.ctx, dseg.ptr, nseg, bits, res.octree, O, doff.ptr, None, 0,
+                                C.byref(total), None, None))
+    ncell = total.value
+    dcell = dev.alloc(20 * ncell)
+
+    def step():
+        dev._check(L.pgvd_cell_walk(dev.ctx, dseg.ptr, nseg, bits, res.octree, O, doff.ptr,
+                                    dcell.ptr, ncell, C.byref(total), None, None))
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    dev.timing(True)
+    dev.timing_reset()
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dev.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    c_tot, c_cnt = dev.timing_read("cell_walk_count")
+    w_tot, w_cnt = dev.timing_read("cell_walk")
+    dev.timing(False)
+    w_launch = w_tot / max(w_cnt, 1)
+    cells = dev.download(dcell, np.int32, 5 * ncell).reshape(-1, 5) if not args.no_cpu else None
+    offs = dev.download(doff, np.int64, nseg + 1) if not args.no_cpu else None
+    octree = dev.download(res.octree, np.int32, O * 5).reshape(O, 5) if not args.no_cpu else None
+    dev.close()
+    wbytes = nseg * 24 + ncell * 20
+    gbps = wbytes / (w_launch * 1e-3) / 1e9
+    out = {
+        "metric": "batched cell walks (CellWalk): visited cells/sec, 2^20 segments in a 2^22-point quadtree",
+        "value": ncell / (ms * 1e-3),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/i32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{nseg} segments (seed 8, length up to 1/256 of the grid) through the "
+                        f"quadtree of {npts} uniform 2D points at {bits} bits/axis ({O} nodes)",
+            "segments": nseg, "cells": ncell, "segments_per_s": nseg / (ms * 1e-3),
+            "count_pass_ms": c_tot / max(c_cnt, 1),
+        },
+        "roofline": {
+            "kernel": "cell_walk_kernel, write pass (one lane per segment: FindLeaf descents and "
+                      "edge intersections per visited cell)",
+            "bound": "hbm",
+            "achieved": gbps,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": gbps / HBM_PEAK_GBPS,
+            "traffic": load_traffic("cell_walk", "cellwalk"),
+            "avg_launch_ms": w_launch,
+            "launches": w_cnt,
+            "algorithmic_bytes": wbytes,
+        },
+    }
+    if not args.no_cpu:
+        import oracle
+        s = min(nseg, 1 << 16)
+        t0 = time.perf_counter()
+        want = oracle.cell_walk(segs[:s], bits, octree[:, :4], octree[:, 4])
+        dt = time.perf_counter() - t0
+        same = (np.array_equal(want["offsets"], offs[: s + 1]) and
+                np.array_equal(want["cells"], cells[: offs[s]]))
+        out["cpu_baseline"] = {
+            "value": len(want["cells"]) / dt, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"first {s} segments walked by the serial restatement on 1 core in {dt:.2f} s",
+            "equal_to_gpu": bool(same),
+        }
+    print(json.dumps(out), flush=True)
+    return 0 if out.get("cpu_baseline", {}).get("equal_to_gpu", True) else 1
+
+
 def bench_text(args) -> int:
     """The octree's text form (operator<< / operator>>, C/OctNode.h:58-98) on
     the device: the octree of the 2^24-point uniform 3D build (21 bits/axis)
@@ -563,7 +669,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="uniform",
                     choices=["uniform", "gauss", "conflict", "batch", "serve", "findleaf",
-                             "octree_text"])
+                             "octree_text", "cellwalk"])
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -590,6 +696,8 @@ def main() -> int:
         return bench_findleaf(args)
     if args.workload == "octree_text":
         return bench_text(args)
+    if args.workload == "cellwalk":
+        return bench_cellwalk(args)
 
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20

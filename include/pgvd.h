@@ -131,6 +131,29 @@ int pgvd_find_leaf(pgvd_ctx *ctx, const int32_t *pts, int64_t q, int dim, int bi
                    const int32_t *octree, int64_t octree_size, int32_t *parent, int32_t *octant,
                    int32_t *data, int32_t *level, uint32_t *status, pgvd_stream s);
 
+/* CellWalk (viewer/Octree2.cpp:265-313), batched: for each of n 2D
+ * segments (segs: device, ax, ay, bx, by as float in grid units, as
+ * obj2Oct gives them) the leaf cells of a 2D octree the walk visits, in
+ * order.  A step finds the leaf of the current point (truncated to int, the
+ * octant masked to two bits as in pgvd_find_leaf), intersects the segment
+ * with the cell's edges (OctreeUtils::FindIntersections,
+ * OctreeUtils.cpp:80-172, the same float operations and epsilons) and moves
+ * to the farthest intersection.  Outputs (device):
+ *   offsets  n + 1 int64: segment i's cells are [offsets[i], offsets[i+1]);
+ *   cells    5 int32 per cell: origin x, origin y, width, parent node,
+ *            octant; NULL to count only;
+ *   status   (nullable) n int32: 0, or 1 when a walk passed 10000 cells
+ *            (the reference throws; the first 10000 are kept).
+ * *total_out (host) receives the number of cells; PGVD_ERR_CAPACITY when
+ * it exceeds capacity (offsets and status are then valid).  Runs the walk
+ * twice (count, then write) and returns after the count is read back; the
+ * write is asynchronous on the stream.  1 <= bits <= 30.  Workspace
+ * buffers belong to the context: calls on one context must not overlap on
+ * different streams. */
+int pgvd_cell_walk(pgvd_ctx *ctx, const float *segs, int64_t n, int bits, const int32_t *octree,
+                   int64_t octree_size, int64_t *offsets, int32_t *cells, int64_t capacity,
+                   int64_t *total_out, int32_t *status, pgvd_stream s);
+
 /* The octree's text form: operator<< for std::vector<OctNode>
  * (C/OctNode.h:58-87, commented out in the reference), formatted on the
  * device.  The text is the node count and a space, then for every node each

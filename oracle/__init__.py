@@ -54,6 +54,9 @@ def lib():
         L.oracle_compute_lcp_length.argtypes = [C.c_uint64] * 4 + [C.c_int]
         L.oracle_compute_lcp_length.restype = C.c_int
         L.oracle_stream_scan.argtypes = [u32p, u32p, C.c_int]
+        L.oracle_cell_walk.argtypes = [f32p, C.c_int64, C.c_int, i32p, C.c_int64, i32p, i32p,
+                                       C.c_void_p, C.c_int64]
+        L.oracle_cell_walk.restype = C.c_int64
         L.oracle_octree_format_text.argtypes = [i32p, C.c_int64, C.c_int, C.c_char_p]
         L.oracle_octree_format_text.restype = C.c_int64
         L.oracle_octree_parse_text.argtypes = [C.c_char_p, C.c_int64, C.c_int, i32p, C.c_int64,
@@ -177,6 +180,26 @@ def find_leaf(q: np.ndarray, bits: int, children: np.ndarray, leaf: np.ndarray) 
     lib().oracle_find_leaf(q.reshape(-1), n, dim, bits, oct_.reshape(-1), O, out["parent"],
                            out["octant"], out["data"], out["level"], out["status"])
     return out
+
+
+def cell_walk(segs: np.ndarray, bits: int, children: np.ndarray, leaf: np.ndarray) -> dict:
+    """CellWalk for every row (ax, ay, bx, by) of segs on a 2D octree:
+    per-segment counts, status, offsets, and the visited cells as rows of
+    (origin x, origin y, width, parent, octant)."""
+    segs = np.ascontiguousarray(segs, dtype=np.float32).reshape(-1, 4)
+    n = segs.shape[0]
+    O = children.shape[0]
+    oct_ = np.ascontiguousarray(np.concatenate([children.reshape(O, -1), leaf.reshape(O, 1)],
+                                               axis=1).astype(np.int32))
+    counts, status = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    total = lib().oracle_cell_walk(segs.reshape(-1), n, bits, oct_.reshape(-1), O, counts, status,
+                                   None, 0)
+    cells = np.zeros((max(total, 1), 5), np.int32)
+    lib().oracle_cell_walk(segs.reshape(-1), n, bits, oct_.reshape(-1), O, counts, status,
+                           cells.ctypes.data, total)
+    offsets = np.zeros(n + 1, np.int64)
+    np.cumsum(counts, out=offsets[1:])
+    return {"counts": counts, "status": status, "offsets": offsets, "cells": cells[:total]}
 
 
 def octree_format_text(octree: np.ndarray, dim: int) -> bytes:

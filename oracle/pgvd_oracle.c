@@ -599,3 +599,147 @@ int oracle_octree_parse_text(const char *text, int64_t len, int dim, int32_t *oc
   }
   return 0;
 }
+
+/* ---- CellWalk ---------------------------------------------------------- */
+/* OctreeUtils.h:24-30 */
+typedef struct { float t; float p[2]; } cell_isect;
+
+/* opencl/vec_cpp.h:526-530, 546-548, 573-575: dot accumulates from 0 in
+ * order, length = sqrt(dot(v, v)). */
+static float dot2(const float *u, const float *v) {
+  float r = 0;
+  r += u[0] * v[0];
+  r += u[1] * v[1];
+  return r;
+}
+
+#define CW_EPSILON 1e-6 /* opencl/defs.h:135, a double */
+
+/* OctreeUtils.cpp:80-172: the segment ab against the four edges of the
+ * cell (bottom, right, top, left), kept when t is in [-eps, len + eps)
+ * (compared in double) and the point is in the cell grown by eps (float);
+ * two points closer than eps count once; of three or more, the pair whose
+ * direction best matches v (largest |dot|, first such pair) is kept.  When
+ * no pair's |dot| exceeds 0 the reference keeps two default-constructed
+ * intersections (uninitialised); here the first two are kept. */
+static int cw_find_intersections(const float a[2], const float b[2], const int32_t origin[2],
+                                 int width, cell_isect ret[4]) {
+  float v[2] = {b[0] - a[0], b[1] - a[1]};
+  const float len = sqrtf(dot2(v, v));
+  v[0] = v[0] / len;
+  v[1] = v[1] / len;
+  const float bmin[2] = {(float)origin[0], (float)origin[1]};
+  const float bmax[2] = {(float)origin[0] + (float)width, (float)origin[1] + (float)width};
+  float ts[4], ps[4][2];
+  {
+    const int y = origin[1];
+    ts[0] = ((float)y - a[1]) / v[1];
+    ps[0][0] = a[0] + v[0] * ts[0];
+    ps[0][1] = (float)y;
+  }
+  {
+    const int x = origin[0] + width;
+    ts[1] = ((float)x - a[0]) / v[0];
+    ps[1][0] = (float)x;
+    ps[1][1] = a[1] + v[1] * ts[1];
+  }
+  {
+    const int y = origin[1] + width;
+    ts[2] = ((float)y - a[1]) / v[1];
+    ps[2][0] = a[0] + v[0] * ts[2];
+    ps[2][1] = (float)y;
+  }
+  {
+    const int x = origin[0];
+    ts[3] = ((float)x - a[0]) / v[0];
+    ps[3][0] = (float)x;
+    ps[3][1] = a[1] + v[1] * ts[3];
+  }
+  const float eps = (float)CW_EPSILON;
+  int n = 0;
+  for (int i = 0; i < 4; ++i) {
+    const float t = ts[i];
+    if ((double)t >= -CW_EPSILON && (double)t < (double)len + CW_EPSILON) {
+      int in = 1;
+      for (int k = 0; k < 2; ++k)   /* BoundingBox.h:105-111 */
+        if (ps[i][k] < bmin[k] - eps || ps[i][k] > bmax[k] + eps) in = 0;
+      if (in) {
+        ret[n].t = t;
+        ret[n].p[0] = ps[i][0];
+        ret[n].p[1] = ps[i][1];
+        ++n;
+      }
+    }
+  }
+  if (n == 2) {
+    const float d[2] = {ret[0].p[0] - ret[1].p[0], ret[0].p[1] - ret[1].p[1]};
+    if ((double)sqrtf(dot2(d, d)) < CW_EPSILON) n = 1;   /* opencl/Geom.h:16-18 */
+  }
+  if (n > 2) {
+    int bi = 0, bj = 1;
+    float best = 0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) {
+        float u[2] = {ret[j].p[0] - ret[i].p[0], ret[j].p[1] - ret[i].p[1]};
+        const float l = sqrtf(dot2(u, u));
+        u[0] = u[0] / l;
+        u[1] = u[1] / l;
+        const float cd = fabsf(dot2(u, v));
+        if (cd > best) { best = cd; bi = i; bj = j; }
+      }
+    const cell_isect x = ret[bi], y = ret[bj];
+    ret[0] = x;
+    ret[1] = y;
+    n = 2;
+  }
+  return n;
+}
+
+/* FindLeaf of a float point after convert_int2 (truncation, opencl/vec_cpp.h
+ * CONVERT_VEC2); origin = the point with its low log2(width) bits cleared. */
+static void cw_leaf(const float p[2], int bits, const int32_t *octree, int64_t O, int32_t cell[5]) {
+  const int64_t q[2] = {(int32_t)p[0], (int32_t)p[1]};
+  int32_t parent, octant, data, level, st;
+  oracle_find_leaf(q, 1, 2, bits, octree, O, &parent, &octant, &data, &level, &st);
+  const int32_t width = (int32_t)(1u << (bits - level));
+  cell[0] = (int32_t)q[0] & ~(width - 1);
+  cell[1] = (int32_t)q[1] & ~(width - 1);
+  cell[2] = width;
+  cell[3] = parent;
+  cell[4] = octant;
+}
+
+int64_t oracle_cell_walk(const float *segs, int64_t n, int bits, const int32_t *octree, int64_t O,
+                         int32_t *counts, int32_t *status, int32_t *cells, int64_t cap) {
+  int64_t total = 0;
+  for (int64_t s = 0; s < n; ++s) {
+    const float *a = segs + 4 * s, *b = segs + 4 * s + 2;
+    int dir[2];
+    for (int i = 0; i < 2; ++i) dir[i] = b[i] > a[i] ? 1 : (b[i] < a[i] ? -1 : 0);
+    cell_isect cur = {0.0f, {a[0], a[1]}};
+    int32_t bcell[5], cell[5], ncell[5];
+    cw_leaf(b, bits, octree, O, bcell);
+    cw_leaf(cur.p, bits, octree, O, cell);
+    int count = 0, st = 0, done = 0;
+    do {
+      if (++count > 10000) { st = 1; --count; break; }
+      if (cells && total < cap) memcpy(cells + 5 * total, cell, sizeof cell);
+      ++total;
+      cell_isect li[4];
+      const int nl = cw_find_intersections(a, b, cell, cell[2], li);
+      for (int i = 0; i < nl; ++i)
+        if (li[i].t > cur.t) cur = li[i];
+      for (int i = 0; i < 2; ++i) {
+        const int end = cell[i];
+        if (cur.p[i] == (float)end && cur.p[i] != 0 && dir[i] == -1) cur.p[i] -= 1;
+      }
+      cw_leaf(cur.p, bits, octree, O, ncell);
+      done = nl == 0 || (cell[0] == bcell[0] && cell[1] == bcell[1]) ||
+             (ncell[0] == cell[0] && ncell[1] == cell[1]);
+      memcpy(cell, ncell, sizeof cell);
+    } while (!done);
+    counts[s] = count;
+    status[s] = st;
+  }
+  return total;
+}

@@ -111,6 +111,15 @@ void oracle_find_leaf(const int64_t *p, int64_t n, int dim, int bits, const int3
  * separated by whitespace (the leaf cast to unsigned char, :76-77); a token
  * that is not an optional sign and 1..10 digits in int32 range, or too few
  * tokens, returns -1; more than `cap` nodes returns -2. */
+/* CellWalk (viewer/Octree2.cpp:265-313) for n 2D segments (segs: ax, ay,
+ * bx, by per segment, grid units): the leaf cells it visits, in order,
+ * located with oracle_find_leaf.  Per segment counts[i] cells and status[i]
+ * (0; 1 after 10000 cells, where the reference throws).  Cells are written
+ * as {origin x, origin y, width, parent, octant} int32 records while fewer
+ * than cap have been written; returns the total. */
+int64_t oracle_cell_walk(const float *segs, int64_t n, int bits, const int32_t *octree, int64_t O,
+                         int32_t *counts, int32_t *status, int32_t *cells, int64_t cap);
+
 int64_t oracle_octree_format_text(const int32_t *octree, int64_t O, int dim, char *out);
 int oracle_octree_parse_text(const char *text, int64_t len, int dim, int32_t *octree,
                              int64_t cap, int64_t *O_out);

@@ -127,6 +127,8 @@ def lib():
             "pgvd_quantize": ([vp, vp, i64, i32, i32, vp, i32, vp, C.POINTER(i64), vp], i32),
             "pgvd_points_to_morton": ([vp, vp, i32, i64, i32, i32, vp, i32, vp], i32),
             "pgvd_find_leaf": ([vp, vp, i64, i32, i32, vp, i64, vp, vp, vp, vp, vp, vp], i32),
+            "pgvd_cell_walk": ([vp, vp, i64, i32, vp, i64, vp, vp, i64, C.POINTER(i64), vp, vp],
+                               i32),
             "pgvd_octree_format_text": ([vp, vp, i64, i32, vp, i64, C.POINTER(i64), vp], i32),
             "pgvd_octree_parse_text": ([vp, vp, i64, i32, vp, i64, C.POINTER(i64), vp], i32),
             "pgvd_octree_save": ([vp, vp, i64, i32, C.c_char_p, i32], i32),
@@ -380,6 +382,34 @@ class Device:
                                          outs["parent"].ptr, outs["octant"].ptr,
                                          outs["data"].ptr, outs["level"].ptr,
                                          st.ptr if st is not None else None, None))
+
+    def cell_walk(self, segs: np.ndarray, bits: int, children: np.ndarray,
+                  leaf: np.ndarray) -> dict:
+        """CellWalk for every row (ax, ay, bx, by) of segs on a 2D octree given as
+        host arrays: offsets (n + 1), cells (rows of origin x, origin y, width,
+        parent, octant) and status."""
+        segs = np.ascontiguousarray(segs, dtype=np.float32).reshape(-1, 4)
+        n = segs.shape[0]
+        O = children.shape[0]
+        oct_ = np.ascontiguousarray(np.concatenate([children.reshape(O, -1), leaf.reshape(O, 1)],
+                                                   axis=1).astype(np.int32))
+        ds, do = self.upload(segs), self.upload(oct_)
+        doff, dst = self.alloc(8 * (n + 1)), self.alloc(4 * max(n, 1))
+        total = C.c_int64()
+        dc = None
+        try:
+            self._check(lib().pgvd_cell_walk(self.ctx, ds.ptr, n, bits, do.ptr, O, doff.ptr, None,
+                                             0, C.byref(total), dst.ptr, None))
+            dc = self.alloc(20 * max(total.value, 1))
+            self._check(lib().pgvd_cell_walk(self.ctx, ds.ptr, n, bits, do.ptr, O, doff.ptr,
+                                             dc.ptr, total.value, C.byref(total), dst.ptr, None))
+            return {"offsets": self.download(doff, np.int64, n + 1),
+                    "cells": self.download(dc, np.int32, 5 * total.value).reshape(-1, 5),
+                    "status": self.download(dst, np.int32, n)}
+        finally:
+            for b in (ds, do, doff, dst, dc):
+                if b is not None:
+                    b.free()
 
     # ---- the octree's text and binary forms (C/OctNode.h:58-98) ----
     def format_octree_text(self, octree, O: int, dim: int) -> bytes:

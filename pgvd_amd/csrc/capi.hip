@@ -755,6 +755,43 @@ int pgvd_find_leaf(pgvd_ctx *ctx, const int32_t *pts, int64_t q, int dim, int bi
   return launch_status("find_leaf");
 }
 
+int pgvd_cell_walk(pgvd_ctx *ctx, const float *segs, int64_t n, int bits, const int32_t *octree,
+                   int64_t octree_size, int64_t *offsets, int32_t *cells, int64_t capacity,
+                   int64_t *total_out, int32_t *status, pgvd_stream s) {
+  if (!ctx || !octree || !offsets || !total_out || (n > 0 && !segs))
+    return fail(PGVD_ERR_INVALID, "null argument");
+  if (bits < 1 || bits > 30) return fail(PGVD_ERR_INVALID, "bits must be in 1..30 (2D)");
+  if (n < 0 || n >= (int64_t(1) << 31)) return fail(PGVD_ERR_INVALID, "n must be in [0, 2^31)");
+  if (octree_size < 1) return fail(PGVD_ERR_INVALID, "octree_size must be >= 1");
+  if (capacity < 0) return fail(PGVD_ERR_INVALID, "capacity must be >= 0");
+  CHK(use_device(ctx));
+  hipStream_t st = pick(ctx, s);
+  if (n == 0) {
+    const int64_t zero = 0;
+    HIPCHK(hipMemcpy(offsets, &zero, 8, hipMemcpyHostToDevice));
+    *total_out = 0;
+    return PGVD_OK;
+  }
+  void *counts = nullptr, *ws = nullptr;
+  CHK(pool_get(ctx, "walk_counts", 4 * static_cast<size_t>(n), &counts));
+  CHK(pool_get(ctx, "walk_ws", 8 * static_cast<size_t>(walk_scan_ws_words(n)), &ws));
+  int64_t total = 0;
+  {
+    Stage t(ctx, "cell_walk_count", st);
+    launch_cell_walk_count(segs, n, bits, octree, octree_size, static_cast<uint32_t *>(counts),
+                           status, offsets, static_cast<uint64_t *>(ws), st);
+    CHK(launch_status("cell_walk_count"));
+  }
+  HIPCHK(hipMemcpyAsync(&total, offsets + n, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *total_out = total;
+  if (!cells) return PGVD_OK;
+  if (total > capacity) return fail(PGVD_ERR_CAPACITY, "cell buffer too small (*total_out cells needed)");
+  Stage t(ctx, "cell_walk", st);
+  launch_cell_walk_write(segs, n, bits, octree, octree_size, offsets, cells, st);
+  return launch_status("cell_walk");
+}
+
 int pgvd_octree_format_text(pgvd_ctx *ctx, const int32_t *octree, int64_t O, int dim, char *text,
                             int64_t capacity, int64_t *len_out, pgvd_stream s) {
   if (!ctx || !len_out || (O > 0 && !octree)) return fail(PGVD_ERR_INVALID, "null argument");

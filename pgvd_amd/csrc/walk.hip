@@ -1,0 +1,245 @@
+// walk.hip -- batched cell walks: CellWalk (viewer/Octree2.cpp:265-313) for
+// many 2D segments at once, one lane per segment.  Each step locates the
+// current point's leaf (FindLeaf with the octant masked to two bits), visits
+// it, intersects the segment with the cell's four edges
+// (OctreeUtils::FindIntersections, OctreeUtils.cpp:80-172) and moves to the
+// farthest intersection.
+//
+// The walk's length is not known in advance, so a batch runs it twice: a
+// counting pass, a scan of the counts to offsets (decoupled look-back), and
+// a pass that writes every visited cell at its segment's offset.  The float
+// arithmetic follows the reference operation by operation (no contraction,
+// correctly rounded division and square root, epsilon tests in double), so
+// both passes and the serial restatement visit the same cells.
+#include "kernels.hpp"
+#include "tilescan.hpp"
+
+namespace pgvd {
+
+namespace {
+
+constexpr int kWalkMax = 10000;        // the reference throws past this many cells
+constexpr double kWalkEps = 1e-6;      // opencl/defs.h:135 (a double)
+
+// FindLeaf of the truncated point (convert_int2); cell = origin x, origin
+// y, width, parent, octant.
+__device__ inline void walk_leaf(float px, float py, int bits, const int32_t *__restrict__ octree,
+                                 int64_t size, int32_t cell[5]) {
+  const int32_t qx = (int32_t)px, qy = (int32_t)py;
+  const int64_t c[2] = {qx, qy};
+  uint64_t lo, hi;
+  morton128<2>(c, bits, lo, hi);
+  int64_t idx = 0;
+  uint32_t o = 0;
+  int32_t lev = 0;
+  for (int sh = 2 * bits - 2; sh >= 0; sh -= 2) {
+    o = (uint32_t)(lo >> sh) & 3u;
+    ++lev;
+    const int32_t *node = octree + idx * 5;
+    const int32_t child = node[o];
+    if ((node[4] >> o) & 1) break;
+    if (child < 0 || child >= size) break;
+    idx = child;
+  }
+  const int32_t w = (int32_t)(1u << (bits - lev));
+  cell[0] = qx & ~(w - 1);
+  cell[1] = qy & ~(w - 1);
+  cell[2] = w;
+  cell[3] = (int32_t)idx;
+  cell[4] = (int32_t)o;
+}
+
+__device__ inline float dot2(float ux, float uy, float vx, float vy) {
+  float r = 0.0f;
+  r += ux * vx;
+  r += uy * vy;
+  return r;
+}
+
+// OctreeUtils.cpp:80-172 for the cell at (ox, oy) of width w: candidates
+// in the reference's order (bottom, right, top, left) in fixed slots t, x,
+// y; returns the mask of those kept (the reference's list is the kept slots
+// in order).
+__device__ inline uint32_t walk_isects(float ax, float ay, float bx, float by, int32_t ox,
+                                       int32_t oy, int32_t w, float (&ts)[4], float (&px)[4],
+                                       float (&py)[4]) {
+  float vx = bx - ax, vy = by - ay;
+  const float len = sqrtf(dot2(vx, vy, vx, vy));
+  vx = vx / len;
+  vy = vy / len;
+  const float mnx = (float)ox, mny = (float)oy;
+  const float mxx = (float)ox + (float)w, mxy = (float)oy + (float)w;
+  ts[0] = ((float)oy - ay) / vy;          // bottom
+  px[0] = ax + vx * ts[0];
+  py[0] = (float)oy;
+  ts[1] = ((float)(ox + w) - ax) / vx;    // right
+  px[1] = (float)(ox + w);
+  py[1] = ay + vy * ts[1];
+  ts[2] = ((float)(oy + w) - ay) / vy;    // top
+  px[2] = ax + vx * ts[2];
+  py[2] = (float)(oy + w);
+  ts[3] = ((float)ox - ax) / vx;          // left
+  px[3] = (float)ox;
+  py[3] = ay + vy * ts[3];
+  const float eps = (float)kWalkEps;
+  uint32_t keep = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float t = ts[i];
+    if ((double)t >= -kWalkEps && (double)t < (double)len + kWalkEps &&
+        !(px[i] < mnx - eps || px[i] > mxx + eps) && !(py[i] < mny - eps || py[i] > mxy + eps))
+      keep |= 1u << i;
+  }
+  const int n = __popc(keep);
+  if (n == 2) {
+    const int f0 = __ffs(keep) - 1, f1 = 31 - __clz(keep);
+    float x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i == f0) { x0 = px[i]; y0 = py[i]; }
+      if (i == f1) { x1 = px[i]; y1 = py[i]; }
+    }
+    const float dx = x0 - x1, dy = y0 - y1;
+    if ((double)sqrtf(dot2(dx, dy, dx, dy)) < kWalkEps) keep &= ~(1u << f1);
+  } else if (n > 2) {
+    // the first two kept slots unless a pair's |dot| with v exceeds 0
+    int bi = __ffs(keep) - 1;
+    int bj = __ffs(keep & (keep - 1)) - 1;
+    float best = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = i + 1; j < 4; ++j) {
+        if (((keep >> i) & 1) && ((keep >> j) & 1)) {
+          float ux = px[j] - px[i], uy = py[j] - py[i];
+          const float l = sqrtf(dot2(ux, uy, ux, uy));
+          ux = ux / l;
+          uy = uy / l;
+          const float cd = fabsf(dot2(ux, uy, vx, vy));
+          if (cd > best) {
+            best = cd;
+            bi = i;
+            bj = j;
+          }
+        }
+      }
+    keep = (1u << bi) | (1u << bj);
+  }
+  return keep;
+}
+
+// One pass over the segments: counts and status (cells == nullptr), or the
+// cells at offsets[i].
+__global__ __launch_bounds__(kBlock) void cell_walk_kernel(
+    const float *__restrict__ segs, int64_t n, int bits, const int32_t *__restrict__ octree,
+    int64_t size, uint32_t *__restrict__ counts, int32_t *__restrict__ status,
+    const int64_t *__restrict__ offsets, int32_t *__restrict__ cells) {
+  for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * kBlock) {
+    const float ax = segs[4 * s], ay = segs[4 * s + 1], bx = segs[4 * s + 2], by = segs[4 * s + 3];
+    const int dx = bx > ax ? 1 : (bx < ax ? -1 : 0);
+    const int dy = by > ay ? 1 : (by < ay ? -1 : 0);
+    float ct = 0.0f, cx = ax, cy = ay;
+    int32_t bcell[5], cell[5], ncell[5];
+    walk_leaf(bx, by, bits, octree, size, bcell);
+    walk_leaf(cx, cy, bits, octree, size, cell);
+    int32_t *out = cells ? cells + 5 * offsets[s] : nullptr;
+    int count = 0, st = 0;
+    bool done = false;
+    do {
+      if (++count > kWalkMax) {
+        st = 1;
+        --count;
+        break;
+      }
+      if (out) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) out[5 * (count - 1) + k] = cell[k];
+      }
+      float ts[4], px[4], py[4];
+      const uint32_t keep = walk_isects(ax, ay, bx, by, cell[0], cell[1], cell[2], ts, px, py);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (((keep >> i) & 1) && ts[i] > ct) {
+          ct = ts[i];
+          cx = px[i];
+          cy = py[i];
+        }
+      if (cx == (float)cell[0] && cx != 0.0f && dx == -1) cx -= 1.0f;
+      if (cy == (float)cell[1] && cy != 0.0f && dy == -1) cy -= 1.0f;
+      walk_leaf(cx, cy, bits, octree, size, ncell);
+      done = keep == 0 || (cell[0] == bcell[0] && cell[1] == bcell[1]) ||
+             (ncell[0] == cell[0] && ncell[1] == cell[1]);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) cell[k] = ncell[k];
+    } while (!done);
+    if (!cells) {
+      counts[s] = (uint32_t)count;
+      if (status) status[s] = st;
+    }
+  }
+}
+
+// offsets[0] = 0, offsets[i + 1] = counts[0] + ... + counts[i]; 1024 counts
+// per tile; ws: word 0 tile counter, status array from word 4.
+__global__ __launch_bounds__(kBlock) void walk_scan_kernel(const uint32_t *__restrict__ counts,
+                                                           int64_t n,
+                                                           int64_t *__restrict__ offsets,
+                                                           uint64_t *__restrict__ ws) {
+  __shared__ uint32_t s_w[kBlock / 64];
+  __shared__ int s_tile;
+  const int t = threadIdx.x;
+  if (t == 0) s_tile = (int)atomicAdd(reinterpret_cast<unsigned int *>(ws), 1u);
+  __syncthreads();
+  const int tile = s_tile;
+  const int64_t i0 = ((int64_t)tile * kBlock + t) * 4;
+  uint32_t c[4], sum = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    c[k] = i0 + k < n ? counts[i0 + k] : 0u;
+    sum += c[k];
+  }
+  uint32_t total;
+  const uint32_t ex = block_excl_sum<uint32_t>(sum, s_w, &total);
+  uint64_t run = tile_prefix<kBlock>(ws + 4, tile, total) + ex;
+  if (tile == 0 && t == 0) offsets[0] = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    run += c[k];
+    if (i0 + k < n) offsets[i0 + k + 1] = (int64_t)run;
+  }
+}
+
+__global__ void walk_ws_zero_kernel(uint64_t *ws, int64_t words) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < words;
+       i += (int64_t)gridDim.x * kBlock)
+    ws[i] = 0;
+}
+
+}  // namespace
+
+int64_t walk_scan_ws_words(int64_t n) { return 4 + (n + 4 * kBlock - 1) / (4 * kBlock); }
+
+void launch_cell_walk_count(const float *segs, int64_t n, int bits, const int32_t *octree,
+                            int64_t size, uint32_t *counts, int32_t *status, int64_t *offsets,
+                            uint64_t *ws, hipStream_t s) {
+  int64_t grid = (n + kBlock - 1) / kBlock;
+  if (grid > 8192) grid = 8192;
+  cell_walk_kernel<<<(unsigned)grid, kBlock, 0, s>>>(segs, n, bits, octree, size, counts, status,
+                                                     nullptr, nullptr);
+  const int64_t words = walk_scan_ws_words(n);
+  walk_ws_zero_kernel<<<(unsigned)std::min<int64_t>((words + kBlock - 1) / kBlock, 1024), kBlock,
+                        0, s>>>(ws, words);
+  const int64_t tiles = (n + 4 * kBlock - 1) / (4 * kBlock);
+  walk_scan_kernel<<<(unsigned)tiles, kBlock, 0, s>>>(counts, n, offsets, ws);
+}
+
+void launch_cell_walk_write(const float *segs, int64_t n, int bits, const int32_t *octree,
+                            int64_t size, const int64_t *offsets, int32_t *cells, hipStream_t s) {
+  int64_t grid = (n + kBlock - 1) / kBlock;
+  if (grid > 8192) grid = 8192;
+  cell_walk_kernel<<<(unsigned)grid, kBlock, 0, s>>>(segs, n, bits, octree, size, nullptr,
+                                                     nullptr, offsets, cells);
+}
+
+}  // namespace pgvd

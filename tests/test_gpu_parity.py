@@ -468,3 +468,73 @@ def test_reference_data_builds(dev, name):
         got = dev.build_octree(pts, bits)
         want = oracle.build(oracle.quantize(pts, 1 << bits), bits)
         assert_tree_equal(got, want, where=f"{name} bits={bits}")
+
+
+# ------------------------------------------------------------ cell walk ----
+def _assert_walk_equal(got, want, where):
+    assert np.array_equal(got["status"], want["status"]), where
+    assert np.array_equal(got["offsets"], want["offsets"]), where
+    assert np.array_equal(got["cells"], want["cells"]), where
+
+
+def _obj2oct(poly, lo, bbw, width):
+    """viewer/Octree2.cpp:216-226 in float32: ((v - min) / bbw) * width,
+    clamped below width."""
+    oct_ = ((poly - lo) / np.float32(bbw)) * np.float32(width)
+    return np.where(oct_ >= width, np.float32(width - 0.0001), oct_).astype(np.float32)
+
+
+@pytest.mark.parametrize("bits,npts,nseg,reach", [(10, 400, 3000, 1.0), (16, 65536, 20000, 0.05),
+                                                  (20, 200000, 5000, 0.01), (3, 20, 500, 1.0)])
+def test_cell_walk_matches_oracle(dev, bits, npts, nseg, reach):
+    """CellWalk on the GPU against the oracle's restatement (pinned by brute
+    force) on a GPU-built quadtree: random segments, axis-parallel and
+    degenerate ones, and segments along cell edges (integer coordinates),
+    which take the reference's corner and decrement paths."""
+    rng = np.random.default_rng(bits)
+    tree = dev.build_octree(_pts(npts, 2, seed=bits), bits)
+    W = 1 << bits
+    a = rng.random((nseg, 2)) * (W - 1e-3)
+    b = np.clip(a + (rng.random((nseg, 2)) * 2 - 1) * reach * W, 0, W - 1e-3)
+    segs = np.concatenate([a, b], axis=1).astype(np.float32)
+    k = nseg // 8
+    segs[:k, 3] = segs[:k, 1]                            # horizontal
+    segs[k:2 * k, 2] = segs[k:2 * k, 0]                  # vertical
+    segs[2 * k:3 * k, 2:] = segs[2 * k:3 * k, :2]        # a == b
+    segs[3 * k:4 * k] = np.floor(segs[3 * k:4 * k])      # on grid lines
+    got = dev.cell_walk(segs, bits, tree["children"], tree["leaf"])
+    want = oracle.cell_walk(segs, bits, tree["children"], tree["leaf"])
+    _assert_walk_equal(got, want, f"bits={bits}")
+    assert (np.diff(want["offsets"]) > 1).mean() > 0.3
+
+
+def test_cell_walk_reference_polylines(dev):
+    """The data2 scenes walked as the viewer walks them (FindMultiCells,
+    viewer/Octree2.cpp:430-455): each polyline's consecutive vertices, mapped
+    by obj2Oct into the octree of the scene's points."""
+    pts = _data_points()
+    for name in ("data2_test1", "data2_test2", "data2_test3", "data2_test4", "data2_test5"):
+        v = pts[name]
+        for bits in (8, 16):
+            tree = dev.build_octree(v, bits)
+            lo = v.min(axis=0)
+            bbw = float((v.max(axis=0) - lo).max())
+            q = _obj2oct(v, lo, bbw, 1 << bits)
+            segs = np.concatenate([q[:-1], q[1:]], axis=1)
+            got = dev.cell_walk(segs, bits, tree["children"], tree["leaf"])
+            want = oracle.cell_walk(segs, bits, tree["children"], tree["leaf"])
+            _assert_walk_equal(got, want, f"{name} bits={bits}")
+
+
+def test_cell_walk_stops_after_10000_cells(dev):
+    """A walk along a diagonal of unit cells passes 10000 cells: status 1 and
+    the first 10000 cells, as the oracle (the reference throws there)."""
+    bits = 16
+    t = np.arange(0, 60000, dtype=np.float32)
+    diag = np.stack([t, t], axis=1) / np.float32(1 << bits) * 2 - 1
+    tree = dev.build_octree(diag.astype(np.float32), bits)
+    segs = np.array([[0.5, 0.7, 59000.5, 59000.2], [10.5, 10.2, 20.5, 20.7]], np.float32)
+    got = dev.cell_walk(segs, bits, tree["children"], tree["leaf"])
+    want = oracle.cell_walk(segs, bits, tree["children"], tree["leaf"])
+    _assert_walk_equal(got, want, "diagonal")
+    assert list(want["status"]) == [1, 0] and want["offsets"][1] == 10000

@@ -382,3 +382,86 @@ def test_octree_text_round_trip(dim, n):
     octree = np.concatenate([t["children"], t["leaf"][:, None]], axis=1)
     text = oracle.octree_format_text(octree, dim)
     assert np.array_equal(oracle.octree_parse_text(text, dim), octree)
+
+
+# ------------------------------------------------------------ cell walk ----
+def _leaf_boxes(t, bits):
+    """(origin x, origin y, width) of every leaf cell, labelled top-down."""
+    cells = _reachable_cells(t["children"], 2)
+    boxes = []
+    for node, (lvl, path) in cells.items():
+        for o in range(4):
+            if (int(t["leaf"][node]) >> o) & 1:
+                p = path + (o,)
+                org = [sum(((d >> k) & 1) << (bits - 1 - l) for l, d in enumerate(p))
+                       for k in range(2)]
+                boxes.append((org[0], org[1], 1 << (bits - len(p))))
+    return boxes
+
+
+def _overlap(a, b, lo, hi):
+    """Parameter interval of segment ab inside the box [lo, hi]^2 (Liang-Barsky,
+    double precision), or None."""
+    t0, t1 = 0.0, 1.0
+    for k in range(2):
+        d = float(b[k]) - float(a[k])
+        if d == 0.0:
+            if not lo[k] <= a[k] <= hi[k]:
+                return None
+            continue
+        ta, tb = (lo[k] - float(a[k])) / d, (hi[k] - float(a[k])) / d
+        t0, t1 = max(t0, min(ta, tb)), min(t1, max(ta, tb))
+        if t0 > t1:
+            return None
+    return t0, t1
+
+
+@pytest.mark.parametrize("bits,npts,seed", [(10, 400, 1), (12, 2000, 2), (8, 60, 3)])
+def test_cell_walk_against_brute_force(bits, npts, seed):
+    """CellWalk (viewer/Octree2.cpp:265-313, oracle) visits every leaf cell
+    the segment crosses and no cell it misses: a cell whose box shrunk by
+    delta meets the segment over a positive length must be visited, one whose
+    box grown by delta misses it must not be (cells between are ambiguous at
+    the reference's epsilons and are not checked).  The first cell holds a."""
+    rng = np.random.default_rng(seed)
+    t = oracle.build(rng.integers(0, 1 << bits, (npts, 2)), bits)
+    boxes = _leaf_boxes(t, bits)
+    segs = (rng.random((300, 4)) * ((1 << bits) - 1e-3)).astype(np.float32)
+    w = oracle.cell_walk(segs, bits, t["children"], t["leaf"])
+    delta = 1e-3
+    checked = 0
+    for s in range(segs.shape[0]):
+        a, b = segs[s, :2], segs[s, 2:]
+        got = w["cells"][w["offsets"][s]:w["offsets"][s + 1]]
+        assert w["status"][s] == 0
+        visited = {tuple(int(x) for x in c[:3]) for c in got}
+        ox, oy, wd = got[0][:3]
+        assert ox <= int(a[0]) < ox + wd and oy <= int(a[1]) < oy + wd
+        for (x, y, wd) in boxes:
+            inner = _overlap(a, b, (x + delta, y + delta), (x + wd - delta, y + wd - delta))
+            outer = _overlap(a, b, (x - delta, y - delta), (x + wd + delta, y + wd + delta))
+            if inner is not None and inner[1] > inner[0]:
+                assert (x, y, wd) in visited, (s, (x, y, wd))
+                checked += 1
+            elif outer is None:
+                assert (x, y, wd) not in visited, (s, (x, y, wd))
+    assert checked > segs.shape[0]
+
+
+def test_cell_walk_planted_mistake_fails():
+    """The brute-force check catches a walk that stops one cell early."""
+    rng = np.random.default_rng(4)
+    t = oracle.build(rng.integers(0, 1 << 10, (400, 2)), 10)
+    segs = (rng.random((50, 4)) * 1023).astype(np.float32)
+    w = oracle.cell_walk(segs, 10, t["children"], t["leaf"])
+    boxes = _leaf_boxes(t, 10)
+    missed = 0
+    for s in range(50):
+        got = w["cells"][w["offsets"][s]:w["offsets"][s + 1] - 1]     # last cell dropped
+        visited = {tuple(int(x) for x in c[:3]) for c in got}
+        a, b = segs[s, :2], segs[s, 2:]
+        for (x, y, wd) in boxes:
+            inner = _overlap(a, b, (x + 1e-3, y + 1e-3), (x + wd - 1e-3, y + wd - 1e-3))
+            if inner is not None and inner[1] > inner[0] and (x, y, wd) not in visited:
+                missed += 1
+    assert missed > 0
