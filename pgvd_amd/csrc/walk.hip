@@ -49,6 +49,50 @@ __device__ inline void walk_leaf(float px, float py, int bits, const int32_t *__
   cell[4] = (int32_t)o;
 }
 
+constexpr int kMaxBits = 30;
+
+// walk_leaf from the deepest node the point shares with the lane's previous
+// descent: that descent's key is pkey, its leaf level plev (0: none) and its
+// nodes by depth are in path[d * kBlock].  The top digits the two keys share
+// select the same nodes and the same octants, none of them a leaf above
+// plev, so the result is walk_leaf's.  Updates pkey, plev and the path.
+__device__ inline void walk_leaf_path(float px, float py, int bits,
+                                      const int32_t *__restrict__ octree, int64_t size,
+                                      int32_t *path, uint64_t &pkey, int &plev, int32_t cell[5]) {
+  const int32_t qx = (int32_t)px, qy = (int32_t)py;
+  const int64_t c[2] = {qx, qy};
+  uint64_t lo, hi;
+  morton128<2>(c, bits, lo, hi);
+  const uint64_t key = lo & ((1ull << (2 * bits)) - 1);
+  int d = 0;
+  if (plev > 0) {
+    const uint64_t x = key ^ pkey;
+    const int common = x ? (__clzll((long long)x) - (64 - 2 * bits)) >> 1 : bits;
+    d = common < plev - 1 ? common : plev - 1;
+  }
+  int64_t idx = path[d * kBlock];
+  uint32_t o = 0;
+  int32_t lev = d;
+  for (int sh = 2 * (bits - d) - 2; sh >= 0; sh -= 2) {
+    o = (uint32_t)(lo >> sh) & 3u;
+    ++lev;
+    const int32_t *node = octree + idx * 5;
+    const int32_t child = node[o];
+    if ((node[4] >> o) & 1) break;
+    if (child < 0 || child >= size) break;
+    idx = child;
+    path[lev * kBlock] = (int32_t)idx;
+  }
+  pkey = key;
+  plev = lev;
+  const int32_t w = (int32_t)(1u << (bits - lev));
+  cell[0] = qx & ~(w - 1);
+  cell[1] = qy & ~(w - 1);
+  cell[2] = w;
+  cell[3] = (int32_t)idx;
+  cell[4] = (int32_t)o;
+}
+
 __device__ inline float dot2(float ux, float uy, float vx, float vy) {
   float r = 0.0f;
   r += ux * vx;
@@ -134,6 +178,10 @@ __global__ __launch_bounds__(kBlock) void cell_walk_kernel(
     const float *__restrict__ segs, int64_t n, int bits, const int32_t *__restrict__ octree,
     int64_t size, uint32_t *__restrict__ counts, int32_t *__restrict__ status,
     const int64_t *__restrict__ offsets, int32_t *__restrict__ cells) {
+  // each lane's last descent, node per depth (depth 0 is the root)
+  __shared__ int32_t s_path[(kMaxBits + 1) * kBlock];
+  int32_t *path = s_path + threadIdx.x;
+  path[0] = 0;
   for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < n;
        s += (int64_t)gridDim.x * kBlock) {
     const float ax = segs[4 * s], ay = segs[4 * s + 1], bx = segs[4 * s + 2], by = segs[4 * s + 3];
@@ -141,8 +189,10 @@ __global__ __launch_bounds__(kBlock) void cell_walk_kernel(
     const int dy = by > ay ? 1 : (by < ay ? -1 : 0);
     float ct = 0.0f, cx = ax, cy = ay;
     int32_t bcell[5], cell[5], ncell[5];
+    uint64_t pkey = 0;
+    int plev = 0;
     walk_leaf(bx, by, bits, octree, size, bcell);
-    walk_leaf(cx, cy, bits, octree, size, cell);
+    walk_leaf_path(cx, cy, bits, octree, size, path, pkey, plev, cell);
     int32_t *out = cells ? cells + 5 * offsets[s] : nullptr;
     int count = 0, st = 0;
     bool done = false;
@@ -167,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void cell_walk_kernel(
         }
       if (cx == (float)cell[0] && cx != 0.0f && dx == -1) cx -= 1.0f;
       if (cy == (float)cell[1] && cy != 0.0f && dy == -1) cy -= 1.0f;
-      walk_leaf(cx, cy, bits, octree, size, ncell);
+      walk_leaf_path(cx, cy, bits, octree, size, path, pkey, plev, ncell);
       done = keep == 0 || (cell[0] == bcell[0] && cell[1] == bcell[1]) ||
              (ncell[0] == cell[0] && ncell[1] == cell[1]);
 #pragma unroll
