@@ -18,6 +18,7 @@
 #pragma once
 
 #include <cstdint>
+#include <iosfwd>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -170,5 +171,55 @@ std::vector<OctNode3> BuildOctreeInParallel(const std::vector<int3> &points, con
 std::vector<OctNode3> BuildOctreeFromPoints(const std::vector<float3> &points, const Resln &r);
 
 }  // namespace Karras
+
+// OctCell (OctCell.h:9-60) without its node pointers: the leaf cell a point
+// falls in, with the node holding it and the leaf's data.
+template <int DIM>
+struct OctCellT {
+  int32_t origin[DIM];
+  int32_t width;
+  int32_t parent_idx;
+  int32_t octant;
+  int32_t data;   // children[octant] of the parent (-1: an empty cell)
+};
+using OctCell2 = OctCellT<2>;
+using OctCell3 = OctCellT<3>;
+
+namespace OctreeUtils {
+
+// OctreeUtils::FindLeaf (OctreeUtils.cpp:18-71) with the octant masked to
+// DIM bits at every level (the reference leaves that mask commented out).
+// The single-point form uploads the octree on each call; FindLeaves locates
+// a batch in one launch.  Throws std::logic_error where no leaf is reached.
+OctCell2 FindLeaf(const int2 &p, const std::vector<OctNode2> &octree, const Resln &resln);
+OctCell3 FindLeaf(const int3 &p, const std::vector<OctNode3> &octree, const Resln &resln);
+std::vector<OctCell2> FindLeaves(const std::vector<int2> &points,
+                                 const std::vector<OctNode2> &octree, const Resln &resln);
+std::vector<OctCell3> FindLeaves(const std::vector<int3> &points,
+                                 const std::vector<OctNode3> &octree, const Resln &resln);
+
+// CellWalk (viewer/Octree2.cpp:259-313): calls v for every cell the segment
+// ab visits, in order.  Throws std::logic_error("Infinite loop") after
+// 10000 cells, as the reference does (once the first 10000 are visited).
+typedef void (*cwv)(const OctCell2 &cell, const float2 &a, const float2 &b,
+                    const std::vector<OctNode2> &octree, const Resln &resln, void *data);
+void CellWalk(const float2 &a, const float2 &b, const std::vector<OctNode2> &octree,
+              const Resln &resln, cwv v, void *data);
+// Many walks in two launches: cells[i] are segment i's cells in order;
+// throws std::logic_error("Infinite loop") when any walk passed 10000 cells.
+std::vector<std::vector<OctCell2>> CellWalks(const std::vector<float2> &a,
+                                             const std::vector<float2> &b,
+                                             const std::vector<OctNode2> &octree,
+                                             const Resln &resln);
+
+}  // namespace OctreeUtils
+
+// operator<< / operator>> for the octree (C/OctNode.h:58-98, commented out
+// in the reference), formatted and parsed on the device.  >> reads the rest
+// of the stream and sets failbit on malformed text.
+std::ostream &operator<<(std::ostream &out, const std::vector<OctNode2> &octree);
+std::ostream &operator<<(std::ostream &out, const std::vector<OctNode3> &octree);
+std::istream &operator>>(std::istream &in, std::vector<OctNode2> &octree);
+std::istream &operator>>(std::istream &in, std::vector<OctNode3> &octree);
 
 }  // namespace pgvd

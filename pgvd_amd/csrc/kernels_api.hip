@@ -5,6 +5,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <istream>
+#include <iterator>
+#include <ostream>
 
 #include "../../include/pgvd_kernels.hpp"
 
@@ -460,5 +463,212 @@ std::vector<OctNode3> BuildOctreeFromPoints(const std::vector<float3> &points, c
 }
 
 }  // namespace Karras
+
+// ------------------------------------------------------------ OctreeUtils ---
+namespace OctreeUtils {
+
+namespace {
+
+void check(int rc, const char *what) {
+  if (rc != PGVD_OK) throw std::runtime_error(std::string(what) + ": " + pgvd_last_error());
+}
+
+template <int DIM, typename P>
+std::vector<OctCellT<DIM>> find_leaves(const std::vector<P> &points,
+                                       const std::vector<OctNodeT<DIM>> &octree,
+                                       const Resln &resln) {
+  std::vector<OctCellT<DIM>> cells(points.size());
+  if (points.empty()) return cells;
+  if (octree.empty()) throw std::logic_error("FindLeaf on an empty octree");
+  Context &c = Context::Default();
+  const size_t q = points.size();
+  Buffer dp = c.alloc(sizeof(P) * q), doct = c.alloc(sizeof(OctNodeT<DIM>) * octree.size());
+  Buffer out = c.alloc(5 * 4 * q);
+  c.upload(dp, points.data(), sizeof(P) * q);
+  c.upload(doct, octree.data(), sizeof(OctNodeT<DIM>) * octree.size());
+  int32_t *o = static_cast<int32_t *>(out.ptr());
+  check(pgvd_find_leaf(c.raw(), static_cast<const int32_t *>(dp.ptr()), (int64_t)q, DIM,
+                       resln.bits, static_cast<const int32_t *>(doct.ptr()),
+                       (int64_t)octree.size(), o, o + q, o + 2 * q, o + 3 * q,
+                       reinterpret_cast<uint32_t *>(o + 4 * q), nullptr),
+        "pgvd_find_leaf");
+  std::vector<int32_t> h(5 * q);
+  c.download(h.data(), out, h.size() * 4);
+  for (size_t i = 0; i < q; ++i) {
+    if (h[4 * q + i] != 0) throw std::logic_error("FindLeaf: no leaf reached");
+    const int32_t *pc = reinterpret_cast<const int32_t *>(&points[i]);
+    const int32_t w = (int32_t)(1u << (resln.bits - h[3 * q + i]));
+    for (int k = 0; k < DIM; ++k) cells[i].origin[k] = pc[k] & ~(w - 1);
+    cells[i].width = w;
+    cells[i].parent_idx = h[i];
+    cells[i].octant = h[q + i];
+    cells[i].data = h[2 * q + i];
+  }
+  return cells;
+}
+
+}  // namespace
+
+OctCell2 FindLeaf(const int2 &p, const std::vector<OctNode2> &octree, const Resln &resln) {
+  return find_leaves<2>(std::vector<int2>{p}, octree, resln)[0];
+}
+OctCell3 FindLeaf(const int3 &p, const std::vector<OctNode3> &octree, const Resln &resln) {
+  return find_leaves<3>(std::vector<int3>{p}, octree, resln)[0];
+}
+std::vector<OctCell2> FindLeaves(const std::vector<int2> &points,
+                                 const std::vector<OctNode2> &octree, const Resln &resln) {
+  return find_leaves<2>(points, octree, resln);
+}
+std::vector<OctCell3> FindLeaves(const std::vector<int3> &points,
+                                 const std::vector<OctNode3> &octree, const Resln &resln) {
+  return find_leaves<3>(points, octree, resln);
+}
+
+namespace {
+
+// The walks and each one's status (1: past 10000 cells).
+std::vector<std::vector<OctCell2>> walks(const std::vector<float2> &a,
+                                         const std::vector<float2> &b,
+                                         const std::vector<OctNode2> &octree, const Resln &resln,
+                                         std::vector<int32_t> &st) {
+  if (a.size() != b.size()) throw std::logic_error("CellWalks: a and b differ in length");
+  const size_t n = a.size();
+  std::vector<std::vector<OctCell2>> out(n);
+  if (n == 0) return out;
+  if (octree.empty()) throw std::logic_error("CellWalk on an empty octree");
+  Context &c = Context::Default();
+  std::vector<float> segs(4 * n);
+  for (size_t i = 0; i < n; ++i) {
+    segs[4 * i] = a[i].x;
+    segs[4 * i + 1] = a[i].y;
+    segs[4 * i + 2] = b[i].x;
+    segs[4 * i + 3] = b[i].y;
+  }
+  Buffer ds = c.alloc(4 * segs.size()), doct = c.alloc(sizeof(OctNode2) * octree.size());
+  Buffer doff = c.alloc(8 * (n + 1)), dst = c.alloc(4 * n);
+  c.upload(ds, segs.data(), 4 * segs.size());
+  c.upload(doct, octree.data(), sizeof(OctNode2) * octree.size());
+  int64_t total = 0;
+  const auto *sp = static_cast<const float *>(ds.ptr());
+  const auto *op = static_cast<const int32_t *>(doct.ptr());
+  auto *offp = static_cast<int64_t *>(doff.ptr());
+  check(pgvd_cell_walk(c.raw(), sp, (int64_t)n, resln.bits, op, (int64_t)octree.size(), offp,
+                       nullptr, 0, &total, static_cast<int32_t *>(dst.ptr()), nullptr),
+        "pgvd_cell_walk");
+  Buffer dc = c.alloc(20 * (size_t)std::max<int64_t>(total, 1));
+  check(pgvd_cell_walk(c.raw(), sp, (int64_t)n, resln.bits, op, (int64_t)octree.size(), offp,
+                       static_cast<int32_t *>(dc.ptr()), total, &total,
+                       static_cast<int32_t *>(dst.ptr()), nullptr),
+        "pgvd_cell_walk");
+  std::vector<int64_t> off(n + 1);
+  std::vector<int32_t> cells(5 * (size_t)total);
+  st.assign(n, 0);
+  c.download(off.data(), doff, 8 * (n + 1));
+  c.download(st.data(), dst, 4 * n);
+  if (total) c.download(cells.data(), dc, 4 * cells.size());
+  for (size_t i = 0; i < n; ++i) {
+    for (int64_t k = off[i]; k < off[i + 1]; ++k) {
+      const int32_t *r = &cells[5 * k];
+      OctCell2 cell;
+      cell.origin[0] = r[0];
+      cell.origin[1] = r[1];
+      cell.width = r[2];
+      cell.parent_idx = r[3];
+      cell.octant = r[4];
+      cell.data = octree[r[3]].children[r[4]];
+      out[i].push_back(cell);
+    }
+  }
+  return out;
+}
+
+}  // namespace
+
+std::vector<std::vector<OctCell2>> CellWalks(const std::vector<float2> &a,
+                                             const std::vector<float2> &b,
+                                             const std::vector<OctNode2> &octree,
+                                             const Resln &resln) {
+  std::vector<int32_t> st;
+  auto out = walks(a, b, octree, resln, st);
+  for (int32_t x : st)
+    if (x) throw std::logic_error("Infinite loop");
+  return out;
+}
+
+void CellWalk(const float2 &a, const float2 &b, const std::vector<OctNode2> &octree,
+              const Resln &resln, cwv v, void *data) {
+  std::vector<int32_t> st;
+  const auto w = walks(std::vector<float2>{a}, std::vector<float2>{b}, octree, resln, st);
+  for (const OctCell2 &cell : w[0]) v(cell, a, b, octree, resln, data);
+  if (st[0]) throw std::logic_error("Infinite loop");   // viewer/Octree2.cpp:286-287
+}
+
+}  // namespace OctreeUtils
+
+// ------------------------------------------------- octree stream operators ---
+namespace {
+
+template <int DIM>
+void write_octree(std::ostream &out, const std::vector<OctNodeT<DIM>> &octree) {
+  Context &c = Context::Default();
+  const size_t bytes = sizeof(OctNodeT<DIM>) * octree.size();
+  Buffer d = c.alloc(std::max<size_t>(bytes, 4));
+  if (bytes) c.upload(d, octree.data(), bytes);
+  int64_t len = 0;
+  const auto *p = static_cast<const int32_t *>(d.ptr());
+  int rc = pgvd_octree_format_text(c.raw(), p, (int64_t)octree.size(), DIM, nullptr, 0, &len,
+                                   nullptr);
+  if (rc != PGVD_OK) throw std::runtime_error(std::string("format: ") + pgvd_last_error());
+  Buffer t = c.alloc((size_t)len);
+  rc = pgvd_octree_format_text(c.raw(), p, (int64_t)octree.size(), DIM,
+                               static_cast<char *>(t.ptr()), len, &len, nullptr);
+  if (rc != PGVD_OK) throw std::runtime_error(std::string("format: ") + pgvd_last_error());
+  std::string text((size_t)len, ' ');
+  c.download(&text[0], t, (size_t)len);
+  out.write(text.data(), (std::streamsize)text.size());
+}
+
+template <int DIM>
+void read_octree(std::istream &in, std::vector<OctNodeT<DIM>> &octree) {
+  const std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+  if (text.empty()) {
+    in.setstate(std::ios::failbit);
+    return;
+  }
+  Context &c = Context::Default();
+  Buffer t = c.alloc(text.size());
+  c.upload(t, text.data(), text.size());
+  const int64_t cap = (int64_t)(text.size() / (2 * ((1 << DIM) + 1)) + 1);
+  Buffer d = c.alloc(sizeof(OctNodeT<DIM>) * (size_t)cap);
+  int64_t O = 0;
+  const int rc = pgvd_octree_parse_text(c.raw(), static_cast<const char *>(t.ptr()),
+                                        (int64_t)text.size(), DIM, static_cast<int32_t *>(d.ptr()),
+                                        cap, &O, nullptr);
+  if (rc != PGVD_OK) {
+    in.setstate(std::ios::failbit);
+    return;
+  }
+  octree.resize((size_t)O);
+  if (O) c.download(octree.data(), d, sizeof(OctNodeT<DIM>) * (size_t)O);
+}
+
+}  // namespace
+
+std::ostream &operator<<(std::ostream &out, const std::vector<OctNode2> &octree) {
+  write_octree<2>(out, octree);
+  return out;
+}
+std::ostream &operator<<(std::ostream &out, const std::vector<OctNode3> &octree) {
+  write_octree<3>(out, octree);
+  return out;
+}
+std::istream &operator>>(std::istream &in, std::vector<OctNode2> &octree) {
+  read_octree<2>(in, octree);
+  return in;
+}
+std::istream &operator>>(std::istream &in, std::vector<OctNode3> &octree) {
+  read_octree<3>(in, octree);
+  return in;
+}
 
 }  // namespace pgvd

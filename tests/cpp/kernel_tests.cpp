@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "pgvd_kernels.hpp"
@@ -252,6 +254,96 @@ static void karras_scenario() {
   std::printf("PASS Karras facade (Quantize, BuildOctreeInParallel) in 3D\n");
 }
 
+// Octree consumers: FindLeaf (OctreeUtils.cpp:18-71), CellWalk
+// (viewer/Octree2.cpp:265-313) with a visitor, and the octree's stream
+// operators (C/OctNode.h:58-98), each against the oracle.
+struct WalkLog {
+  std::vector<OctCell2> cells;
+};
+static void log_cell(const OctCell2 &cell, const float2 &, const float2 &,
+                     const std::vector<OctNode2> &, const Resln &, void *data) {
+  static_cast<WalkLog *>(data)->cells.push_back(cell);
+}
+
+static void query_scenarios() {
+  std::mt19937 rng(9);
+  // FindLeaves, 3D
+  {
+    const Resln r = make_resln(1 << 21, 3);
+    std::uniform_int_distribution<int> u(0, (1 << 21) - 1);
+    std::vector<int3> pts(20000);
+    for (auto &p : pts) p = {u(rng), u(rng), u(rng)};
+    auto oct = Karras::BuildOctreeInParallel(pts, r);
+    std::vector<int3> q(5000);
+    for (auto &p : q) p = {u(rng), u(rng), u(rng)};
+    auto cells = OctreeUtils::FindLeaves(q, oct, r);
+    std::vector<int64_t> oq(3 * q.size());
+    for (size_t i = 0; i < q.size(); ++i) { oq[3 * i] = q[i].x; oq[3 * i + 1] = q[i].y; oq[3 * i + 2] = q[i].z; }
+    std::vector<int32_t> par(q.size()), octn(q.size()), dat(q.size()), lev(q.size()), st(q.size());
+    oracle_find_leaf(oq.data(), (int64_t)q.size(), 3, r.bits, &oct[0].children[0], (int64_t)oct.size(),
+                     par.data(), octn.data(), dat.data(), lev.data(), st.data());
+    bool ok = cells.size() == q.size();
+    for (size_t i = 0; i < q.size() && ok; ++i) {
+      const int32_t w = 1 << (r.bits - lev[i]);
+      ok = st[i] == 0 && cells[i].parent_idx == par[i] && cells[i].octant == octn[i] &&
+           cells[i].data == dat[i] && cells[i].width == w && cells[i].origin[0] == (q[i].x & ~(w - 1)) &&
+           cells[i].origin[2] == (q[i].z & ~(w - 1));
+    }
+    CHECK(ok, "FindLeaves equals the serial FindLeaf");
+    const OctCell3 one = OctreeUtils::FindLeaf(q[7], oct, r);
+    CHECK(one.parent_idx == par[7] && one.octant == octn[7], "FindLeaf of one point");
+  }
+  // CellWalk with a visitor, 2D
+  {
+    const Resln r = make_resln(1 << 12, 2);
+    std::uniform_int_distribution<int> u(0, (1 << 12) - 1);
+    std::uniform_real_distribution<float> f(0.f, 4095.9f);
+    std::vector<int2> pts(3000);
+    for (auto &p : pts) p = {u(rng), u(rng)};
+    auto oct = Karras::BuildOctreeInParallel(pts, r);
+    bool ok = true;
+    for (int s = 0; s < 50 && ok; ++s) {
+      const float2 a = {f(rng), f(rng)}, b = {f(rng), f(rng)};
+      WalkLog log;
+      OctreeUtils::CellWalk(a, b, oct, r, log_cell, &log);
+      const float seg[4] = {a.x, a.y, b.x, b.y};
+      int32_t count = 0, st = 0;
+      const int64_t tot = oracle_cell_walk(seg, 1, r.bits, &oct[0].children[0], (int64_t)oct.size(),
+                                           &count, &st, nullptr, 0);
+      std::vector<int32_t> want(5 * (size_t)tot);
+      oracle_cell_walk(seg, 1, r.bits, &oct[0].children[0], (int64_t)oct.size(), &count, &st,
+                       want.data(), tot);
+      ok = st == 0 && (int64_t)log.cells.size() == tot;
+      for (int64_t k = 0; k < tot && ok; ++k) {
+        const OctCell2 &c = log.cells[k];
+        ok = c.origin[0] == want[5 * k] && c.origin[1] == want[5 * k + 1] && c.width == want[5 * k + 2] &&
+             c.parent_idx == want[5 * k + 3] && c.octant == want[5 * k + 4];
+      }
+    }
+    CHECK(ok, "CellWalk visits the serial walk's cells");
+    // stream operators
+    std::ostringstream os;
+    os << oct;
+    const std::string text = os.str();
+    const int64_t len = oracle_octree_format_text(&oct[0].children[0], (int64_t)oct.size(), 2, nullptr);
+    std::string want((size_t)len, ' ');
+    oracle_octree_format_text(&oct[0].children[0], (int64_t)oct.size(), 2, &want[0]);
+    CHECK(text == want, "operator<< prints the serial text");
+    std::istringstream is(text);
+    std::vector<OctNode2> back;
+    is >> back;
+    ok = !is.fail() && back.size() == oct.size();
+    for (size_t i = 0; i < oct.size() && ok; ++i)
+      for (int k = 0; k < 4; ++k) ok &= back[i].children[k] == oct[i].children[k] && back[i].leaf == oct[i].leaf;
+    CHECK(ok, "operator>> reads the octree back");
+    std::istringstream bad("3 1 2 x");
+    is.clear();
+    bad >> back;
+    CHECK(bad.fail(), "operator>> sets failbit on malformed text");
+  }
+  std::printf("PASS octree queries (FindLeaf, CellWalk) and stream operators\n");
+}
+
 int main() {
   morton_scenario();
   sort_scenario();
@@ -260,6 +352,7 @@ int main() {
   tree_scenarios();
   build_scenarios();
   karras_scenario();
+  query_scenarios();
   std::printf(failures ? "FAILED %d\n" : "ALL PASSED\n", failures);
   return failures ? 1 : 0;
 }

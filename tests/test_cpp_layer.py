@@ -26,7 +26,9 @@ def test_cpp_layer_symbols_exported():
                  "pgvd::Kernels::ComputeLocalSplits_p", "pgvd::Kernels::BinaryRadixToOctree_p",
                  "pgvd::Kernels::StreamScan_p", "pgvd::Kernels::PointsToMorton_p",
                  "pgvd::Kernels::BitPredicate", "pgvd::Kernels::DoubleCompact",
-                 "pgvd::Karras::Quantize", "pgvd::Karras::BuildOctreeInParallel"]:
+                 "pgvd::Karras::Quantize", "pgvd::Karras::BuildOctreeInParallel",
+                 "pgvd::OctreeUtils::FindLeaf", "pgvd::OctreeUtils::FindLeaves",
+                 "pgvd::OctreeUtils::CellWalk", "pgvd::OctreeUtils::CellWalks"]:
         assert name in out, name
 
 
