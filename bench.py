@@ -408,7 +408,7 @@ def bench_findleaf(args) -> int:
             "level_histogram": {int(l): int(c) for l, c in enumerate(levels) if c},
         },
         "roofline": {
-            "kernel": ("find_leaf_table_kernel + find_leaf_kernel (the octree's cells at depth 6 "
+            "kernel": ("find_leaf_table_kernel + find_leaf_kernel (the octree's cells at depth 7 "
                        "tabulated, then one lane per query: one table read, then one dependent "
                        "node read per level; timed together)"),
             "bound": "hbm",

@@ -123,10 +123,10 @@ int pgvd_points_to_morton(pgvd_ctx *ctx, const void *pts, int coord_bytes, int64
  *   status  (nullable) 0; 1 when no leaf was reached (the reference
  *           throws); 2 when a child index leaves the octree (not one the
  *           build wrote).
- * 1 <= bits <= 31.  A batch of at least 2^19 queries (fewer when bits is
- * small) first tabulates the octree's cells at a fixed depth in a 2 MB
- * buffer the context owns: such calls on one context on different streams
- * must not overlap. */
+ * 1 <= bits <= 31.  A batch of at least 2^22 queries in 3D or 2^21 in 2D
+ * (fewer when bits is small) first tabulates the octree's cells at depth 7
+ * (3D) or 10 (2D) in a 16 MB (8 MB) buffer the context owns: such calls on
+ * one context on different streams must not overlap. */
 int pgvd_find_leaf(pgvd_ctx *ctx, const int32_t *pts, int64_t q, int dim, int bits,
                    const int32_t *octree, int64_t octree_size, int32_t *parent, int32_t *octant,
                    int32_t *data, int32_t *level, uint32_t *status, pgvd_stream s);

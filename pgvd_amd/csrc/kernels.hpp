@@ -176,7 +176,10 @@ int launch_onesweep(K *keys, K *keys_alt, const uint32_t *vals_in, uint32_t *val
 // OctreeUtils::FindLeaf for q int32 grid points (see pgvd_find_leaf).  With
 // D > 0 the launch first fills `table` (8 << (D * dim) bytes) with the
 // octree's cells at depth D; find_leaf_table_depth picks D (0: no table).
-constexpr int kFindLeafTableBits = 18;
+#ifndef PGVD_FINDLEAF_TABLE_BITS
+#define PGVD_FINDLEAF_TABLE_BITS 21
+#endif
+constexpr int kFindLeafTableBits = PGVD_FINDLEAF_TABLE_BITS;
 int find_leaf_table_depth(int64_t q, int dim, int bits);
 void launch_find_leaf(const int32_t *pts, int64_t q, int dim, int bits, const int32_t *octree,
                       int64_t octree_size, void *table, int D, int32_t *parent, int32_t *octant,

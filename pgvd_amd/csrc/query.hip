@@ -9,7 +9,9 @@
 // mask), and below the top levels every lane's load falls on its own line.
 //
 // Large batches first build a table of the octree's cells at a fixed depth D
-// (2^18 cells of 8 bytes, which stay in each XCD's L2): entry e is where the
+// (2^21 cells in 3D, 2^20 in 2D, 8 bytes each: 16 MB, which the Infinity
+// Cache holds; 2^18 cells, which L2 holds, left more of the descent's node
+// lines to fetch and was slower): entry e is where the
 // descent stands after the D digits e, either the node to continue from or,
 // when it ended within them, its parent, octant, level and status.  A query
 // then starts with one table read instead of D levels of node reads; its

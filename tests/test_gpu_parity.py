@@ -411,16 +411,16 @@ def test_octree_capacity_is_reported(dev):
 
 # ------------------------------------------------------------ find leaf ----
 @pytest.mark.parametrize("dim,bits,n", [(2, 16, 65536), (3, 21, 1 << 20), (3, 10, 200000),
-                                        (2, 30, 50000), (2, 16, 1 << 19), (3, 5, 100000),
-                                        (2, 2, 40)])
+                                        (2, 30, 50000), (2, 16, 1 << 21), (3, 21, 1 << 22),
+                                        (3, 5, 100000), (2, 2, 40)])
 def test_find_leaf_matches_oracle(dev, dim, bits, n):
     """OctreeUtils::FindLeaf batched on the GPU against the oracle's
     restatement (pinned by brute force), on the octree the GPU built: the
     build's own points and random grid points, every output field.  Batches
-    of at least twice the cell count at the table depth (2^18 cells, fewer
-    when bits * dim is below 18) start from the depth table: here 3D 21 bits,
-    2D 16 bits at 2^19 points, and 3D 5 bits and 2D 2 bits, where the table
-    covers every level."""
+    of at least twice the cell count at the table depth (2^21 cells in 3D,
+    2^20 in 2D, fewer when bits is small) start from the depth table: here
+    3D 21 bits at 2^22 points, 2D 16 bits at 2^21 points, and 3D 5 bits and
+    2D 2 bits, where the table covers every level."""
     rng = np.random.default_rng(dim * 31 + bits)
     pts = _pts(n, dim, seed=dim * 7 + bits)
     tree = dev.build_octree(pts, bits)
