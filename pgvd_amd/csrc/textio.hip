@@ -8,7 +8,7 @@
 //   format: a workgroup takes 256 nodes, sums their text lengths, scans them
 //           in the workgroup and across tiles, prints its nodes into LDS and
 //           copies the run out at its offset;
-//   parse:  a workgroup stages 4096 bytes in LDS, marks where tokens start
+//   parse:  a workgroup stages 8192 bytes in LDS, marks where tokens start
 //           (a non-space after a space), scans the counts to token numbers,
 //           parses every token from LDS and writes the tile's records
 //           k - 1 as one run (token 0 is N, read by the host).
@@ -26,7 +26,11 @@ constexpr int kWsCounter = 0, kWsTotal = 1, kWsBad = 2, kWsStatus = 4;
 constexpr int kFmtBlock = 256;        // nodes per format tile
 constexpr int kMaxNodeText = 9 * 12;  // 3D node: 9 values of at most 11 chars + space
 constexpr int kParseBytes = 16;       // bytes per thread in a parse tile
-constexpr int kParseTile = kBlock * kParseBytes;
+#ifndef PGVD_PARSE_BLOCK
+#define PGVD_PARSE_BLOCK 512
+#endif
+constexpr int kParseBlock = PGVD_PARSE_BLOCK;   // threads per parse workgroup
+constexpr int kParseTile = kParseBlock * kParseBytes;
 
 __device__ inline int dec_len(int32_t v) {
   uint32_t u = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
@@ -157,14 +161,14 @@ __device__ inline bool parse_staged(const uint8_t *s, int p, int32_t *out) {
 // the tile is decided within 12 bytes (sign, 10 digits, one more byte).  Its
 // values go to LDS in token order and out as one coalesced run of records.
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void parse_text_kernel(
+__global__ __launch_bounds__(kParseBlock) void parse_text_kernel(
     const uint8_t *__restrict__ text, int64_t len, int64_t N, int32_t *__restrict__ octree,
     uint64_t *__restrict__ ws, int ntiles) {
   constexpr int STRIDE = (1 << DIM) + 1;
   constexpr int kHalo = 16;
   __shared__ __align__(16) uint8_t s_txt[kParseTile + kHalo];
   __shared__ int32_t s_val[kParseTile / 2];
-  __shared__ uint32_t s_w[kBlock / 64];
+  __shared__ uint32_t s_w[kParseBlock / 64];
   __shared__ int s_tile;
   const int t = threadIdx.x;
   if (t == 0) s_tile = (int)atomicAdd(reinterpret_cast<unsigned int *>(ws + kWsCounter), 1u);
@@ -198,8 +202,9 @@ __global__ __launch_bounds__(kBlock) void parse_text_kernel(
     prev_space = sp;
   }
   uint32_t total;
-  const uint32_t ex = block_excl_sum<uint32_t>((uint32_t)__popc(starts), s_w, &total);
-  const uint64_t before = tile_prefix<kBlock>(ws + kWsStatus, tile, total);
+  const uint32_t ex =
+      block_excl_sum<uint32_t, kParseBlock / 64>((uint32_t)__popc(starts), s_w, &total);
+  const uint64_t before = tile_prefix<kParseBlock>(ws + kWsStatus, tile, total);
   if (tile == ntiles - 1 && t == 0) ws[kWsTotal] = before + total;
   const uint64_t last = (uint64_t)N * STRIDE;   // tokens 1..last are the records
   uint32_t li = ex;
@@ -216,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void parse_text_kernel(
     ++li;
   }
   __syncthreads();
-  for (uint32_t i = t; i < total; i += kBlock) {
+  for (uint32_t i = t; i < total; i += kParseBlock) {
     const uint64_t k = before + i;
     if (k >= 1 && k <= last) {
       const uint64_t r = k - 1;
@@ -253,9 +258,11 @@ void launch_parse_text(const uint8_t *text, int64_t len, int dim, int64_t N, int
   ws_zero_kernel<<<(unsigned)std::min<int64_t>((words + kBlock - 1) / kBlock, 1024), kBlock, 0,
                    s>>>(ws, words);
   if (dim == 2)
-    parse_text_kernel<2><<<(unsigned)tiles, kBlock, 0, s>>>(text, len, N, octree, ws, (int)tiles);
+    parse_text_kernel<2><<<(unsigned)tiles, kParseBlock, 0, s>>>(text, len, N, octree, ws,
+                                                                 (int)tiles);
   else
-    parse_text_kernel<3><<<(unsigned)tiles, kBlock, 0, s>>>(text, len, N, octree, ws, (int)tiles);
+    parse_text_kernel<3><<<(unsigned)tiles, kParseBlock, 0, s>>>(text, len, N, octree, ws,
+                                                                 (int)tiles);
 }
 
 }  // namespace pgvd
