@@ -3,6 +3,7 @@ reference's golden fixtures.  Integer, key, index and tree outputs must be
 bit-exact; there is no floating-point output on this path."""
 from __future__ import annotations
 
+import ctypes as C
 import hashlib
 import os
 
@@ -538,3 +539,42 @@ def test_cell_walk_stops_after_10000_cells(dev):
     want = oracle.cell_walk(segs, bits, tree["children"], tree["leaf"])
     _assert_walk_equal(got, want, "diagonal")
     assert list(want["status"]) == [1, 0] and want["offsets"][1] == 10000
+
+
+def test_query_entry_points_reject_bad_arguments(dev):
+    """Argument checks of the octree consumers: each bad call returns
+    PGVD_ERR_INVALID (1) or PGVD_ERR_CAPACITY (7) with the size it needs,
+    and launches nothing."""
+    L = P.lib()
+    tree = dev.build_octree(_pts(1000, 2, seed=5), 10)
+    rec = np.concatenate([tree["children"], tree["leaf"][:, None]], axis=1).astype(np.int32)
+    d = dev.upload(rec)
+    q = dev.upload(np.zeros((4, 2), np.int32))
+    outs = [dev.alloc(16) for _ in range(5)]
+    O = rec.shape[0]
+    ptrs = [o.ptr for o in outs]
+    assert L.pgvd_find_leaf(dev.ctx, q.ptr, 4, 4, 10, d.ptr, O, *ptrs, None) == 1      # dim
+    assert L.pgvd_find_leaf(dev.ctx, q.ptr, 4, 2, 0, d.ptr, O, *ptrs, None) == 1       # bits
+    assert L.pgvd_find_leaf(dev.ctx, q.ptr, 4, 2, 10, d.ptr, 0, *ptrs, None) == 1      # empty octree
+    assert L.pgvd_find_leaf(dev.ctx, q.ptr, 4, 2, 10, None, O, *ptrs, None) == 1
+    segs = dev.upload(np.array([[1, 1, 900, 700]], np.float32))
+    off = dev.alloc(16)
+    tot = C.c_int64()
+    assert L.pgvd_cell_walk(dev.ctx, segs.ptr, 1, 31, d.ptr, O, off.ptr, None, 0, C.byref(tot),
+                            None, None) == 1                                         # bits
+    assert L.pgvd_cell_walk(dev.ctx, segs.ptr, 1, 10, d.ptr, O, off.ptr, None, 0, None, None,
+                            None) == 1                                               # total_out
+    assert L.pgvd_cell_walk(dev.ctx, segs.ptr, 1, 10, d.ptr, O, off.ptr, None, 0, C.byref(tot),
+                            None, None) == 0 and tot.value > 1                       # count only
+    cells = dev.alloc(20)
+    assert L.pgvd_cell_walk(dev.ctx, segs.ptr, 1, 10, d.ptr, O, off.ptr, cells.ptr, 1,
+                            C.byref(tot), None, None) == 7 and tot.value > 1         # capacity
+    ln = C.c_int64()
+    small = dev.alloc(8)
+    assert L.pgvd_octree_format_text(dev.ctx, d.ptr, O, 2, small.ptr, 8, C.byref(ln), None) == 7
+    assert ln.value == len(oracle.octree_format_text(rec, 2))
+    assert L.pgvd_octree_format_text(dev.ctx, d.ptr, O, 5, None, 0, C.byref(ln), None) == 1
+    assert L.pgvd_octree_save(dev.ctx, d.ptr, O, 2, b"/nonexistent-dir/x.txt", 0) == 1
+    assert L.pgvd_octree_save(dev.ctx, d.ptr, O, 2, b"/tmp/x.txt", 3) == 1               # format
+    for b in [d, q, segs, off, cells, small, *outs]:
+        b.free()
