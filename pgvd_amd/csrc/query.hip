@@ -11,7 +11,8 @@
 // Large batches first build a table of the octree's cells at a fixed depth D
 // (2^21 cells in 3D, 2^20 in 2D, 8 bytes each: 16 MB, which the Infinity
 // Cache holds; 2^18 cells, which L2 holds, left more of the descent's node
-// lines to fetch and was slower): entry e is where the
+// lines to fetch and was slower, and 2^24 cells cost more to build than they
+// saved at 2^24 queries): entry e is where the
 // descent stands after the D digits e, either the node to continue from or,
 // when it ended within them, its parent, octant, level and status.  A query
 // then starts with one table read instead of D levels of node reads; its
@@ -66,7 +67,7 @@ __global__ __launch_bounds__(kBlock) void find_leaf_table_kernel(
   }
 }
 
-// The top D digits of the key (lo, hi), D * DIM <= 18, starting at bit sh.
+// The top D digits of the key (lo, hi), D * DIM <= 21, starting at bit sh.
 __device__ inline uint32_t key_bits(uint64_t lo, uint64_t hi, int sh, int width) {
   uint64_t v;
   if (sh >= 64) v = hi >> (sh - 64);
