@@ -5,9 +5,9 @@
 // space.
 //
 // Both directions are one pass with a decoupled look-back over tiles:
-//   format: a workgroup takes 256 nodes, sums their text lengths, scans them
-//           in the workgroup and across tiles, prints its nodes into LDS and
-//           copies the run out at its offset;
+//   format: a workgroup takes 4 x 256 nodes, sums their text lengths, scans
+//           them in the workgroup and across tiles, and prints each 256 into
+//           LDS and copies the run out at its offset in turn;
 //   parse:  a workgroup stages 8192 bytes in LDS, marks where tokens start
 //           (a non-space after a space), scans the counts to token numbers,
 //           parses every token from LDS and writes the tile's records
@@ -67,34 +67,49 @@ __global__ void ws_zero_kernel(uint64_t *ws, int64_t words) {
     ws[i] = i == kWsBad ? ~0ull : 0ull;
 }
 
+#ifndef PGVD_FMT_ROUNDS
+#define PGVD_FMT_ROUNDS 4
+#endif
+constexpr int kFmtRounds = PGVD_FMT_ROUNDS;   // 256-node rounds per tile
+static_assert(kFmtRounds >= 1 && kFmtRounds <= 4, "round lengths are packed in 16-bit fields");
+
+// A tile is kFmtRounds rounds of 256 nodes with one look-back: the rounds'
+// text lengths are scanned together as 16-bit fields of one word (a round
+// holds at most 256 * 108 bytes, below 2^16), then each round is printed
+// into LDS and copied out in turn.
 template <int DIM>
 __global__ __launch_bounds__(kFmtBlock) void format_text_kernel(
     const int32_t *__restrict__ octree, int64_t O, int64_t N_hdr, int hdr_len,
     uint8_t *__restrict__ text, int64_t capacity, uint64_t *__restrict__ ws, int ntiles) {
   constexpr int STRIDE = (1 << DIM) + 1;
   __shared__ uint8_t s_txt[kFmtBlock * kMaxNodeText];
-  __shared__ uint32_t s_w[kFmtBlock / 64];
+  __shared__ uint64_t s_w[kFmtBlock / 64];
   __shared__ int s_tile;
   const int t = threadIdx.x;
   if (t == 0) s_tile = (int)atomicAdd(reinterpret_cast<unsigned int *>(ws + kWsCounter), 1u);
   __syncthreads();
   const int tile = s_tile;
-  const int64_t i = (int64_t)tile * kFmtBlock + t;
-  int32_t v[STRIDE];
-  uint32_t len = 0;
-  if (i < O) {
+  const int64_t first = (int64_t)tile * kFmtBlock * kFmtRounds;
+  uint64_t packed = 0;
 #pragma unroll
-    for (int k = 0; k < STRIDE; ++k) {
-      v[k] = octree[i * STRIDE + k];
-      len += (uint32_t)dec_len(v[k]) + 1u;
+  for (int r = 0; r < kFmtRounds; ++r) {
+    const int64_t i = first + r * kFmtBlock + t;
+    uint32_t len = 0;
+    if (i < O) {
+#pragma unroll
+      for (int k = 0; k < STRIDE; ++k) len += (uint32_t)dec_len(octree[i * STRIDE + k]) + 1u;
     }
+    packed |= (uint64_t)len << (16 * r);
   }
-  uint32_t total;
-  const uint32_t ex = block_excl_sum<uint32_t, kFmtBlock / 64>(len, s_w, &total);
+  uint64_t tot_packed;
+  const uint64_t ex = block_excl_sum<uint64_t, kFmtBlock / 64>(packed, s_w, &tot_packed);
+  uint32_t total = 0;
+#pragma unroll
+  for (int r = 0; r < kFmtRounds; ++r) total += (uint32_t)(tot_packed >> (16 * r)) & 0xffffu;
   const uint64_t before = tile_prefix<kFmtBlock>(ws + kWsStatus, tile, total);
   if (tile == ntiles - 1 && t == 0) ws[kWsTotal] = (uint64_t)hdr_len + before + total;
   if (!text) return;
-  const int64_t g0 = hdr_len + (int64_t)before;
+  int64_t g0 = hdr_len + (int64_t)before;
   if (g0 + (int64_t)total > capacity) return;
   if (tile == 0 && t == 0) {
     // "N " at the start: N printed from the least significant digit
@@ -106,26 +121,33 @@ __global__ __launch_bounds__(kFmtBlock) void format_text_kernel(
       u /= 10u;
     } while (u);
   }
-  if (i < O) {
-    int at = (int)ex;
 #pragma unroll
-    for (int k = 0; k < STRIDE; ++k) at = put_dec(s_txt, at, v[k]);
+  for (int r = 0; r < kFmtRounds; ++r) {
+    const uint32_t rtotal = (uint32_t)(tot_packed >> (16 * r)) & 0xffffu;
+    const int64_t i = first + r * kFmtBlock + t;
+    if (r > 0) __syncthreads();   // the previous round's copy has read s_txt
+    if (i < O) {
+      int at = (int)((ex >> (16 * r)) & 0xffffu);
+#pragma unroll
+      for (int k = 0; k < STRIDE; ++k) at = put_dec(s_txt, at, octree[i * STRIDE + k]);
+    }
+    __syncthreads();
+    // the round's run: bytes up to the first 4-byte boundary, whole words, tail
+    uint8_t *dst = text + g0;
+    int head = (int)((4 - ((uintptr_t)dst & 3)) & 3);
+    if (head > (int)rtotal) head = (int)rtotal;
+    if (t < head) dst[t] = s_txt[t];
+    const int nw = ((int)rtotal - head) >> 2;
+    uint32_t *dw = reinterpret_cast<uint32_t *>(dst + head);
+    for (int w = t; w < nw; w += kFmtBlock) {
+      const int b = head + 4 * w;
+      dw[w] = (uint32_t)s_txt[b] | ((uint32_t)s_txt[b + 1] << 8) |
+              ((uint32_t)s_txt[b + 2] << 16) | ((uint32_t)s_txt[b + 3] << 24);
+    }
+    const int tail0 = head + 4 * nw;
+    if (t < (int)rtotal - tail0) dst[tail0 + t] = s_txt[tail0 + t];
+    g0 += rtotal;
   }
-  __syncthreads();
-  // the tile's run: bytes up to the first 4-byte boundary, whole words, tail
-  uint8_t *dst = text + g0;
-  int head = (int)((4 - ((uintptr_t)dst & 3)) & 3);
-  if (head > (int)total) head = (int)total;
-  if (t < head) dst[t] = s_txt[t];
-  const int nw = ((int)total - head) >> 2;
-  uint32_t *dw = reinterpret_cast<uint32_t *>(dst + head);
-  for (int w = t; w < nw; w += kFmtBlock) {
-    const int b = head + 4 * w;
-    dw[w] = (uint32_t)s_txt[b] | ((uint32_t)s_txt[b + 1] << 8) | ((uint32_t)s_txt[b + 2] << 16) |
-            ((uint32_t)s_txt[b + 3] << 24);
-  }
-  const int tail0 = head + 4 * nw;
-  if (t < (int)total - tail0) dst[tail0 + t] = s_txt[tail0 + t];
 }
 
 __device__ inline bool is_space(uint8_t c) {
@@ -234,7 +256,9 @@ __global__ __launch_bounds__(kParseBlock) void parse_text_kernel(
 }  // namespace
 
 int64_t text_ws_words(int64_t tiles) { return kWsStatus + tiles; }
-int64_t format_text_tiles(int64_t O) { return (O + kFmtBlock - 1) / kFmtBlock; }
+int64_t format_text_tiles(int64_t O) {
+  return (O + kFmtBlock * kFmtRounds - 1) / (kFmtBlock * kFmtRounds);
+}
 int64_t parse_text_tiles(int64_t len) { return (len + kParseTile - 1) / kParseTile; }
 
 void launch_format_text(const int32_t *octree, int64_t O, int dim, int hdr_len, uint8_t *text,
