@@ -150,8 +150,9 @@ __global__ __launch_bounds__(kFmtBlock) void format_text_kernel(
   }
 }
 
+// ' ', '\t', '\n', '\v', '\f', '\r' (codes 32 and 9..13) as one mask test
 __device__ inline bool is_space(uint8_t c) {
-  return c == ' ' || c == '\n' || c == '\t' || c == '\r' || c == '\v' || c == '\f';
+  return c <= 32 && ((0x100003E00ull >> c) & 1ull);
 }
 
 // Token parse from the staged bytes: an optional sign and 1..10 digits,
