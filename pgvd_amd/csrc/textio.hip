@@ -218,9 +218,10 @@ __global__ __launch_bounds__(kParseBlock) void parse_text_kernel(
   __syncthreads();
   bool prev_space = t == 0 ? prev0 : is_space(s_txt[t * kParseBytes - 1]);
   uint32_t starts = 0;
+  const uint32_t qw[4] = {q.x, q.y, q.z, q.w};   // this thread's bytes, still in registers
 #pragma unroll
   for (int k = 0; k < kParseBytes; ++k) {
-    const bool sp = is_space(s_txt[t * kParseBytes + k]);
+    const bool sp = is_space((uint8_t)(qw[k >> 2] >> (8 * (k & 3))));
     if (!sp && prev_space) starts |= 1u << k;
     prev_space = sp;
   }
