@@ -413,6 +413,7 @@ def test_octree_capacity_is_reported(dev):
 # ------------------------------------------------------------ find leaf ----
 @pytest.mark.parametrize("dim,bits,n", [(2, 16, 65536), (3, 21, 1 << 20), (3, 10, 200000),
                                         (2, 30, 50000), (2, 16, 1 << 21), (3, 21, 1 << 22),
+                                        (3, 23, 1 << 22), (3, 31, 1 << 22), (2, 31, 1 << 21),
                                         (3, 5, 100000), (2, 2, 40)])
 def test_find_leaf_matches_oracle(dev, dim, bits, n):
     """OctreeUtils::FindLeaf batched on the GPU against the oracle's
@@ -421,7 +422,9 @@ def test_find_leaf_matches_oracle(dev, dim, bits, n):
     of at least twice the cell count at the table depth (2^21 cells in 3D,
     2^20 in 2D, fewer when bits is small) start from the depth table: here
     3D 21 bits at 2^22 points, 2D 16 bits at 2^21 points, and 3D 5 bits and
-    2D 2 bits, where the table covers every level."""
+    2D 2 bits, where the table covers every level; 3D 23 bits (69-bit keys:
+    the table index straddles the two key words) and 31 bits (93-bit keys:
+    the index in the high word)."""
     rng = np.random.default_rng(dim * 31 + bits)
     pts = _pts(n, dim, seed=dim * 7 + bits)
     tree = dev.build_octree(pts, bits)
@@ -486,7 +489,8 @@ def _obj2oct(poly, lo, bbw, width):
 
 
 @pytest.mark.parametrize("bits,npts,nseg,reach", [(10, 400, 3000, 1.0), (16, 65536, 20000, 0.05),
-                                                  (20, 200000, 5000, 0.01), (3, 20, 500, 1.0)])
+                                                  (20, 200000, 5000, 0.01), (3, 20, 500, 1.0),
+                                                  (30, 100000, 3000, 0.01)])
 def test_cell_walk_matches_oracle(dev, bits, npts, nseg, reach):
     """CellWalk on the GPU against the oracle's restatement (pinned by brute
     force) on a GPU-built quadtree: random segments, axis-parallel and
