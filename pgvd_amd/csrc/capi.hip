@@ -772,14 +772,21 @@ int pgvd_cell_walk(pgvd_ctx *ctx, const float *segs, int64_t n, int bits, const 
     *total_out = 0;
     return PGVD_OK;
   }
-  void *counts = nullptr, *ws = nullptr;
+  void *counts = nullptr, *ws = nullptr, *scratch = nullptr;
   CHK(pool_get(ctx, "walk_counts", 4 * static_cast<size_t>(n), &counts));
   CHK(pool_get(ctx, "walk_ws", 8 * static_cast<size_t>(walk_scan_ws_words(n)), &ws));
+  // With a cell buffer, the first walk keeps each segment's first 32 cells
+  // (up to 2 GB of scratch), so that only longer walks run twice.
+  constexpr int kLog2K = 5;
+  const size_t spec_bytes = (static_cast<size_t>(n) << kLog2K) * 20;
+  if (cells && spec_bytes <= (size_t(2) << 30))
+    CHK(pool_get(ctx, "walk_scratch", spec_bytes, &scratch));
   int64_t total = 0;
   {
     Stage t(ctx, "cell_walk_count", st);
     launch_cell_walk_count(segs, n, bits, octree, octree_size, static_cast<uint32_t *>(counts),
-                           status, offsets, static_cast<uint64_t *>(ws), st);
+                           status, offsets, static_cast<uint64_t *>(ws),
+                           static_cast<int32_t *>(scratch), kLog2K, st);
     CHK(launch_status("cell_walk_count"));
   }
   HIPCHK(hipMemcpyAsync(&total, offsets + n, 8, hipMemcpyDeviceToHost, st));
@@ -788,7 +795,9 @@ int pgvd_cell_walk(pgvd_ctx *ctx, const float *segs, int64_t n, int bits, const 
   if (!cells) return PGVD_OK;
   if (total > capacity) return fail(PGVD_ERR_CAPACITY, "cell buffer too small (*total_out cells needed)");
   Stage t(ctx, "cell_walk", st);
-  launch_cell_walk_write(segs, n, bits, octree, octree_size, offsets, cells, st);
+  launch_cell_walk_write(segs, n, bits, octree, octree_size, offsets, cells,
+                         static_cast<const int32_t *>(scratch), static_cast<uint32_t *>(counts),
+                         kLog2K, st);
   return launch_status("cell_walk");
 }
 

@@ -189,13 +189,17 @@ void launch_find_leaf(const int32_t *pts, int64_t q, int dim, int bits, const in
 // CellWalk (viewer/Octree2.cpp:265-313) for n 2D segments: the count pass
 // writes counts (n), status (nullable) and offsets (n + 1, exclusive scan);
 // ws holds walk_scan_ws_words(n) words.  The write pass puts each segment's
-// cells (5 int32 each) at cells + 5 * offsets[i].
+// cells (5 int32 each) at cells + 5 * offsets[i].  With scratch (n << log2k
+// cells), the count pass also keeps each segment's first 2^log2k cells
+// there, and the write pass moves them and walks again only the segments
+// that had more.
 int64_t walk_scan_ws_words(int64_t n);
 void launch_cell_walk_count(const float *segs, int64_t n, int bits, const int32_t *octree,
                             int64_t size, uint32_t *counts, int32_t *status, int64_t *offsets,
-                            uint64_t *ws, hipStream_t s);
+                            uint64_t *ws, int32_t *scratch, int log2k, hipStream_t s);
 void launch_cell_walk_write(const float *segs, int64_t n, int bits, const int32_t *octree,
-                            int64_t size, const int64_t *offsets, int32_t *cells, hipStream_t s);
+                            int64_t size, const int64_t *offsets, int32_t *cells,
+                            const int32_t *scratch, uint32_t *counts, int log2k, hipStream_t s);
 
 // ---- textio.hip ----
 // The octree's text form (operator<< / operator>>, C/OctNode.h:58-98).  ws

@@ -172,18 +172,24 @@ __device__ inline uint32_t walk_isects(float ax, float ay, float bx, float by, i
   return keep;
 }
 
-// One pass over the segments: counts and status (cells == nullptr), or the
-// cells at offsets[i].
+// One pass over the segments.  MODE kWalkCount: counts and status;
+// kWalkWrite: the cells at offsets[i]; kWalkSpec: counts and status, and
+// the first K cells of segment i at scratch + 5 * K * i; kWalkRest: for the
+// segments with more than K cells, cells K.. at offsets[i] + K.
+enum { kWalkCount = 0, kWalkWrite = 1, kWalkSpec = 2, kWalkRest = 3 };
+
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void cell_walk_kernel(
     const float *__restrict__ segs, int64_t n, int bits, const int32_t *__restrict__ octree,
     int64_t size, uint32_t *__restrict__ counts, int32_t *__restrict__ status,
-    const int64_t *__restrict__ offsets, int32_t *__restrict__ cells) {
+    const int64_t *__restrict__ offsets, int32_t *__restrict__ cells, int K) {
   // each lane's last descent, node per depth (depth 0 is the root)
   __shared__ int32_t s_path[(kMaxBits + 1) * kBlock];
   int32_t *path = s_path + threadIdx.x;
   path[0] = 0;
   for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < n;
        s += (int64_t)gridDim.x * kBlock) {
+    if (MODE == kWalkRest && counts[s] <= (uint32_t)K) continue;
     const float ax = segs[4 * s], ay = segs[4 * s + 1], bx = segs[4 * s + 2], by = segs[4 * s + 3];
     const int dx = bx > ax ? 1 : (bx < ax ? -1 : 0);
     const int dy = by > ay ? 1 : (by < ay ? -1 : 0);
@@ -193,7 +199,9 @@ __global__ __launch_bounds__(kBlock) void cell_walk_kernel(
     int plev = 0;
     walk_leaf(bx, by, bits, octree, size, bcell);
     walk_leaf_path(cx, cy, bits, octree, size, path, pkey, plev, cell);
-    int32_t *out = cells ? cells + 5 * offsets[s] : nullptr;
+    int32_t *out = MODE == kWalkWrite || MODE == kWalkRest ? cells + 5 * offsets[s]
+                   : MODE == kWalkSpec                     ? cells + 5 * (int64_t)K * s
+                                                           : nullptr;
     int count = 0, st = 0;
     bool done = false;
     do {
@@ -202,7 +210,9 @@ __global__ __launch_bounds__(kBlock) void cell_walk_kernel(
         --count;
         break;
       }
-      if (out) {
+      const bool put = MODE == kWalkWrite || (MODE == kWalkSpec && count <= K) ||
+                       (MODE == kWalkRest && count > K);
+      if (put) {
 #pragma unroll
         for (int k = 0; k < 5; ++k) out[5 * (count - 1) + k] = cell[k];
       }
@@ -223,9 +233,28 @@ __global__ __launch_bounds__(kBlock) void cell_walk_kernel(
 #pragma unroll
       for (int k = 0; k < 5; ++k) cell[k] = ncell[k];
     } while (!done);
-    if (!cells) {
+    if (MODE == kWalkCount || MODE == kWalkSpec) {
       counts[s] = (uint32_t)count;
       if (status) status[s] = st;
+    }
+  }
+}
+
+// Moves the speculative pass's first min(count, K) cells of every segment
+// to their offsets; one thread per (segment, slot), K a power of two.
+__global__ __launch_bounds__(kBlock) void walk_compact_kernel(
+    const int32_t *__restrict__ scratch, const uint32_t *__restrict__ counts,
+    const int64_t *__restrict__ offsets, int64_t n, int log2k, int32_t *__restrict__ cells) {
+  const int64_t slots = n << log2k;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < slots;
+       i += (int64_t)gridDim.x * kBlock) {
+    const int64_t s = i >> log2k;
+    const uint32_t k = (uint32_t)(i & ((int64_t(1) << log2k) - 1));
+    if (k < counts[s]) {
+      const int32_t *src = scratch + 5 * i;
+      int32_t *dst = cells + 5 * (offsets[s] + k);
+#pragma unroll
+      for (int j = 0; j < 5; ++j) dst[j] = src[j];
     }
   }
 }
@@ -272,11 +301,15 @@ int64_t walk_scan_ws_words(int64_t n) { return 4 + (n + 4 * kBlock - 1) / (4 * k
 
 void launch_cell_walk_count(const float *segs, int64_t n, int bits, const int32_t *octree,
                             int64_t size, uint32_t *counts, int32_t *status, int64_t *offsets,
-                            uint64_t *ws, hipStream_t s) {
+                            uint64_t *ws, int32_t *scratch, int log2k, hipStream_t s) {
   int64_t grid = (n + kBlock - 1) / kBlock;
   if (grid > 8192) grid = 8192;
-  cell_walk_kernel<<<(unsigned)grid, kBlock, 0, s>>>(segs, n, bits, octree, size, counts, status,
-                                                     nullptr, nullptr);
+  if (scratch)
+    cell_walk_kernel<kWalkSpec><<<(unsigned)grid, kBlock, 0, s>>>(
+        segs, n, bits, octree, size, counts, status, nullptr, scratch, 1 << log2k);
+  else
+    cell_walk_kernel<kWalkCount><<<(unsigned)grid, kBlock, 0, s>>>(
+        segs, n, bits, octree, size, counts, status, nullptr, nullptr, 0);
   const int64_t words = walk_scan_ws_words(n);
   walk_ws_zero_kernel<<<(unsigned)std::min<int64_t>((words + kBlock - 1) / kBlock, 1024), kBlock,
                         0, s>>>(ws, words);
@@ -285,11 +318,21 @@ void launch_cell_walk_count(const float *segs, int64_t n, int bits, const int32_
 }
 
 void launch_cell_walk_write(const float *segs, int64_t n, int bits, const int32_t *octree,
-                            int64_t size, const int64_t *offsets, int32_t *cells, hipStream_t s) {
+                            int64_t size, const int64_t *offsets, int32_t *cells,
+                            const int32_t *scratch, uint32_t *counts, int log2k, hipStream_t s) {
   int64_t grid = (n + kBlock - 1) / kBlock;
   if (grid > 8192) grid = 8192;
-  cell_walk_kernel<<<(unsigned)grid, kBlock, 0, s>>>(segs, n, bits, octree, size, nullptr,
-                                                     nullptr, offsets, cells);
+  if (!scratch) {
+    cell_walk_kernel<kWalkWrite><<<(unsigned)grid, kBlock, 0, s>>>(
+        segs, n, bits, octree, size, nullptr, nullptr, offsets, cells, 0);
+    return;
+  }
+  int64_t cgrid = ((n << log2k) + kBlock - 1) / kBlock;
+  if (cgrid > 16384) cgrid = 16384;
+  walk_compact_kernel<<<(unsigned)cgrid, kBlock, 0, s>>>(scratch, counts, offsets, n, log2k,
+                                                          cells);
+  cell_walk_kernel<kWalkRest><<<(unsigned)grid, kBlock, 0, s>>>(
+      segs, n, bits, octree, size, counts, nullptr, offsets, cells, 1 << log2k);
 }
 
 }  // namespace pgvd
