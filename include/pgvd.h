@@ -145,11 +145,13 @@ int pgvd_find_leaf(pgvd_ctx *ctx, const int32_t *pts, int64_t q, int dim, int bi
  *   status   (nullable) n int32: 0, or 1 when a walk passed 10000 cells
  *            (the reference throws; the first 10000 are kept).
  * *total_out (host) receives the number of cells; PGVD_ERR_CAPACITY when
- * it exceeds capacity (offsets and status are then valid).  Runs the walk
- * twice (count, then write) and returns after the count is read back; the
- * write is asynchronous on the stream.  1 <= bits <= 30.  Workspace
- * buffers belong to the context: calls on one context must not overlap on
- * different streams. */
+ * it exceeds capacity (offsets and status are then valid).  With a cell
+ * buffer, one walk counts and keeps each segment's first 32 cells in a
+ * scratch buffer the context owns (up to 2 GB; past that, a count walk and
+ * a write walk), and only segments with more cells walk again.  Returns
+ * after the count is read back; the rest is asynchronous on the stream.
+ * 1 <= bits <= 30.  Workspace buffers belong to the context: calls on one
+ * context must not overlap on different streams. */
 int pgvd_cell_walk(pgvd_ctx *ctx, const float *segs, int64_t n, int bits, const int32_t *octree,
                    int64_t octree_size, int64_t *offsets, int32_t *cells, int64_t capacity,
                    int64_t *total_out, int32_t *status, pgvd_stream s);
