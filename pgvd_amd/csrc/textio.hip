@@ -8,7 +8,7 @@
 //   format: a workgroup takes 4 x 256 nodes, sums their text lengths, scans
 //           them in the workgroup and across tiles, and prints each 256 into
 //           LDS and copies the run out at its offset in turn;
-//   parse:  a workgroup stages 8192 bytes in LDS, marks where tokens start
+//   parse:  a workgroup takes 4 x 8192 bytes, marks where tokens start
 //           (a non-space after a space), scans the counts to token numbers,
 //           parses every token from LDS and writes the tile's records
 //           k - 1 as one run (token 0 is N, read by the host).
@@ -30,7 +30,12 @@ constexpr int kParseBytes = 16;       // bytes per thread in a parse tile
 #define PGVD_PARSE_BLOCK 512
 #endif
 constexpr int kParseBlock = PGVD_PARSE_BLOCK;   // threads per parse workgroup
-constexpr int kParseTile = kParseBlock * kParseBytes;
+constexpr int kParseTile = kParseBlock * kParseBytes;   // bytes per round
+#ifndef PGVD_PARSE_ROUNDS
+#define PGVD_PARSE_ROUNDS 4
+#endif
+constexpr int kParseRounds = PGVD_PARSE_ROUNDS;          // rounds per tile
+static_assert(kParseRounds >= 1 && kParseRounds <= 4, "start counts are packed in 16-bit fields");
 
 __device__ inline int dec_len(int32_t v) {
   uint32_t u = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
@@ -180,9 +185,12 @@ __device__ inline bool parse_staged(const uint8_t *s, int p, int32_t *out) {
   return true;
 }
 
-// A tile's bytes and the 16 after it are staged in LDS: a token starting in
-// the tile is decided within 12 bytes (sign, 10 digits, one more byte).  Its
-// values go to LDS in token order and out as one coalesced run of records.
+// A tile is kParseRounds rounds of kParseTile bytes with one look-back: the
+// rounds' token-start counts are scanned together as 16-bit fields of one
+// word (a round holds at most kParseTile / 2 starts), then each round's
+// bytes and the 16 after it are staged in LDS (a token starting in the round
+// is decided within 12 bytes: sign, 10 digits, one more byte), its tokens
+// parsed into LDS in token order and written out as one run of records.
 template <int DIM>
 __global__ __launch_bounds__(kParseBlock) void parse_text_kernel(
     const uint8_t *__restrict__ text, int64_t len, int64_t N, int32_t *__restrict__ octree,
@@ -191,67 +199,84 @@ __global__ __launch_bounds__(kParseBlock) void parse_text_kernel(
   constexpr int kHalo = 16;
   __shared__ __align__(16) uint8_t s_txt[kParseTile + kHalo];
   __shared__ int32_t s_val[kParseTile / 2];
-  __shared__ uint32_t s_w[kParseBlock / 64];
+  __shared__ uint64_t s_w[kParseBlock / 64];
   __shared__ int s_tile;
   const int t = threadIdx.x;
   if (t == 0) s_tile = (int)atomicAdd(reinterpret_cast<unsigned int *>(ws + kWsCounter), 1u);
   __syncthreads();
   const int tile = s_tile;
-  const int64_t tb = (int64_t)tile * kParseTile;
-  const int64_t p0 = tb + (int64_t)t * kParseBytes;
-  uint4 q;
-  if (p0 + kParseBytes <= len && (((uintptr_t)(text + p0)) & 15) == 0) {
-    q = *reinterpret_cast<const uint4 *>(text + p0);
-  } else {
-    uint32_t wv[4] = {0, 0, 0, 0};
+  const int64_t tb = (int64_t)tile * kParseTile * kParseRounds;
+  uint4 qs[kParseRounds];          // this thread's bytes of every round
+  uint32_t smask[kParseRounds];    // and where tokens start in them
+  uint64_t packed = 0;
 #pragma unroll
-    for (int k = 0; k < kParseBytes; ++k)
-      wv[k >> 2] |= (uint32_t)(p0 + k < len ? text[p0 + k] : (uint8_t)' ') << (8 * (k & 3));
-    q = make_uint4(wv[0], wv[1], wv[2], wv[3]);
-  }
-  *reinterpret_cast<uint4 *>(s_txt + t * kParseBytes) = q;
-  if (t < kHalo) {
-    const int64_t p = tb + kParseTile + t;
-    s_txt[kParseTile + t] = p < len ? text[p] : (uint8_t)' ';
-  }
-  const bool prev0 = tb == 0 || tb > len || is_space(text[tb - 1]);
-  __syncthreads();
-  bool prev_space = t == 0 ? prev0 : is_space(s_txt[t * kParseBytes - 1]);
-  uint32_t starts = 0;
-  const uint32_t qw[4] = {q.x, q.y, q.z, q.w};   // this thread's bytes, still in registers
+  for (int r = 0; r < kParseRounds; ++r) {
+    const int64_t p0 = tb + (int64_t)r * kParseTile + (int64_t)t * kParseBytes;
+    if (p0 + kParseBytes <= len && (((uintptr_t)(text + p0)) & 15) == 0) {
+      qs[r] = *reinterpret_cast<const uint4 *>(text + p0);
+    } else {
+      uint32_t wv[4] = {0, 0, 0, 0};
 #pragma unroll
-  for (int k = 0; k < kParseBytes; ++k) {
-    const bool sp = is_space((uint8_t)(qw[k >> 2] >> (8 * (k & 3))));
-    if (!sp && prev_space) starts |= 1u << k;
-    prev_space = sp;
+      for (int k = 0; k < kParseBytes; ++k)
+        wv[k >> 2] |= (uint32_t)(p0 + k < len ? text[p0 + k] : (uint8_t)' ') << (8 * (k & 3));
+      qs[r] = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+    }
+    bool prev_space = p0 == 0 || p0 > len || is_space(text[p0 - 1]);
+    const uint32_t qw[4] = {qs[r].x, qs[r].y, qs[r].z, qs[r].w};
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < kParseBytes; ++k) {
+      const bool sp = is_space((uint8_t)(qw[k >> 2] >> (8 * (k & 3))));
+      if (!sp && prev_space) m |= 1u << k;
+      prev_space = sp;
+    }
+    smask[r] = m;
+    packed |= (uint64_t)__popc(m) << (16 * r);
   }
-  uint32_t total;
-  const uint32_t ex =
-      block_excl_sum<uint32_t, kParseBlock / 64>((uint32_t)__popc(starts), s_w, &total);
+  uint64_t tot_packed;
+  const uint64_t ex = block_excl_sum<uint64_t, kParseBlock / 64>(packed, s_w, &tot_packed);
+  uint32_t total = 0;
+#pragma unroll
+  for (int r = 0; r < kParseRounds; ++r) total += (uint32_t)(tot_packed >> (16 * r)) & 0xffffu;
   const uint64_t before = tile_prefix<kParseBlock>(ws + kWsStatus, tile, total);
   if (tile == ntiles - 1 && t == 0) ws[kWsTotal] = before + total;
   const uint64_t last = (uint64_t)N * STRIDE;   // tokens 1..last are the records
-  uint32_t li = ex;
-  while (starts) {
-    const int j = __ffs(starts) - 1;
-    starts &= starts - 1;
-    const uint64_t k = before + li;
-    if (k >= 1 && k <= last) {
-      int32_t val = 0;
-      if (!parse_staged(s_txt, t * kParseBytes + j, &val))
-        atomicMin(reinterpret_cast<unsigned long long *>(ws + kWsBad), (unsigned long long)k + 1);
-      s_val[li] = val;
+  uint64_t base = before;
+#pragma unroll
+  for (int r = 0; r < kParseRounds; ++r) {
+    const uint32_t rtotal = (uint32_t)(tot_packed >> (16 * r)) & 0xffffu;
+    const int64_t rb = tb + (int64_t)r * kParseTile;
+    if (r > 0) __syncthreads();   // the previous round is done with s_txt and s_val
+    *reinterpret_cast<uint4 *>(s_txt + t * kParseBytes) = qs[r];
+    if (t < kHalo) {
+      const int64_t p = rb + kParseTile + t;
+      s_txt[kParseTile + t] = p < len ? text[p] : (uint8_t)' ';
     }
-    ++li;
-  }
-  __syncthreads();
-  for (uint32_t i = t; i < total; i += kParseBlock) {
-    const uint64_t k = before + i;
-    if (k >= 1 && k <= last) {
-      const uint64_t r = k - 1;
-      const int32_t v = s_val[i];
-      octree[r] = (r % STRIDE == STRIDE - 1) ? (int32_t)(uint8_t)v : v;
+    __syncthreads();
+    uint32_t starts = smask[r];
+    uint32_t li = (uint32_t)(ex >> (16 * r)) & 0xffffu;
+    while (starts) {
+      const int j = __ffs(starts) - 1;
+      starts &= starts - 1;
+      const uint64_t k = base + li;
+      if (k >= 1 && k <= last) {
+        int32_t val = 0;
+        if (!parse_staged(s_txt, t * kParseBytes + j, &val))
+          atomicMin(reinterpret_cast<unsigned long long *>(ws + kWsBad), (unsigned long long)k + 1);
+        s_val[li] = val;
+      }
+      ++li;
     }
+    __syncthreads();
+    for (uint32_t i = t; i < rtotal; i += kParseBlock) {
+      const uint64_t k = base + i;
+      if (k >= 1 && k <= last) {
+        const uint64_t rr = k - 1;
+        const int32_t v = s_val[i];
+        octree[rr] = (rr % STRIDE == STRIDE - 1) ? (int32_t)(uint8_t)v : v;
+      }
+    }
+    base += rtotal;
   }
 }
 
@@ -261,7 +286,9 @@ int64_t text_ws_words(int64_t tiles) { return kWsStatus + tiles; }
 int64_t format_text_tiles(int64_t O) {
   return (O + kFmtBlock * kFmtRounds - 1) / (kFmtBlock * kFmtRounds);
 }
-int64_t parse_text_tiles(int64_t len) { return (len + kParseTile - 1) / kParseTile; }
+int64_t parse_text_tiles(int64_t len) {
+  return (len + (int64_t)kParseTile * kParseRounds - 1) / ((int64_t)kParseTile * kParseRounds);
+}
 
 void launch_format_text(const int32_t *octree, int64_t O, int dim, int hdr_len, uint8_t *text,
                         int64_t capacity, uint64_t *ws, hipStream_t s) {
