@@ -96,15 +96,18 @@ __global__ __launch_bounds__(kFmtBlock) void format_text_kernel(
   const int tile = s_tile;
   const int64_t first = (int64_t)tile * kFmtBlock * kFmtRounds;
   uint64_t packed = 0;
+  int32_t cur[STRIDE];   // the records of the round being printed (round 0 kept from here)
 #pragma unroll
   for (int r = 0; r < kFmtRounds; ++r) {
     const int64_t i = first + r * kFmtBlock + t;
     uint32_t len = 0;
-    if (i < O) {
 #pragma unroll
-      for (int k = 0; k < STRIDE; ++k) len += (uint32_t)dec_len(octree[i * STRIDE + k]) + 1u;
+    for (int k = 0; k < STRIDE; ++k) {
+      const int32_t x = i < O ? octree[i * STRIDE + k] : 0;
+      if (r == 0) cur[k] = x;
+      len += (uint32_t)dec_len(x) + 1u;
     }
-    packed |= (uint64_t)len << (16 * r);
+    packed |= (uint64_t)(i < O ? len : 0u) << (16 * r);
   }
   uint64_t tot_packed;
   const uint64_t ex = block_excl_sum<uint64_t, kFmtBlock / 64>(packed, s_w, &tot_packed);
@@ -130,11 +133,17 @@ __global__ __launch_bounds__(kFmtBlock) void format_text_kernel(
   for (int r = 0; r < kFmtRounds; ++r) {
     const uint32_t rtotal = (uint32_t)(tot_packed >> (16 * r)) & 0xffffu;
     const int64_t i = first + r * kFmtBlock + t;
+    // the next round's records load while this round prints
+    int32_t nxt[STRIDE];
+    const int64_t in = i + kFmtBlock;
+#pragma unroll
+    for (int k = 0; k < STRIDE; ++k)
+      nxt[k] = (r + 1 < kFmtRounds && in < O) ? octree[in * STRIDE + k] : 0;
     if (r > 0) __syncthreads();   // the previous round's copy has read s_txt
     if (i < O) {
       int at = (int)((ex >> (16 * r)) & 0xffffu);
 #pragma unroll
-      for (int k = 0; k < STRIDE; ++k) at = put_dec(s_txt, at, octree[i * STRIDE + k]);
+      for (int k = 0; k < STRIDE; ++k) at = put_dec(s_txt, at, cur[k]);
     }
     __syncthreads();
     // the round's run: bytes up to the first 4-byte boundary, whole words, tail
@@ -152,6 +161,8 @@ __global__ __launch_bounds__(kFmtBlock) void format_text_kernel(
     const int tail0 = head + 4 * nw;
     if (t < (int)rtotal - tail0) dst[tail0 + t] = s_txt[tail0 + t];
     g0 += rtotal;
+#pragma unroll
+    for (int k = 0; k < STRIDE; ++k) cur[k] = nxt[k];
   }
 }
 
