@@ -219,6 +219,7 @@ __global__ __launch_bounds__(kParseBlock) void parse_text_kernel(
   const int64_t tb = (int64_t)tile * kParseTile * kParseRounds;
   uint4 qs[kParseRounds];          // this thread's bytes of every round
   uint32_t smask[kParseRounds];    // and where tokens start in them
+  uint8_t halo[kParseRounds];      // threads below kHalo: byte t after each round
   uint64_t packed = 0;
 #pragma unroll
   for (int r = 0; r < kParseRounds; ++r) {
@@ -243,6 +244,8 @@ __global__ __launch_bounds__(kParseBlock) void parse_text_kernel(
     }
     smask[r] = m;
     packed |= (uint64_t)__popc(m) << (16 * r);
+    const int64_t ph = tb + (int64_t)(r + 1) * kParseTile + t;
+    halo[r] = (t < kHalo && ph < len) ? text[ph] : (uint8_t)' ';
   }
   uint64_t tot_packed;
   const uint64_t ex = block_excl_sum<uint64_t, kParseBlock / 64>(packed, s_w, &tot_packed);
@@ -256,13 +259,9 @@ __global__ __launch_bounds__(kParseBlock) void parse_text_kernel(
 #pragma unroll
   for (int r = 0; r < kParseRounds; ++r) {
     const uint32_t rtotal = (uint32_t)(tot_packed >> (16 * r)) & 0xffffu;
-    const int64_t rb = tb + (int64_t)r * kParseTile;
     if (r > 0) __syncthreads();   // the previous round is done with s_txt and s_val
     *reinterpret_cast<uint4 *>(s_txt + t * kParseBytes) = qs[r];
-    if (t < kHalo) {
-      const int64_t p = rb + kParseTile + t;
-      s_txt[kParseTile + t] = p < len ? text[p] : (uint8_t)' ';
-    }
+    if (t < kHalo) s_txt[kParseTile + t] = halo[r];
     __syncthreads();
     uint32_t starts = smask[r];
     uint32_t li = (uint32_t)(ex >> (16 * r)) & 0xffffu;
