@@ -320,6 +320,41 @@ int pgvd_build_octree(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bi
  * detected (PGVD_ERR_BAD_COORD, PGVD_ERR_DUP_KEY) are returned here. */
 int pgvd_build_octree_finish(pgvd_ctx *ctx, pgvd_octree_result *out);
 
+/* Allocation-free whole path.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It computes what
+ * pgvd_build_octree computes, but every intermediate (run state, keys, BRT,
+ * prefix sums, look-back words) lives in a static device workspace of
+ * PGVD_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on
+ * until the process ends (about 2^20 3D points with 64-bit keys; a larger
+ * build fails with PGVD_ERR_TOO_LARGE before anything is enqueued).  No
+ * context: it runs on the current HIP device and stream s (NULL: the null
+ * stream), calls no hipMalloc or hipFree, and returns once the work is
+ * enqueued.  Outputs (device): octree (octree_cap nodes, the records of
+ * pgvd_build_octree), node_conflict (octree_cap words, required with
+ * labels) and *state, written last on the stream:
+ *   status       pgvd_status of the build: PGVD_ERR_BAD_COORD as the whole
+ *                path reports it; PGVD_ERR_CAPACITY when the octree has more
+ *                than octree_cap nodes (then nothing was written to octree;
+ *                octree_size says how many to provide);
+ *   m            unique keys (including the padding zero key);
+ *   octree_size  octree nodes;
+ *   malformed    1 if some BRT split took the FP32-quirk value.
+ * Every call on one GPU shares that GPU's workspace, which keeps the last
+ * call's contents: calls that overlap on one GPU (other streams or threads)
+ * race and must be serialised by the caller. */
+#define PGVD_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+typedef struct pgvd_build_state {
+  int32_t status;
+  uint32_t m;
+  uint32_t octree_size;
+  uint32_t malformed;
+} pgvd_build_state;
+int pgvd_build_octree_noalloc(const void *pts, int64_t n, int dim, int bits,
+                              const uint32_t *labels, const pgvd_build_opts *opts,
+                              int32_t *octree, int64_t octree_cap, uint32_t *node_conflict,
+                              pgvd_build_state *state, pgvd_stream s);
+
 /* ---- batched form -------------------------------------------------------
  * Many independent octree builds of uneven size through one segmented
  * pipeline: one launch per stage for the whole batch, replayed as a HIP

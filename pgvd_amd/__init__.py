@@ -141,6 +141,8 @@ def lib():
             "pgvd_brt_to_octree": ([vp, vp, vp, i64, i32, i32, vp, i64, C.POINTER(i64), vp], i32),
             "pgvd_build_octree": ([vp, vp, i64, i32, i32, vp, C.POINTER(BuildOpts), vp], i32),
             "pgvd_build_octree_finish": ([vp, C.POINTER(OctreeResult)], i32),
+            "pgvd_build_octree_noalloc": ([vp, i64, i32, i32, vp, C.POINTER(BuildOpts), vp, i64,
+                                           vp, vp, vp], i32),
             "pgvd_batch_create": ([vp, C.POINTER(i64), i64, i32, i32, C.POINTER(BatchOpts),
                                    C.POINTER(vp)], i32),
             "pgvd_batch_destroy": ([vp], i32),

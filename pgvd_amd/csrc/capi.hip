@@ -36,7 +36,7 @@ struct Stage {
   hipEvent_t a = nullptr;
   const char *name;
   Stage(pgvd_ctx *c, const char *nm, hipStream_t st, int mode = 1) : ctx(c), s(st), name(nm) {
-    if (ctx->timing == mode) {
+    if (ctx && ctx->timing == mode) {
       a = take_event(ctx);
       if (a) (void)hipEventRecord(a, s);
     }
@@ -58,7 +58,7 @@ struct PassTimer {
   pgvd_ctx *ctx;
   std::vector<hipEvent_t> ev;
   PassTimer(pgvd_ctx *c, int passes) : ctx(c) {
-    if (ctx->timing != 1) return;
+    if (!ctx || ctx->timing != 1) return;
     for (int i = 0; i < 2 * passes; ++i) {
       hipEvent_t e = take_event(ctx);
       if (!e) { ev.clear(); return; }
@@ -184,44 +184,105 @@ int ensure_octree(pgvd_ctx *ctx, int dim, int64_t nodes, bool labels, int32_t **
   return PGVD_OK;
 }
 
+// The device buffers of one whole-path build.
 template <typename K>
-int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
-               const uint32_t *labels, const pgvd_build_opts *opts, hipStream_t s) {
+struct BuildBufs {
+  K *kA = nullptr, *kB = nullptr;          // sort ping-pong, 16 bytes after a front pad
+  uint32_t *vA = nullptr, *vB = nullptr;   // labels ping-pong
+  BrtNodeT<K> *brt = nullptr;
+  uint32_t *ps = nullptr;
+  int32_t *parent = nullptr;
+  uint32_t *defer = nullptr, *chunks = nullptr;
+  char *zb = nullptr;                      // the block zeroed at the build's start
+  ZeroLayout z;
+  float *bbox_part = nullptr;
+  uint32_t *msd_start = nullptr, *msd_cursor = nullptr;
+  int msd_b = 0;                           // bucket-sort bits (0: LSD passes only)
+};
+
+// What a build leaves for the caller (the unique keys, the packed local
+// splits and the conflict bits live inside the carved buffers).
+template <typename K>
+struct BuildOut {
+  K *uniq = nullptr;
+  uint32_t *lsx = nullptr;
+  uint32_t *cbits = nullptr;
+};
+
+// Takes every buffer of one build from get(key, bytes, &ptr): the
+// context's keyed pool, or the allocation-free entry's static workspace.
+template <typename K, typename Get>
+int carve_build(Get &&get, const void *pts, int64_t n, int dim, int bits, bool lab,
+                const pgvd_build_opts *opts, BuildBufs<K> &b) {
   const int mbits = dim * bits;
   const int kb = (int)sizeof(K);
   const int passes = (mbits + 7) / 8;
-  const int prepend = ref_next_pow2((int)n) != n ? 1 : 0;   // zero-key padding (Q1)
   const int64_t nk = n + 1;
-  const bool lab = labels != nullptr;
   void *p;
   // Each key buffer keeps 16 bytes before its keys: the distinct keys are
   // compacted in place from the sorted ones, one slot to the left when a
   // zero key is prepended.
   constexpr int64_t kPad = 16 / (int64_t)sizeof(K);
-  CHK(pool_get(ctx, "keys_a", (size_t)(nk + kPad) * kb, &p));
-  K *kA = static_cast<K *>(p) + kPad;
-  CHK(pool_get(ctx, "keys_b", (size_t)(nk + kPad) * kb, &p));
-  K *kB = static_cast<K *>(p) + kPad;
-  uint32_t *vA = nullptr, *vB = nullptr;
+  CHK(get("keys_a", (size_t)(nk + kPad) * kb, &p));
+  b.kA = static_cast<K *>(p) + kPad;
+  CHK(get("keys_b", (size_t)(nk + kPad) * kb, &p));
+  b.kB = static_cast<K *>(p) + kPad;
   if (lab) {
-    CHK(pool_get(ctx, "vals_a", (size_t)n * 4, &p));
-    vA = static_cast<uint32_t *>(p);
-    CHK(pool_get(ctx, "vals_b", (size_t)n * 4, &p));
-    vB = static_cast<uint32_t *>(p);
+    CHK(get("vals_a", (size_t)n * 4, &p));
+    b.vA = static_cast<uint32_t *>(p);
+    CHK(get("vals_b", (size_t)n * 4, &p));
+    b.vB = static_cast<uint32_t *>(p);
   }
-  CHK(pool_get(ctx, "brt", (size_t)nk * brt_node_bytes<K>(), &p));
-  BrtNodeT<K> *brt = static_cast<BrtNodeT<K> *>(p);
-  CHK(pool_get(ctx, "ps", (size_t)nk * 4, &p));
-  uint32_t *ps = static_cast<uint32_t *>(p);
-  CHK(pool_get(ctx, "parent", (size_t)nk * 4, &p));
-  int32_t *parent = static_cast<int32_t *>(p);
-  CHK(pool_get(ctx, "defer", (size_t)nk * 4, &p));
-  uint32_t *defer = static_cast<uint32_t *>(p);
-  CHK(pool_get(ctx, "chunks", (size_t)((nk + kBlock - 1) / kBlock) * 4, &p));
-  uint32_t *chunks = static_cast<uint32_t *>(p);
-  const ZeroLayout z = zero_layout(n, kb, passes, lab);
-  CHK(pool_get(ctx, "zero", z.total, &p));
-  char *zb = static_cast<char *>(p);
+  CHK(get("brt", (size_t)nk * brt_node_bytes<K>(), &p));
+  b.brt = static_cast<BrtNodeT<K> *>(p);
+  CHK(get("ps", (size_t)nk * 4, &p));
+  b.ps = static_cast<uint32_t *>(p);
+  CHK(get("parent", (size_t)nk * 4, &p));
+  b.parent = static_cast<int32_t *>(p);
+  CHK(get("defer", (size_t)nk * 4, &p));
+  b.defer = static_cast<uint32_t *>(p);
+  CHK(get("chunks", (size_t)((nk + kBlock - 1) / kBlock) * 4, &p));
+  b.chunks = static_cast<uint32_t *>(p);
+  b.z = zero_layout(n, kb, passes, lab);
+  CHK(get("zero", b.z.total, &p));
+  b.zb = static_cast<char *>(p);
+  const bool quantized = opts && opts->quantized_input;
+  const bool custom = opts && opts->use_custom_bb;
+  if (!quantized && !custom) {
+    CHK(get("bbox_part", kBboxPartBytes, &p));
+    b.bbox_part = static_cast<float *>(p);
+  }
+  // 64-bit keys of a large build without labels: bucket sort, with the LSD
+  // passes launched beside it for inputs whose buckets overflow
+  b.msd_b = 0;
+  if (!quantized && std::is_same<K, uint64_t>::value && !lab &&
+      encode_vectorised(static_cast<const float *>(pts), b.kA))
+    b.msd_b = msd_bits_for(n);
+  if (b.msd_b) {
+    CHK(get("msd_start", ((size_t)kMsdMaxBuckets + 1) * 4, &p));
+    b.msd_start = static_cast<uint32_t *>(p);
+    CHK(get("msd_cursor", (size_t)kMsdCursorWords * 4, &p));
+    b.msd_cursor = static_cast<uint32_t *>(p);
+  }
+  return PGVD_OK;
+}
+
+// Enqueues the whole path on s over the carved buffers b, with its run
+// state st and the octree output (capacity octree_cap nodes; node_conflict
+// the same count, with labels).  ctx (nullable) only provides timing.
+template <typename K>
+int build_launch(pgvd_ctx *ctx, const BuildBufs<K> &b, RunState *st, int32_t *octree,
+                 uint32_t *nconf, int64_t octree_cap, const void *pts, int64_t n, int dim,
+                 int bits, const uint32_t *labels, const pgvd_build_opts *opts, hipStream_t s,
+                 BuildOut<K> &out) {
+  const int mbits = dim * bits;
+  const int passes = (mbits + 7) / 8;
+  const int prepend = ref_next_pow2((int)n) != n ? 1 : 0;   // zero-key padding (Q1)
+  const int64_t nk = n + 1;
+  const bool lab = labels != nullptr;
+  K *kA = b.kA, *kB = b.kB;
+  const ZeroLayout &z = b.z;
+  char *zb = b.zb;
   uint32_t *hist = reinterpret_cast<uint32_t *>(zb + z.hist);
   uint32_t *sort_lb = reinterpret_cast<uint32_t *>(zb + z.sort_lb);
   uint64_t *lb_u = reinterpret_cast<uint64_t *>(zb + z.lb_unique);
@@ -229,32 +290,18 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   uint64_t *lb_b = reinterpret_cast<uint64_t *>(zb + z.lb_brt);
   uint32_t *cbits = lab ? reinterpret_cast<uint32_t *>(zb + z.cbits) : nullptr;
   uint32_t *msd_hist = reinterpret_cast<uint32_t *>(zb + z.msd_hist);
-  int msd_b = 0;
-  int32_t *octree;
-  uint32_t *nconf;
-  int64_t want = 2 * nk;
-  if (want < 1024) want = 1024;
-  if (ctx->octree_cap < want || ctx->octree_stride != (1 << dim) + 1 ||
-      (lab && ctx->pool["node_conflict"].bytes < (size_t)ctx->octree_cap * 4))
-    CHK(ensure_octree<K>(ctx, dim, want, lab, &octree, &nconf));
-  octree = static_cast<int32_t *>(ctx->pool["octree"].p);
-  nconf = lab ? static_cast<uint32_t *>(ctx->pool["node_conflict"].p) : nullptr;
+  const int msd_b = b.msd_b;
 
   const bool quantized = opts && opts->quantized_input;
   const bool custom = opts && opts->use_custom_bb;
   const bool clamped = opts && opts->clamped;
-  // 64-bit keys of a large build without labels: bucket sort, with the LSD
-  // passes launched beside it for inputs whose buckets overflow
-  if (!quantized && std::is_same<K, uint64_t>::value && !lab &&
-      encode_vectorised(static_cast<const float *>(pts), kA))
-    msd_b = msd_bits_for(n);
   const bool msd_first = msd_b != 0;
   // the build's start (state reset, workspace zeroed) rides on the bbox pass
   // when there is one
   const bool bbox_pass = !quantized && !custom;
   if (!bbox_pass) {
     Stage t(ctx, "prep", s);
-    launch_prep(ctx->st, reinterpret_cast<uint32_t *>(zb), z.total / 4, s);
+    launch_prep(st, reinterpret_cast<uint32_t *>(zb), z.total / 4, s);
   }
   CHK(launch_status("prep"));
   if (quantized) {
@@ -263,8 +310,7 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   } else {
     if (bbox_pass) {
       Stage t(ctx, "bbox", s);
-      CHK(pool_get(ctx, "bbox_part", kBboxPartBytes, &p));
-      launch_bbox(static_cast<const float *>(pts), n, dim, static_cast<float *>(p), ctx->st, s,
+      launch_bbox(static_cast<const float *>(pts), n, dim, b.bbox_part, st, s,
                   reinterpret_cast<uint32_t *>(zb), (msd_first ? z.sort_lb : z.total) / 4,
                   true);
     }
@@ -274,11 +320,11 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
     if (msd_b)
       launch_encode<K>(static_cast<const float *>(pts), n, dim, bits, effw, custom ? 1 : 0,
                        custom ? opts->bb_min : nullptr, custom ? opts->bb_max : nullptr, kA,
-                       nullptr, hist, 0, ctx->st, s, msd_hist, mbits - msd_b, 1 << msd_b);
+                       nullptr, hist, 0, st, s, msd_hist, mbits - msd_b, 1 << msd_b);
     else
       launch_encode<K>(static_cast<const float *>(pts), n, dim, bits, effw, custom ? 1 : 0,
                        custom ? opts->bb_min : nullptr, custom ? opts->bb_max : nullptr, kA,
-                       nullptr, hist, passes, ctx->st, s);
+                       nullptr, hist, passes, st, s);
   }
   CHK(launch_status("encode"));
   int parity;
@@ -286,65 +332,88 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
     if (msd_b) {
       // the result lands where the LSD passes would leave it
       Stage t(ctx, "msd", s);
-      CHK(pool_get(ctx, "msd_start", ((size_t)kMsdMaxBuckets + 1) * 4, &p));
-      uint32_t *mstart = static_cast<uint32_t *>(p);
-      CHK(pool_get(ctx, "msd_cursor", (size_t)kMsdCursorWords * 4, &p));
       launch_msd_sort(kA, kB, (passes & 1) ? kB : kA, prepend, n, mbits, msd_b, msd_hist,
-                      mstart, static_cast<uint32_t *>(p), ctx->st, s);
-      launch_histogram<K>(kA, n, hist, passes, s, ctx->st, sort_lb, (z.total - z.sort_lb) / 4);
+                      b.msd_start, b.msd_cursor, st, s);
+      launch_histogram<K>(kA, n, hist, passes, s, st, sort_lb, (z.total - z.sort_lb) / 4);
     }
   }
   {
     Stage span(ctx, "sort", s, 2);
     PassTimer pt(ctx, n > 1 ? passes : 0);
-    parity = launch_onesweep<K>(kA, kB, labels, vA, vB, n, passes, hist, sort_lb, ctx->st, s,
-                                pt.events(), msd_b ? ctx->st : nullptr);
+    parity = launch_onesweep<K>(kA, kB, labels, b.vA, b.vB, n, passes, hist, sort_lb, st, s,
+                                pt.events(), msd_b ? st : nullptr);
   }
   CHK(launch_status("onesweep"));
   K *sorted = parity ? kB : kA;
   K *uniq = sorted - prepend;   // in place (see unique_tile)
   // packed local splits go to the sort's other buffer, free after the sort
   uint32_t *lsx = reinterpret_cast<uint32_t *>(parity ? kA : kB);
-  const uint32_t *sortedv = lab ? (n <= 1 ? labels : (parity ? vB : vA)) : nullptr;
+  const uint32_t *sortedv = lab ? (n <= 1 ? labels : (parity ? b.vB : b.vA)) : nullptr;
   {
     Stage t(ctx, "unique", s);
-    launch_unique<K>(sorted, sortedv, n, prepend, uniq, cbits, lb_u, ctx->st, s,
-                     msd_b ? ctx->st : nullptr);
+    launch_unique<K>(sorted, sortedv, n, prepend, uniq, cbits, lb_u, st, s,
+                     msd_b ? st : nullptr);
   }
   CHK(launch_status("unique"));
   {
     Stage t(ctx, "brt", s);
     // local-split totals per chunk of 256 nodes go to chunks
-    launch_brt_ls<K>(uniq, nk, mbits, dim, brt, parent, lsx, chunks, defer, ctx->st, s);
+    launch_brt_ls<K>(uniq, nk, mbits, dim, b.brt, b.parent, lsx, b.chunks, b.defer, st, s);
   }
   CHK(launch_status("brt"));
   {
     // chunk bases and the octree size; the gather writes ps from them
     Stage t(ctx, "splits_scan", s);
-    launch_chunk_scan(chunks, nk, lb_b, ctx->st, s);
+    launch_chunk_scan(b.chunks, nk, lb_b, st, s);
   }
   CHK(launch_status("splits_scan"));
   {
     // order-exact claims, local splits and scan; no-ops unless some BRT
     // split took the FP32-quirk value
     Stage t(ctx, "malformed", s);
-    launch_claims_if_malformed<K>(brt, parent, nk, ctx->st, s);
-    launch_splits_scan<K>(brt, parent, nullptr, ps, nk, dim, 1, lb_s, ctx->st, s);
+    launch_claims_if_malformed<K>(b.brt, b.parent, nk, st, s);
+    launch_splits_scan<K>(b.brt, b.parent, nullptr, b.ps, nk, dim, 1, lb_s, st, s);
   }
   CHK(launch_status("malformed path"));
   {
     Stage t(ctx, "octree", s);
     hipEvent_t gev[2] = {nullptr, nullptr};
-    if (ctx->timing == 3) {
+    if (ctx && ctx->timing == 3) {
       gev[0] = take_event(ctx);
       gev[1] = take_event(ctx);
     }
     const bool timed = gev[0] && gev[1];
-    launch_octree<K>(brt, parent, ps, lsx, chunks, uniq, cbits, nk, dim, mbits, octree, nconf,
-                     ctx->octree_cap, 0, defer, ctx->st, s, timed ? gev : nullptr);
+    launch_octree<K>(b.brt, b.parent, b.ps, lsx, b.chunks, uniq, cbits, nk, dim, mbits, octree,
+                     nconf, octree_cap, 0, b.defer, st, s, timed ? gev : nullptr);
     if (timed) ctx->recs.push_back({"gather", gev[0], gev[1]});
   }
   CHK(launch_status("octree"));
+  out.uniq = uniq;
+  out.lsx = lsx;
+  out.cbits = cbits;
+  return PGVD_OK;
+}
+
+template <typename K>
+int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
+               const uint32_t *labels, const pgvd_build_opts *opts, hipStream_t s) {
+  const bool lab = labels != nullptr;
+  BuildBufs<K> b;
+  CHK(carve_build<K>([ctx](const char *key, size_t bytes, void **p) {
+        return pool_get(ctx, key, bytes, p);
+      }, pts, n, dim, bits, lab, opts, b));
+  int32_t *octree;
+  uint32_t *nconf;
+  int64_t want = 2 * (n + 1);
+  if (want < 1024) want = 1024;
+  if (ctx->octree_cap < want || ctx->octree_stride != (1 << dim) + 1 ||
+      (lab && ctx->pool["node_conflict"].bytes < (size_t)ctx->octree_cap * 4))
+    CHK(ensure_octree<K>(ctx, dim, want, lab, &octree, &nconf));
+  octree = static_cast<int32_t *>(ctx->pool["octree"].p);
+  nconf = lab ? static_cast<uint32_t *>(ctx->pool["node_conflict"].p) : nullptr;
+  BuildOut<K> o;
+  CHK(build_launch<K>(ctx, b, ctx->st, octree, nconf, ctx->octree_cap, pts, n, dim, bits, labels,
+                      opts, s, o));
   HIPCHK(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(RunState), hipMemcpyDeviceToHost, s));
   HIPCHK(hipEventRecord(ctx->done, s));
   LastBuild &L = ctx->last;
@@ -352,17 +421,17 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   L.n = n;
   L.dim = dim;
   L.bits = bits;
-  L.mbits = mbits;
-  L.kb = kb;
+  L.mbits = dim * bits;
+  L.kb = (int)sizeof(K);
   L.labels = lab;
-  L.keys_u = uniq;
-  L.brt = brt;
-  L.parent = parent;
-  L.ps = ps;
-  L.lsx = lsx;
-  L.chunks = chunks;
-  L.defer = defer;
-  L.conflict_bits = cbits;
+  L.keys_u = o.uniq;
+  L.brt = b.brt;
+  L.parent = b.parent;
+  L.ps = b.ps;
+  L.lsx = o.lsx;
+  L.chunks = b.chunks;
+  L.defer = b.defer;
+  L.conflict_bits = o.cbits;
   L.stream = s;
   return PGVD_OK;
 }
@@ -492,6 +561,67 @@ int brt_to_octree_impl(pgvd_ctx *ctx, const BrtNodeT<K> *brt, const int32_t *par
   CHK(launch_status("octree"));
   CHK(copy_state_and_wait(ctx, s));
   return status_error(ctx->st_host->status);
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free whole path
+// ---------------------------------------------------------------------------
+// Workspace of pgvd_build_octree_noalloc: every intermediate of a build (run
+// state, keys, BRT, prefix sums, look-back words) in one static device
+// array.  Each GPU the library runs on holds its own copy until the process
+// ends.  In host code `scratch` only names the symbol; its device address
+// on the current GPU comes from hipGetSymbolAddress (noalloc_workspace).
+constexpr size_t kNoAllocScratchBytes = PGVD_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) static unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The workspace's address on the current device, looked up on the first
+// call there and kept.
+int noalloc_workspace(void **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(PGVD_ERR_NO_DEVICE, "device index above 63");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(PGVD_ERR_HIP, "hipGetSymbolAddress returned no address for the workspace");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return PGVD_OK;
+}
+
+template <typename K>
+int noalloc_impl(char *base, const void *pts, int64_t n, int dim, int bits,
+                 const uint32_t *labels, const pgvd_build_opts *opts, int32_t *octree,
+                 int64_t octree_cap, uint32_t *node_conflict, int32_t *state, hipStream_t s) {
+  // bump carving in a fixed order; the total is checked before any launch
+  size_t off = 0;
+  auto get = [&](const char *, size_t bytes, void **p) {
+    if (bytes == 0) bytes = 256;
+    *p = base + off;
+    off += align256(bytes);
+    return (int)PGVD_OK;
+  };
+  void *p;
+  CHK(get("state", sizeof(RunState), &p));
+  RunState *st = static_cast<RunState *>(p);
+  BuildBufs<K> b;
+  CHK(carve_build<K>(get, pts, n, dim, bits, labels != nullptr, opts, b));
+  if (off > kNoAllocScratchBytes)
+    return fail(PGVD_ERR_TOO_LARGE,
+                "pgvd_build_octree_noalloc: this build needs " + std::to_string(off) +
+                    " bytes of workspace; the allocation-free entry holds " +
+                    std::to_string(kNoAllocScratchBytes));
+  BuildOut<K> o;
+  CHK(build_launch<K>(nullptr, b, st, octree, node_conflict, octree_cap, pts, n, dim, bits,
+                      labels, opts, s, o));
+  launch_build_state(st, state, s);
+  return launch_status("build state");
 }
 
 }  // namespace
@@ -1149,6 +1279,28 @@ int pgvd_build_octree(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bi
   if (kb == 4) return build_impl<uint32_t>(ctx, pts, n, dim, bits, labels, opts, st);
   if (kb == 8) return build_impl<uint64_t>(ctx, pts, n, dim, bits, labels, opts, st);
   return build_impl<Key128>(ctx, pts, n, dim, bits, labels, opts, st);
+}
+
+int pgvd_build_octree_noalloc(const void *pts, int64_t n, int dim, int bits,
+                              const uint32_t *labels, const pgvd_build_opts *opts,
+                              int32_t *octree, int64_t octree_cap, uint32_t *node_conflict,
+                              pgvd_build_state *state, pgvd_stream s) {
+  if (!pts || !octree || !state) return fail(PGVD_ERR_INVALID, "null argument");
+  if (labels && !node_conflict)
+    return fail(PGVD_ERR_INVALID, "labels need a node_conflict buffer");
+  if (octree_cap < 1) return fail(PGVD_ERR_INVALID, "octree_cap must be at least 1");
+  CHK(check_n(n));
+  CHK(check_dims(dim, bits));
+  if (opts && opts->quantized_input && bits > 31) return fail(PGVD_ERR_INVALID, "int32 grid input allows bits <= 31");
+  void *base = nullptr;
+  CHK(noalloc_workspace(&base));
+  hipStream_t st = static_cast<hipStream_t>(s);
+  int32_t *out4 = reinterpret_cast<int32_t *>(state);
+  char *ws = static_cast<char *>(base);
+  const int kb = key_bytes_for(dim * bits);
+  if (kb == 4) return noalloc_impl<uint32_t>(ws, pts, n, dim, bits, labels, opts, octree, octree_cap, node_conflict, out4, st);
+  if (kb == 8) return noalloc_impl<uint64_t>(ws, pts, n, dim, bits, labels, opts, octree, octree_cap, node_conflict, out4, st);
+  return noalloc_impl<Key128>(ws, pts, n, dim, bits, labels, opts, octree, octree_cap, node_conflict, out4, st);
 }
 
 int pgvd_build_octree_finish(pgvd_ctx *ctx, pgvd_octree_result *out) {

@@ -150,6 +150,8 @@ void launch_init_octree(int32_t *octree, int64_t nodes, int dim, hipStream_t s);
 void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 void launch_set_sizes(RunState *st, const uint32_t *ps, int64_t m, hipStream_t s);
+// out4 (device, 4 int32): pgvd_status of the run, m, octree size, malformed.
+void launch_build_state(const RunState *st, int32_t *out4, hipStream_t s);
 // *out (+)= digest of `words` 32-bit words at p: sum of mix64(w_i | i << 32).
 void launch_digest(const void *p, size_t words, uint64_t *out, bool zero_first, hipStream_t s);
 // *out = digest of count[0] * per_item words at p (at most n_max), the count

@@ -44,6 +44,23 @@ __global__ void set_sizes_kernel(RunState *st, const uint32_t *ps, int64_t m) {
   st->octree_size = ps[m - 1];
 }
 
+// The outcome of a build as four words: a pgvd_status code (the device's
+// error bits in the order the host reports them; an octree above the
+// output capacity is PGVD_ERR_CAPACITY), m, the octree size, malformed.
+__global__ void build_state_kernel(const RunState *st, int32_t *out) {
+  const uint32_t b = st->status;
+  int32_t code = 0;                                        // PGVD_OK
+  if (b & ST_NEG_COORD) code = 3;                          // PGVD_ERR_BAD_COORD
+  else if (b & ST_DUP_KEY) code = 9;                       // PGVD_ERR_DUP_KEY
+  else if (b & (ST_STACK | ST_WALK)) code = 10;            // PGVD_ERR_INTERNAL
+  else if (b & ST_TOO_LARGE) code = 8;                     // PGVD_ERR_TOO_LARGE
+  else if (b & ST_CAPACITY) code = 7;                      // PGVD_ERR_CAPACITY
+  out[0] = code;
+  out[1] = (int32_t)st->m;
+  out[2] = (int32_t)st->octree_size;
+  out[3] = (int32_t)st->malformed;
+}
+
 __device__ inline uint64_t mix64(uint64_t z) {   // splitmix64 finaliser
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
@@ -111,6 +128,10 @@ void launch_digest_count(const void *p, const uint32_t *count, uint32_t per_item
   zero_u64_kernel<<<1, 1, 0, s>>>(o);
   digest_count_kernel<<<grid_for(n_max ? n_max : 1), kBlock, 0, s>>>(
       static_cast<const uint32_t *>(p), count, per_item, n_max, o);
+}
+
+void launch_build_state(const RunState *st, int32_t *out4, hipStream_t s) {
+  build_state_kernel<<<1, 1, 0, s>>>(st, out4);
 }
 
 void launch_set_sizes(RunState *st, const uint32_t *ps, int64_t m, hipStream_t s) {
