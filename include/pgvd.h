@@ -254,6 +254,43 @@ typedef struct pgvd_brt_node128 {
   uint8_t left_leaf, right_leaf;
 } pgvd_brt_node128;
 
+/* ---- the reference's byte layouts (host-side conversion) ----
+ * BigUnsigned (C/BigUnsigned.h:8-16): int len, then 11 8-bit blocks, least
+ * significant first, with no zero block at len - 1 (zapLeadingZeros,
+ * C/BigUnsigned.c:22-26); blocks at and above len are not part of the
+ * value.  BrtNode (C/BrtNode.h:10-21): 32 bytes, lcp a BigUnsigned.
+ * Callers that read keys or BRT nodes back into the reference's structs
+ * (tests/KernelTests.cpp:74,130,190 read BigUnsigned buffers) convert the
+ * library's native records with these.  All pointers are host memory;
+ * nothing runs on the device. */
+typedef struct pgvd_big_unsigned {
+  int32_t len;
+  uint8_t blk[11];
+  uint8_t pad_;
+} pgvd_big_unsigned;
+typedef struct pgvd_ref_brt_node {
+  int32_t left;
+  uint8_t left_leaf, right_leaf, pad_[2];
+  pgvd_big_unsigned lcp;
+  int32_t lcp_length;
+  int32_t parent;
+} pgvd_ref_brt_node;
+/* n native keys (key_bytes 4, 8 or 16) -> BigUnsigned; unused blocks and
+ * padding are written as zero.  PGVD_ERR_TOO_LARGE for a key above 88 bits
+ * (BIG_INTEGER_SIZE 11). */
+int pgvd_keys_to_big_unsigned(const void *keys, int key_bytes, int64_t n, pgvd_big_unsigned *out);
+/* n BigUnsigned -> native keys.  PGVD_ERR_INVALID when a len is outside
+ * 0..11; PGVD_ERR_TOO_LARGE when a value does not fit key_bytes. */
+int pgvd_big_unsigned_to_keys(const pgvd_big_unsigned *in, int64_t n, int key_bytes, void *keys);
+/* compareBU (C/BigUnsigned.c:132-153): -1, 0 or 1. */
+int pgvd_compare_big_unsigned(const pgvd_big_unsigned *x, const pgvd_big_unsigned *y);
+/* count BRT records (pgvd_brt_node32/64/128 by key_bytes) and their parent
+ * links -> BrtNode; and back. */
+int pgvd_brt_to_ref_nodes(const void *brt, const int32_t *parent, int64_t count, int key_bytes,
+                          pgvd_ref_brt_node *out);
+int pgvd_ref_nodes_to_brt(const pgvd_ref_brt_node *in, int64_t count, int key_bytes, void *brt,
+                          int32_t *parent);
+
 /* Kernels::BuildBinaryRadixTree_p (opencl/Kernels.cpp:249-264, kernel
  * C/BuildBRT.c:90-181): m sorted unique keys -> m-1 nodes (brt_out, record
  * type by key_bytes) and parent_out (m-1 int32). */

@@ -12,7 +12,9 @@
 //   * DIM is a property of the data (2 or 3), not a compile-time macro:
 //     points are pgvd::int2 / pgvd::int3, octree nodes OctNodeT<2> / <3>.
 //   * Keys are 32/64/128-bit words picked from mbits instead of the 11-byte
-//     BigUnsigned; a key buffer records its width.
+//     BigUnsigned; a key buffer records its width.  ReadBigUnsigned,
+//     WriteBigUnsigned and ReadBrtNodes convert to and from the reference's
+//     BigUnsigned and BrtNode records.
 // The serial `_s` functions of the reference are the test oracle
 // (oracle/pgvd_oracle.h), not part of this library.
 #pragma once
@@ -52,6 +54,18 @@ struct BrtNode {
   int32_t lcp_length;
   int32_t parent;
 };
+
+// The reference's own records, byte for byte: BigUnsigned
+// (C/BigUnsigned.h:8-16, 16 bytes: len, then 11 8-bit blocks least
+// significant first, no zero block at len - 1) and BrtNode
+// (C/BrtNode.h:10-21, 32 bytes).  Kernels::ReadBigUnsigned / ReadBrtNodes
+// read device results into them, as the reference's tests read their
+// buffers (tests/KernelTests.cpp:74,130,190); CompareBU is compareBU.
+using BigUnsigned = pgvd_big_unsigned;
+using RefBrtNode = pgvd_ref_brt_node;
+inline int CompareBU(const BigUnsigned &x, const BigUnsigned &y) {
+  return pgvd_compare_big_unsigned(&x, &y);
+}
 
 // Resln (Resln.h:29-60) and make_resln (:64-83), with 64-bit width so that
 // grids finer than 2^30 per axis are expressible.
@@ -148,6 +162,13 @@ int BuildOctree_p(const std::vector<int3> &points, std::vector<OctNode3> &octree
 // enqueueReadBuffer in its tests).
 int ReadKeys(const Buffer &keys, int count, std::vector<uint64_t> &lo, std::vector<uint64_t> &hi);
 int ReadBrt(const Buffer &internalBRTNodes, int size, std::vector<BrtNode> &nodes);
+// The same results in the reference's layouts: count keys as BigUnsigned
+// (keys of at most 88 bits), the size - 1 BRT nodes with their parent links
+// as BrtNode.  WriteBigUnsigned uploads BigUnsigned values as key_bytes
+// words (a key buffer for RadixSortBigUnsigned and the other stages).
+int ReadBigUnsigned(const Buffer &keys, int count, std::vector<BigUnsigned> &out);
+int WriteBigUnsigned(const std::vector<BigUnsigned> &in, int key_bytes, Buffer &keys);
+int ReadBrtNodes(const Buffer &internalBRTNodes, int size, std::vector<RefBrtNode> &nodes);
 
 }  // namespace Kernels
 

@@ -114,6 +114,12 @@ def ref(dim: int = 2):
         L.ref_tree_splits.argtypes = [C.c_void_p, u32p, u32p]
         L.ref_tree_octree.argtypes = [C.c_void_p, i32p, i32p]
         L.ref_tree_free.argtypes = [C.c_void_p]
+        L.ref_tree_keys_raw.argtypes = [C.c_void_p, C.c_void_p]
+        L.ref_tree_brt_raw.argtypes = [C.c_void_p, C.c_void_p]
+        L.ref_layout.argtypes = [i32p]
+        L.ref_bu_from_bits.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_void_p]
+        L.ref_compare_bu.argtypes = [C.c_void_p, C.c_void_p]
+        L.ref_compare_bu.restype = C.c_int
         _REF[dim] = L
     return _REF[dim]
 

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -236,5 +237,52 @@ void ref_tree_octree(void *h, int32_t *children, int32_t *leaf) {
 }
 
 void ref_tree_free(void *h) { delete static_cast<RefTree *>(h); }
+
+// The reference's own records, byte for byte: its BigUnsigned keys (16
+// bytes each) and BrtNodes, and their layout (sizes and field offsets) as
+// its headers define it.
+void ref_tree_keys_raw(void *h, void *out) {
+  RefTree *t = static_cast<RefTree *>(h);
+  std::memcpy(out, t->keys.data(), sizeof(BigUnsigned) * t->keys.size());
+}
+void ref_tree_brt_raw(void *h, void *out) {
+  RefTree *t = static_cast<RefTree *>(h);
+  std::memcpy(out, t->I.data(), sizeof(BrtNode) * t->I.size());
+}
+// sizeof(BigUnsigned), offsetof len/blk, sizeof(BrtNode), offsetof left,
+// left_leaf, right_leaf, lcp, lcp_length, parent
+void ref_layout(int32_t *out) {
+  out[0] = (int32_t)sizeof(BigUnsigned);
+  out[1] = (int32_t)offsetof(BigUnsigned, len);
+  out[2] = (int32_t)offsetof(BigUnsigned, blk);
+  out[3] = (int32_t)sizeof(BrtNode);
+  out[4] = (int32_t)offsetof(BrtNode, left);
+  out[5] = (int32_t)offsetof(BrtNode, left_leaf);
+  out[6] = (int32_t)offsetof(BrtNode, right_leaf);
+  out[7] = (int32_t)offsetof(BrtNode, lcp);
+  out[8] = (int32_t)offsetof(BrtNode, lcp_length);
+  out[9] = (int32_t)offsetof(BrtNode, parent);
+}
+// A value built the way the reference's tests build random BigUnsigneds
+// (tests/KernelTests.cpp:106-112: setBUBit per bit, then len and
+// zapLeadingZeros), for nbits bits of (lo, hi).
+void ref_bu_from_bits(uint64_t lo, uint64_t hi, int nbits, void *out) {
+  BigUnsigned b;
+  std::memset(&b, 0, sizeof(b));
+  initBU(&b);
+  for (int j = 0; j < nbits; ++j) {
+    const uint64_t bit = j < 64 ? (lo >> j) & 1 : (hi >> (j - 64)) & 1;
+    setBUBit(&b, j, bit != 0);
+  }
+  b.len = nbits / 8;
+  zapLeadingZeros(&b);
+  std::memcpy(out, &b, sizeof(b));
+}
+int ref_compare_bu(const void *x, const void *y) {
+  BigUnsigned a, b;
+  std::memcpy(&a, x, sizeof(a));
+  std::memcpy(&b, y, sizeof(b));
+  return compareBU(&a, &b);
+}
 
 }  // extern "C"

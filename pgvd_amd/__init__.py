@@ -154,6 +154,11 @@ def lib():
             "pgvd_server_destroy": ([vp], i32),
             "pgvd_server_run": ([vp, C.POINTER(Request), i64, C.POINTER(ServerStats)], i32),
             "pgvd_build_footprint": ([i64, i32, i32], C.c_size_t),
+            "pgvd_keys_to_big_unsigned": ([vp, i32, i64, vp], i32),
+            "pgvd_big_unsigned_to_keys": ([vp, i64, i32, vp], i32),
+            "pgvd_compare_big_unsigned": ([vp, vp], i32),
+            "pgvd_brt_to_ref_nodes": ([vp, vp, i64, i32, vp], i32),
+            "pgvd_ref_nodes_to_brt": ([vp, i64, i32, vp, vp], i32),
             "pgvd_host_alloc": ([C.c_size_t, C.POINTER(vp)], i32),
             "pgvd_host_free": ([vp], i32),
         }
@@ -191,6 +196,48 @@ BRT_DTYPES = {
 assert BRT_DTYPES[4].itemsize == 12 and BRT_DTYPES[8].itemsize == 16
 # pgvd_brt_node128 is 8-byte aligned: 16 + 4 + 2 + 1 + 1 = 24 bytes
 assert BRT_DTYPES[16].itemsize == 24
+
+
+# The reference's records (C/BigUnsigned.h:8-16, C/BrtNode.h:10-21), as
+# pgvd_big_unsigned / pgvd_ref_brt_node lay them out.
+BIG_UNSIGNED = np.dtype([("len", "<i4"), ("blk", "u1", (11,)), ("pad", "u1")])
+REF_BRT_NODE = np.dtype([("left", "<i4"), ("left_leaf", "u1"), ("right_leaf", "u1"),
+                         ("pad", "u1", (2,)), ("lcp", BIG_UNSIGNED), ("lcp_length", "<i4"),
+                         ("parent", "<i4")])
+assert BIG_UNSIGNED.itemsize == 16 and REF_BRT_NODE.itemsize == 32
+
+
+def to_big_unsigned(words: np.ndarray, key_bytes: int) -> np.ndarray:
+    """Native key words (the device key layout) -> BigUnsigned records."""
+    words = np.ascontiguousarray(words)
+    n = words.size // (2 if key_bytes == 16 else 1)
+    out = np.zeros(n, BIG_UNSIGNED)
+    rc = lib().pgvd_keys_to_big_unsigned(words.ctypes.data, key_bytes, n, out.ctypes.data)
+    if rc != 0:
+        raise PgvdError(rc, lib().pgvd_last_error().decode())
+    return out
+
+
+def from_big_unsigned(bu: np.ndarray, key_bytes: int) -> np.ndarray:
+    """BigUnsigned records -> native key words."""
+    bu = np.ascontiguousarray(bu, BIG_UNSIGNED)
+    out = np.zeros(bu.size * (2 if key_bytes == 16 else 1), key_dtype(key_bytes))
+    rc = lib().pgvd_big_unsigned_to_keys(bu.ctypes.data, bu.size, key_bytes, out.ctypes.data)
+    if rc != 0:
+        raise PgvdError(rc, lib().pgvd_last_error().decode())
+    return out
+
+
+def to_ref_brt_nodes(brt: np.ndarray, parent: np.ndarray, key_bytes: int) -> np.ndarray:
+    """Native BRT records + parent links -> the reference's BrtNode records."""
+    brt = np.ascontiguousarray(brt)
+    parent = np.ascontiguousarray(parent, np.int32)
+    out = np.zeros(parent.size, REF_BRT_NODE)
+    rc = lib().pgvd_brt_to_ref_nodes(brt.ctypes.data, parent.ctypes.data, parent.size, key_bytes,
+                                     out.ctypes.data)
+    if rc != 0:
+        raise PgvdError(rc, lib().pgvd_last_error().decode())
+    return out
 
 
 def keys_to_words(lo: np.ndarray, hi, key_bytes: int) -> np.ndarray:

@@ -358,6 +358,37 @@ int ReadKeys(const Buffer &keys, int count, std::vector<uint64_t> &lo, std::vect
   return rc;
 }
 
+int ReadBigUnsigned(const Buffer &keys, int count, std::vector<BigUnsigned> &out) {
+  const int kb = keys.key_bytes;
+  std::vector<unsigned char> raw((size_t)kb * (size_t)(count > 0 ? count : 0));
+  int rc = ctx().download(raw.data(), keys, raw.size());
+  out.assign((size_t)(count > 0 ? count : 0), BigUnsigned{});
+  if (rc == PGVD_OK && count > 0) rc = pgvd_keys_to_big_unsigned(raw.data(), kb, count, out.data());
+  return rc;
+}
+
+int WriteBigUnsigned(const std::vector<BigUnsigned> &in, int key_bytes, Buffer &keys) {
+  std::vector<unsigned char> raw((size_t)key_bytes * in.size());
+  int rc = pgvd_big_unsigned_to_keys(in.data(), (int64_t)in.size(), key_bytes, raw.data());
+  if (rc != PGVD_OK) return rc;
+  if (keys.bytes() < raw.size()) keys = ctx().alloc(raw.size());
+  keys.key_bytes = key_bytes;
+  return ctx().upload(keys, raw.data(), raw.size());
+}
+
+int ReadBrtNodes(const Buffer &I, int size, std::vector<RefBrtNode> &out) {
+  const int kb = I.key_bytes;
+  const size_t nb = brt_node_bytes(kb);
+  const int n = size > 1 ? size - 1 : 0;
+  std::vector<unsigned char> raw(nb * (size_t)n);
+  std::vector<int32_t> parent((size_t)n);
+  int rc = ctx().download(raw.data(), I, raw.size());
+  if (rc == PGVD_OK) rc = ctx().download(parent.data(), I.parent_links(), sizeof(int32_t) * (size_t)n);
+  out.assign((size_t)n, RefBrtNode{});
+  if (rc == PGVD_OK && n > 0) rc = pgvd_brt_to_ref_nodes(raw.data(), parent.data(), n, kb, out.data());
+  return rc;
+}
+
 int ReadBrt(const Buffer &I, int size, std::vector<BrtNode> &nodes) {
   const int kb = I.key_bytes;
   const size_t nb = brt_node_bytes(kb);

@@ -175,6 +175,110 @@ static void tree_scenarios() {
   std::printf("PASS A binary radix tree can be used to construct a Octree/Quadtree\n");
 }
 
+// The same scenarios read back the way the reference's tests read their
+// buffers: as BigUnsigned records (tests/KernelTests.cpp:74,130,190) and
+// BrtNodes (:209-277), compared with compareBU (C/BigUnsigned.c:132-153)
+// and compareBrtNode's fields (C/BrtNode.h:23-31).  The sort takes
+// BIG_INTEGER_SIZE * 8 = 88-bit values built bit by bit (:106-112).
+static BigUnsigned bu_of(uint64_t lo, uint64_t hi) {
+  BigUnsigned b;
+  uint64_t w[2] = {lo, hi};
+  pgvd_keys_to_big_unsigned(w, 16, 1, &b);
+  return b;
+}
+
+static void big_unsigned_scenarios() {
+  {   // Morton keys of (i, i), 30 bits per axis
+    const int n = 1000000, g = Kernels::nextPow2(n);
+    std::vector<int2> pts(n);
+    for (int i = 0; i < n; ++i) pts[i] = {i, i};
+    Buffer p, z;
+    CHECK(Kernels::UploadPoints(pts, p) == 0, "upload");
+    CHECK(Kernels::PointsToMorton_p(p, z, n, kBits) == 0, "PointsToMorton_p");
+    std::vector<BigUnsigned> gpu;
+    CHECK(Kernels::ReadBigUnsigned(z, g, gpu) == 0, "ReadBigUnsigned");
+    int cmp = 0;
+    for (int i = 0; i < g && cmp == 0; ++i) {
+      uint64_t lo = 0, hi = 0;
+      if (i < n) {
+        const int64_t c[2] = {i, i};
+        oracle_xyz2z(c, 2, kBits, &lo, &hi);
+      }
+      const BigUnsigned want = bu_of(lo, hi);
+      cmp = CompareBU(want, gpu[i]);
+    }
+    CHECK(cmp == 0, "Morton keys read as BigUnsigned compare equal (compareBU)");
+  }
+  {   // radix sort of 88-bit BigUnsigneds, then unique
+    const int n = 1 << 20;
+    std::mt19937_64 rng(9);
+    std::vector<BigUnsigned> host(n);
+    for (auto &b : host) {
+      const uint64_t lo = rng(), hi = rng() & ((1ull << 24) - 1);
+      b = bu_of(lo, (rng() & 7) ? hi : 0);   // some shorter values
+    }
+    for (int i = 0; i < n; i += 97) host[i] = host[(i * 31) % n];   // equal values
+    Buffer b;
+    CHECK(Kernels::WriteBigUnsigned(host, 16, b) == 0, "WriteBigUnsigned");
+    b.dim = 2;
+    CHECK(Kernels::RadixSortBigUnsigned(b, n, 88) == 0, "RadixSortBigUnsigned (88 bits)");
+    std::vector<BigUnsigned> gpu;
+    CHECK(Kernels::ReadBigUnsigned(b, n, gpu) == 0, "ReadBigUnsigned");
+    std::sort(host.begin(), host.end(),
+              [](const BigUnsigned &x, const BigUnsigned &y) { return CompareBU(x, y) < 0; });
+    int cmp = 0;
+    for (int i = 0; i < n && cmp == 0; ++i) cmp = CompareBU(host[i], gpu[i]);
+    CHECK(cmp == 0, "sorted BigUnsigneds compare equal to std::sort's (compareBU)");
+    int size = n;
+    CHECK(Kernels::UniqueSorted(b, size) == 0, "UniqueSorted");
+    host.erase(std::unique(host.begin(), host.end(),
+                           [](const BigUnsigned &x, const BigUnsigned &y) { return CompareBU(x, y) == 0; }),
+               host.end());
+    CHECK(size == (int)host.size(), "unique count");
+    CHECK(Kernels::ReadBigUnsigned(b, size, gpu) == 0, "ReadBigUnsigned");
+    for (int i = 0; i < size && cmp == 0; ++i) cmp = CompareBU(host[i], gpu[i]);
+    CHECK(cmp == 0, "unique BigUnsigneds compare equal to std::unique's");
+  }
+  {   // BRT over 87-bit keys, read as BrtNodes
+    std::mt19937_64 rng(10);
+    std::vector<std::pair<uint64_t, uint64_t>> kv(200000);
+    for (auto &k : kv) k = {rng() & ((1ull << 23) - 1), rng()};
+    kv.push_back({0, 0});
+    std::sort(kv.begin(), kv.end());
+    kv.erase(std::unique(kv.begin(), kv.end()), kv.end());
+    const int m = (int)kv.size();
+    std::vector<BigUnsigned> keys(m);
+    std::vector<uint64_t> lo(m), hi(m);
+    for (int i = 0; i < m; ++i) {
+      hi[i] = kv[i].first;
+      lo[i] = kv[i].second;
+      keys[i] = bu_of(lo[i], hi[i]);
+    }
+    Buffer z, I;
+    CHECK(Kernels::WriteBigUnsigned(keys, 16, z) == 0, "WriteBigUnsigned");
+    z.dim = 3;
+    CHECK(Kernels::BuildBinaryRadixTree_p(z, I, m, 87) == 0, "BuildBinaryRadixTree_p");
+    std::vector<RefBrtNode> nodes;
+    CHECK(Kernels::ReadBrtNodes(I, m, nodes) == 0, "ReadBrtNodes");
+    oracle_tree *t = oracle_build_from_keys(lo.data(), hi.data(), m, 3, 87);
+    std::vector<int32_t> left(m - 1), len(m - 1), par(m - 1);
+    std::vector<uint8_t> ll(m - 1), rl(m - 1);
+    std::vector<uint64_t> lcp_lo(m - 1), lcp_hi(m - 1);
+    oracle_tree_brt(t, left.data(), ll.data(), rl.data(), lcp_lo.data(), lcp_hi.data(), len.data(),
+                    par.data());
+    oracle_tree_free(t);
+    bool ok = (int)nodes.size() == m - 1;
+    for (int i = 0; i < m - 1 && ok; ++i) {
+      const BigUnsigned want = bu_of(lcp_lo[i], lcp_hi[i]);
+      ok = CompareBU(nodes[i].lcp, want) == 0 && nodes[i].lcp_length == len[i] &&
+           nodes[i].left == left[i] && nodes[i].left_leaf == ll[i] && nodes[i].parent == par[i] &&
+           nodes[i].right_leaf == rl[i];
+    }
+    CHECK(ok, "BrtNodes equal the serial BRT (compareBrtNode fields)");
+  }
+  std::printf("PASS results read back as BigUnsigned and BrtNode records\n");
+}
+
 static bool same_octree(const std::vector<OctNode2> &a, oracle_tree *t) {
   const int64_t O = oracle_tree_octree_size(t);
   if ((int64_t)a.size() != O) return false;
@@ -350,6 +454,7 @@ int main() {
   bit_serial_scenario();
   unique_scenario();
   tree_scenarios();
+  big_unsigned_scenarios();
   build_scenarios();
   karras_scenario();
   query_scenarios();
