@@ -323,6 +323,11 @@ typedef struct pgvd_build_opts {
   int clamped;              /* Karras::Quantize clamped (Karras.cpp:46-49)       */
   int quantized_input;      /* 1: pts are int32 grid coordinates (n*dim),
                                as Kernels::BuildOctree_p takes vector<intn>     */
+  int no_host_wait;         /* 1: the call never waits for the device: the sort
+                               takes the LSD radix passes instead of the bucket
+                               sort, whose levels are decided on the host while
+                               the device works (for callers that enqueue builds
+                               on several streams from one thread)             */
 } pgvd_build_opts;
 
 typedef struct pgvd_octree_result {
@@ -347,8 +352,12 @@ typedef struct pgvd_octree_result {
  * bounding box and quantization of Karras::Quantize (Karras.cpp:69-99):
  * float points (device, n*dim) -> octree, all on the device.  labels
  * (device, n u32, nullable) are object ids carried through the sort for the
- * conflict-cell pass.  Asynchronous: enqueues the whole path on the stream
- * and returns; pgvd_build_octree_finish waits and reports the sizes. */
+ * conflict-cell pass.  Enqueues the whole path on the stream and returns;
+ * pgvd_build_octree_finish waits and reports the sizes.  With float points
+ * the sort is a bucket sort whose levels the host decides from small counts
+ * the device reports mid-build: the call then returns once the device has
+ * reached the sort's last level (the work queued behind keeps it busy), not
+ * at once, unless opts->no_host_wait. */
 int pgvd_build_octree(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
                       const uint32_t *labels, const pgvd_build_opts *opts, pgvd_stream s);
 

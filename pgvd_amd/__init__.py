@@ -63,7 +63,7 @@ class OctreeResult(C.Structure):
 class BuildOpts(C.Structure):
     _fields_ = [
         ("use_custom_bb", C.c_int), ("bb_min", C.c_float * 3), ("bb_max", C.c_float * 3),
-        ("clamped", C.c_int), ("quantized_input", C.c_int),
+        ("clamped", C.c_int), ("quantized_input", C.c_int), ("no_host_wait", C.c_int),
     ]
 
 
@@ -601,8 +601,10 @@ class Device:
 
     def build_octree_async(self, pts: DeviceBuffer, n: int, dim: int, bits: int,
                            labels: DeviceBuffer | None = None, custom_bb=None,
-                           clamped: bool = False, quantized_input: bool = False):
+                           clamped: bool = False, quantized_input: bool = False,
+                           no_host_wait: bool = False):
         opts = BuildOpts()
+        opts.no_host_wait = int(no_host_wait)
         if custom_bb is not None:
             opts.use_custom_bb = 1
             for k in range(dim):
@@ -649,7 +651,7 @@ class Device:
         return out
 
     def build_octree(self, pts: np.ndarray, bits: int, labels=None, custom_bb=None,
-                     clamped: bool = False) -> dict:
+                     clamped: bool = False, no_host_wait: bool = False) -> dict:
         """Whole path from host points (float n x dim, or int32 grid points)."""
         pts = np.ascontiguousarray(pts)
         n, dim = pts.shape
@@ -659,7 +661,7 @@ class Device:
         dp = self.upload(pts)
         dl = self.upload(np.ascontiguousarray(labels, np.uint32)) if labels is not None else None
         try:
-            self.build_octree_async(dp, n, dim, bits, dl, custom_bb, clamped, quant)
+            self.build_octree_async(dp, n, dim, bits, dl, custom_bb, clamped, quant, no_host_wait)
             r = self.build_octree_finish()
             return self.fetch(r)
         finally:

@@ -99,7 +99,7 @@ ZeroLayout zero_layout(int64_t n, int kb, int passes, bool labels) {
   z.cbits = off;
   if (labels) off += align256((size_t)((n + 1 + 31) / 32) * 4);
   z.msd_hist = off;
-  off += align256((size_t)kMsdMaxBuckets * 4);
+  off += align256((size_t)kBkMaxBuckets * 4);
   // the sort passes' look-back words last: beside the bucket sort they are
   // zeroed only if the passes run (launch_histogram), not at the start
   z.sort_lb = off;
@@ -124,7 +124,8 @@ size_t pgvd_capi::plain_footprint_bytes(int64_t n, int dim, int bits, bool label
   b += (size_t)nk * node;                                // brt
   b += 3 * (size_t)nk * 4;                               // ps, parent, defer
   b += kBboxPartBytes;                                   // bbox_part
-  b += ((size_t)kMsdMaxBuckets + 1 + kMsdCursorWords) * 4;   // msd_start, msd_cursor
+  b += kb == 4 ? bucket_ws_bytes<uint32_t>(n)            // the bucket sort's workspace
+                : (kb == 8 ? bucket_ws_bytes<uint64_t>(n) : bucket_ws_bytes<Key128>(n));
   b += (size_t)((nk + kBlock - 1) / kBlock) * 4;         // chunks
   b += zero_layout(n, kb, (mbits + 7) / 8, labels).total;
   b += (size_t)oct_nodes * ((1u << dim) + 1) * 4;        // octree
@@ -196,8 +197,8 @@ struct BuildBufs {
   char *zb = nullptr;                      // the block zeroed at the build's start
   ZeroLayout z;
   float *bbox_part = nullptr;
-  uint32_t *msd_start = nullptr, *msd_cursor = nullptr;
-  int msd_b = 0;                           // bucket-sort bits (0: LSD passes only)
+  int bk_b = -1;                           // bucket sort's level-0 bits (-1: LSD passes)
+  BucketMem bk;                            // the bucket sort's workspace
 };
 
 // What a build leaves for the caller (the unique keys, the packed local
@@ -213,7 +214,7 @@ struct BuildOut {
 // context's keyed pool, or the allocation-free entry's static workspace.
 template <typename K, typename Get>
 int carve_build(Get &&get, const void *pts, int64_t n, int dim, int bits, bool lab,
-                const pgvd_build_opts *opts, BuildBufs<K> &b) {
+                const pgvd_build_opts *opts, bool bucket_ok, BuildBufs<K> &b) {
   const int mbits = dim * bits;
   const int kb = (int)sizeof(K);
   const int passes = (mbits + 7) / 8;
@@ -243,26 +244,26 @@ int carve_build(Get &&get, const void *pts, int64_t n, int dim, int bits, bool l
   b.defer = static_cast<uint32_t *>(p);
   CHK(get("chunks", (size_t)((nk + kBlock - 1) / kBlock) * 4, &p));
   b.chunks = static_cast<uint32_t *>(p);
-  b.z = zero_layout(n, kb, passes, lab);
-  CHK(get("zero", b.z.total, &p));
-  b.zb = static_cast<char *>(p);
   const bool quantized = opts && opts->quantized_input;
   const bool custom = opts && opts->use_custom_bb;
+  // float points (vectorised encode, which builds the bucket histogram) on
+  // a context (the sort's levels are decided on the host): bucket sort;
+  // otherwise the LSD passes
+  b.bk_b = -1;
+  if (bucket_ok && !quantized && encode_vectorised(static_cast<const float *>(pts), b.kA))
+    b.bk_b = std::min(bucket_bits_for<K>(n), std::min(mbits, kEncodeMaxBucketBits));
+  // the LSD passes' look-back words stay for keys of at most 64 bits (the
+  // bucket sort hands clustered keys to the passes)
+  b.z = zero_layout(n, kb, (b.bk_b >= 0 && kb == 16) ? 0 : passes, lab);
+  CHK(get("zero", b.z.total, &p));
+  b.zb = static_cast<char *>(p);
   if (!quantized && !custom) {
     CHK(get("bbox_part", kBboxPartBytes, &p));
     b.bbox_part = static_cast<float *>(p);
   }
-  // 64-bit keys of a large build without labels: bucket sort, with the LSD
-  // passes launched beside it for inputs whose buckets overflow
-  b.msd_b = 0;
-  if (!quantized && std::is_same<K, uint64_t>::value && !lab &&
-      encode_vectorised(static_cast<const float *>(pts), b.kA))
-    b.msd_b = msd_bits_for(n);
-  if (b.msd_b) {
-    CHK(get("msd_start", ((size_t)kMsdMaxBuckets + 1) * 4, &p));
-    b.msd_start = static_cast<uint32_t *>(p);
-    CHK(get("msd_cursor", (size_t)kMsdCursorWords * 4, &p));
-    b.msd_cursor = static_cast<uint32_t *>(p);
+  if (b.bk_b >= 0) {
+    CHK(get("bucket_ws", bucket_ws_bytes<K>(n), &p));
+    bucket_carve<K>(p, n, b.bk);
   }
   return PGVD_OK;
 }
@@ -290,12 +291,12 @@ int build_launch(pgvd_ctx *ctx, const BuildBufs<K> &b, RunState *st, int32_t *oc
   uint64_t *lb_b = reinterpret_cast<uint64_t *>(zb + z.lb_brt);
   uint32_t *cbits = lab ? reinterpret_cast<uint32_t *>(zb + z.cbits) : nullptr;
   uint32_t *msd_hist = reinterpret_cast<uint32_t *>(zb + z.msd_hist);
-  const int msd_b = b.msd_b;
+  const int bk_b = b.bk_b;
 
   const bool quantized = opts && opts->quantized_input;
   const bool custom = opts && opts->use_custom_bb;
   const bool clamped = opts && opts->clamped;
-  const bool msd_first = msd_b != 0;
+  const bool bucket = bk_b >= 0;
   // the build's start (state reset, workspace zeroed) rides on the bbox pass
   // when there is one
   const bool bbox_pass = !quantized && !custom;
@@ -311,48 +312,54 @@ int build_launch(pgvd_ctx *ctx, const BuildBufs<K> &b, RunState *st, int32_t *oc
     if (bbox_pass) {
       Stage t(ctx, "bbox", s);
       launch_bbox(static_cast<const float *>(pts), n, dim, b.bbox_part, st, s,
-                  reinterpret_cast<uint32_t *>(zb), (msd_first ? z.sort_lb : z.total) / 4,
+                  reinterpret_cast<uint32_t *>(zb), (bucket ? z.sort_lb : z.total) / 4,
                   true);
     }
     const int64_t width = (int64_t)1 << bits;
     const float effw = (float)(clamped ? width : width - 1);
     Stage t(ctx, "encode", s);
-    if (msd_b)
+    if (bucket)
       launch_encode<K>(static_cast<const float *>(pts), n, dim, bits, effw, custom ? 1 : 0,
                        custom ? opts->bb_min : nullptr, custom ? opts->bb_max : nullptr, kA,
-                       nullptr, hist, 0, st, s, msd_hist, mbits - msd_b, 1 << msd_b);
+                       nullptr, hist, 0, st, s, msd_hist, mbits - bk_b, 1 << bk_b);
     else
       launch_encode<K>(static_cast<const float *>(pts), n, dim, bits, effw, custom ? 1 : 0,
                        custom ? opts->bb_min : nullptr, custom ? opts->bb_max : nullptr, kA,
                        nullptr, hist, passes, st, s);
   }
   CHK(launch_status("encode"));
-  int parity;
-  if constexpr (std::is_same<K, uint64_t>::value) {
-    if (msd_b) {
-      // the result lands where the LSD passes would leave it
-      Stage t(ctx, "msd", s);
-      launch_msd_sort(kA, kB, (passes & 1) ? kB : kA, prepend, n, mbits, msd_b, msd_hist,
-                      b.msd_start, b.msd_cursor, st, s);
-      launch_histogram<K>(kA, n, hist, passes, s, st, sort_lb, (z.total - z.sort_lb) / 4);
-    }
+  int parity = 0;
+  bool bucket_sorted = false;
+  if (bucket) {
+    // the bucket sort leaves keys and labels in (kA, vA)
+    Stage t(ctx, "msd", s);
+    BucketArgs<K> ba{kA, kB, b.vA, b.vB, labels, msd_hist, n, mbits, bk_b, prepend, b.bk};
+    const int rc = launch_bucket_sort<K>(ba, st, ctx->st_host, ctx->sync_ev, s);
+    if (rc == 2) return fail(PGVD_ERR_INTERNAL, "bucket sort: a segment outlived its levels");
+    if (rc == 1) return launch_status("bucket sort") != PGVD_OK ? PGVD_ERR_HIP
+                                                                  : fail(PGVD_ERR_HIP, "bucket sort");
+    bucket_sorted = rc == 0;
+    // rc 3: clustered keys of at most 64 bits; the encoded keys are untouched
+    if (!bucket_sorted)
+      launch_histogram<K>(kA, n, hist, passes, s, nullptr, sort_lb, (z.total - z.sort_lb) / 4);
   }
-  {
+  if (!bucket_sorted) {
     Stage span(ctx, "sort", s, 2);
     PassTimer pt(ctx, n > 1 ? passes : 0);
     parity = launch_onesweep<K>(kA, kB, labels, b.vA, b.vB, n, passes, hist, sort_lb, st, s,
-                                pt.events(), msd_b ? st : nullptr);
+                                pt.events(), nullptr);
   }
-  CHK(launch_status("onesweep"));
+  CHK(launch_status("sort"));
   K *sorted = parity ? kB : kA;
   K *uniq = sorted - prepend;   // in place (see unique_tile)
   // packed local splits go to the sort's other buffer, free after the sort
   uint32_t *lsx = reinterpret_cast<uint32_t *>(parity ? kA : kB);
-  const uint32_t *sortedv = lab ? (n <= 1 ? labels : (parity ? b.vB : b.vA)) : nullptr;
+  const uint32_t *sortedv =
+      lab ? (bucket_sorted ? b.vA : (n <= 1 ? labels : (parity ? b.vB : b.vA))) : nullptr;
   {
     Stage t(ctx, "unique", s);
     launch_unique<K>(sorted, sortedv, n, prepend, uniq, cbits, lb_u, st, s,
-                     msd_b ? st : nullptr);
+                     bucket_sorted ? st : nullptr);
   }
   CHK(launch_status("unique"));
   {
@@ -401,7 +408,7 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   BuildBufs<K> b;
   CHK(carve_build<K>([ctx](const char *key, size_t bytes, void **p) {
         return pool_get(ctx, key, bytes, p);
-      }, pts, n, dim, bits, lab, opts, b));
+      }, pts, n, dim, bits, lab, opts, !(opts && opts->no_host_wait), b));
   int32_t *octree;
   uint32_t *nconf;
   int64_t want = 2 * (n + 1);
@@ -611,7 +618,8 @@ int noalloc_impl(char *base, const void *pts, int64_t n, int dim, int bits,
   CHK(get("state", sizeof(RunState), &p));
   RunState *st = static_cast<RunState *>(p);
   BuildBufs<K> b;
-  CHK(carve_build<K>(get, pts, n, dim, bits, labels != nullptr, opts, b));
+  // (no host waits here: the LSD passes sort)
+  CHK(carve_build<K>(get, pts, n, dim, bits, labels != nullptr, opts, false, b));
   if (off > kNoAllocScratchBytes)
     return fail(PGVD_ERR_TOO_LARGE,
                 "pgvd_build_octree_noalloc: this build needs " + std::to_string(off) +
@@ -649,7 +657,8 @@ int pgvd_ctx_create(int device, pgvd_ctx **out) {
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
       hipMalloc(&ctx->st, sizeof(RunState)) != hipSuccess ||
       hipHostMalloc(&ctx->st_host, sizeof(RunState), hipHostMallocDefault) != hipSuccess ||
-      hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->sync_ev, hipEventDisableTiming) != hipSuccess) {
     rc = fail(PGVD_ERR_HIP, "context setup failed");
   }
   if (rc != PGVD_OK) {
@@ -671,6 +680,7 @@ int pgvd_ctx_destroy(pgvd_ctx *ctx) {
   for (auto &r : ctx->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->spare) (void)hipEventDestroy(e);
   if (ctx->done) (void)hipEventDestroy(ctx->done);
+  if (ctx->sync_ev) (void)hipEventDestroy(ctx->sync_ev);
   if (ctx->st) (void)hipFree(ctx->st);
   if (ctx->st_host) (void)hipHostFree(ctx->st_host);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -90,6 +90,7 @@ struct pgvd_ctx {
   pgvd::RunState *st = nullptr;       // device
   pgvd::RunState *st_host = nullptr;  // pinned mirror
   hipEvent_t done = nullptr;
+  hipEvent_t sync_ev = nullptr;       // mid-build waits of the bucket sort
   int timing = 0;               // 0 off, 1 every stage, 2 sort span only
   std::vector<TimingRec> recs;
   std::vector<hipEvent_t> spare;

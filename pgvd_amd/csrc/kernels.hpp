@@ -40,9 +40,13 @@ struct RunState {
   uint32_t malformed;        // 1 if some BRT node took an FP32-quirk split
   uint32_t m;                // unique key count (incl. the padding zero)
   uint32_t octree_size;      // O = ps[m-1]
-  uint32_t msd_ok;           // 1: the bucket sort (msd.hip) sorted the keys
+  uint32_t msd_ok;           // 1: the bucket sort (bucket.hip) sorted the keys
   uint32_t msd_dups;         // 1: the bucket sort saw equal keys (or a zero key to merge)
   uint32_t counters[kNumCounters];
+  // bucket sort (bucket.hip): the current level's segments and their keys,
+  // the next level's, the LDS sort items so far, the current level's tiles
+  uint32_t bk_segs, bk_keys, bk_next_segs, bk_next_keys, bk_items, bk_tiles;
+  uint32_t bk_lsd;           // 1: too clustered, the LSD passes sort instead
 };
 
 // Fixed-layout BRT node in device memory (the reference's BrtNode,
@@ -216,30 +220,71 @@ void launch_format_text(const int32_t *octree, int64_t O, int dim, int hdr_len, 
 void launch_parse_text(const uint8_t *text, int64_t len, int dim, int64_t N, int32_t *octree,
                        uint64_t *ws, hipStream_t s);
 
-// ---- msd.hip ----
-// Bucket sort of 64-bit keys without payload: scatter by the top b bits,
-// then each bucket sorted in one workgroup's LDS.  Active only when every
-// bucket holds at most kMsdCap keys (then st->msd_ok = 1); the caller runs
-// the LSD passes beside it with skip_if_msd = st.
-constexpr int kMsdCap = 5120;
-constexpr int kMsdMaxBuckets = 4096;
-// Words between the first scatter's cursors (one per L2 line: the 4096-odd
-// scatter tiles all claim on these few words).
-#ifndef PGVD_MSD_CSTRIDE
-#define PGVD_MSD_CSTRIDE 32
-#endif
-constexpr int kMsdCStride = PGVD_MSD_CSTRIDE;
-constexpr int kMsdCursorWords = kMsdMaxBuckets + 64 * kMsdCStride + 64 + 1;
-// b for n keys (0: no bucket sort for this size).
-int msd_bits_for(int64_t n);
-// keys -> alt -> keys (two scatters), then every bucket sorted into out
-// (may be keys); st->msd_dups = 1 when unique has work to do (equal keys,
-// or a zero key while a zero key is prepended).  hist: the bucket
-// histogram (2^b counters, from launch_encode); start 2^b + 1 and cursor
-// kMsdCursorWords words of scratch.
-void launch_msd_sort(uint64_t *keys, uint64_t *alt, uint64_t *out, int prepend, int64_t n,
-                     int mbits, int b, const uint32_t *hist, uint32_t *start, uint32_t *cursor,
-                     RunState *st, hipStream_t s);
+// ---- bucket.hip ----
+// The whole path's sort for 32/64/128-bit keys with optional u32 labels:
+// level 0 splits by the top b bits (b <= kBkMaxBits) and sorts every bucket
+// of at most bucket_cap<K>() keys in LDS; larger buckets go through levels
+// that split them below their highest varying bit.  The result (keys, and
+// labels in the same order) is in (kA, vA); not stable among equal keys.
+constexpr int kBkMaxBits = 14;
+constexpr int kBkMaxBuckets = 1 << kBkMaxBits;
+constexpr int kBkCursor1 = kBkMaxBuckets;                       // first scatter's cursors
+constexpr int kBkTileStart = kBkMaxBuckets + 128 * 32;          // second scatter's tile starts
+                                                                // (after 128 cursors 32 words apart)
+constexpr int kBkCursorWords = kBkTileStart + 128 + 1;
+// A segment left to a level: keys [start, start + count) of one buffer,
+// all sharing the bits above their highest varying bit (OR ^ AND).
+struct BkSeg {
+  uint32_t start, count;
+  uint32_t tile_base;   // first tile of the level's scatter
+  uint32_t shift;       // digit = (key >> shift) & (2^nbits - 1)
+  uint32_t nbits;       // 0: every key equal (copied)
+  uint32_t pad_[3];
+  unsigned long long or_lo, or_hi, and_lo, and_hi;
+};
+// An LDS sort item: keys [start, start + count) of buffer `parity` (0: A,
+// 1: B), all sharing their bits from `sbits` up (sbits 0: all equal, copied).
+struct BkItem {
+  uint32_t start, count, parity, sbits;
+};
+struct BucketMem {
+  uint32_t *start = nullptr, *cursor = nullptr;
+  BkSeg *segs[2] = {nullptr, nullptr};
+  uint32_t *hrows[2] = {nullptr, nullptr};
+  uint32_t *lcursor = nullptr;
+  BkItem *items = nullptr;
+  uint32_t max_segs = 0, max_items = 0;
+};
+template <typename K>
+struct BucketArgs {
+  K *kA, *kB;                 // keys in kA (encode's output); kB the other buffer
+  uint32_t *vA, *vB;          // label buffers (with labels)
+  const uint32_t *labels;     // the keys' labels in input order (nullable)
+  const uint32_t *hist;       // histogram of the top b bits (2^b counters)
+  int64_t n;
+  int mbits, b, prepend;
+  BucketMem mem;
+};
+inline constexpr size_t align256_dev(size_t x) { return (x + 255) & ~(size_t)255; }
+template <typename K> int bucket_cap();
+template <typename K> int bucket_bits_for(int64_t n);
+template <typename K> size_t bucket_ws_bytes(int64_t n);
+template <typename K> void bucket_carve(void *ws, int64_t n, BucketMem &m);
+// Enqueues the sort on s.  Waits on the host (ev, st_host: pinned) for the
+// level-0 plan and each level's scan while the device works on.  0: sorted;
+// 3: keys of at most 64 bits too clustered (a quarter of them in buckets
+// above CAP): nothing moved, the caller sorts by the LSD passes; else a
+// launch failed (1) or a segment outlived the levels (2).
+template <typename K>
+int launch_bucket_sort(const BucketArgs<K> &a, RunState *st, RunState *st_host, hipEvent_t ev,
+                       hipStream_t s);
+
+// Bucket histogram bins the vectorised encode kernel holds in LDS (it
+// shares them with the LSD digit histograms: kMaxPasses x 256).
+constexpr int kEncodeMaxBucketBits = 12;
+// Words between the first scatter's cursors (one per L2 line: every scatter
+// tile claims on these few words).
+constexpr int kBkCStride = 32;
 int64_t onesweep_tiles(int key_bytes, bool payload, int64_t n);
 int onesweep_tile_size(int key_bytes, bool payload);
 

@@ -28,6 +28,9 @@ __device__ inline void prep_run(RunState *st, uint4 *zero, size_t nzero16) {
     for (int k = 0; k < 3; ++k) { st->bb_min[k] = 0xFFFFFFFFu; st->bb_max[k] = 0u; }
     st->status = 0; st->malformed = 0; st->m = 0; st->octree_size = 0; st->msd_ok = 0;
     for (int k = 0; k < kNumCounters; ++k) st->counters[k] = 0;
+    st->msd_dups = 0;
+    st->bk_segs = 0; st->bk_keys = 0; st->bk_next_segs = 0; st->bk_next_keys = 0;
+    st->bk_items = 0; st->bk_tiles = 0; st->bk_lsd = 0;
   }
   const uint4 z = make_uint4(0, 0, 0, 0);
   for (size_t i = tid; i < nzero16; i += (size_t)gridDim.x * blockDim.x) zero[i] = z;
@@ -293,7 +296,8 @@ __global__ __launch_bounds__(kBlock) void encode_vec_kernel(
   // One LDS histogram serves either the LSD digit counts (passes > 0) or the
   // bucket counts (msd_hist, built with passes = 0): 16 KB instead of 32 KB
   // keeps eight workgroups per CU.
-  static_assert(kMaxPasses * 256 == kMsdMaxBuckets, "the two histograms share their LDS");
+  static_assert(kMaxPasses * 256 == (1 << kEncodeMaxBucketBits),
+                "the two histograms share their LDS");
   __shared__ uint32_t sh[kMaxPasses * 256];
   uint32_t *s_msd = sh;
   if (msd_hist) passes = 0;
@@ -596,7 +600,7 @@ void launch_encode(const float *pts, int64_t n, int dim, int bits, float effw, i
     for (int k = 0; k < dim; ++k) { a[k] = bbmin[k]; b[k] = bbmax[k]; }
   if (keys && hist && !qout && encode_vectorised(pts, keys)) {
     const int vgrid = grid_for(n / 4 + 1, kBlock);
-    if (msd_nb > kMsdMaxBuckets) msd_hist = nullptr;
+    if (msd_nb > (1 << kEncodeMaxBucketBits)) msd_hist = nullptr;
     if (dim == 2)
       encode_vec_kernel<K, 2><<<vgrid, kBlock, 0, s>>>(pts, n, bits, effw, custom_bb, a[0], a[1],
                                                        a[2], b[0], b[1], b[2], keys, hist, passes,

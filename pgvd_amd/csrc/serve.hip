@@ -65,7 +65,11 @@ int admit(pgvd_server *srv, Slot &sl, pgvd_request &r) {
     sl.pts_bytes = pb + pb / 8;
   }
   HIPCHK(hipMemcpyAsync(sl.d_pts, r.host_pts, pb, hipMemcpyHostToDevice, ctx->stream));
-  CHK(pgvd_build_octree(ctx, sl.d_pts, r.n, r.dim, r.bits, nullptr, nullptr, nullptr));
+  // one thread enqueues every slot's builds: none may wait for the device
+  pgvd_build_opts o;
+  std::memset(&o, 0, sizeof(o));
+  o.no_host_wait = 1;
+  CHK(pgvd_build_octree(ctx, sl.d_pts, r.n, r.dim, r.bits, nullptr, &o, nullptr));
   // digest of the octree as built; the node count is read on the device
   const int stride = (1 << r.dim) + 1;
   launch_digest_count(ctx->pool["octree"].p, &ctx->st->octree_size, (uint32_t)stride,

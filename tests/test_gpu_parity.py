@@ -165,11 +165,12 @@ def test_full_size_3d_digests(dev, golden_digests, name):
     assert got["parent"][0] == -1
 
 
-def test_bucket_sort_and_its_fallback(dev):
-    """Builds of 2^23 or more 64-bit keys scatter by the top key bits into
-    buckets that each sort inside one workgroup; when any bucket is too full
-    the LSD radix passes sort instead.  Both paths against the oracle at a
-    size that is not a power of two."""
+def test_bucket_sort_large_and_clustered(dev):
+    """A build of 2^23 + 4097 64-bit keys: the keys scatter by their top bits
+    into buckets that each sort inside one workgroup; buckets that are too
+    full (points near the centre) are split further below their highest
+    varying bit, or, when they hold more than a quarter of the keys, the LSD
+    passes sort.  Against the oracle at a size that is not a power of two."""
     n = (1 << 23) + 4097
     pts = _pts(n, 3, seed=23)
     got = dev.build_octree(pts, 21)
@@ -190,13 +191,16 @@ def test_bucket_sort_and_its_fallback(dev):
     got = dev.build_octree(corner, 21)
     assert got["bucket_sort"] == 1 and got["keys_lo"][0] == 0
     assert_tree_equal(got, oracle.build(oracle.quantize(corner, 1 << 21), 21), where="zero key")
-    clustered = pts.copy()
-    clustered[: n // 4] *= np.float32(1e-3)         # a quarter of the points near the centre
-    got = dev.build_octree(clustered, 21)
-    assert got["bucket_sort"] == 0
-    assert_tree_equal(got, oracle.build(oracle.quantize(clustered, 1 << 21), 21),
-                      where="fallback")
-    assert (got["keys_lo"][1:] > got["keys_lo"][:-1]).all()
+    # an eighth of the points near the centre: their buckets go through the
+    # levels; half of them: the LSD passes sort instead
+    for frac, bucket in ((8, 1), (2, 0)):
+        clustered = pts.copy()
+        clustered[: n // frac] *= np.float32(1e-3)
+        got = dev.build_octree(clustered, 21)
+        assert got["bucket_sort"] == bucket, frac
+        assert_tree_equal(got, oracle.build(oracle.quantize(clustered, 1 << 21), 21),
+                          where=f"clustered 1/{frac}")
+        assert (got["keys_lo"][1:] > got["keys_lo"][:-1]).all()
 
 
 @pytest.mark.parametrize("dim,bits", [(2, 24), (3, 13)])
