@@ -683,6 +683,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=0, help="points (default 2^24; 2^22 for conflict)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="uniform workload only: skip the Gaussian and conflict-cell records")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="only run the plain copy kernel over MiB mebibytes (5 launches), "
                          "the reference point for HBM byte counters")
@@ -720,17 +722,64 @@ def main() -> int:
         dev.close()
         return 0
 
-    conflict = args.workload == "conflict"
+    out = measure_build(P, args.workload, args.n, args.steps, args.warmup, cpu=not args.no_cpu,
+                        cpu_sample=args.cpu_sample)
+    if args.workload == "uniform" and not args.no_extra:
+        # the configurations the headline hides, measured in the same run:
+        # the Gaussian mixture (~28% duplicate keys) and the 126-bit
+        # conflict-cell build with labels
+        extra = {}
+        for w in ("gauss", "conflict"):
+            r = measure_build(P, w, 0, args.steps, args.warmup, cpu=False, cpu_sample=0)
+            extra[w] = {
+                "workload": r["config"]["workload"],
+                "ms_per_step": r["ms_per_step"],
+                "value": r["value"], "unit": "keys/s",
+                "unique_keys": r["config"]["unique_keys"],
+                "octree_nodes": r["config"]["octree_nodes"],
+                "sort": r["config"]["sort"],
+                "frac_of_hbm_peak": r["pipeline"]["frac_of_hbm_peak"],
+                "counter_GBps": r["pipeline"]["counter_GBps"],
+                "dominant_kernel": r["roofline"]["kernel"],
+                "dominant_frac": r["roofline"]["frac"],
+            }
+        out["other_configs"] = extra
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def counter_bytes_per_step(workload: str):
+    """HBM bytes one build moves by the committed rocprofv3 counters: the
+    per-launch FETCH+WRITE bytes of every kernel times its launches per
+    build (profiles/traffic.json), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(workload, {})
+    except (OSError, ValueError):
+        return None
+    per = rec.get("kernels_bytes_per_launch")
+    if not per:
+        return None
+    launches = rec.get("launches_per_build", {})
+    return int(sum(v * launches.get(k, 1) for k, v in per.items()))
+
+
+def measure_build(P, workload: str, n: int, steps: int, warmup: int, cpu: bool,
+                  cpu_sample: int) -> dict:
+    """One bench record of the whole path on a synthetic workload (uniform,
+    gauss or conflict)."""
+    conflict = workload == "conflict"
     dim, bits = 3, (42 if conflict else 21)
-    n = args.n or (1 << 22 if conflict else 1 << 24)
-    pts, desc = make_points(args.workload, n)
+    n = n or (1 << 22 if conflict else 1 << 24)
+    pts, desc = make_points(workload, n)
     labels = make_points.labels if conflict else None
     dev = P.Device(0)
     dpts = dev.upload(pts)
     dlab = dev.upload(labels) if conflict else None
 
     res = None
-    for _ in range(max(args.warmup, 1)):
+    for _ in range(max(warmup, 1)):
         dev.build_octree_async(dpts, n, dim, bits, dlab)
         res = dev.build_octree_finish()
     bucket = bool(res.bucket_sort)
@@ -744,12 +793,12 @@ def main() -> int:
     dev.timing_reset()
     dev.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         dev.build_octree_async(dpts, n, dim, bits, dlab)
     res = dev.build_octree_finish()
     dev.synchronize()
     elapsed = time.perf_counter() - t0
-    ms_step = elapsed * 1e3 / args.steps
+    ms_step = elapsed * 1e3 / steps
     if bucket:
         dom_ms, dom_cnt = dev.timing_read("gather")
         launch_ms, launches = dom_ms / max(dom_cnt, 1), dom_cnt
@@ -783,7 +832,8 @@ def main() -> int:
     dom_key = "gather" if bucket else "sort_pass"
     dom_gbps = ab[dom_key] / (launch_ms * 1e-3) / 1e9
     total_gbps = ab["total"] / (ms_step * 1e-3) / 1e9
-    traffic = load_traffic(dom_key, args.workload)
+    traffic = load_traffic(dom_key, workload)
+    counted = counter_bytes_per_step(workload)
     if bucket:
         dom_name = (f"gather_kernel (scan of local splits and octree fill: {m} BRT nodes into "
                     f"{O} octree nodes; algorithmic bytes {ab['gather']})")
@@ -799,8 +849,8 @@ def main() -> int:
         "value": n / (ms_step * 1e-3),
         "unit": "keys/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -810,7 +860,7 @@ def main() -> int:
         "config": {
             "workload": f"{desc}, {bits} bits/axis, {dim * bits}-bit keys; full path "
                         "points-in-HBM to octree-in-HBM",
-            "n_points": n, "dim": dim, "bits": bits, "distribution": args.workload,
+            "n_points": n, "dim": dim, "bits": bits, "distribution": workload,
             "unique_keys": m, "octree_nodes": O, "malformed_brt": int(res.malformed),
             "sort": "bucket sort (2 scatters + in-LDS buckets)" if bucket else "LSD onesweep",
             **({"labels": "uint32 loop id per point", "flags": flagged} if conflict else {}),
@@ -831,15 +881,18 @@ def main() -> int:
             "byte_model": {k: v for k, v in ab.items() if k not in ("total", "sort_pass", "gather")},
             "achieved_GBps": total_gbps,
             "frac_of_hbm_peak": total_gbps / HBM_PEAK_GBPS,
+            # HBM bytes the kernels actually move (committed counter
+            # summary) over the measured step time, beside the model's rate
+            "counter_bytes_per_step": counted,
+            "counter_GBps": (counted / (ms_step * 1e-3) / 1e9) if counted else None,
             "stage_ms_per_step": {k: v[0] / prof_steps for k, v in stage_ms.items()},
             "stage_note": "separate steps with an event pair around every stage",
         },
     }
     dev.close()
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pts, bits, min(args.cpu_sample, n), labels)
-    print(json.dumps(out), flush=True)
-    return 0
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline(pts, bits, min(cpu_sample, n), labels)
+    return out
 
 
 if __name__ == "__main__":
