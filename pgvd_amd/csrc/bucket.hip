@@ -413,7 +413,16 @@ __device__ inline void lds_radix(K (&k)[IPT], uint32_t (&v)[IPT], int passes, in
 
 constexpr int kSubBits = 12;                 // sub-buckets of the counting pass
 constexpr int kSubBins = 1 << kSubBits;
-constexpr uint32_t kSubMaxGroup = 32;        // largest sub-bucket ranked in place
+// Largest sub-bucket ranked in place (rank = keys below it in the
+// sub-bucket, one LDS read each): level-0 buckets of uniform keys hold ~1
+// key per sub-bucket; the levels' items hold keys on curves and surfaces,
+// whose top 12 varying bits leave larger sub-buckets, for which ranking
+// stays cheaper than LSD passes over every varying bit.
+#ifndef PGVD_BK_RANK_MAX
+#define PGVD_BK_RANK_MAX 128
+#endif
+constexpr uint32_t kSubMaxGroup = 32;
+constexpr uint32_t kSubMaxGroupList = PGVD_BK_RANK_MAX;
 
 template <typename K, bool LAB>
 struct SortArgs {
@@ -431,9 +440,10 @@ struct SortArgs {
 // One item of at most CAP keys per workgroup, sorted into out at its own
 // positions.  Level-0 mode (list == 0): item b is bucket b, its keys sharing
 // their bits from sbits0 = mbits - b up (buckets above CAP are left to the
-// levels).  List mode: item blockIdx.x of items[] (count st->bk_items).  An
-// item with sbits 0 holds equal keys and is copied.  Otherwise a counting
-// pass on the 12 bits below sbits, then each key's rank within its
+// levels).  List mode: item blockIdx.x of items[] (count st->bk_items), its
+// sbits lowered to the bits that vary among its keys.  An item with sbits 0
+// holds equal keys and is copied.  Otherwise a counting pass on the 12 bits
+// below sbits, then each key's rank within its
 // sub-bucket -- or, when a sub-bucket holds more than 32 keys, LSD passes in
 // LDS over the bits below sbits.  Equal
 // keys share an item, so the workgroup sees every duplicate: it raises
@@ -452,6 +462,7 @@ bk_sort_kernel(SortArgs<K, LAB> a, int list, int sbits0, RunState *st) {
   __shared__ uint32_t s_vals[LAB ? CAP : 1];
   __shared__ uint32_t s_sub[kSubBins + 1];   // counts, then starts; radix counters
   __shared__ uint32_t s_dig[256], s_w[NW], s_mx[NW];
+  __shared__ unsigned long long s_red[MATCH ? NW : 1][4];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   uint32_t s0, cnt;
   int sbits;
@@ -477,27 +488,7 @@ bk_sort_kernel(SortArgs<K, LAB> a, int list, int sbits0, RunState *st) {
     vin = a.v0;
   }
   if (cnt == 0) return;
-  if (sbits == 0) {
-    // equal keys: copied as they are
-    bool dup = false;
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const uint32_t i = j * BS + t;
-      if (i < cnt) {
-        const K x = in[s0 + i];
-        dup |= i > 0 || (s0 == 0 && a.prepend && kzero(x));
-        a.out[s0 + i] = x;
-        if (LAB) a.vout[s0 + i] = vin[s0 + i];
-      }
-    }
-    if (__any(dup) && lane == 0)
-      __hip_atomic_store(&st->msd_dups, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return;
-  }
   for (int i = t; i < kSubBins; i += BS) s_sub[i] = 0;
-  __syncthreads();
-  const int sub_shift = sbits > kSubBits ? sbits - kSubBits : 0;
-  const uint32_t sub_mask = (1u << (sbits - sub_shift)) - 1u;
   K k[IPT];
   uint32_t v[IPT];
   uint32_t rs[IPT];   // slot in sub-bucket | sub-bucket << 16
@@ -507,6 +498,54 @@ bk_sort_kernel(SortArgs<K, LAB> a, int list, int sbits0, RunState *st) {
     k[j] = i < cnt ? in[s0 + i] : kzero_value<K>();
     v[j] = (LAB && i < cnt) ? vin[s0 + i] : 0u;
   }
+  if constexpr (MATCH) {
+    // items of the levels: sort on the bits that vary in the item (OR ^
+    // AND of its keys), so that the counting pass splits clustered keys by
+    // their top 12 varying bits
+    uint64_t olo = 0, ohi = 0, alo = ~0ull, ahi = ~0ull;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+      if (j * BS + t < cnt) {
+        olo |= klo(k[j]); ohi |= khi(k[j]);
+        alo &= klo(k[j]); ahi &= khi(k[j]);
+      }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      olo |= (uint64_t)__shfl_xor((unsigned long long)olo, d, 64);
+      ohi |= (uint64_t)__shfl_xor((unsigned long long)ohi, d, 64);
+      alo &= (uint64_t)__shfl_xor((unsigned long long)alo, d, 64);
+      ahi &= (uint64_t)__shfl_xor((unsigned long long)ahi, d, 64);
+    }
+    if (lane == 0) {
+      s_red[w][0] = olo; s_red[w][1] = ohi; s_red[w][2] = alo; s_red[w][3] = ahi;
+    }
+    __syncthreads();
+    olo = 0; ohi = 0; alo = ~0ull; ahi = ~0ull;
+#pragma unroll
+    for (int ww = 0; ww < NW; ++ww) {
+      olo |= s_red[ww][0]; ohi |= s_red[ww][1]; alo &= s_red[ww][2]; ahi &= s_red[ww][3];
+    }
+    sbits = min(sbits, varying_bits(olo, ohi, alo, ahi));   // uniform
+  }
+  __syncthreads();
+  if (sbits == 0) {
+    // equal keys: written back as they are
+    bool dup = false;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t i = j * BS + t;
+      if (i < cnt) {
+        dup |= i > 0 || (s0 == 0 && a.prepend && kzero(k[j]));
+        a.out[s0 + i] = k[j];
+        if (LAB) a.vout[s0 + i] = v[j];
+      }
+    }
+    if (__any(dup) && lane == 0)
+      __hip_atomic_store(&st->msd_dups, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const int sub_shift = sbits > kSubBits ? sbits - kSubBits : 0;
+  const uint32_t sub_mask = (1u << (sbits - sub_shift)) - 1u;
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t i = j * BS + t;
@@ -536,7 +575,7 @@ bk_sort_kernel(SortArgs<K, LAB> a, int list, int sbits0, RunState *st) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) gmax = max(gmax, (uint32_t)__shfl_xor((int)gmax, d, 64));
   // sbits <= 12: a sub-bucket holds equal keys, the counting pass places them
-  if (sbits <= kSubBits || gmax <= kSubMaxGroup) {
+  if (sbits <= kSubBits || gmax <= (MATCH ? kSubMaxGroupList : kSubMaxGroup)) {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       s_sub[t * PER + i] = ex;
@@ -749,7 +788,7 @@ __global__ __launch_bounds__(kBlock) void bk_lscan_kernel(
     BkSeg *__restrict__ segs, const uint32_t *__restrict__ hrows, uint32_t *__restrict__ cursor,
     BkSeg *__restrict__ nsegs, uint32_t *__restrict__ nhrows, uint32_t max_segs, uint32_t cap,
     int wide, int parity, BkItem *__restrict__ items, uint32_t max_items, RunState *st) {
-  __shared__ uint32_t s_w4[4], s_c[256], s_x[256], s_nid[256];
+  __shared__ uint32_t s_w4[4], s_c[256], s_x[256], s_nid[256], s_nnew;
   const uint32_t si = blockIdx.x;
   if (si >= st->bk_segs) return;
   const int t = threadIdx.x;
@@ -790,13 +829,12 @@ __global__ __launch_bounds__(kBlock) void bk_lscan_kernel(
       nid = ~0u;
     }
   }
-  s_nid[t] = nid;
+  // the new segments' histogram rows start at zero (listed in LDS: few)
+  if (t == 0) s_nnew = 0;
   __syncthreads();
-  // the new segments' histogram rows start at zero
-  for (int d = 0; d < 256; ++d) {
-    const uint32_t id = s_nid[d];
-    if (id != ~0u) nhrows[id * 256u + t] = 0u;
-  }
+  if (nid != ~0u) s_nid[atomicAdd(&s_nnew, 1u)] = nid;
+  __syncthreads();
+  for (uint32_t k = 0; k < s_nnew; ++k) nhrows[s_nid[k] * 256u + t] = 0u;
   // items hold the keys the scatter writes to the other buffer; their keys
   // share the bits from shift + nbits up.  Thread 0 merges the digits into
   // items in LDS (at most one per digit); one atomic claims their slots.

@@ -251,7 +251,7 @@ int carve_build(Get &&get, const void *pts, int64_t n, int dim, int bits, bool l
   // otherwise the LSD passes
   b.bk_b = -1;
   if (bucket_ok && !quantized && encode_vectorised(static_cast<const float *>(pts), b.kA))
-    b.bk_b = std::min(bucket_bits_for<K>(n), std::min(mbits, kEncodeMaxBucketBits));
+    b.bk_b = std::min(bucket_bits_for<K>(n), std::min(mbits, kBkMaxBits));
   // the LSD passes' look-back words stay for keys of at most 64 bits (the
   // bucket sort hands clustered keys to the passes)
   b.z = zero_layout(n, kb, (b.bk_b >= 0 && kb == 16) ? 0 : passes, lab);

@@ -279,8 +279,9 @@ template <typename K>
 int launch_bucket_sort(const BucketArgs<K> &a, RunState *st, RunState *st_host, hipEvent_t ev,
                        hipStream_t s);
 
-// Bucket histogram bins the vectorised encode kernel holds in LDS (it
-// shares them with the LSD digit histograms: kMaxPasses x 256).
+// Bucket histogram bins the vectorised encode kernel holds in LDS beside
+// eight workgroups per CU (it shares them with the LSD digit histograms:
+// kMaxPasses x 256); up to kBkMaxBits it takes a larger LDS histogram.
 constexpr int kEncodeMaxBucketBits = 12;
 // Words between the first scatter's cursors (one per L2 line: every scatter
 // tile claims on these few words).

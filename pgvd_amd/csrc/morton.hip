@@ -287,7 +287,7 @@ __device__ inline void store4(K *__restrict__ keys, int64_t ch, const K (&k)[4])
   for (int v = 0; v < (int)(4 * sizeof(K) / 16); ++v) dst[v] = src[v];
 }
 
-template <typename K, int DIM>
+template <typename K, int DIM, int HBINS>
 __global__ __launch_bounds__(kBlock) void encode_vec_kernel(
     const float *__restrict__ pts, int64_t n, int bits, float effw, int custom, float bmin0,
     float bmin1, float bmin2, float bmax0, float bmax1, float bmax2, K *__restrict__ keys,
@@ -295,10 +295,12 @@ __global__ __launch_bounds__(kBlock) void encode_vec_kernel(
     int msd_shift, int msd_nb) {
   // One LDS histogram serves either the LSD digit counts (passes > 0) or the
   // bucket counts (msd_hist, built with passes = 0): 16 KB instead of 32 KB
-  // keeps eight workgroups per CU.
+  // keeps eight workgroups per CU.  HBINS > 4096: the bucket counts of the
+  // largest builds (up to 2^14 buckets, 64 KB).
   static_assert(kMaxPasses * 256 == (1 << kEncodeMaxBucketBits),
                 "the two histograms share their LDS");
-  __shared__ uint32_t sh[kMaxPasses * 256];
+  static_assert(HBINS >= kMaxPasses * 256, "the LDS holds the digit histograms too");
+  __shared__ uint32_t sh[HBINS];
   uint32_t *s_msd = sh;
   if (msd_hist) passes = 0;
   for (int i = threadIdx.x; i < passes * 256; i += kBlock) sh[i] = 0;
@@ -600,15 +602,26 @@ void launch_encode(const float *pts, int64_t n, int dim, int bits, float effw, i
     for (int k = 0; k < dim; ++k) { a[k] = bbmin[k]; b[k] = bbmax[k]; }
   if (keys && hist && !qout && encode_vectorised(pts, keys)) {
     const int vgrid = grid_for(n / 4 + 1, kBlock);
-    if (msd_nb > (1 << kEncodeMaxBucketBits)) msd_hist = nullptr;
-    if (dim == 2)
-      encode_vec_kernel<K, 2><<<vgrid, kBlock, 0, s>>>(pts, n, bits, effw, custom_bb, a[0], a[1],
-                                                       a[2], b[0], b[1], b[2], keys, hist, passes,
-                                                       st, msd_hist, msd_shift, msd_nb);
+    if (msd_nb > kBkMaxBuckets) msd_hist = nullptr;
+    const bool big = msd_hist && msd_nb > (1 << kEncodeMaxBucketBits);
+    if (dim == 2 && !big)
+      encode_vec_kernel<K, 2, 4096><<<vgrid, kBlock, 0, s>>>(pts, n, bits, effw, custom_bb, a[0],
+                                                             a[1], a[2], b[0], b[1], b[2], keys,
+                                                             hist, passes, st, msd_hist,
+                                                             msd_shift, msd_nb);
+    else if (dim == 2)
+      encode_vec_kernel<K, 2, kBkMaxBuckets><<<vgrid, kBlock, 0, s>>>(
+          pts, n, bits, effw, custom_bb, a[0], a[1], a[2], b[0], b[1], b[2], keys, hist, passes,
+          st, msd_hist, msd_shift, msd_nb);
+    else if (!big)
+      encode_vec_kernel<K, 3, 4096><<<vgrid, kBlock, 0, s>>>(pts, n, bits, effw, custom_bb, a[0],
+                                                             a[1], a[2], b[0], b[1], b[2], keys,
+                                                             hist, passes, st, msd_hist,
+                                                             msd_shift, msd_nb);
     else
-      encode_vec_kernel<K, 3><<<vgrid, kBlock, 0, s>>>(pts, n, bits, effw, custom_bb, a[0], a[1],
-                                                       a[2], b[0], b[1], b[2], keys, hist, passes,
-                                                       st, msd_hist, msd_shift, msd_nb);
+      encode_vec_kernel<K, 3, kBkMaxBuckets><<<vgrid, kBlock, 0, s>>>(
+          pts, n, bits, effw, custom_bb, a[0], a[1], a[2], b[0], b[1], b[2], keys, hist, passes,
+          st, msd_hist, msd_shift, msd_nb);
     return;
   }
   if (dim == 2)

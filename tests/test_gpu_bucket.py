@@ -107,3 +107,9 @@ def test_bucket_labels(dev):
     p2 = _clusters(200_000, 2, seed=14, k=4, sigma=1e-3)
     l2 = np.random.default_rng(15).integers(0, 5, p2.shape[0]).astype(np.uint32)
     _check(dev, p2, 16, labels=l2, where="labels 2D", bucket=None)
+
+
+def test_bucket_13_bit_level0(dev):
+    """20 M uniform points: 2^13 top-level buckets (the encode kernel's larger
+    bucket histogram), against the oracle."""
+    _check(dev, _uniform(20_000_000, 3, seed=25), 21, where="2^13 buckets")
