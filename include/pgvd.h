@@ -41,7 +41,8 @@ typedef enum pgvd_status {
   PGVD_ERR_CAPACITY = 7,    /* caller's output buffer too small; see *required */
   PGVD_ERR_TOO_LARGE = 8,   /* n >= 2^30 */
   PGVD_ERR_DUP_KEY = 9,     /* BRT input has equal neighbouring keys */
-  PGVD_ERR_INTERNAL = 10
+  PGVD_ERR_INTERNAL = 10,
+  PGVD_NOT_READY = 11       /* pgvd_server_poll: the request is not complete yet */
 } pgvd_status;
 
 /* Message for the last failing call on this thread. */
@@ -172,10 +173,14 @@ int pgvd_octree_format_text(pgvd_ctx *ctx, const int32_t *octree, int64_t O, int
  * by any whitespace, the leaf mask cast to unsigned char as the reference
  * reads it; tokens after the last record are ignored.  text: device, len
  * bytes; octree: device, room for cap records.  *O_out (host) receives N;
- * PGVD_ERR_CAPACITY if N > cap.  PGVD_ERR_INVALID when the count is not in
- * the first 4096 bytes, a record token is not an optional sign and 1 to 10
- * digits within int32 (where operator>> would set failbit), or the text
- * ends early.  Synchronous. */
+ * PGVD_ERR_CAPACITY if N > cap.  A token is an optional sign and digits
+ * (leading zeros skipped, as operator>> skips them) whose value is within
+ * int32.  PGVD_ERR_INVALID when the count is not in the first 4096 bytes or
+ * is negative ("-0" is zero), a record token is not such a number (where
+ * operator>> sets failbit), or the text ends early.  One difference: a
+ * token must end at whitespace, so a last record token with other bytes
+ * after its digits ("15x") is rejected where operator>> reads its numeric
+ * prefix and stops.  Synchronous. */
 int pgvd_octree_parse_text(pgvd_ctx *ctx, const char *text, int64_t len, int dim, int32_t *octree,
                            int64_t cap, int64_t *O_out, pgvd_stream s);
 
@@ -457,14 +462,51 @@ int pgvd_batch_tune(pgvd_batch *b, const void *pts, int *sort_ipt, int *brt_rows
 int pgvd_batch_last_ms(pgvd_batch *b, float *ms);
 
 /* ---- served form ---------------------------------------------------------
- * A request loop over whole-path builds of different sizes and dimensions.
- * It keeps `slots` contexts (one stream and one pooled set of buffers each,
- * device-resident across requests).  A request is admitted, in arrival
- * order, as soon as a slot is free and its device footprint fits the HBM
- * budget beside the requests in flight; the builds of admitted requests
- * overlap on the device.  No counterpart in the reference. */
+ * Octree builds requested online.  The server keeps `slots` contexts (one
+ * stream and one pooled set of buffers each, device-resident across
+ * requests) and an engine thread.  A submitted request is admitted, in
+ * arrival order, as soon as a slot is free and the device memory the server
+ * would then hold -- every slot's pool and points buffer as allocated, plus
+ * the result buffers, held or cached -- fits the HBM budget (cached result
+ * buffers, then idle slots' pools, are freed when that makes the request
+ * fit); the builds of admitted
+ * requests overlap on the device.  No counterpart in the reference. */
 typedef struct pgvd_server pgvd_server;
 
+int pgvd_server_create(int device, int slots, size_t hbm_budget_bytes, pgvd_server **out);
+/* Drops requests not yet admitted, waits for the ones on the device, and
+ * frees every result. */
+int pgvd_server_destroy(pgvd_server *s);
+
+/* Queues a build of n points (host_pts: n*dim floats in host memory, best
+ * pinned with pgvd_host_alloc, kept valid until the request is complete)
+ * and returns its ticket at once.  PGVD_ERR_CAPACITY when its footprint
+ * alone exceeds the budget. */
+int pgvd_server_submit(pgvd_server *s, const float *host_pts, int64_t n, int dim, int bits,
+                       int64_t *ticket);
+
+typedef struct pgvd_served_result {
+  int status;               /* pgvd_status of the build                          */
+  int dim, bits;
+  int slot;                 /* the slot that built it                            */
+  int64_t n, m, octree_size;
+  uint64_t octree_digest;   /* pgvd_digest of the octree records                 */
+  double arrival_s, admit_s, done_s;   /* seconds since the server was created  */
+  const int32_t *octree;    /* device: octree_size records of 2^dim + 1 int32,
+                               valid until pgvd_server_release                  */
+} pgvd_served_result;
+
+/* The result of a ticket: PGVD_OK with *out filled once the request is
+ * complete; PGVD_NOT_READY before that when wait is 0 (wait 1 blocks).
+ * The build's own outcome is out->status. */
+int pgvd_server_poll(pgvd_server *s, int64_t ticket, int wait, pgvd_served_result *out);
+/* Gives a complete request's octree back to the server, which keeps the
+ * buffer for a later result (cached buffers still count against the budget
+ * and are freed first when a request would not fit).  Call it only once the
+ * caller's own reads of the octree (on any stream) have completed. */
+int pgvd_server_release(pgvd_server *s, int64_t ticket);
+
+/* A trace replayed in real time through the same engine. */
 typedef struct pgvd_request {
   /* in */
   const float *host_pts;    /* n*dim floats in host memory (pinned: pgvd_host_alloc) */
@@ -482,15 +524,16 @@ typedef struct pgvd_request {
 typedef struct pgvd_server_stats {
   double elapsed_s;          /* first arrival to last completion */
   int max_in_flight;         /* most requests on the device at once */
-  size_t max_in_flight_bytes;
+  size_t max_in_flight_bytes;   /* most device bytes the server held: slot pools,
+                                   points buffers and unreleased results */
   int64_t completed;
+  size_t max_device_bytes;   /* most the device's free memory fell below its
+                                level at pgvd_server_create (hipMemGetInfo) */
 } pgvd_server_stats;
 
-int pgvd_server_create(int device, int slots, size_t hbm_budget_bytes, pgvd_server **out);
-int pgvd_server_destroy(pgvd_server *s);
-/* Replays the requests (sorted by arrival_s) in real time and returns when
- * all are done; per-request results go into reqs[i], totals into *stats
- * (nullable).  Returns the first failing request's status, if any. */
+/* Submits reqs[i] at its arrival time (sorted by arrival_s), waits for all,
+ * fills each request's results and releases their octrees; totals into
+ * *stats (nullable).  Returns the first failing request's status, if any. */
 int pgvd_server_run(pgvd_server *s, pgvd_request *reqs, int64_t count, pgvd_server_stats *stats);
 /* Device bytes one whole-path build of n points occupies (its admission size). */
 size_t pgvd_build_footprint(int64_t n, int dim, int bits);

@@ -120,6 +120,9 @@ def ref(dim: int = 2):
         L.ref_bu_from_bits.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_void_p]
         L.ref_compare_bu.argtypes = [C.c_void_p, C.c_void_p]
         L.ref_compare_bu.restype = C.c_int
+        L.ref_istream_octree.argtypes = [C.c_char_p, C.c_int64, C.c_void_p, C.c_int64,
+                                         C.POINTER(C.c_int64)]
+        L.ref_istream_octree.restype = C.c_int
         _REF[dim] = L
     return _REF[dim]
 
@@ -227,7 +230,7 @@ def octree_parse_text(text: bytes, dim: int, cap: int | None = None):
     rc = lib().oracle_octree_parse_text(text, len(text), dim, out, cap, C.byref(O))
     if rc:
         return None
-    return out[: O.value * ((1 << dim) + 1)].reshape(O.value, -1)
+    return out[: O.value * ((1 << dim) + 1)].reshape(O.value, (1 << dim) + 1)
 
 
 def stream_scan(x: np.ndarray) -> np.ndarray:
@@ -340,3 +343,15 @@ def ref_xyz2z_2d(x: int, y: int, bits: int) -> int:
 def ref_lcp_length(a: int, b: int, mbits: int, dim: int = 2) -> int:
     M = (1 << 64) - 1
     return ref(dim).ref_compute_lcp_length(a & M, a >> 64, b & M, b >> 64, mbits)
+
+
+def ref_istream_octree(text: bytes, dim: int, cap: int | None = None):
+    """The reference's stream input of an octree (C/OctNode.h:69-98) on
+    std::istringstream: the records, or None where the stream fails."""
+    cap = len(text) if cap is None else cap
+    out = np.zeros(max(cap, 1) * ((1 << dim) + 1), np.int32)
+    O = C.c_int64()
+    rc = ref(dim).ref_istream_octree(text, len(text), out.ctypes.data, cap, C.byref(O))
+    if rc:
+        return None
+    return out[: O.value * ((1 << dim) + 1)].reshape(O.value, (1 << dim) + 1)

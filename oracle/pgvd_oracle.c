@@ -571,14 +571,18 @@ static int next_int(const char *t, int64_t len, int64_t *at, int64_t *v) {
   int neg = 0;
   if (t[i] == '+' || t[i] == '-') { neg = t[i] == '-'; ++i; }
   int64_t x = 0;
-  int nd = 0;
+  int nd = 0, digit = 0;   /* significant digits; leading zeros skipped */
   while (i < len && !is_ws(t[i])) {
-    if (t[i] < '0' || t[i] > '9' || nd >= 10) return -1;
-    x = x * 10 + (t[i] - '0');
-    ++nd;
+    if (t[i] < '0' || t[i] > '9') return -1;
+    digit = 1;
+    if (nd || t[i] != '0') {
+      if (nd >= 10) return -1;
+      x = x * 10 + (t[i] - '0');
+      ++nd;
+    }
     ++i;
   }
-  if (nd == 0) return -1;
+  if (!digit) return -1;
   if (neg) x = -x;
   *at = i;
   *v = x;

@@ -109,8 +109,10 @@ void oracle_find_leaf(const int64_t *p, int64_t n, int dim, int bits, const int3
  * decimal, each followed by one space; returns the length (out may be NULL
  * to measure).  parse reads N and then N nodes of 2^dim + 1 integers
  * separated by whitespace (the leaf cast to unsigned char, :76-77); a token
- * that is not an optional sign and 1..10 digits in int32 range, or too few
- * tokens, returns -1; more than `cap` nodes returns -2. */
+ * that is not an optional sign and digits (leading zeros skipped) with a
+ * value in int32 range, a negative count, or too few tokens, returns -1;
+ * more than `cap` nodes returns -2.  A token must end at whitespace (where
+ * operator>> would read a numeric prefix of the last token and stop). */
 /* CellWalk (viewer/Octree2.cpp:265-313) for n 2D segments (segs: ax, ay,
  * bx, by per segment, grid units): the leaf cells it visits, in order,
  * located with oracle_find_leaf.  Per segment counts[i] cells and status[i]

@@ -10,12 +10,18 @@
 // CL/cl.hpp and clfw.hpp).  The library is a checker for the oracle
 // restatement; nothing in the product loads it.
 //
+// It also restates the reference's (commented-out) stream input of an
+// octree, C/OctNode.h:69-98, on std::istringstream, so the text parser's
+// token rules are checked against the standard library's operator>>.
+//
 // The 2D build (-DOCT2D) also exposes the reference's xyz2z
 // (C/z_order.h:14-47).  The 3D build has no Morton entry point: z_order.h
 // includes vec_cl.h, which forces OCT2D (C/vec_cl.h:17).
 #include <sys/mman.h>
 
 #include <algorithm>
+#include <sstream>
+#include <string>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -283,6 +289,32 @@ int ref_compare_bu(const void *x, const void *y) {
   std::memcpy(&a, x, sizeof(a));
   std::memcpy(&b, y, sizeof(b));
   return compareBU(&a, &b);
+}
+
+// operator>> for std::vector<OctNode> (C/OctNode.h:84-95) and OctNode
+// (:69-76) on the text: 0 and the nodes as (children, leaf) int32 records;
+// -1 when the stream fails (failbit) or the count is negative (the
+// vector's resize throws there); -2 when the count exceeds cap.
+int ref_istream_octree(const char *text, int64_t len, int32_t *out, int64_t cap, int64_t *O_out) {
+  std::istringstream in(std::string(text, (size_t)len));
+  int n;
+  in >> n;
+  if (!in || n < 0) return -1;
+  *O_out = n;
+  if (n > cap) return -2;
+  std::vector<OctNode> octree((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    for (int k = 0; k < (1 << DIM); ++k) in >> octree[i].children[k];
+    int leaf;
+    in >> leaf;
+    octree[i].leaf = static_cast<unsigned char>(leaf);
+  }
+  if (!in) return -1;
+  for (int i = 0; i < n; ++i) {
+    for (int k = 0; k < (1 << DIM); ++k) out[(size_t)i * ((1 << DIM) + 1) + k] = octree[i].children[k];
+    out[(size_t)i * ((1 << DIM) + 1) + (1 << DIM)] = octree[i].leaf;
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -37,6 +37,7 @@ STATUS = {
     0: "PGVD_OK", 1: "PGVD_ERR_INVALID", 2: "PGVD_ERR_EMPTY", 3: "PGVD_ERR_BAD_COORD",
     4: "PGVD_ERR_NO_DEVICE", 5: "PGVD_ERR_ALLOC", 6: "PGVD_ERR_HIP", 7: "PGVD_ERR_CAPACITY",
     8: "PGVD_ERR_TOO_LARGE", 9: "PGVD_ERR_DUP_KEY", 10: "PGVD_ERR_INTERNAL",
+    11: "PGVD_NOT_READY",
 }
 
 
@@ -94,7 +95,18 @@ class Request(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("elapsed_s", C.c_double), ("max_in_flight", C.c_int),
-                ("max_in_flight_bytes", C.c_size_t), ("completed", C.c_int64)]
+                ("max_in_flight_bytes", C.c_size_t), ("completed", C.c_int64),
+                ("max_device_bytes", C.c_size_t)]
+
+
+class ServedResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("dim", C.c_int), ("bits", C.c_int), ("slot", C.c_int),
+                ("n", C.c_int64), ("m", C.c_int64), ("octree_size", C.c_int64),
+                ("octree_digest", C.c_uint64), ("arrival_s", C.c_double),
+                ("admit_s", C.c_double), ("done_s", C.c_double), ("octree", C.c_void_p)]
+
+
+PGVD_NOT_READY = 11
 
 
 def lib():
@@ -153,6 +165,9 @@ def lib():
             "pgvd_server_create": ([i32, i32, C.c_size_t, C.POINTER(vp)], i32),
             "pgvd_server_destroy": ([vp], i32),
             "pgvd_server_run": ([vp, C.POINTER(Request), i64, C.POINTER(ServerStats)], i32),
+            "pgvd_server_submit": ([vp, vp, i64, i32, i32, C.POINTER(i64)], i32),
+            "pgvd_server_poll": ([vp, i64, i32, C.POINTER(ServedResult)], i32),
+            "pgvd_server_release": ([vp, i64], i32),
             "pgvd_build_footprint": ([i64, i32, i32], C.c_size_t),
             "pgvd_keys_to_big_unsigned": ([vp, i32, i64, vp], i32),
             "pgvd_big_unsigned_to_keys": ([vp, i64, i32, vp], i32),
@@ -808,6 +823,30 @@ class Server:
     @staticmethod
     def footprint(n: int, dim: int, bits: int) -> int:
         return int(lib().pgvd_build_footprint(n, dim, bits))
+
+    def submit(self, host_pts_ptr: int, n: int, dim: int, bits: int) -> int:
+        """Queue a build of n points at host_pts_ptr (kept valid until the
+        request is complete); returns its ticket."""
+        t = C.c_int64()
+        rc = lib().pgvd_server_submit(self.h, host_pts_ptr, n, dim, bits, C.byref(t))
+        if rc != 0:
+            raise PgvdError(rc, lib().pgvd_last_error().decode())
+        return t.value
+
+    def poll(self, ticket: int, wait: bool = False):
+        """The ServedResult of a complete request, or None when not yet."""
+        r = ServedResult()
+        rc = lib().pgvd_server_poll(self.h, ticket, int(wait), C.byref(r))
+        if rc == PGVD_NOT_READY:
+            return None
+        if rc != 0:
+            raise PgvdError(rc, lib().pgvd_last_error().decode())
+        return r
+
+    def release(self, ticket: int):
+        rc = lib().pgvd_server_release(self.h, ticket)
+        if rc != 0:
+            raise PgvdError(rc, lib().pgvd_last_error().decode())
 
     def run(self, reqs) -> ServerStats:
         """reqs: a ctypes array of Request sorted by arrival_s (filled in place)."""

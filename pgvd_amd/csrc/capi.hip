@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -908,7 +909,9 @@ int pgvd_cell_walk(pgvd_ctx *ctx, const float *segs, int64_t n, int bits, const 
   hipStream_t st = pick(ctx, s);
   if (n == 0) {
     const int64_t zero = 0;
-    HIPCHK(hipMemcpy(offsets, &zero, 8, hipMemcpyHostToDevice));
+    // on the caller's stream, ordered with its work on the buffer
+    HIPCHK(hipMemcpyAsync(offsets, &zero, 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
     *total_out = 0;
     return PGVD_OK;
   }
@@ -916,10 +919,14 @@ int pgvd_cell_walk(pgvd_ctx *ctx, const float *segs, int64_t n, int bits, const 
   CHK(pool_get(ctx, "walk_counts", 4 * static_cast<size_t>(n), &counts));
   CHK(pool_get(ctx, "walk_ws", 8 * static_cast<size_t>(walk_scan_ws_words(n)), &ws));
   // With a cell buffer, the first walk keeps each segment's first 32 cells
-  // (up to 2 GB of scratch), so that only longer walks run twice.
+  // (up to 2 GB of scratch, or PGVD_WALK_SCRATCH_BYTES from the
+  // environment), so that only longer walks run twice; past that, a count
+  // walk and a write walk.
   constexpr int kLog2K = 5;
   const size_t spec_bytes = (static_cast<size_t>(n) << kLog2K) * 20;
-  if (cells && spec_bytes <= (size_t(2) << 30))
+  size_t spec_limit = size_t(2) << 30;
+  if (const char *e = std::getenv("PGVD_WALK_SCRATCH_BYTES")) spec_limit = std::strtoull(e, nullptr, 10);
+  if (cells && spec_bytes <= spec_limit)
     CHK(pool_get(ctx, "walk_scratch", spec_bytes, &scratch));
   int64_t total = 0;
   {
@@ -955,7 +962,8 @@ int pgvd_octree_format_text(pgvd_ctx *ctx, const int32_t *octree, int64_t O, int
     *len_out = hl;
     if (!text) return PGVD_OK;
     if (capacity < hl) return fail(PGVD_ERR_CAPACITY, "text buffer too small (*len_out bytes needed)");
-    HIPCHK(hipMemcpy(text, hdr, hl, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(text, hdr, hl, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
     return PGVD_OK;
   }
   void *ws = nullptr;
@@ -995,17 +1003,25 @@ int pgvd_octree_parse_text(pgvd_ctx *ctx, const char *text, int64_t len, int dim
   HIPCHK(hipStreamSynchronize(st));
   size_t i = 0;
   while (i < head.size() && text_space(head[i])) ++i;
-  if (i < head.size() && head[i] == '+') ++i;
+  bool neg = false;
+  if (i < head.size() && (head[i] == '+' || head[i] == '-')) neg = head[i++] == '-';
   int64_t N = 0;
   int nd = 0;
+  bool digit = false;
   while (i < head.size() && !text_space(head[i])) {
-    if (head[i] < '0' || head[i] > '9' || nd >= 10) return fail(PGVD_ERR_INVALID, "the node count is not a number");
-    N = N * 10 + (head[i] - '0');
-    ++nd;
+    if (head[i] < '0' || head[i] > '9') return fail(PGVD_ERR_INVALID, "the node count is not a number");
+    digit = true;
+    if (nd || head[i] != '0') {   // leading zeros skipped
+      if (nd >= 10) return fail(PGVD_ERR_INVALID, "the node count is not a number");
+      N = N * 10 + (head[i] - '0');
+      ++nd;
+    }
     ++i;
   }
-  if (nd == 0 || (i == head.size() && static_cast<int64_t>(i) < len))
+  if (!digit || (i == head.size() && static_cast<int64_t>(i) < len))
     return fail(PGVD_ERR_INVALID, "no node count in the first 4096 bytes");
+  // a negative count: the reference's resize throws ("-0" is 0)
+  if (neg && N) return fail(PGVD_ERR_INVALID, "negative node count");
   if (N > INT32_MAX) return fail(PGVD_ERR_INVALID, "node count above 2^31 - 1");
   *O_out = N;
   if (N > cap) return fail(PGVD_ERR_CAPACITY, "octree buffer too small (*O_out nodes needed)");

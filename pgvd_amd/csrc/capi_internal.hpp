@@ -97,6 +97,7 @@ struct pgvd_ctx {
   LastBuild last;
   int64_t octree_cap = 0;       // nodes the pooled octree buffer holds
   int octree_stride = 0;        // ints per node of that buffer
+  bool own_stream_only = false; // every call on the context uses its stream
 };
 
 namespace pgvd_capi {
@@ -117,8 +118,11 @@ inline int pool_get(pgvd_ctx *ctx, const char *key, size_t bytes, void **out) {
   PoolBuf &b = ctx->pool[key];
   if (b.bytes < bytes) {
     if (b.p) {
-      // the old buffer may be in use on any stream the caller passed
-      HIPCHK(hipDeviceSynchronize());
+      // the old buffer may be in use on any stream the caller passed (only
+      // on the context's own where it is the only one: the served form's
+      // slots, whose neighbours keep running)
+      if (ctx->own_stream_only) HIPCHK(hipStreamSynchronize(ctx->stream));
+      else HIPCHK(hipDeviceSynchronize());
       HIPCHK(hipFree(b.p));
       b.p = nullptr;
       b.bytes = 0;

@@ -171,25 +171,37 @@ __device__ inline bool is_space(uint8_t c) {
   return c <= 32 && ((0x100003E00ull >> c) & 1ull);
 }
 
-// Token parse from the staged bytes: an optional sign and 1..10 digits,
-// ending at a space (bytes past the text are staged as spaces), within int32.
-__device__ inline bool parse_staged(const uint8_t *s, int p, int32_t *out) {
+// Token parse from the staged bytes: an optional sign and digits, at most
+// 10 of them significant (leading zeros are skipped, as operator>> skips
+// them), ending at a space (bytes past the text read as spaces), within
+// int32.  A token running past the staged bytes (only one with leading
+// zeros can) continues from global memory: byte q of s is text[g0 + q].
+__device__ inline bool parse_staged(const uint8_t *s, int p, int staged, const uint8_t *text,
+                                    int64_t g0, int64_t len, int32_t *out) {
+  auto at = [&](int q) -> uint8_t {
+    if (q < staged) return s[q];
+    return g0 + q < len ? text[g0 + q] : (uint8_t)' ';
+  };
   bool neg = false;
-  uint8_t c = s[p];
+  uint8_t c = at(p);
   if (c == '+' || c == '-') {
     neg = c == '-';
     ++p;
   }
   int64_t x = 0;
   int nd = 0;
+  bool digit = false;
   for (;; ++p) {
-    c = s[p];
+    c = at(p);
     if (is_space(c)) break;
-    if (c < '0' || c > '9' || nd >= 10) return false;
+    if (c < '0' || c > '9') return false;
+    digit = true;
+    if (nd == 0 && c == '0') continue;
+    if (nd >= 10) return false;
     x = x * 10 + (c - '0');
     ++nd;
   }
-  if (nd == 0) return false;
+  if (!digit) return false;
   if (neg) x = -x;
   if (x < INT32_MIN || x > INT32_MAX) return false;
   *out = (int32_t)x;
@@ -200,7 +212,8 @@ __device__ inline bool parse_staged(const uint8_t *s, int p, int32_t *out) {
 // rounds' token-start counts are scanned together as 16-bit fields of one
 // word (a round holds at most kParseTile / 2 starts), then each round's
 // bytes and the 16 after it are staged in LDS (a token starting in the round
-// is decided within 12 bytes: sign, 10 digits, one more byte), its tokens
+// without leading zeros is decided within 12 bytes: sign, 10 digits, one
+// more byte; a longer one reads on from global memory), its tokens
 // parsed into LDS in token order and written out as one run of records.
 template <int DIM>
 __global__ __launch_bounds__(kParseBlock) void parse_text_kernel(
@@ -271,7 +284,8 @@ __global__ __launch_bounds__(kParseBlock) void parse_text_kernel(
       const uint64_t k = base + li;
       if (k >= 1 && k <= last) {
         int32_t val = 0;
-        if (!parse_staged(s_txt, t * kParseBytes + j, &val))
+        if (!parse_staged(s_txt, t * kParseBytes + j, kParseTile + kHalo, text,
+                          tb + (int64_t)r * kParseTile, len, &val))
           atomicMin(reinterpret_cast<unsigned long long *>(ws + kWsBad), (unsigned long long)k + 1);
         s_val[li] = val;
       }

@@ -2,7 +2,8 @@
 data files, read with pgvd_amd.io, as plain float32 arrays.
 
   2D scenes (data2/): test1 (test1-1..5), test2, test3, test4 (their files
-  in order), test5..test8;
+  in order), test5..test8, with each scene's vertex count per file
+  (data2_<scene>_counts: one file is one polyline);
   3D meshes (data3/): simple1..3.obj and bunny/bun1.obj.
 
 Run from the repository root with the reference checked out at
@@ -28,6 +29,9 @@ def main():
         files = sorted(glob.glob(os.path.join(d2, scene + "-*.dat"))) or \
             [os.path.join(d2, scene + ".dat")]
         out["data2_" + scene] = io.read_scene(files)
+        # vertices per file: one file is one polyline
+        out["data2_" + scene + "_counts"] = np.array(
+            [io.read_polyline(f).shape[0] for f in files], np.int64)
     d3 = os.path.join(REF, "data3")
     for name in ("simple1", "simple2", "simple3"):
         out["data3_" + name] = io.read_obj_vertices(os.path.join(d3, name + ".obj"))

@@ -517,19 +517,44 @@ def test_cell_walk_matches_oracle(dev, bits, npts, nseg, reach):
     assert (np.diff(want["offsets"]) > 1).mean() > 0.3
 
 
+def test_cell_walk_count_then_write(dev, monkeypatch):
+    """With no scratch allowed for the first walk's early cells
+    (PGVD_WALK_SCRATCH_BYTES=0), every segment is walked twice -- a count
+    walk, then a write walk into the offsets the scan gave: the same cells
+    as the oracle, and as the default path."""
+    bits = 12
+    rng = np.random.default_rng(31)
+    tree = dev.build_octree(_pts(20000, 2, seed=31), bits)
+    W = 1 << bits
+    a = rng.random((4000, 2)) * (W - 1e-3)
+    b = np.clip(a + (rng.random((4000, 2)) * 2 - 1) * 0.2 * W, 0, W - 1e-3)
+    segs = np.concatenate([a, b], axis=1).astype(np.float32)
+    default = dev.cell_walk(segs, bits, tree["children"], tree["leaf"])
+    monkeypatch.setenv("PGVD_WALK_SCRATCH_BYTES", "0")
+    got = dev.cell_walk(segs, bits, tree["children"], tree["leaf"])
+    want = oracle.cell_walk(segs, bits, tree["children"], tree["leaf"])
+    _assert_walk_equal(got, want, "count then write")
+    _assert_walk_equal(got, default, "count then write vs default")
+    assert (np.diff(want["offsets"]) > 32).any()
+
+
 def test_cell_walk_reference_polylines(dev):
     """The data2 scenes walked as the viewer walks them (FindMultiCells,
-    viewer/Octree2.cpp:430-455): each polyline's consecutive vertices, mapped
-    by obj2Oct into the octree of the scene's points."""
+    viewer/Octree2.cpp:430-455): each polyline's (file's) consecutive
+    vertices, polyline by polyline (:442-450), mapped by obj2Oct into the
+    octree of the scene's points."""
     pts = _data_points()
     for name in ("data2_test1", "data2_test2", "data2_test3", "data2_test4", "data2_test5"):
         v = pts[name]
+        ends = np.cumsum(pts[name + "_counts"])
         for bits in (8, 16):
             tree = dev.build_octree(v, bits)
             lo = v.min(axis=0)
             bbw = float((v.max(axis=0) - lo).max())
             q = _obj2oct(v, lo, bbw, 1 << bits)
-            segs = np.concatenate([q[:-1], q[1:]], axis=1)
+            segs = np.concatenate([np.concatenate([p[:-1], p[1:]], axis=1)
+                                   for p in np.split(q, ends[:-1])])
+            assert segs.shape[0] == v.shape[0] - ends.size
             got = dev.cell_walk(segs, bits, tree["children"], tree["leaf"])
             want = oracle.cell_walk(segs, bits, tree["children"], tree["leaf"])
             _assert_walk_equal(got, want, f"{name} bits={bits}")

@@ -82,7 +82,10 @@ def test_served_budget_limits_what_is_in_flight(dev):
         budget = int(max(feet) * 1.2)            # one large build at a time
         with pgvd_amd.Server(0, slots=4, hbm_budget=budget) as srv:
             st = srv.run(reqs)
+        # what the server allocated (slot pools, points, results) stayed
+        # within the budget, and so did the device's own count of used memory
         assert st.max_in_flight_bytes <= budget
+        assert st.max_device_bytes <= budget + (64 << 20), (st.max_device_bytes, budget)
         assert st.completed == len(reqs)
         check_alone(dev, pins, reqs, range(0, len(reqs), 3))
     finally:
@@ -97,6 +100,49 @@ def test_served_rejects_a_request_over_budget():
         with pgvd_amd.Server(0, slots=2, hbm_budget=1 << 20) as srv:
             with pytest.raises(pgvd_amd.PgvdError):
                 srv.run(reqs)
+    finally:
+        for p in pins:
+            p.free()
+
+
+@pytest.mark.gpu
+def test_served_submit_poll_release(dev):
+    """Requests submitted online: polled until complete, each octree read
+    from its device result (kept until released) and compared with the same
+    build alone and, for three of them, with the oracle."""
+    import oracle
+    pins, reqs = make_trace(40, seed=9, lo=2e3, hi=3e5)
+    try:
+        with pgvd_amd.Server(0, slots=3, hbm_budget=16 << 30) as srv:
+            tickets = [srv.submit(r.host_pts, r.n, r.dim, r.bits) for r in reqs]
+            results = {}
+            while len(results) < len(tickets):
+                for i, t in enumerate(tickets):
+                    if i not in results:
+                        res = srv.poll(t)
+                        if res is not None:
+                            results[i] = res
+            for i, t in enumerate(tickets):
+                res = results[i]
+                assert res.status == 0, (i, res.status)
+                assert res.done_s >= res.admit_s >= res.arrival_s
+                stride = (1 << res.dim) + 1
+                raw = dev.download(res.octree, np.int32, res.octree_size * stride).reshape(-1, stride)
+                want = dev.build_octree(np.array(pins[i].array), reqs[i].bits)
+                assert (res.m, res.octree_size) == (want["m"], want["octree_size"]), i
+                assert np.array_equal(raw[:, :-1], want["children"]), i
+                assert np.array_equal(raw[:, -1], want["leaf"]), i
+                assert res.octree_digest == digest_np(raw), i
+                if i < 3:
+                    q = oracle.quantize(np.array(pins[i].array), 1 << reqs[i].bits)
+                    o = oracle.build(q, reqs[i].bits)
+                    assert np.array_equal(raw[:, :-1], o["children"]), i
+                    assert np.array_equal(raw[:, -1], o["leaf"]), i
+                srv.release(t)
+            with pytest.raises(pgvd_amd.PgvdError):
+                srv.poll(tickets[0])
+            with pytest.raises(pgvd_amd.PgvdError):
+                srv.submit(reqs[0].host_pts, 1 << 29, 3, 21)    # footprint above the budget
     finally:
         for p in pins:
             p.free()

@@ -76,6 +76,29 @@ def test_parse_whitespace_signs_leaf_and_trailing_tokens(dev):
         assert np.array_equal(dev.parse_octree_text(text, dim), want)
 
 
+def test_parse_leading_zeros(dev):
+    """Zero-padded tokens (0 to 300 zeros, so many run past the bytes a
+    round stages and across round and tile edges) read as operator>> reads
+    them, a zero-padded and a "-0" node count, and padded tokens past the
+    int32 limits rejected; equal to the oracle's operator>>."""
+    rng = np.random.default_rng(21)
+    for dim, O in ((2, 4000), (3, 3000)):
+        rec = _records(O, dim, seed=dim + 40)
+        toks = []
+        for x in rec.reshape(-1):
+            z = "0" * int(rng.choice([0, 1, 5, 12, 40, 300], p=[.5, .15, .1, .1, .1, .05]))
+            toks.append(("-" if x < 0 else "") + z + str(abs(int(x))))
+        text = ("000" + str(O) + " " + " ".join(toks) + " ").encode()
+        want = oracle.octree_parse_text(text, dim)
+        assert want is not None and np.array_equal(want[:, :-1], rec[:, :-1])
+        assert np.array_equal(dev.parse_octree_text(text, dim), want)
+    assert dev.parse_octree_text(b"-0 ", 2).shape[0] == 0
+    for bad in (b"1 0 0 0 0 " + b"0" * 100 + b"2147483648 ", b"1 0 -" + b"0" * 20 + b"2147483649 0 0 0 "):
+        assert oracle.octree_parse_text(bad, 2) is None
+        with pytest.raises(P.PgvdError):
+            dev.parse_octree_text(bad, 2)
+
+
 def test_parse_misaligned_text(dev):
     """A text that starts at an odd device address takes the byte loads."""
     rec = _records(4000, 3, seed=9)

@@ -33,7 +33,11 @@ def test_reference_data_fixture():
     sizes = {k: v.shape for k, v in d.items()}
     assert sizes["data2_test1"] == (24, 2) and sizes["data3_bun1"] == (8171, 3)
     for k, v in d.items():
+        if k.endswith("_counts"):        # vertices per file of a data2 scene
+            assert v.dtype == np.int64 and v.sum() == d[k[:-len("_counts")]].shape[0], k
+            continue
         assert v.dtype == np.float32 and np.isfinite(v).all(), k
+    assert list(d["data2_test1_counts"]) and d["data2_test1_counts"].size == 5
     # closed polylines repeat their first vertex: equal points reach unique
     t2 = d["data2_test1"]
     assert len({tuple(x) for x in t2.tolist()}) < len(t2)

@@ -375,6 +375,76 @@ def test_octree_text_parse_rules():
     assert oracle.octree_parse_text(b"1 -2147483648 0 0 0 0 ", 2)[0, 0] == -2147483648
 
 
+def test_octree_text_parse_leading_zeros_and_signs():
+    """Known answers of operator>> on int (std::num_get): leading zeros are
+    skipped however many, a sign may precede them, "-0" is a count of zero
+    nodes, and the value (not the digit count) decides overflow."""
+    want = np.array([[1, -1, 0, 7, 14]], np.int32)
+    for text in (b"1 00000000001 -1 0 +007 14", b"01 1 -0000000000000001 -0 7 0000000000000000000014",
+                 b"+1 1 -1 +0 7 14 junk"):
+        assert np.array_equal(oracle.octree_parse_text(text, 2), want), text
+    assert oracle.octree_parse_text(b"-0 ", 2).shape == (0, 5)
+    assert oracle.octree_parse_text(b"1 -000000000002147483648 0002147483647 0 0 0", 2)[0, :2].tolist() == \
+        [-2147483648, 2147483647]
+    for bad in (b"1 000000000002147483648 0 0 0 0", b"1 99999999999 0 0 0 0", b"-1 ", b"1 0 0 0 0 -",
+                b"1 0 0 0 0 00x"):
+        assert oracle.octree_parse_text(bad, 2) is None, bad
+
+
+def _token_soup(rng, ntok):
+    """Whitespace-separated integer tokens of the kinds operator>> meets:
+    plain, zero-padded, signed, at and past the int32 limits, and a few
+    non-numbers; every token ends at whitespace (a token with a numeric
+    prefix and other bytes after it is not generated: the parser rejects
+    it, operator>> reads the prefix and fails only on the next read)."""
+    out = []
+    for _ in range(ntok):
+        k = rng.integers(0, 12)
+        v = int(rng.integers(-2 ** 31, 2 ** 31))
+        if k == 0:
+            tok = str(v)
+        elif k == 1:
+            tok = ("-" if v < 0 else rng.choice(["", "+"])) + "0" * int(rng.integers(1, 30)) + str(abs(v))
+        elif k == 2:
+            tok = str(rng.choice([2 ** 31 - 1, -2 ** 31, 2 ** 31, -2 ** 31 - 1, 10 ** 10, 0]))
+        elif k == 3:
+            tok = str(rng.choice(["-0", "+0", "000", "-", "+", "x", "1x"[1:], "--1", "+-2"]))
+        else:
+            tok = str(int(rng.integers(-1, 300)))
+        out.append(tok)
+    seps = [" ", "\n", "\t", "  ", "\r\n", "\v", "\f"]
+    return "".join(t + seps[int(rng.integers(0, len(seps)))] for t in out).encode()
+
+
+@pytest.mark.skipif(not oracle.ref_available(2) or not oracle.ref_available(3),
+                    reason="the reference tree was not built into oracle/_ref")
+@pytest.mark.parametrize("dim", [2, 3])
+def test_octree_text_parse_matches_istream(dim):
+    """The oracle's parser against the reference's stream input of an
+    octree (C/OctNode.h:69-98) run on std::istringstream: the same records,
+    or failure on both, over token soups with and without bad tokens."""
+    rng = np.random.default_rng(dim)
+    stride = (1 << dim) + 1
+    agree_ok = agree_bad = 0
+    for trial in range(400):
+        n = int(rng.integers(0, 6))
+        body = _token_soup(rng, n * stride + int(rng.integers(-1, 3)))
+        if trial % 2 == 0:    # valid tokens only, most of the time well formed
+            body = " ".join(str(int(x)) if i % 3 else "0" * int(rng.integers(0, 4)) + str(abs(int(x)))
+                            for i, x in enumerate(rng.integers(-1000, 1000, n * stride))).encode()
+        head = rng.choice([str(n), "0" * int(rng.integers(1, 5)) + str(n), "+" + str(n)])
+        text = head.encode() + b" " + body
+        got = oracle.octree_parse_text(text, dim)
+        want = oracle.ref_istream_octree(text, dim)
+        if want is None:
+            assert got is None, text
+            agree_bad += 1
+        else:
+            assert got is not None and np.array_equal(got, want), text
+            agree_ok += 1
+    assert agree_ok > 100 and agree_bad > 50
+
+
 @pytest.mark.parametrize("dim,n", [(2, 5000), (3, 20000)])
 def test_octree_text_round_trip(dim, n):
     rng = np.random.default_rng(dim)
@@ -385,18 +455,20 @@ def test_octree_text_round_trip(dim, n):
 
 
 # ------------------------------------------------------------ cell walk ----
-def _leaf_boxes(t, bits):
-    """(origin x, origin y, width) of every leaf cell, labelled top-down."""
+def _leaf_boxes(t, bits, owners=False):
+    """(origin x, origin y, width) of every leaf cell, labelled top-down;
+    with owners, a dict from each box to (node, octant) of the node whose
+    octant it is."""
     cells = _reachable_cells(t["children"], 2)
-    boxes = []
+    boxes = {}
     for node, (lvl, path) in cells.items():
         for o in range(4):
             if (int(t["leaf"][node]) >> o) & 1:
                 p = path + (o,)
                 org = [sum(((d >> k) & 1) << (bits - 1 - l) for l, d in enumerate(p))
                        for k in range(2)]
-                boxes.append((org[0], org[1], 1 << (bits - len(p))))
-    return boxes
+                boxes[(org[0], org[1], 1 << (bits - len(p)))] = (node, o)
+    return boxes if owners else list(boxes)
 
 
 def _overlap(a, b, lo, hi):
@@ -465,3 +537,100 @@ def test_cell_walk_planted_mistake_fails():
             if inner is not None and inner[1] > inner[0] and (x, y, wd) not in visited:
                 missed += 1
     assert missed > 0
+
+
+def _walk_violations(segs, w, owners, delta=1e-3):
+    """What a brute force finds wrong with a walk: a cell that is not a leaf
+    or whose (parent, octant) is not the node and octant holding it, a cell
+    visited twice, cells out of order along the segment (the midpoints of
+    their parameter intervals, on boxes grown by delta, must not decrease),
+    a last cell whose closed box does not hold b, or offsets that disagree
+    with the counts."""
+    bad = []
+    if not np.array_equal(np.diff(w["offsets"]), w["counts"]):
+        bad.append("offsets")
+    for s in range(segs.shape[0]):
+        a, b = segs[s, :2], segs[s, 2:]
+        got = w["cells"][w["offsets"][s]:w["offsets"][s + 1]]
+        keys = [tuple(int(x) for x in c[:3]) for c in got]
+        if len(set(keys)) != len(keys):
+            bad.append((s, "repeat"))
+        mids = []
+        for c, k in zip(got, keys):
+            if owners.get(k) != (int(c[3]), int(c[4])):
+                bad.append((s, "owner", k))
+            x, y, wd = k
+            iv = _overlap(a, b, (x - delta, y - delta), (x + wd + delta, y + wd + delta))
+            mids.append(-1.0 if iv is None else 0.5 * (iv[0] + iv[1]))
+        length = float(np.hypot(*(b.astype(np.float64) - a)))
+        tol = 4 * delta / max(length, 1e-9)
+        if any(m1 < m0 - tol for m0, m1 in zip(mids, mids[1:])) or min(mids, default=0) < 0:
+            bad.append((s, "order"))
+        # b in the last cell's closed box (an end on a grid line, reached in
+        # a decreasing direction, ends in the cell below or left of it)
+        x, y, wd = keys[-1]
+        if w["status"][s] == 0 and not (x <= b[0] <= x + wd and y <= b[1] <= y + wd):
+            bad.append((s, "last"))
+    return bad
+
+
+@pytest.mark.parametrize("bits,npts,seed", [(10, 400, 5), (12, 3000, 6), (6, 40, 7), (16, 5000, 8)])
+def test_cell_walk_order_owners_no_repeats(bits, npts, seed):
+    """CellWalk (viewer/Octree2.cpp:265-313, oracle) visits its cells in order
+    along the segment, never twice, ends in b's cell, and labels each cell
+    with the node and octant that hold it (the OctCell's parent_idx and
+    octant); random segments, axis-parallel ones and ones on grid lines."""
+    rng = np.random.default_rng(seed)
+    t = oracle.build(rng.integers(0, 1 << bits, (npts, 2)), bits)
+    owners = _leaf_boxes(t, bits, owners=True)
+    W = 1 << bits
+    segs = (rng.random((400, 4)) * (W - 1e-3)).astype(np.float32)
+    segs[:50, 3] = segs[:50, 1]
+    segs[50:100, 2] = segs[50:100, 0]
+    segs[100:150] = np.floor(segs[100:150]) + np.float32(0.5)
+    segs[150:250] = np.floor(segs[150:250])              # on grid lines and corners
+    segs[150:175, 3] = segs[150:175, 1]
+    segs[175:200, 2] = segs[175:200, 0]
+    w = oracle.cell_walk(segs, bits, t["children"], t["leaf"])
+    assert (w["status"] == 0).all()
+    assert _walk_violations(segs, w, owners) == []
+    assert (w["counts"] > 2).mean() > 0.3
+
+
+def test_cell_walk_order_check_catches_planted_mistakes():
+    """The order / repeat / owner / offset checks each catch their mistake,
+    planted in a copy of a correct walk."""
+    rng = np.random.default_rng(9)
+    t = oracle.build(rng.integers(0, 1 << 10, (400, 2)), 10)
+    owners = _leaf_boxes(t, 10, owners=True)
+    segs = (rng.random((60, 4)) * 1023).astype(np.float32)
+    w = oracle.cell_walk(segs, 10, t["children"], t["leaf"])
+    s = int(np.argmax(w["counts"]))
+    assert w["counts"][s] >= 4
+    o = int(w["offsets"][s])
+
+    def planted(edit):
+        v = {k: np.array(x) for k, x in w.items()}
+        edit(v)
+        return {str(x[1]) if isinstance(x, tuple) else x for x in _walk_violations(segs, v, owners)}
+
+    def swap(v):
+        v["cells"][[o + 1, o + 2]] = v["cells"][[o + 2, o + 1]]
+
+    def repeat(v):
+        v["cells"][o + 2] = v["cells"][o + 1]
+
+    def octant(v):
+        v["cells"][o + 1, 4] ^= 1
+
+    def parent(v):
+        v["cells"][o + 1, 3] += 1
+
+    def offsets(v):
+        v["offsets"][s + 1] -= 1
+
+    assert "order" in planted(swap)
+    assert "repeat" in planted(repeat)
+    assert "owner" in planted(octant)
+    assert "owner" in planted(parent)
+    assert "offsets" in planted(offsets)
