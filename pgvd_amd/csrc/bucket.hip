@@ -92,6 +92,11 @@ __device__ inline void init_seg(BkSeg *sg, uint32_t start, uint32_t count, bool 
 // and the second scatter's tile starts, and the level-1 segments (buckets
 // above CAP) with their accumulators.
 constexpr int kPlanBlock = 1024;
+// The LSD passes take over when more than 1 / PGVD_BK_LSD_SHARE of the keys
+// are left to the levels (0: never).
+#ifndef PGVD_BK_LSD_SHARE
+#define PGVD_BK_LSD_SHARE 4
+#endif
 __global__ __launch_bounds__(kPlanBlock) void bk_plan_kernel(
     const uint32_t *__restrict__ hist, uint32_t n, int nb, int nb2, int tile2, uint32_t cap,
     int wide, int lsd_ok, uint32_t *__restrict__ start, uint32_t *__restrict__ cursor,
@@ -138,7 +143,8 @@ __global__ __launch_bounds__(kPlanBlock) void bk_plan_kernel(
     // levels sort faster by the LSD passes where those are few (keys of at
     // most 64 bits): then the scatters and bucket sorts do nothing, and the
     // host launches the passes over the keys as encoded.
-    const bool lsd = lsd_ok && (uint64_t)s_bigkeys * 4 > (uint64_t)n;
+    const bool lsd = lsd_ok && PGVD_BK_LSD_SHARE > 0 &&
+                     (uint64_t)s_bigkeys * PGVD_BK_LSD_SHARE > (uint64_t)n;
     st->bk_lsd = lsd ? 1u : 0u;
     st->bk_segs = lsd ? 0u : nseg;
     st->bk_keys = lsd ? 0u : s_bigkeys;
@@ -411,18 +417,119 @@ __device__ inline void lds_radix(K (&k)[IPT], uint32_t (&v)[IPT], int passes, in
   }
 }
 
+// ---- merge sort of one item in LDS -----------------------------------------------
+// (key, position) pairs in a strict total order: equal keys by position, so
+// the sort is stable and slots past the item (all-ones keys, positions >=
+// cnt) stay behind every real key.
+template <typename K>
+__device__ inline bool pair_lt(const K &a, uint32_t ia, const K &b, uint32_t ib) {
+  return kless(a, b) || (keq(a, b) && ia < ib);
+}
+
+// The item's cnt keys (and labels) from in[s0 ..] sorted by merging: each
+// thread sorts IPT consecutive keys in registers, then log2(BS) rounds merge
+// runs pairwise, every thread producing IPT outputs of its run pair from a
+// merge-path split found by binary search.  The cost does not depend on how
+// the keys are spread, which suits wide keys that no few digits separate
+// (sparse 128-bit keys on curves).  Positions travel in s_idx (16 bits);
+// labels, staged in s_vals in input order, are gathered by position at the
+// end.  On return s_keys[0 .. cnt) (and s_vals) hold the item in order.
+template <typename K, bool LAB, int BS, int IPT>
+__device__ inline void lds_merge_sort(const K *__restrict__ in, const uint32_t *__restrict__ vin,
+                                      uint32_t s0, uint32_t cnt, K *s_keys, uint32_t *s_vals,
+                                      uint16_t *s_idx) {
+  constexpr uint32_t CAP = (uint32_t)BS * IPT;
+  static_assert(CAP <= 65536, "positions fit 16 bits");
+  const uint32_t t = threadIdx.x;
+  const uint32_t p0 = t * IPT;
+  K k[IPT];
+  uint32_t x[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t p = p0 + j;
+    k[j] = p < cnt ? in[s0 + p] : kones<K>();
+    x[j] = p;
+    if (LAB && p < cnt) s_vals[p] = vin[s0 + p];
+  }
+  // odd-even transposition sort of the thread's IPT pairs
+#pragma unroll
+  for (int r = 0; r < IPT; ++r) {
+#pragma unroll
+    for (int j = r & 1; j + 1 < IPT; j += 2) {
+      if (pair_lt(k[j + 1], x[j + 1], k[j], x[j])) {
+        const K tk = k[j]; k[j] = k[j + 1]; k[j + 1] = tk;
+        const uint32_t tx = x[j]; x[j] = x[j + 1]; x[j + 1] = tx;
+      }
+    }
+  }
+  for (uint32_t w = IPT; w < CAP; w *= 2) {
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      s_keys[p0 + j] = k[j];
+      s_idx[p0 + j] = (uint16_t)x[j];
+    }
+    __syncthreads();
+    const uint32_t base = p0 / (2 * w) * (2 * w);
+    const uint32_t d = p0 - base;   // this thread's first output in the merged pair
+    const uint32_t A = base, B = base + w;
+    // i keys from run A and d - i from run B come first
+    uint32_t lo = d > w ? d - w : 0u, hi = d < w ? d : w;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      const uint32_t bj = B + d - 1 - mid;
+      if (pair_lt(s_keys[bj], (uint32_t)s_idx[bj], s_keys[A + mid], (uint32_t)s_idx[A + mid]))
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    uint32_t ai = A + lo, bi = B + (d - lo);
+    const uint32_t ae = A + w, be = B + w;
+    K ka = ai < ae ? s_keys[ai] : kones<K>(), kb = bi < be ? s_keys[bi] : kones<K>();
+    uint32_t xa = ai < ae ? s_idx[ai] : 0xffffffffu, xb = bi < be ? s_idx[bi] : 0xffffffffu;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const bool take_a = pair_lt(ka, xa, kb, xb);   // an exhausted run compares last
+      k[j] = take_a ? ka : kb;
+      x[j] = take_a ? xa : xb;
+      if (take_a) {
+        ++ai;
+        ka = ai < ae ? s_keys[ai] : kones<K>();
+        xa = ai < ae ? s_idx[ai] : 0xffffffffu;
+      } else {
+        ++bi;
+        kb = bi < be ? s_keys[bi] : kones<K>();
+        xb = bi < be ? s_idx[bi] : 0xffffffffu;
+      }
+    }
+    __syncthreads();   // every read of this round is done before the next writes
+  }
+  uint32_t lab[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) lab[j] = (LAB && x[j] < cnt) ? s_vals[x[j]] : 0u;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    s_keys[p0 + j] = k[j];
+    if (LAB) s_vals[p0 + j] = lab[j];
+  }
+}
+
 constexpr int kSubBits = 12;                 // sub-buckets of the counting pass
 constexpr int kSubBins = 1 << kSubBits;
 // Largest sub-bucket ranked in place (rank = keys below it in the
-// sub-bucket, one LDS read each): level-0 buckets of uniform keys hold ~1
-// key per sub-bucket; the levels' items hold keys on curves and surfaces,
-// whose top 12 varying bits leave larger sub-buckets, for which ranking
-// stays cheaper than LSD passes over every varying bit.
+// sub-bucket, one LDS read each, so the cost grows with its size): level-0
+// buckets of uniform keys hold ~1 key per sub-bucket; the levels' items
+// (PGVD_BK_RANK_MAX) may hold keys on curves and surfaces, whose top 12
+// varying bits leave larger sub-buckets, sorted past this size by LDS radix
+// passes or the merge sort.
 #ifndef PGVD_BK_RANK_MAX
-#define PGVD_BK_RANK_MAX 128
+#define PGVD_BK_RANK_MAX 32
 #endif
 constexpr uint32_t kSubMaxGroup = 32;
 constexpr uint32_t kSubMaxGroupList = PGVD_BK_RANK_MAX;
+// Past ranking: LDS radix passes over at most this many varying bits (three
+// passes), a merge sort over more.
+constexpr int kRadixMaxBits = 24;
 
 template <typename K, bool LAB>
 struct SortArgs {
@@ -443,9 +550,9 @@ struct SortArgs {
 // levels).  List mode: item blockIdx.x of items[] (count st->bk_items), its
 // sbits lowered to the bits that vary among its keys.  An item with sbits 0
 // holds equal keys and is copied.  Otherwise a counting pass on the 12 bits
-// below sbits, then each key's rank within its
-// sub-bucket -- or, when a sub-bucket holds more than 32 keys, LSD passes in
-// LDS over the bits below sbits.  Equal
+// below sbits, then each key's rank within its sub-bucket -- or, when a
+// sub-bucket holds more than 32 keys, LSD passes in LDS over the bits below
+// sbits (at most 24 of them) or a merge sort (more).  Equal
 // keys share an item, so the workgroup sees every duplicate: it raises
 // st->msd_dups for equal neighbours, or for the zero key at the front when
 // a zero key is prepended (UniqueSorted would merge them); only then does
@@ -541,7 +648,7 @@ bk_sort_kernel(SortArgs<K, LAB> a, int list, int sbits0, RunState *st) {
       }
     }
     if (__any(dup) && lane == 0)
-      __hip_atomic_store(&st->msd_dups, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      raise_flag(&st->msd_dups);
     return;
   }
   const int sub_shift = sbits > kSubBits ? sbits - kSubBits : 0;
@@ -622,7 +729,9 @@ bk_sort_kernel(SortArgs<K, LAB> a, int list, int sbits0, RunState *st) {
           if (LAB) s_vals[dst[j]] = v[j];
         }
     }
-  } else {
+  } else if (sbits <= kRadixMaxBits || WPE > 4) {
+    // (the merge sort is compiled where the kernel has the registers for
+    // it: at most 4 waves per SIMD)
     // keys in the radix passes' layout, the slots past the item last; the
     // passes cover the item's bits below sbits
 #pragma unroll
@@ -635,6 +744,11 @@ bk_sort_kernel(SortArgs<K, LAB> a, int list, int sbits0, RunState *st) {
     const int passes = (eff + 7) / 8;
     lds_radix<K, LAB, BS, IPT>(k, v, passes, eff - 8 * (passes - 1), s_keys, s_vals, s_sub, s_dig,
                                s_w);
+  } else if constexpr (WPE <= 4) {
+    // many varying bits and crowded sub-buckets: merge sort (the counters
+    // are no longer needed; their space holds the positions)
+    lds_merge_sort<K, LAB, BS, IPT>(in, vin, s0, cnt, s_keys, s_vals,
+                                    reinterpret_cast<uint16_t *>(s_sub));
   }
   __syncthreads();
   // write out, looking for equal neighbours
@@ -650,7 +764,7 @@ bk_sort_kernel(SortArgs<K, LAB> a, int list, int sbits0, RunState *st) {
     }
   }
   if (__any(dup) && lane == 0)
-    __hip_atomic_store(&st->msd_dups, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    raise_flag(&st->msd_dups);
 }
 
 // ---- levels for segments above CAP -------------------------------------------------------

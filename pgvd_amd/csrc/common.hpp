@@ -157,6 +157,16 @@ __device__ inline void st_rel_u64(uint64_t *p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A flag that any number of workgroups may raise during a kernel and that
+// only later kernels (or the host) read: a store kept in the XCD's L2 and
+// written back when the kernel ends.  Every writer stores the same value, so
+// the order of the write-backs does not matter.  (A device-scope store
+// writes through the L2; thousands of workgroups doing that on one address
+// serialise at the memory and stall every CU's memory pipeline behind it.)
+__device__ inline void raise_flag(uint32_t *p) {
+  __hip_atomic_store(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // ---- wave64 helpers ---------------------------------------------------------
 __device__ inline uint64_t lanemask_lt() {
   const uint32_t lane = __lane_id();
