@@ -79,12 +79,57 @@ def lib():
         L.oracle_tree_free.argtypes = [C.c_void_p]
         L.oracle_run_pipeline.argtypes = [f32p, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_int64)]
         L.oracle_run_pipeline.restype = C.c_int64
+        L.oracle_multi_intersection.argtypes = [f32p] * 4
+        L.oracle_multi_intersection.restype = C.c_int
+        L.oracle_fit_boxes.argtypes = [f32p, f32p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
+        L.oracle_fit_boxes.restype = C.c_int64
+        L.oracle_multi_cells.argtypes = [f32p, i32p, C.c_int64, C.c_int, i32p, C.c_int64, i32p,
+                                         f32p, i32p]
+        L.oracle_find_multi_cells.argtypes = [i32p, f32p, C.c_int64, C.c_float, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                              C.POINTER(C.c_int64), C.POINTER(C.c_int)]
+        L.oracle_find_multi_cells.restype = C.c_int64
+        L.oracle_obj2oct.argtypes = [f32p, C.c_int64, f32p, C.c_float, C.c_int64, f32p]
+        L.oracle_obj2oct.restype = C.c_int
+        L.oracle_find_intersections.argtypes = [f32p, f32p, i32p, C.c_int, f32p]
+        L.oracle_find_intersections.restype = C.c_int
         _LIB = L
     return _LIB
 
 
 def ref_available(dim: int = 2) -> bool:
     return os.path.exists(os.path.join(HERE, "_ref", f"libpgvdref{dim}.so"))
+
+
+def ref_geom_available() -> bool:
+    return os.path.exists(os.path.join(HERE, "_ref", "libpgvdgeom2.so"))
+
+
+_REFG = None
+
+
+def ref_geom():
+    """The reference's 2D geometry (opencl/Geom.cpp, OctreeUtils.cpp,
+    CellIntersections.h) behind ref_geom.cpp."""
+    global _REFG
+    if _REFG is None:
+        L = C.CDLL(os.path.join(HERE, "_ref", "libpgvdgeom2.so"))
+        L.refg_multi_intersection.argtypes = [f32p] * 4
+        L.refg_multi_intersection.restype = C.c_int
+        L.refg_fit_boxes.argtypes = [f32p, f32p, C.c_float, f32p, f32p, f32p, C.c_int64,
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_int)]
+        L.refg_fit_boxes.restype = C.c_int64
+        L.refg_find_intersections.argtypes = [f32p, f32p, i32p, C.c_int, f32p]
+        L.refg_find_intersections.restype = C.c_int
+        L.refg_mc_new.argtypes = [C.c_int64]
+        L.refg_mc_new.restype = C.c_void_p
+        L.refg_mc_free.argtypes = [C.c_void_p]
+        L.refg_mc_visit.argtypes = [C.c_void_p, i32p, f32p, f32p, C.c_int]
+        L.refg_mc_visit.restype = C.c_int
+        L.refg_mc_read.argtypes = [C.c_void_p, i32p, f32p]
+        _REFG = L
+    return _REFG
 
 
 def ref(dim: int = 2):
@@ -355,3 +400,197 @@ def ref_istream_octree(text: bytes, dim: int, cap: int | None = None):
     if rc:
         return None
     return out[: O.value * ((1 << dim) + 1)].reshape(O.value, (1 << dim) + 1)
+
+
+# ---------------------------------------------------------------------------
+# GVD refinement (2D): multi-cells, FitBoxes and the Karras iteration
+# ---------------------------------------------------------------------------
+
+def _f2(p) -> np.ndarray:
+    return np.ascontiguousarray(p, dtype=np.float32).reshape(-1)
+
+
+def multi_intersection(s0, s1) -> bool:
+    """Geom::multi_intersection (opencl/Geom.cpp:109-151) of segments
+    (ax, ay, bx, by)."""
+    s0, s1 = _f2(s0), _f2(s1)
+    return bool(lib().oracle_multi_intersection(s0[:2].copy(), s0[2:].copy(), s1[:2].copy(),
+                                                s1[2:].copy()))
+
+
+def fit_boxes(s0, s1, min_d: float = 1.0) -> dict:
+    """Geom::FitBoxes (opencl/Geom.cpp:667-716): samples (k, 2), box origins
+    (b, 2), box widths (b,) and status (0, or the error it reports)."""
+    a, b = _f2(s0), _f2(s1)
+    L = lib()
+    nb, st = C.c_int64(), C.c_int()
+    k = L.oracle_fit_boxes(a, b, min_d, None, None, None, 0, C.byref(nb), C.byref(st))
+    cap = max(k, nb.value, 1)
+    smp = np.zeros(2 * cap, np.float32)
+    org = np.zeros(2 * cap, np.float32)
+    ln = np.zeros(cap, np.float32)
+    L.oracle_fit_boxes(a, b, min_d, smp.ctypes.data, org.ctypes.data, ln.ctypes.data, cap,
+                       C.byref(nb), C.byref(st))
+    return {"samples": smp[: 2 * k].reshape(k, 2), "origins": org[: 2 * nb.value].reshape(-1, 2),
+            "lengths": ln[: nb.value], "status": st.value}
+
+
+def obj2oct(verts: np.ndarray, bbmin, bbw: float, bits: int) -> np.ndarray:
+    """Octree2::obj2Oct (viewer/Octree2.cpp:215-225) of every vertex."""
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 2)
+    out = np.zeros_like(v)
+    if lib().oracle_obj2oct(v.reshape(-1), v.shape[0], _f2(bbmin), np.float32(bbw), 1 << bits,
+                            out.reshape(-1)):
+        raise ValueError("obj2Oct cannot have coord less than zero")
+    return out
+
+
+def polyline_segments(octv: np.ndarray, counts) -> tuple[np.ndarray, np.ndarray]:
+    """FindMultiCells' segments (viewer/Octree2.cpp:442-450): vertex i to
+    i+1 of every polygon, polygon by polygon, labelled with the polygon's
+    index."""
+    segs, labels, at = [], [], 0
+    for j, c in enumerate(np.asarray(counts, dtype=np.int64)):
+        p = octv[at: at + c]
+        if c >= 2:
+            segs.append(np.concatenate([p[:-1], p[1:]], axis=1))
+            labels.append(np.full(c - 1, j, np.int32))
+        at += c
+    if not segs:
+        return np.zeros((0, 4), np.float32), np.zeros(0, np.int32)
+    return (np.ascontiguousarray(np.concatenate(segs), dtype=np.float32),
+            np.ascontiguousarray(np.concatenate(labels), dtype=np.int32))
+
+
+def multi_cells(segs, labels, bits: int, children: np.ndarray, leaf: np.ndarray) -> dict:
+    """Every segment's cell walk with the MCCallback visitor into the
+    CellIntersections of every node: labels (O, 4, 2) with -1 for an empty
+    slot, segments (O, 4, 2, 4), status per segment."""
+    segs = np.ascontiguousarray(segs, dtype=np.float32).reshape(-1, 4)
+    labels = np.ascontiguousarray(labels, dtype=np.int32)
+    O = children.shape[0]
+    oct_ = np.ascontiguousarray(np.concatenate([children.reshape(O, -1), leaf.reshape(O, 1)],
+                                               axis=1).astype(np.int32))
+    lab = np.zeros(max(O, 1) * 8, np.int32)
+    sg = np.zeros(max(O, 1) * 32, np.float32)
+    st = np.zeros(max(segs.shape[0], 1), np.int32)
+    lib().oracle_multi_cells(segs.reshape(-1), labels, segs.shape[0], bits, oct_.reshape(-1), O,
+                             lab, sg, st)
+    return {"labels": lab[: 8 * O].reshape(O, 4, 2), "segs": sg[: 32 * O].reshape(O, 4, 2, 4),
+            "status": st[: segs.shape[0]]}
+
+
+def find_multi_cells(ci_labels: np.ndarray, ci_segs: np.ndarray, min_d: float = 1.0) -> dict:
+    """FindMultiCells' sampling: extra points (k, 2) int32, samples (k, 2),
+    box origins and widths, status."""
+    lab = np.ascontiguousarray(ci_labels, dtype=np.int32).reshape(-1)
+    sg = np.ascontiguousarray(ci_segs, dtype=np.float32).reshape(-1)
+    O = lab.size // 8
+    if O == 0:
+        lab, sg = np.full(8, -1, np.int32), np.zeros(32, np.float32)
+    L = lib()
+    nb, st = C.c_int64(), C.c_int()
+    k = L.oracle_find_multi_cells(lab, sg, O, min_d, None, None, None, None, 0, C.byref(nb),
+                                  C.byref(st))
+    cap = max(k, nb.value, 1)
+    ex = np.zeros(2 * cap, np.int32)
+    smp = np.zeros(2 * cap, np.float32)
+    org = np.zeros(2 * cap, np.float32)
+    ln = np.zeros(cap, np.float32)
+    L.oracle_find_multi_cells(lab, sg, O, min_d, ex.ctypes.data, smp.ctypes.data, org.ctypes.data,
+                              ln.ctypes.data, cap, C.byref(nb), C.byref(st))
+    return {"extra": ex[: 2 * k].reshape(k, 2), "samples": smp[: 2 * k].reshape(k, 2),
+            "origins": org[: 2 * nb.value].reshape(-1, 2), "lengths": ln[: nb.value],
+            "status": st.value}
+
+
+def gvd_refine(verts: np.ndarray, counts, bits: int, iterations: int, custom_bb=None,
+               min_d: float = 1.0) -> dict:
+    """Octree2::build(const Polylines&) (viewer/Octree2.cpp:127-190) with
+    its FindMultiCells call (:186) in place: quantize the vertices, then
+    build the octree, find the multi-cells and add their FitBoxes samples as
+    extra points, until `iterations` builds or no extra points."""
+    verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 2)
+    if custom_bb is not None:
+        bbmin = np.asarray(custom_bb[0], np.float32)
+        bbmax = np.asarray(custom_bb[1], np.float32)
+    else:
+        bbmin, bbmax = verts.min(axis=0), verts.max(axis=0)
+    bbw = np.float32(max(bbmax[0] - bbmin[0], bbmax[1] - bbmin[1]))
+    q = quantize(verts, 1 << bits).astype(np.int64)
+    segs, labels = polyline_segments(obj2oct(verts, bbmin, bbw, bits), counts)
+    extra = np.zeros((0, 2), np.int64)
+    it, history = 0, []
+    tree = ci = fm = None
+    while True:
+        q = np.concatenate([q, extra])
+        extra = np.zeros((0, 2), np.int64)
+        tree = build(q, bits) if q.shape[0] > 1 else None
+        if tree is not None:
+            ci = multi_cells(segs, labels, bits, tree["children"], tree["leaf"])
+            fm = find_multi_cells(ci["labels"], ci["segs"], min_d)
+            if fm["status"]:
+                raise RuntimeError(f"FitBoxes error {fm['status']}")
+            extra = fm["extra"].astype(np.int64)
+        it += 1
+        history.append(extra.shape[0])
+        if not (it < iterations and extra.shape[0] > 0):
+            break
+    return {"qpoints": q, "tree": tree, "ci": ci, "multi": fm, "extra": extra,
+            "iterations": it, "extra_per_iteration": history, "segs": segs, "labels": labels}
+
+
+def ref_multi_intersection(s0, s1) -> bool:
+    s0, s1 = _f2(s0), _f2(s1)
+    return bool(ref_geom().refg_multi_intersection(s0[:2].copy(), s0[2:].copy(), s1[:2].copy(),
+                                                   s1[2:].copy()))
+
+
+def ref_fit_boxes(s0, s1, min_d: float = 1.0, cap: int = 1 << 16) -> dict:
+    """The reference's Geom::FitBoxes on the segments."""
+    a, b = _f2(s0), _f2(s1)
+    smp = np.zeros(2 * cap, np.float32)
+    org = np.zeros(2 * cap, np.float32)
+    ln = np.zeros(cap, np.float32)
+    nb, st = C.c_int64(), C.c_int()
+    k = ref_geom().refg_fit_boxes(a, b, min_d, smp, org, ln, cap, C.byref(nb), C.byref(st))
+    if k > cap or nb.value > cap:
+        raise ValueError("reference FitBoxes output exceeds the capacity")
+    return {"samples": smp[: 2 * k].reshape(k, 2), "origins": org[: 2 * nb.value].reshape(-1, 2),
+            "lengths": ln[: nb.value], "status": st.value}
+
+
+def find_intersections(a, b, origin, width: int) -> np.ndarray:
+    """OctreeUtils::FindIntersections: rows (t, x, y)."""
+    out = np.zeros(12, np.float32)
+    n = lib().oracle_find_intersections(_f2(a), _f2(b), np.ascontiguousarray(origin, dtype=np.int32),
+                                        int(width), out)
+    return out[: 3 * n].reshape(n, 3)
+
+
+def ref_find_intersections(a, b, origin, width: int) -> np.ndarray:
+    out = np.zeros(12, np.float32)
+    n = ref_geom().refg_find_intersections(_f2(a), _f2(b),
+                                           np.ascontiguousarray(origin, dtype=np.int32),
+                                           int(width), out)
+    return out[: 3 * n].reshape(n, 3)
+
+
+def ref_multi_cells(segs, labels, walk_cells, offsets, O: int) -> dict:
+    """MCCallback on the reference's FindIntersections and CellIntersections
+    for the given walk (cells of segment s at offsets[s]..offsets[s+1])."""
+    L = ref_geom()
+    h = L.refg_mc_new(O)
+    try:
+        segs = np.ascontiguousarray(segs, dtype=np.float32).reshape(-1, 4)
+        cells = np.ascontiguousarray(walk_cells, dtype=np.int32).reshape(-1, 5)
+        for s in range(segs.shape[0]):
+            a, b = segs[s, :2].copy(), segs[s, 2:].copy()
+            for c in range(int(offsets[s]), int(offsets[s + 1])):
+                L.refg_mc_visit(h, cells[c].copy(), a, b, int(labels[s]))
+        lab = np.zeros(max(O, 1) * 8, np.int32)
+        sg = np.zeros(max(O, 1) * 32, np.float32)
+        L.refg_mc_read(h, lab, sg)
+    finally:
+        L.refg_mc_free(h)
+    return {"labels": lab[: 8 * O].reshape(O, 4, 2), "segs": sg[: 32 * O].reshape(O, 4, 2, 4)}

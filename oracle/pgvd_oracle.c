@@ -700,14 +700,19 @@ static int cw_find_intersections(const float a[2], const float b[2], const int32
 }
 
 /* FindLeaf of a float point after convert_int2 (truncation, opencl/vec_cpp.h
- * CONVERT_VEC2); origin = the point with its low log2(width) bits cleared. */
+ * CONVERT_VEC2); origin = the sum of the octant offsets along the descent
+ * (OctreeUtils.cpp:41-47), i.e. the point's low `bits` bits (those its key
+ * holds) with the low log2(width) bits cleared: a coordinate of exactly the
+ * grid width (obj2Oct's clamp rounds back to it in float from 2^12 up) lands
+ * in the cell at 0. */
 static void cw_leaf(const float p[2], int bits, const int32_t *octree, int64_t O, int32_t cell[5]) {
   const int64_t q[2] = {(int32_t)p[0], (int32_t)p[1]};
   int32_t parent, octant, data, level, st;
   oracle_find_leaf(q, 1, 2, bits, octree, O, &parent, &octant, &data, &level, &st);
   const int32_t width = (int32_t)(1u << (bits - level));
-  cell[0] = (int32_t)q[0] & ~(width - 1);
-  cell[1] = (int32_t)q[1] & ~(width - 1);
+  const int32_t gmask = (int32_t)((1u << bits) - 1);   /* the key holds the low `bits` bits */
+  cell[0] = ((int32_t)q[0] & gmask) & ~(width - 1);
+  cell[1] = ((int32_t)q[1] & gmask) & ~(width - 1);
   cell[2] = width;
   cell[3] = parent;
   cell[4] = octant;
@@ -746,4 +751,530 @@ int64_t oracle_cell_walk(const float *segs, int64_t n, int bits, const int32_t *
     status[s] = st;
   }
   return total;
+}
+
+/* ---- GVD refinement: multi-cells and FitBoxes (2D) ---------------------- */
+/* The reference's vector type is OpenCL's cl_float2 with the component-wise
+ * operators of opencl/vec_cpp.h (v op scalar and v op v per component;
+ * dot accumulates from 0; length = sqrt(dot); unit = v / length), and
+ * FloatSegment (opencl/FloatSegment.h) is two of them. */
+typedef struct { float x, y; } gv2;
+typedef struct { gv2 a, b; } gseg;
+
+static gv2 g_mk(float x, float y) { gv2 r; r.x = x; r.y = y; return r; }
+static gv2 g_add(gv2 u, gv2 v) { return g_mk(u.x + v.x, u.y + v.y); }
+static gv2 g_sub(gv2 u, gv2 v) { return g_mk(u.x - v.x, u.y - v.y); }
+static gv2 g_muls(gv2 u, float s) { return g_mk(u.x * s, u.y * s); }
+static gv2 g_divs(gv2 u, float s) { return g_mk(u.x / s, u.y / s); }
+static float g_dot(gv2 u, gv2 v) { float r = 0; r += u.x * v.x; r += u.y * v.y; return r; }
+static float g_len(gv2 v) { return sqrtf(g_dot(v, v)); }
+static gv2 g_unit(gv2 v) { return g_divs(v, g_len(v)); }
+static int g_eq(gv2 u, gv2 v) { return u.x == v.x && u.y == v.y; }
+static float g_dist(gv2 a, gv2 b) { return g_len(g_sub(a, b)); }          /* opencl/Geom.h:16-18 */
+static gseg g_seg(gv2 a, gv2 b) { gseg s; s.a = a; s.b = b; return s; }
+static gv2 g_vec(gseg s) { return g_sub(s.b, s.a); }
+static gseg g_rev(gseg s) { return g_seg(s.b, s.a); }
+static int g_sgn(float f) { return f < 0 ? -1 : (f > 0 ? 1 : 0); }         /* opencl/Geom.h:10-12 */
+
+/* Geom::closest(p, s) (opencl/Geom.cpp:15-29); its EPSILON is a local
+ * float 1e-6. */
+static gv2 g_closest_pt(gv2 p, gseg s) {
+  if (g_eq(s.a, s.b)) return s.a;
+  const float eps = (float)1e-6;
+  gv2 v = g_vec(s);
+  const float len = g_len(v);
+  v = g_divs(v, len);
+  const gv2 u = g_sub(p, s.a);
+  const float t = g_dot(u, v);
+  gv2 q = g_add(s.a, g_muls(v, t));
+  if (t < 0 || g_dist(q, s.a) < eps) q = s.a;
+  else if (t > len || g_dist(q, s.b) < eps) q = s.b;
+  return q;
+}
+
+/* Geom::x_given_y, y_given_x (opencl/Geom.cpp:31-53). */
+static int g_x_given_y(gseg s, float y, float *x) {
+  const gv2 p = s.a, v = g_vec(s);
+  const float t = (y - p.y) / v.y;
+  *x = p.x + t * v.x;
+  return t <= 1;
+}
+static int g_y_given_x(gseg s, float x, float *y) {
+  const gv2 p = s.a, v = g_vec(s);
+  const float t = (x - p.x) / v.x;
+  *y = p.y + t * v.y;
+  return t <= 1;
+}
+
+/* Geom::clip_to_box (opencl/Geom.cpp:55-96). */
+static int g_clip_to_box(gseg s, float d, gseg *out) {
+  const gv2 c = s.a;
+  const float d2 = d / 2;
+  const gv2 sv = g_vec(s);
+  int valid;
+  float x, y;
+  if (fabsf(sv.x) > fabsf(sv.y)) {
+    x = c.x + d2;
+    if (sv.x < 0) x = c.x - d2;
+    valid = g_y_given_x(s, x, &y);
+  } else {
+    y = c.y + d2;
+    if (sv.y < 0) y = c.y - d2;
+    valid = g_x_given_y(s, y, &x);
+  }
+  if (valid) *out = g_seg(g_mk(x, y), s.b);
+  return valid;
+}
+
+/* Geom::multi_intersection (opencl/Geom.cpp:109-151): parallel segments on
+ * one line that overlap in more than a point.  The slope is a float
+ * quotient widened to double; the collinearity test compares in double. */
+int oracle_multi_intersection(const float *p1, const float *p2, const float *p3, const float *p4) {
+  const float x1 = p1[0], x2 = p2[0], x3 = p3[0], x4 = p4[0];
+  const float y1 = p1[1], y2 = p2[1], y3 = p3[1], y4 = p4[1];
+  const float d = (x1 - x2) * (y3 - y4) - (y1 - y2) * (x3 - x4);
+  if (d == 0) {
+    if (x2 - x1 != 0) {
+      const double m = (y2 - y1) / (x2 - x1);
+      if ((double)(y1 - y3) == m * (double)(x1 - x3)) {
+        const float len1 = fabsf(x2 - x1), len2 = fabsf(x4 - x3);
+        const float len = fmaxf(x1, fmaxf(x2, fmaxf(x3, x4))) - fminf(x1, fminf(x2, fminf(x3, x4)));
+        if (len < len1 + len2) return 1;
+      }
+    } else if (x1 == x3) {
+      const float len1 = fabsf(y2 - y1), len2 = fabsf(y4 - y3);
+      const float len = fmaxf(y1, fmaxf(y2, fmaxf(y3, y4))) - fminf(y1, fminf(y2, fminf(y3, y4)));
+      if (len < len1 + len2) return 1;
+    }
+  }
+  return 0;
+}
+
+/* Geom::line_intersection (opencl/Geom.cpp:165-206), in double; the point
+ * is rounded to float. */
+static gv2 g_line_isect(gv2 p1, gv2 p2, gv2 p3, gv2 p4, int *lines, int *segs) {
+  const double x1 = p1.x, x2 = p2.x, x3 = p3.x, x4 = p4.x;
+  const double y1 = p1.y, y2 = p2.y, y3 = p3.y, y4 = p4.y;
+  const double d = (x1 - x2) * (y3 - y4) - (y1 - y2) * (x3 - x4);
+  if (d == 0) {
+    *lines = 0;
+    *segs = 0;
+    return g_mk(0, 0);
+  }
+  *lines = 1;
+  const double pre = (x1 * y2 - y1 * x2), post = (x3 * y4 - y3 * x4);
+  const double x = (pre * (x3 - x4) - (x1 - x2) * post) / d;
+  const double y = (pre * (y3 - y4) - (y1 - y2) * post) / d;
+  *segs = 1;
+  if (x < fmin(x1, x2) || x > fmax(x1, x2) || x < fmin(x3, x4) || x > fmax(x3, x4)) *segs = 0;
+  if (y < fmin(y1, y2) || y > fmax(y1, y2) || y < fmin(y3, y4) || y > fmax(y3, y4)) *segs = 0;
+  return g_mk((float)x, (float)y);
+}
+
+/* Geom::closest(a, b, ...) for two segments (opencl/Geom.cpp:259-346). */
+static void g_closest_segs(gseg a, gseg b, gv2 *ca, gv2 *cb, int *ca_end, int *cb_end) {
+  int li, si;
+  const gv2 l = g_line_isect(a.a, a.b, b.a, b.b, &li, &si);
+  if (si) {
+    *ca = *cb = l;
+    *ca_end = *cb_end = 0;
+    return;
+  }
+  gv2 c = g_closest_pt(a.a, b);
+  *ca = a.a;
+  *cb = c;
+  *ca_end = 1;
+  *cb_end = g_eq(c, b.a) || g_eq(c, b.b);
+  float dist = g_len(g_sub(a.a, c));
+
+  gv2 c_ = g_closest_pt(a.b, b);
+  float dist_ = g_len(g_sub(a.b, c_));
+  if (dist_ < dist) {
+    c = c_;
+    *ca = a.b;
+    *cb = c;
+    *ca_end = 1;
+    *cb_end = g_eq(c, b.a) || g_eq(c, b.b);
+    dist = dist_;
+  }
+  c_ = g_closest_pt(b.a, a);
+  dist_ = g_len(g_sub(b.a, c_));
+  if (dist_ < dist) {
+    c = c_;
+    *ca = c;
+    *cb = b.a;
+    *ca_end = g_eq(c, a.a) || g_eq(c, a.b);
+    *cb_end = 1;
+    dist = dist_;
+  }
+  c_ = g_closest_pt(b.b, a);
+  dist_ = g_len(g_sub(b.b, c_));
+  if (dist_ < dist) {
+    c = c_;
+    *ca = c;
+    *cb = b.b;
+    *ca_end = g_eq(c, a.a) || g_eq(c, a.b);
+    *cb_end = 1;
+    dist = dist_;
+  }
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) {
+      const gv2 a0 = i ? a.b : a.a, a1 = i ? a.a : a.b;
+      const gv2 b0 = j ? b.b : b.a, b1 = j ? b.a : b.b;
+      if (a0.x == b0.x) {
+        const int day = g_sgn(a1.y - a0.y), dby = g_sgn(b1.y - b0.y), daby = g_sgn(b0.y - a0.y);
+        if (day != dby && day != daby) {
+          *ca = a0;
+          *cb = b0;
+          *ca_end = *cb_end = 1;
+        }
+      } else if (a0.y == b0.y) {
+        const int dax = g_sgn(a1.x - a0.x), dbx = g_sgn(b1.x - b0.x), dabx = g_sgn(b0.x - a0.x);
+        if (dax != dbx && dax != dabx) {
+          *ca = a0;
+          *cb = b0;
+          *ca_end = *cb_end = 1;
+        }
+      }
+    }
+}
+
+/* Output of FitBoxes: samples, counted past the capacity. */
+typedef struct {
+  float *samples;
+  int64_t cap, n;
+} g_out;
+
+static void g_emit(g_out *o, gv2 s0, int two, gv2 s1) {
+  const int64_t k = o->n;
+  if (o->samples) {
+    if (k < o->cap) { o->samples[2 * k] = s0.x; o->samples[2 * k + 1] = s0.y; }
+    if (two && k + 1 < o->cap) { o->samples[2 * k + 2] = s1.x; o->samples[2 * k + 3] = s1.y; }
+  }
+  o->n += two ? 2 : 1;
+}
+
+/* Box origins and widths, in the order the reference pushes them. */
+typedef struct {
+  float *origins, *lengths;
+  int64_t cap, n;
+} g_boxes;
+
+static void g_box(g_boxes *b, gv2 org, float len) {
+  if (b->origins && b->n < b->cap) {
+    b->origins[2 * b->n] = org.x;
+    b->origins[2 * b->n + 1] = org.y;
+    b->lengths[b->n] = len;
+  }
+  ++b->n;
+}
+
+#define FIT_OFFSET_FACTOR 4
+#define FIT_MAX_ROUNDS 4096
+
+/* Geom::FitBoxesNoIntersection (opencl/Geom.cpp:471-664).  Returns 0, or
+ * the reference's logic_error as 1 ("doesn't support intersections"), 2 ("no
+ * end but not intersecting"), 3 ("must include one endpoint"); 4 when the
+ * loop runs FIT_MAX_ROUNDS rounds (the reference has no bound). */
+static int g_fit_no_isect(gseg A, gseg B, float min_d, g_out *o, g_boxes *bx) {
+  int done = 0, rounds = 0;
+  while (!done) {
+    if (++rounds > FIT_MAX_ROUNDS) return 4;
+    gv2 ca, cb, c_cb;
+    int ca_end, cb_end, c_cb_end;
+    g_closest_segs(A, B, &ca, &cb, &ca_end, &cb_end);
+    if (g_eq(ca, cb)) return 1;
+    if (!ca_end && !cb_end) return 2;
+    if (cb_end) {
+      gv2 t = ca; ca = cb; cb = t;
+      int te = ca_end; ca_end = cb_end; cb_end = te;
+      gseg ts = A; A = B; B = ts;
+    }
+    if (!ca_end) return 3;
+    if (!g_eq(A.a, ca)) A = g_rev(A);
+    if (cb_end && !g_eq(B.a, cb)) B = g_rev(B);
+
+    float d;
+    gv2 org;
+    gv2 off = g_mk(0, 0);
+    if (g_dist(ca, cb) < min_d) {
+      org = g_sub(g_divs(g_add(ca, cb), 2), g_mk(min_d / 2, min_d / 2));
+      d = min_d;
+      c_cb = cb;
+      c_cb_end = cb_end;
+    } else if (cb_end && (A.a.x == B.a.x || A.a.y == B.a.y)) {
+      d = g_dist(A.a, B.a);
+      org = g_mk(fminf(A.a.x, B.a.x), fminf(A.a.y, B.a.y));
+      if (A.a.x == B.a.x) {
+        off = g_mk(d / FIT_OFFSET_FACTOR, 0);
+        if (g_vec(A).x < 0) {
+          org = g_sub(org, g_mk(d, 0));
+          off = g_muls(off, -1);
+        }
+      } else if (A.a.y == B.a.y) {
+        off = g_mk(0, d / 16);
+        if (g_vec(A).y < 0) {
+          org = g_sub(org, g_mk(0, d));
+          off = g_muls(off, -1);
+        }
+      }
+      c_cb = cb;
+      c_cb_end = 1;
+    } else {
+      const gv2 v = g_sub(cb, ca);
+      d = g_len(v);
+      const float cx = (v.x >= 0) ? 2 * d : -2 * d;
+      const float cy = (v.y >= 0) ? 2 * d : -2 * d;
+      const gseg Cs = g_seg(ca, g_add(ca, g_mk(cx, cy)));
+      gv2 c_cc;
+      int c_cc_end;
+      g_closest_segs(B, Cs, &c_cb, &c_cc, &c_cb_end, &c_cc_end);
+      float d_ = g_dist(ca, c_cb) / (float)sqrt(2.0);
+      if (!g_eq(c_cb, c_cc)) d_ = fmaxf(fabsf(c_cb.x - ca.x), fabsf(c_cb.y - ca.y));
+      if (d_ < d) d = d_;
+      const float bxd = (v.x >= 0) ? 0 : -d;
+      const float byd = (v.y >= 0) ? 0 : -d;
+      org = g_add(ca, g_mk(bxd, byd));
+    }
+    const gv2 s0 = g_add(g_add(ca, g_divs(g_sub(cb, ca), FIT_OFFSET_FACTOR)), off);
+    const gv2 s1 = g_add(g_add(cb, g_divs(g_sub(ca, cb), FIT_OFFSET_FACTOR)), off);
+    g_emit(o, s0, 1, s1);
+    g_box(bx, org, d);
+
+    if (c_cb_end) {
+      if (!g_eq(B.a, c_cb)) B = g_rev(B);
+    } else {
+      const gseg B0 = g_seg(c_cb, B.a), B1 = g_seg(c_cb, B.b);
+      if (g_dot(g_unit(g_vec(A)), g_unit(g_vec(B0))) > 0) B = B0;
+      else B = B1;
+    }
+
+    if (g_dot(g_unit(g_vec(A)), g_unit(g_vec(B))) <= 0) {
+      done = 1;
+    } else {
+      const gv2 bis = g_unit(g_add(g_unit(g_vec(A)), g_unit(g_vec(B))));
+      int horizontal;
+      if (A.a.x == B.a.x) horizontal = 1;
+      else if (A.a.y == B.a.y) horizontal = 0;
+      else horizontal = fabsf(bis.x) > fabsf(bis.y);
+      float Ax, Ay, Bx, By;
+      int Ai, Bi;
+      if (horizontal) {
+        Ax = Bx = (bis.x < 0) ? org.x : org.x + d;
+        Ai = g_y_given_x(A, Ax, &Ay);
+        Bi = g_y_given_x(B, Bx, &By);
+      } else {
+        Ay = By = (bis.y < 0) ? org.y : org.y + d;
+        Ai = g_x_given_y(A, Ay, &Ax);
+        Bi = g_x_given_y(B, By, &Bx);
+      }
+      done = !Ai || !Bi;
+      if (g_eq(A.a, g_mk(Ax, Ay)) && g_eq(B.a, g_mk(Bx, By))) done = 1;
+      A = g_seg(g_mk(Ax, Ay), A.b);
+      B = g_seg(g_mk(Bx, By), B.b);
+    }
+  }
+  return 0;
+}
+
+/* Geom::FitBoxes (opencl/Geom.cpp:667-716).  Where the segments meet, the
+ * reference clips the four half-segments to the box around the meeting
+ * point and reads the unit vectors of the first pair even when clipping
+ * failed (default-constructed, uninitialised segments); here a failed clip
+ * leaves a zero segment, whose unit vector is NaN, so the pairs are not
+ * swapped. */
+static int g_fit_boxes(gseg A, gseg B, float min_d, g_out *o, g_boxes *bx) {
+  gv2 ca, cb;
+  int ca_end, cb_end;
+  g_closest_segs(A, B, &ca, &cb, &ca_end, &cb_end);
+  if (g_eq(ca, cb)) {
+    g_emit(o, ca, 0, ca);
+    g_box(bx, g_sub(ca, g_mk(min_d / 2, min_d / 2)), min_d);
+    if (A.a.x > A.b.x) A = g_rev(A);
+    if (B.a.x > B.b.x) B = g_rev(B);
+    const gseg A0o = g_seg(ca, A.a), A1o = g_seg(ca, A.b);
+    const gseg B0o = g_seg(cb, B.a), B1o = g_seg(cb, B.b);
+    gseg A0, A1, B0, B1;
+    memset(&A0, 0, sizeof A0); memset(&A1, 0, sizeof A1);
+    memset(&B0, 0, sizeof B0); memset(&B1, 0, sizeof B1);
+    int A0v = g_clip_to_box(A0o, min_d, &A0);
+    int A1v = g_clip_to_box(A1o, min_d, &A1);
+    const int B0v = g_clip_to_box(B0o, min_d, &B0);
+    const int B1v = g_clip_to_box(B1o, min_d, &B1);
+    const float dotp = g_dot(g_unit(g_vec(A0)), g_unit(g_vec(B0)));
+    if (dotp < 0) {
+      gseg t = A0; A0 = A1; A1 = t;
+      int tv = A0v; A0v = A1v; A1v = tv;
+    }
+    int rc = 0;
+    if (A0v && B0v) rc = g_fit_no_isect(A0, B0, min_d, o, bx);
+    if (!rc && A1v && B1v) rc = g_fit_no_isect(A1, B1, min_d, o, bx);
+    return rc;
+  }
+  return g_fit_no_isect(A, B, min_d, o, bx);
+}
+
+int64_t oracle_fit_boxes(const float *a, const float *b, float min_d, float *samples,
+                         float *origins, float *lengths, int64_t cap, int64_t *boxes_out,
+                         int *status) {
+  g_out o = {samples, cap, 0};
+  g_boxes bx = {origins, lengths, cap, 0};
+  const gseg A = g_seg(g_mk(a[0], a[1]), g_mk(a[2], a[3]));
+  const gseg B = g_seg(g_mk(b[0], b[1]), g_mk(b[2], b[3]));
+  *status = g_fit_boxes(A, B, min_d, &o, &bx);
+  if (boxes_out) *boxes_out = bx.n;
+  return o.n;
+}
+
+/* CellIntersections::set (CellIntersections.h:20-34): per octant two label
+ * slots; a new label takes the first empty slot, a known label keeps its
+ * longer segment (strictly longer replaces), a third label is dropped. */
+static void ci_set(int32_t *l, float *segs, int octant, int32_t label, gseg s) {
+  for (int i = octant * 2; i < (octant + 1) * 2; ++i) {
+    if (l[i] == -1) {
+      l[i] = label;
+      segs[4 * i] = s.a.x; segs[4 * i + 1] = s.a.y;
+      segs[4 * i + 2] = s.b.x; segs[4 * i + 3] = s.b.y;
+      break;
+    }
+    if (l[i] == label) {
+      const gseg cur = g_seg(g_mk(segs[4 * i], segs[4 * i + 1]), g_mk(segs[4 * i + 2], segs[4 * i + 3]));
+      if (g_dot(g_vec(s), g_vec(s)) > g_dot(g_vec(cur), g_vec(cur))) {
+        segs[4 * i] = s.a.x; segs[4 * i + 1] = s.a.y;
+        segs[4 * i + 2] = s.b.x; segs[4 * i + 3] = s.b.y;
+      }
+      break;
+    }
+  }
+}
+
+/* MCCallback (viewer/Octree2.cpp:366-428) for one visited cell: the
+ * segment's part inside the cell (two edge intersections, or one and the
+ * endpoint whose truncated coordinates lie in the cell's half-open box),
+ * stored under the cell's parent node and octant.  Returns 1 and the part
+ * when it is stored. */
+static int mc_part(const float a[2], const float b[2], const int32_t cell[5], gseg *part) {
+  cell_isect li[4];
+  const int n = cw_find_intersections(a, b, cell, cell[2], li);
+  if (n == 2) {
+    *part = g_seg(g_mk(li[0].p[0], li[0].p[1]), g_mk(li[1].p[0], li[1].p[1]));
+    return 1;
+  }
+  if (n == 1) {
+    const int32_t mn[2] = {cell[0], cell[1]}, mx[2] = {cell[0] + cell[2], cell[1] + cell[2]};
+    const int32_t ai[2] = {(int32_t)a[0], (int32_t)a[1]}, bi[2] = {(int32_t)b[0], (int32_t)b[1]};
+    gv2 q;
+    if (ai[0] >= mn[0] && ai[0] < mx[0] && ai[1] >= mn[1] && ai[1] < mx[1]) q = g_mk(a[0], a[1]);
+    else if (bi[0] >= mn[0] && bi[0] < mx[0] && bi[1] >= mn[1] && bi[1] < mx[1]) q = g_mk(b[0], b[1]);
+    else return 0;   /* degenerate: nothing stored */
+    *part = g_seg(g_mk(li[0].p[0], li[0].p[1]), q);
+    return 1;
+  }
+  return 0;
+}
+
+/* FindMultiCells' walk (viewer/Octree2.cpp:430-455): every segment (n, as
+ * ax ay bx by in grid units, in the reference's polygon-by-polygon order)
+ * walked with CellWalk and the MCCallback visitor into the
+ * CellIntersections of every octree node (ci_labels O*8, ci_segs O*32).
+ * status[s] = 1 when segment s's walk passed 10000 cells. */
+void oracle_multi_cells(const float *segs, const int32_t *labels, int64_t n, int bits,
+                        const int32_t *octree, int64_t O, int32_t *ci_labels, float *ci_segs,
+                        int32_t *status) {
+  for (int64_t i = 0; i < O * 8; ++i) ci_labels[i] = -1;
+  memset(ci_segs, 0, sizeof(float) * (size_t)(O * 32));
+  for (int64_t s = 0; s < n; ++s) {
+    const float *a = segs + 4 * s, *b = segs + 4 * s + 2;
+    int dir[2];
+    for (int i = 0; i < 2; ++i) dir[i] = b[i] > a[i] ? 1 : (b[i] < a[i] ? -1 : 0);
+    cell_isect cur = {0.0f, {a[0], a[1]}};
+    int32_t bcell[5], cell[5], ncell[5];
+    cw_leaf(b, bits, octree, O, bcell);
+    cw_leaf(cur.p, bits, octree, O, cell);
+    int count = 0, st = 0, done = 0;
+    do {
+      if (++count > 10000) { st = 1; break; }
+      gseg part;
+      if (mc_part(a, b, cell, &part) && cell[3] >= 0 && cell[3] < O)
+        ci_set(ci_labels + 8 * (int64_t)cell[3], ci_segs + 32 * (int64_t)cell[3], cell[4], labels[s], part);
+      cell_isect li[4];
+      const int nl = cw_find_intersections(a, b, cell, cell[2], li);
+      for (int i = 0; i < nl; ++i)
+        if (li[i].t > cur.t) cur = li[i];
+      for (int i = 0; i < 2; ++i) {
+        const int end = cell[i];
+        if (cur.p[i] == (float)end && cur.p[i] != 0 && dir[i] == -1) cur.p[i] -= 1;
+      }
+      cw_leaf(cur.p, bits, octree, O, ncell);
+      done = nl == 0 || (cell[0] == bcell[0] && cell[1] == bcell[1]) ||
+             (ncell[0] == cell[0] && ncell[1] == cell[1]);
+      memcpy(cell, ncell, sizeof cell);
+    } while (!done);
+    if (status) status[s] = st;
+  }
+}
+
+/* FindMultiCells' sampling (viewer/Octree2.cpp:457-505): for every node and
+ * octant holding two labels whose segments do not overlap on one line
+ * (Geom::multi_intersection), FitBoxes with min_d; the samples truncated to
+ * int (convert_intn) are the extra points, in node, octant, sample order.
+ * Returns the number of extra points (written up to cap); boxes get the
+ * box origins and widths (nullable).  *status: 0, or the first FitBoxes
+ * error (see g_fit_no_isect). */
+int64_t oracle_find_multi_cells(const int32_t *ci_labels, const float *ci_segs, int64_t O,
+                                float min_d, int32_t *extra, float *samples, float *origins,
+                                float *lengths, int64_t cap, int64_t *boxes_out, int *status) {
+  g_out o = {samples, cap, 0};
+  g_boxes bx = {origins, lengths, cap, 0};
+  *status = 0;
+  for (int64_t i = 0; i < O; ++i)
+    for (int oc = 0; oc < 4; ++oc) {
+      const int32_t *l = ci_labels + 8 * i + 2 * oc;
+      const int nl = l[0] < 0 ? 0 : (l[1] < 0 ? 1 : 2);
+      if (nl < 2) continue;
+      const float *s0 = ci_segs + 32 * i + 8 * oc, *s1 = s0 + 4;
+      if (oracle_multi_intersection(s0, s0 + 2, s1, s1 + 2)) continue;
+      const gseg A = g_seg(g_mk(s0[0], s0[1]), g_mk(s0[2], s0[3]));
+      const gseg B = g_seg(g_mk(s1[0], s1[1]), g_mk(s1[2], s1[3]));
+      const int64_t before = o.n;
+      const int rc = g_fit_boxes(A, B, min_d, &o, &bx);
+      if (rc && !*status) *status = rc;
+      if (extra && samples)
+        for (int64_t k = before; k < o.n && k < cap; ++k) {
+          extra[2 * k] = (int32_t)samples[2 * k];
+          extra[2 * k + 1] = (int32_t)samples[2 * k + 1];
+        }
+    }
+  if (boxes_out) *boxes_out = bx.n;
+  return o.n;
+}
+
+/* Octree2::obj2Oct (viewer/Octree2.cpp:215-225): ((v - min) / bbw) * width
+ * per component in float; a coordinate at or past the width becomes
+ * width - 0.0001 (a double rounded to float).  Returns -1 when a coordinate
+ * is below zero (the reference throws). */
+int oracle_obj2oct(const float *v, int64_t n, const float *bbmin, float bbw, int64_t width,
+                   float *out) {
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < 2; ++k) {
+      float c = ((v[2 * i + k] - bbmin[k]) / bbw) * (float)width;
+      if (c >= (float)width) c = (float)((double)width - 0.0001);
+      if (c < 0) return -1;
+      out[2 * i + k] = c;
+    }
+  return 0;
+}
+
+/* OctreeUtils::FindIntersections (OctreeUtils.cpp:80-172) of segment ab
+ * with the cell at origin of the given width: up to 4 rows (t, x, y). */
+int oracle_find_intersections(const float *a, const float *b, const int32_t *origin, int width,
+                              float *out) {
+  cell_isect r[4];
+  const int n = cw_find_intersections(a, b, origin, width, r);
+  for (int i = 0; i < n; ++i) {
+    out[3 * i] = r[i].t;
+    out[3 * i + 1] = r[i].p[0];
+    out[3 * i + 2] = r[i].p[1];
+  }
+  return n;
 }

@@ -132,6 +132,33 @@ int oracle_octree_parse_text(const char *text, int64_t len, int dim, int32_t *oc
 int64_t oracle_run_pipeline(const float *pts, int64_t n, int dim, int bits,
                             int64_t *m_out);
 
+/* ---- GVD refinement (2D) ------------------------------------------- */
+/* OctreeUtils::FindIntersections (OctreeUtils.cpp:80-172): rows (t, x, y). */
+int oracle_find_intersections(const float *a, const float *b, const int32_t *origin, int width,
+                              float *out);
+/* Geom::multi_intersection (opencl/Geom.cpp:109-151). */
+int oracle_multi_intersection(const float *p1, const float *p2, const float *p3, const float *p4);
+/* Geom::FitBoxes (opencl/Geom.cpp:467-716) of segments a, b (ax ay bx by):
+ * returns the sample count; samples (2 floats each), box origins (2 each)
+ * and widths written while fewer than cap; *boxes_out the box count;
+ * *status 0 or the FitBoxes error (1-3 the reference's logic_errors, 4 its
+ * loop past 4096 rounds). */
+int64_t oracle_fit_boxes(const float *a, const float *b, float min_d, float *samples,
+                         float *origins, float *lengths, int64_t cap, int64_t *boxes_out,
+                         int *status);
+/* FindMultiCells' walks with MCCallback (viewer/Octree2.cpp:366-455):
+ * CellIntersections of every node, ci_labels O*8 (-1 empty), ci_segs O*32. */
+void oracle_multi_cells(const float *segs, const int32_t *labels, int64_t n, int bits,
+                        const int32_t *octree, int64_t O, int32_t *ci_labels, float *ci_segs,
+                        int32_t *status);
+/* FindMultiCells' sampling (viewer/Octree2.cpp:457-505): extra points. */
+int64_t oracle_find_multi_cells(const int32_t *ci_labels, const float *ci_segs, int64_t O,
+                                float min_d, int32_t *extra, float *samples, float *origins,
+                                float *lengths, int64_t cap, int64_t *boxes_out, int *status);
+/* Octree2::obj2Oct (viewer/Octree2.cpp:215-225). */
+int oracle_obj2oct(const float *v, int64_t n, const float *bbmin, float bbw, int64_t width,
+                   float *out);
+
 #ifdef __cplusplus
 }
 #endif

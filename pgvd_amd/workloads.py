@@ -109,3 +109,38 @@ def batch_points(sizes, seed: int = 5, dim: int = 3) -> np.ndarray:
         off += int(n)
     return out
 
+
+
+def polyline_scene(loops: int = 25, sub: int = 2, seed: int = 7, gap: float = 0.02,
+                   jitter: float = 0.05):
+    """A 2D scene of closed polylines for the GVD refinement: `loops`
+    quadrilaterals on a g x g grid (g = ceil(sqrt(loops))) over [-1,1]^2
+    whose column widths and row heights vary by up to +-25% (so that the
+    gaps do not fall on the octree's power-of-two cell boundaries), each
+    rotated by U[-jitter, jitter] radians and sized so that neighbours face
+    each other across about `gap`; every side is cut into `sub` segments.
+    Across each narrow gap two loops pass through the same empty octree
+    cells, where the refinement adds points.  Every loop repeats its first
+    vertex at the end, as the reference's polyline files close their loops.
+    Returns (vertices float32 (sum counts, 2), counts int64 per loop)."""
+    rng = np.random.default_rng(seed)
+    g = int(np.ceil(np.sqrt(loops)))
+    edges = []
+    for _ in range(2):
+        w = rng.uniform(0.75, 1.25, g)
+        edges.append(np.concatenate([[0.0], np.cumsum(w / w.sum() * 2.0)]) - 1.0)
+    unit = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], np.float64)
+    out, counts = [], []
+    for i in range(loops):
+        x0, x1 = edges[0][i % g], edges[0][i % g + 1]
+        y0, y1 = edges[1][i // g], edges[1][i // g + 1]
+        half = np.array([(x1 - x0 - gap) / 2, (y1 - y0 - gap) / 2])
+        rot = rng.uniform(-jitter, jitter)
+        c, s_ = np.cos(rot), np.sin(rot)
+        C = (unit * half) @ np.array([[c, s_], [-s_, c]]) / (abs(c) + abs(s_))
+        P = np.array([C[e] + (C[(e + 1) % 4] - C[e]) * (t / sub) for e in range(4) for t in range(sub)])
+        P = P + [(x0 + x1) / 2, (y0 + y1) / 2]
+        P = np.concatenate([P, P[:1]])
+        out.append(P)
+        counts.append(P.shape[0])
+    return np.concatenate(out).astype(np.float32), np.asarray(counts, np.int64)

@@ -1,0 +1,330 @@
+"""The oracle's restatement of the GVD refinement step (2D): FindMultiCells'
+walks with MCCallback (viewer/Octree2.cpp:366-455), CellIntersections::set
+(CellIntersections.h:20-34), Geom::multi_intersection and Geom::FitBoxes
+(opencl/Geom.cpp:109-151, 467-716) and the Karras iteration
+(viewer/Octree2.cpp:171-189).
+
+Each is pinned against the reference's own code: opencl/Geom.cpp,
+OctreeUtils.cpp and CellIntersections.h compiled unmodified into
+oracle/_ref/libpgvdgeom2.so.  Outputs are compared bit for bit."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from pgvd_amd import workloads
+
+needs_ref = pytest.mark.skipif(not oracle.ref_geom_available(),
+                               reason="reference geometry checker (oracle/_ref) not built")
+
+
+def _bits_equal(x, y) -> bool:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    return x.shape == y.shape and np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+def _pairs(kind: str, n: int, seed: int) -> np.ndarray:
+    """Segment pairs (n, 2, 4) of the shapes FitBoxes meets."""
+    rng = np.random.default_rng(seed)
+    W = float(1 << int(rng.integers(6, 21)))
+    a = (rng.random((n, 4)) * W).astype(np.float32)
+    if kind == "random":
+        b = (rng.random((n, 4)) * W).astype(np.float32)
+    elif kind == "near_parallel":
+        # b: a turned by a small angle and moved off by a small distance
+        d = a[:, 2:] - a[:, :2]
+        nrm = np.stack([-d[:, 1], d[:, 0]], 1) / np.linalg.norm(d, axis=1, keepdims=True)
+        off = nrm * rng.uniform(0.2, 40.0, (n, 1))
+        ang = rng.uniform(-0.2, 0.2, n)
+        c, s = np.cos(ang), np.sin(ang)
+        dd = np.stack([c * d[:, 0] - s * d[:, 1], s * d[:, 0] + c * d[:, 1]], 1)
+        p = a[:, :2] + off + d * rng.uniform(-0.3, 0.3, (n, 1))
+        b = np.concatenate([p, p + dd * rng.uniform(0.3, 1.2, (n, 1))], 1).astype(np.float32)
+    elif kind == "crossing":
+        m = (a[:, :2] + a[:, 2:]) / 2
+        d = rng.standard_normal((n, 2)) * W / 8
+        b = np.concatenate([m - d, m + d], 1).astype(np.float32)
+    elif kind == "axis_aligned":
+        # endpoints that share an x or a y with the other segment's end
+        b = (rng.random((n, 4)) * W).astype(np.float32)
+        same = rng.integers(0, 2, n)
+        b[same == 0, 0] = a[same == 0, 0]
+        b[same == 1, 1] = a[same == 1, 1]
+    elif kind == "close":
+        # an endpoint of b within about one grid unit of a
+        t = rng.random((n, 1)).astype(np.float32)
+        p = a[:, :2] + (a[:, 2:] - a[:, :2]) * t + rng.uniform(-1.2, 1.2, (n, 2))
+        q = p + rng.standard_normal((n, 2)) * W / 16
+        b = np.concatenate([p, q], 1).astype(np.float32)
+    elif kind == "grid":
+        # cell-clipped parts: endpoints on integer cell edges, as MCCallback
+        # stores them
+        a = np.floor(a)
+        b = np.floor(rng.random((n, 4)) * W)
+        b[:, 0] = a[:, 0] + rng.integers(-3, 4, n)
+        a = a.astype(np.float32)
+        b = b.astype(np.float32)
+    else:
+        raise ValueError(kind)
+    return np.stack([a, b.astype(np.float32)], 1)
+
+
+KINDS = ["random", "near_parallel", "crossing", "axis_aligned", "close", "grid"]
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", KINDS)
+def test_fit_boxes_matches_reference(kind):
+    """Samples, box origins and widths and the error status of FitBoxes
+    (min_d = 1, as FindMultiCells calls it) equal the reference's bit for
+    bit on 1500 pairs of each shape.  Pairs where the oracle's loop would
+    pass its 4096-round bound (the reference's has none) are not run on
+    the reference."""
+    pairs = _pairs(kind, 1500, seed=KINDS.index(kind) + 11)
+    compared = errors = 0
+    for s0, s1 in pairs:
+        o = oracle.fit_boxes(s0, s1, 1.0)
+        if o["status"] == 4:
+            continue
+        r = oracle.ref_fit_boxes(s0, s1, 1.0)
+        assert o["status"] == r["status"], (s0, s1)
+        assert _bits_equal(o["samples"], r["samples"]), (s0, s1)
+        assert _bits_equal(o["origins"], r["origins"]), (s0, s1)
+        assert _bits_equal(o["lengths"], r["lengths"]), (s0, s1)
+        compared += 1
+        errors += o["status"] != 0
+    assert compared >= 1400
+    if kind == "near_parallel":
+        # the loop runs many rounds on these
+        assert max(len(oracle.fit_boxes(p[0], p[1])["lengths"]) for p in pairs[:200]) > 5
+
+
+@needs_ref
+def test_multi_intersection_matches_reference():
+    """Collinear overlapping, collinear disjoint, vertical, parallel and
+    random pairs, including exact integer-grid cases."""
+    rng = np.random.default_rng(21)
+    cases = []
+    for _ in range(3000):
+        k = int(rng.integers(0, 5))
+        if k == 0:     # collinear on a random integer line, overlapping or not
+            p = rng.integers(0, 1000, 2)
+            d = rng.integers(-20, 21, 2)
+            t = np.sort(rng.integers(-10, 11, 4))
+            if rng.random() < 0.5:
+                t = t[[0, 2, 1, 3]]
+            pts = [p + d * ti for ti in t]
+            cases.append(np.array([*pts[0], *pts[1], *pts[2], *pts[3]], np.float32))
+        elif k == 1:   # vertical
+            x = float(rng.integers(0, 100))
+            ys = rng.integers(0, 50, 4).astype(np.float32)
+            x2 = x if rng.random() < 0.7 else x + 1
+            cases.append(np.array([x, ys[0], x, ys[1], x2, ys[2], x2, ys[3]], np.float32))
+        elif k == 2:   # parallel offset
+            p = rng.random(2) * 100
+            d = rng.standard_normal(2)
+            q = p + rng.standard_normal(2)
+            cases.append(np.array([*p, *(p + d), *q, *(q + 2 * d)], np.float32))
+        else:
+            cases.append((rng.random(8) * 100).astype(np.float32))
+    hits = 0
+    for c in cases:
+        want = oracle.ref_multi_intersection(c[:4], c[4:])
+        assert oracle.multi_intersection(c[:4], c[4:]) == want, c
+        hits += want
+    assert 200 < hits < len(cases) - 200
+
+
+@needs_ref
+def test_find_intersections_matches_reference():
+    """OctreeUtils::FindIntersections on cells of every size, segments
+    through, along and past them (cases with three or more candidates whose
+    pairs all have |dot| = 0 are where the reference keeps uninitialised
+    values, and are not generated: no such case arises from a segment of
+    positive length)."""
+    rng = np.random.default_rng(22)
+    for _ in range(4000):
+        w = 1 << int(rng.integers(0, 12))
+        o = rng.integers(0, 1 << 12, 2) & ~(w - 1)
+        if rng.random() < 0.3:
+            a = (o + rng.integers(0, 2, 2) * w).astype(np.float32)
+            b = (a + rng.integers(-2, 3, 2) * w).astype(np.float32)
+        else:
+            a = (o + rng.uniform(-w, 2 * w, 2)).astype(np.float32)
+            b = (o + rng.uniform(-w, 2 * w, 2)).astype(np.float32)
+        if np.array_equal(a, b):
+            continue
+        got = oracle.find_intersections(a, b, o, w)
+        want = oracle.ref_find_intersections(a, b, o, w)
+        assert _bits_equal(got, want), (a, b, o, w)
+
+
+def _scene_tree(loops, sub, seed, gap, bits):
+    v, c = workloads.polyline_scene(loops, sub, seed, gap)
+    lo = v.min(axis=0)
+    bbw = np.float32((v.max(axis=0) - lo).max())
+    q = oracle.quantize(v, 1 << bits)
+    tree = oracle.build(q, bits)
+    segs, labels = oracle.polyline_segments(oracle.obj2oct(v, lo, bbw, bits), c)
+    return tree, segs, labels
+
+
+def _data2_scene(name):
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "data_points.npz"))
+    return d[name], d[name + "_counts"]
+
+
+@needs_ref
+@pytest.mark.parametrize("loops,sub,seed,gap,bits", [
+    (9, 1, 7, 0.02, 12), (25, 2, 7, 0.02, 16), (64, 2, 8, 0.01, 16), (16, 3, 3, 0.05, 8),
+    (100, 2, 5, 0.004, 20)])
+def test_multi_cells_matches_reference(loops, sub, seed, gap, bits):
+    """Every segment's walk fed through the reference's FindIntersections,
+    MCCallback's decisions and CellIntersections::set gives the oracle's
+    CellIntersections: labels and kept segments of every node and octant,
+    bit for bit."""
+    tree, segs, labels = _scene_tree(loops, sub, seed, gap, bits)
+    O = tree["octree_size"]
+    got = oracle.multi_cells(segs, labels, bits, tree["children"], tree["leaf"])
+    assert not got["status"].any()
+    w = oracle.cell_walk(segs, bits, tree["children"], tree["leaf"])
+    want = oracle.ref_multi_cells(segs, labels, w["cells"], w["offsets"], O)
+    assert np.array_equal(got["labels"], want["labels"])
+    assert _bits_equal(got["segs"], want["segs"])
+    assert (got["labels"][:, :, 0] >= 0).sum() > segs.shape[0] // 2
+
+
+@needs_ref
+@pytest.mark.parametrize("name", ["data2_test1", "data2_test2", "data2_test3", "data2_test4"])
+@pytest.mark.parametrize("bits", [8, 12, 16])
+def test_multi_cells_reference_scenes(name, bits):
+    """The reference's own polyline scenes (data2/), walked polyline by
+    polyline as FindMultiCells walks them."""
+    v, c = _data2_scene(name)
+    lo = v.min(axis=0)
+    bbw = np.float32((v.max(axis=0) - lo).max())
+    tree = oracle.build(oracle.quantize(v, 1 << bits), bits)
+    segs, labels = oracle.polyline_segments(oracle.obj2oct(v, lo, bbw, bits), c)
+    got = oracle.multi_cells(segs, labels, bits, tree["children"], tree["leaf"])
+    w = oracle.cell_walk(segs, bits, tree["children"], tree["leaf"])
+    want = oracle.ref_multi_cells(segs, labels, w["cells"], w["offsets"], tree["octree_size"])
+    assert np.array_equal(got["labels"], want["labels"])
+    assert _bits_equal(got["segs"], want["segs"])
+
+
+@needs_ref
+@pytest.mark.parametrize("loops,sub,seed,gap,bits", [(25, 2, 7, 0.02, 16), (100, 3, 5, 0.004, 20)])
+def test_find_multi_cells_matches_reference(loops, sub, seed, gap, bits):
+    """The extra points are the reference's FitBoxes samples, truncated to
+    int, of every node and octant with two labels whose segments do not
+    overlap on one line, in node, octant, sample order."""
+    tree, segs, labels = _scene_tree(loops, sub, seed, gap, bits)
+    ci = oracle.multi_cells(segs, labels, bits, tree["children"], tree["leaf"])
+    got = oracle.find_multi_cells(ci["labels"], ci["segs"])
+    assert got["status"] == 0
+    smp, org, ln = [], [], []
+    for i in range(ci["labels"].shape[0]):
+        for o in range(4):
+            if ci["labels"][i, o, 1] < 0:
+                continue
+            s0, s1 = ci["segs"][i, o, 0], ci["segs"][i, o, 1]
+            if oracle.ref_multi_intersection(s0, s1):
+                continue
+            r = oracle.ref_fit_boxes(s0, s1)
+            assert r["status"] == 0
+            smp.append(r["samples"])
+            org.append(r["origins"])
+            ln.append(r["lengths"])
+    assert smp, "the scene has no multi-cell"
+    smp = np.concatenate(smp)
+    assert _bits_equal(got["samples"], smp)
+    assert _bits_equal(got["origins"], np.concatenate(org))
+    assert _bits_equal(got["lengths"], np.concatenate(ln))
+    assert np.array_equal(got["extra"], smp.astype(np.int32))
+
+
+def test_cell_intersections_set_brute_force():
+    """CellIntersections::set's fold, restated in Python on random visit
+    sequences (first two labels per octant in arrival order, strictly
+    longer segment replaces), against the oracle's on a one-node octree:
+    every walk visits the root's octants."""
+    rng = np.random.default_rng(31)
+    bits = 4
+    tree = oracle.build(np.array([[0, 0], [15, 15]]), bits)      # root with two point leaves
+    for trial in range(50):
+        n = int(rng.integers(1, 30))
+        segs = (rng.random((n, 4)) * 15.99).astype(np.float32)
+        labels = rng.integers(0, 4, n).astype(np.int32)
+        got = oracle.multi_cells(segs, labels, bits, tree["children"], tree["leaf"])
+        w = oracle.cell_walk(segs, bits, tree["children"], tree["leaf"])
+        lab = np.full((tree["octree_size"], 4, 2), -1, np.int32)
+        sg = np.zeros((tree["octree_size"], 4, 2, 4), np.float32)
+        for s in range(n):
+            a, b = segs[s, :2], segs[s, 2:]
+            for c in w["cells"][w["offsets"][s]:w["offsets"][s + 1]]:
+                part = _mc_part(a, b, c)
+                if part is None:
+                    continue
+                node, oc = c[3], c[4]
+                for j in range(2):
+                    if lab[node, oc, j] == -1:
+                        lab[node, oc, j] = labels[s]
+                        sg[node, oc, j] = part
+                        break
+                    if lab[node, oc, j] == labels[s]:
+                        if _len2(part) > _len2(sg[node, oc, j]):
+                            sg[node, oc, j] = part
+                        break
+        assert np.array_equal(got["labels"], lab), trial
+        assert _bits_equal(got["segs"], sg), trial
+
+
+def _len2(s):
+    d = np.float32(s[2]) - np.float32(s[0]), np.float32(s[3]) - np.float32(s[1])
+    return np.float32(np.float32(0) + d[0] * d[0]) + d[1] * d[1]
+
+
+def _mc_part(a, b, c):
+    r = oracle.find_intersections(a, b, c[:2], int(c[2]))
+    if r.shape[0] == 2:
+        return np.array([r[0, 1], r[0, 2], r[1, 1], r[1, 2]], np.float32)
+    if r.shape[0] == 1:
+        lo, hi = c[:2], c[:2] + c[2]
+        for q in (a, b):
+            qi = q.astype(np.int32)
+            if (qi >= lo).all() and (qi < hi).all():
+                return np.array([r[0, 1], r[0, 2], q[0], q[1]], np.float32)
+    return None
+
+
+def test_refine_iteration_contract():
+    """The Karras loop: every build's extra points are added to the next
+    build's points; it stops at the iteration limit or when a build adds
+    none, and the last build's extra points are reported."""
+    v, c = workloads.polyline_scene(25, 2, 7, 0.02)
+    full = oracle.gvd_refine(v, c, 16, 8)
+    n0 = v.shape[0]
+    hist = full["extra_per_iteration"]
+    assert full["iterations"] == len(hist) and hist[-1] == 0 and hist[0] > 0
+    assert full["qpoints"].shape[0] == n0 + sum(hist[:-1])
+    one = oracle.gvd_refine(v, c, 16, 1)
+    assert one["iterations"] == 1 and one["qpoints"].shape[0] == n0
+    assert np.array_equal(one["extra"], full["qpoints"][n0:n0 + hist[0]])
+    assert full["qpoints"].shape[0] > n0
+
+
+def test_obj2oct_clamp():
+    """obj2Oct: the bounding box's far side maps to width - 0.0001 rounded
+    to float, which is the width itself from 2^12 up; below zero raises."""
+    v = np.array([[0, 0], [1, 1], [0.5, 0.25]], np.float32)
+    for bits in (8, 11, 12, 13, 16, 20):
+        q = oracle.obj2oct(v, np.zeros(2, np.float32), np.float32(1), bits)
+        assert q[1, 0] == np.float32(float(1 << bits) - 0.0001)
+        assert (q[1, 0] == (1 << bits)) == (bits >= 12)
+        assert q[2, 0] == np.float32(0.5 * (1 << bits))
+    with pytest.raises(ValueError):
+        oracle.obj2oct(v, np.ones(2, np.float32), np.float32(1), 8)
