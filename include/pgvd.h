@@ -42,7 +42,11 @@ typedef enum pgvd_status {
   PGVD_ERR_TOO_LARGE = 8,   /* n >= 2^30 */
   PGVD_ERR_DUP_KEY = 9,     /* BRT input has equal neighbouring keys */
   PGVD_ERR_INTERNAL = 10,
-  PGVD_NOT_READY = 11       /* pgvd_server_poll: the request is not complete yet */
+  PGVD_NOT_READY = 11,      /* pgvd_server_poll: the request is not complete yet */
+  PGVD_ERR_GEOMETRY = 12    /* the GVD refinement met a case where the reference
+                               throws: FitBoxes' logic_errors (opencl/Geom.cpp:
+                               492-505) or a cell walk past 10000 cells
+                               (viewer/Octree2.cpp:290) */
 } pgvd_status;
 
 /* Message for the last failing call on this thread. */
@@ -405,6 +409,124 @@ int pgvd_build_octree_noalloc(const void *pts, int64_t n, int dim, int bits,
                               const uint32_t *labels, const pgvd_build_opts *opts,
                               int32_t *octree, int64_t octree_cap, uint32_t *node_conflict,
                               pgvd_build_state *state, pgvd_stream s);
+
+/* ---- GVD refinement (2D) ------------------------------------------------
+ * The step after the octree in the reference's viewer: Octree2::build(const
+ * Polylines&) (viewer/Octree2.cpp:127-190) builds the octree of the
+ * polylines' vertices, finds the cells two polylines pass through
+ * (FindMultiCells, :430-506, whose call at :186 the reference keeps
+ * commented out) and adds the samples Geom::FitBoxes places between the two
+ * (opencl/Geom.cpp:467-716) as extra points of the next build.  Every
+ * output is bit-exact with the reference's float and double arithmetic.
+ * Workspace buffers belong to the context: calls on one context must not
+ * overlap on different streams. */
+
+/* CellIntersections (CellIntersections.h:9-55) of one octree node: per
+ * octant o the first two labels whose segments reached the cell, at slots
+ * 2o and 2o+1 (-1 where empty), and the longest part of each label's
+ * segments inside the cell (a strictly longer part replaced the kept one).
+ * 160 bytes. */
+typedef struct pgvd_cell_intersections {
+  int32_t label[8];
+  float seg[8][4];          /* ax, ay, bx, by in grid units; 0 where empty */
+} pgvd_cell_intersections;
+
+/* FindMultiCells' segments (viewer/Octree2.cpp:442-450): Octree2::obj2Oct
+ * (:215-225) of every vertex (verts: device, nv = sum(counts) points, x y
+ * float, polygon after polygon; counts: host, P vertex counts), grid
+ * coordinates ((v - bb_min) / bbw * 2^bits in float; a coordinate at or past
+ * 2^bits becomes 2^bits - 0.0001 rounded to float), then vertex i to i+1
+ * of every polygon, labelled with the polygon's index.  segs (device,
+ * 4 floats per segment) and labels (device int32) receive the
+ * sum(max(counts - 1, 0)) segments; *nseg_out (host) their number.
+ * PGVD_ERR_BAD_COORD when a coordinate falls below zero (obj2Oct throws).
+ * 1 <= bits <= 30.  Synchronizes. */
+int pgvd_polyline_segments(pgvd_ctx *ctx, const float *verts, const int64_t *counts, int64_t P,
+                           const float bb_min[2], float bbw, int bits, float *segs,
+                           int32_t *labels, int64_t *nseg_out, pgvd_stream s);
+
+/* FindMultiCells' walks (viewer/Octree2.cpp:430-455): each of n segments
+ * (device, ax ay bx by in grid units, in the order the reference walks
+ * them: polygon by polygon) is walked through the 2D octree (device,
+ * octree_size nodes as pgvd_build_octree writes them) as pgvd_cell_walk
+ * walks it, and MCCallback (:366-428) keeps, for every visited cell, the
+ * segment's part inside it (the two edge intersections, or the one and the
+ * endpoint whose truncated coordinates lie in the cell's half-open box)
+ * under labels[i] (device int32) in ci (device, octree_size records).
+ * walk_status (device, n int32, nullable): 1 where a walk passed 10000
+ * cells (the reference throws; the first 10000 are kept).  Returns after
+ * the walk's cell count is read back; the rest is asynchronous. */
+int pgvd_multi_cells(pgvd_ctx *ctx, const float *segs, const int32_t *labels, int64_t n, int bits,
+                     const int32_t *octree, int64_t octree_size, pgvd_cell_intersections *ci,
+                     int32_t *walk_status, pgvd_stream s);
+
+/* Geom::FitBoxes(A, B, min_d, &samples, &origins, &lengths) for n pairs
+ * (pairs: device, 8 floats each, A's ax ay bx by then B's).  Outputs
+ * (device): the samples (2 floats each), box origins (2 floats each) and
+ * widths of all pairs, pair after pair, while they fit `capacity` entries;
+ * sample_offsets and box_offsets (n + 1 int64 each, nullable): where each
+ * pair's entries start; status (n int32, nullable): 0, or the reference's
+ * logic_error as 1 ("doesn't support intersections"), 2 ("no end but not
+ * intersecting"), 3 ("must include one endpoint"), or 4 when the loop ran
+ * 4096 rounds (the reference's has no bound).  *samples_out and *boxes_out
+ * (host) receive the totals; PGVD_ERR_CAPACITY when either exceeds
+ * capacity (nothing is then written), PGVD_ERR_GEOMETRY when some pair
+ * failed (everything is written; the message names the first).
+ * Synchronizes. */
+int pgvd_fit_boxes(pgvd_ctx *ctx, const float *pairs, int64_t n, float min_d, float *samples,
+                   float *origins, float *lengths, int64_t capacity, int64_t *sample_offsets,
+                   int64_t *box_offsets, int64_t *samples_out, int64_t *boxes_out, int32_t *status,
+                   pgvd_stream s);
+
+/* FindMultiCells' sampling (viewer/Octree2.cpp:457-505): for every node
+ * and octant of ci (device, octree_size records) that holds two labels
+ * whose parts do not overlap on one line (Geom::multi_intersection,
+ * opencl/Geom.cpp:109-151), FitBoxes with min_d; the samples truncated to
+ * int (convert_intn) are the extra points, in node, octant and sample
+ * order.  Outputs (device, nullable each): extra (2 int32 per point),
+ * samples, origins (2 floats each) and lengths, up to capacity entries.
+ * *samples_out and *boxes_out (host) receive the totals;
+ * PGVD_ERR_CAPACITY or PGVD_ERR_GEOMETRY as pgvd_fit_boxes.  Synchronizes. */
+int pgvd_find_multi_cells(pgvd_ctx *ctx, const pgvd_cell_intersections *ci, int64_t octree_size,
+                          float min_d, int32_t *extra, float *samples, float *origins,
+                          float *lengths, int64_t capacity, int64_t *samples_out,
+                          int64_t *boxes_out, pgvd_stream s);
+
+typedef struct pgvd_gvd_result {
+  int iterations;           /* octree builds run                                    */
+  int64_t n_qpoints;        /* points of the last build: the quantized vertices and
+                               every extra point added before it                 */
+  int64_t octree_size;      /* nodes of the last octree (0: fewer than 2 points)  */
+  int64_t n_segments;
+  int64_t n_extra;          /* extra points found on the last octree (not added)  */
+  int64_t n_boxes;
+  /* device pointers owned by the context, valid until its next call:          */
+  const int32_t *qpoints;   /* n_qpoints x 2                                        */
+  const int32_t *octree;    /* octree_size nodes of 5 int32                         */
+  const pgvd_cell_intersections *ci;  /* octree_size records                       */
+  const int32_t *extra;     /* n_extra x 2                                          */
+  const float *samples;     /* n_extra x 2 (the extra points before truncation)     */
+  const float *origins;     /* n_boxes x 2                                          */
+  const float *lengths;     /* n_boxes                                              */
+} pgvd_gvd_result;
+
+/* Octree2::build(const Polylines&) (viewer/Octree2.cpp:127-190) with its
+ * FindMultiCells call in place, on the device: the vertices (device, nv =
+ * sum(counts) 2D points, polygon after polygon; counts on the host) are
+ * quantized with Karras::Quantize (their own box, not clamped); the
+ * segments come from obj2Oct with custom_bb (host: min x, min y, max x,
+ * max y; NULL: the vertices' box).  Then, up to max_iterations times: the
+ * octree of the points (BuildOctree_p), the multi-cells, and FitBoxes'
+ * extra points (min_d; the reference passes 1), which join the points of
+ * the next build; it stops early when a build yields none.
+ * extra_per_iteration (host, max_iterations int64, nullable) receives each
+ * build's extra-point count.  With fewer than two points no octree is
+ * built and nothing is walked (the reference reads an empty octree there).
+ * PGVD_ERR_GEOMETRY where the reference throws (FitBoxes' logic_errors, a
+ * walk past 10000 cells).  1 <= bits <= 30.  Synchronizes. */
+int pgvd_gvd_refine(pgvd_ctx *ctx, const float *verts, const int64_t *counts, int64_t P, int bits,
+                    const float *custom_bb, int max_iterations, float min_d,
+                    int64_t *extra_per_iteration, pgvd_gvd_result *out, pgvd_stream s);
 
 /* ---- batched form -------------------------------------------------------
  * Many independent octree builds of uneven size through one segmented

@@ -109,6 +109,14 @@ class ServedResult(C.Structure):
 PGVD_NOT_READY = 11
 
 
+class GvdResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("n_qpoints", C.c_int64), ("octree_size", C.c_int64),
+                ("n_segments", C.c_int64), ("n_extra", C.c_int64), ("n_boxes", C.c_int64),
+                ("qpoints", C.c_void_p), ("octree", C.c_void_p), ("ci", C.c_void_p),
+                ("extra", C.c_void_p), ("samples", C.c_void_p), ("origins", C.c_void_p),
+                ("lengths", C.c_void_p)]
+
+
 def lib():
     """Load libpgvd.so; raises if it has not been built."""
     global _LIB
@@ -174,6 +182,15 @@ def lib():
             "pgvd_compare_big_unsigned": ([vp, vp], i32),
             "pgvd_brt_to_ref_nodes": ([vp, vp, i64, i32, vp], i32),
             "pgvd_ref_nodes_to_brt": ([vp, i64, i32, vp, vp], i32),
+            "pgvd_polyline_segments": ([vp, vp, C.POINTER(i64), i64, C.POINTER(C.c_float),
+                                        C.c_float, i32, vp, vp, C.POINTER(i64), vp], i32),
+            "pgvd_multi_cells": ([vp, vp, vp, i64, i32, vp, i64, vp, vp, vp], i32),
+            "pgvd_fit_boxes": ([vp, vp, i64, C.c_float, vp, vp, vp, i64, vp, vp, C.POINTER(i64),
+                                C.POINTER(i64), vp, vp], i32),
+            "pgvd_find_multi_cells": ([vp, vp, i64, C.c_float, vp, vp, vp, vp, i64,
+                                       C.POINTER(i64), C.POINTER(i64), vp], i32),
+            "pgvd_gvd_refine": ([vp, vp, C.POINTER(i64), i64, i32, C.POINTER(C.c_float), i32,
+                                 C.c_float, C.POINTER(i64), C.POINTER(GvdResult), vp], i32),
             "pgvd_host_alloc": ([C.c_size_t, C.POINTER(vp)], i32),
             "pgvd_host_free": ([vp], i32),
         }
@@ -474,6 +491,162 @@ class Device:
             for b in (ds, do, doff, dst, dc):
                 if b is not None:
                     b.free()
+
+    # ---- GVD refinement (2D; viewer/Octree2.cpp:127-190, 366-506) ----
+    def polyline_segments(self, verts: np.ndarray, counts, bb_min, bbw: float, bits: int):
+        """FindMultiCells' segments: obj2Oct of every vertex and vertex i to i+1
+        of every polygon, labelled by polygon.  Returns (segs (k, 4), labels)."""
+        v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 2)
+        cnt = np.ascontiguousarray(counts, dtype=np.int64)
+        nseg = int(np.maximum(cnt - 1, 0).sum())
+        dv = self.upload(v)
+        ds, dl = self.alloc(16 * max(nseg, 1)), self.alloc(4 * max(nseg, 1))
+        n = C.c_int64()
+        bmin = (C.c_float * 2)(*[float(x) for x in bb_min])
+        try:
+            self._check(lib().pgvd_polyline_segments(
+                self.ctx, dv.ptr, cnt.ctypes.data_as(C.POINTER(C.c_int64)), cnt.size, bmin,
+                float(bbw), bits, ds.ptr, dl.ptr, C.byref(n), None))
+            return (self.download(ds, np.float32, 4 * n.value).reshape(-1, 4),
+                    self.download(dl, np.int32, n.value))
+        finally:
+            for b in (dv, ds, dl):
+                b.free()
+
+    def multi_cells(self, segs: np.ndarray, labels, bits: int, children: np.ndarray,
+                    leaf: np.ndarray) -> dict:
+        """The CellIntersections of every node of a 2D octree (host arrays) for
+        the labelled segments: labels (O, 4, 2), segs (O, 4, 2, 4), and each
+        walk's status."""
+        segs = np.ascontiguousarray(segs, dtype=np.float32).reshape(-1, 4)
+        lab = np.ascontiguousarray(labels, dtype=np.int32)
+        n = segs.shape[0]
+        O = children.shape[0]
+        oct_ = np.ascontiguousarray(np.concatenate([children.reshape(O, -1), leaf.reshape(O, 1)],
+                                                   axis=1).astype(np.int32))
+        ds, dl, do = self.upload(segs), self.upload(lab), self.upload(oct_)
+        dci, dst = self.alloc(160 * O), self.alloc(4 * max(n, 1))
+        try:
+            self._check(lib().pgvd_multi_cells(self.ctx, ds.ptr, dl.ptr, n, bits, do.ptr, O,
+                                               dci.ptr, dst.ptr, None))
+            raw = self.download(dci, np.int32, 40 * O).reshape(O, 40)
+            return {"labels": raw[:, :8].reshape(O, 4, 2).copy(),
+                    "segs": raw[:, 8:].copy().view(np.float32).reshape(O, 4, 2, 4),
+                    "status": self.download(dst, np.int32, n)}
+        finally:
+            for b in (ds, dl, do, dci, dst):
+                b.free()
+
+    def fit_boxes(self, pairs: np.ndarray, min_d: float = 1.0) -> dict:
+        """Geom::FitBoxes for every row (A's ax ay bx by, B's) of pairs:
+        samples, origins, lengths, per-pair offsets and status.  Pairs that
+        fail keep their status; the call's PGVD_ERR_GEOMETRY is not raised."""
+        pairs = np.ascontiguousarray(pairs, dtype=np.float32).reshape(-1, 8)
+        n = pairs.shape[0]
+        dp = self.upload(pairs)
+        dso, dbo, dst = self.alloc(8 * (n + 1)), self.alloc(8 * (n + 1)), self.alloc(4 * max(n, 1))
+        ns, nb = C.c_int64(), C.c_int64()
+        bufs = [dp, dso, dbo, dst]
+        try:
+            rc = lib().pgvd_fit_boxes(self.ctx, dp.ptr, n, min_d, None, None, None, 0, dso.ptr,
+                                      dbo.ptr, C.byref(ns), C.byref(nb), dst.ptr, None)
+            if rc not in (0, 7, 12):
+                self._check(rc)
+            cap = max(ns.value, nb.value, 1)
+            ds, do, dln = self.alloc(8 * cap), self.alloc(8 * cap), self.alloc(4 * cap)
+            bufs += [ds, do, dln]
+            rc = lib().pgvd_fit_boxes(self.ctx, dp.ptr, n, min_d, ds.ptr, do.ptr, dln.ptr, cap,
+                                      dso.ptr, dbo.ptr, C.byref(ns), C.byref(nb), dst.ptr, None)
+            if rc not in (0, 12):
+                self._check(rc)
+            return {"samples": self.download(ds, np.float32, 2 * ns.value).reshape(-1, 2),
+                    "origins": self.download(do, np.float32, 2 * nb.value).reshape(-1, 2),
+                    "lengths": self.download(dln, np.float32, nb.value),
+                    "sample_offsets": self.download(dso, np.int64, n + 1),
+                    "box_offsets": self.download(dbo, np.int64, n + 1),
+                    "status": self.download(dst, np.int32, n)}
+        finally:
+            for b in bufs:
+                b.free()
+
+    def find_multi_cells(self, ci_labels: np.ndarray, ci_segs: np.ndarray,
+                         min_d: float = 1.0) -> dict:
+        """FindMultiCells' sampling of CellIntersections records (host arrays
+        as multi_cells returns them): extra points, samples, origins, lengths."""
+        lab = np.ascontiguousarray(ci_labels, dtype=np.int32).reshape(-1, 8)
+        O = lab.shape[0]
+        rec = np.concatenate([lab, np.ascontiguousarray(ci_segs, dtype=np.float32)
+                              .reshape(O, 32).view(np.int32)], axis=1)
+        dci = self.upload(np.ascontiguousarray(rec))
+        ns, nb = C.c_int64(), C.c_int64()
+        bufs = [dci]
+        try:
+            rc = lib().pgvd_find_multi_cells(self.ctx, dci.ptr, O, min_d, None, None, None, None, 0,
+                                             C.byref(ns), C.byref(nb), None)
+            if rc not in (0, 7):
+                self._check(rc)
+            cap = max(ns.value, nb.value, 1)
+            de, ds, do, dln = (self.alloc(8 * cap), self.alloc(8 * cap), self.alloc(8 * cap),
+                               self.alloc(4 * cap))
+            bufs += [de, ds, do, dln]
+            self._check(lib().pgvd_find_multi_cells(self.ctx, dci.ptr, O, min_d, de.ptr, ds.ptr,
+                                                    do.ptr, dln.ptr, cap, C.byref(ns),
+                                                    C.byref(nb), None))
+            return {"extra": self.download(de, np.int32, 2 * ns.value).reshape(-1, 2),
+                    "samples": self.download(ds, np.float32, 2 * ns.value).reshape(-1, 2),
+                    "origins": self.download(do, np.float32, 2 * nb.value).reshape(-1, 2),
+                    "lengths": self.download(dln, np.float32, nb.value)}
+        finally:
+            for b in bufs:
+                b.free()
+
+    def gvd_refine_device(self, dverts: DeviceBuffer, counts, bits: int, iterations: int,
+                          custom_bb=None, min_d: float = 1.0):
+        """pgvd_gvd_refine on device vertices: (GvdResult, extra points per
+        iteration)."""
+        cnt = np.ascontiguousarray(counts, dtype=np.int64)
+        hist = np.zeros(max(iterations, 1), np.int64)
+        bb = None if custom_bb is None else (C.c_float * 4)(*[float(x) for x in
+                                                              np.asarray(custom_bb).reshape(-1)])
+        r = GvdResult()
+        self._check(lib().pgvd_gvd_refine(self.ctx, dverts.ptr,
+                                          cnt.ctypes.data_as(C.POINTER(C.c_int64)), cnt.size,
+                                          bits, bb, iterations, min_d,
+                                          hist.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(r),
+                                          None))
+        return r, hist[: r.iterations].tolist()
+
+    def gvd_refine(self, verts: np.ndarray, counts, bits: int, iterations: int,
+                   custom_bb=None, min_d: float = 1.0) -> dict:
+        """Octree2::build(const Polylines&) with FindMultiCells on the device;
+        the last build's points, octree, CellIntersections and extra points
+        as host arrays."""
+        v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 2)
+        dv = self.upload(v)
+        try:
+            r, hist = self.gvd_refine_device(dv, counts, bits, iterations, custom_bb, min_d)
+            O = r.octree_size
+            out = {"iterations": r.iterations, "extra_per_iteration": hist,
+                   "qpoints": self.download(r.qpoints, np.int32, 2 * r.n_qpoints).reshape(-1, 2),
+                   "n_segments": r.n_segments,
+                   "extra": self.download(r.extra, np.int32, 2 * r.n_extra).reshape(-1, 2)
+                   if r.n_extra else np.zeros((0, 2), np.int32),
+                   "samples": self.download(r.samples, np.float32, 2 * r.n_extra).reshape(-1, 2)
+                   if r.n_extra else np.zeros((0, 2), np.float32),
+                   "origins": self.download(r.origins, np.float32, 2 * r.n_boxes).reshape(-1, 2)
+                   if r.n_boxes else np.zeros((0, 2), np.float32),
+                   "lengths": self.download(r.lengths, np.float32, r.n_boxes)
+                   if r.n_boxes else np.zeros(0, np.float32),
+                   "octree_size": O}
+            if O:
+                octv = self.download(r.octree, np.int32, 5 * O).reshape(O, 5)
+                raw = self.download(r.ci, np.int32, 40 * O).reshape(O, 40)
+                out.update(children=octv[:, :4].copy(), leaf=octv[:, 4].copy(),
+                           ci_labels=raw[:, :8].reshape(O, 4, 2).copy(),
+                           ci_segs=raw[:, 8:].copy().view(np.float32).reshape(O, 4, 2, 4))
+            return out
+        finally:
+            dv.free()
 
     # ---- the octree's text and binary forms (C/OctNode.h:58-98) ----
     def format_octree_text(self, octree, O: int, dim: int) -> bytes:

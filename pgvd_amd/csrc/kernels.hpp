@@ -206,6 +206,37 @@ void launch_cell_walk_count(const float *segs, int64_t n, int bits, const int32_
 void launch_cell_walk_write(const float *segs, int64_t n, int bits, const int32_t *octree,
                             int64_t size, const int64_t *offsets, int32_t *cells,
                             const int32_t *scratch, uint32_t *counts, int log2k, hipStream_t s);
+// offsets[0] = 0, offsets[i + 1] = counts[0] + ... + counts[i] (n + 1
+// entries); ws holds walk_scan_ws_words(n) words.
+void launch_offsets_scan(const uint32_t *counts, int64_t n, int64_t *offsets, uint64_t *ws,
+                         hipStream_t s);
+
+// ---- gvd.hip ----
+// obj2Oct of every vertex and the segments of every polygon (vertices
+// [voff[j], voff[j+1]), segments from soff[j]); bad: 1 if a coordinate fell
+// below zero.
+void launch_poly_segments(const float *verts, int64_t nv, const int64_t *voff, const int64_t *soff,
+                          int64_t P, const float bbmin[2], float bbw, int bits, float *segs,
+                          int32_t *labels, uint32_t *bad, hipStream_t s);
+// MCCallback over a finished walk (offsets, cells; `total` cells) into the
+// CellIntersections records ci (40 words per node).  parts/vlabel: total
+// entries; first/second: 4*O; best: 8*O.
+void launch_multi_cells(const float *segs, const int32_t *labels, int64_t n,
+                        const int64_t *offsets, const int32_t *cells, int64_t total, int64_t O,
+                        float4 *parts, int32_t *vlabel, uint32_t *first, uint32_t *second,
+                        unsigned long long *best, int32_t *ci, hipStream_t s);
+// FitBoxes over items: pairs (8 floats each) or CellIntersections slots
+// (from_ci, items = 4*O).  Count pass: ns/nb per item, status (nullable),
+// *err = min(item << 8 | code) over failing items (~0 if none).
+void launch_fit_count(const float *src, bool from_ci, int64_t items, float min_d, uint32_t *ns,
+                      uint32_t *nb, int32_t *status, unsigned long long *err, hipStream_t s);
+void launch_fit_write(const float *src, bool from_ci, int64_t items, float min_d,
+                      const int64_t *soff, const int64_t *boff, const uint32_t *ns, float *samples,
+                      float *origins, float *lengths, int32_t *extra, hipStream_t s);
+// out[i] = (int32_t)in[i]
+void launch_narrow_i64(const int64_t *in, int64_t n, int32_t *out, hipStream_t s);
+// *flag = 1 if some v[i] != 0, else 0
+void launch_any_nonzero(const int32_t *v, int64_t n, uint32_t *flag, hipStream_t s);
 
 // ---- textio.hip ----
 // The octree's text form (operator<< / operator>>, C/OctNode.h:58-98).  ws

@@ -529,7 +529,10 @@ std::vector<OctCellT<DIM>> find_leaves(const std::vector<P> &points,
     if (h[4 * q + i] != 0) throw std::logic_error("FindLeaf: no leaf reached");
     const int32_t *pc = reinterpret_cast<const int32_t *>(&points[i]);
     const int32_t w = (int32_t)(1u << (resln.bits - h[3 * q + i]));
-    for (int k = 0; k < DIM; ++k) cells[i].origin[k] = pc[k] & ~(w - 1);
+    // the octant offsets along the descent (OctreeUtils.cpp:41-47): the
+    // coordinate bits the key holds, above the cell's width
+    const int32_t gmask = (int32_t)((1u << resln.bits) - 1u);
+    for (int k = 0; k < DIM; ++k) cells[i].origin[k] = (pc[k] & gmask) & ~(w - 1);
     cells[i].width = w;
     cells[i].parent_idx = h[i];
     cells[i].octant = h[q + i];
