@@ -6,6 +6,7 @@
     python bench.py --workload findleaf                     (point location)
     python bench.py --workload octree_text                  (text form of the octree)
     python bench.py --workload cellwalk                     (batched cell walks)
+    python bench.py --workload gvd                          (GVD refinement, 2D polylines)
 
 One step = the whole path (bounding box, FP32 quantize, 63-bit Morton keys,
 radix sort, unique, Karras BRT with local splits, octree fill) over 2^24
@@ -551,6 +552,109 @@ def bench_cellwalk(args) -> int:
     return 0 if out.get("cpu_baseline", {}).get("equal_to_gpu", True) else 1
 
 
+def bench_gvd(args) -> int:
+    """The GVD refinement (Octree2::build(const Polylines&) with its
+    FindMultiCells, viewer/Octree2.cpp:127-190, 430-506): a scene of 10^4
+    closed quadrilateral polylines (seed 11, 2 segments a side, neighbours
+    about 5e-4 apart; 90,000 vertices), 24 bits/axis, up to 8 builds.  One
+    step = one pgvd_gvd_refine call with the vertices in HBM: quantize, then
+    per build the octree of the points, every segment's cell walk with the
+    multi-cell visitor, and FitBoxes on the cells two polylines share, whose
+    samples join the next build; the host reads one count per stage.
+    value = polyline segments walked per second (segments x builds)."""
+    import pgvd_amd as P
+    from pgvd_amd import workloads
+
+    loops = args.n or 10000
+    bits, iters = 24, 8
+    v, c = workloads.polyline_scene(loops, 2, 11, 0.0005)
+    dev = P.Device(0)
+    dv = dev.upload(v)
+    for _ in range(max(args.warmup, 1)):
+        r, hist = dev.gvd_refine_device(dv, c, bits, iters)
+    dev.timing(True)
+    dev.timing_reset()
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r, hist = dev.gvd_refine_device(dv, c, bits, iters)
+    dev.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    stages = {}
+    for name in ("gvd_walk", "gvd_walk_write", "gvd_multi_cells", "gvd_fit_count", "gvd_fit_write",
+                 "bbox", "encode", "msd", "sort_pass", "unique", "brt", "splits_scan", "malformed",
+                 "octree"):
+        tot, cnt = dev.timing_read(name)
+        if cnt:
+            stages[name] = {"ms_per_step": tot / args.steps, "launches_per_step": cnt / args.steps,
+                            "avg_ms": tot / cnt}
+    dev.timing(False)
+    nseg, builds = int(r.n_segments), int(r.iterations)
+    got = None
+    if not args.no_cpu:
+        got = dev.gvd_refine(v, c, bits, iters)
+    dev.close()
+    walk = stages.get("gvd_walk", {"avg_ms": float("nan"), "launches_per_step": 0})
+    # the walk that keeps each segment's first 32 cells: 16 B in, 8 B of
+    # offset and (up to 32) 20 B cells out per segment; the visited cells are
+    # not counted here (they are known only per build)
+    wbytes = nseg * 24
+    gbps = wbytes / (walk["avg_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": "GVD refinement (Octree2::build with FindMultiCells): polyline segments walked/sec",
+        "value": nseg * builds / (ms * 1e-3),
+        "unit": "segments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64/i32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{loops} closed quadrilateral polylines (seed 11, 2 segments a side, gaps "
+                        f"5e-4) = {v.shape[0]} vertices, {nseg} segments, {bits} bits/axis, up to "
+                        f"{iters} builds",
+            "segments": nseg, "builds": builds, "extra_points_per_build": hist,
+            "final_points": int(r.n_qpoints), "final_octree_nodes": int(r.octree_size),
+            "stages": stages,
+        },
+        "roofline": {
+            "kernel": "cell_walk_kernel of the multi-cell walks (one lane per segment)",
+            "bound": "hbm",
+            "achieved": gbps,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": gbps / HBM_PEAK_GBPS,
+            "traffic": None,
+            "avg_launch_ms": walk["avg_ms"],
+            "launches": walk["launches_per_step"] * args.steps,
+            "algorithmic_bytes": wbytes,
+        },
+    }
+    if not args.no_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        want = oracle.gvd_refine(v, c, bits, iters)
+        dt = time.perf_counter() - t0
+        same = (want["extra_per_iteration"] == got["extra_per_iteration"] and
+                np.array_equal(want["qpoints"].astype(np.int32), got["qpoints"]) and
+                np.array_equal(want["multi"]["extra"], got["extra"]) and
+                np.array_equal(want["tree"]["children"], got["children"]) and
+                np.array_equal(want["ci"]["labels"], got["ci_labels"]))
+        out["cpu_baseline"] = {
+            "value": nseg * want["iterations"] / dt, "unit": "segments/s", "cores": 1,
+            "kind": "port",
+            "sample": f"the whole refinement ({want['iterations']} builds) by the serial "
+                      f"restatement on 1 core in {dt:.2f} s",
+            "equal_to_gpu": bool(same),
+        }
+    print(json.dumps(out), flush=True)
+    return 0 if out.get("cpu_baseline", {}).get("equal_to_gpu", True) else 1
+
+
 def bench_text(args) -> int:
     """The octree's text form (operator<< / operator>>, C/OctNode.h:58-98) on
     the device: the octree of the 2^24-point uniform 3D build (21 bits/axis)
@@ -675,7 +779,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="uniform",
                     choices=["uniform", "gauss", "conflict", "batch", "serve", "findleaf",
-                             "octree_text", "cellwalk"])
+                             "octree_text", "cellwalk", "gvd"])
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -706,6 +810,8 @@ def main() -> int:
         return bench_text(args)
     if args.workload == "cellwalk":
         return bench_cellwalk(args)
+    if args.workload == "gvd":
+        return bench_gvd(args)
 
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20

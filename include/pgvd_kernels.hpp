@@ -235,6 +235,72 @@ std::vector<std::vector<OctCell2>> CellWalks(const std::vector<float2> &a,
 
 }  // namespace OctreeUtils
 
+// ---- GVD refinement (2D) -----------------------------------------------------
+
+// FloatSegment (opencl/FloatSegment.h): two float points.
+struct FloatSegment {
+  float2 a, b;
+};
+
+// CellIntersections (CellIntersections.h:9-55): per octant the first two
+// labels whose segments reached the cell and the longest part of each;
+// the record pgvd_cell_intersections lays out (160 bytes).
+struct CellIntersections {
+  static const int NUM_LABELS = 2;
+  static const int NUM_OCTANTS = 4;
+  int32_t l[NUM_LABELS * NUM_OCTANTS];
+  FloatSegment segs[NUM_LABELS * NUM_OCTANTS];
+  bool is_multi(int octant) const { return l[octant * NUM_LABELS + 1] > -1; }
+  int num_labels(int octant) const {
+    for (int i = 0; i < NUM_LABELS; ++i)
+      if (l[octant * NUM_LABELS + i] < 0) return i;
+    return NUM_LABELS;
+  }
+  FloatSegment seg(int i, int octant) const { return segs[octant * NUM_LABELS + i]; }
+  int label(int i, int octant) const { return l[octant * NUM_LABELS + i]; }
+};
+
+namespace Geom {
+
+// Geom::FitBoxes (opencl/Geom.cpp:667-716) on the device: appends the
+// samples, box origins and widths.  Throws std::logic_error where the
+// reference throws (and when its loop passes 4096 rounds).
+void FitBoxes(const FloatSegment &A, const FloatSegment &B, float min_d,
+              std::vector<float2> *samples, std::vector<float2> *origins,
+              std::vector<float> *lengths);
+
+}  // namespace Geom
+
+// Octree2's polyline build (viewer/Octree2.cpp:127-190) with its
+// FindMultiCells (:430-506) in place: the octree of the polylines'
+// vertices, refined by the FitBoxes samples of the cells two polylines
+// share, for up to karras_iterations builds; all on the device.
+class Octree2 {
+ public:
+  explicit Octree2(const Resln &resln, int karras_iterations = 1, float min_d = 1.0f);
+  // Octree2::build(const Polylines&, const BoundingBox<float2>*): each
+  // polygon is a polyline (closed loops repeat their first vertex).
+  void build(const std::vector<std::vector<float2>> &polygons, const float2 *bbMin = nullptr,
+             const float2 *bbMax = nullptr);
+  const std::vector<OctNode2> &octree() const { return octree_; }
+  // cell_intersections of the last FindMultiCells, one per octree node
+  const std::vector<CellIntersections> &cell_intersections() const { return ci_; }
+  // the points of the last build (the quantized vertices, then extra points)
+  const std::vector<int2> &qpoints() const { return qpoints_; }
+  // extra_qpoints of the last FindMultiCells (not added to a build)
+  const std::vector<int2> &extra_qpoints() const { return extra_; }
+  int iterations() const { return iterations_; }
+
+ private:
+  Resln resln_;
+  int karras_iterations_;
+  float min_d_;
+  int iterations_ = 0;
+  std::vector<OctNode2> octree_;
+  std::vector<CellIntersections> ci_;
+  std::vector<int2> qpoints_, extra_;
+};
+
 // operator<< / operator>> for the octree (C/OctNode.h:58-98, commented out
 // in the reference), formatted and parsed on the device.  >> reads the rest
 // of the stream and sets failbit on malformed text.

@@ -30,29 +30,6 @@ using namespace pgvd_capi;
 
 namespace {
 
-// Records HIP events around a stage's launches when timing is on.
-struct Stage {
-  pgvd_ctx *ctx;
-  hipStream_t s;
-  hipEvent_t a = nullptr;
-  const char *name;
-  Stage(pgvd_ctx *c, const char *nm, hipStream_t st, int mode = 1) : ctx(c), s(st), name(nm) {
-    if (ctx && ctx->timing == mode) {
-      a = take_event(ctx);
-      if (a) (void)hipEventRecord(a, s);
-    }
-  }
-  ~Stage() {
-    if (a) {
-      hipEvent_t b = take_event(ctx);
-      if (b) {
-        (void)hipEventRecord(b, s);
-        ctx->recs.push_back({name, a, b});
-      }
-    }
-  }
-};
-
 // Events around each onesweep pass, recorded as separate "sort_pass" stage
 // entries so that a per-launch average can be formed.
 struct PassTimer {

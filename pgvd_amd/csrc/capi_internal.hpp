@@ -149,6 +149,29 @@ inline hipEvent_t take_event(pgvd_ctx *ctx) {
   return e;
 }
 
+// Records HIP events around a stage's launches when timing is on.
+struct Stage {
+  pgvd_ctx *ctx;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  const char *name;
+  Stage(pgvd_ctx *c, const char *nm, hipStream_t st, int mode = 1) : ctx(c), s(st), name(nm) {
+    if (ctx && ctx->timing == mode) {
+      a = take_event(ctx);
+      if (a) (void)hipEventRecord(a, s);
+    }
+  }
+  ~Stage() {
+    if (a) {
+      hipEvent_t b = take_event(ctx);
+      if (b) {
+        (void)hipEventRecord(b, s);
+        ctx->recs.push_back({name, a, b});
+      }
+    }
+  }
+};
+
 // Device bytes a whole-path build of n points takes in a context's pool
 // (every pooled buffer with its headroom, plus the input points).
 size_t plain_footprint_bytes(int64_t n, int dim, int bits, bool labels);

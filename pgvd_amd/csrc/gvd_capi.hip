@@ -42,18 +42,24 @@ int walk_pooled(pgvd_ctx *ctx, const float *segs, int64_t n, int bits, const int
   constexpr int kLog2K = 5;
   const size_t spec_bytes = (static_cast<size_t>(n) << kLog2K) * 20;
   if (spec_bytes <= (size_t(2) << 30)) CHK(pool_get(ctx, "walk_scratch", spec_bytes, &scratch));
-  launch_cell_walk_count(segs, n, bits, octree, O, static_cast<uint32_t *>(counts), status,
-                         static_cast<int64_t *>(offsets), static_cast<uint64_t *>(ws),
-                         static_cast<int32_t *>(scratch), kLog2K, st);
+  {
+    Stage t(ctx, "gvd_walk", st);
+    launch_cell_walk_count(segs, n, bits, octree, O, static_cast<uint32_t *>(counts), status,
+                           static_cast<int64_t *>(offsets), static_cast<uint64_t *>(ws),
+                           static_cast<int32_t *>(scratch), kLog2K, st);
+  }
   HIPCHK(hipGetLastError());
   int64_t total = 0;
   HIPCHK(hipMemcpyAsync(&total, static_cast<int64_t *>(offsets) + n, 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   if (total >= int64_t(0xFFFFFFFF)) return fail(PGVD_ERR_TOO_LARGE, "more than 2^32 - 2 visited cells");
   CHK(pool_get(ctx, "mc_cells", 20 * static_cast<size_t>(std::max<int64_t>(total, 1)), &cells));
-  launch_cell_walk_write(segs, n, bits, octree, O, static_cast<const int64_t *>(offsets),
-                         static_cast<int32_t *>(cells), static_cast<const int32_t *>(scratch),
-                         static_cast<uint32_t *>(counts), kLog2K, st);
+  {
+    Stage t(ctx, "gvd_walk_write", st);
+    launch_cell_walk_write(segs, n, bits, octree, O, static_cast<const int64_t *>(offsets),
+                           static_cast<int32_t *>(cells), static_cast<const int32_t *>(scratch),
+                           static_cast<uint32_t *>(counts), kLog2K, st);
+  }
   HIPCHK(hipGetLastError());
   *offsets_out = static_cast<int64_t *>(offsets);
   *cells_out = static_cast<int32_t *>(cells);
@@ -74,23 +80,32 @@ int multi_cells_impl(pgvd_ctx *ctx, const float *segs, const int32_t *labels, in
   CHK(pool_get(ctx, "mc_first", 16 * static_cast<size_t>(O), &first));
   CHK(pool_get(ctx, "mc_second", 16 * static_cast<size_t>(O), &second));
   CHK(pool_get(ctx, "mc_best", 64 * static_cast<size_t>(O), &best));
-  launch_multi_cells(segs, labels, n, offsets, cells, total, O, static_cast<float4 *>(parts),
-                     static_cast<int32_t *>(vlabel), static_cast<uint32_t *>(first),
-                     static_cast<uint32_t *>(second), static_cast<unsigned long long *>(best), ci,
-                     st);
+  {
+    Stage t(ctx, "gvd_multi_cells", st);
+    launch_multi_cells(segs, labels, n, offsets, cells, total, O, static_cast<float4 *>(parts),
+                       static_cast<int32_t *>(vlabel), static_cast<uint32_t *>(first),
+                       static_cast<uint32_t *>(second), static_cast<unsigned long long *>(best),
+                       ci, st);
+  }
   HIPCHK(hipGetLastError());
   return PGVD_OK;
 }
 
-// FitBoxes over items (pairs or CellIntersections slots): count, scan,
-// totals back to the host, and the write pass into the caller's buffers
-// when they hold the totals.  *first_err: ~0, or (item << 8 | code) of the
-// first failing item.
-int fit_impl(pgvd_ctx *ctx, const float *src, bool from_ci, int64_t items, float min_d,
-             int32_t *extra, float *samples, float *origins, float *lengths, int64_t capacity,
-             int64_t *sample_offsets, int64_t *box_offsets, int32_t *status,
-             int64_t *samples_out, int64_t *boxes_out, unsigned long long *first_err,
-             hipStream_t st) {
+// FitBoxes over items (pairs or CellIntersections slots) in three steps:
+// fit_count enqueues the counting pass and the scans of the counts;
+// fit_totals reads the totals, the first error (~0, or item << 8 | code)
+// and, optionally, one more device word back and waits; fit_write enqueues
+// the writing pass into buffers that hold the totals.
+struct FitWork {
+  uint32_t *ns = nullptr, *nb = nullptr;
+  int64_t *so = nullptr, *bo = nullptr;
+  unsigned long long *err = nullptr;
+  int64_t items = 0;
+};
+
+int fit_count(pgvd_ctx *ctx, const float *src, bool from_ci, int64_t items, float min_d,
+              int64_t *sample_offsets, int64_t *box_offsets, int32_t *status, FitWork *w,
+              hipStream_t st) {
   void *ns = nullptr, *nb = nullptr, *soff = nullptr, *boff = nullptr, *ws = nullptr, *err = nullptr;
   const size_t m = static_cast<size_t>(std::max<int64_t>(items, 1));
   CHK(pool_get(ctx, "fit_ns", 4 * m, &ns));
@@ -99,36 +114,71 @@ int fit_impl(pgvd_ctx *ctx, const float *src, bool from_ci, int64_t items, float
   if (!box_offsets) CHK(pool_get(ctx, "fit_boff", 8 * (m + 1), &boff));
   CHK(pool_get(ctx, "fit_ws", 8 * static_cast<size_t>(walk_scan_ws_words(items)), &ws));
   CHK(pool_get(ctx, "fit_err", 8, &err));
-  int64_t *so = sample_offsets ? sample_offsets : static_cast<int64_t *>(soff);
-  int64_t *bo = box_offsets ? box_offsets : static_cast<int64_t *>(boff);
-  launch_fit_count(src, from_ci, items, min_d, static_cast<uint32_t *>(ns),
-                   static_cast<uint32_t *>(nb), status, static_cast<unsigned long long *>(err), st);
+  w->ns = static_cast<uint32_t *>(ns);
+  w->nb = static_cast<uint32_t *>(nb);
+  w->so = sample_offsets ? sample_offsets : static_cast<int64_t *>(soff);
+  w->bo = box_offsets ? box_offsets : static_cast<int64_t *>(boff);
+  w->err = static_cast<unsigned long long *>(err);
+  w->items = items;
+  {
+    Stage t(ctx, "gvd_fit_count", st);
+    launch_fit_count(src, from_ci, items, min_d, w->ns, w->nb, status, w->err, st);
+  }
   HIPCHK(hipGetLastError());
-  int64_t totals[2] = {0, 0};
-  unsigned long long e = ~0ull;
   if (items > 0) {
-    launch_offsets_scan(static_cast<uint32_t *>(ns), items, so, static_cast<uint64_t *>(ws), st);
-    launch_offsets_scan(static_cast<uint32_t *>(nb), items, bo, static_cast<uint64_t *>(ws), st);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(&totals[0], so + items, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&totals[1], bo + items, 8, hipMemcpyDeviceToHost, st));
+    launch_offsets_scan(w->ns, items, w->so, static_cast<uint64_t *>(ws), st);
+    launch_offsets_scan(w->nb, items, w->bo, static_cast<uint64_t *>(ws), st);
   } else {
     const int64_t zero = 0;
-    HIPCHK(hipMemcpyAsync(so, &zero, 8, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(bo, &zero, 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(w->so, &zero, 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(w->bo, &zero, 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));      // `zero` is a stack value
   }
-  HIPCHK(hipMemcpyAsync(&e, err, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipGetLastError());
+  return PGVD_OK;
+}
+
+int fit_totals(const FitWork &w, int64_t *samples_out, int64_t *boxes_out,
+               unsigned long long *first_err, const uint32_t *extra_word, uint32_t *extra_out,
+               hipStream_t st) {
+  int64_t totals[2] = {0, 0};
+  unsigned long long e = ~0ull;
+  uint32_t x = 0;
+  HIPCHK(hipMemcpyAsync(&totals[0], w.so + w.items, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&totals[1], w.bo + w.items, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&e, w.err, 8, hipMemcpyDeviceToHost, st));
+  if (extra_word) HIPCHK(hipMemcpyAsync(&x, extra_word, 4, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   *samples_out = totals[0];
   *boxes_out = totals[1];
   *first_err = e;
-  if (totals[0] > capacity || totals[1] > capacity)
+  if (extra_out) *extra_out = x;
+  return PGVD_OK;
+}
+
+int fit_write(pgvd_ctx *ctx, const FitWork &w, const float *src, bool from_ci, float min_d,
+              int32_t *extra, float *samples, float *origins, float *lengths, hipStream_t st) {
+  if (w.items == 0) return PGVD_OK;
+  Stage t(ctx, "gvd_fit_write", st);
+  launch_fit_write(src, from_ci, w.items, min_d, w.so, w.bo, w.ns, samples, origins, lengths,
+                   extra, st);
+  HIPCHK(hipGetLastError());
+  return PGVD_OK;
+}
+
+// count, totals, capacity check, write
+int fit_impl(pgvd_ctx *ctx, const float *src, bool from_ci, int64_t items, float min_d,
+             int32_t *extra, float *samples, float *origins, float *lengths, int64_t capacity,
+             int64_t *sample_offsets, int64_t *box_offsets, int32_t *status,
+             int64_t *samples_out, int64_t *boxes_out, unsigned long long *first_err,
+             hipStream_t st) {
+  FitWork w;
+  CHK(fit_count(ctx, src, from_ci, items, min_d, sample_offsets, box_offsets, status, &w, st));
+  CHK(fit_totals(w, samples_out, boxes_out, first_err, nullptr, nullptr, st));
+  if (*samples_out > capacity || *boxes_out > capacity)
     return fail(PGVD_ERR_CAPACITY, "FitBoxes output exceeds the capacity (*samples_out, *boxes_out)");
-  if (items > 0 && (totals[0] > 0 || totals[1] > 0)) {
-    launch_fit_write(src, from_ci, items, min_d, so, bo, static_cast<uint32_t *>(ns), samples,
-                     origins, lengths, extra, st);
-    HIPCHK(hipGetLastError());
-  }
+  if (*samples_out > 0 || *boxes_out > 0)
+    CHK(fit_write(ctx, w, src, from_ci, min_d, extra, samples, origins, lengths, st));
   return PGVD_OK;
 }
 
@@ -344,28 +394,24 @@ int pgvd_gvd_refine(pgvd_ctx *ctx, const float *verts, const int64_t *counts, in
                            static_cast<int32_t *>(wstat), st));
       launch_any_nonzero(static_cast<int32_t *>(wstat), nseg, static_cast<uint32_t *>(flag), st);
       HIPCHK(hipGetLastError());
-      uint32_t walk_bad = 0;
-      HIPCHK(hipMemcpyAsync(&walk_bad, flag, 4, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      if (walk_bad) return fail(PGVD_ERR_GEOMETRY, "a cell walk passed 10000 cells (Infinite loop)");
-      // the sampling: sizes first, then the points into buffers that hold them
+      // the sampling: counts, then the points into buffers that hold them
+      FitWork w;
+      CHK(fit_count(ctx, static_cast<float *>(ci), true, 4 * O, min_d, nullptr, nullptr, nullptr,
+                    &w, st));
       unsigned long long e = ~0ull;
-      int rc = fit_impl(ctx, static_cast<float *>(ci), true, 4 * O, min_d, nullptr, nullptr,
-                        nullptr, nullptr, 0, nullptr, nullptr, nullptr, &n_extra, &n_boxes, &e, st);
-      if (rc != PGVD_OK && rc != PGVD_ERR_CAPACITY) return rc;
+      uint32_t walk_bad = 0;
+      CHK(fit_totals(w, &n_extra, &n_boxes, &e, static_cast<uint32_t *>(flag), &walk_bad, st));
+      if (walk_bad) return fail(PGVD_ERR_GEOMETRY, "a cell walk passed 10000 cells (Infinite loop)");
       CHK(fit_error(e, "node*4+octant"));
       ecap = std::max<int64_t>(std::max(n_extra, n_boxes), 1);
       CHK(pool_get(ctx, "gvd_extra", 8 * static_cast<size_t>(ecap), &extra));
       CHK(pool_get(ctx, "gvd_samples", 8 * static_cast<size_t>(ecap), &samples));
       CHK(pool_get(ctx, "gvd_origins", 8 * static_cast<size_t>(ecap), &origins));
       CHK(pool_get(ctx, "gvd_lengths", 4 * static_cast<size_t>(ecap), &lengths));
-      if (n_extra > 0 || n_boxes > 0) {
-        int64_t ns2 = 0, nb2 = 0;
-        CHK(fit_impl(ctx, static_cast<float *>(ci), true, 4 * O, min_d,
-                     static_cast<int32_t *>(extra), static_cast<float *>(samples),
-                     static_cast<float *>(origins), static_cast<float *>(lengths), ecap, nullptr,
-                     nullptr, nullptr, &ns2, &nb2, &e, st));
-      }
+      if (n_extra > 0 || n_boxes > 0)
+        CHK(fit_write(ctx, w, static_cast<float *>(ci), true, min_d, static_cast<int32_t *>(extra),
+                      static_cast<float *>(samples), static_cast<float *>(origins),
+                      static_cast<float *>(lengths), st));
     }
     if (extra_per_iteration) extra_per_iteration[it] = n_extra;
     ++it;

@@ -639,6 +639,109 @@ void CellWalk(const float2 &a, const float2 &b, const std::vector<OctNode2> &oct
 
 }  // namespace OctreeUtils
 
+// ------------------------------------------------------ GVD refinement ---
+static_assert(sizeof(CellIntersections) == sizeof(pgvd_cell_intersections),
+              "CellIntersections has the layout of pgvd_cell_intersections");
+
+namespace {
+
+void gvd_check(int rc, const char *what) {
+  if (rc == PGVD_ERR_GEOMETRY || rc == PGVD_ERR_BAD_COORD)
+    throw std::logic_error(std::string(what) + ": " + pgvd_last_error());
+  if (rc != PGVD_OK) throw std::runtime_error(std::string(what) + ": " + pgvd_last_error());
+}
+
+}  // namespace
+
+namespace Geom {
+
+void FitBoxes(const FloatSegment &A, const FloatSegment &B, float min_d,
+              std::vector<float2> *samples, std::vector<float2> *origins,
+              std::vector<float> *lengths) {
+  Context &c = Context::Default();
+  const float pair[8] = {A.a.x, A.a.y, A.b.x, A.b.y, B.a.x, B.a.y, B.b.x, B.b.y};
+  Buffer dp = c.alloc(sizeof pair);
+  c.upload(dp, pair, sizeof pair);
+  const auto *pp = static_cast<const float *>(dp.ptr());
+  int64_t ns = 0, nb = 0;
+  const int rc = pgvd_fit_boxes(c.raw(), pp, 1, min_d, nullptr, nullptr, nullptr, 0, nullptr,
+                                nullptr, &ns, &nb, nullptr, nullptr);
+  if (rc != PGVD_OK && rc != PGVD_ERR_CAPACITY && rc != PGVD_ERR_GEOMETRY)
+    gvd_check(rc, "Geom::FitBoxes");
+  const int64_t cap = std::max<int64_t>(std::max(ns, nb), 1);
+  Buffer ds = c.alloc(8 * (size_t)cap), dorg = c.alloc(8 * (size_t)cap), dl = c.alloc(4 * (size_t)cap);
+  const int rc2 = pgvd_fit_boxes(c.raw(), pp, 1, min_d, static_cast<float *>(ds.ptr()),
+                                 static_cast<float *>(dorg.ptr()), static_cast<float *>(dl.ptr()),
+                                 cap, nullptr, nullptr, &ns, &nb, nullptr, nullptr);
+  // the reference pushes what it found before it throws
+  std::vector<float2> s((size_t)ns), o((size_t)nb);
+  std::vector<float> l((size_t)nb);
+  if (ns) c.download(s.data(), ds, 8 * (size_t)ns);
+  if (nb) {
+    c.download(o.data(), dorg, 8 * (size_t)nb);
+    c.download(l.data(), dl, 4 * (size_t)nb);
+  }
+  if (samples) samples->insert(samples->end(), s.begin(), s.end());
+  if (origins) origins->insert(origins->end(), o.begin(), o.end());
+  if (lengths) lengths->insert(lengths->end(), l.begin(), l.end());
+  gvd_check(rc2, "Geom::FitBoxes");
+}
+
+}  // namespace Geom
+
+Octree2::Octree2(const Resln &resln, int karras_iterations, float min_d)
+    : resln_(resln), karras_iterations_(karras_iterations), min_d_(min_d) {}
+
+void Octree2::build(const std::vector<std::vector<float2>> &polygons, const float2 *bbMin,
+                    const float2 *bbMax) {
+  octree_.clear();
+  ci_.clear();
+  qpoints_.clear();
+  extra_.clear();
+  iterations_ = 0;
+  std::vector<float2> verts;
+  std::vector<int64_t> counts;
+  for (const auto &p : polygons) {
+    verts.insert(verts.end(), p.begin(), p.end());
+    counts.push_back((int64_t)p.size());
+  }
+  if (verts.empty()) return;               // Octree2.cpp:152-155
+  Context &c = Context::Default();
+  Buffer dv = c.alloc(sizeof(float2) * verts.size());
+  c.upload(dv, verts.data(), sizeof(float2) * verts.size());
+  float bb[4];
+  const float *bbp = nullptr;
+  if (bbMin && bbMax) {
+    bb[0] = bbMin->x;
+    bb[1] = bbMin->y;
+    bb[2] = bbMax->x;
+    bb[3] = bbMax->y;
+    bbp = bb;
+  }
+  pgvd_gvd_result r;
+  gvd_check(pgvd_gvd_refine(c.raw(), static_cast<const float *>(dv.ptr()), counts.data(),
+                            (int64_t)counts.size(), resln_.bits, bbp, karras_iterations_, min_d_,
+                            nullptr, &r, nullptr),
+            "Octree2::build");
+  iterations_ = r.iterations;
+  qpoints_.resize((size_t)r.n_qpoints);
+  gvd_check(pgvd_memcpy_dtoh(c.raw(), qpoints_.data(), r.qpoints, 8 * qpoints_.size(), nullptr),
+        "download");
+  extra_.resize((size_t)r.n_extra);
+  if (r.n_extra)
+    gvd_check(pgvd_memcpy_dtoh(c.raw(), extra_.data(), r.extra, 8 * extra_.size(), nullptr), "download");
+  octree_.resize((size_t)r.octree_size);
+  ci_.resize((size_t)r.octree_size);
+  if (r.octree_size) {
+    gvd_check(pgvd_memcpy_dtoh(c.raw(), octree_.data(), r.octree, sizeof(OctNode2) * octree_.size(),
+                           nullptr),
+          "download");
+    gvd_check(pgvd_memcpy_dtoh(c.raw(), ci_.data(), r.ci, sizeof(CellIntersections) * ci_.size(),
+                           nullptr),
+          "download");
+  }
+}
+
 // ------------------------------------------------- octree stream operators ---
 namespace {
 

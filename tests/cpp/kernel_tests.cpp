@@ -448,6 +448,116 @@ static void query_scenarios() {
   std::printf("PASS octree queries (FindLeaf, CellWalk) and stream operators\n");
 }
 
+// The GVD refinement: Geom::FitBoxes against the oracle's, its logic_error
+// on meeting segments that the reference rejects, and Octree2's polyline
+// build with one FindMultiCells (viewer/Octree2.cpp:127-190) against the
+// oracle's octree, CellIntersections and extra points.
+static void gvd_scenarios() {
+  std::mt19937 rng(8);
+  std::uniform_real_distribution<float> u(0.f, 1000.f);
+  for (int k = 0; k < 200; ++k) {
+    const FloatSegment A{{u(rng), u(rng)}, {u(rng), u(rng)}};
+    FloatSegment B{{A.a.x + 3.f, A.a.y + 7.f}, {A.b.x + 5.f, A.b.y + 2.f}};
+    if (k % 2) B = FloatSegment{{u(rng), u(rng)}, {u(rng), u(rng)}};
+    const float a[4] = {A.a.x, A.a.y, A.b.x, A.b.y}, b[4] = {B.a.x, B.a.y, B.b.x, B.b.y};
+    int64_t nb = 0;
+    int st = 0;
+    const int64_t ns = oracle_fit_boxes(a, b, 1.f, nullptr, nullptr, nullptr, 0, &nb, &st);
+    std::vector<float> ws(2 * (size_t)std::max<int64_t>(ns, 1)), wo(2 * (size_t)std::max<int64_t>(nb, 1)),
+        wl((size_t)std::max<int64_t>(nb, 1));
+    oracle_fit_boxes(a, b, 1.f, ws.data(), wo.data(), wl.data(), std::max(ns, nb), &nb, &st);
+    std::vector<float2> smp, org;
+    std::vector<float> len;
+    bool threw = false;
+    try {
+      Geom::FitBoxes(A, B, 1.f, &smp, &org, &len);
+    } catch (const std::logic_error &) {
+      threw = true;
+    }
+    CHECK(threw == (st != 0), "FitBoxes throws where the oracle reports an error");
+    bool ok = (int64_t)smp.size() == ns && (int64_t)org.size() == nb;
+    for (int64_t i = 0; i < ns && ok; ++i) ok = smp[i].x == ws[2 * i] && smp[i].y == ws[2 * i + 1];
+    for (int64_t i = 0; i < nb && ok; ++i)
+      ok = org[i].x == wo[2 * i] && org[i].y == wo[2 * i + 1] && len[i] == wl[i];
+    CHECK(ok, "FitBoxes equals the oracle's");
+  }
+  // two polylines facing each other across a narrow gap
+  std::vector<std::vector<float2>> polys = {
+      {{0.f, 0.f}, {0.595f, 0.01f}, {0.575f, 1.f}, {0.f, 1.f}, {0.f, 0.f}},
+      {{0.605f, 0.f}, {1.f, 0.f}, {1.f, 1.f}, {0.615f, 0.99f}, {0.605f, 0.f}}};
+  const int bits = 12;
+  const Resln r = make_resln(1 << bits, 2);
+  Octree2 o2(r, 1);
+  o2.build(polys);
+  std::vector<float> flat;
+  for (const auto &p : polys)
+    for (const auto &v : p) { flat.push_back(v.x); flat.push_back(v.y); }
+  const int64_t nv = (int64_t)flat.size() / 2;
+  std::vector<int64_t> q(2 * (size_t)nv);
+  CHECK(oracle_quantize(flat.data(), nv, 2, r.width, 0, nullptr, nullptr, 0, q.data()) == nv, "quantize");
+  oracle_tree *t = oracle_build(q.data(), nv, 2, bits, nullptr);
+  const bool same = same_octree(o2.octree(), t);
+  const int64_t O = oracle_tree_octree_size(t);
+  std::vector<int32_t> ch(O * 4), lf(O), oct(O * 5);
+  oracle_tree_octree(t, ch.data(), lf.data());
+  oracle_tree_free(t);
+  CHECK(same, "Octree2 octree equals the serial octree");
+  for (int64_t i = 0; i < O; ++i) {
+    for (int c = 0; c < 4; ++c) oct[5 * i + c] = ch[4 * i + c];
+    oct[5 * i + 4] = lf[i];
+  }
+  float bbmin[2] = {flat[0], flat[1]}, bbmax[2] = {flat[0], flat[1]};
+  for (int64_t i = 0; i < nv; ++i)
+    for (int k = 0; k < 2; ++k) {
+      bbmin[k] = std::min(bbmin[k], flat[2 * i + k]);
+      bbmax[k] = std::max(bbmax[k], flat[2 * i + k]);
+    }
+  const float bbw = std::max(bbmax[0] - bbmin[0], bbmax[1] - bbmin[1]);
+  std::vector<float> octv(flat.size());
+  CHECK(oracle_obj2oct(flat.data(), nv, bbmin, bbw, r.width, octv.data()) == 0, "obj2oct");
+  std::vector<float> segs;
+  std::vector<int32_t> labels;
+  int64_t at = 0;
+  for (size_t j = 0; j < polys.size(); ++j) {
+    for (size_t i = 0; i + 1 < polys[j].size(); ++i) {
+      for (int k = 0; k < 2; ++k) segs.push_back(octv[2 * (at + i) + k]);
+      for (int k = 0; k < 2; ++k) segs.push_back(octv[2 * (at + i + 1) + k]);
+      labels.push_back((int32_t)j);
+    }
+    at += (int64_t)polys[j].size();
+  }
+  std::vector<int32_t> cil(O * 8), wst(labels.size());
+  std::vector<float> cis(O * 32);
+  oracle_multi_cells(segs.data(), labels.data(), (int64_t)labels.size(), bits, oct.data(), O,
+                     cil.data(), cis.data(), wst.data());
+  bool ok = (int64_t)o2.cell_intersections().size() == O;
+  int multi = 0;
+  for (int64_t i = 0; i < O && ok; ++i) {
+    const CellIntersections &c = o2.cell_intersections()[i];
+    for (int j = 0; j < 8; ++j) {
+      ok &= c.l[j] == cil[8 * i + j];
+      ok &= c.segs[j].a.x == cis[32 * i + 4 * j] && c.segs[j].a.y == cis[32 * i + 4 * j + 1] &&
+            c.segs[j].b.x == cis[32 * i + 4 * j + 2] && c.segs[j].b.y == cis[32 * i + 4 * j + 3];
+    }
+    for (int oc = 0; oc < 4; ++oc) multi += c.is_multi(oc);
+  }
+  CHECK(ok, "Octree2 cell_intersections equal the oracle's");
+  CHECK(multi > 0, "the scene has multi-cells");
+  int64_t nb = 0;
+  int st = 0;
+  const int64_t ne = oracle_find_multi_cells(cil.data(), cis.data(), O, 1.f, nullptr, nullptr,
+                                             nullptr, nullptr, 0, &nb, &st);
+  std::vector<int32_t> ex(2 * (size_t)std::max<int64_t>(ne, 1));
+  std::vector<float> smp(2 * (size_t)std::max<int64_t>(ne, 1));
+  oracle_find_multi_cells(cil.data(), cis.data(), O, 1.f, ex.data(), smp.data(), nullptr, nullptr,
+                          ne, &nb, &st);
+  ok = st == 0 && (int64_t)o2.extra_qpoints().size() == ne && ne > 0;
+  for (int64_t i = 0; i < ne && ok; ++i)
+    ok = o2.extra_qpoints()[i].x == ex[2 * i] && o2.extra_qpoints()[i].y == ex[2 * i + 1];
+  CHECK(ok, "Octree2 extra_qpoints equal the oracle's");
+  std::printf("PASS GVD refinement (FitBoxes, Octree2 polyline build with FindMultiCells)\n");
+}
+
 int main() {
   morton_scenario();
   sort_scenario();
@@ -458,6 +568,7 @@ int main() {
   build_scenarios();
   karras_scenario();
   query_scenarios();
+  gvd_scenarios();
   std::printf(failures ? "FAILED %d\n" : "ALL PASSED\n", failures);
   return failures ? 1 : 0;
 }

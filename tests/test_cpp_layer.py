@@ -28,7 +28,8 @@ def test_cpp_layer_symbols_exported():
                  "pgvd::Kernels::BitPredicate", "pgvd::Kernels::DoubleCompact",
                  "pgvd::Karras::Quantize", "pgvd::Karras::BuildOctreeInParallel",
                  "pgvd::OctreeUtils::FindLeaf", "pgvd::OctreeUtils::FindLeaves",
-                 "pgvd::OctreeUtils::CellWalk", "pgvd::OctreeUtils::CellWalks"]:
+                 "pgvd::OctreeUtils::CellWalk", "pgvd::OctreeUtils::CellWalks",
+                 "pgvd::Geom::FitBoxes", "pgvd::Octree2::build"]:
         assert name in out, name
 
 
