@@ -84,6 +84,8 @@ def lib():
         L.oracle_fit_boxes.argtypes = [f32p, f32p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
         L.oracle_fit_boxes.restype = C.c_int64
+        L.oracle_fit_boxes_undefined.argtypes = [f32p, f32p, C.c_float]
+        L.oracle_fit_boxes_undefined.restype = C.c_int
         L.oracle_multi_cells.argtypes = [f32p, i32p, C.c_int64, C.c_int, i32p, C.c_int64, i32p,
                                          f32p, i32p]
         L.oracle_find_multi_cells.argtypes = [i32p, f32p, C.c_int64, C.c_float, C.c_void_p,
@@ -433,6 +435,12 @@ def fit_boxes(s0, s1, min_d: float = 1.0) -> dict:
                        C.byref(nb), C.byref(st))
     return {"samples": smp[: 2 * k].reshape(k, 2), "origins": org[: 2 * nb.value].reshape(-1, 2),
             "lengths": ln[: nb.value], "status": st.value}
+
+
+def fit_boxes_undefined(s0, s1, min_d: float = 1.0) -> bool:
+    """True where the reference's FitBoxes reads a segment it never set
+    (opencl/Geom.cpp:684-697), so its result depends on stack contents."""
+    return bool(lib().oracle_fit_boxes_undefined(_f2(s0), _f2(s1), min_d))
 
 
 def obj2oct(verts: np.ndarray, bbmin, bbw: float, bits: int) -> np.ndarray:

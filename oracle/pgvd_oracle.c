@@ -1083,10 +1083,11 @@ static int g_fit_no_isect(gseg A, gseg B, float min_d, g_out *o, g_boxes *bx) {
  * failed (default-constructed, uninitialised segments); here a failed clip
  * leaves a zero segment, whose unit vector is NaN, so the pairs are not
  * swapped. */
-static int g_fit_boxes(gseg A, gseg B, float min_d, g_out *o, g_boxes *bx) {
+static int g_fit_boxes_u(gseg A, gseg B, float min_d, g_out *o, g_boxes *bx, int *undef) {
   gv2 ca, cb;
   int ca_end, cb_end;
   g_closest_segs(A, B, &ca, &cb, &ca_end, &cb_end);
+  if (undef) *undef = 0;
   if (g_eq(ca, cb)) {
     g_emit(o, ca, 0, ca);
     g_box(bx, g_sub(ca, g_mk(min_d / 2, min_d / 2)), min_d);
@@ -1101,6 +1102,7 @@ static int g_fit_boxes(gseg A, gseg B, float min_d, g_out *o, g_boxes *bx) {
     int A1v = g_clip_to_box(A1o, min_d, &A1);
     const int B0v = g_clip_to_box(B0o, min_d, &B0);
     const int B1v = g_clip_to_box(B1o, min_d, &B1);
+    if (undef && (!A0v || !B0v)) *undef = 1;   /* the reference reads an unset segment */
     const float dotp = g_dot(g_unit(g_vec(A0)), g_unit(g_vec(B0)));
     if (dotp < 0) {
       gseg t = A0; A0 = A1; A1 = t;
@@ -1112,6 +1114,23 @@ static int g_fit_boxes(gseg A, gseg B, float min_d, g_out *o, g_boxes *bx) {
     return rc;
   }
   return g_fit_no_isect(A, B, min_d, o, bx);
+}
+
+static int g_fit_boxes(gseg A, gseg B, float min_d, g_out *o, g_boxes *bx) {
+  return g_fit_boxes_u(A, B, min_d, o, bx, NULL);
+}
+
+/* 1 when the reference's FitBoxes on these segments reads a segment it
+ * never set (the meeting-point branch with a failed clip of A0 or B0,
+ * opencl/Geom.cpp:684-697): its result then depends on stack contents. */
+int oracle_fit_boxes_undefined(const float *a, const float *b, float min_d) {
+  g_out o = {NULL, 0, 0};
+  g_boxes bx = {NULL, NULL, 0, 0};
+  int undef = 0;
+  const gseg A = g_seg(g_mk(a[0], a[1]), g_mk(a[2], a[3]));
+  const gseg B = g_seg(g_mk(b[0], b[1]), g_mk(b[2], b[3]));
+  (void)g_fit_boxes_u(A, B, min_d, &o, &bx, &undef);
+  return undef;
 }
 
 int64_t oracle_fit_boxes(const float *a, const float *b, float min_d, float *samples,

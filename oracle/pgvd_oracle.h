@@ -146,6 +146,8 @@ int oracle_multi_intersection(const float *p1, const float *p2, const float *p3,
 int64_t oracle_fit_boxes(const float *a, const float *b, float min_d, float *samples,
                          float *origins, float *lengths, int64_t cap, int64_t *boxes_out,
                          int *status);
+/* 1 when the reference's FitBoxes reads an unset segment on these inputs. */
+int oracle_fit_boxes_undefined(const float *a, const float *b, float min_d);
 /* FindMultiCells' walks with MCCallback (viewer/Octree2.cpp:366-455):
  * CellIntersections of every node, ci_labels O*8 (-1 empty), ci_segs O*32. */
 void oracle_multi_cells(const float *segs, const int32_t *labels, int64_t n, int bits,

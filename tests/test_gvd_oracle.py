@@ -82,13 +82,18 @@ def test_fit_boxes_matches_reference(kind):
     """Samples, box origins and widths and the error status of FitBoxes
     (min_d = 1, as FindMultiCells calls it) equal the reference's bit for
     bit on 1500 pairs of each shape.  Pairs where the oracle's loop would
-    pass its 4096-round bound (the reference's has none) are not run on
-    the reference."""
+    pass its 4096-round bound (the reference's has none), and pairs on which
+    the reference reads a half-segment it never set (where the segments
+    meet and a clip fails; there its result depends on stack contents), are
+    not compared."""
     pairs = _pairs(kind, 1500, seed=KINDS.index(kind) + 11)
-    compared = errors = 0
+    compared = errors = undefined = 0
     for s0, s1 in pairs:
         o = oracle.fit_boxes(s0, s1, 1.0)
         if o["status"] == 4:
+            continue
+        if oracle.fit_boxes_undefined(s0, s1):
+            undefined += 1
             continue
         r = oracle.ref_fit_boxes(s0, s1, 1.0)
         assert o["status"] == r["status"], (s0, s1)
@@ -97,7 +102,7 @@ def test_fit_boxes_matches_reference(kind):
         assert _bits_equal(o["lengths"], r["lengths"]), (s0, s1)
         compared += 1
         errors += o["status"] != 0
-    assert compared >= 1400
+    assert compared >= 1300 and undefined <= 200
     if kind == "near_parallel":
         # the loop runs many rounds on these
         assert max(len(oracle.fit_boxes(p[0], p[1])["lengths"]) for p in pairs[:200]) > 5
