@@ -253,7 +253,7 @@ template <typename K>
 int build_launch(pgvd_ctx *ctx, const BuildBufs<K> &b, RunState *st, int32_t *octree,
                  uint32_t *nconf, int64_t octree_cap, const void *pts, int64_t n, int dim,
                  int bits, const uint32_t *labels, const pgvd_build_opts *opts, hipStream_t s,
-                 BuildOut<K> &out) {
+                 BuildOut<K> &out, bool defer_exact) {
   const int mbits = dim * bits;
   const int passes = (mbits + 7) / 8;
   const int prepend = ref_next_pow2((int)n) != n ? 1 : 0;   // zero-key padding (Q1)
@@ -352,9 +352,10 @@ int build_launch(pgvd_ctx *ctx, const BuildBufs<K> &b, RunState *st, int32_t *oc
     launch_chunk_scan(b.chunks, nk, lb_b, st, s);
   }
   CHK(launch_status("splits_scan"));
-  {
+  if (!defer_exact) {
     // order-exact claims, local splits and scan; no-ops unless some BRT
-    // split took the FP32-quirk value
+    // split took the FP32-quirk value (with a host to wait for the build,
+    // they run at its finish, and only when a split did)
     Stage t(ctx, "malformed", s);
     launch_claims_if_malformed<K>(b.brt, b.parent, nk, st, s);
     launch_splits_scan<K>(b.brt, b.parent, nullptr, b.ps, nk, dim, 1, lb_s, st, s);
@@ -369,7 +370,8 @@ int build_launch(pgvd_ctx *ctx, const BuildBufs<K> &b, RunState *st, int32_t *oc
     }
     const bool timed = gev[0] && gev[1];
     launch_octree<K>(b.brt, b.parent, b.ps, lsx, b.chunks, uniq, cbits, nk, dim, mbits, octree,
-                     nconf, octree_cap, 0, b.defer, st, s, timed ? gev : nullptr);
+                     nconf, octree_cap, defer_exact ? 2 : 0, b.defer, st, s,
+                     timed ? gev : nullptr);
     if (timed) ctx->recs.push_back({"gather", gev[0], gev[1]});
   }
   CHK(launch_status("octree"));
@@ -398,7 +400,7 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   nconf = lab ? static_cast<uint32_t *>(ctx->pool["node_conflict"].p) : nullptr;
   BuildOut<K> o;
   CHK(build_launch<K>(ctx, b, ctx->st, octree, nconf, ctx->octree_cap, pts, n, dim, bits, labels,
-                      opts, s, o));
+                      opts, s, o, true));
   HIPCHK(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(RunState), hipMemcpyDeviceToHost, s));
   HIPCHK(hipEventRecord(ctx->done, s));
   LastBuild &L = ctx->last;
@@ -417,8 +419,33 @@ int build_impl(pgvd_ctx *ctx, const void *pts, int64_t n, int dim, int bits,
   L.chunks = b.chunks;
   L.defer = b.defer;
   L.conflict_bits = o.cbits;
+  L.lb_splits = reinterpret_cast<uint64_t *>(b.zb + b.z.lb_splits);
+  L.exact_deferred = true;
   L.stream = s;
   return PGVD_OK;
+}
+
+// The order-exact path of a build whose tree turned out malformed (some BRT
+// split took the FP32-quirk value): claims, order-exact local splits and
+// scan, and the scatter fill, as build_launch would have run them.
+template <typename K>
+int exact_path(pgvd_ctx *ctx) {
+  LastBuild &L = ctx->last;
+  const int64_t nk = L.n + 1;
+  int32_t *octree = static_cast<int32_t *>(ctx->pool["octree"].p);
+  uint32_t *nconf = L.labels ? static_cast<uint32_t *>(ctx->pool["node_conflict"].p) : nullptr;
+  const BrtNodeT<K> *brt = static_cast<const BrtNodeT<K> *>(L.brt);
+  {
+    Stage t(ctx, "malformed", L.stream);
+    launch_claims_if_malformed<K>(brt, L.parent, nk, ctx->st, L.stream);
+    launch_splits_scan<K>(brt, L.parent, nullptr, L.ps, nk, L.dim, 1, L.lb_splits, ctx->st,
+                          L.stream);
+    launch_octree<K>(brt, L.parent, L.ps, L.lsx, L.chunks, static_cast<const K *>(L.keys_u),
+                     L.conflict_bits, nk, L.dim, L.mbits, octree, nconf, ctx->octree_cap, 0,
+                     L.defer, ctx->st, L.stream);
+  }
+  CHK(launch_status("order-exact path"));
+  return copy_state_and_wait(ctx, L.stream);
 }
 
 template <typename K>
@@ -605,7 +632,7 @@ int noalloc_impl(char *base, const void *pts, int64_t n, int dim, int bits,
                     std::to_string(kNoAllocScratchBytes));
   BuildOut<K> o;
   CHK(build_launch<K>(nullptr, b, st, octree, node_conflict, octree_cap, pts, n, dim, bits,
-                      labels, opts, s, o));
+                      labels, opts, s, o, false));
   launch_build_state(st, state, s);
   return launch_status("build state");
 }
@@ -1312,6 +1339,12 @@ int pgvd_build_octree_finish(pgvd_ctx *ctx, pgvd_octree_result *out) {
   CHK(use_device(ctx));
   HIPCHK(hipEventSynchronize(ctx->done));
   LastBuild &L = ctx->last;
+  if (L.exact_deferred && ctx->st_host->malformed) {
+    L.exact_deferred = false;
+    if (L.kb == 4) CHK(exact_path<uint32_t>(ctx));
+    else if (L.kb == 8) CHK(exact_path<uint64_t>(ctx));
+    else CHK(exact_path<Key128>(ctx));
+  }
   for (int attempt = 0; attempt < 2 && (ctx->st_host->status & ST_CAPACITY); ++attempt) {
     const int64_t need = (int64_t)ctx->st_host->octree_size;
     if (L.kb == 4) CHK(refill_octree<uint32_t>(ctx, need + need / 4 + 64));

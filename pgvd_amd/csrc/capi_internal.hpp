@@ -75,6 +75,8 @@ struct LastBuild {
   uint32_t *chunks = nullptr;
   uint32_t *defer = nullptr;
   uint32_t *conflict_bits = nullptr;
+  uint64_t *lb_splits = nullptr;   // look-back words of the order-exact scan (zero)
+  bool exact_deferred = false;     // the order-exact path waits for finish
   hipStream_t stream = nullptr;
 };
 

@@ -373,10 +373,12 @@ void launch_claims_if_malformed(const BrtNodeT<K> *brt, int32_t *parent, int64_t
                                 RunState *st, hipStream_t s);
 
 // ---- octree.hip ----
-// Octree fill.  parent[] in the reference form.  force_scatter selects the
+// Octree fill.  parent[] in the reference form.  force_scatter 1 selects the
 // order-exact scatter path unconditionally (stage API, where the BRT comes
-// from the caller; lsx and defer may then be null); otherwise st->malformed
-// selects it on the device and the gather path reads lsx from
+// from the caller; lsx and defer may then be null); 0 lets st->malformed
+// select it on the device; 2 launches the gather path only (the caller
+// runs the order-exact path itself when the tree turns out malformed).
+// The gather path reads lsx from
 // launch_brt_ls and uses defer[] for its miss words ((max_m + 63) / 64
 // 64-bit words, 8-byte aligned).
 // The gather path also writes ps[0..m) from chunk_excl (launch_chunk_scan)

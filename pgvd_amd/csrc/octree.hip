@@ -682,7 +682,7 @@ void launch_dim(const BrtNodeT<K> *brt, const int32_t *parent, uint32_t *ps,
   const int64_t nodes = max_m > 1 ? max_m - 1 : 1;
   const unsigned grid = (unsigned)((nodes + kBlock - 1) / kBlock);
   const unsigned sgrid = grid < 1024 ? grid : 1024;
-  if (!force_scatter) {
+  if (force_scatter != 1) {
     uint64_t *words = reinterpret_cast<uint64_t *>(defer);
     const unsigned ggrid =
         (unsigned)((nodes + kGatherRows * kBlock - 1) / (kGatherRows * kBlock));
@@ -696,6 +696,7 @@ void launch_dim(const BrtNodeT<K> *brt, const int32_t *parent, uint32_t *ps,
     gather_defer_kernel<K, DIM><<<dgrid, kBlock, 0, s>>>(brt, ps, lsx, keys, conflict_bits,
                                                          mbits, octree, node_conflict,
                                                          capacity, words, st);
+    if (force_scatter == 2) return;
   }
   // The scatter path is a no-op unless forced or st->malformed; its kernels
   // loop over their ranges, so a capped grid keeps the no-op cheap.

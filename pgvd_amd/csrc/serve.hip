@@ -192,13 +192,14 @@ int admit(pgvd_server *srv, Slot &sl, Ticket &r) {
 int collect(pgvd_server *srv, Slot &sl, Ticket &r) {
   pgvd_ctx *ctx = sl.ctx;
   pgvd_octree_result res;
-  const bool refilled = (ctx->st_host->status & ST_CAPACITY) != 0;
+  // a refill, or the order-exact path of a malformed tree, rewrites the
+  // octree in finish, after the digest enqueued at admission
+  const bool refilled = (ctx->st_host->status & ST_CAPACITY) != 0 || ctx->st_host->malformed;
   CHK(pgvd_build_octree_finish(ctx, &res));
   r.m = res.m;
   r.octree_size = res.octree_size;
   const int stride = (1 << r.dim) + 1;
   if (refilled) {
-    // the digest enqueued at admission saw the unfilled buffer
     launch_digest(res.octree, (size_t)res.octree_size * stride, sl.d_digest, true, ctx->stream);
     HIPCHK(hipMemcpyAsync(sl.h_digest, sl.d_digest, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
