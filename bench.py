@@ -135,6 +135,21 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
+def _batch_roofline(sizes, counts, dim: int, batch_ms: float) -> dict:
+    """The byte model summed over the instances (their own m and O), moved in
+    one graph replay of the whole segmented pipeline (events around it): the
+    replay is one unit, so the pipeline stands for the dominant kernel here;
+    per-kernel times are in profiles/batch_4096_instances.md."""
+    total = 0
+    for n, m, O in zip(sizes, counts["m"], counts["octree_size"]):
+        total += algorithmic_bytes(int(n), int(m), int(O), dim, 8, 8)["total"]
+    gbps = total / (batch_ms * 1e-3) / 1e9
+    return {"kernel": "the segmented pipeline of all instances (one graph replay)",
+            "bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": gbps / HBM_PEAK_GBPS, "traffic": None, "avg_launch_ms": batch_ms,
+            "algorithmic_bytes": int(total)}
+
+
 def bench_batch(args) -> int:
     """Batched form: `--batch-count` independent 3D point sets (sizes
     log-uniform in [1e3, 1e6], even instances uniform, odd ones clustered;
@@ -157,8 +172,9 @@ def bench_batch(args) -> int:
     dpts = dev.upload(pts)
     offs = np.concatenate([[0], np.cumsum(sizes)])
     b = P.Batch(dev, sizes, dim, bits)
-    ipt, rows = b.tune(dpts)
-    print(f"batch: tuned sort_ipt={ipt} brt_rows={rows}", file=sys.stderr, flush=True)
+    ipt, rows, dbits, waves = b.tune(dpts)
+    print(f"batch: tuned sort_ipt={ipt} brt_rows={rows} digit_bits={dbits} sort_waves={waves}",
+          file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         b.run(dpts)
         b.finish()
@@ -235,8 +251,10 @@ def bench_batch(args) -> int:
                         f"clusters; {bits} bits/axis, 63-bit keys; one segmented pipeline "
                         f"replayed as a HIP graph",
             "instances": int(sizes.size), "total_points": total,
-            "sort_ipt": ipt, "brt_rows": rows,
+            "tuned": {"sort_ipt": ipt, "brt_rows": rows, "digit_bits": dbits,
+                      "sort_waves": waves},
         },
+        "roofline": _batch_roofline(sizes, counts, dim, float(np.median(gpu_ms))),
         "batched_gpu_ms": float(np.median(gpu_ms)),
         "per_instance_ms": plain_s * 1e3,
         "per_instance_keys_per_s": total / plain_s,

@@ -301,8 +301,12 @@ int pgvd_ref_nodes_to_brt(const pgvd_ref_brt_node *in, int64_t count, int key_by
                           int32_t *parent);
 
 /* Kernels::BuildBinaryRadixTree_p (opencl/Kernels.cpp:249-264, kernel
- * C/BuildBRT.c:90-181): m sorted unique keys -> m-1 nodes (brt_out, record
- * type by key_bytes) and parent_out (m-1 int32). */
+ * C/BuildBRT.c:90-181): m sorted keys -> m-1 nodes (brt_out, record type by
+ * key_bytes) and parent_out (m-1 int32: a child claimed twice keeps its
+ * later claimer, as the reference's node loop leaves it).  Equal
+ * neighbouring keys give the tree the reference computes from them (where
+ * three equal keys end the array, the reference also writes one parent
+ * field past its nodes; that write is dropped). */
 int pgvd_build_brt(pgvd_ctx *ctx, const void *keys, int64_t m, int key_bytes, int mbits,
                    void *brt_out, int32_t *parent_out, pgvd_stream s);
 
@@ -545,6 +549,10 @@ typedef struct pgvd_batch_opts {
   int sort_ipt;         /* keys per thread of the sort tile: 8, 12, 15 or 16    */
                         /* (4, 7 or 8 for 128-bit keys); 0: default           */
   int brt_rows;         /* BRT nodes per thread of a workgroup window (0: def.) */
+  int digit_bits;       /* radix digit width of the sort passes: 6, 7 or 8 (8 for
+                           128-bit keys); 0: 8                                */
+  int sort_waves;       /* waves per SIMD the sort kernel is compiled for: 2, 3
+                           or 4; 0: 3                                         */
 } pgvd_batch_opts;
 
 typedef struct pgvd_batch_result {
@@ -577,9 +585,13 @@ int pgvd_batch_run(pgvd_batch *b, const void *pts);
 /* Waits for the last run, grows the octree buffer and refills it if it was
  * too small, and fills *out. */
 int pgvd_batch_finish(pgvd_batch *b, pgvd_batch_result *out);
-/* Times every supported (sort_ipt, brt_rows) pair on pts, keeps the fastest
- * for this plan and reports it.  Either output may be NULL. */
-int pgvd_batch_tune(pgvd_batch *b, const void *pts, int *sort_ipt, int *brt_rows);
+/* Picks the plan's sort tile (sort_ipt), BRT window (brt_rows), radix digit
+ * width (digit_bits) and the sort kernel's waves per SIMD (sort_waves) by
+ * timing runs on pts: every (sort_ipt, brt_rows) pair, then each digit
+ * width, then each wave count, keeping the fastest at each step.  Every
+ * setting gives bit-identical results.  Any output may be NULL. */
+int pgvd_batch_tune(pgvd_batch *b, const void *pts, int *sort_ipt, int *brt_rows,
+                    int *digit_bits, int *sort_waves);
 /* Milliseconds of the last pgvd_batch_run, from events around it. */
 int pgvd_batch_last_ms(pgvd_batch *b, float *ms);
 

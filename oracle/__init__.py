@@ -79,6 +79,9 @@ def lib():
         L.oracle_tree_free.argtypes = [C.c_void_p]
         L.oracle_run_pipeline.argtypes = [f32p, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_int64)]
         L.oracle_run_pipeline.restype = C.c_int64
+        L.oracle_brt_only.argtypes = [u64p, u64p, C.c_int64, C.c_int, i32p, u8p, u8p, u64p,
+                                      u64p, i32p, i32p]
+        L.oracle_brt_only.restype = C.c_int
         L.oracle_multi_intersection.argtypes = [f32p] * 4
         L.oracle_multi_intersection.restype = C.c_int
         L.oracle_fit_boxes.argtypes = [f32p, f32p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -170,6 +173,9 @@ def ref(dim: int = 2):
         L.ref_istream_octree.argtypes = [C.c_char_p, C.c_int64, C.c_void_p, C.c_int64,
                                          C.POINTER(C.c_int64)]
         L.ref_istream_octree.restype = C.c_int
+        L.ref_brt_only.argtypes = [u64p, u64p, C.c_int, C.c_int, i32p, u8p, u8p, u64p, u64p,
+                                   i32p, i32p]
+        L.ref_brt_only.restype = C.c_int
         _REF[dim] = L
     return _REF[dim]
 
@@ -602,3 +608,29 @@ def ref_multi_cells(segs, labels, walk_cells, offsets, O: int) -> dict:
     finally:
         L.refg_mc_free(h)
     return {"labels": lab[: 8 * O].reshape(O, 4, 2), "segs": sg[: 32 * O].reshape(O, 4, 2, 4)}
+
+
+def _brt_only(fn, lo, hi, mbits: int) -> dict:
+    lo = np.ascontiguousarray(lo, dtype=np.uint64)
+    hi = np.zeros_like(lo) if hi is None else np.ascontiguousarray(hi, dtype=np.uint64)
+    m = lo.size
+    n = max(m - 1, 1)
+    out = dict(left=np.zeros(n, np.int32), left_leaf=np.zeros(n, np.uint8),
+               right_leaf=np.zeros(n, np.uint8), lcp_lo=np.zeros(n, np.uint64),
+               lcp_hi=np.zeros(n, np.uint64), lcp_length=np.zeros(n, np.int32),
+               parent=np.zeros(n, np.int32))
+    rc = fn(lo, hi, m, mbits, out["left"], out["left_leaf"], out["right_leaf"], out["lcp_lo"],
+            out["lcp_hi"], out["lcp_length"], out["parent"])
+    if rc:
+        raise ValueError(f"BRT of {m} keys failed ({rc})")
+    return {k: v[: m - 1] for k, v in out.items()}
+
+
+def brt_only(lo, hi, mbits: int) -> dict:
+    """BuildBinaryRadixTree_s alone on sorted keys that may repeat."""
+    return _brt_only(lib().oracle_brt_only, lo, hi, mbits)
+
+
+def ref_brt_only(lo, hi, mbits: int, dim: int = 2) -> dict:
+    """The reference's BuildBinaryRadixTree on the same keys (<= 88 bits)."""
+    return _brt_only(ref(dim).ref_brt_only, lo, hi, mbits)

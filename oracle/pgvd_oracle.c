@@ -1297,3 +1297,38 @@ int oracle_find_intersections(const float *a, const float *b, const int32_t *ori
   }
   return n;
 }
+
+/* BuildBinaryRadixTree_s (opencl/Kernels.cpp:266-273, kernel
+ * C/BuildBRT.c:90-181) alone, on m sorted keys that may hold equal
+ * neighbours (the reference returns a tree for them): nodes in gid order
+ * into a zeroed array, so a child claimed twice keeps its later claimer. */
+int oracle_brt_only(const uint64_t *lo, const uint64_t *hi, int64_t m, int mbits, int32_t *left,
+                    uint8_t *left_leaf, uint8_t *right_leaf, uint64_t *lcp_lo, uint64_t *lcp_hi,
+                    int32_t *lcp_length, int32_t *parent) {
+  if (m < 2 || mbits < 1 || mbits > 128) return -1;
+  oracle_tree t;
+  memset(&t, 0, sizeof t);
+  t.mbits = mbits;
+  t.m = m;
+  t.keys = (u128 *)zalloc(sizeof(u128) * (size_t)m);
+  t.left = (int32_t *)zalloc(4 * (size_t)m);
+  t.lcp_length = (int32_t *)zalloc(4 * (size_t)m);
+  t.parent = (int32_t *)zalloc(4 * (size_t)m);
+  t.left_leaf = (uint8_t *)zalloc((size_t)m);
+  t.right_leaf = (uint8_t *)zalloc((size_t)m);
+  t.lcp = (u128 *)zalloc(sizeof(u128) * (size_t)m);
+  for (int64_t i = 0; i < m; ++i) t.keys[i] = mk(lo[i], hi ? hi[i] : 0);
+  for (int gid = 0; gid < (int)m - 1; ++gid) build_brt_node(&t, (int)m, gid);
+  for (int64_t i = 0; i < m - 1; ++i) {
+    left[i] = t.left[i];
+    left_leaf[i] = t.left_leaf[i];
+    right_leaf[i] = t.right_leaf[i];
+    lcp_lo[i] = (uint64_t)t.lcp[i];
+    lcp_hi[i] = (uint64_t)(t.lcp[i] >> 64);
+    lcp_length[i] = t.lcp_length[i];
+    parent[i] = t.parent[i];
+  }
+  free(t.keys); free(t.left); free(t.lcp_length); free(t.parent);
+  free(t.left_leaf); free(t.right_leaf); free(t.lcp);
+  return 0;
+}

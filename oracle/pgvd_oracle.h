@@ -132,6 +132,13 @@ int oracle_octree_parse_text(const char *text, int64_t len, int dim, int32_t *oc
 int64_t oracle_run_pipeline(const float *pts, int64_t n, int dim, int bits,
                             int64_t *m_out);
 
+/* BuildBinaryRadixTree_s alone on m sorted keys, equal neighbours allowed:
+ * m-1 nodes (a child claimed twice keeps its later claimer); -1 on bad
+ * arguments. */
+int oracle_brt_only(const uint64_t *lo, const uint64_t *hi, int64_t m, int mbits, int32_t *left,
+                    uint8_t *left_leaf, uint8_t *right_leaf, uint64_t *lcp_lo, uint64_t *lcp_hi,
+                    int32_t *lcp_length, int32_t *parent);
+
 /* ---- GVD refinement (2D) ------------------------------------------- */
 /* OctreeUtils::FindIntersections (OctreeUtils.cpp:80-172): rows (t, x, y). */
 int oracle_find_intersections(const float *a, const float *b, const int32_t *origin, int width,

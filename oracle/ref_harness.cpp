@@ -201,6 +201,30 @@ void *ref_build_from_keys(const uint64_t *lo, const uint64_t *hi, int m, int mbi
   return t;
 }
 
+// BuildBinaryRadixTree for every gid of m sorted keys (<= 88 bits; equal
+// neighbours allowed) into a zeroed BrtNode array, nothing else.  With three
+// equal keys at the end, node m-2 writes the parent field of node m-1, one
+// past the reference's m-1 nodes; the array has a spare node for it.
+int ref_brt_only(const uint64_t *lo, const uint64_t *hi, int m, int mbits, int32_t *left,
+                 uint8_t *left_leaf, uint8_t *right_leaf, uint64_t *lcp_lo, uint64_t *lcp_hi,
+                 int32_t *lcp_length, int32_t *parent) {
+  std::vector<BrtNode> I((size_t)m);
+  std::memset(I.data(), 0, sizeof(BrtNode) * I.size());
+  KeyArena arena((size_t)m);
+  if (!arena.base) return -2;
+  for (int i = 0; i < m; ++i) to_bu(lo[i], hi ? hi[i] : 0, &arena.keys()[i]);
+  for (int i = 0; i < m - 1; ++i) BuildBinaryRadixTree(I.data(), arena.keys(), mbits, m, i);
+  for (int i = 0; i < m - 1; ++i) {
+    left[i] = I[i].left;
+    left_leaf[i] = I[i].left_leaf;
+    right_leaf[i] = I[i].right_leaf;
+    from_bu(I[i].lcp, &lcp_lo[i], &lcp_hi[i]);
+    lcp_length[i] = I[i].lcp_length;
+    parent[i] = I[i].parent;
+  }
+  return 0;
+}
+
 int ref_tree_status(void *h) { return static_cast<RefTree *>(h)->status; }
 int ref_tree_m(void *h) { return (int)static_cast<RefTree *>(h)->keys.size(); }
 int ref_tree_octree_size(void *h) { return (int)static_cast<RefTree *>(h)->octree.size(); }

@@ -70,7 +70,8 @@ class BuildOpts(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("quantized_input", C.c_int), ("clamped", C.c_int), ("use_graph", C.c_int),
-                ("sort_ipt", C.c_int), ("brt_rows", C.c_int)]
+                ("sort_ipt", C.c_int), ("brt_rows", C.c_int), ("digit_bits", C.c_int),
+                ("sort_waves", C.c_int)]
 
 
 class BatchResult(C.Structure):
@@ -168,7 +169,8 @@ def lib():
             "pgvd_batch_destroy": ([vp], i32),
             "pgvd_batch_run": ([vp, vp], i32),
             "pgvd_batch_finish": ([vp, C.POINTER(BatchResult)], i32),
-            "pgvd_batch_tune": ([vp, vp, C.POINTER(i32), C.POINTER(i32)], i32),
+            "pgvd_batch_tune": ([vp, vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                 C.POINTER(i32)], i32),
             "pgvd_batch_last_ms": ([vp, C.POINTER(C.c_float)], i32),
             "pgvd_server_create": ([i32, i32, C.c_size_t, C.POINTER(vp)], i32),
             "pgvd_server_destroy": ([vp], i32),
@@ -864,11 +866,11 @@ class Batch:
 
     def __init__(self, dev: "Device", sizes, dim: int, bits: int, quantized: bool = False,
                  clamped: bool = False, use_graph: bool = True, sort_ipt: int = 0,
-                 brt_rows: int = 0):
+                 brt_rows: int = 0, digit_bits: int = 0, sort_waves: int = 0):
         self.dev = dev
         self.sizes = np.ascontiguousarray(sizes, np.int64)
         opts = BatchOpts(int(quantized), int(clamped), int(use_graph), int(sort_ipt),
-                         int(brt_rows))
+                         int(brt_rows), int(digit_bits), int(sort_waves))
         h = C.c_void_p()
         dev._check(lib().pgvd_batch_create(
             dev.ctx, self.sizes.ctypes.data_as(C.POINTER(C.c_int64)), self.sizes.size, dim, bits,
@@ -901,11 +903,13 @@ class Batch:
         self.dev._check(lib().pgvd_batch_last_ms(self.h, C.byref(ms)))
         return ms.value
 
-    def tune(self, pts) -> tuple[int, int]:
+    def tune(self, pts) -> tuple[int, int, int, int]:
+        """(sort_ipt, brt_rows, digit_bits, sort_waves) the tuner kept."""
         p = pts.ptr if isinstance(pts, DeviceBuffer) else int(pts)
-        a, b = C.c_int(), C.c_int()
-        self.dev._check(lib().pgvd_batch_tune(self.h, p, C.byref(a), C.byref(b)))
-        return a.value, b.value
+        a, b, d, w = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self.dev._check(lib().pgvd_batch_tune(self.h, p, C.byref(a), C.byref(b), C.byref(d),
+                                              C.byref(w)))
+        return a.value, b.value, d.value, w.value
 
     @staticmethod
     def counts(r: BatchResult) -> dict:

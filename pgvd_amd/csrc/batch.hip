@@ -25,6 +25,7 @@ struct pgvd_batch {
   bool quantized = false, clamped = false, use_graph = true;
   int64_t B = 0, total_pts = 0, total_slots = 0;
   int sort_ipt = 16, brt_rows = 4;
+  int digit_bits = 8, sort_waves = 3;
   std::vector<SegInfo> h_seg;
   std::vector<int64_t> h_m, h_O, h_koff, h_ooff;
   // device
@@ -129,6 +130,9 @@ int build_tables(pgvd_batch *b) {
   }
   d.sort_ipt = b->sort_ipt;
   d.brt_rows = b->brt_rows;
+  d.digit_bits = b->digit_bits;
+  d.sort_waves = b->sort_waves;
+  b->passes = (b->mbits + b->digit_bits - 1) / b->digit_bits;
   // zero-initialised workspace: digit histograms, sort look-back (all
   // passes), unique and local-split scan look-back
   size_t o = 0;
@@ -316,6 +320,15 @@ int pgvd_batch_create(pgvd_ctx *ctx, const int64_t *sizes, int64_t count, int di
     if (!brt_seg_rows_ok(b->kb, opts->brt_rows)) { delete b; return fail(PGVD_ERR_INVALID, "unsupported brt_rows for this key width"); }
     b->brt_rows = opts->brt_rows;
   }
+  if (opts && opts->digit_bits) {
+    if (!onesweep_seg_digit_ok(b->kb, opts->digit_bits)) { delete b; return fail(PGVD_ERR_INVALID, "unsupported digit_bits for this key width"); }
+    b->digit_bits = opts->digit_bits;
+  }
+  if (opts && opts->sort_waves) {
+    if (!onesweep_seg_waves_ok(opts->sort_waves)) { delete b; return fail(PGVD_ERR_INVALID, "sort_waves must be 2, 3 or 4"); }
+    b->sort_waves = opts->sort_waves;
+  }
+  b->passes = (b->mbits + b->digit_bits - 1) / b->digit_bits;
   b->h_seg.resize(count);
   b->h_koff.resize(count);
   b->h_m.assign(count, 0);
@@ -432,41 +445,69 @@ int pgvd_batch_last_ms(pgvd_batch *b, float *ms) {
   return PGVD_OK;
 }
 
-int pgvd_batch_tune(pgvd_batch *b, const void *pts, int *sort_ipt, int *brt_rows) {
+int pgvd_batch_tune(pgvd_batch *b, const void *pts, int *sort_ipt, int *brt_rows,
+                    int *digit_bits, int *sort_waves) {
   if (!b || !pts) return fail(PGVD_ERR_INVALID, "null argument");
   CHK(use_device(b->ctx));
   const bool graph = b->use_graph;
   b->use_graph = false;
+  // the fastest of three runs (after one that warms up) of the current setting
+  auto time_it = [&](float *t) -> int {
+    CHK(build_tables(b));
+    *t = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      CHK(run_impl(b, pts));
+      CHK(finish_impl(b));
+      float ms = 0;
+      CHK(pgvd_batch_last_ms(b, &ms));
+      if (rep > 0) *t = std::min(*t, ms);
+    }
+    return PGVD_OK;
+  };
+  // Coordinate descent: sort tile and BRT window together, then the digit
+  // width, then the sort kernel's waves per SIMD, each keeping the best.
   const std::vector<int> ipts = b->kb == 16 ? std::vector<int>{4, 7, 8} : std::vector<int>{8, 12, 15, 16};
   const std::vector<int> rows = {2, 4};
-  float best = 1e30f;
-  int bi = b->sort_ipt, br = b->brt_rows;
+  const std::vector<int> digits = b->kb == 16 ? std::vector<int>{8} : std::vector<int>{6, 7, 8};
+  const std::vector<int> waves = {2, 3, 4};
+  float best = 1e30f, t = 0;
+  int bi = b->sort_ipt, br = b->brt_rows, bd = b->digit_bits, bw = b->sort_waves;
   int rc = PGVD_OK;
   for (int ipt : ipts) {
     for (int r : rows) {
       b->sort_ipt = ipt;
       b->brt_rows = r;
-      if ((rc = build_tables(b))) break;
-      float t = 1e30f;
-      for (int rep = 0; rep < 3 && rc == PGVD_OK; ++rep) {   // first run warms up
-        if ((rc = run_impl(b, pts))) break;
-        if ((rc = finish_impl(b))) break;
-        float ms = 0;
-        if ((rc = pgvd_batch_last_ms(b, &ms))) break;
-        if (rep > 0) t = std::min(t, ms);
-      }
-      if (rc) break;
+      if ((rc = time_it(&t))) break;
       if (t < best) { best = t; bi = ipt; br = r; }
     }
     if (rc) break;
   }
+  b->sort_ipt = bi;
+  b->brt_rows = br;
+  for (size_t k = 0; k < digits.size() && rc == PGVD_OK; ++k) {
+    if (digits[k] == bd) continue;
+    b->digit_bits = digits[k];
+    if ((rc = time_it(&t))) break;
+    if (t < best) { best = t; bd = digits[k]; }
+  }
+  b->digit_bits = bd;
+  for (size_t k = 0; k < waves.size() && rc == PGVD_OK; ++k) {
+    if (waves[k] == bw) continue;
+    b->sort_waves = waves[k];
+    if ((rc = time_it(&t))) break;
+    if (t < best) { best = t; bw = waves[k]; }
+  }
   b->use_graph = graph;
   b->sort_ipt = bi;
   b->brt_rows = br;
+  b->digit_bits = bd;
+  b->sort_waves = bw;
   CHK(build_tables(b));
   if (rc) return rc;
   if (sort_ipt) *sort_ipt = bi;
   if (brt_rows) *brt_rows = br;
+  if (digit_bits) *digit_bits = bd;
+  if (sort_waves) *sort_waves = bw;
   return PGVD_OK;
 }
 

@@ -68,8 +68,10 @@ __global__ __launch_bounds__(kBlock) void brt_kernel(const K *__restrict__ keys,
   const BrtOut<K> o = brt_eval(cur, [&](int x) { return keys[x]; }, mbits, m, i);
   brt_flags(o, st);
   brt[i] = o.nd;
+  // (with three equal keys at the end, node m-2 claims node m-1, one past
+  // the m-1 nodes; the reference writes past its array there)
   if (!o.nd.left_leaf) atomicMax(&parent[o.nd.left], i + 1);
-  if (!o.nd.right_leaf) atomicMax(&parent[o.nd.left + 1], i + 1);
+  if (!o.nd.right_leaf && o.nd.left + 1 < m - 1) atomicMax(&parent[o.nd.left + 1], i + 1);
 }
 
 // Pipeline store of node c: the record, its children's parent links as
@@ -339,8 +341,10 @@ __global__ __launch_bounds__(kBlock) void finalize_parents_kernel(int32_t *paren
   const int m = m_host >= 0 ? (int)m_host : (int)st->m;
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= m - 1) return;
+  // the last claimer in node order; node 0 sets its own -1 before any
+  // claim (C/BuildBRT.c:169-179), others keep the zero they start with
   const int32_t pc = parent[c];
-  parent[c] = (c == 0) ? -1 : (pc ? pc - 1 : 0);
+  parent[c] = pc ? pc - 1 : (c == 0 ? -1 : 0);
 }
 
 }  // namespace

@@ -530,7 +530,8 @@ int brt_impl(pgvd_ctx *ctx, const K *keys, int64_t m, int mbits, BrtNodeT<K> *br
   }
   CHK(launch_status("brt"));
   CHK(copy_state_and_wait(ctx, s));
-  return status_error(ctx->st_host->status);
+  // equal neighbouring keys give the tree the reference builds from them
+  return status_error(ctx->st_host->status & ~(uint32_t)ST_DUP_KEY);
 }
 
 template <typename K>

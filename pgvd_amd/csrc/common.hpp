@@ -69,6 +69,20 @@ __device__ inline uint32_t kdigit(Key128 a, int shift) {
   return shift >= 64 ? (uint32_t)(a.hi >> (shift - 64)) & 0xffu : (uint32_t)(a.lo >> shift) & 0xffu;
 }
 
+// `mask`-wide digit (mask = 2^w - 1, w <= 8) starting at bit `shift`, which
+// may straddle the two words of a 128-bit key.
+__device__ inline uint32_t kdigit_w(uint32_t a, int shift, uint32_t mask) {
+  return (a >> shift) & mask;
+}
+__device__ inline uint32_t kdigit_w(uint64_t a, int shift, uint32_t mask) {
+  return (uint32_t)(a >> shift) & mask;
+}
+__device__ inline uint32_t kdigit_w(Key128 a, int shift, uint32_t mask) {
+  if (shift >= 64) return (uint32_t)(a.hi >> (shift - 64)) & mask;
+  const uint64_t w = shift ? (a.lo >> shift) | (a.hi << (64 - shift)) : a.lo;
+  return (uint32_t)w & mask;
+}
+
 // Key with bit b (0 <= b < width) flipped.
 __host__ __device__ inline uint32_t kflip(uint32_t a, int b) { return a ^ (1u << b); }
 __host__ __device__ inline uint64_t kflip(uint64_t a, int b) { return a ^ (1ull << b); }

@@ -409,11 +409,17 @@ struct BatchDev {
   SegState *ss = nullptr;
   int64_t B = 0;
   int sort_ipt = 16, brt_rows = 4;
+  int digit_bits = 8;   // radix digit width of the sort passes (6, 7 or 8)
+  int sort_waves = 3;   // waves per SIMD the sort kernel is compiled for (2, 3 or 4)
   TileTable chunk{}, sort{}, scan{}, brt{}, node{};
   int64_t n_chunk = 0, n_sort = 0, n_scan = 0, n_brt = 0, n_node = 0;
 };
-// Sort tile configurations the batched sort is built for (keys per thread).
+// Sort tile configurations the batched sort is built for (keys per thread),
+// radix digit widths (6, 7, 8 bits; 128-bit keys: 8, so that at most 16
+// passes run) and waves per SIMD (2, 3, 4).
 bool onesweep_seg_ipt_ok(int key_bytes, int ipt);
+bool onesweep_seg_digit_ok(int key_bytes, int digit_bits);
+bool onesweep_seg_waves_ok(int waves);
 // BRT window configurations (nodes per thread).
 bool brt_seg_rows_ok(int key_bytes, int rows);
 

@@ -500,8 +500,9 @@ template <typename K, int DIM, bool INT>
 __global__ __launch_bounds__(kBlock) void encode_seg_kernel(
     const void *__restrict__ pts, const SegInfo *__restrict__ seg, TileTable ct,
     const SegState *__restrict__ ss, int bits, float effw, K *__restrict__ keys,
-    uint32_t *__restrict__ bhist, int passes, RunState *st) {
+    uint32_t *__restrict__ bhist, int passes, int dbits, RunState *st) {
   __shared__ uint32_t sh[kMaxPasses * 256];
+  const uint32_t dmask = (1u << dbits) - 1u;
   for (int i = threadIdx.x; i < passes * 256; i += kBlock) sh[i] = 0;
   const uint32_t chunk = blockIdx.x;
   const uint32_t s = ct.seg[chunk];
@@ -543,7 +544,7 @@ __global__ __launch_bounds__(kBlock) void encode_seg_kernel(
     morton128<DIM, (sizeof(K) > 8)>(c, bits, lo, hi);
     const K key = from128<K>(lo, hi);
     out[i] = key;
-    for (int r = 0; r < passes; ++r) atomicAdd(&sh[r * 256 + kdigit(key, 8 * r)], 1u);
+    for (int r = 0; r < passes; ++r) atomicAdd(&sh[r * 256 + kdigit_w(key, dbits * r, dmask)], 1u);
   }
   if (neg) atomicOr(&st->status, (uint32_t)ST_NEG_COORD);
   __syncthreads();
@@ -680,12 +681,13 @@ void launch_encode_seg(const void *pts, int quantized, int dim, int bits, float 
                        hipStream_t s) {
   if (b.n_chunk <= 0) return;
   const unsigned g = (unsigned)b.n_chunk;
+  const int db = b.digit_bits;
   if (quantized) {
-    if (dim == 2) encode_seg_kernel<K, 2, true><<<g, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss, bits, effw, keys, bhist, passes, st);
-    else encode_seg_kernel<K, 3, true><<<g, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss, bits, effw, keys, bhist, passes, st);
+    if (dim == 2) encode_seg_kernel<K, 2, true><<<g, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss, bits, effw, keys, bhist, passes, db, st);
+    else encode_seg_kernel<K, 3, true><<<g, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss, bits, effw, keys, bhist, passes, db, st);
   } else {
-    if (dim == 2) encode_seg_kernel<K, 2, false><<<g, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss, bits, effw, keys, bhist, passes, st);
-    else encode_seg_kernel<K, 3, false><<<g, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss, bits, effw, keys, bhist, passes, st);
+    if (dim == 2) encode_seg_kernel<K, 2, false><<<g, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss, bits, effw, keys, bhist, passes, db, st);
+    else encode_seg_kernel<K, 3, false><<<g, kBlock, 0, s>>>(pts, b.seg, b.chunk, b.ss, bits, effw, keys, bhist, passes, db, st);
   }
 }
 

@@ -76,13 +76,16 @@ __device__ inline OnesweepLds<K, PAY, IPT> &onesweep_lds() {
 // the tile's index in the look-back array (tiles lo..tile-1 precede it in the
 // same run); ghist is the run's digit histogram for this pass.  FULL: nvalid
 // is the whole tile, so no key needs a range test.
-template <typename K, bool PAY, int IPT, bool FULL>
+// DBITS: the digit width (<= 8); the tile's 256 digit slots hold 2^DBITS
+// digits, and slots above them stay empty.
+template <typename K, bool PAY, int IPT, bool FULL, int DBITS = 8>
 __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__restrict__ kin, K *__restrict__ kout,
                                      const uint32_t *__restrict__ vin, uint32_t *__restrict__ vout,
                                      uint32_t base, uint32_t nvalid, uint32_t tile, uint32_t lo,
                                      uint32_t seg0, int shift, const uint32_t *__restrict__ ghist,
                                      uint32_t *__restrict__ lookback) {
   constexpr int TILE = kBlock * IPT;
+  constexpr uint32_t DMASK = (1u << DBITS) - 1u;
   OnesweepLds<K, PAY, IPT> &L = onesweep_lds<K, PAY, IPT>();
   uint32_t *s_cnt = reinterpret_cast<uint32_t *>(L.raw);
   K *s_keys = reinterpret_cast<K *>(L.raw);
@@ -116,7 +119,7 @@ __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__r
   for (int j = 0; j < IPT; ++j) {
     const uint32_t le = wofs + j * 64 + lane;
     const bool valid = FULL || le < nvalid;
-    const uint32_t d = kdigit(k[j], shift);
+    const uint32_t d = DBITS == 8 ? kdigit(k[j], shift) : kdigit_w(k[j], shift, DMASK);
     uint32_t plo = ~0u, phi = ~0u;
     if (!FULL) {
       const uint64_t vb = __ballot(valid);
@@ -124,7 +127,7 @@ __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__r
       phi = (uint32_t)(vb >> 32);
     }
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < DBITS; ++b) {
       const int32_t sb = __builtin_amdgcn_sbfe((int32_t)d, b, 1);
       const uint64_t bb = __ballot(sb < 0);
       plo = __builtin_amdgcn_bitop3_b32(plo, (uint32_t)bb, (uint32_t)sb, 0x90);
@@ -159,7 +162,7 @@ __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__r
   // trip, consume them nearest first until an inclusive prefix; a tile that
   // has not published yet is re-read on the next round.
   uint32_t excl = 0;
-  if (tile > lo) {
+  if (tile > lo && (DBITS == 8 || (uint32_t)t <= DMASK)) {   // slots above the digits stay 0
     constexpr int kBatch = 8;
     int kk = (int)tile - 1;
     for (;;) {
@@ -192,7 +195,7 @@ __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__r
   for (int j = 0; j < IPT; ++j) {
     const uint32_t le = wofs + j * 64 + lane;
     if (FULL || le < nvalid) {
-      const uint32_t d = kdigit(k[j], shift);
+      const uint32_t d = DBITS == 8 ? kdigit(k[j], shift) : kdigit_w(k[j], shift, DMASK);
       rk[j] += s_dig[d] + s_cnt[w * 256 + d];
     }
   }
@@ -213,7 +216,8 @@ __device__ __attribute__((always_inline)) inline void onesweep_tile(const K *__r
     const uint32_t p = j * kBlock + t;
     if (FULL || p < nvalid) {
       const K key = s_keys[p];
-      const uint32_t g = s_dig[kdigit(key, shift)] + p;
+      const uint32_t g =
+          s_dig[DBITS == 8 ? kdigit(key, shift) : kdigit_w(key, shift, DMASK)] + p;
       kout[g] = key;
       if (PAY) vout[g] = s_vals[p];
     }
@@ -250,8 +254,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PGVD_SOR
 // Segmented form: instance s owns keys [seg.kofs, +seg.n) and sort tiles
 // [tt.first[s], tt.first[s+1]); each instance is sorted on its own, in one
 // launch per digit for the whole batch.
-template <typename K, int IPT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PGVD_SORT_WAVES))) void onesweep_seg_kernel(
+template <typename K, int IPT, int DBITS, int WAVES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void onesweep_seg_kernel(
     const K *__restrict__ kin, K *__restrict__ kout, const SegInfo *__restrict__ seg,
     TileTable tt, int passes, int pass, const uint32_t *__restrict__ bhist,
     uint32_t *__restrict__ lookback, uint32_t *tile_counter) {
@@ -264,11 +268,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PGVD_SOR
   const uint32_t nvalid = (si.n - lofs) < TILE ? (si.n - lofs) : TILE;
   const uint32_t *gh = bhist + ((size_t)s * passes + pass) * 256;
   if (nvalid == TILE)
-    onesweep_tile<K, false, IPT, true>(kin, kout, nullptr, nullptr, si.kofs + lofs, nvalid, tile,
-                                       lo, si.kofs, 8 * pass, gh, lookback);
+    onesweep_tile<K, false, IPT, true, DBITS>(kin, kout, nullptr, nullptr, si.kofs + lofs, nvalid,
+                                              tile, lo, si.kofs, DBITS * pass, gh, lookback);
   else
-    onesweep_tile<K, false, IPT, false>(kin, kout, nullptr, nullptr, si.kofs + lofs, nvalid, tile,
-                                        lo, si.kofs, 8 * pass, gh, lookback);
+    onesweep_tile<K, false, IPT, false, DBITS>(kin, kout, nullptr, nullptr, si.kofs + lofs, nvalid,
+                                               tile, lo, si.kofs, DBITS * pass, gh, lookback);
 }
 
 }  // namespace
@@ -328,14 +332,40 @@ bool onesweep_seg_ipt_ok(int key_bytes, int ipt) {
                          : (ipt == 8 || ipt == 12 || ipt == 15 || ipt == 16);
 }
 
-template <typename K, int IPT>
+bool onesweep_seg_digit_ok(int key_bytes, int digit_bits) {
+  return key_bytes == 16 ? digit_bits == 8 : (digit_bits >= 6 && digit_bits <= 8);
+}
+
+bool onesweep_seg_waves_ok(int waves) { return waves >= 2 && waves <= 4; }
+
+template <typename K, int IPT, int DBITS, int WAVES>
 void seg_passes(K *&src, K *&dst, const BatchDev &b, int passes, const uint32_t *bhist,
                 uint32_t *lookback, RunState *st, hipStream_t s) {
   for (int r = 0; r < passes; ++r) {
     uint32_t *lb = lookback + (size_t)r * (size_t)b.n_sort * 256;
-    onesweep_seg_kernel<K, IPT><<<(unsigned)b.n_sort, kBlock, 0, s>>>(
+    onesweep_seg_kernel<K, IPT, DBITS, WAVES><<<(unsigned)b.n_sort, kBlock, 0, s>>>(
         src, dst, b.seg, b.sort, passes, r, bhist, lb, &st->counters[r]);
     K *tk = src; src = dst; dst = tk;
+  }
+}
+
+template <typename K, int IPT, int DBITS>
+void seg_passes_w(K *&src, K *&dst, const BatchDev &b, int passes, const uint32_t *bhist,
+                  uint32_t *lookback, RunState *st, hipStream_t s) {
+  if (b.sort_waves == 2) seg_passes<K, IPT, DBITS, 2>(src, dst, b, passes, bhist, lookback, st, s);
+  else if (b.sort_waves == 4) seg_passes<K, IPT, DBITS, 4>(src, dst, b, passes, bhist, lookback, st, s);
+  else seg_passes<K, IPT, DBITS, 3>(src, dst, b, passes, bhist, lookback, st, s);
+}
+
+template <typename K, int IPT>
+void seg_passes_d(K *&src, K *&dst, const BatchDev &b, int passes, const uint32_t *bhist,
+                  uint32_t *lookback, RunState *st, hipStream_t s) {
+  if constexpr (sizeof(K) == 16) {
+    seg_passes_w<K, IPT, 8>(src, dst, b, passes, bhist, lookback, st, s);
+  } else {
+    if (b.digit_bits == 6) seg_passes_w<K, IPT, 6>(src, dst, b, passes, bhist, lookback, st, s);
+    else if (b.digit_bits == 7) seg_passes_w<K, IPT, 7>(src, dst, b, passes, bhist, lookback, st, s);
+    else seg_passes_w<K, IPT, 8>(src, dst, b, passes, bhist, lookback, st, s);
   }
 }
 
@@ -345,14 +375,14 @@ int launch_onesweep_seg(K *keys, K *keys_alt, const BatchDev &b, int passes,
   if (passes <= 0 || b.n_sort <= 0) return 0;
   K *src = keys, *dst = keys_alt;
   if constexpr (sizeof(K) == 16) {
-    if (b.sort_ipt == 4) seg_passes<K, 4>(src, dst, b, passes, bhist, lookback, st, s);
-    else if (b.sort_ipt == 7) seg_passes<K, 7>(src, dst, b, passes, bhist, lookback, st, s);
-    else seg_passes<K, 8>(src, dst, b, passes, bhist, lookback, st, s);
+    if (b.sort_ipt == 4) seg_passes_d<K, 4>(src, dst, b, passes, bhist, lookback, st, s);
+    else if (b.sort_ipt == 7) seg_passes_d<K, 7>(src, dst, b, passes, bhist, lookback, st, s);
+    else seg_passes_d<K, 8>(src, dst, b, passes, bhist, lookback, st, s);
   } else {
-    if (b.sort_ipt == 8) seg_passes<K, 8>(src, dst, b, passes, bhist, lookback, st, s);
-    else if (b.sort_ipt == 12) seg_passes<K, 12>(src, dst, b, passes, bhist, lookback, st, s);
-    else if (b.sort_ipt == 15) seg_passes<K, 15>(src, dst, b, passes, bhist, lookback, st, s);
-    else seg_passes<K, 16>(src, dst, b, passes, bhist, lookback, st, s);
+    if (b.sort_ipt == 8) seg_passes_d<K, 8>(src, dst, b, passes, bhist, lookback, st, s);
+    else if (b.sort_ipt == 12) seg_passes_d<K, 12>(src, dst, b, passes, bhist, lookback, st, s);
+    else if (b.sort_ipt == 15) seg_passes_d<K, 15>(src, dst, b, passes, bhist, lookback, st, s);
+    else seg_passes_d<K, 16>(src, dst, b, passes, bhist, lookback, st, s);
   }
   return passes & 1;
 }

@@ -147,8 +147,9 @@ def test_batch_tuner_keeps_results(dev):
     d = dev.upload(pts)
     b = pgvd_amd.Batch(dev, [len(p) for p in parts], dim, bits)
     try:
-        ipt, rows = b.tune(d)
+        ipt, rows, dbits, waves = b.tune(d)
         assert ipt in (8, 12, 15, 16) and rows in (2, 4)
+        assert dbits in (6, 7, 8) and waves in (2, 3, 4)
         b.run(d)
         all_ = b.fetch_all(b.finish())
         for i, p in enumerate(parts):
@@ -169,6 +170,31 @@ def test_batch_every_config(dev, ipt, rows):
         for i, p in enumerate(parts):
             got = pgvd_amd.Batch.instance(all_, i, 8, dim)
             assert_tree_equal(got, dev.build_octree(p, bits), BATCH_FIELDS, where=f"cfg {i}")
+    finally:
+        b.close()
+        d.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim,bits", [(3, 21), (2, 16), (3, 42)])
+@pytest.mark.parametrize("dbits", [6, 7, 8])
+@pytest.mark.parametrize("waves", [2, 3, 4])
+def test_batch_every_digit_width_and_wave_count(dev, dim, bits, dbits, waves):
+    """Every radix digit width and sort-kernel wave count the tuner can pick
+    gives every instance's plain build, field by field (128-bit keys take
+    8-bit digits only)."""
+    if bits == 42 and dbits != 8:
+        with pytest.raises(pgvd_amd.PgvdError):
+            pgvd_amd.Batch(dev, [10], dim, bits, digit_bits=dbits)
+        return
+    parts = make_batch([1, 3000, 70000, 2, 9001], dim, seed=dbits * 10 + waves + bits)
+    b, r, all_, d = run_batch(dev, parts, dim, bits, digit_bits=dbits, sort_waves=waves)
+    kb = 16 if dim * bits > 64 else (8 if dim * bits > 32 else 4)
+    try:
+        for i, p in enumerate(parts):
+            got = pgvd_amd.Batch.instance(all_, i, kb, dim)
+            assert_tree_equal(got, dev.build_octree(p, bits), BATCH_FIELDS,
+                              where=f"digits {dbits} waves {waves} instance {i}")
     finally:
         b.close()
         d.free()
@@ -200,6 +226,10 @@ def test_batch_rejects_bad_sizes(dev):
         pgvd_amd.Batch(dev, [(1 << 23) + 1], 3, 21)
     with pytest.raises(pgvd_amd.PgvdError):
         pgvd_amd.Batch(dev, [10], 3, 21, sort_ipt=7)
+    with pytest.raises(pgvd_amd.PgvdError):
+        pgvd_amd.Batch(dev, [10], 3, 21, digit_bits=5)
+    with pytest.raises(pgvd_amd.PgvdError):
+        pgvd_amd.Batch(dev, [10], 3, 21, sort_waves=5)
 
 
 @pytest.mark.gpu

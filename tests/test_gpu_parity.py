@@ -400,10 +400,40 @@ def test_brt_and_octree_stages(dev, mbits, dim):
     assert np.array_equal(ch, want["children"]) and np.array_equal(lf, want["leaf"])
 
 
-def test_brt_rejects_duplicate_keys(dev):
-    with pytest.raises(P.PgvdError) as e:
-        dev.build_brt(np.array([1, 5, 5, 9], np.uint64), None, 32, key_bytes=8)
-    assert e.value.code == 9
+@pytest.mark.parametrize("kb,mbits", [(4, 30), (8, 63), (16, 87), (16, 126)])
+def test_brt_of_duplicate_keys_matches_oracle(dev, kb, mbits):
+    """BuildBinaryRadixTree_p on sorted keys with runs of equal neighbours
+    (pairs, triples, runs at both ends) returns the nodes and parent links
+    of the reference's node loop (the oracle's, which equals the reference's
+    own BuildBinaryRadixTree on such keys up to 88 bits)."""
+    rng = np.random.default_rng(kb + mbits)
+    for trial in range(40):
+        m = int(rng.integers(2, 3000))
+        lo = rng.integers(0, 2**63, m, dtype=np.uint64)
+        hi = rng.integers(0, 2**63, m, dtype=np.uint64) if mbits > 64 else np.zeros(m, np.uint64)
+        if mbits < 64:
+            lo &= np.uint64((1 << mbits) - 1)
+        if mbits > 64:
+            hi &= np.uint64((1 << (mbits - 64)) - 1)
+        for _ in range(m // 3):
+            i = int(rng.integers(0, m))
+            lo[(i + 1) % m], hi[(i + 1) % m] = lo[i], hi[i]
+        order = np.lexsort((lo, hi))
+        lo, hi = lo[order], hi[order]
+        if trial % 4 == 0:
+            lo[-3:], hi[-3:] = lo[-1], hi[-1]       # three equal keys at the end
+        brt, parent = dev.build_brt(lo, hi if mbits > 64 else None, mbits, key_bytes=kb)
+        want = oracle.brt_only(lo, hi, mbits)
+        assert np.array_equal(brt["left"], want["left"]), trial
+        assert np.array_equal(brt["left_leaf"], want["left_leaf"]), trial
+        assert np.array_equal(brt["right_leaf"], want["right_leaf"]), trial
+        assert np.array_equal(brt["lcp_length"], want["lcp_length"]), trial
+        assert np.array_equal(parent, want["parent"]), trial
+        if kb == 16:
+            assert np.array_equal(brt["lcp_lo"], want["lcp_lo"]), trial
+            assert np.array_equal(brt["lcp_hi"], want["lcp_hi"]), trial
+        else:
+            assert np.array_equal(brt["lcp"].astype(np.uint64), want["lcp_lo"]), trial
 
 
 def test_octree_capacity_is_reported(dev):

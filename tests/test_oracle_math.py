@@ -634,3 +634,36 @@ def test_cell_walk_order_check_catches_planted_mistakes():
     assert "owner" in planted(octant)
     assert "owner" in planted(parent)
     assert "offsets" in planted(offsets)
+
+
+@pytest.mark.skipif(not oracle.ref_available(2), reason="reference checker not built here")
+def test_brt_of_duplicate_keys_matches_reference():
+    """The reference's BuildBinaryRadixTree on sorted keys with equal
+    neighbours (pairs, triples, equal runs at the end) equals the oracle's
+    node loop field by field, parents included (up to 88 bits)."""
+    rng = np.random.default_rng(5)
+    for t in range(400):
+        mbits = int(rng.integers(8, 89))
+        m = int(rng.integers(2, 300))
+        k = rng.integers(0, 2**62, m, dtype=np.uint64)
+        hi = None
+        if mbits < 62:
+            k >>= np.uint64(62 - mbits)
+        elif mbits > 64:
+            hi = rng.integers(0, 1 << (mbits - 64), m, dtype=np.uint64)
+        for _ in range(m // 3):
+            i = int(rng.integers(0, m))
+            k[(i + 1) % m] = k[i]
+            if hi is not None:
+                hi[(i + 1) % m] = hi[i]
+        order = np.lexsort((k,) if hi is None else (k, hi))
+        k = k[order]
+        hi = None if hi is None else hi[order]
+        if t % 5 == 0:
+            k[-3:] = k[-1]
+            if hi is not None:
+                hi[-3:] = hi[-1]
+        a = oracle.brt_only(k, hi, mbits)
+        b = oracle.ref_brt_only(k, hi, mbits)
+        for f in a:
+            assert np.array_equal(a[f], b[f]), (t, f)
