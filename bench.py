@@ -367,6 +367,15 @@ def bench_serve(args) -> int:
     return 0 if not bad and not mism else 1
 
 
+def _morton3(q, bits: int) -> np.ndarray:
+    """Morton keys of 3D grid points (bit i of axis k at bit 3i + k)."""
+    key = np.zeros(q.shape[0], np.uint64)
+    for i in range(bits):
+        for k in range(3):
+            key |= ((q[:, k].astype(np.uint64) >> np.uint64(i)) & np.uint64(1)) << np.uint64(3 * i + k)
+    return key
+
+
 def bench_findleaf(args) -> int:
     """Batched point location, OctreeUtils::FindLeaf (OctreeUtils.cpp:18-71):
     2^24 uniform 3D grid points (21 bits/axis, seed 7) located in the octree
@@ -385,6 +394,8 @@ def bench_findleaf(args) -> int:
     res = dev.build_octree_finish()
     O = int(res.octree_size)
     q = np.random.default_rng(7).integers(0, 1 << bits, (nq, dim), dtype=np.int32)
+    if args.query_order == "morton":
+        q = q[np.argsort(_morton3(q, bits), kind="stable")]
     dq = dev.upload(q)
     outs = {k: dev.alloc(nq * 4) for k in ("parent", "octant", "data", "level", "status")}
     for _ in range(max(args.warmup, 1)):
@@ -420,8 +431,10 @@ def bench_findleaf(args) -> int:
         "dtype": "i32",
         "data": "synthetic",
         "config": {
-            "workload": f"{nq} uniform grid queries (seed 7) in the octree of {desc}, "
-                        f"{bits} bits/axis ({O} octree nodes)",
+            "workload": f"{nq} uniform grid queries (seed 7"
+                        f"{', sorted by Morton key' if args.query_order == 'morton' else ''}) "
+                        f"in the octree of {desc}, {bits} bits/axis ({O} octree nodes)",
+            "query_order": args.query_order,
             "queries": nq, "octree_nodes": O, "dim": dim, "bits": bits,
             "failed_queries": int((got["status"] != 0).sum()),
             "level_histogram": {int(l): int(c) for l, c in enumerate(levels) if c},
@@ -435,7 +448,8 @@ def bench_findleaf(args) -> int:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": gbps / HBM_PEAK_GBPS,
-            "traffic": load_traffic("find_leaf", "findleaf"),
+            "traffic": load_traffic("find_leaf", "findleaf" if args.query_order == "random"
+                                    else "findleaf_morton"),
             "avg_launch_ms": launch_ms,
             "launches": kcnt,
             "algorithmic_bytes": abytes,
@@ -803,6 +817,9 @@ def main() -> int:
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--n", type=int, default=0, help="points (default 2^24; 2^22 for conflict)")
+    ap.add_argument("--query-order", choices=["random", "morton"], default="random",
+                    help="findleaf: queries as drawn, or sorted by Morton key (a caller "
+                         "whose queries are spatially coherent)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",

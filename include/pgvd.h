@@ -60,6 +60,22 @@ const char *pgvd_version(void);
 int pgvd_ctx_create(int device, pgvd_ctx **out);
 int pgvd_ctx_destroy(pgvd_ctx *ctx);
 int pgvd_ctx_stream(pgvd_ctx *ctx, pgvd_stream *out);
+/* Most device bytes the context's pool may hold (0, the default: no limit).
+ * A call whose pool buffers would pass it fails with PGVD_ERR_ALLOC before
+ * allocating, as when the device is out of memory; the buffers already
+ * held stay, and the context stays usable.  Lowering the limit frees
+ * nothing.  No counterpart in the reference (for callers that share a
+ * device between contexts). */
+int pgvd_ctx_set_pool_limit(pgvd_ctx *ctx, size_t bytes);
+/* Device bytes the context's pool holds now. */
+int pgvd_ctx_pool_bytes(pgvd_ctx *ctx, size_t *out);
+/* Test hook, no counterpart in the reference: the next launch check of the
+ * named stage on this thread ("encode", "sort", "unique", "brt",
+ * "splits_scan", "octree", ...; the stage names of the message a failing
+ * call leaves) reports PGVD_ERR_HIP as if its launch had failed, and the
+ * call returns at once, with work already enqueued left to run.  NULL or ""
+ * clears it.  For tests of the error paths. */
+int pgvd_debug_fail_launch(const char *stage);
 int pgvd_synchronize(pgvd_ctx *ctx, pgvd_stream s);
 int pgvd_alloc(pgvd_ctx *ctx, size_t bytes, void **dptr);
 int pgvd_free(pgvd_ctx *ctx, void *dptr);

@@ -720,6 +720,7 @@ static void cw_leaf(const float p[2], int bits, const int32_t *octree, int64_t O
 
 int64_t oracle_cell_walk(const float *segs, int64_t n, int bits, const int32_t *octree, int64_t O,
                          int32_t *counts, int32_t *status, int32_t *cells, int64_t cap) {
+  if (bits < 1 || bits > 30) return -1;   /* int32 cells, as pgvd_cell_walk checks */
   int64_t total = 0;
   for (int64_t s = 0; s < n; ++s) {
     const float *a = segs + 4 * s, *b = segs + 4 * s + 2;
@@ -1202,6 +1203,7 @@ void oracle_multi_cells(const float *segs, const int32_t *labels, int64_t n, int
                         int32_t *status) {
   for (int64_t i = 0; i < O * 8; ++i) ci_labels[i] = -1;
   memset(ci_segs, 0, sizeof(float) * (size_t)(O * 32));
+  if (bits < 1 || bits > 30) return;   /* int32 cells, as pgvd_multi_cells checks */
   for (int64_t s = 0; s < n; ++s) {
     const float *a = segs + 4 * s, *b = segs + 4 * s + 2;
     int dir[2];

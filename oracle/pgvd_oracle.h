@@ -118,7 +118,8 @@ void oracle_find_leaf(const int64_t *p, int64_t n, int dim, int bits, const int3
  * located with oracle_find_leaf.  Per segment counts[i] cells and status[i]
  * (0; 1 after 10000 cells, where the reference throws).  Cells are written
  * as {origin x, origin y, width, parent, octant} int32 records while fewer
- * than cap have been written; returns the total. */
+ * than cap have been written; returns the total, or -1 when bits is outside
+ * 1..30 (the cells are int32). */
 int64_t oracle_cell_walk(const float *segs, int64_t n, int bits, const int32_t *octree, int64_t O,
                          int32_t *counts, int32_t *status, int32_t *cells, int64_t cap);
 
@@ -156,7 +157,8 @@ int64_t oracle_fit_boxes(const float *a, const float *b, float min_d, float *sam
 /* 1 when the reference's FitBoxes reads an unset segment on these inputs. */
 int oracle_fit_boxes_undefined(const float *a, const float *b, float min_d);
 /* FindMultiCells' walks with MCCallback (viewer/Octree2.cpp:366-455):
- * CellIntersections of every node, ci_labels O*8 (-1 empty), ci_segs O*32. */
+ * CellIntersections of every node, ci_labels O*8 (-1 empty), ci_segs O*32;
+ * nothing is walked when bits is outside 1..30. */
 void oracle_multi_cells(const float *segs, const int32_t *labels, int64_t n, int bits,
                         const int32_t *octree, int64_t O, int32_t *ci_labels, float *ci_segs,
                         int32_t *status);

@@ -37,7 +37,7 @@ STATUS = {
     0: "PGVD_OK", 1: "PGVD_ERR_INVALID", 2: "PGVD_ERR_EMPTY", 3: "PGVD_ERR_BAD_COORD",
     4: "PGVD_ERR_NO_DEVICE", 5: "PGVD_ERR_ALLOC", 6: "PGVD_ERR_HIP", 7: "PGVD_ERR_CAPACITY",
     8: "PGVD_ERR_TOO_LARGE", 9: "PGVD_ERR_DUP_KEY", 10: "PGVD_ERR_INTERNAL",
-    11: "PGVD_NOT_READY",
+    11: "PGVD_NOT_READY", 12: "PGVD_ERR_GEOMETRY",
 }
 
 
@@ -134,6 +134,9 @@ def lib():
             "pgvd_ctx_create": ([i32, C.POINTER(vp)], i32),
             "pgvd_ctx_destroy": ([vp], i32),
             "pgvd_ctx_stream": ([vp, C.POINTER(vp)], i32),
+            "pgvd_ctx_set_pool_limit": ([vp, C.c_size_t], i32),
+            "pgvd_ctx_pool_bytes": ([vp, C.POINTER(C.c_size_t)], i32),
+            "pgvd_debug_fail_launch": ([C.c_char_p], i32),
             "pgvd_synchronize": ([vp, vp], i32),
             "pgvd_alloc": ([vp, C.c_size_t, C.POINTER(vp)], i32),
             "pgvd_free": ([vp, vp], i32),
@@ -296,6 +299,12 @@ def words_to_keys(words: np.ndarray, key_bytes: int):
     return lo, np.zeros_like(lo)
 
 
+def debug_fail_launch(stage) -> None:
+    """Test hook: the next launch check of `stage` on this thread reports
+    PGVD_ERR_HIP (None clears it)."""
+    lib().pgvd_debug_fail_launch(stage.encode() if stage else None)
+
+
 @dataclass
 class DeviceBuffer:
     dev: "Device"
@@ -339,6 +348,16 @@ class Device:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_pool_limit(self, nbytes: int) -> None:
+        """Most device bytes this context's pool may hold (0: no limit);
+        a call that would pass it raises PgvdError PGVD_ERR_ALLOC."""
+        self._check(lib().pgvd_ctx_set_pool_limit(self.ctx, int(nbytes)))
+
+    def pool_bytes(self) -> int:
+        out = C.c_size_t()
+        self._check(lib().pgvd_ctx_pool_bytes(self.ctx, C.byref(out)))
+        return int(out.value)
 
     def alloc(self, nbytes: int) -> DeviceBuffer:
         p = C.c_void_p()

@@ -58,8 +58,10 @@ template <typename T>
 int dalloc(T **p, size_t bytes) {
   void *q = nullptr;
   if (bytes == 0) bytes = 256;
-  if (hipMalloc(&q, bytes) != hipSuccess)
+  if (hipMalloc(&q, bytes) != hipSuccess) {
+    (void)hipGetLastError();   // not left for the next launch check to report
     return fail(PGVD_ERR_ALLOC, "hipMalloc failed for a batch buffer (" + std::to_string(bytes) + " bytes)");
+  }
   *p = static_cast<T *>(q);
   return PGVD_OK;
 }

@@ -241,7 +241,8 @@ brt_ls_kernel(const K *__restrict__ keys, int mbits,
 
 // Pipeline, second pass: the nodes the window could not hold (set bits of
 // the first pass's miss words, one word per 64 nodes), against keys in
-// global memory.
+// global memory, with the 8-ary searches (7 independent loads a round in
+// place of one dependent load a step).
 template <typename K, int DIM>
 __global__ __launch_bounds__(kBlock) void brt_ls_defer_kernel(const K *__restrict__ keys,
                                                               int mbits,
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void brt_ls_defer_kernel(const K *__restric
   for_each_masked(miss_words, nwords, [&](uint32_t cu) {
     const int c = (int)cu;
     const K cur = keys[c];
-    const BrtOut<K> o = brt_eval(cur, [&](int x) { return keys[x]; }, mbits, m, c);
+    const BrtOut<K> o = brt_eval_wide(cur, [&](int x) { return keys[x]; }, mbits, m, c);
     brt_flags(o, st);
     const uint32_t ls = brt_store<K, DIM>(c, cur, o, mbits, brt, parent, lsx);
     if (ls) atomicAdd(&chunk_tot[c >> 8], ls);
@@ -310,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void brt_seg_defer_kernel(const K *__restri
     const int m = (int)ss[s].m;
     const K *kk = keys + kofs;
     const K cur = kk[c];
-    const BrtOut<K> o = brt_eval(cur, [&](int x) { return kk[x]; }, mbits, m, c);
+    const BrtOut<K> o = brt_eval_wide(cur, [&](int x) { return kk[x]; }, mbits, m, c);
     brt_flags(o, st);
     (void)brt_store<K, DIM>(c, cur, o, mbits, brt + kofs, parent + kofs, lsx + kofs);
   }

@@ -110,6 +110,129 @@ __host__ __device__ inline BrtOut<K> brt_eval(K cur, Key key, int mbits, int m, 
   return o;
 }
 
+// Largest x in [lo, hi] with pred(x), for a predicate that holds on a prefix
+// of (lo, hi] (pred(lo) is taken as true): each round probes 7 evenly spaced
+// offsets of the candidates at once (their loads are independent), then keeps
+// the part after the last probe that holds.  ~log8(hi - lo) rounds.
+template <typename Pred>
+__host__ __device__ inline int kary_last_true(int lo, int hi, Pred pred) {
+  while (hi > lo) {
+    const int step = (hi - lo + 8) / 8;
+    // every probe loads (a probe past hi reads hi and is dropped), so that
+    // the seven loads are issued together rather than each behind a branch
+    bool p[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      const int x = lo + (j + 1) * step;
+      p[j] = pred(x < hi ? x : hi) & (x <= hi);
+    }
+    int best = lo;
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+      if (p[j]) best = lo + (j + 1) * step;
+    const int top = best + step - 1;
+    hi = top < hi ? top : hi;
+    lo = best;
+  }
+  return lo;
+}
+
+// kary_last_true(0, hi, pred) for a prefix that is short against hi (a
+// node's range against the whole key array): one round at 8, 64, ..., 8^7
+// brackets the answer by its power of 8, and the 8-ary search runs inside
+// the bracket.
+template <typename Pred>
+__host__ __device__ inline int gallop_last_true(int hi, Pred pred) {
+  bool p[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const int x = 8 << (3 * j);
+    p[j] = pred(x < hi ? x : hi) & (x <= hi);
+  }
+  int k = -1;
+#pragma unroll
+  for (int j = 0; j < 7; ++j)
+    if (p[j]) k = j;
+  const int lo = k >= 0 ? 8 << (3 * k) : 0;
+  const int top = k < 6 ? (8 << (3 * (k + 1))) - 1 : hi;   // below the first probe that fails
+  return kary_last_true(lo, top < hi ? top : hi, pred);
+}
+
+// The reference's split search (Karras' ceil-halving steps; the FP32 step
+// of fp32_split_step above 2^23) replayed on a predicate that holds exactly
+// on offsets [0, p]: no key is read.  *quirk: the FP32 search ends elsewhere
+// than the exact one.  brt_eval's searches, on a monotone predicate whose
+// last true offset is p, take the same steps.
+__host__ __device__ inline int split_replay(int l, int s_cutoff, int p, bool *quirk) {
+  int s = 0;
+  *quirk = false;
+  if (l <= (1 << 23)) {
+    int k = 1;
+    for (int den = 2; den < 2 * l; den <<= 1, ++k) {
+      const int t = (l + den - 1) >> k;
+      if (s + t <= s_cutoff && s + t <= p) s += t;
+    }
+  } else {
+    int se = 0;
+    for (long long den = 2; den < 2ll * l; den *= 2) {
+      const int t = fp32_split_step(l, (int)den);
+      const int te = (int)(((long long)l + den - 1) / den);
+      if (s + t <= s_cutoff && s + t <= p) s += t;
+      if (se + te <= s_cutoff && se + te <= p) se += te;
+    }
+    *quirk = se != s;
+  }
+  return s;
+}
+
+// brt_eval for the nodes whose ranges reach beyond the window kernels'
+// windows (the overflow pass over global memory, where every probe of
+// brt_eval's binary searches is a dependent load).  Both of its searches
+// ask a monotone predicate (a key shares more than lcp bits with the
+// node's key: true on a run of offsets from 0): the range end by a
+// gallop and 8-ary search (seven independent loads a round), and the
+// split by finding the predicate's last true offset the same way and
+// replaying the reference's step sequence on it (split_replay), the FP32
+// steps above 2^23 included.
+template <typename K, typename Key>
+__host__ __device__ inline BrtOut<K> brt_eval_wide(K cur, Key key, int mbits, int m, int i) {
+  BrtOut<K> o;
+  o.lcp_min = -1;
+  const int l_pos = delta(cur, key(i + 1), mbits);
+  o.dup = l_pos == mbits;
+  int dn = 0, l, limit;
+  if (i == 0) {
+    l = m - 1;
+    limit = m - 1;
+  } else {
+    const int l_neg = delta(cur, key(i - 1), mbits);
+    dn = l_pos >= l_neg ? 0 : -1;
+    const int lcp_min = dn ? l_pos : l_neg;
+    o.lcp_min = lcp_min;
+    limit = dn ? i : m - 1 - i;
+    const int sh = mbits - lcp_min - 1;
+    l = sh < 0 ? 0 : gallop_last_true(limit, [&](int off) {
+      return kzero(kshr(kxor(cur, key(i + ((off ^ dn) - dn))), sh));
+    });
+  }
+  const int j = i + ((l ^ dn) - dn);
+  const int lcp_node = delta(cur, key(j), mbits);
+  const int shn = mbits - lcp_node - 1;
+  // last offset in [0, l - 1] sharing more than lcp_node bits (0 when none
+  // but the node's own key: the replay then never steps)
+  const int p = (shn < 0 || l < 2) ? 0 : kary_last_true(0, l - 1, [&](int off) {
+    return kzero(kshr(kxor(cur, key(i + ((off ^ dn) - dn))), shn));
+  });
+  const int s = split_replay(l, limit - 1, p, &o.quirk);
+  const int split = i + ((s ^ dn) - dn) + dn;
+  o.nd.left = split;
+  o.nd.left_leaf = ((i < j ? i : j) == split);
+  o.nd.right_leaf = ((i > j ? i : j) == split + 1);
+  o.nd.lcp = kshr(cur, mbits - lcp_node);
+  o.nd.lcp_length = (uint16_t)lcp_node;
+  return o;
+}
+
 // ---- the window form: a sparse table of adjacent-key deltas ----------------
 // For sorted distinct keys, delta(keys[i], keys[x]) is the minimum of the
 // adjacent deltas D_k = delta(keys[k], keys[k+1]) for k between i and x, so

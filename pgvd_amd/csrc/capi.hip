@@ -50,6 +50,10 @@ struct PassTimer {
 };
 
 int launch_status(const char *what) {
+  if (!g_fail_launch.empty() && g_fail_launch == what) {
+    g_fail_launch.clear();
+    return fail(PGVD_ERR_HIP, std::string(what) + ": injected launch failure");
+  }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(PGVD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
   return PGVD_OK;
@@ -700,6 +704,23 @@ int pgvd_ctx_stream(pgvd_ctx *ctx, pgvd_stream *out) {
   return PGVD_OK;
 }
 
+int pgvd_ctx_set_pool_limit(pgvd_ctx *ctx, size_t bytes) {
+  if (!ctx) return fail(PGVD_ERR_INVALID, "null context");
+  ctx->pool_limit = bytes;
+  return PGVD_OK;
+}
+
+int pgvd_ctx_pool_bytes(pgvd_ctx *ctx, size_t *out) {
+  if (!ctx || !out) return fail(PGVD_ERR_INVALID, "null argument");
+  *out = ctx->pool_bytes;
+  return PGVD_OK;
+}
+
+int pgvd_debug_fail_launch(const char *stage) {
+  g_fail_launch = stage ? stage : "";
+  return PGVD_OK;
+}
+
 int pgvd_synchronize(pgvd_ctx *ctx, pgvd_stream s) {
   if (!ctx) return fail(PGVD_ERR_INVALID, "null context");
   CHK(use_device(ctx));
@@ -710,7 +731,10 @@ int pgvd_synchronize(pgvd_ctx *ctx, pgvd_stream s) {
 int pgvd_alloc(pgvd_ctx *ctx, size_t bytes, void **dptr) {
   if (!ctx || !dptr) return fail(PGVD_ERR_INVALID, "null argument");
   CHK(use_device(ctx));
-  if (hipMalloc(dptr, bytes ? bytes : 1) != hipSuccess) return fail(PGVD_ERR_ALLOC, "hipMalloc failed");
+  if (hipMalloc(dptr, bytes ? bytes : 1) != hipSuccess) {
+    (void)hipGetLastError();   // not left for the next launch check to report
+    return fail(PGVD_ERR_ALLOC, "hipMalloc failed");
+  }
   return PGVD_OK;
 }
 

@@ -16,6 +16,9 @@ namespace pgvd_capi {
 using pgvd::RunState;
 
 inline thread_local std::string g_err;
+// Stage whose next launch check on this thread reports a failure
+// (pgvd_debug_fail_launch); empty: none.
+inline thread_local std::string g_fail_launch;
 
 inline int fail(int code, const std::string &msg) {
   g_err = msg;
@@ -100,6 +103,8 @@ struct pgvd_ctx {
   int64_t octree_cap = 0;       // nodes the pooled octree buffer holds
   int octree_stride = 0;        // ints per node of that buffer
   bool own_stream_only = false; // every call on the context uses its stream
+  size_t pool_bytes = 0;        // device bytes the pool holds
+  size_t pool_limit = 0;        // most it may hold (0: no limit)
 };
 
 namespace pgvd_capi {
@@ -126,15 +131,22 @@ inline int pool_get(pgvd_ctx *ctx, const char *key, size_t bytes, void **out) {
       if (ctx->own_stream_only) HIPCHK(hipStreamSynchronize(ctx->stream));
       else HIPCHK(hipDeviceSynchronize());
       HIPCHK(hipFree(b.p));
+      ctx->pool_bytes -= b.bytes;
       b.p = nullptr;
       b.bytes = 0;
     }
     const size_t want = align256(bytes + bytes / 8);
+    if (ctx->pool_limit && ctx->pool_bytes + want > ctx->pool_limit)
+      return fail(PGVD_ERR_ALLOC, std::string("pool buffer ") + key + " of " +
+                                      std::to_string(want) + " bytes would pass the pool limit of " +
+                                      std::to_string(ctx->pool_limit) + " bytes");
     if (hipMalloc(&b.p, want) != hipSuccess) {
+      (void)hipGetLastError();   // not left for the next launch check to report
       b.p = nullptr;
       return fail(PGVD_ERR_ALLOC, std::string("hipMalloc failed for pool buffer ") + key);
     }
     b.bytes = want;
+    ctx->pool_bytes += want;
   }
   *out = b.p;
   return PGVD_OK;

@@ -147,11 +147,15 @@ constexpr uint32_t kPlLsx = (1u << kPlShift) - 1;
 // is either the top of a child chain (local split > 0: recorded in node0 at
 // its octant) or a same-level node, whose two children become slots of the
 // next round.  All LDS reads of a round are independent.
-template <typename K, int DIM, int W, bool LAST>
+// fetch(x, internal, need_links, ps, lx, lnk) reads BRT node x's prefix sum,
+// packed local split (local split | attach octant << 8) and, when
+// need_links, its packed child links; it returns false when x is
+// unavailable (outside a window), and reads nothing that matters unless
+// internal.
+template <typename K, int DIM, int W, bool LAST, typename Fetch>
 __device__ __forceinline__ void bfs_round(const uint32_t (&idx)[W], const bool (&lf)[W],
                                  const bool (&live)[W], uint32_t (&nidx)[2 * W],
-                                 bool (&nlf)[2 * W], bool (&nlive)[2 * W], int w0, int wn,
-                                 const uint32_t *s_lnk, const uint32_t *s_pl, uint32_t pbase,
+                                 bool (&nlf)[2 * W], bool (&nlive)[2 * W], Fetch fetch,
                                  const K *__restrict__ keys,
                                  const uint32_t *__restrict__ conflict_bits, int lvl, int mbits,
                                  int32_t *node0, uint32_t &found, uint32_t &cmask, bool &miss,
@@ -159,19 +163,16 @@ __device__ __forceinline__ void bfs_round(const uint32_t (&idx)[W], const bool (
 #pragma unroll
   for (int s = 0; s < W; ++s) {
     const bool internal = live[s] && !lf[s];
-    const unsigned k = (unsigned)((int)idx[s] - w0);
-    const bool inwin = k < (unsigned)wn;
+    uint32_t psx, lx, v = 0;
+    const bool inwin = fetch(idx[s], internal, !LAST, psx, lx, v);
     miss |= internal && !inwin;
     const bool use = internal && inwin;
-    const unsigned kk = use ? k : 0u;
-    const uint32_t pl = s_pl[kk];   // prefix sum - pbase, packed local split
-    const uint32_t lx = pl & kPlLsx;
     const bool term = use && (lx & 0xffu) != 0;
     const bool expand = use && (lx & 0xffu) == 0;
     // the child chain's top, in its octant; otherwise a dummy store to the
     // leaf word, which is written last
     const uint32_t o = term ? lx >> 8 : (uint32_t)(1 << DIM);
-    node0[o] = (int32_t)(pbase + (pl >> kPlShift) - 1);
+    node0[o] = (int32_t)(psx - 1);
     found |= term ? 1u << o : 0u;
     if (live[s] && lf[s] && conflict_bits && test_bit(conflict_bits, idx[s]))
       cmask |= 1u << digit_at<K, DIM>(keys[idx[s]], lvl, mbits);
@@ -180,7 +181,6 @@ __device__ __forceinline__ void bfs_round(const uint32_t (&idx)[W], const bool (
       // the BRT stage did not flag is reported
       if (expand) atomicOr(&st->status, (uint32_t)ST_STACK);
     } else {
-      const uint32_t v = s_lnk[kk];
       nidx[2 * s] = v & 0x3fffffffu;
       nidx[2 * s + 1] = (v & 0x3fffffffu) + 1;
       nlf[2 * s] = (v >> 30) & 1u;
@@ -191,21 +191,22 @@ __device__ __forceinline__ void bfs_round(const uint32_t (&idx)[W], const bool (
   }
 }
 
-// gather_one for the window pass with its output staged in LDS: q's nodes
-// node0[0 .. lsq STRIDE) are written whole here (children -1 unless found,
-// leaf masks).  The
+// gather_one in breadth-first form: q's nodes node0[0 .. lsq STRIDE) are
+// written whole here (children -1 unless found, leaf masks).  The
 // same-level BRT nodes below q split on later bits of the digit q splits,
 // so they form a binary trie of fewer than DIM levels: DIM rounds over
-// fixed slots (2, 4, 8) find every child chain without a loop or stack.
-// node0 is q's lowest staged node; lk is q's packed child links.  q's record
-// is read from brt only for a chain (lsq > 1) or the conflict octants.
-// Returns false when an internal node of that trie lies outside the window;
-// the global-memory pass then rewrites q's nodes whole.
-template <typename K, int DIM>
+// fixed slots (2, 4, 8) find every child chain without a loop or stack,
+// and every read of a round is independent of the others.  node0 is q's
+// lowest node (staged in LDS by the window pass, in the octree for the
+// global-memory pass); lk is q's packed child links; fetch reads the
+// entries of the trie's nodes (bfs_round).  q's record is read from brt
+// only for a chain (lsq > 1) or the conflict octants.  Returns false when
+// fetch could not read an internal node of the trie; the global-memory
+// pass then rewrites q's nodes whole.
+template <typename K, int DIM, typename Fetch>
 __device__ __forceinline__ bool gather_one_bfs(uint32_t q, uint32_t lk,
                                       const BrtNodeT<K> *__restrict__ brt, uint32_t excl,
-                                      uint32_t lsq, int w0, int wn, const uint32_t *s_lnk,
-                                      const uint32_t *s_pl, uint32_t pbase,
+                                      uint32_t lsq, Fetch fetch,
                                       const K *__restrict__ keys,
                                       const uint32_t *__restrict__ conflict_bits, int mbits,
                                       int32_t *node0, uint32_t *__restrict__ node_conflict,
@@ -224,17 +225,17 @@ __device__ __forceinline__ bool gather_one_bfs(uint32_t q, uint32_t lk,
   const bool l1[2] = {true, true};
   uint32_t i2[4];
   bool f2[4], l2[4];
-  bfs_round<K, DIM, 2, DIM == 1>(i1, f1, l1, i2, f2, l2, w0, wn, s_lnk, s_pl, pbase, keys,
-                                 conflict_bits, lvl, mbits, node0, found, cmask, miss, st);
+  bfs_round<K, DIM, 2, DIM == 1>(i1, f1, l1, i2, f2, l2, fetch, keys, conflict_bits, lvl,
+                                 mbits, node0, found, cmask, miss, st);
   uint32_t i3[8];
   bool f3[8], l3[8];
-  bfs_round<K, DIM, 4, DIM == 2>(i2, f2, l2, i3, f3, l3, w0, wn, s_lnk, s_pl, pbase, keys,
-                                 conflict_bits, lvl, mbits, node0, found, cmask, miss, st);
+  bfs_round<K, DIM, 4, DIM == 2>(i2, f2, l2, i3, f3, l3, fetch, keys, conflict_bits, lvl,
+                                 mbits, node0, found, cmask, miss, st);
   if (DIM == 3) {
     uint32_t i4[16];
     bool f4[16], l4[16];
-    bfs_round<K, DIM, 8, true>(i3, f3, l3, i4, f4, l4, w0, wn, s_lnk, s_pl, pbase, keys,
-                               conflict_bits, lvl, mbits, node0, found, cmask, miss, st);
+    bfs_round<K, DIM, 8, true>(i3, f3, l3, i4, f4, l4, fetch, keys, conflict_bits, lvl,
+                               mbits, node0, found, cmask, miss, st);
   }
   if (miss) return false;
   node0[NC] = kFull & ~(int32_t)found;
@@ -392,8 +393,19 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
     const uint32_t excl = base + excl_r;
     bool ok;
     if (staged) {
-      ok = gather_one_bfs<K, DIM>(q, s_lnk[q - w0], brt, excl, lsq, w0, WN, s_lnk, s_pl, base,
-                                  keys, conflict_bits, mbits,
+      auto fetch = [&](uint32_t x, bool internal, bool need_links, uint32_t &psx, uint32_t &lx,
+                       uint32_t &lnk) -> bool {
+        const unsigned k = (unsigned)((int)x - w0);
+        const bool inwin = k < (unsigned)WN;
+        const unsigned kk = internal && inwin ? k : 0u;
+        const uint32_t pl = s_pl[kk];   // prefix sum - base, packed local split
+        psx = base + (pl >> kPlShift);
+        lx = pl & kPlLsx;
+        if (need_links) lnk = s_lnk[kk];
+        return inwin;
+      };
+      ok = gather_one_bfs<K, DIM>(q, s_lnk[q - w0], brt, excl, lsq, fetch, keys, conflict_bits,
+                                  mbits,
                                   s_nodes + a0 + (excl_r - lo) * STRIDE,
                                   node_conflict, st);
     } else {
@@ -517,7 +529,8 @@ __global__ __launch_bounds__(kBlock) void gather_seg_kernel(
 }
 
 // Gather, second pass: the BRT nodes marked in miss_words, reading global
-// memory.
+// memory, breadth first (a round's loads are independent, so a node waits
+// on DIM + 1 dependent loads, not one per trie node).
 template <typename K, int DIM>
 __global__ __launch_bounds__(kBlock) void gather_defer_kernel(
     const BrtNodeT<K> *__restrict__ brt, const uint32_t *__restrict__ ps,
@@ -525,26 +538,27 @@ __global__ __launch_bounds__(kBlock) void gather_defer_kernel(
     const uint32_t *__restrict__ conflict_bits, int mbits, int32_t *__restrict__ octree,
     uint32_t *__restrict__ node_conflict, int64_t capacity,
     const uint64_t *__restrict__ miss_words, RunState *st) {
+  constexpr int STRIDE = (1 << DIM) + 1;
   if (st->malformed || (int64_t)st->octree_size > capacity) return;
   const uint32_t m = st->m;
   const uint32_t nwords = m > 1 ? (m - 1 + 63) / 64u : 0u;
-  auto links = [&](uint32_t x, uint32_t &left, bool &ll, bool &rl) -> bool {
-    const BrtNodeT<K> &b = brt[x];
-    left = (uint32_t)b.left;
-    ll = b.left_leaf;
-    rl = b.right_leaf;
+  // x of an internal slot is a BRT node (< m - 1); other slots read nothing
+  auto fetch = [&](uint32_t x, bool internal, bool need_links, uint32_t &psx, uint32_t &lx,
+                   uint32_t &lnk) -> bool {
+    psx = 0;
+    lx = 0;
+    if (internal) {
+      psx = ps[x];
+      lx = lsx[x];
+      if (need_links) lnk = load_links(brt + x);
+    }
     return true;
   };
-  auto child = [&](uint32_t c, uint32_t &psc, uint32_t &lxc) -> bool {
-    psc = ps[c];
-    lxc = lsx[c];
-    return true;
-  };
-  auto put = [&](uint32_t idx, const int32_t *c) { write_node<DIM>(octree, idx, c); };
   for_each_masked(miss_words, nwords, [&](uint32_t q) {
     const uint32_t excl = q ? ps[q - 1] : 0u;
-    (void)gather_one<K, DIM>(q, brt[q], excl, ps[q] - excl, links, child, keys, conflict_bits,
-                             mbits, put, node_conflict, st);
+    (void)gather_one_bfs<K, DIM>(q, load_links(brt + q), brt, excl, ps[q] - excl, fetch, keys,
+                                 conflict_bits, mbits, octree + (size_t)excl * STRIDE,
+                                 node_conflict, st);
   });
 }
 
@@ -566,21 +580,21 @@ __global__ __launch_bounds__(kBlock) void gather_seg_defer_kernel(
     const uint32_t *p = ps + kofs;
     const uint32_t *lx = lsx + kofs;
     int32_t *oct = octree + (size_t)ss[s].oct_off * STRIDE;
-    auto links = [&](uint32_t x, uint32_t &left, bool &ll, bool &rl) -> bool {
-      left = (uint32_t)b[x].left;
-      ll = b[x].left_leaf;
-      rl = b[x].right_leaf;
+    auto fetch = [&](uint32_t x, bool internal, bool need_links, uint32_t &psx, uint32_t &lxx,
+                     uint32_t &lnk) -> bool {
+      psx = 0;
+      lxx = 0;
+      if (internal) {
+        psx = p[x];
+        lxx = lx[x];
+        if (need_links) lnk = load_links(b + x);
+      }
       return true;
     };
-    auto child = [&](uint32_t c, uint32_t &psc, uint32_t &lxc) -> bool {
-      psc = p[c];
-      lxc = lx[c];
-      return true;
-    };
-    auto put = [&](uint32_t idx, const int32_t *c) { write_node<DIM>(oct, idx, c); };
     const uint32_t excl = q ? p[q - 1] : 0u;
-    (void)gather_one<K, DIM>(q, b[q], excl, p[q] - excl, links, child, (const K *)nullptr,
-                             nullptr, 0, put, nullptr, st);
+    (void)gather_one_bfs<K, DIM>(q, load_links(b + q), b, excl, p[q] - excl, fetch,
+                                 (const K *)nullptr, nullptr, 0, oct + (size_t)excl * STRIDE,
+                                 nullptr, st);
   }
 }
 

@@ -81,11 +81,7 @@ size_t result_class(size_t bytes) {
   return (c + q - 1) / q * q;
 }
 
-size_t pool_bytes(const pgvd_ctx *ctx) {
-  size_t b = 0;
-  for (const auto &kv : ctx->pool) b += kv.second.bytes;
-  return b;
-}
+size_t pool_bytes(const pgvd_ctx *ctx) { return ctx->pool_bytes; }
 
 }  // namespace
 
@@ -140,6 +136,7 @@ int release_slot_memory(Slot &sl) {
   for (auto &kv : ctx->pool)
     if (kv.second.p) HIPCHK(hipFree(kv.second.p));
   ctx->pool.clear();
+  ctx->pool_bytes = 0;
   ctx->octree_cap = 0;
   ctx->octree_stride = 0;
   ctx->last.valid = false;
@@ -217,6 +214,7 @@ int collect(pgvd_server *srv, Slot &sl, Ticket &r) {
       srv->spare_bytes -= r.result_bytes;
     } else {
       if (hipMalloc(&r.result, cls) != hipSuccess) {
+        (void)hipGetLastError();
         r.result = nullptr;
         return fail(PGVD_ERR_ALLOC, "no device memory for a served result");
       }

@@ -108,6 +108,14 @@ int check(const char *name, int mbits, int64_t want_m, uint64_t seed, bool clust
     const K cur = keys[i];
     full[i] = brt_eval(cur, [&](int x) { return keys[x]; }, mbits, (int)m, (int)i);
     const BrtOut<K> &o = full[i];
+    const BrtOut<K> w = brt_eval_wide(cur, [&](int x) { return keys[x]; }, mbits, (int)m, (int)i);
+    if ((w.nd.left != o.nd.left || w.nd.left_leaf != o.nd.left_leaf ||
+         w.nd.right_leaf != o.nd.right_leaf || w.nd.lcp_length != o.nd.lcp_length ||
+         lo_of(w.nd.lcp) != lo_of(o.nd.lcp) || hi_of(w.nd.lcp) != hi_of(o.nd.lcp) ||
+         w.lcp_min != o.lcp_min || w.dup != o.dup || w.quirk != o.quirk) &&
+        bad++ < 5)
+      std::printf("%s: k-ary node %lld differs: left %d/%d\n", name, (long long)i, w.nd.left,
+                  o.nd.left);
     const bool ok = o.nd.left == left[i] && o.nd.left_leaf == ll[i] && o.nd.right_leaf == rl[i] &&
                     o.nd.lcp_length == lcp_len[i] && lo_of(o.nd.lcp) == clo[i] &&
                     hi_of(o.nd.lcp) == chi[i] && !o.dup && !o.quirk;
@@ -196,8 +204,170 @@ int check(const char *name, int mbits, int64_t want_m, uint64_t seed, bool clust
 
 }  // namespace
 
-int main() {
+// The searches both node forms rely on, over every prefix predicate of
+// small ranges: kary_last_true's last true offset, and the Karras split
+// search's exact ceil steps (brt_eval for l <= 2^23), which must find the
+// same offset for a predicate that holds on [0, p] and fails at l.
+int check_searches() {
   int bad = 0;
+  for (int hi = 0; hi <= 1200; ++hi)
+    for (int p = 0; p <= hi; ++p) {
+      const int got = kary_last_true(0, hi, [&](int x) { return x <= p; });
+      if (got != p && bad++ < 5) std::printf("kary_last_true(0, %d) = %d, want %d\n", hi, got, p);
+    }
+  {
+    std::vector<int> his;
+    for (int hi = 0; hi <= 1200; ++hi) his.push_back(hi);
+    for (int e = 11; e <= 25; ++e) {
+      his.push_back((1 << e) - 1);
+      his.push_back(1 << e);
+      his.push_back((1 << e) + 12345);
+    }
+    uint64_t x = 12345;
+    for (int hi : his) {
+      std::vector<int> ps;
+      for (int p = 0; p <= hi && p <= 600; ++p) ps.push_back(p);
+      for (int e = 0; e < 31 && (1 << e) <= hi; ++e) {
+        ps.push_back((1 << e) - 1);
+        ps.push_back(1 << e);
+      }
+      for (int r = 0; r < 64 && hi > 0; ++r) {
+        x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+        ps.push_back((int)(x % (uint64_t)(hi + 1)));
+      }
+      ps.push_back(hi);
+      for (int p : ps) {
+        if (p > hi) continue;
+        const int got = gallop_last_true(hi, [&](int y) { return y <= p; });
+        if (got != p && bad++ < 5)
+          std::printf("gallop_last_true(%d) = %d, want %d\n", hi, got, p);
+      }
+    }
+  }
+  for (int l = 1; l <= 1200; ++l)
+    for (int p = 0; p < l; ++p) {
+      int s = 0, k = 1;
+      for (int den = 2; den < 2 * l; den <<= 1, ++k) {
+        const int t = (l + den - 1) >> k;
+        if (s + t <= p) s += t;
+      }
+      if (s != p && bad++ < 5) std::printf("ceil split search l=%d gives %d, want %d\n", l, s, p);
+    }
+  // equal neighbouring keys: the k-ary form against brt_eval itself
+  {
+    std::vector<uint64_t> keys;
+    uint64_t x = 0x9e3779b97f4a7c15ull, v = 0;
+    for (int i = 0; i < 20000; ++i) {
+      x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+      v += (x % 5 == 0) ? 0 : (x >> 40);     // runs of equal keys
+      keys.push_back(v & ((1ull << 63) - 1));
+    }
+    std::sort(keys.begin(), keys.end());
+    const int m = (int)keys.size();
+    for (int i = 0; i + 1 < m; ++i) {
+      auto key = [&](int y) { return keys[y]; };
+      const BrtOut<uint64_t> a = brt_eval(keys[i], key, 63, m, i);
+      const BrtOut<uint64_t> b = brt_eval_wide(keys[i], key, 63, m, i);
+      if ((a.nd.left != b.nd.left || a.nd.lcp_length != b.nd.lcp_length ||
+           a.nd.left_leaf != b.nd.left_leaf || a.nd.right_leaf != b.nd.right_leaf ||
+           a.dup != b.dup || a.lcp_min != b.lcp_min) && bad++ < 5)
+        std::printf("equal keys: node %d differs: left %d/%d\n", i, b.nd.left, a.nd.left);
+    }
+  }
+  // a root range above 2^23: the FP32 split steps, replayed by the k-ary
+  // form, against brt_eval's own search (random keys; the root and a
+  // sample of the other nodes)
+  {
+    const int m = (1 << 23) + 4097;
+    std::vector<uint64_t> keys(m);
+    uint64_t x = 0x2545f4914f6cdd1dull;
+    for (int i = 0; i < m; ++i) {
+      x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+      keys[i] = x >> 1;
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const int mm = (int)keys.size();
+    auto key = [&](int y) { return keys[y]; };
+    int quirks = 0;
+    for (int i = 0; i + 1 < mm; i = i < 4096 ? i + 1 : i + 257) {
+      const BrtOut<uint64_t> a = brt_eval(keys[i], key, 63, mm, i);
+      const BrtOut<uint64_t> b = brt_eval_wide(keys[i], key, 63, mm, i);
+      quirks += a.quirk;
+      if ((a.nd.left != b.nd.left || a.nd.lcp_length != b.nd.lcp_length ||
+           a.nd.left_leaf != b.nd.left_leaf || a.nd.right_leaf != b.nd.right_leaf ||
+           a.quirk != b.quirk || a.lcp_min != b.lcp_min) && bad++ < 5)
+        std::printf("m=%d: node %d differs: left %d/%d quirk %d/%d\n", mm, i, b.nd.left,
+                    a.nd.left, b.quirk, a.quirk);
+    }
+    std::printf("m=%d: %d quirk splits sampled\n", mm, quirks);
+  }
+  // a root whose FP32 split differs from the exact one (l = 2^24 - 1,
+  // where l + den - 1 rounds in FP32): the first p + 1 keys below 2^62,
+  // the rest above, for a p where the steps diverge
+  {
+    const int mm = 1 << 24, l = mm - 1;
+    int pq = -1;
+    for (int p = l / 2; p < l && pq < 0; ++p) {
+      bool q;
+      (void)split_replay(l, l - 1, p, &q);
+      if (q) pq = p;
+    }
+    if (pq < 0) {
+      std::printf("no diverging split for l=%d\n", l);
+      ++bad;
+    } else {
+      std::vector<uint64_t> keys(mm);
+      for (int i = 0; i < mm; ++i) keys[i] = (i <= pq ? 0ull : 1ull << 62) | (uint64_t)i;
+      auto key = [&](int y) { return keys[y]; };
+      const BrtOut<uint64_t> a = brt_eval(keys[0], key, 63, mm, 0);
+      const BrtOut<uint64_t> b = brt_eval_wide(keys[0], key, 63, mm, 0);
+      if (!a.quirk || a.nd.left != b.nd.left || a.quirk != b.quirk ||
+          a.nd.lcp_length != b.nd.lcp_length) {
+        std::printf("quirk root (p=%d): left %d/%d quirk %d/%d\n", pq, b.nd.left, a.nd.left,
+                    (int)b.quirk, (int)a.quirk);
+        ++bad;
+      }
+    }
+  }
+  // the replayed split search against the probing one, over every last-true
+  // offset p of ranges l near and above 2^23 (the FP32 steps)
+  {
+    uint64_t x = 99;
+    for (int r = 0; r < 4000; ++r) {
+      x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+      const int l = (1 << 23) - 64 + (int)(x % (uint64_t)(3 << 23));
+      const int p = (int)((x >> 32) % (uint64_t)l);
+      const int cutoff = l + (int)((x >> 16) & 7) - 1;
+      int s = 0, se = 0;
+      if (l <= (1 << 23)) {
+        int k = 1;
+        for (int den = 2; den < 2 * l; den <<= 1, ++k) {
+          const int t = (l + den - 1) >> k;
+          if (s + t <= cutoff && s + t <= p) s += t;
+        }
+        se = s;
+      } else {
+        for (long long den = 2; den < 2ll * l; den *= 2) {
+          const int t = fp32_split_step(l, (int)den);
+          const int te = (int)(((long long)l + den - 1) / den);
+          if (s + t <= cutoff && s + t <= p) s += t;
+          if (se + te <= cutoff && se + te <= p) se += te;
+        }
+      }
+      bool q;
+      const int got = split_replay(l, cutoff, p, &q);
+      if ((got != s || q != (se != s) || se != p) && bad++ < 5)
+        std::printf("split_replay(l=%d, p=%d) = %d quirk %d; probing search %d exact %d\n", l,
+                    p, got, (int)q, s, se);
+    }
+  }
+  std::printf("searches: %s\n", bad ? "FAIL" : "ok");
+  return bad;
+}
+
+int main() {
+  int bad = check_searches();
   for (bool cl : {false, true}) {
     const char *tag = cl ? "clustered" : "uniform";
     char name[64];
