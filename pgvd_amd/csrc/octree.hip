@@ -142,29 +142,33 @@ __device__ __forceinline__ bool gather_one(uint32_t q, const BrtNodeT<K> &nq, ui
 constexpr int kPlShift = 11;
 constexpr uint32_t kPlLsx = (1u << kPlShift) - 1;
 
-// One breadth-first round of gather_one_bfs over W slots (BRT child
-// indices, leaf flags, live flags): a live internal slot inside the window
-// is either the top of a child chain (local split > 0: recorded in node0 at
-// its octant) or a same-level node, whose two children become slots of the
-// next round.  All LDS reads of a round are independent.
-// fetch(x, internal, need_links, ps, lx, lnk) reads BRT node x's prefix sum,
-// packed local split (local split | attach octant << 8) and, when
-// need_links, its packed child links; it returns false when x is
+// A slot of gather_one_bfs's rounds in one word: BRT index (30 bits), then
+// its leaf flag (bit 30) and live flag (bit 31) -- so that a round's state
+// stays in vector registers rather than in lane masks.
+constexpr uint32_t kSlotIdx = 0x3fffffffu, kSlotLeaf = 1u << 30, kSlotLive = 1u << 31;
+
+// One breadth-first round of gather_one_bfs over W slots: a live internal
+// slot inside the window is either the top of a child chain (local split >
+// 0: recorded in node0 at its octant) or a same-level node, whose two
+// children become slots of the next round.  All reads of a round are
+// independent.  fetch(x, internal, need_links, ps, lx, lnk) reads BRT node
+// x's prefix sum, packed local split (local split | attach octant << 8)
+// and, when need_links, its packed child links; it returns false when x is
 // unavailable (outside a window), and reads nothing that matters unless
-// internal.
-template <typename K, int DIM, int W, bool LAST, typename Fetch>
-__device__ __forceinline__ void bfs_round(const uint32_t (&idx)[W], const bool (&lf)[W],
-                                 const bool (&live)[W], uint32_t (&nidx)[2 * W],
-                                 bool (&nlf)[2 * W], bool (&nlive)[2 * W], Fetch fetch,
-                                 const K *__restrict__ keys,
-                                 const uint32_t *__restrict__ conflict_bits, int lvl, int mbits,
-                                 int32_t *node0, uint32_t &found, uint32_t &cmask, bool &miss,
-                                 RunState *st) {
+// internal.  CONF: collect the octants of leaf children whose keys carry a
+// conflict flag.
+template <typename K, int DIM, int W, bool LAST, bool CONF, typename Fetch>
+__device__ __forceinline__ void bfs_round(const uint32_t (&sl)[W], uint32_t (&nsl)[2 * W],
+                                          Fetch fetch, const K *__restrict__ keys,
+                                          const uint32_t *__restrict__ conflict_bits, int lvl,
+                                          int mbits, int32_t *node0, uint32_t &cmask,
+                                          bool &miss, RunState *st) {
 #pragma unroll
   for (int s = 0; s < W; ++s) {
-    const bool internal = live[s] && !lf[s];
+    const uint32_t x = sl[s] & kSlotIdx;
+    const bool internal = (sl[s] & (kSlotLive | kSlotLeaf)) == kSlotLive;
     uint32_t psx, lx, v = 0;
-    const bool inwin = fetch(idx[s], internal, !LAST, psx, lx, v);
+    const bool inwin = fetch(x, internal, !LAST, psx, lx, v);
     miss |= internal && !inwin;
     const bool use = internal && inwin;
     const bool term = use && (lx & 0xffu) != 0;
@@ -173,20 +177,19 @@ __device__ __forceinline__ void bfs_round(const uint32_t (&idx)[W], const bool (
     // leaf word, which is written last
     const uint32_t o = term ? lx >> 8 : (uint32_t)(1 << DIM);
     node0[o] = (int32_t)(psx - 1);
-    found |= term ? 1u << o : 0u;
-    if (live[s] && lf[s] && conflict_bits && test_bit(conflict_bits, idx[s]))
-      cmask |= 1u << digit_at<K, DIM>(keys[idx[s]], lvl, mbits);
+    if (CONF && (sl[s] & (kSlotLive | kSlotLeaf)) == (kSlotLive | kSlotLeaf) &&
+        test_bit(conflict_bits, x))
+      cmask |= 1u << digit_at<K, DIM>(keys[x], lvl, mbits);
     if (LAST) {
       // a well-formed tree has no same-level node this deep; a malformed one
       // the BRT stage did not flag is reported
       if (expand) atomicOr(&st->status, (uint32_t)ST_STACK);
     } else {
-      nidx[2 * s] = v & 0x3fffffffu;
-      nidx[2 * s + 1] = (v & 0x3fffffffu) + 1;
-      nlf[2 * s] = (v >> 30) & 1u;
-      nlf[2 * s + 1] = v >> 31;
-      nlive[2 * s] = expand;
-      nlive[2 * s + 1] = expand;
+      // v: left | left leaf << 30 | right leaf << 31
+      const uint32_t a = (v & (kSlotIdx | kSlotLeaf)) | kSlotLive;
+      const uint32_t b = ((v & kSlotIdx) + 1) | ((v >> 1) & kSlotLeaf) | kSlotLive;
+      nsl[2 * s] = expand ? a : 0u;
+      nsl[2 * s + 1] = expand ? b : 0u;
     }
   }
 }
@@ -198,12 +201,14 @@ __device__ __forceinline__ void bfs_round(const uint32_t (&idx)[W], const bool (
 // fixed slots (2, 4, 8) find every child chain without a loop or stack,
 // and every read of a round is independent of the others.  node0 is q's
 // lowest node (staged in LDS by the window pass, in the octree for the
-// global-memory pass); lk is q's packed child links; fetch reads the
-// entries of the trie's nodes (bfs_round).  q's record is read from brt
-// only for a chain (lsq > 1) or the conflict octants.  Returns false when
-// fetch could not read an internal node of the trie; the global-memory
-// pass then rewrites q's nodes whole.
-template <typename K, int DIM, typename Fetch>
+// global-memory pass); its leaf mask is read back from the children it
+// holds.  lk is q's packed child links; fetch reads the entries of the
+// trie's nodes (bfs_round).  q's record is read from brt only for a chain
+// (lsq > 1) or the conflict octants (CONF: conflict_bits, keys and
+// node_conflict are given).  Returns false when fetch could not read an
+// internal node of the trie; the global-memory pass then rewrites q's
+// nodes whole.
+template <typename K, int DIM, bool CONF, typename Fetch>
 __device__ __forceinline__ bool gather_one_bfs(uint32_t q, uint32_t lk,
                                       const BrtNodeT<K> *__restrict__ brt, uint32_t excl,
                                       uint32_t lsq, Fetch fetch,
@@ -214,32 +219,30 @@ __device__ __forceinline__ bool gather_one_bfs(uint32_t q, uint32_t lk,
   constexpr int NC = 1 << DIM;
   constexpr int STRIDE = NC + 1;
   constexpr int32_t kFull = (1 << NC) - 1;
-  const int lvl = conflict_bits ? brt[q].lcp_length / DIM : 0;
-  uint32_t found = 0, cmask = 0;
+  const int lvl = CONF ? brt[q].lcp_length / DIM : 0;
+  uint32_t cmask = 0;
   bool miss = false;
 #pragma unroll
   for (int c = 0; c < NC; ++c) node0[c] = -1;
-  const uint32_t left = lk & 0x3fffffffu;
-  const uint32_t i1[2] = {left, left + 1};
-  const bool f1[2] = {((lk >> 30) & 1u) != 0u, (lk >> 31) != 0u};
-  const bool l1[2] = {true, true};
-  uint32_t i2[4];
-  bool f2[4], l2[4];
-  bfs_round<K, DIM, 2, DIM == 1>(i1, f1, l1, i2, f2, l2, fetch, keys, conflict_bits, lvl,
-                                 mbits, node0, found, cmask, miss, st);
-  uint32_t i3[8];
-  bool f3[8], l3[8];
-  bfs_round<K, DIM, 4, DIM == 2>(i2, f2, l2, i3, f3, l3, fetch, keys, conflict_bits, lvl,
-                                 mbits, node0, found, cmask, miss, st);
+  const uint32_t s1[2] = {(lk & (kSlotIdx | kSlotLeaf)) | kSlotLive,
+                          ((lk & kSlotIdx) + 1) | ((lk >> 1) & kSlotLeaf) | kSlotLive};
+  uint32_t s2[4];
+  bfs_round<K, DIM, 2, DIM == 1, CONF>(s1, s2, fetch, keys, conflict_bits, lvl, mbits, node0,
+                                       cmask, miss, st);
+  uint32_t s3[8];
+  bfs_round<K, DIM, 4, DIM == 2, CONF>(s2, s3, fetch, keys, conflict_bits, lvl, mbits, node0,
+                                       cmask, miss, st);
   if (DIM == 3) {
-    uint32_t i4[16];
-    bool f4[16], l4[16];
-    bfs_round<K, DIM, 8, true>(i3, f3, l3, i4, f4, l4, fetch, keys, conflict_bits, lvl,
-                               mbits, node0, found, cmask, miss, st);
+    uint32_t s4[16];
+    bfs_round<K, DIM, 8, true, CONF>(s3, s4, fetch, keys, conflict_bits, lvl, mbits, node0,
+                                     cmask, miss, st);
   }
   if (miss) return false;
-  node0[NC] = kFull & ~(int32_t)found;
-  if (node_conflict) node_conflict[excl] = cmask;
+  int32_t leaf = 0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) leaf |= (node0[c] == -1) << c;
+  node0[NC] = leaf;
+  if (CONF) node_conflict[excl] = cmask;
   if (lsq > 1) {
     const BrtNodeT<K> nq = brt[q];
     for (uint32_t k = 1; k < lsq; ++k) {
@@ -252,7 +255,7 @@ __device__ __forceinline__ bool gather_one_bfs(uint32_t q, uint32_t lk,
         nd[o] = (int32_t)(excl + k - 1);
         nd[NC] = kFull & ~(1 << o);
       }
-      if (node_conflict) node_conflict[excl + k] = 0;
+      if (CONF) node_conflict[excl + k] = 0;
     }
   }
   return true;
@@ -286,7 +289,7 @@ constexpr int kGatherStagePct = PGVD_GATHER_STAGE_PCT;
 // 256 nodes), the window's prefix sums are computed here from lsx instead
 // of read from ps, and the workgroup writes them to ps_out for its own
 // nodes (and for node m-1 when that starts the next chunk).
-template <typename K, int DIM, int R, typename Defer>
+template <typename K, int DIM, int R, bool CONF, typename Defer>
 __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
                                      const uint32_t *__restrict__ ps,
                                      const uint32_t *__restrict__ lsx,
@@ -404,7 +407,8 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
         if (need_links) lnk = s_lnk[kk];
         return inwin;
       };
-      ok = gather_one_bfs<K, DIM>(q, s_lnk[q - w0], brt, excl, lsq, fetch, keys, conflict_bits,
+      ok = gather_one_bfs<K, DIM, CONF>(q, s_lnk[q - w0], brt, excl, lsq, fetch, keys,
+                                        conflict_bits,
                                   mbits,
                                   s_nodes + a0 + (excl_r - lo) * STRIDE,
                                   node_conflict, st);
@@ -468,7 +472,7 @@ __device__ inline void single_node(int32_t *octree, uint32_t *node_conflict) {
 // BRT nodes) for the global-memory pass.
 // With chunk_excl the prefix sums ps[0..m) are written here (see
 // gather_window); without it they are read.
-template <typename K, int DIM>
+template <typename K, int DIM, bool CONF>
 __global__ __launch_bounds__(kBlock) void gather_kernel(
     const BrtNodeT<K> *__restrict__ brt, uint32_t *__restrict__ ps,
     const uint32_t *__restrict__ lsx, const uint32_t *__restrict__ chunk_excl,
@@ -490,7 +494,8 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(
     return;
   }
   if (q0 >= m - 1) return;
-  gather_window<K, DIM, kGatherRows>(brt, ps, lsx, chunk_excl, ps, keys, conflict_bits, mbits, m,
+  gather_window<K, DIM, kGatherRows, CONF>(brt, ps, lsx, chunk_excl, ps, keys, conflict_bits,
+                                           mbits, m,
                                      q0, octree,
                         node_conflict, st,
                         [&](bool miss, uint32_t q) { wave_mask(miss, miss_words, q >> 6); });
@@ -521,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void gather_seg_kernel(
   }
   if (q0 >= m - 1) return;
   const uint32_t kofs = seg[s].kofs;
-  gather_window<K, DIM, 1>(brt + kofs, ps + kofs, lsx + kofs, nullptr, nullptr,
+  gather_window<K, DIM, 1, false>(brt + kofs, ps + kofs, lsx + kofs, nullptr, nullptr,
                         (const K *)nullptr, nullptr, 0, m, q0,
                         oct, nullptr, st, [&](bool miss, uint32_t q) {
                           wave_append(miss, kofs + q, &st->counters[CTR_OCT_DEFER], defer);
@@ -531,7 +536,7 @@ __global__ __launch_bounds__(kBlock) void gather_seg_kernel(
 // Gather, second pass: the BRT nodes marked in miss_words, reading global
 // memory, breadth first (a round's loads are independent, so a node waits
 // on DIM + 1 dependent loads, not one per trie node).
-template <typename K, int DIM>
+template <typename K, int DIM, bool CONF>
 __global__ __launch_bounds__(kBlock) void gather_defer_kernel(
     const BrtNodeT<K> *__restrict__ brt, const uint32_t *__restrict__ ps,
     const uint32_t *__restrict__ lsx, const K *__restrict__ keys,
@@ -556,9 +561,9 @@ __global__ __launch_bounds__(kBlock) void gather_defer_kernel(
   };
   for_each_masked(miss_words, nwords, [&](uint32_t q) {
     const uint32_t excl = q ? ps[q - 1] : 0u;
-    (void)gather_one_bfs<K, DIM>(q, load_links(brt + q), brt, excl, ps[q] - excl, fetch, keys,
-                                 conflict_bits, mbits, octree + (size_t)excl * STRIDE,
-                                 node_conflict, st);
+    (void)gather_one_bfs<K, DIM, CONF>(q, load_links(brt + q), brt, excl, ps[q] - excl, fetch,
+                                       keys, conflict_bits, mbits,
+                                       octree + (size_t)excl * STRIDE, node_conflict, st);
   });
 }
 
@@ -592,7 +597,7 @@ __global__ __launch_bounds__(kBlock) void gather_seg_defer_kernel(
       return true;
     };
     const uint32_t excl = q ? p[q - 1] : 0u;
-    (void)gather_one_bfs<K, DIM>(q, load_links(b + q), b, excl, p[q] - excl, fetch,
+    (void)gather_one_bfs<K, DIM, false>(q, load_links(b + q), b, excl, p[q] - excl, fetch,
                                  (const K *)nullptr, nullptr, 0, oct + (size_t)excl * STRIDE,
                                  nullptr, st);
   }
@@ -701,15 +706,26 @@ void launch_dim(const BrtNodeT<K> *brt, const int32_t *parent, uint32_t *ps,
     const unsigned ggrid =
         (unsigned)((nodes + kGatherRows * kBlock - 1) / (kGatherRows * kBlock));
     if (gather_ev) (void)hipEventRecord(gather_ev[0], s);
-    gather_kernel<K, DIM><<<ggrid, kBlock, 0, s>>>(brt, ps, lsx, chunk_excl, keys, conflict_bits,
-                                                  mbits, octree, node_conflict, capacity, words,
-                                                  st);
+    // the conflict octants are compiled in only for builds with labels
+    if (conflict_bits)
+      gather_kernel<K, DIM, true><<<ggrid, kBlock, 0, s>>>(brt, ps, lsx, chunk_excl, keys,
+                                                           conflict_bits, mbits, octree,
+                                                           node_conflict, capacity, words, st);
+    else
+      gather_kernel<K, DIM, false><<<ggrid, kBlock, 0, s>>>(brt, ps, lsx, chunk_excl, keys,
+                                                            nullptr, mbits, octree, nullptr,
+                                                            capacity, words, st);
     if (gather_ev) (void)hipEventRecord(gather_ev[1], s);
     const int64_t g2 = ((nodes + 63) / 64 + kBlock - 1) / kBlock;
     const unsigned dgrid = (unsigned)(g2 < 4096 ? g2 : 4096);
-    gather_defer_kernel<K, DIM><<<dgrid, kBlock, 0, s>>>(brt, ps, lsx, keys, conflict_bits,
-                                                         mbits, octree, node_conflict,
-                                                         capacity, words, st);
+    if (conflict_bits)
+      gather_defer_kernel<K, DIM, true><<<dgrid, kBlock, 0, s>>>(brt, ps, lsx, keys, conflict_bits,
+                                                                 mbits, octree, node_conflict,
+                                                                 capacity, words, st);
+    else
+      gather_defer_kernel<K, DIM, false><<<dgrid, kBlock, 0, s>>>(brt, ps, lsx, keys, nullptr,
+                                                                  mbits, octree, nullptr,
+                                                                  capacity, words, st);
     if (force_scatter == 2) return;
   }
   // The scatter path is a no-op unless forced or st->malformed; its kernels
