@@ -198,6 +198,21 @@ __device__ inline T wave_incl_scan_add(T v) {
   return v;
 }
 
+// 32-bit inclusive scan over the wave in DPP moves: shifts by 1, 2, 4 and 8
+// within each row of 16 lanes (a lane with no source adds 0), then lane 15
+// of each row added to the next row (rows 1 and 3) and lane 31 to rows 2
+// and 3 -- six adds, no LDS.
+__device__ inline uint32_t wave_incl_scan_add(uint32_t v) {
+  int x = (int)v;
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);   // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);   // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);   // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);   // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);   // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);   // row_bcast:31
+  return (uint32_t)x;
+}
+
 template <typename T>
 __device__ inline T wave_reduce_add(T v) {
 #pragma unroll

@@ -41,3 +41,22 @@ def test_cpp_kernel_scenarios():
     print(p.stdout)
     assert p.returncode == 0, p.stdout + p.stderr
     assert "ALL PASSED" in p.stdout
+
+
+SAN_BIN = os.path.join(HERE, "cpp", "san", "kernel_tests")
+
+
+@pytest.mark.gpu
+def test_cpp_kernel_scenarios_under_host_sanitizers():
+    """The same scenarios against libpgvd.so built with its host code under
+    AddressSanitizer and UndefinedBehaviorSanitizer (device code as usual;
+    tests/cpp/build_tests.py --sanitized, run by __graft_entry__.build()).
+    Leak checking is off: the HIP runtime keeps allocations to the end."""
+    if not os.path.exists(SAN_BIN):
+        pytest.fail("tests/cpp/san/kernel_tests was not built (run __graft_entry__.build())")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:protect_shadow_gap=0",
+               UBSAN_OPTIONS="print_stacktrace=1")
+    p = subprocess.run([SAN_BIN], capture_output=True, text=True, timeout=280, env=env)
+    print(p.stdout)
+    assert p.returncode == 0, p.stdout + p.stderr[-4000:]
+    assert "ALL PASSED" in p.stdout
