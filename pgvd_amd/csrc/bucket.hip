@@ -100,7 +100,7 @@ constexpr int kPlanBlock = 1024;
 __global__ __launch_bounds__(kPlanBlock) void bk_plan_kernel(
     const uint32_t *__restrict__ hist, uint32_t n, int nb, int nb2, int tile2, uint32_t cap,
     int wide, int lsd_ok, uint32_t *__restrict__ start, uint32_t *__restrict__ cursor,
-    BkSeg *__restrict__ segs, uint32_t *__restrict__ hrows, RunState *st) {
+    BkSeg *__restrict__ segs, uint32_t *__restrict__ hrows, RunState *st, RunState *host) {
   __shared__ uint32_t s_w[kPlanBlock / 64];
   __shared__ uint32_t s_nseg, s_bigkeys;
   const int t = threadIdx.x;
@@ -148,6 +148,13 @@ __global__ __launch_bounds__(kPlanBlock) void bk_plan_kernel(
     st->bk_lsd = lsd ? 1u : 0u;
     st->bk_segs = lsd ? 0u : nseg;
     st->bk_keys = lsd ? 0u : s_bigkeys;
+    // the same three words straight into the host's pinned mirror, which
+    // reads them once the plan's completion event has fired (no copy
+    // launch between the plan and the first scatter)
+    host->bk_lsd = lsd ? 1u : 0u;
+    host->bk_segs = lsd ? 0u : nseg;
+    host->bk_keys = lsd ? 0u : s_bigkeys;
+    __threadfence_system();
     // (a histogram that does not count every key leaves the keys unsorted:
     // then the status says so and nothing downstream trusts the order)
     st->msd_ok = (total == n && !lsd) ? 1u : 0u;
@@ -1098,12 +1105,11 @@ int launch_bucket_sort(const BucketArgs<K> &a, RunState *st, RunState *st_host,
   constexpr int wide = sizeof(K) > 8 ? 1 : 0;
   bk_plan_kernel<<<1, kPlanBlock, 0, s>>>(a.hist, (uint32_t)n, nb, nb2, TILE, (uint32_t)C::CAP,
                                           wide, sizeof(K) <= 8 ? 1 : 0, m.start, m.cursor,
-                                          m.segs[0], m.hrows[0], st);
+                                          m.segs[0], m.hrows[0], st, st_host);
   if (hipGetLastError() != hipSuccess) return 1;
-  // the host learns the level-1 segment count while the scatters run
-  if (hipMemcpyAsync(st_host, st, sizeof(RunState), hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipEventRecord(ev, s) != hipSuccess)
-    return 1;
+  // the host learns the level-1 segment count (written by the plan into its
+  // pinned mirror) while the scatters run
+  if (hipEventRecord(ev, s) != hipSuccess) return 1;
   K *kA = a.kA, *kB = a.kB;
   uint32_t *vA = a.vA, *vB = a.vB;
   const unsigned tiles1 = (unsigned)((n + TILE - 1) / TILE);

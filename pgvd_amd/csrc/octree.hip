@@ -268,11 +268,11 @@ constexpr int kGatherHalo = 256;
 #define PGVD_GATHER_ROWS 2
 #endif
 constexpr int kGatherRows = PGVD_GATHER_ROWS;
-// Staged octree nodes per workgroup, in percent of its BRT nodes: 62% of
-// 512 keeps the LDS at 24.9 KB, six workgroups per CU (all of 512 would
-// leave five).
+// Staged octree nodes per workgroup, in percent of its BRT nodes: 58% of
+// 512 (296 nodes, a uniform window owns ~240) keeps the LDS at 20.0 KB, so
+// that eight workgroups fit a CU (62% gave 20.8 KB and seven).
 #ifndef PGVD_GATHER_STAGE_PCT
-#define PGVD_GATHER_STAGE_PCT 62
+#define PGVD_GATHER_STAGE_PCT 58
 #endif
 constexpr int kGatherStagePct = PGVD_GATHER_STAGE_PCT;
 
@@ -472,8 +472,15 @@ __device__ inline void single_node(int32_t *octree, uint32_t *node_conflict) {
 // BRT nodes) for the global-memory pass.
 // With chunk_excl the prefix sums ps[0..m) are written here (see
 // gather_window); without it they are read.
+// Waves per SIMD the window pass is compiled for (its register budget):
+// eight, with the LDS of eight workgroups per CU, ran it 5% faster than
+// seven although some SGPRs then spill to VGPR lanes.
+#ifndef PGVD_GATHER_WPE
+#define PGVD_GATHER_WPE 8
+#endif
 template <typename K, int DIM, bool CONF>
-__global__ __launch_bounds__(kBlock) void gather_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PGVD_GATHER_WPE))) void
+gather_kernel(
     const BrtNodeT<K> *__restrict__ brt, uint32_t *__restrict__ ps,
     const uint32_t *__restrict__ lsx, const uint32_t *__restrict__ chunk_excl,
     const K *__restrict__ keys, const uint32_t *__restrict__ conflict_bits, int mbits,
