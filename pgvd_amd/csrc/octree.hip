@@ -261,6 +261,183 @@ __device__ __forceinline__ bool gather_one_bfs(uint32_t q, uint32_t lk,
   return true;
 }
 
+// Appends val to a wave's queue from each lane where want holds (lanes in
+// order); entries the queue cannot hold (more than cap - (tail - head)
+// queued) are dropped and flag their lane's miss.  Returns the new tail.
+// Call with the whole wave converged; tail and head are uniform.
+__device__ inline uint32_t queue_push(uint16_t *q, uint32_t cap, uint32_t head, uint32_t tail,
+                                      bool want, uint32_t val, bool &miss) {
+  const uint64_t b = __ballot(want);
+  const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+  const uint32_t room = cap - (tail - head);
+  if (want) {
+    if (below < room) q[(tail + below) & (cap - 1)] = (uint16_t)val;
+    else miss = true;
+  }
+  const uint32_t n = (uint32_t)__popcll(b);
+  return tail + (n < room ? n : room);
+}
+
+// The staged window pass's owners without conflict octants: gather_one_bfs
+// over a wave at a time, with its rounds' work queued.  Lane l of wave w
+// takes the owners a = w 64 + l + i 256 (as the per-lane loop does); the
+// first round (the owner's two children) runs per lane, and the internal
+// nodes it expands into go to the wave's queue as (window position, owner
+// lane) in 16 bits, so that the later rounds spend a lane on each queued
+// node rather than fixed 4 and 8 slots per owner (of which about 2.2 hold
+// an internal node for uniform points).  A lane reads its entry's owner
+// node offset and index from the owner's registers (a cross-lane read).
+// The queues live in s_aidx, whose owner indices are read into registers
+// first: STAGE / 4 entries per wave.  A node outside the window, or a queue
+// full, sends its owner to the global-memory pass (s_miss), which rewrites
+// the owner's nodes whole.
+template <typename K, int DIM, int R>
+__device__ inline void gather_owners_queued(uint32_t nact, uint32_t q0, int w0, int WN,
+                                            uint32_t base, uint32_t lo, uint32_t a0,
+                                            uint16_t *s_aidx, const uint32_t *s_pl,
+                                            const uint32_t *s_lnk, int32_t *s_nodes,
+                                            uint32_t *s_miss,
+                                            const BrtNodeT<K> *__restrict__ brt, RunState *st) {
+  constexpr int NC = 1 << DIM;
+  constexpr int STRIDE = NC + 1;
+  constexpr int32_t kFull = (1 << NC) - 1;
+  constexpr uint32_t CAP = (uint32_t)(R * kBlock) / 4;   // queue entries per wave
+  static_assert((CAP & (CAP - 1)) == 0, "queue indices wrap by mask");
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t e_own[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const uint32_t a = (uint32_t)(t + i * kBlock);
+    e_own[i] = a < nact ? s_aidx[a] : 0xffffu;
+  }
+  __syncthreads();   // every owner index is read: s_aidx becomes the queues
+  uint16_t *qw = s_aidx + w * CAP;
+  const uint32_t nit = (nact + kBlock - 1) / kBlock;   // uniform
+  for (uint32_t i = 0; i < nit; ++i) {
+    uint32_t e = 0xffffu;
+#pragma unroll
+    for (int ii = 0; ii < R; ++ii)
+      if ((uint32_t)ii == i) e = e_own[ii];
+    const bool has = e != 0xffffu;
+    const uint32_t q = q0 + (has ? e : 0u);
+    const uint32_t excl_r = q ? s_pl[q - 1 - w0] >> kPlShift : 0u;
+    const uint32_t lsq = (s_pl[q - w0] >> kPlShift) - excl_r;
+    const uint32_t n0 = a0 + (excl_r - lo) * STRIDE;   // the owner's lowest staged node
+    if (has) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) s_nodes[n0 + c] = -1;
+    }
+    const uint32_t lk = has ? s_lnk[q - w0] : 0u;
+    bool miss = false;
+    // round 1: the owner's two children
+    bool want[4];
+    uint32_t cand[4];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const uint32_t c = (lk & kSlotIdx) + side;
+      const bool leaf = side ? (lk >> 31) != 0u : ((lk >> 30) & 1u) != 0u;
+      const bool internal = has && !leaf;
+      const unsigned k = (unsigned)((int)c - w0);
+      const bool inwin = k < (unsigned)WN;
+      miss |= internal && !inwin;
+      const bool use = internal && inwin;
+      const uint32_t pl = s_pl[use ? k : 0u];
+      const uint32_t lx = pl & kPlLsx;
+      const bool term = use && (lx & 0xffu) != 0;
+      const bool expand = use && (lx & 0xffu) == 0;
+      if (has) s_nodes[n0 + (term ? lx >> 8 : (uint32_t)NC)] = (int32_t)(base + (pl >> kPlShift) - 1);
+      if (DIM == 1) {
+        if (expand) atomicOr(&st->status, (uint32_t)ST_STACK);
+        want[2 * side] = want[2 * side + 1] = false;
+        cand[2 * side] = cand[2 * side + 1] = 0u;
+      } else {
+        const uint32_t v = s_lnk[use ? k : 0u];
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch) {
+          const uint32_t cc = (v & kSlotIdx) + ch;
+          const bool cleaf = ch ? (v >> 31) != 0u : ((v >> 30) & 1u) != 0u;
+          const bool cint = expand && !cleaf;
+          const unsigned kk = (unsigned)((int)cc - w0);
+          const bool cin = kk < (unsigned)WN;
+          miss |= cint && !cin;
+          want[2 * side + ch] = cint && cin;
+          cand[2 * side + ch] = kk | ((uint32_t)lane << 10);
+        }
+      }
+    }
+    uint32_t head = 0, tail = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) tail = queue_push(qw, CAP, head, tail, want[j], cand[j], miss);
+    // rounds 2 .. DIM over the queued nodes, a generation at a time
+    uint32_t glo = 0, ghi = tail;
+#pragma unroll
+    for (int r = 2; r <= DIM; ++r) {
+      for (uint32_t b0 = glo; b0 < ghi; b0 += 64) {
+        const uint32_t idx = b0 + (uint32_t)lane;
+        const bool valid = idx < ghi;
+        const uint32_t ent = valid ? (uint32_t)qw[idx & (CAP - 1)] : 0u;
+        head = min(b0 + 64u, ghi);   // this chunk's entries are in registers
+        const uint32_t k = ent & 1023u, ol = ent >> 10;
+        const uint32_t n0o = (uint32_t)__shfl((int)n0, (int)ol, 64);
+        const uint32_t eo = (uint32_t)__shfl((int)e, (int)ol, 64);
+        const uint32_t pl = s_pl[k];
+        const uint32_t lx = pl & kPlLsx;
+        const bool term = valid && (lx & 0xffu) != 0;
+        const bool expand = valid && (lx & 0xffu) == 0;
+        if (valid) s_nodes[n0o + (term ? lx >> 8 : (uint32_t)NC)] = (int32_t)(base + (pl >> kPlShift) - 1);
+        bool m2 = false;
+        if (r == DIM) {
+          // a well-formed tree has no same-level node this deep
+          if (expand) atomicOr(&st->status, (uint32_t)ST_STACK);
+        } else {
+          const uint32_t v = s_lnk[k];
+          bool wnt[2];
+          uint32_t cd[2];
+#pragma unroll
+          for (int ch = 0; ch < 2; ++ch) {
+            const uint32_t cc = (v & kSlotIdx) + ch;
+            const bool cleaf = ch ? (v >> 31) != 0u : ((v >> 30) & 1u) != 0u;
+            const bool cint = expand && !cleaf;
+            const unsigned kk = (unsigned)((int)cc - w0);
+            const bool cin = kk < (unsigned)WN;
+            m2 |= cint && !cin;
+            wnt[ch] = cint && cin;
+            cd[ch] = kk | (ol << 10);
+          }
+#pragma unroll
+          for (int ch = 0; ch < 2; ++ch) tail = queue_push(qw, CAP, head, tail, wnt[ch], cd[ch], m2);
+        }
+        if (m2) atomicOr(&s_miss[eo >> 5], 1u << (eo & 31));
+      }
+      glo = ghi;
+      ghi = tail;
+    }
+    if (!has) continue;
+    if (miss) atomicOr(&s_miss[e >> 5], 1u << (e & 31));
+    // the leaf mask from the children the rounds wrote, then the chain
+    int32_t leafm = 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) leafm |= (s_nodes[n0 + c] == -1) << c;
+    s_nodes[n0 + NC] = leafm;
+    if (lsq > 1) {
+      const BrtNodeT<K> nq = brt[q];
+      const uint32_t excl = base + excl_r;
+      for (uint32_t kq = 1; kq < lsq; ++kq) {
+        int32_t *nd = s_nodes + n0 + kq * STRIDE;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) nd[c] = -1;
+        nd[NC] = kFull;
+        if (q != 0) {
+          const int o = quadrant<K, DIM>(nq, (int)kq - 1);
+          nd[o] = (int32_t)(excl + kq - 1);
+          nd[NC] = kFull & ~(1 << o);
+        }
+      }
+    }
+  }
+}
+
 // BRT entries (links, ps, lsx) staged either side of a workgroup's nodes.
 constexpr int kGatherHalo = 256;
 // BRT nodes per workgroup of the single-tree gather: kGatherRows rows of 256.
@@ -388,6 +565,10 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
   for (int j = 0; j < R; ++j)
     if ((amask >> j) & 1u) s_aidx[pos++] = (uint16_t)(j * kBlock + t);
   __syncthreads();
+  if (!CONF && staged) {
+    gather_owners_queued<K, DIM, R>(nact, q0, w0, WN, base, lo, a0, s_aidx, s_pl, s_lnk, s_nodes,
+                                    s_miss, brt, st);
+  } else {
   for (uint32_t a = t; a < nact; a += kBlock) {
     const uint32_t e = s_aidx[a];
     const uint32_t q = q0 + e;
@@ -434,6 +615,7 @@ __device__ inline void gather_window(const BrtNodeT<K> *__restrict__ brt,
                               put, node_conflict, st);
     }
     if (!ok) atomicOr(&s_miss[e >> 5], 1u << (e & 31));
+  }
   }
   __syncthreads();
 #pragma unroll
