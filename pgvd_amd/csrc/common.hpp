@@ -220,6 +220,12 @@ __device__ inline T wave_reduce_add(T v) {
   return v;
 }
 
+// 32-bit sum over the wave: the DPP scan's last lane, read into a scalar.
+// Call with the whole wave converged.
+__device__ inline uint32_t wave_reduce_add(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan_add(v), 63);
+}
+
 // Appends `idx` to list[] for every lane with `take` set, one atomic per wave
 // on *count.  Call with the whole wave converged.
 template <typename T>
