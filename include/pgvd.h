@@ -76,6 +76,13 @@ int pgvd_ctx_pool_bytes(pgvd_ctx *ctx, size_t *out);
  * call returns at once, with work already enqueued left to run.  NULL or ""
  * clears it.  For tests of the error paths. */
 int pgvd_debug_fail_launch(const char *stage);
+/* Test hook, no counterpart in the reference: the octree gather's per-wave
+ * queues of breadth-first work hold at most `entries` entries (0 restores
+ * their full size), so that tests can force the path where a full queue
+ * sends its node to the global-memory pass.  Applies to every later build
+ * on the current device.  Results do not change, only where they are
+ * computed. */
+int pgvd_debug_gather_queue(int entries);
 int pgvd_synchronize(pgvd_ctx *ctx, pgvd_stream s);
 int pgvd_alloc(pgvd_ctx *ctx, size_t bytes, void **dptr);
 int pgvd_free(pgvd_ctx *ctx, void *dptr);

@@ -137,6 +137,7 @@ def lib():
             "pgvd_ctx_set_pool_limit": ([vp, C.c_size_t], i32),
             "pgvd_ctx_pool_bytes": ([vp, C.POINTER(C.c_size_t)], i32),
             "pgvd_debug_fail_launch": ([C.c_char_p], i32),
+            "pgvd_debug_gather_queue": ([i32], i32),
             "pgvd_synchronize": ([vp, vp], i32),
             "pgvd_alloc": ([vp, C.c_size_t, C.POINTER(vp)], i32),
             "pgvd_free": ([vp, vp], i32),
@@ -297,6 +298,14 @@ def words_to_keys(words: np.ndarray, key_bytes: int):
         return words[0::2].copy(), words[1::2].copy()
     lo = words.astype(np.uint64)
     return lo, np.zeros_like(lo)
+
+
+def debug_gather_queue(entries: int) -> None:
+    """Test hook: the octree gather's queues hold at most `entries` entries
+    (0: full size); results must not change."""
+    rc = lib().pgvd_debug_gather_queue(int(entries))
+    if rc != 0:
+        raise PgvdError(rc, lib().pgvd_last_error().decode())
 
 
 def debug_fail_launch(stage) -> None:

@@ -721,6 +721,13 @@ int pgvd_debug_fail_launch(const char *stage) {
   return PGVD_OK;
 }
 
+int pgvd_debug_gather_queue(int entries) {
+  if (entries < 0) return fail(PGVD_ERR_INVALID, "entries must be >= 0");
+  if (pgvd::set_gather_queue_cap((uint32_t)entries) != 0)
+    return fail(PGVD_ERR_HIP, "hipMemcpyToSymbol failed for the gather queue cap");
+  return PGVD_OK;
+}
+
 int pgvd_synchronize(pgvd_ctx *ctx, pgvd_stream s) {
   if (!ctx) return fail(PGVD_ERR_INVALID, "null context");
   CHK(use_device(ctx));

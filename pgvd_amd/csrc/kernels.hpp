@@ -373,6 +373,10 @@ void launch_claims_if_malformed(const BrtNodeT<K> *brt, int32_t *parent, int64_t
                                 RunState *st, hipStream_t s);
 
 // ---- octree.hip ----
+// Test hook of the octree window pass (octree.hip): its per-wave queues hold
+// at most cap entries (0: their full size).  Returns 0 on success.
+int set_gather_queue_cap(uint32_t cap);
+
 // Octree fill.  parent[] in the reference form.  force_scatter 1 selects the
 // order-exact scatter path unconditionally (stage API, where the BRT comes
 // from the caller; lsx and defer may then be null); 0 lets st->malformed

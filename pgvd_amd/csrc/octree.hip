@@ -265,14 +265,20 @@ __device__ __forceinline__ bool gather_one_bfs(uint32_t q, uint32_t lk,
 // order); entries the queue cannot hold (more than cap - (tail - head)
 // queued) are dropped and flag their lane's miss.  Returns the new tail.
 // Call with the whole wave converged; tail and head are uniform.
-__device__ inline uint32_t queue_push(uint16_t *q, uint32_t cap, uint32_t head, uint32_t tail,
-                                      bool want, uint32_t val, bool &miss) {
+// Entries a queue of the window pass may hold (tests lower it to force the
+// overflow path; the queue's own size is the ceiling).
+__device__ uint32_t g_gather_queue_cap = 0xffffffffu;
+
+// The queue has size slots (a power of two) and holds at most cap <= size
+// entries.
+__device__ inline uint32_t queue_push(uint16_t *q, uint32_t size, uint32_t cap, uint32_t head,
+                                      uint32_t tail, bool want, uint32_t val, bool &miss) {
   const uint64_t b = __ballot(want);
   const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
   const uint32_t room = cap - (tail - head);
   if (want) {
-    if (below < room) q[(tail + below) & (cap - 1)] = (uint16_t)val;
+    if (below < room) q[(tail + below) & (size - 1)] = (uint16_t)val;
     else miss = true;
   }
   const uint32_t n = (uint32_t)__popcll(b);
@@ -313,6 +319,7 @@ __device__ inline void gather_owners_queued(uint32_t nact, uint32_t q0, int w0, 
   }
   __syncthreads();   // every owner index is read: s_aidx becomes the queues
   uint16_t *qw = s_aidx + w * CAP;
+  const uint32_t cap = min(CAP, g_gather_queue_cap);
   const uint32_t nit = (nact + kBlock - 1) / kBlock;   // uniform
   for (uint32_t i = 0; i < nit; ++i) {
     uint32_t e = 0xffffu;
@@ -368,7 +375,7 @@ __device__ inline void gather_owners_queued(uint32_t nact, uint32_t q0, int w0, 
     }
     uint32_t head = 0, tail = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) tail = queue_push(qw, CAP, head, tail, want[j], cand[j], miss);
+    for (int j = 0; j < 4; ++j) tail = queue_push(qw, CAP, cap, head, tail, want[j], cand[j], miss);
     // rounds 2 .. DIM over the queued nodes, a generation at a time
     uint32_t glo = 0, ghi = tail;
 #pragma unroll
@@ -406,7 +413,8 @@ __device__ inline void gather_owners_queued(uint32_t nact, uint32_t q0, int w0, 
             cd[ch] = kk | (ol << 10);
           }
 #pragma unroll
-          for (int ch = 0; ch < 2; ++ch) tail = queue_push(qw, CAP, head, tail, wnt[ch], cd[ch], m2);
+          for (int ch = 0; ch < 2; ++ch)
+            tail = queue_push(qw, CAP, cap, head, tail, wnt[ch], cd[ch], m2);
         }
         if (m2) atomicOr(&s_miss[eo >> 5], 1u << (eo & 31));
       }
@@ -959,6 +967,14 @@ __global__ __launch_bounds__(kBlock) void local_splits_finish_kernel(
 }
 
 }  // namespace
+
+// Test hook: the window pass's queues hold at most cap entries (0: their
+// full size).  Synchronous; every device the process uses takes it.
+int set_gather_queue_cap(uint32_t cap) {
+  const uint32_t v = cap ? cap : 0xffffffffu;
+  return hipMemcpyToSymbol(HIP_SYMBOL(g_gather_queue_cap), &v, sizeof v) == hipSuccess ? 0 : 1;
+}
+
 
 template <typename K>
 void launch_local_splits(const BrtNodeT<K> *brt, int64_t m, int dim, uint32_t *ls, uint64_t *tmp,

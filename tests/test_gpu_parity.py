@@ -641,3 +641,23 @@ def test_query_entry_points_reject_bad_arguments(dev):
     assert L.pgvd_octree_save(dev.ctx, d.ptr, O, 2, b"/tmp/x.txt", 3) == 1               # format
     for b in [d, q, segs, off, cells, small, *outs]:
         b.free()
+
+
+@pytest.mark.parametrize("cap", [1, 3, 16])
+def test_gather_queue_overflow_matches_oracle(dev, cap):
+    """The octree gather's per-wave queues shrunk to `cap` entries
+    (pgvd_debug_gather_queue): the owners whose breadth-first work no longer
+    fits go to the global-memory pass, and every output still equals the
+    oracle's -- uniform and clustered 3D points and a 2D quadtree."""
+    cases = [(_pts(1 << 18, 3, 31), 21), (_pts(1 << 17, 2, 32), 16)]
+    g, _ = workloads.gaussian_mixture(1 << 17, seed=33)
+    cases.append((g, 21))
+    P.debug_gather_queue(cap)
+    try:
+        for pts, bits in cases:
+            print(f"queue cap={cap} n={pts.shape[0]} dim={pts.shape[1]}", flush=True)
+            got = dev.build_octree(pts, bits)
+            want = oracle.build(oracle.quantize(pts, 1 << bits), bits)
+            assert_tree_equal(got, want, where=f"cap={cap} n={pts.shape[0]} dim={pts.shape[1]}")
+    finally:
+        P.debug_gather_queue(0)
