@@ -613,6 +613,12 @@ bool encode_vectorised(const float *pts, const void *keys) {
   return aligned16(pts) && aligned16(keys);
 }
 
+// Workgroups of the vectorised encode at most.  Each flushes its histograms
+// (up to 4096 bucket counters) with global atomics once: at 2^24 3D points
+// 1024 workgroups took 79 us, 2048 took 90 and 512 took 96.
+#ifndef PGVD_ENCODE_GRID
+#define PGVD_ENCODE_GRID 1024
+#endif
 template <typename K>
 void launch_encode(const float *pts, int64_t n, int dim, int bits, float effw, int custom_bb,
                    const float *bbmin, const float *bbmax, K *keys, int64_t *qout,
@@ -623,7 +629,7 @@ void launch_encode(const float *pts, int64_t n, int dim, int bits, float effw, i
   if (custom_bb)
     for (int k = 0; k < dim; ++k) { a[k] = bbmin[k]; b[k] = bbmax[k]; }
   if (keys && hist && !qout && encode_vectorised(pts, keys)) {
-    const int vgrid = grid_for(n / 4 + 1, kBlock);
+    const int vgrid = std::min(grid_for(n / 4 + 1, kBlock), PGVD_ENCODE_GRID);
     if (msd_nb > kBkMaxBuckets) msd_hist = nullptr;
     const bool big = msd_hist && msd_nb > (1 << kEncodeMaxBucketBits);
     if (dim == 2 && !big)
